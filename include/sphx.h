@@ -1,0 +1,352 @@
+/*
+ * sphx.h - C ABI of the MI355X-native particle hot path (spatial-hash grid,
+ * SPH/IISPH and multiphase neighbour loops, MPM transfer, RPY mobility).
+ *
+ * Two layers are exported from libsphx.so:
+ *
+ *   1. The legacy drop-in entry points, with the names, argument meaning and
+ *      call order of the reference's host driver
+ *      (SPH_Solver/fluids/fluid_system_host.cuh:44-107).  They act on one
+ *      process-wide default context, synchronise after every call and latch
+ *      errors (printed to stderr, readable with sphx_last_error()), exactly
+ *      like the reference's void wrappers, which print cudaGetLastError()
+ *      and carry on.  A C++ reference `T&` argument is passed as `T*` at the
+ *      ABI level, so the reference's own declarations link against these.
+ *
+ *   2. The context API (sphx_*): explicit context objects, int status
+ *      returns (0 = success, <0 = error with a thread-local message),
+ *      asynchronous work on the context's HIP stream.
+ *
+ * All particle data crosses the boundary in the reference's AoS formats
+ * (displayPack 32 B, calculationPack 224 B, ParamCarrier 464 B); the device
+ * layout behind them is private (structure of arrays, cell-sorted).
+ */
+#ifndef SPHX_H
+#define SPHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Types (layout-identical to SPH_Solver/catpaw/geometry.h:48-176 and        */
+/* SPH_Solver/common_header.h:28-137; every field is 4 bytes, no padding).   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { float x, y, z; } sphx_float3;
+typedef struct { float x, y, z, w; } sphx_float4;
+typedef struct { int x, y, z; } sphx_int3;
+typedef struct { float data[9]; } sphx_mat3; /* row-major, geometry.h:142-176 */
+
+#define SPHX_MAX_FLUIDNUM 3   /* common_header.h:17 MAX_FLUIDNUM / FLUID_NUM */
+#define SPHX_TYPE_FLUID 0     /* common_header.h:20-23 */
+#define SPHX_TYPE_BOUNDARY 1
+#define SPHX_TYPE_ELASTIC 2
+#define SPHX_TYPE_GRANULAR 3
+#define SPHX_GRID_UNDEF 0xFFFFFFFFu /* fluid_system_kern.cuh:16 GRID_UNDEF */
+
+/* common_header.h:28-109 ParamCarrier */
+typedef struct sphx_ParamCarrier {
+  float acclimit;
+  float vlimit;
+  sphx_float3 gravity;
+  sphx_float3 volmin;
+  sphx_float3 volmax;
+  sphx_float3 softminx;
+  sphx_float3 softmaxx;
+  int num;
+  int maxNum;
+  float dt;
+  float simscale;
+  float viscosity;
+  float restdensity;
+  float mass;
+  float radius;
+  float smoothradius;
+  float scalep;
+  float massArr[SPHX_MAX_FLUIDNUM];
+  float densArr[SPHX_MAX_FLUIDNUM];
+  float viscArr[SPHX_MAX_FLUIDNUM];
+  float boundstiff;
+  float kpoly6;
+  float kspiky;
+  float kspikydiff;
+  float klaplacian;
+  float kspline;
+  float bmass;
+  float bRestdensity;
+  float bvisc;
+  float cellsize;
+  int searchnum;
+  int neighbornum;
+  int neighborid[27];
+  sphx_float3 gridmin, gridmax;
+  sphx_float3 gridsize;
+  sphx_float3 gridIdfac;
+  sphx_int3 gridres;
+  int gridtotal;
+  float mpmcellsize;
+  sphx_float3 mpmXmin, mpmXmax;
+  sphx_int3 mpmRes;
+  int mpmNodeNum;
+  float solidK;
+  float solidG;
+  float a_phi;
+  float a_psi;
+  float k_c;
+  float w_deltax;
+  float fijn;
+  float SFlipControl;
+  float LFlipControl;
+  int mpmSplitNum;
+  float collisionStiff;
+  float surfaceTensionK;
+  float surfaceTensionFluidC;
+  float fBlending;
+} sphx_ParamCarrier;
+
+/* common_header.h:112-116 displayPack */
+typedef struct sphx_displayPack {
+  sphx_float3 pos;
+  sphx_float4 color;
+  int type;
+} sphx_displayPack;
+
+/* common_header.h:118-137 calculationPack.  `dens` holds 1/rho after the
+ * WCSPH density pass, rho after the IISPH density pass, and the boundary
+ * volume for boundary particles (fluid_system_kern.cu:276,306,1114). */
+typedef struct sphx_calculationPack {
+  sphx_float3 vel;
+  sphx_float3 veleval;
+  float pressure;
+  float dens;
+  float restdens;
+  float mass;
+  float visc;
+  int bornid;
+  sphx_float3 accel;
+  sphx_mat3 deformGrad;
+  sphx_mat3 stress;
+  sphx_mat3 B;
+  float solidG;
+  float solidK;
+  sphx_float3 X;
+  sphx_mat3 invA;
+} sphx_calculationPack;
+
+/* SPH_Solver/stokesian/Mobility.cuh:6-16 stokesianBufList (device pointers). */
+typedef struct sphx_stokesianBufList {
+  sphx_float3* cuUnew;
+  sphx_displayPack* dispBuffer;
+  sphx_float3* cuOmega;
+  sphx_float3* cuForce;
+  float dt;
+  float simscale;
+} sphx_stokesianBufList;
+
+/* Solver choices that the reference fixes in code.  The defaults reproduce
+ * the reference; every other value is a documented extension.  No
+ * counterpart in the reference (it hard-codes each of these). */
+typedef struct sphx_SolverOptions {
+  /* IISPH (fluid_system_kern.cu:730-1050, fluid_system_host.cu:525-552) */
+  float omega;              /* relaxed-Jacobi weight; reference 0.1 (:1026) */
+  float warm_start;         /* p0 = warm_start * p(t-1); reference 0.5 (:899) */
+  int full_support;         /* 0: pressure loops test r < h as the reference
+                               (:750,:822,:867,:931,:985); 1: r < 2h */
+  int boundary_pressure;    /* 1: enable the boundary terms the reference has
+                               commented out (:763-764, :833-834, :1000-1002)
+                               plus the matching a_ii term; 0: reference */
+  float tol;                /* stop when mean |rho-rho0|/rho0 <= tol (FP64);
+                               the reference loop never stops (A1) */
+  int min_iters;            /* reference criterion needs iter >= 2 (:545) */
+  int max_iters;
+  int clamp_to_grid;        /* 1: integration keeps particles inside the
+                               valid cell range (guard for long runs) */
+  /* WCSPH kernel family / EOS (fluid_system.cpp:308-313, kern.cu:310) */
+  int wcsph_kernel;         /* 0: poly6 + spiky, support h (reference);
+                               1: cubic spline, support 2h */
+  float eos_B;              /* Tait stiffness B; <= 0 selects 2.5*rho0 */
+  float eos_gamma;          /* Tait exponent; reference 7 */
+  float art_visc_alpha;     /* >0: Monaghan artificial viscosity between
+                               fluid pairs with sound speed c0 */
+  float sound_speed;        /* c0 for art_visc_alpha */
+  /* multiphase (multiphase.cu spec) */
+  float drift_relax;        /* tau numerator: tau = drift_relax / nu_i */
+  float drift_diffusion;    /* sigma_D of the alpha-gradient term */
+  float drift_pressure_scale; /* cont*cont2 of multiphase.cu:133 */
+  float drift_diffusion_scale;/* cont*cont1 of multiphase.cu:162 */
+  int boussinesq;           /* 1: mixture buoyancy (rho_m-rho_ref)/rho_ref*g */
+  float rho_ref;            /* reference density of the mixture solve */
+  int clamp_pressure;       /* 1: p = max(p, 0) after each iteration (the
+                               clamp the reference has commented out at
+                               kern.cu:1035-1036) and the density error
+                               counts compression only, max(rho-rho0,0)/rho0
+                               (DFSPH-style); 0: reference, |rho-rho0|/rho0 */
+  int reserved[7];
+} sphx_SolverOptions;
+
+/* ------------------------------------------------------------------------ */
+/* Errors                                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* sphx_last_error(void);
+/* Number of HIP devices visible (0 without a GPU; never fails). */
+int sphx_device_count(void);
+/* Library version string. */
+const char* sphx_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Legacy drop-in entry points (fluid_system_host.cuh:44-107)                */
+/* ------------------------------------------------------------------------ */
+
+void cudaInit(int argc, char** argv);                 /* host.cu:30 */
+void cudaExit(int argc, char** argv);                 /* host.cu:26 */
+void FluidClearCUDA(void);                            /* host.cu:79 */
+void FluidSetupCUDA(sphx_ParamCarrier* params);       /* host.cu:103 */
+void CarryParam(sphx_ParamCarrier* hostCarrier);      /* kern.cu:23 */
+void computeNumBlocks(int numPnts, int maxThreads, int* numBlocks,
+                      int* numThreads);               /* host.cu:70 */
+void GetParticleIndexCUDA(void);                      /* host.cu:147 */
+void GetGridListCUDA(void);                           /* host.cu:161 */
+void RearrageDataCUDA(void);                          /* host.cu:187 [sic] */
+void ComputeBoundaryDensity(void);                    /* host.cu:510 */
+void MfComputePressureCUDA(void);                     /* host.cu:337 */
+void MfComputeForceCUDA(void);                        /* host.cu:421 */
+void MfAdvanceCUDA(void);                             /* host.cu:385 */
+void ComputeDensityIISPH_CUDA(void);                  /* host.cu:505 */
+void PredictAdvection(void);                          /* host.cu:515 */
+void PressureSolve(void);                             /* host.cu:525 */
+void Integration(void);                               /* host.cu:554 */
+
+/* Host-side transfers the reference performs with raw cudaMemcpy on the
+ * global buffers (fluid_system.cpp:979-999). */
+void sphx_legacy_upload(const sphx_displayPack* disp,
+                        const sphx_calculationPack* calc, int start, int end);
+void sphx_legacy_download(sphx_displayPack* disp, sphx_calculationPack* calc,
+                          int count);
+/* The default context behind the legacy entry points (NULL before
+ * FluidSetupCUDA). */
+struct sphx_context* sphx_legacy_context(void);
+
+/* ------------------------------------------------------------------------ */
+/* Context API                                                               */
+/* ------------------------------------------------------------------------ */
+
+typedef struct sphx_context sphx_context;
+
+/* Allocate device buffers for p->maxNum particles and the grid p->gridtotal
+ * (fluid_system_host.cu:103-144); copies the parameter block (CarryParam). */
+int sphx_create(sphx_context** out, const sphx_ParamCarrier* p);
+int sphx_destroy(sphx_context* ctx);
+/* CarryParam: replace the parameter block (num, dt, grid ...). Grid size
+ * changes reallocate the cell table. */
+int sphx_set_params(sphx_context* ctx, const sphx_ParamCarrier* p);
+int sphx_get_params(const sphx_context* ctx, sphx_ParamCarrier* p);
+void sphx_default_options(sphx_SolverOptions* o);
+int sphx_set_options(sphx_context* ctx, const sphx_SolverOptions* o);
+/* Block until every queued kernel of the context has finished. */
+int sphx_sync(sphx_context* ctx);
+
+/* Particle transfers (AoS at the boundary).  upload writes slots
+ * [start, start+count) and sets num = max(num, start+count) when grow != 0. */
+int sphx_upload(sphx_context* ctx, const sphx_displayPack* disp,
+                const sphx_calculationPack* calc, int start, int count,
+                int grow);
+int sphx_download(sphx_context* ctx, sphx_displayPack* disp,
+                  sphx_calculationPack* calc, int start, int count);
+int sphx_num_particles(const sphx_context* ctx);
+
+/* Grid build (fluid_system_kern.cu:36-121).  Each pointer may be NULL.
+ * mgcell/midsort/mgndx have num entries; gridoff/gridcnt gridtotal entries;
+ * MFidTable maps bornid -> sorted slot (num entries, -1 when absent). */
+int sphx_download_grid(sphx_context* ctx, uint32_t* mgcell, uint32_t* midsort,
+                       uint32_t* mgndx, int32_t* gridoff, int32_t* gridcnt,
+                       int32_t* MFidTable);
+int sphx_grid_hash(sphx_context* ctx);      /* InitialSort   kern.cu:36 */
+int sphx_grid_sort(sphx_context* ctx);      /* sort_by_key + CalcFirstCnt +
+                                               GetCnt host.cu:161-185 */
+int sphx_grid_reorder(sphx_context* ctx);   /* RearrangeData kern.cu:98 */
+int sphx_grid_build(sphx_context* ctx);     /* the three above */
+
+/* WCSPH (fluid_system_kern.cu:249-534) */
+int sphx_boundary_volume(sphx_context* ctx);        /* :249 */
+int sphx_wcsph_density_pressure(sphx_context* ctx); /* :279 */
+int sphx_wcsph_force(sphx_context* ctx);            /* :397 */
+int sphx_wcsph_advance(sphx_context* ctx);          /* :426 */
+/* sort + density/pressure + force + advance (fluid_system.cpp:514-539) */
+int sphx_wcsph_step(sphx_context* ctx);
+
+/* IISPH (fluid_system_kern.cu:635-1115, fluid_system_host.cu:505-556) */
+int sphx_iisph_density(sphx_context* ctx);          /* :1093 */
+int sphx_iisph_predict_advection(sphx_context* ctx);/* :768 + :883 */
+/* One Pressure_DP + Pressure_Iter + press_l <- press_l1 (:944, :1009,
+ * host.cu:530-536).  Optionally returns the mean signed residual over all
+ * num slots (host.cu:539-541, kg/m^3) and the FP64 mean |rho-rho0|/rho0
+ * over fluid particles. Either pointer may be NULL (then no host sync). */
+int sphx_iisph_pressure_iteration(sphx_context* ctx, double* mean_signed,
+                                  double* mean_abs_rel);
+/* Iterate to options.tol (min_iters, max_iters); returns iterations done and
+ * the final FP64 error.  The convergence test runs on the device. */
+int sphx_iisph_pressure_solve(sphx_context* ctx, int* iters, double* err);
+int sphx_iisph_integrate(sphx_context* ctx);        /* :1052 */
+/* sort + density + predict + solve + integrate (fluid_system.cpp:567-588
+ * with Integration enabled). */
+int sphx_iisph_step(sphx_context* ctx, int* iters, double* err);
+/* IISPH per-slot buffers (any pointer may be NULL); vectors are 3*num. */
+int sphx_download_iisph(sphx_context* ctx, float* dii, float* dijpj,
+                        float* aii, float* rho_adv, float* press_l,
+                        float* press_l1, float* residue);
+int sphx_upload_iisph(sphx_context* ctx, const float* dii, const float* dijpj,
+                      const float* aii, const float* rho_adv,
+                      const float* press_l);
+
+/* ------------------------------------------------------------------------ */
+/* Kernel timing (HIP events on the context stream).                         */
+/* ------------------------------------------------------------------------ */
+int sphx_timing_enable(sphx_context* ctx, int on);
+int sphx_timing_reset(sphx_context* ctx);
+/* Accumulated device time (ms) and launch count of the named kernel. */
+int sphx_timing_get(sphx_context* ctx, const char* kernel, double* total_ms,
+                    int* launches);
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic scenes (host only; no GPU needed).                              */
+/* ------------------------------------------------------------------------ */
+
+/* FluidSystem::SetupGrid (fluid_system.cpp:119-145): from volmin, volmax,
+ * cellsize and simscale fill gridmin/max/size/res/total, gridIdfac and
+ * neighborid with the reference's float/double expression types. */
+void sphx_setup_grid(sphx_ParamCarrier* p);
+/* Kernel constants of fluid_system.cpp:308-313 from smoothradius
+ * (cellsize = 2h, kpoly6, kspiky, kspikydiff, klaplacian, kspline). */
+void sphx_setup_kernels(sphx_ParamCarrier* p);
+
+/* Counts for a scene so callers can size arrays; returns total particles. */
+typedef struct sphx_SceneInfo {
+  int n_fluid;
+  int n_boundary;
+  int n_total;
+  int n_phase;
+  float spacing;
+} sphx_SceneInfo;
+
+/* Single-phase dam break: nx*ny*nz fluid lattice of spacing s in the corner
+ * of a box [0,Lx]x[0,Ly]x[0,Lz] (metres) with a boundary-particle box of
+ * `layers` layers; jitter +-jitter from seed.  kernel_family as in
+ * sphx_SolverOptions.wcsph_kernel.  Fills params (grid, constants) and the
+ * particle arrays (may be NULL to query sizes). */
+int sphx_scene_dambreak(int nx, int ny, int nz, float s, float Lx, float Ly,
+                        float Lz, float h, int layers, float jitter,
+                        uint64_t seed, int kernel_family,
+                        sphx_ParamCarrier* params, sphx_displayPack* disp,
+                        sphx_calculationPack* calc, sphx_SceneInfo* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPHX_H */

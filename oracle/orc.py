@@ -1,0 +1,165 @@
+"""Python binding of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+this module, as the checker or as the CPU baseline.  It converts the
+reference's AoS particle arrays into the oracle's typed SoA state, runs the
+restated reference kernels in FP32 (faithful) or FP64 (twin), and converts
+back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import sys
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "liboracle.so"
+sys.path.insert(0, str(_HERE.parent / "sph-multiphase_amd"))
+import sphx  # noqa: E402  (types only)
+
+UNDEF = 0xFFFFFFFF
+
+
+class OrcState(C.Structure):
+    _fields_ = [("n", C.c_int), ("gridtotal", C.c_int)] + [
+        (name, C.c_void_p) for name in
+        ["pos", "type", "vel", "veleval", "accel", "pressure", "dens", "restdens", "mass",
+         "visc", "bornid", "mgcell", "midsort", "mgndx", "gridoff", "gridcnt", "MFidTable",
+         "dii", "dijpj", "aii", "rho_adv", "press_l", "press_l1", "residue"]]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} not built (make -C oracle)")
+        _lib = C.CDLL(str(LIB_PATH))
+        _lib.orc_f32_ii_solve.restype = C.c_int
+        _lib.orc_f64_ii_solve.restype = C.c_int
+    return _lib
+
+
+VEC = ["pos", "vel", "veleval", "accel", "dii", "dijpj"]
+SCAL = ["pressure", "dens", "restdens", "mass", "visc", "aii", "rho_adv", "press_l",
+        "press_l1", "residue"]
+
+
+class State:
+    """Oracle state for n particles in a grid of gridtotal cells."""
+
+    def __init__(self, disp, calc, params: sphx.ParamCarrier, dtype=np.float32):
+        n = len(disp)
+        self.dtype = np.dtype(dtype)
+        self.params = params
+        self.n = n
+        self.a = {}
+        for k in VEC:
+            self.a[k] = np.zeros((n, 3), self.dtype)
+        for k in SCAL:
+            self.a[k] = np.zeros(n, self.dtype)
+        self.a["pos"][:] = disp["pos"]
+        self.a["type"] = np.ascontiguousarray(disp["type"], np.int32)
+        for k in ["vel", "veleval", "accel"]:
+            self.a[k][:] = calc[k]
+        for k in ["pressure", "dens", "restdens", "mass", "visc"]:
+            self.a[k][:] = calc[k]
+        self.a["bornid"] = np.ascontiguousarray(calc["bornid"], np.int32)
+        gt = params.gridtotal
+        self.a["mgcell"] = np.zeros(n, np.uint32)
+        self.a["midsort"] = np.zeros(n, np.uint32)
+        self.a["mgndx"] = np.zeros(n, np.uint32)
+        self.a["gridoff"] = np.zeros(gt, np.int32)
+        self.a["gridcnt"] = np.zeros(gt, np.int32)
+        self.a["MFidTable"] = np.full(n, -1, np.int32)
+        self.s = OrcState()
+        self.s.n = n
+        self.s.gridtotal = gt
+        for name, _ in OrcState._fields_[2:]:
+            setattr(self.s, name, self.a[name].ctypes.data)
+        self.opts = sphx.default_options()
+
+    @property
+    def pfx(self):
+        return "orc_f32_" if self.dtype == np.float32 else "orc_f64_"
+
+    def __getitem__(self, k):
+        return self.a[k]
+
+    def copy_grid_from(self, other: "State"):
+        for k in ["mgcell", "midsort", "mgndx", "gridoff", "gridcnt", "MFidTable"]:
+            self.a[k][:] = other.a[k]
+
+    # -- grid build (FP32 positions, integer outputs) --
+    def grid_build(self, mfid_mode=0):
+        L = lib()
+        if self.dtype == np.float32:
+            L.orc_initial_sort(C.byref(self.params), C.byref(self.s))
+            L.orc_sort_by_key(C.byref(self.s))
+            L.orc_cell_ranges(C.byref(self.s))
+            L.orc_rearrange(C.byref(self.s), mfid_mode)
+            return
+        # FP64 twin: the hash is defined on FP32 positions (kern.cu:52), so
+        # the index arrays come from an FP32 build and the same gather is
+        # applied to the FP64 fields
+        disp = np.zeros(self.n, sphx.display_dtype)
+        calc = np.zeros(self.n, sphx.calc_dtype)
+        disp["pos"] = self.a["pos"].astype(np.float32)
+        disp["type"] = self.a["type"]
+        calc["bornid"] = self.a["bornid"]
+        f = State(disp, calc, self.params, np.float32)
+        f.grid_build(mfid_mode)
+        self.copy_grid_from(f)
+        valid = f.a["mgcell"] != UNDEF
+        src = np.where(valid, f.a["midsort"], np.arange(self.n)).astype(np.int64)
+        for k in ["pos", "vel", "veleval", "accel", "pressure", "dens", "restdens", "mass",
+                  "visc", "type", "bornid"]:
+            self.a[k][:] = self.a[k][src]
+        self.a["pos"][~valid] = -1000.0
+
+    def hash_only(self):
+        lib().orc_initial_sort(C.byref(self.params), C.byref(self.s))
+
+    def _call(self, name, with_opts=True):
+        f = getattr(lib(), self.pfx + name)
+        if with_opts:
+            f(C.byref(self.params), C.byref(self.opts), C.byref(self.s))
+        else:
+            f(C.byref(self.params), C.byref(self.s))
+
+    def boundary_volume(self): self._call("boundary_volume")
+    def density_pressure(self): self._call("density_pressure")
+    def force(self): self._call("force")
+    def advance(self): self._call("advance", False)
+    def ii_density(self): self._call("ii_density", False)
+    def ii_dii(self): self._call("ii_dii")
+    def ii_aii(self): self._call("ii_aii")
+    def ii_dp(self): self._call("ii_dp")
+    def ii_iter(self): self._call("ii_iter")
+    def ii_integrate(self): self._call("ii_integrate", False)
+
+    def ii_iteration(self):
+        ms, ma = C.c_double(), C.c_double()
+        getattr(lib(), self.pfx + "ii_iteration")(C.byref(self.params), C.byref(self.opts),
+                                                  C.byref(self.s), C.byref(ms), C.byref(ma))
+        return ms.value, ma.value
+
+    def ii_solve(self):
+        err = C.c_double()
+        it = getattr(lib(), self.pfx + "ii_solve")(C.byref(self.params), C.byref(self.opts),
+                                                   C.byref(self.s), C.byref(err))
+        return it, err.value
+
+
+def relerr(a, b, ref=None):
+    """max |a-b| / max |ref| (normwise relative, infinity norm)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    r = np.asarray(b if ref is None else ref, np.float64)
+    den = np.max(np.abs(r)) if r.size else 0.0
+    num = np.max(np.abs(a - b)) if a.size else 0.0
+    return num / den if den > 0 else num

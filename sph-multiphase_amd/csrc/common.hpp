@@ -1,0 +1,190 @@
+// Internal definitions shared by the HIP translation units of libsphx.
+//
+// Device layout (private; the public boundary is the AoS of sphx.h):
+//   particles live in cell-sorted order as structure-of-arrays; the fields
+//   read by neighbour loops are packed into 16-byte records so one
+//   global_load_dwordx4 fetches what a neighbour visit needs:
+//     pos  = {x, y, z, type bits}        vel  = {vx, vy, vz, visc}
+//     phys = {mass, dens, pressure, restdens}
+//   Fields no hot-path kernel reads (color, deformGrad, X, invA and, until
+//   MPM is enabled, stress/B/solidG/solidK) stay in upload order and are
+//   reached through `perm` (sorted slot -> upload slot), so the per-step
+//   reorder moves ~100 B per particle instead of the reference's 512 B
+//   (fluid_system_host.cu:192-202).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <algorithm>
+#include <cstdio>
+#include <string>
+#include <vector>
+#include <map>
+
+#include "../../include/sphx.h"
+
+namespace sphx {
+
+// ---------------------------------------------------------------- errors --
+void set_error(const std::string& msg);
+int fail(const char* where, const char* what);
+int hip_fail(const char* where, hipError_t e);
+
+#define SPHX_HIP(call)                                                  \
+  do {                                                                  \
+    hipError_t e__ = (call);                                            \
+    if (e__ != hipSuccess) return ::sphx::hip_fail(#call, e__);         \
+  } while (0)
+
+#define SPHX_CHECK(cond, where, what)                                   \
+  do {                                                                  \
+    if (!(cond)) return ::sphx::fail(where, what);                      \
+  } while (0)
+
+#define SPHX_TRY(call)                                                  \
+  do {                                                                  \
+    int r__ = (call);                                                   \
+    if (r__ != 0) return r__;                                           \
+  } while (0)
+
+// ------------------------------------------------------------ device math --
+__host__ __device__ inline float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+__host__ __device__ inline float3 xyz(const float4& a) { return make_float3(a.x, a.y, a.z); }
+__host__ __device__ inline float3 add(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ inline float3 sub(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ inline float3 mul(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__host__ __device__ inline float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline int type_of(const float4& p) { return __float_as_int(p.w); }
+
+// Cubic spline kernel W and gradient, support 2h (fluid_system_kern.cu:637-671).
+// `r` is x_i - x_j in metres.
+__device__ inline float cubic_w(float rmod, float inv_h, float kspline) {
+  float q = rmod * inv_h;
+  if (q > 2.f) return 0.f;
+  if (q > 1.f) { float t = 2.f - q; return 0.25f * t * t * t * kspline; }
+  return (1.f - 1.5f * q * q + 0.75f * q * q * q) * kspline;
+}
+// Returns the scalar f with grad W = r * f (0 outside the support).
+__device__ inline float cubic_gradf(float rmod, float inv_rmod, float inv_h, float kspline) {
+  float q = rmod * inv_h;
+  if (q > 2.f) return 0.f;
+  if (q > 1.f) { float t = 2.f - q; return -0.75f * t * t * inv_h * inv_rmod * kspline; }
+  return (-3.f * q + 2.25f * q * q) * inv_h * inv_rmod * kspline;
+}
+
+// ------------------------------------------------------- kernel constants --
+// Passed by value to every kernel (no __device__ parameter symbol, so
+// several contexts can coexist and launches are graph-capturable).
+struct KP {
+  int n;
+  int gridtotal;
+  float ss;          // simscale
+  float h, h2, inv_h;
+  float kpoly6, kspikydiff, kspline;
+  float dt;
+  float3 g;
+  float bvisc;
+  float3 gridmin, gridfac;
+  int3 gridres;
+  int rowoff[9];     // neighborid[3r]: the dx=-1 cell of neighbour row r
+  // options
+  float omega, warm;
+  int full_support, boundary_pressure;
+  int wcsph_kernel;
+  float eos_B, eos_gamma, art_alpha, c0;
+  int clamp;
+  int clamp_p;
+};
+
+// ----------------------------------------------------------------- buffer --
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count) {
+    if (count <= n && p) return 0;
+    release();
+    if (count == 0) return 0;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+    if (e != hipSuccess) { p = nullptr; n = 0; return hip_fail("hipMalloc", e); }
+    n = count;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct Timer {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  std::vector<std::string> names;
+  std::map<std::string, std::pair<double, int>> acc;
+};
+
+}  // namespace sphx
+
+// ---------------------------------------------------------------- context --
+struct sphx_context {
+  sphx_ParamCarrier P;
+  sphx_SolverOptions O;
+  int num = 0;
+  int cap = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  // hot particle fields, double-buffered for the reorder gather
+  sphx::DevBuf<float4> pos[2], vel[2], veleval[2], accel[2], phys[2];
+  sphx::DevBuf<int> bornid[2], perm[2];
+  int cur = 0;
+  // cold fields in upload order (reached through perm)
+  sphx::DevBuf<float4> cold_color;
+  sphx::DevBuf<float> cold_mat;   // 9*4 mats: deformGrad, stress, B, invA
+  sphx::DevBuf<float4> cold_X;    // X, w unused
+  sphx::DevBuf<float2> cold_GK;   // solidG, solidK
+
+  // grid
+  sphx::DevBuf<uint32_t> key, idsort, key_tmp, val_tmp, block_hist;
+  sphx::DevBuf<int> cell_start;   // gridtotal + 1 entries (dense lower bounds)
+  sphx::DevBuf<int> mfid;         // bornid -> slot
+  int key_bits = 1;
+  bool sorted_valid = false;
+
+  // IISPH per-slot buffers (not permuted, as in the reference)
+  sphx::DevBuf<float4> dii;       // d_ii.xyz, a_ii
+  sphx::DevBuf<float4> dijpj;     // sum_j d_ij p_j .xyz, unused
+  sphx::DevBuf<float4> qm;        // (d_ii p + sum d_ij p_j).xyz, mass
+  sphx::DevBuf<float> press_l, press_l1, rho_adv, residue, sfac;
+  sphx::DevBuf<double> red;       // reduction partials
+  sphx::DevBuf<int> solve_state;  // [0]=done [1]=iters
+  sphx::DevBuf<double> solve_err; // [0]=last mean |rho-rho0|/rho0 [1]=mean signed
+  double* host_pinned = nullptr;  // 4 doubles: done, iters, err, signed
+
+  // timing
+  bool timing = false;
+  sphx::Timer timer;
+
+  float4* POS() { return pos[cur].p; }
+  float4* VEL() { return vel[cur].p; }
+  float4* VELEVAL() { return veleval[cur].p; }
+  float4* ACCEL() { return accel[cur].p; }
+  float4* PHYS() { return phys[cur].p; }
+  int* BORN() { return bornid[cur].p; }
+  int* PERM() { return perm[cur].p; }
+};
+
+namespace sphx {
+
+KP make_kp(const sphx_context* c);
+int ensure_device(sphx_context* c);
+// Time the enclosed launch(es) when timing is on.
+void timer_begin(sphx_context* c, const char* name);
+void timer_end(sphx_context* c);
+int timer_flush(sphx_context* c);
+
+inline int blocks_for(int n, int threads) { return (n + threads - 1) / threads; }
+
+}  // namespace sphx

@@ -1,0 +1,354 @@
+// Spatial-hash grid build: cell hash, stable LSD radix sort, cell ranges,
+// reorder gather.
+//
+// Reference: InitialSort / CalcFirstCnt / GetCnt / RearrangeData
+// (fluid_system_kern.cu:36-121) and thrust::sort_by_key
+// (fluid_system_host.cu:147-210).  Outputs are bit-identical to a stable
+// sort of (cell, index) with GRID_UNDEF last.
+//
+// Design:
+//  * keys are cell ids in [0, gridtotal); out-of-grid particles get the
+//    sentinel `gridtotal`, so a key needs only bit_width(gridtotal) bits
+//    (18 for a 52^3 grid) and the sort runs ceil(bits/9) passes of <= 9 bits.
+//  * each pass is upsweep (per-tile digit histogram) -> one-workgroup scan
+//    of the digit-major histogram -> downsweep (stable in-tile ranking with
+//    wave64 ballots: lanes holding equal digits find each other with one
+//    ballot per digit bit; a per-wave running count in LDS orders the
+//    rounds; a block prefix over the four waves orders the waves).
+//  * cell ranges are stored dense: cell_start[c] = number of valid particles
+//    in cells < c, for every c in [0, gridtotal].  The reference's
+//    mgridoff/mgridcnt are derived from it on download.  Because cell ids
+//    are x-fastest (kern.cu:55), the three cells dx=-1..1 of a neighbour
+//    row are contiguous and a row is the single range
+//    [cell_start[c-1], cell_start[c+2]).
+#include "common.hpp"
+
+namespace sphx {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // 4096 keys per workgroup
+constexpr int RS_MAX_BITS = 9;
+constexpr int RS_MAX_BINS = 1 << RS_MAX_BITS;
+constexpr int RS_WAVES = RS_THREADS / 64;
+
+// --------------------------------------------------------------- hashing --
+// InitialSort (kern.cu:36-64): gc = int((pos - gridmin) * gridIdfac), valid
+// when 1 <= gc < res-1 on every axis; key = (gy*resz + gz)*resx + gx.
+__global__ void k_hash(const float4* __restrict__ pos, uint32_t* __restrict__ key,
+                       uint32_t* __restrict__ val, int n, float3 gmin, float3 fac, int3 res,
+                       uint32_t sentinel) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = pos[i];
+  // plain FP32 subtract then multiply (no contraction possible), truncation
+  float fx = (p.x - gmin.x) * fac.x;
+  float fy = (p.y - gmin.y) * fac.y;
+  float fz = (p.z - gmin.z) * fac.z;
+  int gx = (int)fx, gy = (int)fy, gz = (int)fz;
+  uint32_t k = sentinel;
+  if (gx >= 1 && gx < res.x - 1 && gy >= 1 && gy < res.y - 1 && gz >= 1 && gz < res.z - 1)
+    k = (uint32_t)((gy * res.z + gz) * res.x + gx);
+  key[i] = k;
+  val[i] = (uint32_t)i;
+}
+
+// Lanes of the wave whose `digit` equals this lane's (only `active` lanes).
+__device__ inline uint64_t match_digit(uint32_t digit, int bits, uint64_t active) {
+  uint64_t m = active;
+  for (int b = 0; b < bits; ++b) {
+    uint64_t bal = __ballot((digit >> b) & 1u);
+    m &= ((digit >> b) & 1u) ? bal : ~bal;
+  }
+  return m;
+}
+
+__device__ inline uint64_t lanemask_lt() {
+  int lane = threadIdx.x & 63;
+  return (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_upsweep(const uint32_t* __restrict__ keys,
+                                                            int n, int shift, int bits,
+                                                            uint32_t* __restrict__ hist,
+                                                            int nblocks) {
+  __shared__ uint32_t h[RS_MAX_BINS];
+  const int bins = 1 << bits;
+  const uint32_t mask = (uint32_t)bins - 1u;
+  for (int t = threadIdx.x; t < bins; t += RS_THREADS) h[t] = 0;
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int base = blockIdx.x * RS_TILE + wave * (64 * RS_ITEMS);
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    int idx = base + it * 64 + lane;
+    bool valid = idx < n;
+    uint32_t d = valid ? ((keys[idx] >> shift) & mask) : 0u;
+    uint64_t active = __ballot(valid);
+    if (!valid) continue;
+    uint64_t peers = match_digit(d, bits, active);
+    if ((peers & lanemask_lt()) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < bins; t += RS_THREADS) hist[(size_t)t * nblocks + blockIdx.x] = h[t];
+}
+
+// Exclusive scan of `count` entries by one workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ a, int count) {
+  __shared__ uint32_t part[1024];
+  const int t = threadIdx.x;
+  const int per = (count + 1023) / 1024;
+  const int b = t * per;
+  const int e = min(count, b + per);
+  uint32_t s = 0;
+  for (int i = b; i < e; ++i) s += a[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    uint32_t v = (t >= off) ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - s;   // exclusive prefix of this chunk
+  for (int i = b; i < e; ++i) {
+    uint32_t v = a[i];
+    a[i] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_downsweep(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int n, int shift, int bits,
+    const uint32_t* __restrict__ hist, int nblocks) {
+  __shared__ uint32_t wcnt[RS_WAVES][RS_MAX_BINS];
+  __shared__ uint32_t goff[RS_MAX_BINS];
+  const int bins = 1 << bits;
+  const uint32_t mask = (uint32_t)bins - 1u;
+  for (int t = threadIdx.x; t < bins; t += RS_THREADS) {
+    goff[t] = hist[(size_t)t * nblocks + blockIdx.x];
+    for (int w = 0; w < RS_WAVES; ++w) wcnt[w][t] = 0;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int base = blockIdx.x * RS_TILE + wave * (64 * RS_ITEMS);
+  uint32_t kk[RS_ITEMS], vv[RS_ITEMS], rk[RS_ITEMS];
+  const uint64_t lt = lanemask_lt();
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    int idx = base + it * 64 + lane;
+    bool valid = idx < n;
+    uint32_t k = valid ? kin[idx] : 0u;
+    uint32_t v = valid ? vin[idx] : 0u;
+    uint32_t d = (k >> shift) & mask;
+    uint64_t active = __ballot(valid);
+    uint64_t peers = match_digit(d, bits, active);
+    uint32_t prev = 0;
+    if (valid) prev = wcnt[wave][d];
+    // every peer has read `prev` before the group leader publishes the
+    // new running count (same wave, program order on the LDS queue)
+    __builtin_amdgcn_wave_barrier();
+    if (valid && (peers & lt) == 0) wcnt[wave][d] = prev + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
+    kk[it] = k;
+    vv[it] = v;
+    rk[it] = valid ? prev + (uint32_t)__popcll(peers & lt) : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < bins; t += RS_THREADS) {
+    uint32_t s = 0;
+    for (int w = 0; w < RS_WAVES; ++w) {
+      uint32_t c = wcnt[w][t];
+      wcnt[w][t] = s;
+      s += c;
+    }
+  }
+  __syncthreads();
+  for (int it = 0; it < RS_ITEMS; ++it) {
+    if (rk[it] == 0xFFFFFFFFu) continue;
+    uint32_t d = (kk[it] >> shift) & mask;
+    uint32_t dst = goff[d] + wcnt[wave][d] + rk[it];
+    kout[dst] = kk[it];
+    vout[dst] = vv[it];
+  }
+}
+
+// cell_start[c] = lower_bound(sorted keys, c) for c in [0, gridtotal].
+__global__ void k_cell_start(const uint32_t* __restrict__ skey, int n, int gridtotal,
+                             int* __restrict__ cell_start) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  int prev = (i == 0) ? -1 : (int)skey[i - 1];
+  int cur = (i == n) ? gridtotal : (int)skey[i];
+  for (int c = prev + 1; c <= cur; ++c) cell_start[c] = i;
+}
+
+// RearrangeData (kern.cu:98-121): slot s takes the particle idsort[s]; an
+// out-of-grid slot keeps its own pre-sort data with pos = (-1000)^3 (the
+// reference does not copy calc data there, A9).  perm follows the same
+// gather so cold fields stay reachable; mfid maps bornid -> slot (the
+// intended inverse permutation, A8).
+__global__ void k_reorder(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ idsort,
+                          int n, uint32_t sentinel,
+                          const float4* __restrict__ pos_in, float4* __restrict__ pos_out,
+                          const float4* __restrict__ vel_in, float4* __restrict__ vel_out,
+                          const float4* __restrict__ ve_in, float4* __restrict__ ve_out,
+                          const float4* __restrict__ acc_in, float4* __restrict__ acc_out,
+                          const float4* __restrict__ phys_in, float4* __restrict__ phys_out,
+                          const int* __restrict__ born_in, int* __restrict__ born_out,
+                          const int* __restrict__ perm_in, int* __restrict__ perm_out,
+                          int* __restrict__ mfid) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  bool valid = skey[s] != sentinel;
+  int src = valid ? (int)idsort[s] : s;
+  float4 p = pos_in[src];
+  if (!valid) { p.x = -1000.f; p.y = -1000.f; p.z = -1000.f; }
+  pos_out[s] = p;
+  vel_out[s] = vel_in[src];
+  ve_out[s] = ve_in[src];
+  acc_out[s] = acc_in[src];
+  phys_out[s] = phys_in[src];
+  int b = born_in[src];
+  born_out[s] = b;
+  perm_out[s] = perm_in[src];
+  if (valid && b >= 0 && b < n) mfid[b] = s;
+}
+
+__global__ void k_fill_i32(int* p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+int grid_hash(sphx_context* c) {
+  int n = c->num;
+  c->sorted_valid = false;
+  if (n == 0) return 0;
+  KP k = make_kp(c);
+  timer_begin(c, "hash");
+  k_hash<<<blocks_for(n, 256), 256, 0, c->stream>>>(c->POS(), c->key.p, c->idsort.p, n, k.gridmin,
+                                                      k.gridfac, k.gridres,
+                                                      (uint32_t)c->P.gridtotal);
+  timer_end(c);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+int grid_sort(sphx_context* c) {
+  int n = c->num;
+  if (n == 0) {
+    SPHX_HIP(hipMemsetAsync(c->cell_start.p, 0, ((size_t)c->P.gridtotal + 1) * sizeof(int),
+                            c->stream));
+    c->sorted_valid = true;
+    return 0;
+  }
+  int nblocks = blocks_for(n, RS_TILE);
+  int passes = (c->key_bits + RS_MAX_BITS - 1) / RS_MAX_BITS;
+  int per = (c->key_bits + passes - 1) / passes;
+  SPHX_TRY(c->block_hist.alloc((size_t)nblocks * RS_MAX_BINS));
+  uint32_t *ka = c->key.p, *va = c->idsort.p, *kb = c->key_tmp.p, *vb = c->val_tmp.p;
+  int shift = 0;
+  timer_begin(c, "radix_sort");
+  for (int ps = 0; ps < passes; ++ps) {
+    int bits = std::min(per, c->key_bits - shift);
+    int bins = 1 << bits;
+    k_rs_upsweep<<<nblocks, RS_THREADS, 0, c->stream>>>(ka, n, shift, bits, c->block_hist.p, nblocks);
+    k_rs_scan<<<1, 1024, 0, c->stream>>>(c->block_hist.p, bins * nblocks);
+    k_rs_downsweep<<<nblocks, RS_THREADS, 0, c->stream>>>(ka, va, kb, vb, n, shift, bits,
+                                                          c->block_hist.p, nblocks);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+    shift += bits;
+  }
+  timer_end(c);
+  SPHX_HIP(hipGetLastError());
+  if (ka != c->key.p) {
+    SPHX_HIP(hipMemcpyAsync(c->key.p, ka, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+    SPHX_HIP(hipMemcpyAsync(c->idsort.p, va, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+  }
+  timer_begin(c, "cell_start");
+  k_cell_start<<<blocks_for(n + 1, 256), 256, 0, c->stream>>>(c->key.p, n, c->P.gridtotal,
+                                                               c->cell_start.p);
+  timer_end(c);
+  SPHX_HIP(hipGetLastError());
+  c->sorted_valid = true;
+  return 0;
+}
+
+int grid_reorder(sphx_context* c) {
+  int n = c->num;
+  if (n == 0) return 0;
+  SPHX_CHECK(c->sorted_valid, "sphx_grid_reorder", "grid not sorted (call hash and sort first)");
+  int a = c->cur, b = 1 - c->cur;
+  k_fill_i32<<<blocks_for(c->cap, 256), 256, 0, c->stream>>>(c->mfid.p, c->cap, -1);
+  timer_begin(c, "reorder");
+  k_reorder<<<blocks_for(n, 256), 256, 0, c->stream>>>(
+      c->key.p, c->idsort.p, n, (uint32_t)c->P.gridtotal, c->pos[a].p, c->pos[b].p, c->vel[a].p,
+      c->vel[b].p, c->veleval[a].p, c->veleval[b].p, c->accel[a].p, c->accel[b].p, c->phys[a].p,
+      c->phys[b].p, c->bornid[a].p, c->bornid[b].p, c->perm[a].p, c->perm[b].p, c->mfid.p);
+  timer_end(c);
+  SPHX_HIP(hipGetLastError());
+  c->cur = b;
+  return 0;
+}
+
+}  // namespace sphx
+
+using namespace sphx;
+
+extern "C" {
+
+int sphx_grid_hash(sphx_context* c) {
+  SPHX_CHECK(c, "sphx_grid_hash", "null context");
+  SPHX_TRY(ensure_device(c));
+  return grid_hash(c);
+}
+
+int sphx_grid_sort(sphx_context* c) {
+  SPHX_CHECK(c, "sphx_grid_sort", "null context");
+  SPHX_TRY(ensure_device(c));
+  return grid_sort(c);
+}
+
+int sphx_grid_reorder(sphx_context* c) {
+  SPHX_CHECK(c, "sphx_grid_reorder", "null context");
+  SPHX_TRY(ensure_device(c));
+  return grid_reorder(c);
+}
+
+int sphx_grid_build(sphx_context* c) {
+  SPHX_CHECK(c, "sphx_grid_build", "null context");
+  SPHX_TRY(ensure_device(c));
+  SPHX_TRY(grid_hash(c));
+  SPHX_TRY(grid_sort(c));
+  return grid_reorder(c);
+}
+
+int sphx_download_grid(sphx_context* c, uint32_t* mgcell, uint32_t* midsort, uint32_t* mgndx,
+                       int32_t* gridoff, int32_t* gridcnt, int32_t* MFidTable) {
+  SPHX_CHECK(c, "sphx_download_grid", "null context");
+  SPHX_TRY(ensure_device(c));
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  int n = c->num, gt = c->P.gridtotal;
+  std::vector<uint32_t> key(n);
+  std::vector<int> start((size_t)gt + 1);
+  if (n > 0) SPHX_HIP(hipMemcpy(key.data(), c->key.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  SPHX_HIP(hipMemcpy(start.data(), c->cell_start.p, ((size_t)gt + 1) * sizeof(int),
+                     hipMemcpyDeviceToHost));
+  if (mgcell)
+    for (int i = 0; i < n; ++i) mgcell[i] = key[i] == (uint32_t)gt ? SPHX_GRID_UNDEF : key[i];
+  if (midsort && n > 0)
+    SPHX_HIP(hipMemcpy(midsort, c->idsort.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (mgndx)
+    for (int i = 0; i < n; ++i)
+      mgndx[i] = key[i] == (uint32_t)gt ? SPHX_GRID_UNDEF : (uint32_t)(i - start[key[i]]);
+  for (int cidx = 0; cidx < gt; ++cidx) {
+    int cnt = start[cidx + 1] - start[cidx];
+    if (gridcnt) gridcnt[cidx] = cnt;
+    if (gridoff) gridoff[cidx] = cnt > 0 ? start[cidx] : 0;
+  }
+  if (MFidTable && n > 0)
+    SPHX_HIP(hipMemcpy(MFidTable, c->mfid.p, n * sizeof(int), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // extern "C"

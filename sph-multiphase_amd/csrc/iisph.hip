@@ -1,0 +1,550 @@
+// IISPH: density, advection prediction (d_ii, a_ii, rho_adv), relaxed
+// Jacobi pressure iterations with an on-device FP64 convergence test, and
+// integration.
+//
+// Reference: fluid_system_kern.cu:635-1115 (kern, nablakern, ComputeDii,
+// ComputeAii, Pressure_DP, Pressure_Iter, IntegrateIISPH,
+// ComputeDensityIISPH) and fluid_system_host.cu:505-556 (PressureSolve,
+// whose loop never terminates; here it stops at the density tolerance).
+//
+// Data flow of one iteration l (same arithmetic as Pressure_DP +
+// Pressure_Iter, regrouped so no kernel reads a value another thread of the
+// same kernel writes):
+//   A  dijpj_i = -dt^2 sum_j s_j gradW_ij      with s_j = m_j p_j / rho_j^2
+//      q_i     = d_ii p_i + dijpj_i            (= a_p,i dt^2)
+//   B  S_i     = sum_j m_j (dijpj_i - q_j + dt^2 m_i p_i / rho_i^2 gradW_ij) . gradW_ij
+//      p_i    <- (1-w) p_i + w/a_ii (rho0 - rho_adv - S_i)
+//      residue = rho_adv + S_i + a_ii p_i(old) - rho0,   s_i <- m_i p_i(new)/rho_i^2
+// B touches p only at its own index, so the update is in place and the
+// reference's press_l1 -> press_l copy (host.cu:536) is not needed.
+#include "nbr.hpp"
+
+namespace sphx {
+
+constexpr int NB_THREADS = 256;
+
+struct IterCtl {
+  int min_iters, max_iters;
+  float tol;
+};
+
+__device__ __forceinline__ bool in_support(const KP& k, float dsq) {
+  float lim = k.full_support ? 4.f * k.h2 : k.h2;
+  return dsq < lim && dsq > 0.f;
+}
+
+// ComputeDensityIISPH + contributeIIDensity (kern.cu:1067-1115): rho = sum m W
+__global__ void k_ii_density(KP k, const uint32_t* __restrict__ key, const int* __restrict__ cs,
+                             const float4* __restrict__ pos, float4* __restrict__ phys) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.n) return;
+  float4 pi = pos[i];
+  if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
+  uint32_t gc = key[i];
+  if ((int)gc == k.gridtotal) return;
+  const float rest = phys[i].w;
+  float sum = 0.f;
+  for_each_neighbor(k, cs, gc, [&](int j) {
+    float4 pj = pos[j];
+    float4 qj = phys[j];
+    float mj = (type_of(pj) == SPHX_TYPE_BOUNDARY) ? rest * qj.y : qj.x;
+    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+    float r = sqrtf(dx * dx + dy * dy + dz * dz);
+    sum += cubic_w(r, k.inv_h, k.kspline) * mj;
+  });
+  phys[i].y = sum;
+}
+
+// ComputeDii (kern.cu:673-798): v_adv = v + dt (a_bvisc + g);
+// d_ii = -dt^2 / rho_i^2 sum_j m_j gradW_ij
+__global__ void k_ii_dii(KP k, const uint32_t* __restrict__ key, const int* __restrict__ cs,
+                         const float4* __restrict__ pos, const float4* __restrict__ vel,
+                         const float4* __restrict__ phys, float4* __restrict__ veleval,
+                         float4* __restrict__ dii) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.n) return;
+  float4 pi = pos[i];
+  if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
+  uint32_t gc = key[i];
+  if ((int)gc == k.gridtotal) return;
+  const float4 vi = vel[i];
+  const float4 qi = phys[i];
+  const float rest = qi.w;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  float dx_ = 0.f, dy_ = 0.f, dz_ = 0.f;
+  for_each_neighbor(k, cs, gc, [&](int j) {
+    float4 pj = pos[j];
+    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+    float dsq = dx * dx + dy * dy + dz * dz;
+    float r = sqrtf(dsq);
+    if (!(r < 2.f * k.h && r > 0.f)) return;
+    float gf = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline);
+    int tj = type_of(pj);
+    float4 qj = phys[j];
+    if (tj == SPHX_TYPE_BOUNDARY) {
+      // artificial boundary viscosity (:712-723)
+      float4 vj = vel[j];
+      float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+      float pij = dx * vx + dy * vy + dz * vz;
+      if (pij < 0.f) {
+        pij = pij / (r * r + k.h2 * 0.01f);
+        pij = pij * k.bvisc * k.h / 2.f / rest;
+        pij = rest * qj.y * pij;
+        ax += dx * gf * pij; ay += dy * gf * pij; az += dz * gf * pij;
+      }
+    }
+    bool sup = k.full_support ? true : (r < k.h);
+    if (!sup) return;
+    float mj;
+    if (tj != SPHX_TYPE_BOUNDARY) mj = qj.x;
+    else if (k.boundary_pressure) mj = rest * qj.y;   // :763-764
+    else return;
+    dx_ -= dx * gf * mj; dy_ -= dy * gf * mj; dz_ -= dz * gf * mj;
+  });
+  ax += k.g.x; ay += k.g.y; az += k.g.z;
+  veleval[i] = make_float4(vi.x + ax * k.dt, vi.y + ay * k.dt, vi.z + az * k.dt, 0.f);
+  float rho = qi.y;
+  float f = 1.f / rho / rho * k.dt * k.dt;
+  dii[i] = make_float4(dx_ * f, dy_ * f, dz_ * f, 0.f);
+}
+
+// ComputeAii (kern.cu:801-908): rho_adv, warm start p0 = w p(t-1), a_ii.
+__global__ void k_ii_aii(KP k, const uint32_t* __restrict__ key, const int* __restrict__ cs,
+                         const float4* __restrict__ pos, const float4* __restrict__ veleval,
+                         const float4* __restrict__ phys, float4* __restrict__ dii,
+                         float* __restrict__ rho_adv, float* __restrict__ press_l,
+                         float* __restrict__ sfac) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.n) return;
+  float4 pi = pos[i];
+  if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
+  uint32_t gc = key[i];
+  if ((int)gc == k.gridtotal) return;
+  const float4 vi = veleval[i];
+  const float4 qi = phys[i];
+  const float rest = qi.w, mi = qi.x, rhoi = qi.y;
+  const float4 di = dii[i];
+  const float cji = mi / rhoi / rhoi * k.dt * k.dt;
+  float dchange = 0.f, aii = 0.f;
+  for_each_neighbor(k, cs, gc, [&](int j) {
+    float4 pj = pos[j];
+    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+    float dsq = dx * dx + dy * dy + dz * dz;
+    if (!in_support(k, dsq)) return;
+    float r = sqrtf(dsq);
+    float gf = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline);
+    float gx = dx * gf, gy = dy * gf, gz = dz * gf;
+    int tj = type_of(pj);
+    float4 qj = phys[j];
+    float4 vj = veleval[j];
+    float vdg = (vi.x - vj.x) * gx + (vi.y - vj.y) * gy + (vi.z - vj.z) * gz;
+    if (tj != SPHX_TYPE_BOUNDARY) {
+      dchange += vdg * qj.x * k.dt;
+      // d_ji = gradW m_i / rho_i^2 dt^2; a_ii += (d_ii - d_ji) . gradW m_j
+      float tx = di.x - gx * cji, ty = di.y - gy * cji, tz = di.z - gz * cji;
+      aii += (tx * gx + ty * gy + tz * gz) * qj.x;
+    } else if (k.boundary_pressure) {
+      float mb = rest * qj.y;   // :833-834
+      dchange += vdg * mb * k.dt;
+      aii += (di.x * gx + di.y * gy + di.z * gz) * mb;
+    }
+  });
+  rho_adv[i] = rhoi + dchange;
+  float p0 = k.warm * qi.z;
+  press_l[i] = p0;
+  sfac[i] = mi * p0 / rhoi / rhoi;
+  dii[i].w = aii;
+}
+
+// Pressure_DP (kern.cu:910-958) plus q_i = d_ii p_i + dijpj_i.
+__global__ void k_ii_dp(KP k, const int* __restrict__ done, const uint32_t* __restrict__ key,
+                        const int* __restrict__ cs, const float4* __restrict__ pos,
+                        const float* __restrict__ sfac, const float4* __restrict__ phys,
+                        const float4* __restrict__ dii, const float* __restrict__ press_l,
+                        float4* __restrict__ dijpj, float4* __restrict__ qm) {
+  if (*done) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.n) return;
+  float4 pi = pos[i];
+  if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
+  uint32_t gc = key[i];
+  if ((int)gc == k.gridtotal) return;
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+  for_each_neighbor(k, cs, gc, [&](int j) {
+    float4 pj = pos[j];
+    if (type_of(pj) == SPHX_TYPE_BOUNDARY) return;
+    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+    float dsq = dx * dx + dy * dy + dz * dz;
+    if (!in_support(k, dsq)) return;
+    float r = sqrtf(dsq);
+    float gf = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline) * sfac[j];
+    sx += dx * gf; sy += dy * gf; sz += dz * gf;
+  });
+  float f = -k.dt * k.dt;
+  float4 d = make_float4(sx * f, sy * f, sz * f, 0.f);
+  dijpj[i] = d;
+  float4 di = dii[i];
+  float p = press_l[i];
+  qm[i] = make_float4(di.x * p + d.x, di.y * p + d.y, di.z * p + d.z, phys[i].x);
+}
+
+// Block reduction of (signed residue, |residue|/rho0, fluid count) in FP64.
+__device__ void block_reduce3(double a, double b, double c, double* out) {
+  __shared__ double sh[3][NB_THREADS / 64];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+    c += __shfl_down(c, off, 64);
+  }
+  if (lane == 0) { sh[0][w] = a; sh[1][w] = b; sh[2][w] = c; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0, y = 0, z = 0;
+    for (int q = 0; q < NB_THREADS / 64; ++q) { x += sh[0][q]; y += sh[1][q]; z += sh[2][q]; }
+    out[3 * blockIdx.x + 0] = x;
+    out[3 * blockIdx.x + 1] = y;
+    out[3 * blockIdx.x + 2] = z;
+  }
+}
+
+// Pressure_Iter (kern.cu:960-1050)
+__global__ __launch_bounds__(NB_THREADS) void k_ii_iter(
+    KP k, const int* __restrict__ done, const uint32_t* __restrict__ key,
+    const int* __restrict__ cs, const float4* __restrict__ pos, float4* __restrict__ phys,
+    const float4* __restrict__ dii, const float4* __restrict__ dijpj,
+    const float4* __restrict__ qm, const float* __restrict__ rho_adv,
+    float* __restrict__ press_l, float* __restrict__ press_l1, float* __restrict__ sfac,
+    float* __restrict__ residue, float4* __restrict__ accel, double* __restrict__ partial) {
+  if (*done) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double rs = 0.0, ra = 0.0, rc = 0.0;
+  if (i < k.n) {
+    float4 pi = pos[i];
+    uint32_t gc = key[i];
+    if (type_of(pi) == SPHX_TYPE_BOUNDARY || (int)gc == k.gridtotal) {
+      residue[i] = 0.f;
+    } else {
+      const float4 qi = phys[i];
+      const float mi = qi.x, rhoi = qi.y, rest = qi.w;
+      const float4 di = dii[i];
+      const float4 dpi = dijpj[i];
+      const float pl = press_l[i];
+      const float cji = k.dt * k.dt * mi / rhoi / rhoi * pl;
+      float S = 0.f;
+      for_each_neighbor(k, cs, gc, [&](int j) {
+        float4 pj = pos[j];
+        float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+        float dsq = dx * dx + dy * dy + dz * dz;
+        if (!in_support(k, dsq)) return;
+        float r = sqrtf(dsq);
+        float gf = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline);
+        float gx = dx * gf, gy = dy * gf, gz = dz * gf;
+        if (type_of(pj) != SPHX_TYPE_BOUNDARY) {
+          float4 q = qm[j];
+          float tx = dpi.x - q.x + cji * gx, ty = dpi.y - q.y + cji * gy, tz = dpi.z - q.z + cji * gz;
+          S += q.w * (tx * gx + ty * gy + tz * gz);
+        } else if (k.boundary_pressure) {
+          float mb = rest * phys[j].y;   // :1000-1002
+          S += mb * (dpi.x * gx + dpi.y * gy + dpi.z * gz);
+        }
+      });
+      float aii = di.w;
+      float pn;
+      if (fabsf(aii) < 1e-6f) pn = pl;
+      else pn = (1.f - k.omega) * pl + k.omega / aii * (rest - rho_adv[i] - S);
+      if (k.clamp_p && pn < 0.f) pn = 0.f;   // kern.cu:1035-1036 (commented there)
+      float res = -rest + (rho_adv[i] + S + aii * pl);
+      residue[i] = res;
+      float4 q = qm[i];
+      float idt2 = 1.f / (k.dt * k.dt);
+      accel[i] = make_float4(q.x * idt2, q.y * idt2, q.z * idt2, 0.f);
+      phys[i].z = pn;
+      press_l[i] = pn;
+      press_l1[i] = pn;
+      sfac[i] = mi * pn / rhoi / rhoi;
+      rs = res;
+      ra = (k.clamp_p ? fmax((double)res, 0.0) : fabs((double)res)) / (double)rest;
+      rc = 1.0;
+    }
+  }
+  block_reduce3(rs, ra, rc, partial);
+}
+
+// Sum the partials, count the iteration and decide convergence.
+__global__ __launch_bounds__(1024) void k_ii_finalize(int* __restrict__ state,
+                                                      double* __restrict__ err,
+                                                      const double* __restrict__ partial,
+                                                      int nblocks, int num, IterCtl ctl,
+                                                      int respect_done) {
+  if (respect_done && state[0]) return;
+  __shared__ double sh[3][16];
+  double a = 0, b = 0, c = 0;
+  for (int q = threadIdx.x; q < nblocks; q += 1024) {
+    a += partial[3 * q];
+    b += partial[3 * q + 1];
+    c += partial[3 * q + 2];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+    c += __shfl_down(c, off, 64);
+  }
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { sh[0][w] = a; sh[1][w] = b; sh[2][w] = c; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0, y = 0, z = 0;
+    for (int q = 0; q < 16; ++q) { x += sh[0][q]; y += sh[1][q]; z += sh[2][q]; }
+    double e = z > 0 ? y / z : 0.0;
+    err[0] = e;
+    err[1] = num > 0 ? x / num : 0.0;
+    int it = state[1] + 1;
+    state[1] = it;
+    if ((it >= ctl.min_iters && e <= (double)ctl.tol) || it >= ctl.max_iters) state[0] = 1;
+  }
+}
+
+// IntegrateIISPH (kern.cu:1052-1064); optional clamp into the valid cells.
+__global__ void k_ii_integrate(KP k, const uint32_t* __restrict__ key, float4* __restrict__ pos,
+                               float4* __restrict__ vel, const float4* __restrict__ veleval,
+                               const float4* __restrict__ accel) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.n) return;
+  float4 p = pos[i];
+  if (type_of(p) == SPHX_TYPE_BOUNDARY) return;
+  if ((int)key[i] == k.gridtotal) return;
+  float4 ve = veleval[i], a = accel[i], v = vel[i];
+  float vx = ve.x + a.x * k.dt, vy = ve.y + a.y * k.dt, vz = ve.z + a.z * k.dt;
+  p.x += vx * k.dt / k.ss;
+  p.y += vy * k.dt / k.ss;
+  p.z += vz * k.dt / k.ss;
+  if (k.clamp) {
+    // keep 1 <= cell < res-1 on each axis (reflect the normal velocity)
+    float lo[3], hi[3];
+    lo[0] = k.gridmin.x + 1.0001f / k.gridfac.x; hi[0] = k.gridmin.x + (k.gridres.x - 1.0001f) / k.gridfac.x;
+    lo[1] = k.gridmin.y + 1.0001f / k.gridfac.y; hi[1] = k.gridmin.y + (k.gridres.y - 1.0001f) / k.gridfac.y;
+    lo[2] = k.gridmin.z + 1.0001f / k.gridfac.z; hi[2] = k.gridmin.z + (k.gridres.z - 1.0001f) / k.gridfac.z;
+    if (p.x < lo[0]) { p.x = lo[0]; vx = fmaxf(vx, 0.f); }
+    if (p.x > hi[0]) { p.x = hi[0]; vx = fminf(vx, 0.f); }
+    if (p.y < lo[1]) { p.y = lo[1]; vy = fmaxf(vy, 0.f); }
+    if (p.y > hi[1]) { p.y = hi[1]; vy = fminf(vy, 0.f); }
+    if (p.z < lo[2]) { p.z = lo[2]; vz = fmaxf(vz, 0.f); }
+    if (p.z > hi[2]) { p.z = hi[2]; vz = fminf(vz, 0.f); }
+  }
+  pos[i] = p;
+  vel[i] = make_float4(vx, vy, vz, v.w);
+}
+
+int grid_hash(sphx_context* c);
+int grid_sort(sphx_context* c);
+int grid_reorder(sphx_context* c);
+
+static int need_sorted(sphx_context* c, const char* who) {
+  SPHX_CHECK(c, who, "null context");
+  SPHX_TRY(ensure_device(c));
+  SPHX_CHECK(c->sorted_valid, who, "grid not built (call sphx_grid_build first)");
+  return 0;
+}
+
+static int ensure_red(sphx_context* c) {
+  size_t nb = (size_t)blocks_for(c->cap, NB_THREADS);
+  return c->red.alloc(3 * nb + 16);
+}
+
+// one DP + Iter + finalize; respect_done=1 inside the pipelined solve
+static int enqueue_iteration(sphx_context* c, const KP& k, int respect_done) {
+  int nb = blocks_for(k.n, NB_THREADS);
+  IterCtl ctl{c->O.min_iters, c->O.max_iters, c->O.tol};
+  timer_begin(c, "iisph_dp");
+  k_ii_dp<<<nb, NB_THREADS, 0, c->stream>>>(k, c->solve_state.p, c->key.p, c->cell_start.p,
+                                            c->POS(), c->sfac.p, c->PHYS(), c->dii.p,
+                                            c->press_l.p, c->dijpj.p, c->qm.p);
+  timer_end(c);
+  timer_begin(c, "iisph_iter");
+  k_ii_iter<<<nb, NB_THREADS, 0, c->stream>>>(k, c->solve_state.p, c->key.p, c->cell_start.p,
+                                              c->POS(), c->PHYS(), c->dii.p, c->dijpj.p, c->qm.p,
+                                              c->rho_adv.p, c->press_l.p, c->press_l1.p,
+                                              c->sfac.p, c->residue.p, c->ACCEL(), c->red.p);
+  timer_end(c);
+  k_ii_finalize<<<1, 1024, 0, c->stream>>>(c->solve_state.p, c->solve_err.p, c->red.p, nb, k.n,
+                                           ctl, respect_done);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+static int read_state(sphx_context* c) {
+  SPHX_HIP(hipMemcpyAsync(c->host_pinned, c->solve_err.p, 2 * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
+  int st[2];
+  SPHX_HIP(hipMemcpyAsync(c->host_pinned + 4, c->solve_state.p, 2 * sizeof(int),
+                          hipMemcpyDeviceToHost, c->stream));
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  std::memcpy(st, c->host_pinned + 4, 2 * sizeof(int));
+  c->host_pinned[2] = st[0];
+  c->host_pinned[3] = st[1];
+  return 0;
+}
+
+}  // namespace sphx
+
+using namespace sphx;
+
+extern "C" {
+
+int sphx_iisph_density(sphx_context* c) {
+  SPHX_TRY(need_sorted(c, "sphx_iisph_density"));
+  KP k = make_kp(c);
+  if (k.n == 0) return 0;
+  timer_begin(c, "iisph_density");
+  k_ii_density<<<blocks_for(k.n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
+      k, c->key.p, c->cell_start.p, c->POS(), c->PHYS());
+  timer_end(c);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+int sphx_iisph_predict_advection(sphx_context* c) {
+  SPHX_TRY(need_sorted(c, "sphx_iisph_predict_advection"));
+  KP k = make_kp(c);
+  if (k.n == 0) return 0;
+  int nb = blocks_for(k.n, NB_THREADS);
+  timer_begin(c, "iisph_dii");
+  k_ii_dii<<<nb, NB_THREADS, 0, c->stream>>>(k, c->key.p, c->cell_start.p, c->POS(), c->VEL(),
+                                             c->PHYS(), c->VELEVAL(), c->dii.p);
+  timer_end(c);
+  timer_begin(c, "iisph_aii");
+  k_ii_aii<<<nb, NB_THREADS, 0, c->stream>>>(k, c->key.p, c->cell_start.p, c->POS(),
+                                             c->VELEVAL(), c->PHYS(), c->dii.p, c->rho_adv.p,
+                                             c->press_l.p, c->sfac.p);
+  timer_end(c);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+int sphx_iisph_pressure_iteration(sphx_context* c, double* mean_signed, double* mean_abs_rel) {
+  SPHX_TRY(need_sorted(c, "sphx_iisph_pressure_iteration"));
+  SPHX_TRY(ensure_red(c));
+  KP k = make_kp(c);
+  if (k.n == 0) return 0;
+  SPHX_HIP(hipMemsetAsync(c->solve_state.p, 0, 2 * sizeof(int), c->stream));
+  SPHX_TRY(enqueue_iteration(c, k, 0));
+  if (mean_signed || mean_abs_rel) {
+    SPHX_TRY(read_state(c));
+    if (mean_abs_rel) *mean_abs_rel = c->host_pinned[0];
+    if (mean_signed) *mean_signed = c->host_pinned[1];
+  }
+  return 0;
+}
+
+int sphx_iisph_pressure_solve(sphx_context* c, int* iters, double* err) {
+  SPHX_TRY(need_sorted(c, "sphx_iisph_pressure_solve"));
+  SPHX_TRY(ensure_red(c));
+  KP k = make_kp(c);
+  if (iters) *iters = 0;
+  if (err) *err = 0.0;
+  if (k.n == 0) return 0;
+  SPHX_HIP(hipMemsetAsync(c->solve_state.p, 0, 2 * sizeof(int), c->stream));
+  int enq = 0;
+  int chunk = std::max(c->O.min_iters, 2);
+  static thread_local int last_iters = 0;
+  if (last_iters > chunk) chunk = last_iters;
+  for (;;) {
+    int todo = std::min(chunk, c->O.max_iters - enq);
+    for (int t = 0; t < todo; ++t) SPHX_TRY(enqueue_iteration(c, k, 1));
+    enq += todo;
+    SPHX_TRY(read_state(c));
+    if (c->host_pinned[2] != 0.0 || enq >= c->O.max_iters) break;
+    chunk = 2;
+  }
+  int it = (int)c->host_pinned[3];
+  last_iters = it;
+  if (iters) *iters = it;
+  if (err) *err = c->host_pinned[0];
+  return timer_flush(c);
+}
+
+int sphx_iisph_integrate(sphx_context* c) {
+  SPHX_TRY(need_sorted(c, "sphx_iisph_integrate"));
+  KP k = make_kp(c);
+  if (k.n == 0) return 0;
+  timer_begin(c, "iisph_integrate");
+  k_ii_integrate<<<blocks_for(k.n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
+      k, c->key.p, c->POS(), c->VEL(), c->VELEVAL(), c->ACCEL());
+  timer_end(c);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+int sphx_iisph_step(sphx_context* c, int* iters, double* err) {
+  SPHX_CHECK(c, "sphx_iisph_step", "null context");
+  SPHX_TRY(ensure_device(c));
+  SPHX_TRY(grid_hash(c));
+  SPHX_TRY(grid_sort(c));
+  SPHX_TRY(grid_reorder(c));
+  SPHX_TRY(sphx_iisph_density(c));
+  SPHX_TRY(sphx_iisph_predict_advection(c));
+  SPHX_TRY(sphx_iisph_pressure_solve(c, iters, err));
+  return sphx_iisph_integrate(c);
+}
+
+int sphx_download_iisph(sphx_context* c, float* dii, float* dijpj, float* aii, float* rho_adv,
+                        float* press_l, float* press_l1, float* residue) {
+  SPHX_CHECK(c, "sphx_download_iisph", "null context");
+  SPHX_TRY(ensure_device(c));
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  int n = c->num;
+  if (n == 0) return 0;
+  std::vector<float4> t(n);
+  if (dii || aii) {
+    SPHX_HIP(hipMemcpy(t.data(), c->dii.p, n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+      if (dii) { dii[3 * i] = t[i].x; dii[3 * i + 1] = t[i].y; dii[3 * i + 2] = t[i].z; }
+      if (aii) aii[i] = t[i].w;
+    }
+  }
+  if (dijpj) {
+    SPHX_HIP(hipMemcpy(t.data(), c->dijpj.p, n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) { dijpj[3 * i] = t[i].x; dijpj[3 * i + 1] = t[i].y; dijpj[3 * i + 2] = t[i].z; }
+  }
+  if (rho_adv) SPHX_HIP(hipMemcpy(rho_adv, c->rho_adv.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  if (press_l) SPHX_HIP(hipMemcpy(press_l, c->press_l.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  if (press_l1) SPHX_HIP(hipMemcpy(press_l1, c->press_l1.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  if (residue) SPHX_HIP(hipMemcpy(residue, c->residue.p, n * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int sphx_upload_iisph(sphx_context* c, const float* dii, const float* dijpj, const float* aii,
+                      const float* rho_adv, const float* press_l) {
+  SPHX_CHECK(c, "sphx_upload_iisph", "null context");
+  SPHX_TRY(ensure_device(c));
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  int n = c->num;
+  if (n == 0) return 0;
+  std::vector<float4> t(n);
+  if (dii || aii) {
+    SPHX_HIP(hipMemcpy(t.data(), c->dii.p, n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+      if (dii) { t[i].x = dii[3 * i]; t[i].y = dii[3 * i + 1]; t[i].z = dii[3 * i + 2]; }
+      if (aii) t[i].w = aii[i];
+    }
+    SPHX_HIP(hipMemcpy(c->dii.p, t.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  }
+  if (dijpj) {
+    for (int i = 0; i < n; ++i) t[i] = make_float4(dijpj[3 * i], dijpj[3 * i + 1], dijpj[3 * i + 2], 0.f);
+    SPHX_HIP(hipMemcpy(c->dijpj.p, t.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  }
+  if (rho_adv) SPHX_HIP(hipMemcpy(c->rho_adv.p, rho_adv, n * sizeof(float), hipMemcpyHostToDevice));
+  if (press_l) {
+    SPHX_HIP(hipMemcpy(c->press_l.p, press_l, n * sizeof(float), hipMemcpyHostToDevice));
+    // keep s_j = m_j p_j / rho_j^2 consistent with the uploaded pressures
+    std::vector<float4> ph(n);
+    std::vector<float> s(n);
+    SPHX_HIP(hipMemcpy(ph.data(), c->PHYS(), n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) s[i] = ph[i].x * press_l[i] / ph[i].y / ph[i].y;
+    SPHX_HIP(hipMemcpy(c->sfac.p, s.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+}  // extern "C"

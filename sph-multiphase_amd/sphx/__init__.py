@@ -1,0 +1,295 @@
+"""ctypes binding of libsphx.so (the C ABI declared in include/sphx.h).
+
+Particle data crosses the boundary in the reference's AoS formats
+(SPH_Solver/common_header.h:28-137); here they are numpy structured dtypes
+with the same byte layout.  The library is the HIP implementation; there is
+no CPU fallback: without the built library or without a GPU the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent.parent
+LIB_PATH = _PKG / "libsphx.so"
+
+f3 = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4")])
+f4 = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("w", "<f4")])
+i3 = np.dtype([("x", "<i4"), ("y", "<i4"), ("z", "<i4")])
+
+display_dtype = np.dtype([("pos", "<f4", (3,)), ("color", "<f4", (4,)), ("type", "<i4")])
+calc_dtype = np.dtype([
+    ("vel", "<f4", (3,)), ("veleval", "<f4", (3,)), ("pressure", "<f4"), ("dens", "<f4"),
+    ("restdens", "<f4"), ("mass", "<f4"), ("visc", "<f4"), ("bornid", "<i4"),
+    ("accel", "<f4", (3,)), ("deformGrad", "<f4", (9,)), ("stress", "<f4", (9,)),
+    ("B", "<f4", (9,)), ("solidG", "<f4"), ("solidK", "<f4"), ("X", "<f4", (3,)),
+    ("invA", "<f4", (9,)),
+])
+assert display_dtype.itemsize == 32 and calc_dtype.itemsize == 224
+
+TYPE_FLUID, TYPE_BOUNDARY, TYPE_ELASTIC, TYPE_GRANULAR = 0, 1, 2, 3
+GRID_UNDEF = 0xFFFFFFFF
+
+
+class Float3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+    def tolist(self):
+        return [self.x, self.y, self.z]
+
+
+class Int3(C.Structure):
+    _fields_ = [("x", C.c_int), ("y", C.c_int), ("z", C.c_int)]
+
+
+class ParamCarrier(C.Structure):
+    """common_header.h:28-109 (464 bytes)."""
+    _fields_ = [
+        ("acclimit", C.c_float), ("vlimit", C.c_float), ("gravity", Float3),
+        ("volmin", Float3), ("volmax", Float3), ("softminx", Float3), ("softmaxx", Float3),
+        ("num", C.c_int), ("maxNum", C.c_int), ("dt", C.c_float), ("simscale", C.c_float),
+        ("viscosity", C.c_float), ("restdensity", C.c_float), ("mass", C.c_float),
+        ("radius", C.c_float), ("smoothradius", C.c_float), ("scalep", C.c_float),
+        ("massArr", C.c_float * 3), ("densArr", C.c_float * 3), ("viscArr", C.c_float * 3),
+        ("boundstiff", C.c_float), ("kpoly6", C.c_float), ("kspiky", C.c_float),
+        ("kspikydiff", C.c_float), ("klaplacian", C.c_float), ("kspline", C.c_float),
+        ("bmass", C.c_float), ("bRestdensity", C.c_float), ("bvisc", C.c_float),
+        ("cellsize", C.c_float), ("searchnum", C.c_int), ("neighbornum", C.c_int),
+        ("neighborid", C.c_int * 27), ("gridmin", Float3), ("gridmax", Float3),
+        ("gridsize", Float3), ("gridIdfac", Float3), ("gridres", Int3), ("gridtotal", C.c_int),
+        ("mpmcellsize", C.c_float), ("mpmXmin", Float3), ("mpmXmax", Float3),
+        ("mpmRes", Int3), ("mpmNodeNum", C.c_int), ("solidK", C.c_float),
+        ("solidG", C.c_float), ("a_phi", C.c_float), ("a_psi", C.c_float), ("k_c", C.c_float),
+        ("w_deltax", C.c_float), ("fijn", C.c_float), ("SFlipControl", C.c_float),
+        ("LFlipControl", C.c_float), ("mpmSplitNum", C.c_int), ("collisionStiff", C.c_float),
+        ("surfaceTensionK", C.c_float), ("surfaceTensionFluidC", C.c_float),
+        ("fBlending", C.c_float),
+    ]
+
+
+assert C.sizeof(ParamCarrier) == 464
+
+
+class SolverOptions(C.Structure):
+    _fields_ = [
+        ("omega", C.c_float), ("warm_start", C.c_float), ("full_support", C.c_int),
+        ("boundary_pressure", C.c_int), ("tol", C.c_float), ("min_iters", C.c_int),
+        ("max_iters", C.c_int), ("clamp_to_grid", C.c_int), ("wcsph_kernel", C.c_int),
+        ("eos_B", C.c_float), ("eos_gamma", C.c_float), ("art_visc_alpha", C.c_float),
+        ("sound_speed", C.c_float), ("drift_relax", C.c_float), ("drift_diffusion", C.c_float),
+        ("drift_pressure_scale", C.c_float), ("drift_diffusion_scale", C.c_float),
+        ("boussinesq", C.c_int), ("rho_ref", C.c_float), ("clamp_pressure", C.c_int),
+        ("reserved", C.c_int * 7),
+    ]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("n_fluid", C.c_int), ("n_boundary", C.c_int), ("n_total", C.c_int),
+                ("n_phase", C.c_int), ("spacing", C.c_float)]
+
+
+class SphxError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libsphx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise SphxError(f"{LIB_PATH} not built; run __graft_entry__.build()")
+        L = C.CDLL(str(LIB_PATH))
+        P, VP, I, D = C.POINTER, C.c_void_p, C.c_int, C.c_double
+        L.sphx_last_error.restype = C.c_char_p
+        L.sphx_version.restype = C.c_char_p
+        L.sphx_device_count.restype = I
+        L.sphx_create.argtypes = [P(VP), P(ParamCarrier)]
+        L.sphx_destroy.argtypes = [VP]
+        L.sphx_set_params.argtypes = [VP, P(ParamCarrier)]
+        L.sphx_get_params.argtypes = [VP, P(ParamCarrier)]
+        L.sphx_default_options.argtypes = [P(SolverOptions)]
+        L.sphx_default_options.restype = None
+        L.sphx_set_options.argtypes = [VP, P(SolverOptions)]
+        L.sphx_sync.argtypes = [VP]
+        L.sphx_upload.argtypes = [VP, VP, VP, I, I, I]
+        L.sphx_download.argtypes = [VP, VP, VP, I, I]
+        L.sphx_num_particles.argtypes = [VP]
+        L.sphx_download_grid.argtypes = [VP, VP, VP, VP, VP, VP, VP]
+        for name in ["sphx_grid_hash", "sphx_grid_sort", "sphx_grid_reorder", "sphx_grid_build",
+                     "sphx_boundary_volume", "sphx_wcsph_density_pressure", "sphx_wcsph_force",
+                     "sphx_wcsph_advance", "sphx_wcsph_step", "sphx_iisph_density",
+                     "sphx_iisph_predict_advection", "sphx_iisph_integrate"]:
+            getattr(L, name).argtypes = [VP]
+        L.sphx_iisph_pressure_iteration.argtypes = [VP, P(D), P(D)]
+        L.sphx_iisph_pressure_solve.argtypes = [VP, P(I), P(D)]
+        L.sphx_iisph_step.argtypes = [VP, P(I), P(D)]
+        L.sphx_download_iisph.argtypes = [VP] * 8
+        L.sphx_upload_iisph.argtypes = [VP] * 6
+        L.sphx_timing_enable.argtypes = [VP, I]
+        L.sphx_timing_reset.argtypes = [VP]
+        L.sphx_timing_get.argtypes = [VP, C.c_char_p, P(D), P(I)]
+        L.sphx_setup_grid.argtypes = [P(ParamCarrier)]
+        L.sphx_setup_grid.restype = None
+        L.sphx_setup_kernels.argtypes = [P(ParamCarrier)]
+        L.sphx_setup_kernels.restype = None
+        L.sphx_scene_dambreak.argtypes = [I, I, I, C.c_float, C.c_float, C.c_float, C.c_float,
+                                          C.c_float, I, C.c_float, C.c_uint64, I,
+                                          P(ParamCarrier), VP, VP, P(SceneInfo)]
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().sphx_last_error().decode()
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        raise SphxError(f"{what}: {last_error()} (rc={rc})")
+
+
+def ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def default_options() -> SolverOptions:
+    o = SolverOptions()
+    lib().sphx_default_options(C.byref(o))
+    return o
+
+
+# ------------------------------------------------------------------ scenes --
+def scene_dambreak(nx=32, ny=32, nz=32, s=0.01, L=(0.64, 0.32, 0.32), h=0.02, layers=2,
+                   jitter=1e-4, seed=20261202, kernel_family=1):
+    """Seeded dam-break scene; returns (params, disp, calc, info)."""
+    L_ = lib()
+    info = SceneInfo()
+    p = ParamCarrier()
+    L_.sphx_scene_dambreak(nx, ny, nz, s, L[0], L[1], L[2], h, layers, jitter, seed,
+                           kernel_family, C.byref(p), None, None, C.byref(info))
+    disp = np.zeros(info.n_total, display_dtype)
+    calc = np.zeros(info.n_total, calc_dtype)
+    L_.sphx_scene_dambreak(nx, ny, nz, s, L[0], L[1], L[2], h, layers, jitter, seed,
+                           kernel_family, C.byref(p), ptr(disp), ptr(calc), C.byref(info))
+    return p, disp, calc, info
+
+
+# ----------------------------------------------------------------- context --
+class Context:
+    """Owns one sphx_context (device buffers for params.maxNum particles)."""
+
+    def __init__(self, params: ParamCarrier, options: SolverOptions | None = None):
+        self.L = lib()
+        self.h = C.c_void_p()
+        check(self.L.sphx_create(C.byref(self.h), C.byref(params)), "sphx_create")
+        self.params = params
+        if options is not None:
+            self.set_options(options)
+
+    def close(self):
+        if self.h:
+            self.L.sphx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_options(self, o: SolverOptions):
+        check(self.L.sphx_set_options(self.h, C.byref(o)), "sphx_set_options")
+
+    def set_params(self, p: ParamCarrier):
+        check(self.L.sphx_set_params(self.h, C.byref(p)), "sphx_set_params")
+        self.params = p
+
+    def upload(self, disp, calc, start=0, grow=1):
+        disp = np.ascontiguousarray(disp, display_dtype)
+        calc = np.ascontiguousarray(calc, calc_dtype)
+        check(self.L.sphx_upload(self.h, ptr(disp), ptr(calc), start, len(disp), grow),
+              "sphx_upload")
+
+    @property
+    def num(self) -> int:
+        return self.L.sphx_num_particles(self.h)
+
+    def download(self):
+        n = self.num
+        disp = np.zeros(n, display_dtype)
+        calc = np.zeros(n, calc_dtype)
+        check(self.L.sphx_download(self.h, ptr(disp), ptr(calc), 0, n), "sphx_download")
+        return disp, calc
+
+    def download_grid(self):
+        n, gt = self.num, self.params.gridtotal
+        out = dict(mgcell=np.zeros(n, np.uint32), midsort=np.zeros(n, np.uint32),
+                   mgndx=np.zeros(n, np.uint32), gridoff=np.zeros(gt, np.int32),
+                   gridcnt=np.zeros(gt, np.int32), MFidTable=np.zeros(n, np.int32))
+        check(self.L.sphx_download_grid(self.h, *[ptr(out[k]) for k in
+                                                  ["mgcell", "midsort", "mgndx", "gridoff",
+                                                   "gridcnt", "MFidTable"]]),
+              "sphx_download_grid")
+        return out
+
+    def download_iisph(self):
+        n = self.num
+        out = dict(dii=np.zeros((n, 3), np.float32), dijpj=np.zeros((n, 3), np.float32),
+                   aii=np.zeros(n, np.float32), rho_adv=np.zeros(n, np.float32),
+                   press_l=np.zeros(n, np.float32), press_l1=np.zeros(n, np.float32),
+                   residue=np.zeros(n, np.float32))
+        check(self.L.sphx_download_iisph(self.h, *[ptr(out[k]) for k in
+                                                   ["dii", "dijpj", "aii", "rho_adv", "press_l",
+                                                    "press_l1", "residue"]]),
+              "sphx_download_iisph")
+        return out
+
+    def call(self, name: str):
+        check(getattr(self.L, "sphx_" + name)(self.h), name)
+
+    def sync(self):
+        check(self.L.sphx_sync(self.h), "sphx_sync")
+
+    def pressure_iteration(self):
+        ms, ma = C.c_double(), C.c_double()
+        check(self.L.sphx_iisph_pressure_iteration(self.h, C.byref(ms), C.byref(ma)),
+              "pressure_iteration")
+        return ms.value, ma.value
+
+    def pressure_solve(self):
+        it, err = C.c_int(), C.c_double()
+        check(self.L.sphx_iisph_pressure_solve(self.h, C.byref(it), C.byref(err)),
+              "pressure_solve")
+        return it.value, err.value
+
+    def iisph_step(self):
+        it, err = C.c_int(), C.c_double()
+        check(self.L.sphx_iisph_step(self.h, C.byref(it), C.byref(err)), "iisph_step")
+        return it.value, err.value
+
+    def timing(self, on=True):
+        check(self.L.sphx_timing_enable(self.h, 1 if on else 0), "timing")
+
+    def timing_reset(self):
+        check(self.L.sphx_timing_reset(self.h), "timing_reset")
+
+    def timing_get(self, name: str):
+        t, n = C.c_double(), C.c_int()
+        check(self.L.sphx_timing_get(self.h, name.encode(), C.byref(t), C.byref(n)), "timing_get")
+        return t.value, n.value
+
+
+def device_count() -> int:
+    return lib().sphx_device_count()

@@ -1,0 +1,195 @@
+"""WCSPH and IISPH neighbour loops on the GPU vs the oracle (FP32 faithful
+restatement and FP64 twin) on identical inputs.
+
+Tolerances (normwise, infinity norm, relative to the FP64 oracle):
+density 1e-5, pressures/accelerations 1e-4, or 4x the CPU-FP32
+restatement's own error when that is larger (summation order differs)."""
+import numpy as np
+import pytest
+
+import orc
+import sphx
+from helpers import assert_close, options, shuffled_scene, state_from_gpu
+
+pytestmark = pytest.mark.gpu
+
+DENS_TOL = 1e-5
+FORCE_TOL = 1e-4
+
+
+def fluid_mask(st):
+    return (st["type"] == 0) & (st["mgcell"] != 0xFFFFFFFF)
+
+
+def run_both(ctx, p, opts, kernel):
+    """Copy the GPU state into FP32 and FP64 oracle states, run `kernel`
+    (a name of orc.State) on both and the GPU, return the three states."""
+    s32 = state_from_gpu(ctx, p, np.float32, opts)
+    s64 = state_from_gpu(ctx, p, np.float64, opts)
+    getattr(s32, kernel)()
+    getattr(s64, kernel)()
+    return s32, s64
+
+
+@pytest.mark.parametrize("family", [0, 1])
+def test_wcsph_kernels(gpu, family):
+    # family 0: poly6/spiky with support h (reference); cell = 2h
+    # family 1: cubic spline, support 2h, Tait B = rho0 c0^2 / 7, art. visc.
+    p, disp, calc, info = shuffled_scene(seed=3, kernel_family=family)
+    calc["visc"][disp["type"] == 0] = 1e-3 if family == 0 else 0.0
+    o = options(wcsph_kernel=family)
+    if family == 1:
+        o.eos_B = 1000.0 * 20.0 ** 2 / 7.0
+        o.art_visc_alpha = 0.01
+        o.sound_speed = 20.0
+    rng = np.random.default_rng(1)
+    calc["vel"][disp["type"] == 0] = rng.normal(0, 0.05, (np.count_nonzero(disp["type"] == 0), 3))
+    print(f"wcsph family={family} n={len(disp)}", flush=True)
+    with sphx.Context(p, o) as ctx:
+        ctx.upload(disp, calc)
+        ctx.call("grid_build")
+        # boundary volume
+        s32, s64 = run_both(ctx, p, o, "boundary_volume")
+        ctx.call("boundary_volume")
+        d, c = ctx.download()
+        bmask = s32["type"] == 1
+        assert_close("boundary volume", c["dens"], s32["dens"], s64["dens"], DENS_TOL, bmask)
+        # density + EOS
+        s32, s64 = run_both(ctx, p, o, "density_pressure")
+        ctx.call("wcsph_density_pressure")
+        d, c = ctx.download()
+        m = fluid_mask(s32)
+        assert_close("1/rho", c["dens"], s32["dens"], s64["dens"], DENS_TOL, m)
+        assert_close("pressure", c["pressure"], s32["pressure"], s64["pressure"], FORCE_TOL, m)
+        # force
+        s32, s64 = run_both(ctx, p, o, "force")
+        ctx.call("wcsph_force")
+        d, c = ctx.download()
+        assert_close("accel", c["accel"], s32["accel"], s64["accel"], FORCE_TOL, m)
+        # advance
+        s32, s64 = run_both(ctx, p, o, "advance")
+        ctx.call("wcsph_advance")
+        d, c = ctx.download()
+        assert_close("pos", d["pos"], s32["pos"], s64["pos"], 1e-6, m)
+        assert_close("vel", c["vel"], s32["vel"], s64["vel"], 1e-5, m)
+
+
+def test_wcsph_step_matches_oracle_steps(gpu):
+    """Five full steps (sort + density + force + advance): aggregates agree."""
+    p, disp, calc, info = shuffled_scene(seed=4, kernel_family=1)
+    o = options(wcsph_kernel=1, eos_B=1000.0 * 400.0 / 7.0, art_visc_alpha=0.01, sound_speed=20.0)
+    st = orc.State(disp, calc, p, np.float64)
+    st.opts = o
+    s32 = orc.State(disp, calc, p, np.float32)
+    with sphx.Context(p, o) as ctx:
+        ctx.upload(disp, calc)
+        ctx.call("grid_build")
+        ctx.call("boundary_volume")
+        s32.grid_build()
+        s32.opts = o
+        s32.boundary_volume()
+        for _ in range(5):
+            ctx.call("wcsph_step")
+            s32.grid_build()
+            s32.density_pressure()
+            s32.force()
+            s32.advance()
+        d, c = ctx.download()
+    m = (d["type"] == 0)
+    com_gpu = d["pos"][m].mean(0)
+    com_cpu = s32["pos"][s32["type"] == 0].mean(0)
+    assert np.allclose(com_gpu, com_cpu, atol=1e-6)
+    ke_gpu = (c["vel"][m] ** 2).sum()
+    ke_cpu = (s32["vel"][s32["type"] == 0].astype(np.float64) ** 2).sum()
+    assert abs(ke_gpu - ke_cpu) <= 1e-3 * ke_cpu + 1e-12
+
+
+def iisph_scene(seed=5, full_support=0, boundary_pressure=0, omega=0.1, clamp=0):
+    p, disp, calc, info = shuffled_scene(seed=seed, kernel_family=1)
+    p.dt = 1e-3
+    o = options(wcsph_kernel=1, full_support=full_support, boundary_pressure=boundary_pressure,
+                omega=omega, clamp_pressure=clamp)
+    rng = np.random.default_rng(seed)
+    fl = disp["type"] == 0
+    calc["vel"][fl] = rng.normal(0, 0.05, (np.count_nonzero(fl), 3))
+    calc["pressure"][fl] = rng.uniform(0, 200.0, np.count_nonzero(fl))
+    return p, disp, calc, o
+
+
+@pytest.mark.parametrize("variant", [(0, 0, 0.1, 0), (1, 1, 0.5, 1)])
+def test_iisph_kernels(gpu, variant):
+    full, bp, omega, clamp = variant
+    p, disp, calc, o = iisph_scene(full_support=full, boundary_pressure=bp, omega=omega,
+                                   clamp=clamp)
+    print(f"iisph variant full_support={full} boundary_pressure={bp} omega={omega}", flush=True)
+    with sphx.Context(p, o) as ctx:
+        ctx.upload(disp, calc)
+        ctx.call("grid_build")
+        ctx.call("boundary_volume")
+        s32, s64 = run_both(ctx, p, o, "ii_density")
+        ctx.call("iisph_density")
+        d, c = ctx.download()
+        m = fluid_mask(s32)
+        assert_close("rho", c["dens"], s32["dens"], s64["dens"], DENS_TOL, m)
+        # d_ii and v_adv
+        s32, s64 = run_both(ctx, p, o, "ii_dii")
+        ctx.L.sphx_iisph_predict_advection(ctx.h)
+        ctx.sync()
+        b = ctx.download_iisph()
+        d, c = ctx.download()
+        assert_close("veleval", c["veleval"], s32["veleval"], s64["veleval"], 1e-6, m)
+        assert_close("dii", b["dii"], s32["dii"], s64["dii"], FORCE_TOL, m)
+        # a_ii, rho_adv, warm start: oracle runs dii+aii from the same inputs
+        s32.ii_aii()
+        s64.ii_aii()
+        assert_close("rho_adv", b["rho_adv"], s32["rho_adv"], s64["rho_adv"], DENS_TOL, m)
+        assert_close("aii", b["aii"], s32["aii"], s64["aii"], FORCE_TOL, m)
+        assert_close("press_l0", b["press_l"], s32["press_l"], s64["press_l"], 1e-6, m)
+        # three pressure iterations, each from identical inputs
+        for it in range(3):
+            s32 = state_from_gpu(ctx, p, np.float32, o)
+            s64 = state_from_gpu(ctx, p, np.float64, o)
+            ms32, ma32 = s32.ii_iteration()
+            ms64, ma64 = s64.ii_iteration()
+            ms, ma = ctx.pressure_iteration()
+            b = ctx.download_iisph()
+            d, c = ctx.download()
+            assert_close(f"dijpj[{it}]", b["dijpj"], s32["dijpj"], s64["dijpj"], FORCE_TOL, m)
+            assert_close(f"p[{it}]", b["press_l"], s32["press_l"], s64["press_l"], FORCE_TOL, m)
+            assert_close(f"residue[{it}]", b["residue"], s32["residue"], s64["residue"],
+                         FORCE_TOL, m)
+            assert_close(f"accel[{it}]", c["accel"], s32["accel"], s64["accel"], FORCE_TOL, m)
+            assert abs(ma - ma64) <= 1e-4 * max(ma64, 1e-12) + 1e-9
+            assert abs(ms - ms64) <= 1e-3 * max(abs(ms64), 1e-6)
+        # integrate
+        s32 = state_from_gpu(ctx, p, np.float32, o)
+        s64 = state_from_gpu(ctx, p, np.float64, o)
+        s32.ii_integrate()
+        s64.ii_integrate()
+        ctx.call("iisph_integrate")
+        d, c = ctx.download()
+        assert_close("pos", d["pos"], s32["pos"], s64["pos"], 1e-6, m)
+        assert_close("vel", c["vel"], s32["vel"], s64["vel"], 1e-5, m)
+
+
+def test_iisph_solve_converges_like_oracle(gpu):
+    p, disp, calc, o = iisph_scene(seed=9, full_support=1, boundary_pressure=1, omega=0.5,
+                                   clamp=1)
+    o.tol = 1e-3
+    o.max_iters = 200
+    with sphx.Context(p, o) as ctx:
+        ctx.upload(disp, calc)
+        ctx.call("grid_build")
+        ctx.call("boundary_volume")
+        ctx.call("iisph_density")
+        ctx.call("iisph_predict_advection")
+        s64 = state_from_gpu(ctx, p, np.float64, o)
+        it_ref, err_ref = s64.ii_solve()
+        it, err = ctx.pressure_solve()
+        b = ctx.download_iisph()
+    print(f"solve: gpu {it} iters err {err:.3e}; fp64 oracle {it_ref} iters err {err_ref:.3e}")
+    assert err <= o.tol
+    assert abs(it - it_ref) <= 1
+    m = fluid_mask(s64)
+    if it == it_ref:
+        assert_close("p solved", b["press_l"], s64["press_l"], s64["press_l"], 1e-4, m)
