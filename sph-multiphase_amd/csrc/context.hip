@@ -156,9 +156,27 @@ static int materialize_cold(sphx_context* c) {
   return 0;
 }
 
+// The neighbour loops read the rows [c-1, c+2) of the dense cell table; that
+// is in range only for the grid SetupGrid builds (fluid_system.cpp:136-143).
+static int validate_grid(const sphx_ParamCarrier& P) {
+  const sphx_int3 r = P.gridres;
+  SPHX_CHECK(r.x >= 3 && r.y >= 3 && r.z >= 3, "grid", "gridres must be >= 3 on every axis");
+  SPHX_CHECK((long long)r.x * r.y * r.z == (long long)P.gridtotal && P.gridtotal < (1 << 30),
+             "grid", "gridtotal must equal gridres.x*gridres.y*gridres.z (< 2^30)");
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      for (int k = 0; k < 3; k++)
+        SPHX_CHECK(P.neighborid[i * 9 + j * 3 + k] ==
+                       (i - 1) * r.x * r.z + (j - 1) * r.x + (k - 1),
+                   "grid", "neighborid does not match gridres (use sphx_setup_grid)");
+  SPHX_CHECK(P.simscale > 0.f && P.smoothradius > 0.f, "grid",
+             "simscale and smoothradius must be positive");
+  return 0;
+}
+
 static int alloc_grid(sphx_context* c) {
   int gt = c->P.gridtotal;
-  SPHX_CHECK(gt > 0, "sphx_set_params", "gridtotal must be positive");
+  SPHX_TRY(validate_grid(c->P));
   SPHX_TRY(c->cell_start.alloc((size_t)gt + 1));
   SPHX_HIP(hipMemsetAsync(c->cell_start.p, 0, ((size_t)gt + 1) * sizeof(int), c->stream));
   int bits = 1;
@@ -306,6 +324,7 @@ int sphx_set_params(sphx_context* c, const sphx_ParamCarrier* p) {
   SPHX_CHECK(c && p, "sphx_set_params", "null argument");
   SPHX_TRY(ensure_device(c));
   SPHX_CHECK(p->num >= 0 && p->num <= c->cap, "sphx_set_params", "num outside [0, capacity]");
+  SPHX_TRY(validate_grid(*p));
   bool regrid = p->gridtotal != c->P.gridtotal;
   c->P = *p;
   c->P.maxNum = c->cap;
