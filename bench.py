@@ -1,0 +1,248 @@
+#!/usr/bin/env python3
+"""Headline benchmark: particle-steps/s of the two-phase bubble tank.
+
+Workload: 1,048,576 fluid particles (water 1000 / air 1.2, 512 seeded
+bubbles, radii U[0.01, 0.04] m) filling a 1 m tank to 0.8 m, h = 0.01 m,
+IISPH pressure solve to a mean compression error of 0.1 % (FP64 reduction),
+dt = 1e-3 s.  A step is the whole hot path: grid build, neighbour list,
+density, drift velocity, fraction advection and correction, advected
+velocity with mixture forces, a_ii, the pressure iterations and the
+integration.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (rank 0, N = 1).  `value` counts fluid particles only.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "sph-multiphase_amd"))
+
+RHO = [1000.0, 1.2]
+NU = [1e-6, 1.5e-5]
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per fluid particle per launch (fields read or written
+# once, natural sizes: float3 = 12 B).  See DESIGN.md, "Measurement".
+ALGO_BYTES = {
+    "iisph_dp": 48,        # x 12, p 4, m 4, rho 4, d_ii 12 -> dijpj 12
+    "iisph_iter": 84,      # x, m, rho, rho0, d_ii, a_ii, dijpj, rho_adv, p -> p, res, a_p
+    "iisph_density": 36,   # x 12, m 4, rho0 4, type 4 (+ boundary V 4) -> rho 4
+    "iisph_dii": 60,       # x, v, rho, m, rho0 -> v_adv 12, d_ii 12
+    "iisph_aii": 60,       # x, v_adv, d_ii, m, rho, p -> rho_adv, a_ii, p0
+    "mf_drift": 64,        # x, p, m, rho, alpha(2), nu, g-a -> u_m (2x12)
+    "mf_alpha": 72,        # x, v, m, rho, alpha, u_m -> alpha_new
+    "mf_force": 76,        # x, m, rho, alpha, alpha_new, u_m, v_adv -> v_adv
+    "radix_sort": 32,      # 2 passes x (key+val read, key+val write)
+    "reorder": 104,        # 5 x 16 B + 2 x 4 B gathered and written
+    "nbr_build": 24,       # x read, count + list head written
+}
+
+
+def options(sphx):
+    o = sphx.default_options()
+    o.wcsph_kernel = 1          # cubic spline family (boundary volumes)
+    o.full_support = 1          # pressure loops over the whole 2h support
+    o.boundary_pressure = 1     # boundary terms of kern.cu:763,833,1000
+    o.omega = 0.5
+    o.clamp_pressure = 1
+    o.tol = 1e-3
+    o.min_iters = 2
+    o.max_iters = 300
+    o.clamp_to_grid = 1
+    o.warm_start = 0.5          # reference warm start p0 = 0.5 p(t-1)
+    o.drift_relax = 2e-11       # tau = relax / nu: 2e-5 s in water
+    o.drift_diffusion = 1e-4
+    o.boussinesq = 1
+    o.rho_ref = 1000.0
+    return o
+
+
+def build_scene(sphx, n_fluid, L=(1.0, 1.0, 1.0), seed=20261203):
+    p, disp, calc, alpha, info = sphx.scene_bubbles(n_fluid=n_fluid, L=L, fill=0.8, h=0.01,
+                                                    n_bubbles=max(1, int(512 * n_fluid / (1 << 20))),
+                                                    rmin=0.01, rmax=0.04, layers=2, seed=seed)
+    p.dt = 1e-3
+    return p, disp, calc, alpha, info
+
+
+def cpu_baseline(sample_fluid=131072, mean_iters=10.0, timed_iters=20):
+    """The oracle (FP32 restatement of the reference step, OpenMP) on a
+    bounded sample of the workload: a tank of the same spacing, h and bubble
+    density holding `sample_fluid` particles.  The fixed stages of one step
+    and `timed_iters` pressure iterations are timed apart, and the rate is
+    reported at the GPU run's mean iteration count:
+    n / (t_fixed + mean_iters * t_iteration)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import sphx
+    import orc
+    side = (sample_fluid / (1 << 20)) ** (1.0 / 3.0)
+    p, disp, calc, alpha, info = build_scene(sphx, sample_fluid, L=(side, side, side))
+    o = options(sphx)
+    st = orc.State(disp, calc, p, np.float32)
+    st.opts = o
+    mf = orc.MF(st, RHO, NU, o)
+    rho = np.array([np.float32(r) for r in RHO], np.float64)
+    st.grid_build()
+    src = np.where(st["mgcell"] != 0xFFFFFFFF, st["midsort"], np.arange(st.n))
+    alpha = alpha[src]
+    mf.set_alpha(alpha, alpha @ rho)
+    mf.a["mforce"][:] = [0.0, -9.81, 0.0]
+    fl = st["type"] == 0
+    st.a["visc"][fl] = (alpha[fl] @ np.array(NU, np.float32))
+    st.boundary_volume()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    before = mf.a["alpha"].copy(), mf.a["mforce"].copy()
+    st.grid_build()    # carries the fractions with the permutation
+    src = np.where(st["mgcell"] != 0xFFFFFFFF, st["midsort"], np.arange(st.n))
+    mf.a["alpha"][:] = before[0][src]
+    mf.a["mforce"][:] = before[1][src]
+    st.ii_density()
+    mf.drift()
+    mf.alpha_advance()
+    mf.correction()
+    st.ii_dii()
+    mf.force()
+    st.ii_aii()
+    t1 = time.perf_counter()
+    for _ in range(timed_iters):
+        st.ii_iteration()
+    t2 = time.perf_counter()
+    st.ii_integrate()
+    t3 = time.perf_counter()
+    t_fixed = (t1 - t0) + (t3 - t2)
+    t_iter = (t2 - t1) / timed_iters
+    step = t_fixed + mean_iters * t_iter
+    return {"value": info.n_fluid / step, "unit": "particle-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"one two-phase step of a {info.n_fluid}-particle tank (same spacing, h and "
+                      f"bubble density; {info.n_boundary} boundary particles), FP32 oracle on "
+                      f"{threads} threads: fixed stages {t_fixed:.2f} s + {timed_iters} pressure "
+                      f"iterations at {t_iter:.3f} s each, rate taken at {mean_iters:.1f} "
+                      f"iterations/step (the GPU run's mean)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1 << 20, help="fluid particles")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=131072)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
+              "decomposition is described in DESIGN.md and not built", file=sys.stderr)
+        sys.exit(2)
+    import sphx
+    if sphx.device_count() < 1:
+        print("no HIP device visible: bench.py needs an MI355X", file=sys.stderr)
+        sys.exit(1)
+
+    p, disp, calc, alpha, info = build_scene(sphx, args.n)
+    o = options(sphx)
+    print(f"# scene: {info.n_fluid} fluid + {info.n_boundary} boundary particles, "
+          f"grid {p.gridres.x}x{p.gridres.y}x{p.gridres.z}", file=sys.stderr, flush=True)
+    ctx = sphx.Context(p, o)
+    ctx.upload(disp, calc)
+    ctx.mf_setup(RHO, NU)
+    ctx.mf_upload(alpha)
+    ctx.call("grid_build")
+    ctx.call("boundary_volume")
+    ctx.sync()
+    for s in range(args.warmup):
+        it, err = ctx.mf_step()
+        if s % 10 == 0:
+            print(f"# warmup step {s}: {it} iterations, error {err:.2e}", file=sys.stderr,
+                  flush=True)
+    ctx.sync()
+    ctx.timing(True)
+    ctx.timing_reset()
+    iters = []
+    errs = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        it, err = ctx.mf_step()
+        iters.append(it)
+        errs.append(err)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    names = ["hash", "radix_sort", "cell_start", "reorder", "nbr_build", "iisph_density",
+             "mf_drift", "mf_alpha", "mf_correction", "iisph_dii", "mf_force", "iisph_aii",
+             "iisph_dp", "iisph_iter", "iisph_integrate"]
+    kt = {k: ctx.timing_get(k) for k in names}
+    ctx.close()
+    total_k = sum(t for t, n in kt.values())
+    dom = max(kt, key=lambda k: kt[k][0])
+    t_dom, n_dom = kt[dom]
+    avg_ms = t_dom / max(n_dom, 1)
+    nf = info.n_fluid
+    algo = ALGO_BYTES.get(dom, 0) * nf
+    achieved = algo / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    for k in names:
+        t, n = kt[k]
+        if n:
+            print(f"# {k:16s} {t / n * 1e3:9.1f} us/launch x {n:5d}  ({100 * t / total_k:5.1f}% of kernel time)",
+                  file=sys.stderr)
+    print(f"# mean pressure iterations {np.mean(iters):.2f} (max {max(iters)}), "
+          f"max error {max(errs):.2e}", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(args.cpu_sample, float(np.mean(iters)))
+        except Exception as e:   # the baseline is a report, not the measurement
+            print(f"# cpu baseline failed: {e}", file=sys.stderr)
+    value = nf * args.steps / elapsed
+    line = {
+        "metric": "particle-steps/s at 1M two-phase particles; pressure iters learned vs plain @0.1% err",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "two-phase water/air bubble tank, drift-flux mixture + IISPH to 0.1% error",
+            "n_fluid": nf,
+            "n_boundary": info.n_boundary,
+            "bubbles": 512,
+            "h": 0.01,
+            "dt": 1e-3,
+            "pressure_iters_plain_mean": float(np.mean(iters)),
+            "pressure_iters_learned_mean": None,
+            "warm_start": "0.5 p(t-1) (reference)",
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": algo,
+            "avg_launch_ms": avg_ms,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

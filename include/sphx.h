@@ -296,6 +296,10 @@ int sphx_iisph_integrate(sphx_context* ctx);        /* :1052 */
 /* sort + density + predict + solve + integrate (fluid_system.cpp:567-588
  * with Integration enabled). */
 int sphx_iisph_step(sphx_context* ctx, int* iters, double* err);
+/* The two kernels of sphx_iisph_predict_advection, callable apart so a
+ * caller can add accelerations to the advected velocity in between. */
+int sphx_iisph_predict_dii(sphx_context* ctx);      /* ComputeDii :768 */
+int sphx_iisph_predict_aii(sphx_context* ctx);      /* ComputeAii :883 */
 /* IISPH per-slot buffers (any pointer may be NULL); vectors are 3*num. */
 int sphx_download_iisph(sphx_context* ctx, float* dii, float* dijpj,
                         float* aii, float* rho_adv, float* press_l,
@@ -303,6 +307,39 @@ int sphx_download_iisph(sphx_context* ctx, float* dii, float* dijpj,
 int sphx_upload_iisph(sphx_context* ctx, const float* dii, const float* dijpj,
                       const float* aii, const float* rho_adv,
                       const float* press_l);
+
+/* ------------------------------------------------------------------------ */
+/* Multiphase drift-flux mixture (formula spec: SPH_Solver/fluids/           */
+/* multiphase.cu, not compiled in the reference).                            */
+/* ------------------------------------------------------------------------ */
+
+/* Enable nphase (1..3) phases with rest densities rho[k] and kinematic
+ * viscosities nu[k] (ParamCarrier densArr/viscArr semantics). */
+int sphx_mf_setup(sphx_context* ctx, int nphase, const float* rho, const float* nu);
+/* Volume fractions [count][nphase] for slots [start, start+count); sets the
+ * mixture density/viscosity and g - a = g. */
+int sphx_mf_upload(sphx_context* ctx, const float* alpha, int start, int count);
+/* alpha [num][nphase], rho_m [num], drift [num][nphase][3], mforce [num][3];
+ * any may be NULL. */
+int sphx_mf_download(sphx_context* ctx, float* alpha, float* rho_m, float* drift,
+                     float* mforce);
+int sphx_mf_drift_velocity(sphx_context* ctx);   /* mfComputeDriftVel :78 */
+int sphx_mf_alpha_advance(sphx_context* ctx);    /* mfComputeAlphaAdvance :218 */
+int sphx_mf_correction(sphx_context* ctx);       /* mfComputeCorrection :275 */
+/* drift stress (contributeForce_projectu :555-562) + Boussinesq buoyancy
+ * added to the advected velocity (after sphx_iisph_predict_dii). */
+int sphx_mf_force(sphx_context* ctx);
+/* Full two-phase step (sort, density, drift, alpha, correction, predict,
+ * mixture force, a_ii, pressure solve, integrate). */
+int sphx_mf_step(sphx_context* ctx, int* iters, double* err);
+
+/* Legacy multiphase wrappers: declared by the reference
+ * (fluid_system_host.cuh:62-64,77) with bodies commented out
+ * (fluid_system_host.cu:353-419); here they run the stages above. */
+void MfComputeDriftVelCUDA(void);
+void MfComputeAlphaAdvanceCUDA(void);
+void MfComputeCorrectionCUDA(void);
+void ComputeForceCUDA_ProjectU(void);
 
 /* ------------------------------------------------------------------------ */
 /* Kernel timing (HIP events on the context stream).                         */
@@ -344,6 +381,17 @@ int sphx_scene_dambreak(int nx, int ny, int nz, float s, float Lx, float Ly,
                         uint64_t seed, int kernel_family,
                         sphx_ParamCarrier* params, sphx_displayPack* disp,
                         sphx_calculationPack* calc, sphx_SceneInfo* info);
+
+/* Two-phase bubble tank (water 1000 / air 1.2): n_fluid particles on a
+ * lattice of spacing s = (fill*Lx*Lz/n_fluid)^(1/3) filling a Lx*Ly*Lz tank
+ * from the floor (y-major order); n_bubbles seeded spheres, radii uniform in
+ * [rmin, rmax], mark air.  Every fluid particle carries the solver mass
+ * 1000*s^3 and rest density 1000; `alpha` receives [n_total][2] volume
+ * fractions (water, air).  Boundary: `layers` wall layers, open top. */
+int sphx_scene_bubbles(int n_fluid, float Lx, float Ly, float Lz, float fill, float h,
+                       int n_bubbles, float rmin, float rmax, int layers, uint64_t seed,
+                       sphx_ParamCarrier* params, sphx_displayPack* disp,
+                       sphx_calculationPack* calc, float* alpha, sphx_SceneInfo* info);
 
 #ifdef __cplusplus
 }

@@ -155,6 +155,54 @@ class State:
         return it, err.value
 
 
+class OrcMF(C.Structure):
+    _fields_ = [("nphase", C.c_int), ("rho", C.c_float * 3), ("nu", C.c_float * 3),
+                ("relax", C.c_float), ("sigma", C.c_float), ("cpress", C.c_float),
+                ("cdiff", C.c_float), ("boussinesq", C.c_int), ("rho_ref", C.c_float),
+                ("alpha", C.c_void_p), ("alpha_new", C.c_void_p), ("drift", C.c_void_p),
+                ("mforce", C.c_void_p), ("amf", C.c_void_p)]
+
+
+class MF:
+    """Multiphase arrays attached to an oracle State (same dtype)."""
+
+    def __init__(self, st: State, rho, nu, opts: sphx.SolverOptions):
+        n, dt = st.n, st.dtype
+        self.st = st
+        self.nphase = len(rho)
+        self.a = dict(alpha=np.zeros((n, 4), dt), alpha_new=np.zeros((n, 4), dt),
+                      drift=np.zeros((n, 3, 3), dt), mforce=np.zeros((n, 3), dt),
+                      amf=np.zeros((n, 3), dt))
+        self.m = OrcMF()
+        self.m.nphase = len(rho)
+        for k in range(len(rho)):
+            self.m.rho[k] = rho[k]
+            self.m.nu[k] = nu[k]
+        self.m.relax, self.m.sigma = opts.drift_relax, opts.drift_diffusion
+        self.m.cpress, self.m.cdiff = opts.drift_pressure_scale, opts.drift_diffusion_scale
+        self.m.boussinesq, self.m.rho_ref = opts.boussinesq, opts.rho_ref
+        for k in ["alpha", "alpha_new", "drift", "mforce", "amf"]:
+            setattr(self.m, k, self.a[k].ctypes.data)
+
+    def __getitem__(self, k):
+        return self.a[k]
+
+    def set_alpha(self, alpha, rho_m):
+        k = alpha.shape[1]
+        self.a["alpha"][:] = 0
+        self.a["alpha"][:, :k] = alpha
+        self.a["alpha"][:, 3] = rho_m
+
+    def _call(self, name):
+        getattr(lib(), self.st.pfx + name)(C.byref(self.st.params), C.byref(self.st.s),
+                                           C.byref(self.m))
+
+    def drift(self): self._call("mf_drift")
+    def alpha_advance(self): self._call("mf_alpha")
+    def correction(self): self._call("mf_correction")
+    def force(self): self._call("mf_force")
+
+
 def relerr(a, b, ref=None):
     """max |a-b| / max |ref| (normwise relative, infinity norm)."""
     a = np.asarray(a, np.float64)

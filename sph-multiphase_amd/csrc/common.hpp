@@ -162,6 +162,19 @@ struct sphx_context {
   sphx::DevBuf<int> solve_state;  // [0]=done [1]=iters
   sphx::DevBuf<double> solve_err; // [0]=last mean |rho-rho0|/rho0 [1]=mean signed
   double* host_pinned = nullptr;  // 4 doubles: done, iters, err, signed
+  int last_iters = 0;             // iterations of the previous solve
+  // per-step neighbour list (ELL: idx[q * stride + i]); rebuilt after any
+  // change of positions or order
+  sphx::DevBuf<int> nbr_idx, nbr_cnt;
+  int nbr_cap = 0, nbr_stride = 0;
+  bool nbr_valid = false;
+
+  // multiphase (drift-flux mixture); nphase == 0 when disabled
+  int nphase = 0;
+  float mf_rho[3] = {0, 0, 0}, mf_nu[3] = {0, 0, 0};
+  sphx::DevBuf<float4> mf_alpha[2], mf_force[2];  // permuted with particles
+  sphx::DevBuf<float4> mf_alpha_new, mf_drift[3], mf_amf;
+  bool mf_pending = false;        // alpha_new holds the corrected fractions
 
   // timing
   bool timing = false;

@@ -364,6 +364,7 @@ int sphx_upload(sphx_context* c, const sphx_displayPack* disp, const sphx_calcul
   SPHX_TRY(ensure_device(c));
   if (count == 0) return 0;
   SPHX_TRY(materialize_cold(c));
+  c->nbr_valid = false;
   std::vector<float4> pos(count), vel(count), ve(count), acc(count), phys(count), color(count), X(count);
   std::vector<int> born(count);
   std::vector<float> mat((size_t)count * 36);

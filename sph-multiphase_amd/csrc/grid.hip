@@ -92,28 +92,96 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_upsweep(const uint32_t* __res
   for (int t = threadIdx.x; t < bins; t += RS_THREADS) hist[(size_t)t * nblocks + blockIdx.x] = h[t];
 }
 
-// Exclusive scan of `count` entries by one workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ a, int count) {
-  __shared__ uint32_t part[1024];
-  const int t = threadIdx.x;
-  const int per = (count + 1023) / 1024;
-  const int b = t * per;
-  const int e = min(count, b + per);
-  uint32_t s = 0;
-  for (int i = b; i < e; ++i) s += a[i];
-  part[t] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    uint32_t v = (t >= off) ? part[t - off] : 0u;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+// Device-wide exclusive scan of uint32 in three coalesced launches:
+// per-tile sums, a scan of the tile sums, per-tile scan plus tile offset.
+constexpr int SC_THREADS = 256;
+constexpr int SC_ITEMS = 8;
+constexpr int SC_TILE = SC_THREADS * SC_ITEMS;   // 2048 entries per tile
+
+// inclusive wave64 scan
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
   }
-  uint32_t run = part[t] - s;   // exclusive prefix of this chunk
-  for (int i = b; i < e; ++i) {
-    uint32_t v = a[i];
-    a[i] = run;
-    run += v;
+  return v;
+}
+
+// exclusive block scan of one value per thread; returns the block total too
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t ws[SC_THREADS / 64];
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t inc = wave_incl_scan(v);
+  if (lane == 63) ws[w] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+#pragma unroll
+  for (int q = 0; q < SC_THREADS / 64; ++q) {
+    uint32_t s = ws[q];
+    if (q < w) off += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
+__global__ __launch_bounds__(SC_THREADS) void k_scan_reduce(const uint32_t* __restrict__ a,
+                                                             int count,
+                                                             uint32_t* __restrict__ tile_sum) {
+  int base = blockIdx.x * SC_TILE + threadIdx.x * SC_ITEMS;
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < SC_ITEMS; ++k) {
+    int i = base + k;
+    if (i < count) s += a[i];
+  }
+  uint32_t tot;
+  (void)block_excl_scan(s, &tot);
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
+}
+
+// exclusive scan of the tile sums (count <= SC_THREADS * SC_ITEMS * 64)
+__global__ __launch_bounds__(SC_THREADS) void k_scan_top(uint32_t* __restrict__ t, int count) {
+  uint32_t carry = 0;
+  for (int base = 0; base < count; base += SC_TILE) {
+    uint32_t v[SC_ITEMS], s = 0;
+    int b = base + threadIdx.x * SC_ITEMS;
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; ++k) {
+      v[k] = (b + k < count) ? t[b + k] : 0u;
+      s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = carry + block_excl_scan(s, &tot);
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; ++k) {
+      if (b + k < count) t[b + k] = run;
+      run += v[k];
+    }
+    carry += tot;
+  }
+}
+
+__global__ __launch_bounds__(SC_THREADS) void k_scan_down(uint32_t* __restrict__ a, int count,
+                                                           const uint32_t* __restrict__ tile_off) {
+  int base = blockIdx.x * SC_TILE + threadIdx.x * SC_ITEMS;
+  uint32_t v[SC_ITEMS], s = 0;
+#pragma unroll
+  for (int k = 0; k < SC_ITEMS; ++k) {
+    int i = base + k;
+    v[k] = (i < count) ? a[i] : 0u;
+    s += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = tile_off[blockIdx.x] + block_excl_scan(s, &tot);
+#pragma unroll
+  for (int k = 0; k < SC_ITEMS; ++k) {
+    int i = base + k;
+    if (i < count) a[i] = run;
+    run += v[k];
   }
 }
 
@@ -172,14 +240,19 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_downsweep(
   }
 }
 
-// cell_start[c] = lower_bound(sorted keys, c) for c in [0, gridtotal].
+// cell_start[c] = lower_bound(sorted keys, c) for c in [0, gridtotal]: one
+// thread per cell, binary search over the sorted keys (no serial gaps).
 __global__ void k_cell_start(const uint32_t* __restrict__ skey, int n, int gridtotal,
                              int* __restrict__ cell_start) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > n) return;
-  int prev = (i == 0) ? -1 : (int)skey[i - 1];
-  int cur = (i == n) ? gridtotal : (int)skey[i];
-  for (int c = prev + 1; c <= cur; ++c) cell_start[c] = i;
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > gridtotal) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if ((int)skey[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  cell_start[c] = lo;
 }
 
 // RearrangeData (kern.cu:98-121): slot s takes the particle idsort[s]; an
@@ -219,9 +292,12 @@ __global__ void k_fill_i32(int* p, int n, int v) {
   if (i < n) p[i] = v;
 }
 
+int mf_reorder(sphx_context* c, int a, int b);
+
 int grid_hash(sphx_context* c) {
   int n = c->num;
   c->sorted_valid = false;
+  c->nbr_valid = false;
   if (n == 0) return 0;
   KP k = make_kp(c);
   timer_begin(c, "hash");
@@ -244,15 +320,23 @@ int grid_sort(sphx_context* c) {
   int nblocks = blocks_for(n, RS_TILE);
   int passes = (c->key_bits + RS_MAX_BITS - 1) / RS_MAX_BITS;
   int per = (c->key_bits + passes - 1) / passes;
-  SPHX_TRY(c->block_hist.alloc((size_t)nblocks * RS_MAX_BINS));
+  size_t hist_n = (size_t)nblocks * RS_MAX_BINS;
+  int scan_tiles = blocks_for((int)hist_n, SC_TILE);
+  SPHX_CHECK(scan_tiles <= SC_TILE * 64, "sphx_grid_sort", "too many particles for one scan");
+  SPHX_TRY(c->block_hist.alloc(hist_n + scan_tiles + 1));
+  uint32_t* tile_sum = c->block_hist.p + hist_n;
   uint32_t *ka = c->key.p, *va = c->idsort.p, *kb = c->key_tmp.p, *vb = c->val_tmp.p;
   int shift = 0;
   timer_begin(c, "radix_sort");
   for (int ps = 0; ps < passes; ++ps) {
     int bits = std::min(per, c->key_bits - shift);
     int bins = 1 << bits;
+    int cnt = bins * nblocks;
+    int tiles = blocks_for(cnt, SC_TILE);
     k_rs_upsweep<<<nblocks, RS_THREADS, 0, c->stream>>>(ka, n, shift, bits, c->block_hist.p, nblocks);
-    k_rs_scan<<<1, 1024, 0, c->stream>>>(c->block_hist.p, bins * nblocks);
+    k_scan_reduce<<<tiles, SC_THREADS, 0, c->stream>>>(c->block_hist.p, cnt, tile_sum);
+    k_scan_top<<<1, SC_THREADS, 0, c->stream>>>(tile_sum, tiles);
+    k_scan_down<<<tiles, SC_THREADS, 0, c->stream>>>(c->block_hist.p, cnt, tile_sum);
     k_rs_downsweep<<<nblocks, RS_THREADS, 0, c->stream>>>(ka, va, kb, vb, n, shift, bits,
                                                           c->block_hist.p, nblocks);
     std::swap(ka, kb);
@@ -266,8 +350,8 @@ int grid_sort(sphx_context* c) {
     SPHX_HIP(hipMemcpyAsync(c->idsort.p, va, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
   }
   timer_begin(c, "cell_start");
-  k_cell_start<<<blocks_for(n + 1, 256), 256, 0, c->stream>>>(c->key.p, n, c->P.gridtotal,
-                                                               c->cell_start.p);
+  k_cell_start<<<blocks_for(c->P.gridtotal + 1, 256), 256, 0, c->stream>>>(
+      c->key.p, n, c->P.gridtotal, c->cell_start.p);
   timer_end(c);
   SPHX_HIP(hipGetLastError());
   c->sorted_valid = true;
@@ -279,6 +363,7 @@ int grid_reorder(sphx_context* c) {
   if (n == 0) return 0;
   SPHX_CHECK(c->sorted_valid, "sphx_grid_reorder", "grid not sorted (call hash and sort first)");
   int a = c->cur, b = 1 - c->cur;
+  c->nbr_valid = false;
   k_fill_i32<<<blocks_for(c->cap, 256), 256, 0, c->stream>>>(c->mfid.p, c->cap, -1);
   timer_begin(c, "reorder");
   k_reorder<<<blocks_for(n, 256), 256, 0, c->stream>>>(
@@ -287,6 +372,7 @@ int grid_reorder(sphx_context* c) {
       c->phys[b].p, c->bornid[a].p, c->bornid[b].p, c->perm[a].p, c->perm[b].p, c->mfid.p);
   timer_end(c);
   SPHX_HIP(hipGetLastError());
+  SPHX_TRY(mf_reorder(c, a, b));
   c->cur = b;
   return 0;
 }

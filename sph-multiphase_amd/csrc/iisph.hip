@@ -19,6 +19,8 @@
 // reference's press_l1 -> press_l copy (host.cu:536) is not needed.
 #include "nbr.hpp"
 
+#include <cmath>
+
 namespace sphx {
 
 constexpr int NB_THREADS = 256;
@@ -34,7 +36,7 @@ __device__ __forceinline__ bool in_support(const KP& k, float dsq) {
 }
 
 // ComputeDensityIISPH + contributeIIDensity (kern.cu:1067-1115): rho = sum m W
-__global__ void k_ii_density(KP k, const uint32_t* __restrict__ key, const int* __restrict__ cs,
+__global__ void k_ii_density(KP k, NbrList L, const uint32_t* __restrict__ key, const int* __restrict__ cs,
                              const float4* __restrict__ pos, float4* __restrict__ phys) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= k.n) return;
@@ -42,22 +44,25 @@ __global__ void k_ii_density(KP k, const uint32_t* __restrict__ key, const int* 
   if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
   uint32_t gc = key[i];
   if ((int)gc == k.gridtotal) return;
-  const float rest = phys[i].w;
-  float sum = 0.f;
-  for_each_neighbor(k, cs, gc, [&](int j) {
+  const float4 qi = phys[i];
+  const float rest = qi.w;
+  // self term (the reference's scan visits i itself, W(0) = kspline)
+  float sum = k.kspline * qi.x;
+  for_each_listed(k, L, cs, gc, i, [&](int j) {
     float4 pj = pos[j];
     float4 qj = phys[j];
     float mj = (type_of(pj) == SPHX_TYPE_BOUNDARY) ? rest * qj.y : qj.x;
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
-    float r = sqrtf(dx * dx + dy * dy + dz * dz);
-    sum += cubic_w(r, k.inv_h, k.kspline) * mj;
+    float dsq = dx * dx + dy * dy + dz * dz;
+    if (dsq >= 4.f * k.h2) return;
+    sum += cubic_w(sqrtf(dsq), k.inv_h, k.kspline) * mj;
   });
   phys[i].y = sum;
 }
 
 // ComputeDii (kern.cu:673-798): v_adv = v + dt (a_bvisc + g);
 // d_ii = -dt^2 / rho_i^2 sum_j m_j gradW_ij
-__global__ void k_ii_dii(KP k, const uint32_t* __restrict__ key, const int* __restrict__ cs,
+__global__ void k_ii_dii(KP k, NbrList L, const uint32_t* __restrict__ key, const int* __restrict__ cs,
                          const float4* __restrict__ pos, const float4* __restrict__ vel,
                          const float4* __restrict__ phys, float4* __restrict__ veleval,
                          float4* __restrict__ dii) {
@@ -72,7 +77,7 @@ __global__ void k_ii_dii(KP k, const uint32_t* __restrict__ key, const int* __re
   const float rest = qi.w;
   float ax = 0.f, ay = 0.f, az = 0.f;
   float dx_ = 0.f, dy_ = 0.f, dz_ = 0.f;
-  for_each_neighbor(k, cs, gc, [&](int j) {
+  for_each_listed(k, L, cs, gc, i, [&](int j) {
     float4 pj = pos[j];
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
@@ -109,7 +114,7 @@ __global__ void k_ii_dii(KP k, const uint32_t* __restrict__ key, const int* __re
 }
 
 // ComputeAii (kern.cu:801-908): rho_adv, warm start p0 = w p(t-1), a_ii.
-__global__ void k_ii_aii(KP k, const uint32_t* __restrict__ key, const int* __restrict__ cs,
+__global__ void k_ii_aii(KP k, NbrList L, const uint32_t* __restrict__ key, const int* __restrict__ cs,
                          const float4* __restrict__ pos, const float4* __restrict__ veleval,
                          const float4* __restrict__ phys, float4* __restrict__ dii,
                          float* __restrict__ rho_adv, float* __restrict__ press_l,
@@ -126,7 +131,7 @@ __global__ void k_ii_aii(KP k, const uint32_t* __restrict__ key, const int* __re
   const float4 di = dii[i];
   const float cji = mi / rhoi / rhoi * k.dt * k.dt;
   float dchange = 0.f, aii = 0.f;
-  for_each_neighbor(k, cs, gc, [&](int j) {
+  for_each_listed(k, L, cs, gc, i, [&](int j) {
     float4 pj = pos[j];
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
@@ -157,7 +162,7 @@ __global__ void k_ii_aii(KP k, const uint32_t* __restrict__ key, const int* __re
 }
 
 // Pressure_DP (kern.cu:910-958) plus q_i = d_ii p_i + dijpj_i.
-__global__ void k_ii_dp(KP k, const int* __restrict__ done, const uint32_t* __restrict__ key,
+__global__ void k_ii_dp(KP k, NbrList L, const int* __restrict__ done, const uint32_t* __restrict__ key,
                         const int* __restrict__ cs, const float4* __restrict__ pos,
                         const float* __restrict__ sfac, const float4* __restrict__ phys,
                         const float4* __restrict__ dii, const float* __restrict__ press_l,
@@ -170,7 +175,7 @@ __global__ void k_ii_dp(KP k, const int* __restrict__ done, const uint32_t* __re
   uint32_t gc = key[i];
   if ((int)gc == k.gridtotal) return;
   float sx = 0.f, sy = 0.f, sz = 0.f;
-  for_each_neighbor(k, cs, gc, [&](int j) {
+  for_each_listed(k, L, cs, gc, i, [&](int j) {
     float4 pj = pos[j];
     if (type_of(pj) == SPHX_TYPE_BOUNDARY) return;
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
@@ -210,7 +215,7 @@ __device__ void block_reduce3(double a, double b, double c, double* out) {
 
 // Pressure_Iter (kern.cu:960-1050)
 __global__ __launch_bounds__(NB_THREADS) void k_ii_iter(
-    KP k, const int* __restrict__ done, const uint32_t* __restrict__ key,
+    KP k, NbrList L, const int* __restrict__ done, const uint32_t* __restrict__ key,
     const int* __restrict__ cs, const float4* __restrict__ pos, float4* __restrict__ phys,
     const float4* __restrict__ dii, const float4* __restrict__ dijpj,
     const float4* __restrict__ qm, const float* __restrict__ rho_adv,
@@ -232,7 +237,7 @@ __global__ __launch_bounds__(NB_THREADS) void k_ii_iter(
       const float pl = press_l[i];
       const float cji = k.dt * k.dt * mi / rhoi / rhoi * pl;
       float S = 0.f;
-      for_each_neighbor(k, cs, gc, [&](int j) {
+      for_each_listed(k, L, cs, gc, i, [&](int j) {
         float4 pj = pos[j];
         float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
         float dsq = dx * dx + dy * dy + dz * dz;
@@ -264,7 +269,10 @@ __global__ __launch_bounds__(NB_THREADS) void k_ii_iter(
       press_l1[i] = pn;
       sfac[i] = mi * pn / rhoi / rhoi;
       rs = res;
-      ra = (k.clamp_p ? fmax((double)res, 0.0) : fabs((double)res)) / (double)rest;
+      // NaN must propagate into the sum (fmax would drop it and report
+      // convergence), so the compression-only error is written as a branch
+      double rr = (double)res;
+      ra = (k.clamp_p ? (rr < 0.0 ? 0.0 : rr) : fabs(rr)) / (double)rest;
       rc = 1.0;
     }
   }
@@ -308,7 +316,7 @@ __global__ __launch_bounds__(1024) void k_ii_finalize(int* __restrict__ state,
 // IntegrateIISPH (kern.cu:1052-1064); optional clamp into the valid cells.
 __global__ void k_ii_integrate(KP k, const uint32_t* __restrict__ key, float4* __restrict__ pos,
                                float4* __restrict__ vel, const float4* __restrict__ veleval,
-                               const float4* __restrict__ accel) {
+                               const float4* __restrict__ accel, float4* __restrict__ mforce) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= k.n) return;
   float4 p = pos[i];
@@ -334,6 +342,39 @@ __global__ void k_ii_integrate(KP k, const uint32_t* __restrict__ key, float4* _
   }
   pos[i] = p;
   vel[i] = make_float4(vx, vy, vz, v.w);
+  if (mforce) {
+    // g - a of this step, read by the next drift-velocity pass
+    float inv = 1.f / k.dt;
+    mforce[i] = make_float4(k.g.x - (vx - v.x) * inv, k.g.y - (vy - v.y) * inv,
+                            k.g.z - (vz - v.z) * inv, 0.f);
+  }
+}
+
+// Neighbour list of every fluid particle: all j != i with |x_ij| < 2h, in
+// the reference's 27-cell visiting order.
+__global__ void k_nbr_build(KP k, const uint32_t* __restrict__ key, const int* __restrict__ cs,
+                            const float4* __restrict__ pos, int* __restrict__ idx,
+                            int* __restrict__ cnt, int stride, int cap) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.n) return;
+  float4 pi = pos[i];
+  uint32_t gc = key[i];
+  if (type_of(pi) == SPHX_TYPE_BOUNDARY || (int)gc == k.gridtotal) {
+    cnt[i] = 0;
+    return;
+  }
+  const float rc2 = 4.f * k.h2;
+  int c = 0;
+  for_each_neighbor(k, cs, gc, [&](int j) {
+    float4 pj = pos[j];
+    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+    float dsq = dx * dx + dy * dy + dz * dz;
+    if (dsq < rc2 && j != i) {
+      if (c < cap) idx[(size_t)c * stride + i] = j;
+      ++c;
+    }
+  });
+  cnt[i] = c;
 }
 
 int grid_hash(sphx_context* c);
@@ -347,6 +388,43 @@ static int need_sorted(sphx_context* c, const char* who) {
   return 0;
 }
 
+// Capacity from the particle spacing s = (m/rho0)^(1/3): 1.6x the
+// neighbours of a lattice within 2h, plus slack, rounded to 8, <= 256.
+static int nbr_capacity(const sphx_ParamCarrier& P) {
+  double s = std::cbrt((double)P.mass / (double)P.restdensity) / (double)P.simscale;
+  double ratio = s > 0 ? 2.0 * P.smoothradius / P.simscale / s : 4.0;
+  double nn = 4.18879 * ratio * ratio * ratio;
+  int cap = (int)(1.6 * nn + 8.0);
+  cap = (cap + 7) & ~7;
+  return std::max(16, std::min(256, cap));
+}
+
+int nbr_list(sphx_context* c, NbrList* out) {
+  int n = c->num;
+  int stride = (n + 63) & ~63;
+  if (!c->nbr_valid) {
+    int cap = nbr_capacity(c->P);
+    SPHX_TRY(c->nbr_idx.alloc((size_t)cap * std::max(stride, 64)));
+    SPHX_TRY(c->nbr_cnt.alloc(std::max(n, 1)));
+    c->nbr_cap = cap;
+    c->nbr_stride = stride;
+    if (n > 0) {
+      KP k = make_kp(c);
+      timer_begin(c, "nbr_build");
+      k_nbr_build<<<blocks_for(n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
+          k, c->key.p, c->cell_start.p, c->POS(), c->nbr_idx.p, c->nbr_cnt.p, stride, cap);
+      timer_end(c);
+      SPHX_HIP(hipGetLastError());
+    }
+    c->nbr_valid = true;
+  }
+  out->idx = c->nbr_idx.p;
+  out->cnt = c->nbr_cnt.p;
+  out->stride = c->nbr_stride;
+  out->cap = c->nbr_cap;
+  return 0;
+}
+
 static int ensure_red(sphx_context* c) {
   size_t nb = (size_t)blocks_for(c->cap, NB_THREADS);
   return c->red.alloc(3 * nb + 16);
@@ -355,14 +433,16 @@ static int ensure_red(sphx_context* c) {
 // one DP + Iter + finalize; respect_done=1 inside the pipelined solve
 static int enqueue_iteration(sphx_context* c, const KP& k, int respect_done) {
   int nb = blocks_for(k.n, NB_THREADS);
+  NbrList L;
+  SPHX_TRY(nbr_list(c, &L));
   IterCtl ctl{c->O.min_iters, c->O.max_iters, c->O.tol};
   timer_begin(c, "iisph_dp");
-  k_ii_dp<<<nb, NB_THREADS, 0, c->stream>>>(k, c->solve_state.p, c->key.p, c->cell_start.p,
+  k_ii_dp<<<nb, NB_THREADS, 0, c->stream>>>(k, L, c->solve_state.p, c->key.p, c->cell_start.p,
                                             c->POS(), c->sfac.p, c->PHYS(), c->dii.p,
                                             c->press_l.p, c->dijpj.p, c->qm.p);
   timer_end(c);
   timer_begin(c, "iisph_iter");
-  k_ii_iter<<<nb, NB_THREADS, 0, c->stream>>>(k, c->solve_state.p, c->key.p, c->cell_start.p,
+  k_ii_iter<<<nb, NB_THREADS, 0, c->stream>>>(k, L, c->solve_state.p, c->key.p, c->cell_start.p,
                                               c->POS(), c->PHYS(), c->dii.p, c->dijpj.p, c->qm.p,
                                               c->rho_adv.p, c->press_l.p, c->press_l1.p,
                                               c->sfac.p, c->residue.p, c->ACCEL(), c->red.p);
@@ -396,30 +476,48 @@ int sphx_iisph_density(sphx_context* c) {
   SPHX_TRY(need_sorted(c, "sphx_iisph_density"));
   KP k = make_kp(c);
   if (k.n == 0) return 0;
+  NbrList L;
+  SPHX_TRY(nbr_list(c, &L));
   timer_begin(c, "iisph_density");
   k_ii_density<<<blocks_for(k.n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
-      k, c->key.p, c->cell_start.p, c->POS(), c->PHYS());
+      k, L, c->key.p, c->cell_start.p, c->POS(), c->PHYS());
+  timer_end(c);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+int sphx_iisph_predict_dii(sphx_context* c) {
+  SPHX_TRY(need_sorted(c, "sphx_iisph_predict_dii"));
+  KP k = make_kp(c);
+  if (k.n == 0) return 0;
+  NbrList L;
+  SPHX_TRY(nbr_list(c, &L));
+  timer_begin(c, "iisph_dii");
+  k_ii_dii<<<blocks_for(k.n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
+      k, L, c->key.p, c->cell_start.p, c->POS(), c->VEL(), c->PHYS(), c->VELEVAL(), c->dii.p);
+  timer_end(c);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+int sphx_iisph_predict_aii(sphx_context* c) {
+  SPHX_TRY(need_sorted(c, "sphx_iisph_predict_aii"));
+  KP k = make_kp(c);
+  if (k.n == 0) return 0;
+  NbrList L;
+  SPHX_TRY(nbr_list(c, &L));
+  timer_begin(c, "iisph_aii");
+  k_ii_aii<<<blocks_for(k.n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
+      k, L, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p, c->rho_adv.p,
+      c->press_l.p, c->sfac.p);
   timer_end(c);
   SPHX_HIP(hipGetLastError());
   return 0;
 }
 
 int sphx_iisph_predict_advection(sphx_context* c) {
-  SPHX_TRY(need_sorted(c, "sphx_iisph_predict_advection"));
-  KP k = make_kp(c);
-  if (k.n == 0) return 0;
-  int nb = blocks_for(k.n, NB_THREADS);
-  timer_begin(c, "iisph_dii");
-  k_ii_dii<<<nb, NB_THREADS, 0, c->stream>>>(k, c->key.p, c->cell_start.p, c->POS(), c->VEL(),
-                                             c->PHYS(), c->VELEVAL(), c->dii.p);
-  timer_end(c);
-  timer_begin(c, "iisph_aii");
-  k_ii_aii<<<nb, NB_THREADS, 0, c->stream>>>(k, c->key.p, c->cell_start.p, c->POS(),
-                                             c->VELEVAL(), c->PHYS(), c->dii.p, c->rho_adv.p,
-                                             c->press_l.p, c->sfac.p);
-  timer_end(c);
-  SPHX_HIP(hipGetLastError());
-  return 0;
+  SPHX_TRY(sphx_iisph_predict_dii(c));
+  return sphx_iisph_predict_aii(c);
 }
 
 int sphx_iisph_pressure_iteration(sphx_context* c, double* mean_signed, double* mean_abs_rel) {
@@ -447,8 +545,9 @@ int sphx_iisph_pressure_solve(sphx_context* c, int* iters, double* err) {
   SPHX_HIP(hipMemsetAsync(c->solve_state.p, 0, 2 * sizeof(int), c->stream));
   int enq = 0;
   int chunk = std::max(c->O.min_iters, 2);
-  static thread_local int last_iters = 0;
-  if (last_iters > chunk) chunk = last_iters;
+  // enqueue about as many iterations as the previous solve needed before
+  // the first host check, then two at a time
+  if (c->last_iters - 1 > chunk) chunk = c->last_iters - 1;
   for (;;) {
     int todo = std::min(chunk, c->O.max_iters - enq);
     for (int t = 0; t < todo; ++t) SPHX_TRY(enqueue_iteration(c, k, 1));
@@ -458,7 +557,7 @@ int sphx_iisph_pressure_solve(sphx_context* c, int* iters, double* err) {
     chunk = 2;
   }
   int it = (int)c->host_pinned[3];
-  last_iters = it;
+  c->last_iters = it;
   if (iters) *iters = it;
   if (err) *err = c->host_pinned[0];
   return timer_flush(c);
@@ -468,9 +567,11 @@ int sphx_iisph_integrate(sphx_context* c) {
   SPHX_TRY(need_sorted(c, "sphx_iisph_integrate"));
   KP k = make_kp(c);
   if (k.n == 0) return 0;
+  c->nbr_valid = false;
   timer_begin(c, "iisph_integrate");
   k_ii_integrate<<<blocks_for(k.n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
-      k, c->key.p, c->POS(), c->VEL(), c->VELEVAL(), c->ACCEL());
+      k, c->key.p, c->POS(), c->VEL(), c->VELEVAL(), c->ACCEL(),
+      c->nphase > 0 ? c->mf_force[c->cur].p : nullptr);
   timer_end(c);
   SPHX_HIP(hipGetLastError());
   return 0;

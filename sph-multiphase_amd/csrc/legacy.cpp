@@ -108,6 +108,10 @@ void PressureSolve(void) {
 }
 
 void Integration(void) { run("Integration", sphx_iisph_integrate); }
+void MfComputeDriftVelCUDA(void) { run("MfComputeDriftVelCUDA", sphx_mf_drift_velocity); }
+void MfComputeAlphaAdvanceCUDA(void) { run("MfComputeAlphaAdvanceCUDA", sphx_mf_alpha_advance); }
+void MfComputeCorrectionCUDA(void) { run("MfComputeCorrectionCUDA", sphx_mf_correction); }
+void ComputeForceCUDA_ProjectU(void) { run("ComputeForceCUDA_ProjectU", sphx_mf_force); }
 
 void sphx_legacy_upload(const sphx_displayPack* disp, const sphx_calculationPack* calc, int start,
                         int end) {

@@ -25,4 +25,34 @@ __device__ __forceinline__ void for_each_neighbor(const KP& k, const int* __rest
   }
 }
 
+// Per-step neighbour list in ELL layout: entry q of particle i is
+// idx[q * stride + i] (consecutive particles read consecutive words), in the
+// reference's visiting order, holding every j != i with |x_ij| < 2h (the
+// widest support any kernel of the step uses).  Positions do not move
+// between the density pass and the last pressure iteration, so one list
+// serves every neighbour loop of an IISPH step.  A particle with more than
+// `cap` neighbours keeps its true count and its loops fall back to the cell
+// scan, which visits a superset in the same order.
+struct NbrList {
+  const int* __restrict__ idx;
+  const int* __restrict__ cnt;
+  int stride;
+  int cap;
+};
+
+template <class F>
+__device__ __forceinline__ void for_each_listed(const KP& k, const NbrList& L,
+                                                const int* __restrict__ cell_start, uint32_t cell,
+                                                int i, F&& f) {
+  const int c = L.cnt[i];
+  if (c <= L.cap) {
+#pragma unroll 2
+    for (int q = 0; q < c; ++q) f(L.idx[(size_t)q * L.stride + i]);
+  } else {
+    for_each_neighbor(k, cell_start, cell, [&](int j) {
+      if (j != i) f(j);
+    });
+  }
+}
+
 }  // namespace sphx

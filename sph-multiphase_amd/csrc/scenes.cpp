@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 #include <cstdint>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/sphx.h"
@@ -169,6 +170,129 @@ int sphx_scene_dambreak(int nx, int ny, int nz, float s, float Lx, float Ly, flo
       init_particle(disp[p], calc[p], bx[b], by[b], bz[b], SPHX_TYPE_BOUNDARY, m, rho0, 0.4f, p);
   }
   return nf + nb;
+}
+
+// Two-phase bubble tank: n_fluid particles on a cubic lattice of spacing
+// s = (fill * Lx * Lz / n_fluid)^(1/3) filling a Lx x Ly x Lz tank from the
+// floor in y-major order; n_bubbles spheres with centres uniform in the
+// water and radii uniform in [rmin, rmax] mark air (alpha_air = 1).  All
+// fluid particles carry the solver mass rho_ref * s^3 and rest density
+// rho_ref (the pressure solve enforces the mixture's volume); the phase
+// fractions go to `alpha` ([n_total][2]: water, air; boundary rows 0).
+int sphx_scene_bubbles(int n_fluid, float Lx, float Ly, float Lz, float fill, float h,
+                       int n_bubbles, float rmin, float rmax, int layers, uint64_t seed,
+                       sphx_ParamCarrier* params, sphx_displayPack* disp,
+                       sphx_calculationPack* calc, float* alpha, sphx_SceneInfo* info) {
+  if (n_fluid <= 0 || !(Lx > 0) || !(Ly > 0) || !(Lz > 0) || !(fill > 0) || !(h > 0) ||
+      layers < 0 || n_bubbles < 0 || rmax < rmin)
+    return -1;
+  const float rho_ref = 1000.f;
+  double sd = std::cbrt((double)fill * Lx * Lz / n_fluid);
+  float s = (float)sd;
+  int nx = (int)std::floor(Lx / s), nz = (int)std::floor(Lz / s);
+  if (nx < 1 || nz < 1) return -1;
+  int ny = (n_fluid + nx * nz - 1) / (nx * nz);
+  // boundary walls (floor + 4 sides, open top), `layers` layers outside
+  std::vector<float> bx, by, bz;
+  int bnx = nx, bnz = nz, bny = (int)std::ceil(Ly / s);
+  for (int iy = -layers; iy < bny; ++iy)
+    for (int iz = -layers; iz < bnz + layers; ++iz)
+      for (int ix = -layers; ix < bnx + layers; ++ix) {
+        bool inside = ix >= 0 && ix < bnx && iz >= 0 && iz < bnz && iy >= 0;
+        if (inside) continue;
+        bx.push_back((ix + 0.5f) * s);
+        by.push_back((iy + 0.5f) * s);
+        bz.push_back((iz + 0.5f) * s);
+      }
+  int nb = (int)bx.size();
+  if (info) {
+    info->n_fluid = n_fluid;
+    info->n_boundary = nb;
+    info->n_total = n_fluid + nb;
+    info->n_phase = 2;
+    info->spacing = s;
+  }
+  const float m = rho_ref * s * s * s;
+  if (params) {
+    sphx_ParamCarrier& P = *params;
+    std::memset(&P, 0, sizeof(P));
+    P.acclimit = 1500.f;
+    P.vlimit = 100.f;
+    P.gravity = {0.f, -9.81f, 0.f};
+    P.simscale = 1.f;
+    P.dt = 1e-3f;
+    P.smoothradius = h;
+    P.radius = s * 0.5f;
+    P.restdensity = rho_ref;
+    P.mass = m;
+    P.viscosity = 1e-6f;
+    P.massArr[0] = m; P.massArr[1] = m; P.massArr[2] = m;
+    P.densArr[0] = 1000.f; P.densArr[1] = 1.2f; P.densArr[2] = 1000.f;
+    P.viscArr[0] = 1e-6f; P.viscArr[1] = 1.5e-5f; P.viscArr[2] = 1e-6f;
+    P.bmass = m;
+    P.bRestdensity = rho_ref;
+    P.bvisc = 0.4f;
+    sphx_setup_kernels(&P);
+    float cell = P.cellsize;
+    int margin = (int)std::ceil(layers * s / cell) + 1;
+    P.volmin = {-margin * cell, -margin * cell, -margin * cell};
+    P.volmax = {nx * s + margin * cell, bny * s + margin * cell, nz * s + margin * cell};
+    P.softminx = {0.f, 0.f, 0.f};
+    P.softmaxx = {nx * s, bny * s, nz * s};
+    sphx_setup_grid(&P);
+    P.num = n_fluid + nb;
+    P.maxNum = n_fluid + nb;
+  }
+  if (disp && calc) {
+    Rng rng(seed);
+    const float water_top = (float)ny * s;
+    std::vector<float> cx(n_bubbles), cy(n_bubbles), cz(n_bubbles), rr(n_bubbles);
+    for (int b = 0; b < n_bubbles; ++b) {
+      cx[b] = rng.uniform() * nx * s;
+      cy[b] = rng.uniform() * water_top;
+      cz[b] = rng.uniform() * nz * s;
+      rr[b] = rmin + rng.uniform() * (rmax - rmin);
+    }
+    // bucket bubbles by a coarse grid for the inside test
+    const float cb = 2.f * rmax > s ? 2.f * rmax : s;
+    int gx = (int)std::ceil(nx * s / cb) + 1, gy = (int)std::ceil(water_top / cb) + 1,
+        gz = (int)std::ceil(nz * s / cb) + 1;
+    std::vector<std::vector<int>> bucket((size_t)gx * gy * gz);
+    for (int b = 0; b < n_bubbles; ++b) {
+      int ix = std::min(gx - 1, (int)(cx[b] / cb)), iy = std::min(gy - 1, (int)(cy[b] / cb)),
+          iz = std::min(gz - 1, (int)(cz[b] / cb));
+      bucket[((size_t)iy * gz + iz) * gx + ix].push_back(b);
+    }
+    int p = 0;
+    for (int iy = 0; iy < ny && p < n_fluid; ++iy)
+      for (int iz = 0; iz < nz && p < n_fluid; ++iz)
+        for (int ix = 0; ix < nx && p < n_fluid; ++ix, ++p) {
+          float x = (ix + 0.5f) * s, y = (iy + 0.5f) * s, z = (iz + 0.5f) * s;
+          init_particle(disp[p], calc[p], x, y, z, SPHX_TYPE_FLUID, m, rho_ref, 1e-6f, p);
+          bool air = false;
+          int bx0 = (int)(x / cb), by0 = (int)(y / cb), bz0 = (int)(z / cb);
+          for (int dy = -1; dy <= 1 && !air; ++dy)
+            for (int dz = -1; dz <= 1 && !air; ++dz)
+              for (int dx = -1; dx <= 1 && !air; ++dx) {
+                int qx = bx0 + dx, qy = by0 + dy, qz = bz0 + dz;
+                if (qx < 0 || qy < 0 || qz < 0 || qx >= gx || qy >= gy || qz >= gz) continue;
+                for (int b : bucket[((size_t)qy * gz + qz) * gx + qx]) {
+                  float ddx = x - cx[b], ddy = y - cy[b], ddz = z - cz[b];
+                  if (ddx * ddx + ddy * ddy + ddz * ddz < rr[b] * rr[b]) { air = true; break; }
+                }
+              }
+          if (alpha) {
+            alpha[2 * p] = air ? 0.f : 1.f;
+            alpha[2 * p + 1] = air ? 1.f : 0.f;
+          }
+          if (air) disp[p].color = {0.9f, 0.9f, 1.f, 0.3f};
+        }
+    for (int b = 0; b < nb; ++b, ++p) {
+      init_particle(disp[p], calc[p], bx[b], by[b], bz[b], SPHX_TYPE_BOUNDARY, m, rho_ref, 0.4f, p);
+      if (alpha) { alpha[2 * p] = 0.f; alpha[2 * p + 1] = 0.f; }
+    }
+  }
+  return n_fluid + nb;
 }
 
 }  // extern "C"

@@ -141,6 +141,16 @@ def lib() -> C.CDLL:
         L.sphx_scene_dambreak.argtypes = [I, I, I, C.c_float, C.c_float, C.c_float, C.c_float,
                                           C.c_float, I, C.c_float, C.c_uint64, I,
                                           P(ParamCarrier), VP, VP, P(SceneInfo)]
+        for name in ["sphx_iisph_predict_dii", "sphx_iisph_predict_aii", "sphx_mf_drift_velocity",
+                     "sphx_mf_alpha_advance", "sphx_mf_correction", "sphx_mf_force"]:
+            getattr(L, name).argtypes = [VP]
+        L.sphx_mf_setup.argtypes = [VP, I, VP, VP]
+        L.sphx_mf_upload.argtypes = [VP, VP, I, I]
+        L.sphx_mf_download.argtypes = [VP, VP, VP, VP, VP]
+        L.sphx_mf_step.argtypes = [VP, P(I), P(D)]
+        L.sphx_scene_bubbles.argtypes = [I, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                         I, C.c_float, C.c_float, I, C.c_uint64, P(ParamCarrier),
+                                         VP, VP, VP, P(SceneInfo)]
         _lib = L
     return _lib
 
@@ -178,6 +188,23 @@ def scene_dambreak(nx=32, ny=32, nz=32, s=0.01, L=(0.64, 0.32, 0.32), h=0.02, la
     L_.sphx_scene_dambreak(nx, ny, nz, s, L[0], L[1], L[2], h, layers, jitter, seed,
                            kernel_family, C.byref(p), ptr(disp), ptr(calc), C.byref(info))
     return p, disp, calc, info
+
+
+def scene_bubbles(n_fluid=1 << 20, L=(1.0, 1.0, 1.0), fill=0.8, h=0.01, n_bubbles=512,
+                  rmin=0.01, rmax=0.04, layers=2, seed=20261203):
+    """Two-phase bubble tank; returns (params, disp, calc, alpha[n][2], info)."""
+    L_ = lib()
+    info = SceneInfo()
+    p = ParamCarrier()
+    args = (n_fluid, L[0], L[1], L[2], fill, h, n_bubbles, rmin, rmax, layers, seed)
+    rc = L_.sphx_scene_bubbles(*args, C.byref(p), None, None, None, C.byref(info))
+    if rc < 0:
+        raise SphxError("sphx_scene_bubbles: bad arguments")
+    disp = np.zeros(info.n_total, display_dtype)
+    calc = np.zeros(info.n_total, calc_dtype)
+    alpha = np.zeros((info.n_total, 2), np.float32)
+    L_.sphx_scene_bubbles(*args, C.byref(p), ptr(disp), ptr(calc), ptr(alpha), C.byref(info))
+    return p, disp, calc, alpha, info
 
 
 # ----------------------------------------------------------------- context --
@@ -277,6 +304,30 @@ class Context:
     def iisph_step(self):
         it, err = C.c_int(), C.c_double()
         check(self.L.sphx_iisph_step(self.h, C.byref(it), C.byref(err)), "iisph_step")
+        return it.value, err.value
+
+    # -- multiphase --
+    def mf_setup(self, rho, nu):
+        rho = np.ascontiguousarray(rho, np.float32)
+        nu = np.ascontiguousarray(nu, np.float32)
+        self.nphase = len(rho)
+        check(self.L.sphx_mf_setup(self.h, len(rho), ptr(rho), ptr(nu)), "sphx_mf_setup")
+
+    def mf_upload(self, alpha, start=0):
+        alpha = np.ascontiguousarray(alpha, np.float32)
+        check(self.L.sphx_mf_upload(self.h, ptr(alpha), start, len(alpha)), "sphx_mf_upload")
+
+    def mf_download(self):
+        n, k = self.num, self.nphase
+        out = dict(alpha=np.zeros((n, k), np.float32), rho_m=np.zeros(n, np.float32),
+                   drift=np.zeros((n, k, 3), np.float32), mforce=np.zeros((n, 3), np.float32))
+        check(self.L.sphx_mf_download(self.h, ptr(out["alpha"]), ptr(out["rho_m"]),
+                                      ptr(out["drift"]), ptr(out["mforce"])), "sphx_mf_download")
+        return out
+
+    def mf_step(self):
+        it, err = C.c_int(), C.c_double()
+        check(self.L.sphx_mf_step(self.h, C.byref(it), C.byref(err)), "mf_step")
         return it.value, err.value
 
     def timing(self, on=True):

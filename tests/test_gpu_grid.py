@@ -77,6 +77,7 @@ def test_grid_build_twice_nearly_sorted(gpu):
 
 def test_grid_empty_and_single(gpu):
     p, disp, calc, info = shuffled_scene(nx=2, ny=2, nz=2, layers=0)
+    p.num = 0
     for n in [0, 1]:
         with sphx.Context(p) as ctx:
             if n:

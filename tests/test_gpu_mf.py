@@ -1,0 +1,126 @@
+"""Drift-flux mixture stages on the GPU vs the oracle (FP32 restatement and
+FP64 twin), each stage from identical inputs, after three two-phase steps
+so pressures, velocities and fractions are non-trivial.
+
+Tolerances (normwise, infinity norm, vs FP64): drift velocity 1e-4,
+fractions 1e-5, mixture acceleration 1e-4 (or 4x the CPU-FP32 error)."""
+import numpy as np
+import pytest
+
+import orc
+import sphx
+from helpers import assert_close, options, state_from_gpu
+
+pytestmark = pytest.mark.gpu
+
+RHO = [1000.0, 1.2]
+NU = [1e-6, 1.5e-5]
+UNDEF = 0xFFFFFFFF
+
+
+def setup():
+    p, disp, calc, alpha, info = sphx.scene_bubbles(n_fluid=6000, L=(0.2, 0.2, 0.2), fill=0.8,
+                                                    h=0.012, n_bubbles=12, rmin=0.02, rmax=0.04,
+                                                    layers=2, seed=3)
+    p.dt = 1e-3
+    o = options(wcsph_kernel=1, full_support=1, boundary_pressure=1, omega=0.5,
+                clamp_pressure=1, tol=1e-3, max_iters=200, drift_relax=2e-11,
+                drift_diffusion=1e-4, boussinesq=1, clamp_to_grid=1)
+    return p, disp, calc, alpha, o
+
+
+def oracle_mf(ctx, p, o, dtype, alpha_pre, mforce, drift=None):
+    st = state_from_gpu(ctx, p, dtype, o)
+    mf = orc.MF(st, RHO, NU, o)
+    rho = np.array([np.float32(r) for r in RHO], np.float64)
+    mf.set_alpha(alpha_pre, alpha_pre @ rho)
+    mf.a["mforce"][:] = mforce
+    if drift is not None:
+        mf.a["drift"][:, :2, :] = drift
+    return st, mf
+
+
+def test_mf_stages_match_oracle(gpu):
+    p, disp, calc, alpha, o = setup()
+    with sphx.Context(p, o) as ctx:
+        ctx.upload(disp, calc)
+        ctx.mf_setup(RHO, NU)
+        ctx.mf_upload(alpha)
+        ctx.call("grid_build")
+        ctx.call("boundary_volume")
+        for _ in range(3):
+            ctx.mf_step()
+        ctx.call("grid_build")
+        ctx.call("iisph_density")
+        pre = ctx.mf_download()
+        s32, m32 = oracle_mf(ctx, p, o, np.float32, pre["alpha"], pre["mforce"])
+        s64, m64 = oracle_mf(ctx, p, o, np.float64, pre["alpha"], pre["mforce"])
+        fl = (s64["type"] == 0) & (s64["mgcell"] != UNDEF)
+        mixed = fl & (pre["alpha"][:, 1] > 1e-6) & (pre["alpha"][:, 1] < 1 - 1e-6)
+        print(f"fluid {fl.sum()} mixed {mixed.sum()}", flush=True)
+        assert mixed.sum() > 10
+        # drift velocity
+        m32.drift()
+        m64.drift()
+        ctx.call("mf_drift_velocity")
+        g = ctx.mf_download()
+        assert_close("drift", g["drift"], m32["drift"][:, :2], m64["drift"][:, :2], 1e-4,
+                     fl[:, None, None] & np.ones((1, 2, 3), bool))
+        # fraction advection + correction, from the GPU's drift
+        s32, m32 = oracle_mf(ctx, p, o, np.float32, pre["alpha"], pre["mforce"], g["drift"])
+        s64, m64 = oracle_mf(ctx, p, o, np.float64, pre["alpha"], pre["mforce"], g["drift"])
+        for m in (m32, m64):
+            m.alpha_advance()
+            m.correction()
+        ctx.call("mf_alpha_advance")
+        ctx.call("mf_correction")
+        g2 = ctx.mf_download()
+        d, c = ctx.download()
+        assert_close("alpha", g2["alpha"], m32["alpha_new"][:, :2], m64["alpha_new"][:, :2], 1e-5,
+                     fl[:, None] & np.ones((1, 2), bool))
+        assert_close("rho_m", g2["rho_m"], m32["alpha_new"][:, 3], m64["alpha_new"][:, 3], 1e-5, fl)
+        assert_close("nu_m", c["visc"], s32["visc"], s64["visc"], 1e-5, fl)
+        assert np.allclose(g2["alpha"][fl].sum(1), 1.0, atol=1e-6)
+        # mixture acceleration (drift stress + buoyancy) on the advected velocity
+        ctx.call("iisph_predict_dii")
+        d0, c0 = ctx.download()
+        s32, m32 = oracle_mf(ctx, p, o, np.float32, pre["alpha"], pre["mforce"], g["drift"])
+        s64, m64 = oracle_mf(ctx, p, o, np.float64, pre["alpha"], pre["mforce"], g["drift"])
+        rho = np.array([np.float32(r) for r in RHO], np.float64)
+        for m in (m32, m64):
+            m.a["alpha_new"][:, :2] = g2["alpha"]
+            m.a["alpha_new"][:, 3] = g2["rho_m"]
+            m.force()
+        ctx.call("mf_force")
+        d1, c1 = ctx.download()
+        a_gpu = (c1["veleval"].astype(np.float64) - c0["veleval"]) / p.dt
+        assert_close("a_mf", a_gpu, m32["amf"], m64["amf"], 1e-4, fl[:, None] & np.ones((1, 3), bool))
+
+
+def test_mf_steps_keep_fractions_and_converge(gpu):
+    """Ten two-phase steps: fractions stay in [0,1] and sum to 1, the solve
+    converges every step, and the air rises (buoyancy)."""
+    p, disp, calc, alpha, o = setup()
+    with sphx.Context(p, o) as ctx:
+        ctx.upload(disp, calc)
+        ctx.mf_setup(RHO, NU)
+        ctx.mf_upload(alpha)
+        ctx.call("grid_build")
+        ctx.call("boundary_volume")
+        d0, c0 = ctx.download()
+        a0 = ctx.mf_download()["alpha"]
+        fl0 = d0["type"] == 0
+        y_air0 = np.average(d0["pos"][fl0, 1], weights=a0[fl0, 1])
+        its = []
+        for _ in range(10):
+            it, err = ctx.mf_step()
+            assert np.isfinite(err) and err <= o.tol, (it, err)
+            its.append(it)
+        d, c = ctx.download()
+        a = ctx.mf_download()["alpha"]
+    fl = d["type"] == 0
+    assert np.all(a[fl] >= 0) and np.all(a[fl] <= 1 + 1e-6)
+    assert np.allclose(a[fl].sum(1), 1.0, atol=1e-5)
+    y_air = np.average(d["pos"][fl, 1], weights=np.maximum(a[fl, 1], 0) + 1e-30)
+    print("iterations", its, "air centroid", y_air0, "->", y_air)
+    assert y_air > y_air0
