@@ -97,7 +97,7 @@ def cpu_baseline(sample_fluid=131072, mean_iters=10.0, timed_iters=20):
     mf.a["mforce"][:] = [0.0, -9.81, 0.0]
     fl = st["type"] == 0
     st.a["visc"][fl] = (alpha[fl] @ np.array(NU, np.float32))
-    st.boundary_volume()
+    # boundary volumes come with the scene (lattice volume)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.perf_counter()
     before = mf.a["alpha"].copy(), mf.a["mforce"].copy()
@@ -157,7 +157,6 @@ def main():
     ctx.mf_setup(RHO, NU)
     ctx.mf_upload(alpha)
     ctx.call("grid_build")
-    ctx.call("boundary_volume")
     ctx.sync()
     for s in range(args.warmup):
         it, err = ctx.mf_step()

@@ -383,7 +383,7 @@ int sphx_scene_dambreak(int nx, int ny, int nz, float s, float Lx, float Ly,
                         sphx_calculationPack* calc, sphx_SceneInfo* info);
 
 /* Two-phase bubble tank (water 1000 / air 1.2): n_fluid particles on a
- * lattice of spacing s = (fill*Lx*Lz/n_fluid)^(1/3) filling a Lx*Ly*Lz tank
+ * lattice of spacing s = (fill*Lx*Ly*Lz/n_fluid)^(1/3) filling a Lx*Ly*Lz tank
  * from the floor (y-major order); n_bubbles seeded spheres, radii uniform in
  * [rmin, rmax], mark air.  Every fluid particle carries the solver mass
  * 1000*s^3 and rest density 1000; `alpha` receives [n_total][2] volume

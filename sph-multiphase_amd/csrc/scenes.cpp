@@ -187,7 +187,7 @@ int sphx_scene_bubbles(int n_fluid, float Lx, float Ly, float Lz, float fill, fl
       layers < 0 || n_bubbles < 0 || rmax < rmin)
     return -1;
   const float rho_ref = 1000.f;
-  double sd = std::cbrt((double)fill * Lx * Lz / n_fluid);
+  double sd = std::cbrt((double)fill * Lx * Ly * Lz / n_fluid);
   float s = (float)sd;
   int nx = (int)std::floor(Lx / s), nz = (int)std::floor(Lz / s);
   if (nx < 1 || nz < 1) return -1;
@@ -212,7 +212,25 @@ int sphx_scene_bubbles(int n_fluid, float Lx, float Ly, float Lz, float fill, fl
     info->n_phase = 2;
     info->spacing = s;
   }
-  const float m = rho_ref * s * s * s;
+  // Lattice sum f = sum_lattice s^3 W(r) of the cubic spline (support 2h):
+  // mass rho_ref s^3 / f puts a resting lattice exactly at rho_ref, and the
+  // wall particles (a continuation of the lattice) get the volume s^3 / f,
+  // so fluid next to walls, edges and corners sees a complete neighbourhood.
+  double fsum = 0.0;
+  {
+    const double kspl = 1.0 / 3.141593 / ((double)h * h * h);
+    int R = (int)std::ceil(2.0 * h / s) + 1;
+    for (int a = -R; a <= R; ++a)
+      for (int b = -R; b <= R; ++b)
+        for (int cc = -R; cc <= R; ++cc) {
+          double q = std::sqrt((double)a * a + (double)b * b + (double)cc * cc) * s / h;
+          double w = q > 2 ? 0.0 : (q > 1 ? 0.25 * (2 - q) * (2 - q) * (2 - q)
+                                           : 1 - 1.5 * q * q + 0.75 * q * q * q);
+          fsum += w * kspl * sd * sd * sd;
+        }
+  }
+  const float m = (float)(rho_ref * sd * sd * sd / fsum);
+  const float vb = (float)(sd * sd * sd / fsum);
   if (params) {
     sphx_ParamCarrier& P = *params;
     std::memset(&P, 0, sizeof(P));
@@ -269,6 +287,8 @@ int sphx_scene_bubbles(int n_fluid, float Lx, float Ly, float Lz, float fill, fl
         for (int ix = 0; ix < nx && p < n_fluid; ++ix, ++p) {
           float x = (ix + 0.5f) * s, y = (iy + 0.5f) * s, z = (iz + 0.5f) * s;
           init_particle(disp[p], calc[p], x, y, z, SPHX_TYPE_FLUID, m, rho_ref, 1e-6f, p);
+          // hydrostatic initial pressure rho_ref g (top - y)
+          calc[p].pressure = rho_ref * 9.81f * std::max(0.f, water_top - y);
           bool air = false;
           int bx0 = (int)(x / cb), by0 = (int)(y / cb), bz0 = (int)(z / cb);
           for (int dy = -1; dy <= 1 && !air; ++dy)
@@ -289,6 +309,7 @@ int sphx_scene_bubbles(int n_fluid, float Lx, float Ly, float Lz, float fill, fl
         }
     for (int b = 0; b < nb; ++b, ++p) {
       init_particle(disp[p], calc[p], bx[b], by[b], bz[b], SPHX_TYPE_BOUNDARY, m, rho_ref, 0.4f, p);
+      calc[p].dens = vb;   // boundary volume (lattice volume, see above)
       if (alpha) { alpha[2 * p] = 0.f; alpha[2 * p + 1] = 0.f; }
     }
   }

@@ -35,7 +35,6 @@ def mf_state(mixed=True, seed=3, still=False):
     # alpha travels with the sort
     src = np.where(st["mgcell"] != UNDEF, st["midsort"], np.arange(len(disp)))
     alpha = alpha[src]
-    st.boundary_volume()
     st.ii_density()
     mf = orc.MF(st, RHO, NU, o)
     mf.set_alpha(alpha, alpha @ np.array(RHO))
