@@ -58,6 +58,17 @@ __host__ __device__ inline float3 mul(float3 a, float s) { return f3(a.x * s, a.
 __host__ __device__ inline float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ inline int type_of(const float4& p) { return __float_as_int(p.w); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Give XCD x the contiguous tile range [x*q + min(x,r), ...) (q, r =
+// gridDim / 8, gridDim % 8) so the cell-sorted neighbours its waves gather
+// stay in its L2 instead of every XCD streaming the whole domain.
+__device__ __forceinline__ int xcd_tile() {
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+__device__ __forceinline__ int xcd_index() { return xcd_tile() * (int)blockDim.x + (int)threadIdx.x; }
+
 // Cubic spline kernel W and gradient, support 2h (fluid_system_kern.cu:637-671).
 // `r` is x_i - x_j in metres.
 __device__ inline float cubic_w(float rmod, float inv_h, float kspline) {
@@ -166,7 +177,8 @@ struct sphx_context {
   // per-step neighbour list (ELL: idx[q * stride + i]); rebuilt after any
   // change of positions or order
   sphx::DevBuf<int> nbr_idx, nbr_cnt;
-  int nbr_cap = 0, nbr_stride = 0;
+  sphx::DevBuf<float4> ps;        // {x, y, z, s = m p / rho^2} for the DP pass
+  int nbr_cap = 0;
   bool nbr_valid = false;
 
   // multiphase (drift-flux mixture); nphase == 0 when disabled

@@ -38,7 +38,7 @@ __device__ __forceinline__ bool in_support(const KP& k, float dsq) {
 // ComputeDensityIISPH + contributeIIDensity (kern.cu:1067-1115): rho = sum m W
 __global__ void k_ii_density(KP k, NbrList L, const uint32_t* __restrict__ key, const int* __restrict__ cs,
                              const float4* __restrict__ pos, float4* __restrict__ phys) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
   if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
@@ -66,7 +66,7 @@ __global__ void k_ii_dii(KP k, NbrList L, const uint32_t* __restrict__ key, cons
                          const float4* __restrict__ pos, const float4* __restrict__ vel,
                          const float4* __restrict__ phys, float4* __restrict__ veleval,
                          float4* __restrict__ dii) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
   if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
@@ -118,8 +118,8 @@ __global__ void k_ii_aii(KP k, NbrList L, const uint32_t* __restrict__ key, cons
                          const float4* __restrict__ pos, const float4* __restrict__ veleval,
                          const float4* __restrict__ phys, float4* __restrict__ dii,
                          float* __restrict__ rho_adv, float* __restrict__ press_l,
-                         float* __restrict__ sfac) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+                         float4* __restrict__ ps) {
+  int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
   if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
@@ -157,32 +157,33 @@ __global__ void k_ii_aii(KP k, NbrList L, const uint32_t* __restrict__ key, cons
   rho_adv[i] = rhoi + dchange;
   float p0 = k.warm * qi.z;
   press_l[i] = p0;
-  sfac[i] = mi * p0 / rhoi / rhoi;
+  ps[i].w = mi * p0 / rhoi / rhoi;
   dii[i].w = aii;
 }
 
 // Pressure_DP (kern.cu:910-958) plus q_i = d_ii p_i + dijpj_i.
 __global__ void k_ii_dp(KP k, NbrList L, const int* __restrict__ done, const uint32_t* __restrict__ key,
                         const int* __restrict__ cs, const float4* __restrict__ pos,
-                        const float* __restrict__ sfac, const float4* __restrict__ phys,
+                        const float4* __restrict__ ps, const float4* __restrict__ phys,
                         const float4* __restrict__ dii, const float* __restrict__ press_l,
                         float4* __restrict__ dijpj, float4* __restrict__ qm) {
   if (*done) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
   if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
   uint32_t gc = key[i];
   if ((int)gc == k.gridtotal) return;
   float sx = 0.f, sy = 0.f, sz = 0.f;
+  // one 16-byte gather per neighbour: {x_j, s_j}; boundary s_j = 0 adds
+  // nothing, as the reference's skip of boundary j (:925-926)
   for_each_listed(k, L, cs, gc, i, [&](int j) {
-    float4 pj = pos[j];
-    if (type_of(pj) == SPHX_TYPE_BOUNDARY) return;
+    float4 pj = ps[j];
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
     if (!in_support(k, dsq)) return;
     float r = sqrtf(dsq);
-    float gf = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline) * sfac[j];
+    float gf = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline) * pj.w;
     sx += dx * gf; sy += dy * gf; sz += dz * gf;
   });
   float f = -k.dt * k.dt;
@@ -219,10 +220,10 @@ __global__ __launch_bounds__(NB_THREADS) void k_ii_iter(
     const int* __restrict__ cs, const float4* __restrict__ pos, float4* __restrict__ phys,
     const float4* __restrict__ dii, const float4* __restrict__ dijpj,
     const float4* __restrict__ qm, const float* __restrict__ rho_adv,
-    float* __restrict__ press_l, float* __restrict__ press_l1, float* __restrict__ sfac,
+    float* __restrict__ press_l, float* __restrict__ press_l1, float4* __restrict__ ps,
     float* __restrict__ residue, float4* __restrict__ accel, double* __restrict__ partial) {
   if (*done) return;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   double rs = 0.0, ra = 0.0, rc = 0.0;
   if (i < k.n) {
     float4 pi = pos[i];
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(NB_THREADS) void k_ii_iter(
       phys[i].z = pn;
       press_l[i] = pn;
       press_l1[i] = pn;
-      sfac[i] = mi * pn / rhoi / rhoi;
+      ps[i].w = mi * pn / rhoi / rhoi;
       rs = res;
       // NaN must propagate into the sum (fmax would drop it and report
       // convergence), so the compression-only error is written as a branch
@@ -317,7 +318,7 @@ __global__ __launch_bounds__(1024) void k_ii_finalize(int* __restrict__ state,
 __global__ void k_ii_integrate(KP k, const uint32_t* __restrict__ key, float4* __restrict__ pos,
                                float4* __restrict__ vel, const float4* __restrict__ veleval,
                                const float4* __restrict__ accel, float4* __restrict__ mforce) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i >= k.n) return;
   float4 p = pos[i];
   if (type_of(p) == SPHX_TYPE_BOUNDARY) return;
@@ -352,25 +353,30 @@ __global__ void k_ii_integrate(KP k, const uint32_t* __restrict__ key, float4* _
 
 // Neighbour list of every fluid particle: all j != i with |x_ij| < 2h, in
 // the reference's 27-cell visiting order.
+// Also seeds ps[i] = {x, y, z, 0}: the pressure pass gathers position and
+// s_j = m_j p_j / rho_j^2 as one record (s stays 0 for boundary particles,
+// whose pressure is not an unknown).
 __global__ void k_nbr_build(KP k, const uint32_t* __restrict__ key, const int* __restrict__ cs,
                             const float4* __restrict__ pos, int* __restrict__ idx,
-                            int* __restrict__ cnt, int stride, int cap) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+                            int* __restrict__ cnt, float4* __restrict__ ps, int cap) {
+  int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
+  ps[i] = make_float4(pi.x, pi.y, pi.z, 0.f);
   uint32_t gc = key[i];
   if (type_of(pi) == SPHX_TYPE_BOUNDARY || (int)gc == k.gridtotal) {
     cnt[i] = 0;
     return;
   }
   const float rc2 = 4.f * k.h2;
+  int* __restrict__ out = idx + nbr_slot(i, 0, cap);
   int c = 0;
   for_each_neighbor(k, cs, gc, [&](int j) {
     float4 pj = pos[j];
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
     if (dsq < rc2 && j != i) {
-      if (c < cap) idx[(size_t)c * stride + i] = j;
+      if (c < cap) out[(size_t)c * NBR_SLICE] = j;
       ++c;
     }
   });
@@ -401,18 +407,18 @@ static int nbr_capacity(const sphx_ParamCarrier& P) {
 
 int nbr_list(sphx_context* c, NbrList* out) {
   int n = c->num;
-  int stride = (n + 63) & ~63;
   if (!c->nbr_valid) {
     int cap = nbr_capacity(c->P);
-    SPHX_TRY(c->nbr_idx.alloc((size_t)cap * std::max(stride, 64)));
+    size_t slices = (size_t)(n + NBR_SLICE - 1) / NBR_SLICE;
+    SPHX_TRY(c->nbr_idx.alloc(std::max<size_t>(slices, 1) * cap * NBR_SLICE));
     SPHX_TRY(c->nbr_cnt.alloc(std::max(n, 1)));
+    SPHX_TRY(c->ps.alloc(c->cap));
     c->nbr_cap = cap;
-    c->nbr_stride = stride;
     if (n > 0) {
       KP k = make_kp(c);
       timer_begin(c, "nbr_build");
       k_nbr_build<<<blocks_for(n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
-          k, c->key.p, c->cell_start.p, c->POS(), c->nbr_idx.p, c->nbr_cnt.p, stride, cap);
+          k, c->key.p, c->cell_start.p, c->POS(), c->nbr_idx.p, c->nbr_cnt.p, c->ps.p, cap);
       timer_end(c);
       SPHX_HIP(hipGetLastError());
     }
@@ -420,7 +426,6 @@ int nbr_list(sphx_context* c, NbrList* out) {
   }
   out->idx = c->nbr_idx.p;
   out->cnt = c->nbr_cnt.p;
-  out->stride = c->nbr_stride;
   out->cap = c->nbr_cap;
   return 0;
 }
@@ -438,14 +443,14 @@ static int enqueue_iteration(sphx_context* c, const KP& k, int respect_done) {
   IterCtl ctl{c->O.min_iters, c->O.max_iters, c->O.tol};
   timer_begin(c, "iisph_dp");
   k_ii_dp<<<nb, NB_THREADS, 0, c->stream>>>(k, L, c->solve_state.p, c->key.p, c->cell_start.p,
-                                            c->POS(), c->sfac.p, c->PHYS(), c->dii.p,
+                                            c->POS(), c->ps.p, c->PHYS(), c->dii.p,
                                             c->press_l.p, c->dijpj.p, c->qm.p);
   timer_end(c);
   timer_begin(c, "iisph_iter");
   k_ii_iter<<<nb, NB_THREADS, 0, c->stream>>>(k, L, c->solve_state.p, c->key.p, c->cell_start.p,
                                               c->POS(), c->PHYS(), c->dii.p, c->dijpj.p, c->qm.p,
                                               c->rho_adv.p, c->press_l.p, c->press_l1.p,
-                                              c->sfac.p, c->residue.p, c->ACCEL(), c->red.p);
+                                              c->ps.p, c->residue.p, c->ACCEL(), c->red.p);
   timer_end(c);
   k_ii_finalize<<<1, 1024, 0, c->stream>>>(c->solve_state.p, c->solve_err.p, c->red.p, nb, k.n,
                                            ctl, respect_done);
@@ -509,7 +514,7 @@ int sphx_iisph_predict_aii(sphx_context* c) {
   timer_begin(c, "iisph_aii");
   k_ii_aii<<<blocks_for(k.n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
       k, L, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p, c->rho_adv.p,
-      c->press_l.p, c->sfac.p);
+      c->press_l.p, c->ps.p);
   timer_end(c);
   SPHX_HIP(hipGetLastError());
   return 0;
@@ -639,11 +644,20 @@ int sphx_upload_iisph(sphx_context* c, const float* dii, const float* dijpj, con
   if (press_l) {
     SPHX_HIP(hipMemcpy(c->press_l.p, press_l, n * sizeof(float), hipMemcpyHostToDevice));
     // keep s_j = m_j p_j / rho_j^2 consistent with the uploaded pressures
-    std::vector<float4> ph(n);
-    std::vector<float> s(n);
+    NbrList L;
+    SPHX_TRY(nbr_list(c, &L));
+    SPHX_HIP(hipStreamSynchronize(c->stream));
+    std::vector<float4> ph(n), s(n);
     SPHX_HIP(hipMemcpy(ph.data(), c->PHYS(), n * sizeof(float4), hipMemcpyDeviceToHost));
-    for (int i = 0; i < n; ++i) s[i] = ph[i].x * press_l[i] / ph[i].y / ph[i].y;
-    SPHX_HIP(hipMemcpy(c->sfac.p, s.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    SPHX_HIP(hipMemcpy(s.data(), c->ps.p, n * sizeof(float4), hipMemcpyDeviceToHost));
+    std::vector<float4> pos(n);
+    SPHX_HIP(hipMemcpy(pos.data(), c->POS(), n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+      int t;
+      std::memcpy(&t, &pos[i].w, 4);
+      s[i].w = t == SPHX_TYPE_BOUNDARY ? 0.f : ph[i].x * press_l[i] / ph[i].y / ph[i].y;
+    }
+    SPHX_HIP(hipMemcpy(c->ps.p, s.data(), n * sizeof(float4), hipMemcpyHostToDevice));
   }
   return 0;
 }

@@ -43,7 +43,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_drift(
     const float4* __restrict__ phys, const float4* __restrict__ alpha,
     const float4* __restrict__ mforce, float4* __restrict__ drift0, float4* __restrict__ drift1,
     float4* __restrict__ drift2) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
   uint32_t gc = key[i];
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_alpha(
     const float4* __restrict__ phys, const float4* __restrict__ alpha,
     const float4* __restrict__ drift0, const float4* __restrict__ drift1,
     const float4* __restrict__ drift2, float4* __restrict__ alpha_new) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
   uint32_t gc = key[i];
@@ -174,7 +174,7 @@ __global__ void k_mf_correction(KP k, MFP m, const float4* __restrict__ pos,
                                 const uint32_t* __restrict__ key,
                                 const float4* __restrict__ alpha_pre,
                                 float4* __restrict__ alpha_new, float4* __restrict__ vel) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i >= k.n) return;
   if (type_of(pos[i]) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) return;
   float4 a = alpha_new[i];
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_force(
     const float4* __restrict__ alpha, const float4* __restrict__ alpha_new,
     const float4* __restrict__ drift0, const float4* __restrict__ drift1,
     const float4* __restrict__ drift2, float4* __restrict__ veleval, float4* __restrict__ amf) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
   uint32_t gc = key[i];

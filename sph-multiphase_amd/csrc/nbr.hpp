@@ -25,18 +25,25 @@ __device__ __forceinline__ void for_each_neighbor(const KP& k, const int* __rest
   }
 }
 
-// Per-step neighbour list in ELL layout: entry q of particle i is
-// idx[q * stride + i] (consecutive particles read consecutive words), in the
-// reference's visiting order, holding every j != i with |x_ij| < 2h (the
-// widest support any kernel of the step uses).  Positions do not move
-// between the density pass and the last pressure iteration, so one list
-// serves every neighbour loop of an IISPH step.  A particle with more than
-// `cap` neighbours keeps its true count and its loops fall back to the cell
-// scan, which visits a superset in the same order.
+// Per-step neighbour list in sliced-ELL layout: particles are grouped in
+// slices of 256; entry q of particle i is idx[(slice * cap + q) * 256 +
+// (i & 255)], so a wave reading entry q of its 64 particles touches one
+// 256-byte run and a slice's whole list is one contiguous cap KiB block.
+// Entries follow the reference's visiting order and hold every j != i with
+// |x_ij| < 2h (the widest support any kernel of the step uses).  Positions
+// do not move between the density pass and the last pressure iteration, so
+// one list serves every neighbour loop of an IISPH step.  A particle with
+// more than `cap` neighbours keeps its true count and its loops fall back
+// to the cell scan, which visits a superset in the same order.
+constexpr int NBR_SLICE = 256;
+
+__host__ __device__ __forceinline__ size_t nbr_slot(int i, int q, int cap) {
+  return ((size_t)(i / NBR_SLICE) * cap + q) * NBR_SLICE + (i % NBR_SLICE);
+}
+
 struct NbrList {
   const int* __restrict__ idx;
   const int* __restrict__ cnt;
-  int stride;
   int cap;
 };
 
@@ -46,8 +53,9 @@ __device__ __forceinline__ void for_each_listed(const KP& k, const NbrList& L,
                                                 int i, F&& f) {
   const int c = L.cnt[i];
   if (c <= L.cap) {
-#pragma unroll 2
-    for (int q = 0; q < c; ++q) f(L.idx[(size_t)q * L.stride + i]);
+    const int* __restrict__ p = L.idx + nbr_slot(i, 0, L.cap);
+#pragma unroll 4
+    for (int q = 0; q < c; ++q) f(p[(size_t)q * NBR_SLICE]);
   } else {
     for_each_neighbor(k, cell_start, cell, [&](int j) {
       if (j != i) f(j);

@@ -20,7 +20,7 @@ __device__ __forceinline__ float poly6_c3(float r2, float dsq) {
 __global__ void k_boundary_volume(KP k, const uint32_t* __restrict__ key,
                                   const int* __restrict__ cs, const float4* __restrict__ pos,
                                   float4* __restrict__ phys) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
   if (type_of(pi) != SPHX_TYPE_BOUNDARY) return;
@@ -48,7 +48,7 @@ __global__ void k_boundary_volume(KP k, const uint32_t* __restrict__ key,
 __global__ void k_density_pressure(KP k, const uint32_t* __restrict__ key,
                                    const int* __restrict__ cs, const float4* __restrict__ pos,
                                    float4* __restrict__ phys) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
   if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
@@ -94,7 +94,7 @@ __global__ void k_density_pressure(KP k, const uint32_t* __restrict__ key,
 __global__ void k_force(KP k, const uint32_t* __restrict__ key, const int* __restrict__ cs,
                         const float4* __restrict__ pos, const float4* __restrict__ vel,
                         const float4* __restrict__ phys, float4* __restrict__ accel) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
   if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
@@ -160,7 +160,7 @@ __global__ void k_force(KP k, const uint32_t* __restrict__ key, const int* __res
 __global__ void k_advance(KP k, const uint32_t* __restrict__ key, float4* __restrict__ pos,
                           float4* __restrict__ vel, float4* __restrict__ veleval,
                           const float4* __restrict__ accel) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_index();
   if (i >= k.n) return;
   float4 p = pos[i];
   if (type_of(p) == SPHX_TYPE_BOUNDARY) return;
