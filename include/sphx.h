@@ -182,9 +182,10 @@ typedef struct sphx_SolverOptions {
   float rho_ref;            /* reference density of the mixture solve */
   int clamp_pressure;       /* 1: p = max(p, 0) after each iteration (the
                                clamp the reference has commented out at
-                               kern.cu:1035-1036) and the density error
-                               counts compression only, max(rho-rho0,0)/rho0
-                               (DFSPH-style); 0: reference, |rho-rho0|/rho0 */
+                               kern.cu:1035-1036); the density error is then
+                               the complementarity residual: |rho-rho0|/rho0
+                               where the iterate's p > 0, max(rho-rho0,0)/rho0
+                               where p = 0.  0: reference, |rho-rho0|/rho0 */
   int reserved[7];
 } sphx_SolverOptions;
 
@@ -340,6 +341,56 @@ void MfComputeDriftVelCUDA(void);
 void MfComputeAlphaAdvanceCUDA(void);
 void MfComputeCorrectionCUDA(void);
 void ComputeForceCUDA_ProjectU(void);
+
+/* ------------------------------------------------------------------------ */
+/* Learned warm start and drift closure.  No counterpart in the reference:  */
+/* a 24-64-64-4 ReLU MLP (6,020 weights; layout W1[24][64] b1[64]           */
+/* W2[64][64] b2[64] W3[64][4] b3[4]) evaluated per fluid particle after    */
+/* the a_ii pass.  Output 0 replaces the warm start p0 = 0.5 p(t-1) of      */
+/* fluid_system_kern.cu:899 by p0 = max(0, p(t-1) + p_scale*y0/10); outputs */
+/* 1-3 predict the change of the phase-1 drift (multiphase.cu:131-163)      */
+/* under the pressure update and are added to the next step's drift.  A     */
+/* solve from the learned start that misses options.tol within max_iters is */
+/* redone from the plain start (iterations of both are reported).           */
+/* ------------------------------------------------------------------------ */
+int sphx_mlp_nparam(void);
+/* He-initialised hidden layers, zero output layer (predicts no change);
+ * p_scale (Pa) and u_scale (m/s) normalise pressures and drift. */
+int sphx_mlp_init(sphx_context* ctx, uint64_t seed, float p_scale, float u_scale);
+int sphx_mlp_set_weights(sphx_context* ctx, const float* w, int n);
+int sphx_mlp_get_weights(sphx_context* ctx, float* w, int n);
+/* Flags: learned warm start, learned drift correction, sample recording
+ * (drift targets are recorded only while the drift correction is off). */
+int sphx_mlp_mode(sphx_context* ctx, int warm, int drift, int record);
+/* Online refresh: every `every` solves (0 = off) run `iters` Adam steps of
+ * `batch` samples drawn on the device from the sample store, which then
+ * records every solve as a ring; lw[4] weights the outputs' errors. */
+int sphx_mlp_online(sphx_context* ctx, int every, int iters, int batch, float lr,
+                    const float* lw);
+int sphx_mlp_set_drift_blend(sphx_context* ctx, float blend);
+/* Sample store: one sample per particle slot and solve (masked when the
+ * slot is boundary or out of the grid). */
+int sphx_mlp_reserve(sphx_context* ctx, int64_t max_samples);
+int sphx_mlp_clear_samples(sphx_context* ctx);
+int sphx_mlp_samples(sphx_context* ctx, int64_t* total, int64_t* valid);
+int sphx_mlp_download_samples(sphx_context* ctx, int64_t start, int64_t count,
+                              float* X, float* T, float* M);
+/* Features [num][24] of the current state (after the a_ii pass), the
+ * previous pressures [num] and each particle's pressure increment of the
+ * last solve [num] (feature 20); any may be NULL. */
+int sphx_mlp_features(sphx_context* ctx, float* X, float* pprev, float* dp_last);
+/* The network with FP16 weights and FP32 accumulation: Y [n][4]. */
+int sphx_mlp_eval(sphx_context* ctx, const float* X, int n, float* Y);
+/* Loss (1/n_valid) sum M_s sum_o lw_o (y_so - T_so)^2 and its gradient
+ * (FP32 master weights) by the training kernels; M may be NULL. */
+int sphx_mlp_grad(sphx_context* ctx, const float* X, const float* T, const float* M,
+                  int n, const float* lw, float* grad, double* loss);
+/* Adam (beta 0.9/0.999) over the recorded valid samples; loss_out[epochs]
+ * may be NULL. */
+int sphx_mlp_train(sphx_context* ctx, int epochs, int batch, float lr, const float* lw,
+                   uint64_t seed, double* loss_out);
+int sphx_mlp_stats(sphx_context* ctx, int64_t* learned, int64_t* fallbacks,
+                   int* last_fallback);
 
 /* ------------------------------------------------------------------------ */
 /* Kernel timing (HIP events on the context stream).                         */

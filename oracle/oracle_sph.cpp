@@ -617,17 +617,23 @@ void ii_iteration(const sphx_ParamCarrier& P, const sphx_SolverOptions& O, OrcSt
                   double* mean_signed, double* mean_abs_rel) {
   ii_dp<T>(P, O, s);
   ii_iter<T>(P, O, s);
-  std::memcpy(s.press_l, s.press_l1, sizeof(T) * s.n);
   double sum = 0, ab = 0;
   long cnt = 0;
   for (int i = 0; i < s.n; ++i) {
     double r = (double)at<T>(s.residue, i);
     sum += r;
     if (s.type[i] != SPHX_TYPE_BOUNDARY && s.mgcell[i] != UNDEF) {
-      ab += (O.clamp_pressure ? std::max(r, 0.0) : std::fabs(r)) / (double)at<T>(s.restdens, i);
+      // with the clamp the solve is a complementarity problem (p > 0 and
+      // rho = rho0, or p = 0 and rho <= rho0): error |r| where the residual's
+      // pressure p_l is positive, compression only where it is zero
+      double e = O.clamp_pressure ? ((double)at<T>(s.press_l, i) > 0.0 ? std::fabs(r)
+                                                                      : std::max(r, 0.0))
+                                  : std::fabs(r);
+      ab += e / (double)at<T>(s.restdens, i);
       ++cnt;
     }
   }
+  std::memcpy(s.press_l, s.press_l1, sizeof(T) * s.n);
   if (mean_signed) *mean_signed = s.n ? sum / s.n : 0.0;
   if (mean_abs_rel) *mean_abs_rel = cnt ? ab / cnt : 0.0;
 }

@@ -211,3 +211,53 @@ def relerr(a, b, ref=None):
     den = np.max(np.abs(r)) if r.size else 0.0
     num = np.max(np.abs(a - b)) if a.size else 0.0
     return num / den if den > 0 else num
+
+
+# -- learned warm start (FP64 network; features from an FP32 state) --------
+MLP_NF, MLP_H, MLP_NO = 24, 64, 4
+MLP_NP = MLP_NF * MLP_H + MLP_H + MLP_H * MLP_H + MLP_H + MLP_H * MLP_NO + MLP_NO
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def mlp_features(st: State, alpha4=None, p_scale=1.0, dp_last=None):
+    """24 features per particle of the FP32 state `st` after its a_ii pass;
+    alpha4 [n][4] = (alpha_0..2, rho_m) or None for one phase; dp_last [n]
+    the pressure increment of each particle's last solve or None."""
+    assert st.dtype == np.float32
+    X = np.zeros((st.n, MLP_NF), np.float64)
+    fp = C.POINTER(C.c_float)
+    a = None if alpha4 is None else np.ascontiguousarray(alpha4, np.float32)
+    d = None if dp_last is None else np.ascontiguousarray(dp_last, np.float32)
+    lib().orc_mlp_features(C.byref(st.params), C.byref(st.s),
+                           None if a is None else a.ctypes.data_as(fp),
+                           None if d is None else d.ctypes.data_as(fp),
+                           C.c_double(p_scale), _dp(X))
+    return X
+
+
+def mlp_forward(w, X):
+    """Y [n][4], H1, H2 [n][64] (post-ReLU) in FP64."""
+    w = np.ascontiguousarray(w, np.float64)
+    X = np.ascontiguousarray(X, np.float64)
+    n = len(X)
+    Y = np.zeros((n, MLP_NO))
+    H1 = np.zeros((n, MLP_H))
+    H2 = np.zeros((n, MLP_H))
+    lib().orc_mlp_forward(_dp(w), _dp(X), C.c_int(n), _dp(Y), _dp(H1), _dp(H2))
+    return Y, H1, H2
+
+
+def mlp_loss_grad(w, X, T, lw):
+    """Loss (1/n) sum lw_o (y_so - T_so)^2 and its gradient (FP64)."""
+    w = np.ascontiguousarray(w, np.float64)
+    X = np.ascontiguousarray(X, np.float64)
+    T = np.ascontiguousarray(T, np.float64)
+    lw = np.ascontiguousarray(lw, np.float64)
+    g = np.zeros(MLP_NP)
+    L = lib()
+    L.orc_mlp_loss_grad.restype = C.c_double
+    loss = L.orc_mlp_loss_grad(_dp(w), _dp(X), _dp(T), C.c_int(len(X)), _dp(lw), _dp(g))
+    return loss, g

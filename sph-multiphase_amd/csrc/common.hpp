@@ -188,6 +188,34 @@ struct sphx_context {
   sphx::DevBuf<float4> mf_alpha_new, mf_drift[3], mf_amf;
   bool mf_pending = false;        // alpha_new holds the corrected fractions
 
+  // learned warm start / drift closure (24-64-64-4 MLP)
+  struct MLP {
+    bool ready = false;
+    int warm = 0, drift = 0, record = 0;
+    float p_scale = 1.f, u_scale = 1.f, drift_blend = 1.f;
+    int adam_t = 0;
+    size_t nsamples = 0, max_samples = 0, wpos = 0;
+    // online refresh: every `online_every` solves, `online_iters` Adam steps
+    // on batches drawn from the sample ring
+    int online_every = 0, online_iters = 0, online_batch = 0, solves = 0;
+    float online_lr = 0.f, online_lw[4] = {1.f, 1.f, 1.f, 1.f};
+    uint64_t online_seed = 0;
+    // per particle, permuted with the particles: {drift correction xyz,
+    // pressure increment of the last solve}
+    sphx::DevBuf<float4> hist[2];
+    long long learned_solves = 0, fallbacks = 0;
+    int last_fallback = 0;
+    sphx::DevBuf<float> w, m, v, g;
+    sphx::DevBuf<uint16_t> w16;         // FP16 bits of w
+    sphx::DevBuf<float> X, pprev;       // per slot, current step
+    sphx::DevBuf<float4> unew;          // drift with the converged pressure
+    sphx::DevBuf<float> sX, sT, sM;     // recorded samples
+    sphx::DevBuf<float> H1, dH1, H2, dH2, dY;
+    sphx::DevBuf<int> order;
+    sphx::DevBuf<double> loss;
+    int batch_cap = 0;
+  } mlp;
+
   // timing
   bool timing = false;
   sphx::Timer timer;

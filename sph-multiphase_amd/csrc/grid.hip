@@ -293,6 +293,7 @@ __global__ void k_fill_i32(int* p, int n, int v) {
 }
 
 int mf_reorder(sphx_context* c, int a, int b);
+int mlp_reorder(sphx_context* c, int a, int b);
 
 int grid_hash(sphx_context* c) {
   int n = c->num;
@@ -373,6 +374,7 @@ int grid_reorder(sphx_context* c) {
   timer_end(c);
   SPHX_HIP(hipGetLastError());
   SPHX_TRY(mf_reorder(c, a, b));
+  SPHX_TRY(mlp_reorder(c, a, b));
   c->cur = b;
   return 0;
 }

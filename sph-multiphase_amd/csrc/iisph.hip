@@ -270,10 +270,13 @@ __global__ __launch_bounds__(NB_THREADS) void k_ii_iter(
       press_l1[i] = pn;
       ps[i].w = mi * pn / rhoi / rhoi;
       rs = res;
-      // NaN must propagate into the sum (fmax would drop it and report
-      // convergence), so the compression-only error is written as a branch
+      // With the clamp the solve is a complementarity problem (p > 0 and
+      // rho = rho0, or p = 0 and rho <= rho0): the error is |res| where the
+      // residual's pressure p_l is positive, compression only where it is
+      // zero.  NaN must propagate into the sum (fmax would drop it and
+      // report convergence), so the one-sided part is written as a branch.
       double rr = (double)res;
-      ra = (k.clamp_p ? (rr < 0.0 ? 0.0 : rr) : fabs(rr)) / (double)rest;
+      ra = (k.clamp_p && !(pl > 0.f) ? (rr < 0.0 ? 0.0 : rr) : fabs(rr)) / (double)rest;
       rc = 1.0;
     }
   }
@@ -386,6 +389,7 @@ __global__ void k_nbr_build(KP k, const uint32_t* __restrict__ key, const int* _
 int grid_hash(sphx_context* c);
 int grid_sort(sphx_context* c);
 int grid_reorder(sphx_context* c);
+int mlp_solve(sphx_context* c, int* iters, double* err);
 
 static int need_sorted(sphx_context* c, const char* who) {
   SPHX_CHECK(c, who, "null context");
@@ -542,6 +546,16 @@ int sphx_iisph_pressure_iteration(sphx_context* c, double* mean_signed, double* 
 
 int sphx_iisph_pressure_solve(sphx_context* c, int* iters, double* err) {
   SPHX_TRY(need_sorted(c, "sphx_iisph_pressure_solve"));
+  return mlp_solve(c, iters, err);
+}
+
+}  // extern "C"
+
+namespace sphx {
+
+// PressureSolve (fluid_system_host.cu:525-552) from the start the a_ii pass
+// (or the learned start) left in press_l.
+int iisph_solve_plain(sphx_context* c, int* iters, double* err) {
   SPHX_TRY(ensure_red(c));
   KP k = make_kp(c);
   if (iters) *iters = 0;
@@ -567,6 +581,10 @@ int sphx_iisph_pressure_solve(sphx_context* c, int* iters, double* err) {
   if (err) *err = c->host_pinned[0];
   return timer_flush(c);
 }
+
+}  // namespace sphx
+
+extern "C" {
 
 int sphx_iisph_integrate(sphx_context* c) {
   SPHX_TRY(need_sorted(c, "sphx_iisph_integrate"));

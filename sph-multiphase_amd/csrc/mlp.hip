@@ -1,0 +1,996 @@
+// Learned warm start and drift closure: a 24-64-64-4 MLP (6,020 weights)
+// evaluated per fluid particle after the a_ii pass.  Output 0 predicts the
+// pressure increment over the previous step, p0 = max(0, p(t-1) + P*y0/10),
+// replacing the warm start p0 = 0.5 p(t-1) of fluid_system_kern.cu:899;
+// outputs 1-3 predict the change of the dispersed phase's drift velocity
+// when it is evaluated with the converged instead of the previous pressure
+// (the lag of multiphase.cu:62, which reads the last step's pressure), and
+// are added to that drift at the next step.  The plain solve stays the
+// fallback: a learned start that does not reach the tolerance is discarded
+// and the step is solved again from the reference warm start.
+//
+// Training is on the GPU with these kernels: the solver records (features,
+// converged increments) pairs, k_mlp_fwdbwd runs forward + backward per
+// sample (FP32), k_mlp_wgrad reduces the weight gradients over the batch
+// in LDS tiles, k_mlp_adam applies Adam and refreshes the FP16 copy the
+// inference kernel reads (FP16 weights, FP32 accumulation).
+#include "nbr.hpp"
+
+#include <hip/hip_fp16.h>
+
+#include <cmath>
+#include <random>
+
+namespace sphx {
+
+constexpr int MLP_NF = 24, MLP_H = 64, MLP_NO = 4;
+constexpr int MLP_W1 = 0, MLP_B1 = MLP_NF * MLP_H, MLP_W2 = MLP_B1 + MLP_H,
+              MLP_B2 = MLP_W2 + MLP_H * MLP_H, MLP_W3 = MLP_B2 + MLP_H,
+              MLP_B3 = MLP_W3 + MLP_H * MLP_NO, MLP_NP = MLP_B3 + MLP_NO;   // 6020
+constexpr float MLP_TSCALE = 10.f;   // target = (p* - p) / P * 10
+
+struct MLPP {
+  float p_scale;      // P (Pa)
+  float u_scale;      // drift normalisation (m/s)
+  float drift_blend;  // weight of the drift correction applied next step
+};
+
+// ------------------------------------------------------------- features --
+// Same map as the oracle's orc_mlp_features (24 values per fluid particle).
+__global__ __launch_bounds__(256) void k_mlp_features(
+    KP k, NbrList L, MLPP mp, const uint32_t* __restrict__ key, const int* __restrict__ cs,
+    const float4* __restrict__ pos, const float4* __restrict__ veleval,
+    const float4* __restrict__ phys, const float4* __restrict__ dii,
+    const float* __restrict__ rho_adv, const float4* __restrict__ alpha,
+    const float4* __restrict__ hist, float* __restrict__ X, float* __restrict__ pprev) {
+  int i = xcd_index();
+  if (i >= k.n) return;
+  float* x = X + (size_t)i * MLP_NF;
+  float4 pi = pos[i];
+  uint32_t gc = key[i];
+  pprev[i] = phys[i].z;
+  if (type_of(pi) == SPHX_TYPE_BOUNDARY || (int)gc == k.gridtotal) {
+    for (int q = 0; q < MLP_NF; ++q) x[q] = 0.f;
+    return;
+  }
+  const float4 qi = phys[i];
+  const float rho0 = qi.w, rho = qi.y, radv = rho_adv[i];
+  const float4 di = dii[i];
+  const float aii = di.w;
+  const float p_i = fmaxf(0.f, qi.z);
+  const float gn = sqrtf(k.g.x * k.g.x + k.g.y * k.g.y + k.g.z * k.g.z);
+  const float3 gh = gn > 0.f ? f3(k.g.x / gn, k.g.y / gn, k.g.z / gn) : f3(0.f, -1.f, 0.f);
+  float wsum = 0.f, wp = 0.f, pmax = p_i, pmin = p_i, wsrc = 0.f, gpx = 0.f, gpy = 0.f,
+        gpz = 0.f;
+  int nf = 0, nb = 0;
+  for_each_listed(k, L, cs, gc, i, [&](int j) {
+    float4 pj = pos[j];
+    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+    float dsq = dx * dx + dy * dy + dz * dz;
+    float r = sqrtf(dsq);
+    if (!(r < 2.f * k.h && r > 0.f)) return;
+    if (type_of(pj) == SPHX_TYPE_BOUNDARY) { ++nb; return; }
+    ++nf;
+    float4 qj = phys[j];
+    float w = cubic_w(r, k.inv_h, k.kspline);
+    float pjp = fmaxf(0.f, qj.z);
+    wsum += w;
+    wp += w * pjp;
+    pmax = fmaxf(pmax, pjp);
+    pmin = fminf(pmin, pjp);
+    wsrc += w * (rho_adv[j] - qj.w) / qj.w;
+    float v = qj.x / qj.y * cubic_gradf(r, 1.f / r, k.inv_h, k.kspline) * (pjp - p_i);
+    gpx += v * dx; gpy += v * dy; gpz += v * dz;
+  });
+  const float P = mp.p_scale;
+  float pav = wsum > 0.f ? wp / wsum : p_i;
+  float pj_est = fabsf(aii) > 1e-12f ? fmaxf(0.f, (rho0 - radv) / aii) : 0.f;
+  x[0] = p_i / P;
+  x[1] = (radv - rho0) / rho0 * 100.f;
+  x[2] = (rho - rho0) / rho0 * 100.f;
+  x[3] = -aii * P / rho0;
+  x[4] = pj_est / P;
+  x[5] = pav / P;
+  x[6] = pmax / P;
+  x[7] = pmin / P;
+  x[8] = (pav - p_i) / P;
+  x[9] = nf / 64.f;
+  x[10] = nb / 64.f;
+  float4 al = alpha ? alpha[i] : make_float4(1.f, 0.f, 0.f, rho0);
+  x[11] = alpha ? al.y : 0.f;
+  x[12] = alpha ? al.w / rho0 : 1.f;
+  x[13] = di.x * P / k.h; x[14] = di.y * P / k.h; x[15] = di.z * P / k.h;
+  float4 v = veleval[i];
+  x[16] = v.x * k.dt / k.h; x[17] = v.y * k.dt / k.h; x[18] = v.z * k.dt / k.h;
+  float den = rho0 * (gn > 0.f ? gn : 1.f);
+  x[19] = (gpx * gh.x + gpy * gh.y + gpz * gh.z) / den;
+  x[20] = hist[i].w / P * MLP_TSCALE;   // pressure increment of the last solve
+  x[21] = gpz / den;
+  x[22] = wsum > 0.f ? wsrc / wsum * 100.f : 0.f;
+  x[23] = 1.f;
+}
+
+// ------------------------------------------------------------ inference --
+// FP16 weights staged in LDS, FP32 accumulation.  Loops run over output
+// units innermost so each LDS read fetches consecutive weights.
+// Layer 1 is unrolled over inputs and units; layers 2 and 3 are fused in
+// chunks of 8 hidden units, so no register array is indexed dynamically.
+constexpr int MLP_CH = 8;
+
+__device__ __forceinline__ void mlp_forward16(const __half* __restrict__ sw,
+                                              const float* __restrict__ xin, float* __restrict__ y) {
+  float h1[MLP_H];
+#pragma unroll
+  for (int a = 0; a < MLP_H; ++a) h1[a] = __half2float(sw[MLP_B1 + a]);
+#pragma unroll
+  for (int q = 0; q < MLP_NF; ++q) {
+    const float xq = xin[q];
+#pragma unroll
+    for (int a = 0; a < MLP_H; ++a) h1[a] = fmaf(xq, __half2float(sw[MLP_W1 + q * MLP_H + a]), h1[a]);
+  }
+#pragma unroll
+  for (int a = 0; a < MLP_H; ++a) h1[a] = fmaxf(h1[a], 0.f);
+#pragma unroll
+  for (int o = 0; o < MLP_NO; ++o) y[o] = __half2float(sw[MLP_B3 + o]);
+#pragma unroll 1
+  for (int bc = 0; bc < MLP_H; bc += MLP_CH) {
+    float acc[MLP_CH];
+#pragma unroll
+    for (int j = 0; j < MLP_CH; ++j) acc[j] = __half2float(sw[MLP_B2 + bc + j]);
+#pragma unroll
+    for (int a = 0; a < MLP_H; ++a) {
+      const __half* wr = sw + MLP_W2 + a * MLP_H + bc;
+#pragma unroll
+      for (int j = 0; j < MLP_CH; ++j) acc[j] = fmaf(h1[a], __half2float(wr[j]), acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < MLP_CH; ++j) {
+      const float hb = fmaxf(acc[j], 0.f);
+#pragma unroll
+      for (int o = 0; o < MLP_NO; ++o)
+        y[o] = fmaf(hb, __half2float(sw[MLP_W3 + (bc + j) * MLP_NO + o]), y[o]);
+    }
+  }
+}
+
+__device__ __forceinline__ void stage_w16(__half* sw, const __half* __restrict__ w16) {
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(w16);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(sw);
+  for (int t = threadIdx.x; t < MLP_NP / 2; t += blockDim.x) dst[t] = src[t];
+}
+
+// Writes the warm start p0 = max(0, p(t-1) + P y0 / 10) (and s = m p0 /
+// rho^2 for the pressure pass) when `warm`, and the drift correction for
+// the next step when dcorr is non-null.
+__global__ __launch_bounds__(256) void k_mlp_infer(
+    KP k, MLPP mp, int warm, const float* __restrict__ X, const __half* __restrict__ w16,
+    const float4* __restrict__ pos, const uint32_t* __restrict__ key,
+    const float4* __restrict__ phys, float* __restrict__ press_l, float4* __restrict__ ps,
+    float4* __restrict__ dcorr) {
+  __shared__ __align__(16) __half sw[MLP_NP];
+  stage_w16(sw, w16);
+  __syncthreads();
+  int i = xcd_index();
+  if (i >= k.n) return;
+  float4 pi = pos[i];
+  if (type_of(pi) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) {
+    if (dcorr) dcorr[i] = make_float4(0.f, 0.f, 0.f, dcorr[i].w);
+    return;
+  }
+  const float4* xv = reinterpret_cast<const float4*>(X + (size_t)i * MLP_NF);
+  float xin[MLP_NF];
+#pragma unroll
+  for (int q = 0; q < MLP_NF / 4; ++q) {
+    float4 t = xv[q];
+    xin[4 * q] = t.x; xin[4 * q + 1] = t.y; xin[4 * q + 2] = t.z; xin[4 * q + 3] = t.w;
+  }
+  float y[MLP_NO];
+  mlp_forward16(sw, xin, y);
+  if (warm) {
+    const float4 qi = phys[i];
+    float p0 = fmaxf(0.f, fmaxf(0.f, qi.z) + y[0] * (mp.p_scale / MLP_TSCALE));
+    if (!(p0 == p0)) p0 = k.warm * qi.z;   // NaN from the network: plain start
+    press_l[i] = p0;
+    ps[i].w = qi.x * p0 / (qi.y * qi.y);
+  }
+  if (dcorr) {
+    float s = mp.drift_blend * mp.u_scale / MLP_TSCALE;
+    float4 d = make_float4(y[1] * s, y[2] * s, y[3] * s, dcorr[i].w);
+    if (!(d.x == d.x && d.y == d.y && d.z == d.z)) d = make_float4(0.f, 0.f, 0.f, d.w);
+    dcorr[i] = d;
+  }
+}
+
+// The network alone on given features (n x 24) -> Y (n x 4).
+__global__ __launch_bounds__(256) void k_mlp_eval(const float* __restrict__ X, int n,
+                                                  const __half* __restrict__ w16,
+                                                  float* __restrict__ Y) {
+  __shared__ __align__(16) __half sw[MLP_NP];
+  stage_w16(sw, w16);
+  __syncthreads();
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float xin[MLP_NF];
+  for (int q = 0; q < MLP_NF; ++q) xin[q] = X[(size_t)i * MLP_NF + q];
+  float y[MLP_NO];
+  mlp_forward16(sw, xin, y);
+  for (int o = 0; o < MLP_NO; ++o) Y[(size_t)i * MLP_NO + o] = y[o];
+}
+
+// Plain warm start p0 = warm * p(t-1) (fluid_system_kern.cu:899), used when
+// a solve from the learned start does not converge.
+__global__ void k_mlp_plain_start(KP k, const float4* __restrict__ pos,
+                                  const uint32_t* __restrict__ key, float4* __restrict__ phys,
+                                  const float* __restrict__ pprev, float* __restrict__ press_l,
+                                  float4* __restrict__ ps) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.n) return;
+  if (type_of(pos[i]) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) return;
+  float4 q = phys[i];
+  q.z = pprev[i];
+  phys[i] = q;
+  float p0 = k.warm * q.z;
+  press_l[i] = p0;
+  ps[i].w = q.x * p0 / (q.y * q.y);
+}
+
+// ------------------------------------------------------------- training --
+// One thread per sample: forward, loss, and the backward pass down to the
+// layer inputs.  Stores H1, dH1, H2, dH2, dY (batch-local rows) for the
+// weight gradients.  Same chunking as the inference to stay in registers.
+__global__ __launch_bounds__(256) void k_mlp_fwdbwd(
+    const float* __restrict__ Xall, const float* __restrict__ Tall, const float* __restrict__ Mall,
+    const int* __restrict__ perm, int base, int nb, int nvalid_total, const float* __restrict__ w,
+    float4 lw, float* __restrict__ H1, float* __restrict__ dH1, float* __restrict__ H2,
+    float* __restrict__ dH2, float* __restrict__ dY, double* __restrict__ loss) {
+  __shared__ float sw[MLP_NP];
+  __shared__ double sl[4];
+  for (int t = threadIdx.x; t < MLP_NP; t += blockDim.x) sw[t] = w[t];
+  if (threadIdx.x < 4) sl[threadIdx.x] = 0.0;
+  __syncthreads();
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  double my_loss = 0.0;
+  if (s < nb) {
+    const int src = perm[base + s];
+    const float4* xv = reinterpret_cast<const float4*>(Xall + (size_t)src * MLP_NF);
+    float xin[MLP_NF];
+#pragma unroll
+    for (int q = 0; q < MLP_NF / 4; ++q) {
+      float4 t = xv[q];
+      xin[4 * q] = t.x; xin[4 * q + 1] = t.y; xin[4 * q + 2] = t.z; xin[4 * q + 3] = t.w;
+    }
+    const float4 tv = reinterpret_cast<const float4*>(Tall)[src];
+    const float msk = Mall[src];
+    float h1[MLP_H];
+#pragma unroll
+    for (int a = 0; a < MLP_H; ++a) h1[a] = sw[MLP_B1 + a];
+#pragma unroll
+    for (int q = 0; q < MLP_NF; ++q) {
+#pragma unroll
+      for (int a = 0; a < MLP_H; ++a) h1[a] = fmaf(xin[q], sw[MLP_W1 + q * MLP_H + a], h1[a]);
+    }
+    float4* h1o = reinterpret_cast<float4*>(H1 + (size_t)s * MLP_H);
+#pragma unroll
+    for (int a = 0; a < MLP_H; ++a) h1[a] = fmaxf(h1[a], 0.f);
+#pragma unroll
+    for (int a = 0; a < MLP_H / 4; ++a)
+      h1o[a] = make_float4(h1[4 * a], h1[4 * a + 1], h1[4 * a + 2], h1[4 * a + 3]);
+    float y[MLP_NO];
+#pragma unroll
+    for (int o = 0; o < MLP_NO; ++o) y[o] = sw[MLP_B3 + o];
+    float* h2o = H2 + (size_t)s * MLP_H;
+#pragma unroll 1
+    for (int bc = 0; bc < MLP_H; bc += MLP_CH) {
+      float acc[MLP_CH];
+#pragma unroll
+      for (int j = 0; j < MLP_CH; ++j) acc[j] = sw[MLP_B2 + bc + j];
+#pragma unroll
+      for (int a = 0; a < MLP_H; ++a) {
+#pragma unroll
+        for (int j = 0; j < MLP_CH; ++j)
+          acc[j] = fmaf(h1[a], sw[MLP_W2 + a * MLP_H + bc + j], acc[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < MLP_CH; ++j) {
+        const float hb = fmaxf(acc[j], 0.f);
+        h2o[bc + j] = hb;
+#pragma unroll
+        for (int o = 0; o < MLP_NO; ++o) y[o] = fmaf(hb, sw[MLP_W3 + (bc + j) * MLP_NO + o], y[o]);
+      }
+    }
+    const float tt[4] = {tv.x, tv.y, tv.z, tv.w};
+    const float lwv[4] = {lw.x, lw.y, lw.z, lw.w};
+    float dy[MLP_NO];
+#pragma unroll
+    for (int o = 0; o < MLP_NO; ++o) {
+      float e = y[o] - tt[o];
+      my_loss += (double)msk * lwv[o] * e * e;
+      dy[o] = 2.f * msk * lwv[o] * e / (float)nvalid_total;
+    }
+    reinterpret_cast<float4*>(dY)[s] = make_float4(dy[0], dy[1], dy[2], dy[3]);
+    // backward: dh2 chunk by chunk, accumulating dh1 = W2 dh2
+    float g1[MLP_H];
+#pragma unroll
+    for (int a = 0; a < MLP_H; ++a) g1[a] = 0.f;
+    float* dh2o = dH2 + (size_t)s * MLP_H;
+#pragma unroll 1
+    for (int bc = 0; bc < MLP_H; bc += MLP_CH) {
+      float d2[MLP_CH];
+#pragma unroll
+      for (int j = 0; j < MLP_CH; ++j) {
+        float v = 0.f;
+#pragma unroll
+        for (int o = 0; o < MLP_NO; ++o) v = fmaf(dy[o], sw[MLP_W3 + (bc + j) * MLP_NO + o], v);
+        d2[j] = h2o[bc + j] > 0.f ? v : 0.f;
+        dh2o[bc + j] = d2[j];
+      }
+#pragma unroll
+      for (int a = 0; a < MLP_H; ++a) {
+#pragma unroll
+        for (int j = 0; j < MLP_CH; ++j) g1[a] = fmaf(d2[j], sw[MLP_W2 + a * MLP_H + bc + j], g1[a]);
+      }
+    }
+    float4* dh1o = reinterpret_cast<float4*>(dH1 + (size_t)s * MLP_H);
+#pragma unroll
+    for (int a = 0; a < MLP_H / 4; ++a)
+      dh1o[a] = make_float4(h1[4 * a] > 0.f ? g1[4 * a] : 0.f, h1[4 * a + 1] > 0.f ? g1[4 * a + 1] : 0.f,
+                            h1[4 * a + 2] > 0.f ? g1[4 * a + 2] : 0.f,
+                            h1[4 * a + 3] > 0.f ? g1[4 * a + 3] : 0.f);
+  }
+  // loss partial (FP64)
+  for (int off = 32; off > 0; off >>= 1) my_loss += __shfl_down(my_loss, off, 64);
+  if ((threadIdx.x & 63) == 0) sl[threadIdx.x >> 6] = my_loss;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(loss, sl[0] + sl[1] + sl[2] + sl[3]);
+}
+
+// Weight gradients over the batch: grad[e] += sum_s A_s[a] * B_s[b] for
+// every entry e (W1: X x dH1, W2: H1 x dH2, W3: H2 x dY, biases: sum dH).
+// A block takes a chunk of samples, stages them in LDS 32 at a time, each
+// thread accumulates ~24 entries in registers, then adds them to grad.
+constexpr int WG_SUB = 32;
+constexpr int WG_PER = (MLP_NP + 255) / 256;
+
+__global__ __launch_bounds__(256) void k_mlp_wgrad(
+    const float* __restrict__ Xall, const int* __restrict__ perm, int base, int nb, int chunk,
+    const float* __restrict__ H1, const float* __restrict__ dH1, const float* __restrict__ H2,
+    const float* __restrict__ dH2, const float* __restrict__ dY, float* __restrict__ grad) {
+  __shared__ float sX[WG_SUB][MLP_NF], sH1[WG_SUB][MLP_H], sdH1[WG_SUB][MLP_H],
+      sH2[WG_SUB][MLP_H], sdH2[WG_SUB][MLP_H], sdY[WG_SUB][MLP_NO];
+  float acc[WG_PER];
+  for (int u = 0; u < WG_PER; ++u) acc[u] = 0.f;
+  const int s0 = blockIdx.x * chunk, s1 = min(nb, s0 + chunk);
+  for (int sb = s0; sb < s1; sb += WG_SUB) {
+    int cnt = min(WG_SUB, s1 - sb);
+    __syncthreads();
+    for (int t = threadIdx.x; t < WG_SUB * MLP_H; t += blockDim.x) {
+      int r = t / MLP_H, c = t % MLP_H;
+      bool ok = r < cnt;
+      size_t g = (size_t)(sb + r) * MLP_H + c;
+      sH1[r][c] = ok ? H1[g] : 0.f;
+      sdH1[r][c] = ok ? dH1[g] : 0.f;
+      sH2[r][c] = ok ? H2[g] : 0.f;
+      sdH2[r][c] = ok ? dH2[g] : 0.f;
+    }
+    for (int t = threadIdx.x; t < WG_SUB * MLP_NF; t += blockDim.x) {
+      int r = t / MLP_NF, c = t % MLP_NF;
+      sX[r][c] = r < cnt ? Xall[(size_t)perm[base + sb + r] * MLP_NF + c] : 0.f;
+    }
+    for (int t = threadIdx.x; t < WG_SUB * MLP_NO; t += blockDim.x) {
+      int r = t / MLP_NO, c = t % MLP_NO;
+      sdY[r][c] = r < cnt ? dY[(size_t)(sb + r) * MLP_NO + c] : 0.f;
+    }
+    __syncthreads();
+    for (int u = 0; u < WG_PER; ++u) {
+      int e = threadIdx.x + 256 * u;
+      if (e >= MLP_NP) break;
+      float v = 0.f;
+      if (e < MLP_B1) {
+        int q = e / MLP_H, a = e % MLP_H;
+        for (int r = 0; r < WG_SUB; ++r) v += sX[r][q] * sdH1[r][a];
+      } else if (e < MLP_W2) {
+        int a = e - MLP_B1;
+        for (int r = 0; r < WG_SUB; ++r) v += sdH1[r][a];
+      } else if (e < MLP_B2) {
+        int a = (e - MLP_W2) / MLP_H, b = (e - MLP_W2) % MLP_H;
+        for (int r = 0; r < WG_SUB; ++r) v += sH1[r][a] * sdH2[r][b];
+      } else if (e < MLP_W3) {
+        int b = e - MLP_B2;
+        for (int r = 0; r < WG_SUB; ++r) v += sdH2[r][b];
+      } else if (e < MLP_B3) {
+        int b = (e - MLP_W3) / MLP_NO, o = (e - MLP_W3) % MLP_NO;
+        for (int r = 0; r < WG_SUB; ++r) v += sH2[r][b] * sdY[r][o];
+      } else {
+        int o = e - MLP_B3;
+        for (int r = 0; r < WG_SUB; ++r) v += sdY[r][o];
+      }
+      acc[u] += v;
+    }
+  }
+  for (int u = 0; u < WG_PER; ++u) {
+    int e = threadIdx.x + 256 * u;
+    if (e < MLP_NP) atomicAdd(&grad[e], acc[u]);
+  }
+}
+
+__global__ void k_mlp_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                           const float* __restrict__ g, __half* __restrict__ w16, int np,
+                           float lr, float b1, float b2, float eps, float bc1, float bc2) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= np) return;
+  float gg = g[e];
+  float mm = b1 * m[e] + (1.f - b1) * gg;
+  float vv = b2 * v[e] + (1.f - b2) * gg * gg;
+  m[e] = mm;
+  v[e] = vv;
+  float mh = mm / bc1, vh = vv / bc2;
+  float nw = w[e] - lr * mh / (sqrtf(vh) + eps);
+  w[e] = nw;
+  w16[e] = __float2half(nw);
+}
+
+__global__ void k_f32_to_f16(const float* __restrict__ a, __half* __restrict__ b, int n) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) b[e] = __float2half(a[e]);
+}
+
+// Record one step's samples at offset `base` of the sample store: the
+// features computed before the solve, the converged pressure increment and
+// (multiphase) the change of the phase-1 drift when it is evaluated with the
+// converged instead of the previous pressure.
+__global__ void k_mlp_record(KP k, MLPP mp, const float4* __restrict__ pos,
+                             const uint32_t* __restrict__ key, const float4* __restrict__ phys,
+                             const float* __restrict__ pprev, const float* __restrict__ X,
+                             const float4* __restrict__ u_old, const float4* __restrict__ u_new,
+                             float* __restrict__ Xs, float* __restrict__ Ts,
+                             float* __restrict__ Ms) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.n) return;
+  bool ok = type_of(pos[i]) != SPHX_TYPE_BOUNDARY && (int)key[i] != k.gridtotal;
+  const float4* xs = reinterpret_cast<const float4*>(X + (size_t)i * MLP_NF);
+  float4* xd = reinterpret_cast<float4*>(Xs + (size_t)i * MLP_NF);
+#pragma unroll
+  for (int q = 0; q < MLP_NF / 4; ++q) xd[q] = xs[q];
+  float t0 = 0.f, t1 = 0.f, t2 = 0.f, t3 = 0.f;
+  if (ok) {
+    t0 = (phys[i].z - fmaxf(0.f, pprev[i])) / mp.p_scale * MLP_TSCALE;
+    if (u_old && u_new) {
+      float s = MLP_TSCALE / mp.u_scale;
+      float4 a = u_old[i], b = u_new[i];
+      t1 = (b.x - a.x) * s; t2 = (b.y - a.y) * s; t3 = (b.z - a.z) * s;
+    }
+  }
+  bool finite = t0 == t0 && t1 == t1 && t2 == t2 && t3 == t3 && fabsf(t0) < 1e30f;
+  float m = ok && finite ? 1.f : 0.f;
+  reinterpret_cast<float4*>(Ts)[i] = m > 0.f ? make_float4(t0, t1, t2, t3)
+                                             : make_float4(0.f, 0.f, 0.f, 0.f);
+  Ms[i] = m;
+}
+
+// p(t-1) when the features are not computed.
+__global__ void k_mlp_pprev(int n, const float4* __restrict__ phys, float* __restrict__ pprev) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) pprev[i] = phys[i].z;
+}
+
+// hist.w = p - max(0, p(t-1)) after a solve (0 off the fluid).
+__global__ void k_mlp_hist(KP k, const float4* __restrict__ pos, const uint32_t* __restrict__ key,
+                           const float4* __restrict__ phys, const float* __restrict__ pprev,
+                           float4* __restrict__ hist) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.n) return;
+  bool ok = type_of(pos[i]) != SPHX_TYPE_BOUNDARY && (int)key[i] != k.gridtotal;
+  float d = ok ? phys[i].z - fmaxf(0.f, pprev[i]) : 0.f;
+  if (!(d == d)) d = 0.f;
+  hist[i].w = d;
+}
+
+__global__ void k_mlp_reorder(const uint32_t* __restrict__ skey,
+                              const uint32_t* __restrict__ idsort, int n, uint32_t sentinel,
+                              const float4* __restrict__ a, float4* __restrict__ b) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  b[s] = a[skey[s] != sentinel ? (int)idsort[s] : s];
+}
+
+// Uniform sample indices in [0, total) for `count` draws (SplitMix64).
+__global__ void k_mlp_draw(int* __restrict__ order, int count, uint64_t total, uint64_t seed) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(t + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  order[t] = (int)(z % total);
+}
+
+MLPP make_mlpp(const sphx_context* c) {
+  return MLPP{c->mlp.p_scale, c->mlp.u_scale, c->mlp.drift_blend};
+}
+
+int nbr_list(sphx_context* c, NbrList* out);
+int mf_drift_into(sphx_context* c, const float4* dcorr, float4* out0, float4* out1,
+                  float4* out2);
+
+static int mlp_need(sphx_context* c, const char* who) {
+  SPHX_CHECK(c, who, "null context");
+  SPHX_TRY(ensure_device(c));
+  SPHX_CHECK(c->mlp.ready, who, "network not initialised (call sphx_mlp_init)");
+  return 0;
+}
+
+static int mlp_batch_buffers(sphx_context* c, int batch) {
+  auto& m = c->mlp;
+  if (batch <= m.batch_cap) return 0;
+  size_t b = (size_t)batch;
+  SPHX_TRY(m.H1.alloc(b * MLP_H));
+  SPHX_TRY(m.dH1.alloc(b * MLP_H));
+  SPHX_TRY(m.H2.alloc(b * MLP_H));
+  SPHX_TRY(m.dH2.alloc(b * MLP_H));
+  SPHX_TRY(m.dY.alloc(b * MLP_NO));
+  m.batch_cap = batch;
+  return 0;
+}
+
+// Gradient of the loss over samples order[base .. base+nb) into m.g (not
+// zeroed here); loss sum added to m.loss[0].
+static int mlp_grad_batch(sphx_context* c, const float* X, const float* T, const float* M,
+                          const int* order, int base, int nb, int nvalid, float4 lw) {
+  auto& m = c->mlp;
+  SPHX_TRY(mlp_batch_buffers(c, nb));
+  k_mlp_fwdbwd<<<blocks_for(nb, 256), 256, 0, c->stream>>>(
+      X, T, M, order, base, nb, nvalid, m.w.p, lw, m.H1.p, m.dH1.p, m.H2.p, m.dH2.p, m.dY.p,
+      m.loss.p);
+  SPHX_HIP(hipGetLastError());
+  int blocks = std::min(512, std::max(1, nb / 64));
+  int chunk = (nb + blocks - 1) / blocks;
+  chunk = (chunk + WG_SUB - 1) / WG_SUB * WG_SUB;
+  blocks = (nb + chunk - 1) / chunk;
+  k_mlp_wgrad<<<blocks, 256, 0, c->stream>>>(X, order, base, nb, chunk, m.H1.p, m.dH1.p, m.H2.p,
+                                             m.dH2.p, m.dY.p, m.g.p);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+static int mlp_refresh16(sphx_context* c) {
+  auto& m = c->mlp;
+  k_f32_to_f16<<<blocks_for(MLP_NP, 256), 256, 0, c->stream>>>(
+      m.w.p, reinterpret_cast<__half*>(m.w16.p), MLP_NP);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+int iisph_solve_plain(sphx_context* c, int* iters, double* err);
+
+// Online refresh: online_iters Adam steps, each on online_batch samples
+// drawn uniformly (on the device) from the recorded ring.
+static int mlp_online_refresh(sphx_context* c) {
+  auto& m = c->mlp;
+  const int nb = m.online_batch, steps = m.online_iters;
+  const size_t draws = (size_t)nb * steps;
+  SPHX_TRY(m.order.alloc(draws));
+  m.online_seed += 0x632BE59BD9B4E019ull;
+  k_mlp_draw<<<blocks_for((int)draws, 256), 256, 0, c->stream>>>(m.order.p, (int)draws,
+                                                                  m.nsamples, m.online_seed);
+  SPHX_HIP(hipGetLastError());
+  float4 l4 = make_float4(m.online_lw[0], m.online_lw[1], m.online_lw[2], m.online_lw[3]);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int t = 0; t < steps; ++t) {
+    SPHX_HIP(hipMemsetAsync(m.g.p, 0, MLP_NP * sizeof(float), c->stream));
+    SPHX_TRY(mlp_grad_batch(c, m.sX.p, m.sT.p, m.sM.p, m.order.p, t * nb, nb, nb, l4));
+    ++m.adam_t;
+    float bc1 = 1.f - std::pow(b1, (float)m.adam_t), bc2 = 1.f - std::pow(b2, (float)m.adam_t);
+    k_mlp_adam<<<blocks_for(MLP_NP, 256), 256, 0, c->stream>>>(
+        m.w.p, m.m.p, m.v.p, m.g.p, reinterpret_cast<__half*>(m.w16.p), MLP_NP, m.online_lr, b1,
+        b2, eps, bc1, bc2);
+    SPHX_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+// The pressure solve of a step with the learned start, the recorder and
+// the fallback.  Called after the a_ii pass.
+int mlp_solve(sphx_context* c, int* iters, double* err) {
+  auto& m = c->mlp;
+  if (!m.ready || c->num == 0) return iisph_solve_plain(c, iters, err);
+  KP k = make_kp(c);
+  MLPP mp = make_mlpp(c);
+  size_t n = (size_t)c->num;
+  const bool mf = c->nphase > 0;
+  const bool online = m.online_every > 0;
+  const bool record = m.record || online;
+  SPHX_TRY(m.X.alloc((size_t)c->cap * MLP_NF));
+  SPHX_TRY(m.pprev.alloc((size_t)c->cap));
+  float4* hist = m.hist[c->cur].p;
+  if (m.warm || record) {
+    NbrList L;
+    SPHX_TRY(nbr_list(c, &L));
+    timer_begin(c, "mlp_features");
+    k_mlp_features<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+        k, L, mp, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p,
+        c->rho_adv.p, mf ? c->mf_alpha[c->cur].p : nullptr, hist, m.X.p, m.pprev.p);
+    timer_end(c);
+  } else {
+    k_mlp_pprev<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(k.n, c->PHYS(), m.pprev.p);
+  }
+  SPHX_HIP(hipGetLastError());
+  float4* dcorr = m.drift && mf ? hist : nullptr;
+  if (m.warm || dcorr) {
+    timer_begin(c, "mlp_infer");
+    k_mlp_infer<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+        k, mp, m.warm, m.X.p, reinterpret_cast<const __half*>(m.w16.p), c->POS(), c->key.p,
+        c->PHYS(), c->press_l.p, c->ps.p, dcorr);
+    timer_end(c);
+    SPHX_HIP(hipGetLastError());
+  }
+  int it = 0;
+  double e = 0.0;
+  SPHX_TRY(iisph_solve_plain(c, &it, &e));
+  m.last_fallback = 0;
+  if (m.warm) {
+    ++m.learned_solves;
+    // the done flag is also set at max_iters: judge by the error itself
+    bool converged = e == e && e <= (double)c->O.tol;
+    if (!converged) {
+      ++m.fallbacks;
+      m.last_fallback = 1;
+      k_mlp_plain_start<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+          k, c->POS(), c->key.p, c->PHYS(), m.pprev.p, c->press_l.p, c->ps.p);
+      SPHX_HIP(hipGetLastError());
+      int it2 = 0;
+      SPHX_TRY(iisph_solve_plain(c, &it2, &e));
+      it += it2;
+    }
+  }
+  k_mlp_hist<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(k, c->POS(), c->key.p, c->PHYS(),
+                                                          m.pprev.p, hist);
+  SPHX_HIP(hipGetLastError());
+  if (record && m.max_samples >= n) {
+    size_t base = m.wpos;
+    if (base + n > m.max_samples) base = online ? 0 : m.max_samples;   // ring or full
+    if (base + n <= m.max_samples) {
+      const float4* u_old = nullptr;
+      const float4* u_new = nullptr;
+      if (mf && c->nphase >= 2 && !m.drift) {
+        SPHX_TRY(m.unew.alloc((size_t)c->cap * 3));
+        float4* u = m.unew.p;
+        SPHX_TRY(mf_drift_into(c, nullptr, u, u + c->cap, u + 2 * (size_t)c->cap));
+        u_old = c->mf_drift[1].p;
+        u_new = u + c->cap;
+      }
+      timer_begin(c, "mlp_record");
+      k_mlp_record<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+          k, mp, c->POS(), c->key.p, c->PHYS(), m.pprev.p, m.X.p, u_old, u_new,
+          m.sX.p + base * MLP_NF, m.sT.p + base * MLP_NO, m.sM.p + base);
+      timer_end(c);
+      SPHX_HIP(hipGetLastError());
+      m.wpos = base + n;
+      m.nsamples = std::max(m.nsamples, m.wpos);
+    }
+  }
+  ++m.solves;
+  if (online && m.solves % m.online_every == 0 && m.nsamples > 0) {
+    timer_begin(c, "mlp_train");
+    SPHX_TRY(mlp_online_refresh(c));
+    timer_end(c);
+  }
+  if (iters) *iters = it;
+  if (err) *err = e;
+  return 0;
+}
+
+int mlp_reorder(sphx_context* c, int a, int b) {
+  if (!c->mlp.ready || c->num == 0) return 0;
+  k_mlp_reorder<<<blocks_for(c->num, 256), 256, 0, c->stream>>>(
+      c->key.p, c->idsort.p, c->num, (uint32_t)c->P.gridtotal, c->mlp.hist[a].p,
+      c->mlp.hist[b].p);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace sphx
+
+using namespace sphx;
+
+extern "C" {
+
+int sphx_mlp_nparam(void) { return MLP_NP; }
+
+int sphx_mlp_init(sphx_context* c, uint64_t seed, float p_scale, float u_scale) {
+  SPHX_CHECK(c, "sphx_mlp_init", "null context");
+  SPHX_CHECK(p_scale > 0.f && u_scale > 0.f, "sphx_mlp_init", "scales must be positive");
+  SPHX_TRY(ensure_device(c));
+  auto& m = c->mlp;
+  SPHX_TRY(m.w.alloc(MLP_NP));
+  SPHX_TRY(m.m.alloc(MLP_NP));
+  SPHX_TRY(m.v.alloc(MLP_NP));
+  SPHX_TRY(m.g.alloc(MLP_NP));
+  SPHX_TRY(m.w16.alloc(MLP_NP + 1));
+  SPHX_TRY(m.loss.alloc(4));
+  for (int b = 0; b < 2; ++b) {
+    SPHX_TRY(m.hist[b].alloc((size_t)std::max(c->cap, 1)));
+    SPHX_HIP(hipMemset(m.hist[b].p, 0, (size_t)std::max(c->cap, 1) * sizeof(float4)));
+  }
+  // He initialisation of the weights, zero biases; the output layer starts
+  // at zero so an untrained network predicts "no change".
+  std::mt19937_64 rng(seed);
+  std::normal_distribution<double> nd(0.0, 1.0);
+  std::vector<float> w(MLP_NP, 0.f);
+  for (int e = MLP_W1; e < MLP_B1; ++e) w[e] = (float)(nd(rng) * std::sqrt(2.0 / MLP_NF));
+  for (int e = MLP_W2; e < MLP_B2; ++e) w[e] = (float)(nd(rng) * std::sqrt(2.0 / MLP_H));
+  SPHX_HIP(hipMemcpy(m.w.p, w.data(), MLP_NP * sizeof(float), hipMemcpyHostToDevice));
+  SPHX_HIP(hipMemset(m.m.p, 0, MLP_NP * sizeof(float)));
+  SPHX_HIP(hipMemset(m.v.p, 0, MLP_NP * sizeof(float)));
+  m.adam_t = 0;
+  m.p_scale = p_scale;
+  m.u_scale = u_scale;
+  m.ready = true;
+  SPHX_TRY(mlp_refresh16(c));
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int sphx_mlp_set_weights(sphx_context* c, const float* w, int n) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_set_weights"));
+  SPHX_CHECK(w && n == MLP_NP, "sphx_mlp_set_weights", "expects 6020 weights");
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  SPHX_HIP(hipMemcpy(c->mlp.w.p, w, MLP_NP * sizeof(float), hipMemcpyHostToDevice));
+  SPHX_HIP(hipMemset(c->mlp.m.p, 0, MLP_NP * sizeof(float)));
+  SPHX_HIP(hipMemset(c->mlp.v.p, 0, MLP_NP * sizeof(float)));
+  c->mlp.adam_t = 0;
+  SPHX_TRY(mlp_refresh16(c));
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int sphx_mlp_get_weights(sphx_context* c, float* w, int n) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_get_weights"));
+  SPHX_CHECK(w && n == MLP_NP, "sphx_mlp_get_weights", "expects 6020 weights");
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  SPHX_HIP(hipMemcpy(w, c->mlp.w.p, MLP_NP * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// warm: learned pressure warm start; drift: learned drift correction;
+// record: store (features, targets) of every solve.  Drift targets are
+// recorded only while the drift correction is off (they are the change of
+// the uncorrected drift); with it on they are stored as zero.
+int sphx_mlp_mode(sphx_context* c, int warm, int drift, int record) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_mode"));
+  auto& m = c->mlp;
+  m.warm = warm != 0;
+  m.drift = drift != 0;
+  m.record = record != 0;
+  return 0;
+}
+
+// Online refresh: every `every` solves (0 = off), `iters` Adam steps with
+// learning rate lr on batches of `batch` samples drawn uniformly from the
+// sample store, which then records every solve as a ring.
+int sphx_mlp_online(sphx_context* c, int every, int iters, int batch, float lr, const float* lw) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_online"));
+  SPHX_CHECK(every >= 0 && iters >= 0 && lr >= 0.f, "sphx_mlp_online", "bad argument");
+  SPHX_CHECK(every == 0 || (batch > 0 && lw), "sphx_mlp_online", "bad batch or weights");
+  SPHX_CHECK(every == 0 || (size_t)batch * (size_t)iters < (size_t)INT32_MAX, "sphx_mlp_online",
+             "batch * iters too large");
+  SPHX_CHECK(every == 0 || c->mlp.max_samples >= (size_t)c->num, "sphx_mlp_online",
+             "reserve room for at least one step of samples first");
+  auto& m = c->mlp;
+  m.online_every = every;
+  m.online_iters = iters;
+  m.online_batch = batch;
+  m.online_lr = lr;
+  if (lw)
+    for (int o = 0; o < 4; ++o) m.online_lw[o] = lw[o];
+  return 0;
+}
+
+int sphx_mlp_set_drift_blend(sphx_context* c, float blend) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_set_drift_blend"));
+  c->mlp.drift_blend = blend;
+  return 0;
+}
+
+// Reserve room for `max_samples` recorded samples (one per particle slot
+// and step; boundary and out-of-grid slots are stored masked).
+int sphx_mlp_reserve(sphx_context* c, int64_t max_samples) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_reserve"));
+  SPHX_CHECK(max_samples >= 0, "sphx_mlp_reserve", "negative size");
+  auto& m = c->mlp;
+  size_t s = (size_t)max_samples;
+  SPHX_TRY(m.sX.alloc(std::max<size_t>(s, 1) * MLP_NF));
+  SPHX_TRY(m.sT.alloc(std::max<size_t>(s, 1) * MLP_NO));
+  SPHX_TRY(m.sM.alloc(std::max<size_t>(s, 1)));
+  m.max_samples = s;
+  m.nsamples = 0;
+  m.wpos = 0;
+  return 0;
+}
+
+int sphx_mlp_clear_samples(sphx_context* c) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_clear_samples"));
+  c->mlp.nsamples = 0;
+  c->mlp.wpos = 0;
+  return 0;
+}
+
+// Recorded sample count and the number of them that are valid (unmasked).
+int sphx_mlp_samples(sphx_context* c, int64_t* total, int64_t* valid) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_samples"));
+  auto& m = c->mlp;
+  if (total) *total = (int64_t)m.nsamples;
+  if (valid) {
+    SPHX_HIP(hipStreamSynchronize(c->stream));
+    std::vector<float> msk(m.nsamples);
+    if (m.nsamples)
+      SPHX_HIP(hipMemcpy(msk.data(), m.sM.p, m.nsamples * sizeof(float), hipMemcpyDeviceToHost));
+    int64_t v = 0;
+    for (float x : msk) v += x > 0.f;
+    *valid = v;
+  }
+  return 0;
+}
+
+// Copy recorded samples [start, start+count) to the host (any pointer may
+// be NULL): X [count][24], T [count][4], M [count].
+int sphx_mlp_download_samples(sphx_context* c, int64_t start, int64_t count, float* X, float* T,
+                              float* M) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_download_samples"));
+  auto& m = c->mlp;
+  SPHX_CHECK(start >= 0 && count >= 0 && (size_t)(start + count) <= m.nsamples,
+             "sphx_mlp_download_samples", "range exceeds the recorded samples");
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  if (count == 0) return 0;
+  if (X) SPHX_HIP(hipMemcpy(X, m.sX.p + start * MLP_NF, count * MLP_NF * sizeof(float), hipMemcpyDeviceToHost));
+  if (T) SPHX_HIP(hipMemcpy(T, m.sT.p + start * MLP_NO, count * MLP_NO * sizeof(float), hipMemcpyDeviceToHost));
+  if (M) SPHX_HIP(hipMemcpy(M, m.sM.p + start, count * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// Features of the current state (after the a_ii pass) into X [num][24],
+// and the previous pressures into pprev [num] (either may be NULL).
+int sphx_mlp_features(sphx_context* c, float* X, float* pprev, float* dp_last) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_features"));
+  SPHX_CHECK(c->sorted_valid, "sphx_mlp_features", "grid not built");
+  KP k = make_kp(c);
+  if (k.n == 0) return 0;
+  NbrList L;
+  SPHX_TRY(nbr_list(c, &L));
+  auto& m = c->mlp;
+  SPHX_TRY(m.X.alloc((size_t)c->cap * MLP_NF));
+  SPHX_TRY(m.pprev.alloc((size_t)c->cap));
+  k_mlp_features<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+      k, L, make_mlpp(c), c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p,
+      c->rho_adv.p, c->nphase > 0 ? c->mf_alpha[c->cur].p : nullptr, m.hist[c->cur].p, m.X.p,
+      m.pprev.p);
+  SPHX_HIP(hipGetLastError());
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  if (dp_last) {
+    std::vector<float4> h(k.n);
+    SPHX_HIP(hipMemcpy(h.data(), m.hist[c->cur].p, (size_t)k.n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < k.n; ++i) dp_last[i] = h[i].w;
+  }
+  if (X) SPHX_HIP(hipMemcpy(X, m.X.p, (size_t)k.n * MLP_NF * sizeof(float), hipMemcpyDeviceToHost));
+  if (pprev) SPHX_HIP(hipMemcpy(pprev, m.pprev.p, (size_t)k.n * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// The network (FP16 weights, FP32 accumulation) on host features.
+int sphx_mlp_eval(sphx_context* c, const float* X, int n, float* Y) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_eval"));
+  SPHX_CHECK(X && Y && n >= 0, "sphx_mlp_eval", "null argument");
+  if (n == 0) return 0;
+  DevBuf<float> dx, dy;
+  SPHX_TRY(dx.alloc((size_t)n * MLP_NF));
+  SPHX_TRY(dy.alloc((size_t)n * MLP_NO));
+  SPHX_HIP(hipMemcpy(dx.p, X, (size_t)n * MLP_NF * sizeof(float), hipMemcpyHostToDevice));
+  k_mlp_eval<<<blocks_for(n, 256), 256, 0, c->stream>>>(
+      dx.p, n, reinterpret_cast<const __half*>(c->mlp.w16.p), dy.p);
+  SPHX_HIP(hipGetLastError());
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  SPHX_HIP(hipMemcpy(Y, dy.p, (size_t)n * MLP_NO * sizeof(float), hipMemcpyDeviceToHost));
+  dx.release();
+  dy.release();
+  return 0;
+}
+
+// Loss (1/n_valid) sum_s M_s sum_o lw_o (y_so - T_so)^2 and its gradient
+// with respect to the FP32 master weights, by the training kernels, for
+// host samples X [n][24], T [n][4], M [n] (M may be NULL: all valid).
+int sphx_mlp_grad(sphx_context* c, const float* X, const float* T, const float* M, int n,
+                  const float* lw, float* grad, double* loss) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_grad"));
+  SPHX_CHECK(X && T && lw && n > 0, "sphx_mlp_grad", "bad argument");
+  auto& m = c->mlp;
+  std::vector<float> msk(n, 1.f);
+  if (M) std::memcpy(msk.data(), M, n * sizeof(float));
+  int nvalid = 0;
+  for (float v : msk) nvalid += v > 0.f;
+  std::vector<int> ord(n);
+  for (int s = 0; s < n; ++s) ord[s] = s;
+  DevBuf<float> dx, dt, dm;
+  DevBuf<int> dord;
+  SPHX_TRY(dx.alloc((size_t)n * MLP_NF));
+  SPHX_TRY(dt.alloc((size_t)n * MLP_NO));
+  SPHX_TRY(dm.alloc(n));
+  SPHX_TRY(dord.alloc(n));
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  SPHX_HIP(hipMemcpy(dx.p, X, (size_t)n * MLP_NF * sizeof(float), hipMemcpyHostToDevice));
+  SPHX_HIP(hipMemcpy(dt.p, T, (size_t)n * MLP_NO * sizeof(float), hipMemcpyHostToDevice));
+  SPHX_HIP(hipMemcpy(dm.p, msk.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  SPHX_HIP(hipMemcpy(dord.p, ord.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  SPHX_HIP(hipMemsetAsync(m.g.p, 0, MLP_NP * sizeof(float), c->stream));
+  SPHX_HIP(hipMemsetAsync(m.loss.p, 0, sizeof(double), c->stream));
+  float4 l4 = make_float4(lw[0], lw[1], lw[2], lw[3]);
+  SPHX_TRY(mlp_grad_batch(c, dx.p, dt.p, dm.p, dord.p, 0, n, std::max(nvalid, 1), l4));
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  if (grad) SPHX_HIP(hipMemcpy(grad, m.g.p, MLP_NP * sizeof(float), hipMemcpyDeviceToHost));
+  if (loss) {
+    double l;
+    SPHX_HIP(hipMemcpy(&l, m.loss.p, sizeof(double), hipMemcpyDeviceToHost));
+    *loss = l / std::max(nvalid, 1);
+  }
+  return 0;
+}
+
+// Adam on the recorded samples: `epochs` passes in shuffled mini-batches
+// of `batch` valid samples.  loss_out[e] (may be NULL) receives the mean
+// loss of epoch e.  The FP16 inference copy is refreshed after each step.
+int sphx_mlp_train(sphx_context* c, int epochs, int batch, float lr, const float* lw,
+                   uint64_t seed, double* loss_out) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_train"));
+  SPHX_CHECK(epochs >= 0 && batch > 0 && lr > 0.f && lw, "sphx_mlp_train", "bad argument");
+  auto& m = c->mlp;
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  std::vector<float> msk(m.nsamples);
+  if (m.nsamples)
+    SPHX_HIP(hipMemcpy(msk.data(), m.sM.p, m.nsamples * sizeof(float), hipMemcpyDeviceToHost));
+  std::vector<int> valid;
+  valid.reserve(m.nsamples);
+  SPHX_CHECK(m.nsamples < (size_t)INT32_MAX, "sphx_mlp_train", "too many samples");
+  for (size_t s = 0; s < m.nsamples; ++s)
+    if (msk[s] > 0.f) valid.push_back((int)s);
+  SPHX_CHECK(!valid.empty(), "sphx_mlp_train", "no recorded samples");
+  const int nv = (int)valid.size();
+  SPHX_TRY(m.order.alloc(nv));
+  std::mt19937_64 rng(seed);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  float4 l4 = make_float4(lw[0], lw[1], lw[2], lw[3]);
+  for (int ep = 0; ep < epochs; ++ep) {
+    std::shuffle(valid.begin(), valid.end(), rng);
+    SPHX_HIP(hipMemcpyAsync(m.order.p, valid.data(), (size_t)nv * sizeof(int),
+                            hipMemcpyHostToDevice, c->stream));
+    SPHX_HIP(hipMemsetAsync(m.loss.p, 0, sizeof(double), c->stream));
+    for (int base = 0; base < nv; base += batch) {
+      int nb = std::min(batch, nv - base);
+      SPHX_HIP(hipMemsetAsync(m.g.p, 0, MLP_NP * sizeof(float), c->stream));
+      SPHX_TRY(mlp_grad_batch(c, m.sX.p, m.sT.p, m.sM.p, m.order.p, base, nb, nb, l4));
+      ++m.adam_t;
+      float bc1 = 1.f - std::pow(b1, (float)m.adam_t), bc2 = 1.f - std::pow(b2, (float)m.adam_t);
+      k_mlp_adam<<<blocks_for(MLP_NP, 256), 256, 0, c->stream>>>(
+          m.w.p, m.m.p, m.v.p, m.g.p, reinterpret_cast<__half*>(m.w16.p), MLP_NP, lr, b1, b2,
+          eps, bc1, bc2);
+      SPHX_HIP(hipGetLastError());
+    }
+    // host copy of `valid` is reshuffled next epoch: wait for the upload
+    SPHX_HIP(hipStreamSynchronize(c->stream));
+    if (loss_out) {
+      double l;
+      SPHX_HIP(hipMemcpy(&l, m.loss.p, sizeof(double), hipMemcpyDeviceToHost));
+      loss_out[ep] = l / nv;
+    }
+  }
+  return 0;
+}
+
+// Learned solves so far, how many fell back to the plain start, and
+// whether the last one did.
+int sphx_mlp_stats(sphx_context* c, int64_t* learned, int64_t* fallbacks, int* last_fallback) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_stats"));
+  if (learned) *learned = c->mlp.learned_solves;
+  if (fallbacks) *fallbacks = c->mlp.fallbacks;
+  if (last_fallback) *last_fallback = c->mlp.last_fallback;
+  return 0;
+}
+
+}  // extern "C"

@@ -41,8 +41,8 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_drift(
     KP k, MFP m, NbrList L, const uint32_t* __restrict__ key, const int* __restrict__ cs,
     const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float4* __restrict__ phys, const float4* __restrict__ alpha,
-    const float4* __restrict__ mforce, float4* __restrict__ drift0, float4* __restrict__ drift1,
-    float4* __restrict__ drift2) {
+    const float4* __restrict__ mforce, const float4* __restrict__ dcorr,
+    float4* __restrict__ drift0, float4* __restrict__ drift1, float4* __restrict__ drift2) {
   int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
@@ -98,6 +98,15 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_drift(
     ssum = add(ssum, mul(st[f], t * m.rho[f] * invdens));
   }
   const float4 acc = mforce[i];
+  // learned closure: dispersed phase 1 gets +d, phase 0 -(c_1/c_0) d so
+  // that sum_k c_k u_mk stays zero
+  float3 dc = f3(0.f, 0.f, 0.f);
+  float dc0 = 0.f;
+  if (dcorr && m.nphase >= 2 && conc[0] > 1e-6f) {
+    float4 d = dcorr[i];
+    dc = f3(d.x, d.y, d.z);
+    dc0 = -conc[1] / conc[0];
+  }
   float4* out[3] = {drift0, drift1, drift2};
   for (int f = 0; f < m.nphase; ++f) {
     float tdiff = (m.rho[f] - dsum) * relax;
@@ -105,6 +114,8 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_drift(
     u.x += acc.x * tdiff - m.cdiff * m.sigma * (st[f].x - ssum.x);
     u.y += acc.y * tdiff - m.cdiff * m.sigma * (st[f].y - ssum.y);
     u.z += acc.z * tdiff - m.cdiff * m.sigma * (st[f].z - ssum.z);
+    float cf = f == 1 ? 1.f : (f == 0 ? dc0 : 0.f);
+    u = add(u, mul(dc, cf));
     out[f][i] = make_float4(u.x, u.y, u.z, 0.f);
   }
 }
@@ -306,6 +317,23 @@ int mf_reorder(sphx_context* c, int a, int b) {
 
 int nbr_list(sphx_context* c, NbrList* out);
 
+// Drift velocities of the current state (fractions alpha[cur], pressure
+// phys.z, g - a of the last step) into out0..2, plus the optional learned
+// correction.  Does not commit pending fractions.
+int mf_drift_into(sphx_context* c, const float4* dcorr, float4* out0, float4* out1,
+                  float4* out2) {
+  KP k = make_kp(c);
+  if (k.n == 0) return 0;
+  MFP m = make_mfp(c);
+  NbrList L;
+  SPHX_TRY(nbr_list(c, &L));
+  k_mf_drift<<<blocks_for(k.n, MF_THREADS), MF_THREADS, 0, c->stream>>>(
+      k, m, L, c->key.p, c->cell_start.p, c->POS(), c->VEL(), c->PHYS(), c->mf_alpha[c->cur].p,
+      c->mf_force[c->cur].p, dcorr, out0, out1, out2);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
 static int need_mf(sphx_context* c, const char* who) {
   SPHX_CHECK(c, who, "null context");
   SPHX_TRY(ensure_device(c));
@@ -424,17 +452,12 @@ int sphx_mf_download(sphx_context* c, float* alpha, float* rho_m, float* drift, 
 int sphx_mf_drift_velocity(sphx_context* c) {
   SPHX_TRY(need_mf(c, "sphx_mf_drift_velocity"));
   SPHX_TRY(mf_commit(c));
-  KP k = make_kp(c);
-  if (k.n == 0) return 0;
-  MFP m = make_mfp(c);
-  NbrList L;
-  SPHX_TRY(nbr_list(c, &L));
+  if (c->num == 0) return 0;
+  const float4* dcorr =
+      c->mlp.ready && c->mlp.drift ? c->mlp.hist[c->cur].p : nullptr;
   timer_begin(c, "mf_drift");
-  k_mf_drift<<<blocks_for(k.n, MF_THREADS), MF_THREADS, 0, c->stream>>>(
-      k, m, L, c->key.p, c->cell_start.p, c->POS(), c->VEL(), c->PHYS(), c->mf_alpha[c->cur].p,
-      c->mf_force[c->cur].p, c->mf_drift[0].p, c->mf_drift[1].p, c->mf_drift[2].p);
+  SPHX_TRY(mf_drift_into(c, dcorr, c->mf_drift[0].p, c->mf_drift[1].p, c->mf_drift[2].p));
   timer_end(c);
-  SPHX_HIP(hipGetLastError());
   return 0;
 }
 

@@ -151,6 +151,23 @@ def lib() -> C.CDLL:
         L.sphx_scene_bubbles.argtypes = [I, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                          I, C.c_float, C.c_float, I, C.c_uint64, P(ParamCarrier),
                                          VP, VP, VP, P(SceneInfo)]
+        I64 = C.c_int64
+        L.sphx_mlp_nparam.restype = I
+        L.sphx_mlp_init.argtypes = [VP, C.c_uint64, C.c_float, C.c_float]
+        L.sphx_mlp_set_weights.argtypes = [VP, VP, I]
+        L.sphx_mlp_get_weights.argtypes = [VP, VP, I]
+        L.sphx_mlp_mode.argtypes = [VP, I, I, I]
+        L.sphx_mlp_set_drift_blend.argtypes = [VP, C.c_float]
+        L.sphx_mlp_reserve.argtypes = [VP, I64]
+        L.sphx_mlp_clear_samples.argtypes = [VP]
+        L.sphx_mlp_samples.argtypes = [VP, P(I64), P(I64)]
+        L.sphx_mlp_download_samples.argtypes = [VP, I64, I64, VP, VP, VP]
+        L.sphx_mlp_features.argtypes = [VP, VP, VP, VP]
+        L.sphx_mlp_online.argtypes = [VP, I, I, I, C.c_float, VP]
+        L.sphx_mlp_eval.argtypes = [VP, VP, I, VP]
+        L.sphx_mlp_grad.argtypes = [VP, VP, VP, VP, I, VP, VP, P(D)]
+        L.sphx_mlp_train.argtypes = [VP, I, I, C.c_float, VP, C.c_uint64, VP]
+        L.sphx_mlp_stats.argtypes = [VP, P(I64), P(I64), P(I)]
         _lib = L
     return _lib
 
@@ -329,6 +346,86 @@ class Context:
         it, err = C.c_int(), C.c_double()
         check(self.L.sphx_mf_step(self.h, C.byref(it), C.byref(err)), "mf_step")
         return it.value, err.value
+
+    # -- learned warm start / drift closure --
+    MLP_NF, MLP_NO = 24, 4
+
+    def mlp_init(self, seed=1, p_scale=1.0, u_scale=1.0):
+        check(self.L.sphx_mlp_init(self.h, seed, p_scale, u_scale), "sphx_mlp_init")
+
+    def mlp_weights(self):
+        w = np.zeros(self.L.sphx_mlp_nparam(), np.float32)
+        check(self.L.sphx_mlp_get_weights(self.h, ptr(w), len(w)), "sphx_mlp_get_weights")
+        return w
+
+    def mlp_set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        check(self.L.sphx_mlp_set_weights(self.h, ptr(w), len(w)), "sphx_mlp_set_weights")
+
+    def mlp_mode(self, warm=False, drift=False, record=False):
+        check(self.L.sphx_mlp_mode(self.h, int(warm), int(drift), int(record)), "sphx_mlp_mode")
+
+    def mlp_set_drift_blend(self, b):
+        check(self.L.sphx_mlp_set_drift_blend(self.h, b), "sphx_mlp_set_drift_blend")
+
+    def mlp_reserve(self, n):
+        check(self.L.sphx_mlp_reserve(self.h, int(n)), "sphx_mlp_reserve")
+
+    def mlp_clear_samples(self):
+        check(self.L.sphx_mlp_clear_samples(self.h), "sphx_mlp_clear_samples")
+
+    def mlp_samples(self):
+        t, v = C.c_int64(), C.c_int64()
+        check(self.L.sphx_mlp_samples(self.h, C.byref(t), C.byref(v)), "sphx_mlp_samples")
+        return t.value, v.value
+
+    def mlp_download_samples(self, start, count):
+        X = np.zeros((count, self.MLP_NF), np.float32)
+        T = np.zeros((count, self.MLP_NO), np.float32)
+        M = np.zeros(count, np.float32)
+        check(self.L.sphx_mlp_download_samples(self.h, start, count, ptr(X), ptr(T), ptr(M)),
+              "sphx_mlp_download_samples")
+        return X, T, M
+
+    def mlp_features(self):
+        X = np.zeros((self.num, self.MLP_NF), np.float32)
+        pp = np.zeros(self.num, np.float32)
+        dp = np.zeros(self.num, np.float32)
+        check(self.L.sphx_mlp_features(self.h, ptr(X), ptr(pp), ptr(dp)), "sphx_mlp_features")
+        return X, pp, dp
+
+    def mlp_online(self, every, iters=0, batch=0, lr=0.0, lw=(1.0, 1.0, 1.0, 1.0)):
+        lw = np.ascontiguousarray(lw, np.float32)
+        check(self.L.sphx_mlp_online(self.h, every, iters, batch, lr, ptr(lw)), "sphx_mlp_online")
+
+    def mlp_eval(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((len(X), self.MLP_NO), np.float32)
+        check(self.L.sphx_mlp_eval(self.h, ptr(X), len(X), ptr(Y)), "sphx_mlp_eval")
+        return Y
+
+    def mlp_grad(self, X, T, lw, M=None):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        lw = np.ascontiguousarray(lw, np.float32)
+        M = None if M is None else np.ascontiguousarray(M, np.float32)
+        g = np.zeros(self.L.sphx_mlp_nparam(), np.float32)
+        loss = C.c_double()
+        check(self.L.sphx_mlp_grad(self.h, ptr(X), ptr(T), ptr(M), len(X), ptr(lw), ptr(g),
+                                   C.byref(loss)), "sphx_mlp_grad")
+        return loss.value, g
+
+    def mlp_train(self, epochs, batch, lr, lw=(1.0, 0.1, 0.1, 0.1), seed=1):
+        lw = np.ascontiguousarray(lw, np.float32)
+        out = np.zeros(max(epochs, 1), np.float64)
+        check(self.L.sphx_mlp_train(self.h, epochs, batch, lr, ptr(lw), seed, ptr(out)),
+              "sphx_mlp_train")
+        return out[:epochs]
+
+    def mlp_stats(self):
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int()
+        check(self.L.sphx_mlp_stats(self.h, C.byref(a), C.byref(b), C.byref(c)), "sphx_mlp_stats")
+        return dict(learned=a.value, fallbacks=b.value, last_fallback=c.value)
 
     def timing(self, on=True):
         check(self.L.sphx_timing_enable(self.h, 1 if on else 0), "timing")

@@ -1,0 +1,208 @@
+"""Learned warm start on the GPU: features, FP16 inference and training
+gradients against the FP64 oracle, then what the trained network gains
+over the plain warm start on a two-phase tank.
+
+Tolerances (normwise, infinity norm): features 1e-4 per column vs the
+oracle on the same FP32 state; FP16-weight inference 1e-5 vs the FP64
+forward pass with the same FP16-rounded weights; gradient 1e-5 and loss
+1e-6 vs the FP64 oracle."""
+import numpy as np
+import pytest
+
+import orc
+import sphx
+from helpers import state_from_gpu
+from test_gpu_mf import NU, RHO, setup
+
+pytestmark = pytest.mark.gpu
+
+UNDEF = 0xFFFFFFFF
+
+
+def scales(p, fill_height):
+    g = float(np.linalg.norm(p.gravity.tolist()))
+    return 1000.0 * g * fill_height, 1e-3
+
+
+def staged_to_aii(ctx):
+    """The stages of one two-phase step up to the a_ii pass; returns the
+    fractions the step uses."""
+    ctx.call("grid_build")
+    ctx.call("iisph_density")
+    pre = ctx.mf_download()
+    for s in ["mf_drift_velocity", "mf_alpha_advance", "mf_correction", "iisph_predict_dii",
+              "mf_force", "iisph_predict_aii"]:
+        ctx.call(s)
+    return pre
+
+
+def test_features_match_oracle(gpu):
+    p, disp, calc, alpha, o = setup()
+    P, U = scales(p, 0.16)
+    with sphx.Context(p, o) as ctx:
+        ctx.upload(disp, calc)
+        ctx.mf_setup(RHO, NU)
+        ctx.mf_upload(alpha)
+        ctx.call("grid_build")
+        for _ in range(3):
+            ctx.mf_step()
+        pre = staged_to_aii(ctx)
+        ctx.mlp_init(1, P, U)
+        Xg, pp, dpl = ctx.mlp_features()
+        st = state_from_gpu(ctx, p, np.float32, o)
+        a4 = np.zeros((st.n, 4), np.float32)
+        a4[:, :2] = pre["alpha"]
+        a4[:, 3] = pre["rho_m"]
+        Xo = orc.mlp_features(st, a4, P, dpl)
+        fl = (st["type"] == 0) & (st["mgcell"] != UNDEF)
+        print(f"fluid {fl.sum()}", flush=True)
+        np.testing.assert_array_equal(pp, st["pressure"])
+        worst = 0.0
+        for q in range(orc.MLP_NF):
+            e = orc.relerr(Xg[fl, q], Xo[fl, q])
+            worst = max(worst, e)
+            assert e <= 1e-4, (q, e)
+        assert np.all(Xg[~fl] == 0)
+        print(f"features: worst column error {worst:.2e}")
+
+
+def test_fp16_inference_matches_oracle(gpu):
+    p, disp, calc, alpha, o = setup(n=2000, L=0.15)
+    rng = np.random.default_rng(11)
+    w = (rng.normal(0, 0.25, orc.MLP_NP)).astype(np.float32)
+    X = rng.normal(0, 1, (5000, orc.MLP_NF)).astype(np.float32)
+    with sphx.Context(p, o) as ctx:
+        ctx.mlp_init(1, 1.0, 1.0)
+        ctx.mlp_set_weights(w)
+        Yg = ctx.mlp_eval(X)
+    w16 = w.astype(np.float16).astype(np.float64)
+    Y16, _, _ = orc.mlp_forward(w16, X.astype(np.float64))
+    Y32, _, _ = orc.mlp_forward(w.astype(np.float64), X.astype(np.float64))
+    e16 = orc.relerr(Yg, Y16)
+    e32 = orc.relerr(Yg, Y32)
+    print(f"inference: vs fp16-weight fp64 {e16:.2e}, vs fp32-weight fp64 {e32:.2e}")
+    assert e16 <= 1e-5
+    assert e32 <= 1e-2
+
+
+def test_training_gradient_matches_oracle(gpu):
+    p, disp, calc, alpha, o = setup(n=2000, L=0.15)
+    rng = np.random.default_rng(12)
+    w = (rng.normal(0, 0.2, orc.MLP_NP)).astype(np.float32)
+    n = 3000
+    X = rng.normal(0, 1, (n, orc.MLP_NF)).astype(np.float32)
+    T = rng.normal(0, 1, (n, orc.MLP_NO)).astype(np.float32)
+    M = (rng.uniform(size=n) > 0.2).astype(np.float32)
+    lw = np.array([1.0, 0.25, 0.5, 0.125], np.float32)
+    with sphx.Context(p, o) as ctx:
+        ctx.mlp_init(1, 1.0, 1.0)
+        ctx.mlp_set_weights(w)
+        loss_g, grad_g = ctx.mlp_grad(X, T, lw, M)
+        # all-valid call as well (mask NULL)
+        loss_a, grad_a = ctx.mlp_grad(X[:257], T[:257], lw)
+    v = M > 0
+    loss_o, grad_o = orc.mlp_loss_grad(w.astype(np.float64), X[v], T[v], lw)
+    e = orc.relerr(grad_g, grad_o)
+    print(f"gradient: {e:.2e}, loss {loss_g:.6e} vs {loss_o:.6e}")
+    assert e <= 1e-5
+    assert loss_g == pytest.approx(loss_o, rel=1e-6)
+    loss_o2, grad_o2 = orc.mlp_loss_grad(w.astype(np.float64), X[:257], T[:257], lw)
+    assert orc.relerr(grad_a, grad_o2) <= 1e-5
+    assert loss_a == pytest.approx(loss_o2, rel=1e-6)
+
+
+def drift_scale(ctx):
+    """RMS of the phase-1 drift over the fluid: the drift normalisation."""
+    d = ctx.mf_download()["drift"][:, 1, :]
+    disp, _ = ctx.download()
+    fl = disp["type"] == 0
+    return max(float(np.sqrt(np.mean(np.sum(d[fl].astype(np.float64) ** 2, axis=1)))), 1e-9)
+
+
+def loss_weights(T, M):
+    """Each output's squared error over its target variance, the pressure
+    increment weighted 4x."""
+    var = np.var(T[M > 0].astype(np.float64), axis=0) + 1e-12
+    return np.array([4.0 / var[0], 1.0 / var[1], 1.0 / var[2], 1.0 / var[3]], np.float32)
+
+
+def run_steps(ctx, n):
+    its, errs = [], []
+    for _ in range(n):
+        it, err = ctx.mf_step()
+        its.append(it)
+        errs.append(err)
+    return np.array(its), np.array(errs)
+
+
+def column(n=12000, seed=5):
+    """A 0.1 x 1 x 0.1 m column filled to 0.8 m (the 1M tank's depth, spacing
+    ratio h / s and bubble density) so the solver needs as many iterations
+    as in the large tank."""
+    Lx, Ly, Lz = 0.1, 1.0, 0.1
+    s = (0.8 * Lx * Ly * Lz / n) ** (1.0 / 3.0)
+    h = 1.0944 * s
+    p, disp, calc, alpha, info = sphx.scene_bubbles(n_fluid=n, L=(Lx, Ly, Lz), fill=0.8, h=h,
+                                                    n_bubbles=6, rmin=0.01, rmax=0.025,
+                                                    layers=2, seed=seed)
+    p.dt = 1e-3
+    from test_gpu_mf import setup as _setup   # same solver options
+    o = _setup(n=2000, L=0.15)[4]
+    return p, disp, calc, alpha, o
+
+
+def r2(Y, T, M):
+    v = M > 0
+    res = ((Y[v] - T[v]) ** 2).sum(axis=0)
+    tot = ((T[v] - T[v].mean(axis=0)) ** 2).sum(axis=0) + 1e-30
+    return 1.0 - res / tot
+
+
+def test_learned_warm_start_cuts_iterations(gpu):
+    """Record the plain solver's converged pressures, train on the GPU, and
+    run the same column on from identical states with the learned and with
+    the plain (reference 0.5 p(t-1)) warm start."""
+    p, disp, calc, alpha, o = column()
+    g = float(np.linalg.norm(p.gravity.tolist()))
+    P = 1000.0 * g * 0.8
+    tol = o.tol
+    rec = 30
+    with sphx.Context(p, o) as A, sphx.Context(p, o) as B:
+        for c in (A, B):
+            c.upload(disp, calc)
+            c.mf_setup(RHO, NU)
+            c.mf_upload(alpha)
+            c.call("grid_build")
+        run_steps(A, 10)
+        run_steps(B, 10)
+        U = drift_scale(A)
+        A.mlp_init(7, P, U)
+        A.mlp_reserve(rec * A.num)
+        A.mlp_mode(record=True)
+        ir, _ = run_steps(A, rec)
+        run_steps(B, rec)
+        total, valid = A.mlp_samples()
+        print(f"samples {total} valid {valid}; plain iterations while recording {ir.mean():.2f}",
+              flush=True)
+        assert valid >= 0.99 * rec * disp["type"].tolist().count(0)
+        X, T, M = A.mlp_download_samples(0, total)
+        assert np.all(np.isfinite(X)) and np.all(np.isfinite(T))
+        lw = loss_weights(T, M)
+        v = M > 0
+        loss0 = float(np.mean(np.sum(lw * T[v].astype(np.float64) ** 2, axis=1)))
+        print("target mean", T[v].mean(axis=0), "std", T[v].std(axis=0), flush=True)
+        loss = A.mlp_train(epochs=20, batch=2048, lr=2e-3, lw=lw, seed=3)
+        print(f"train loss (zero predictor {loss0:.3e}):", " ".join(f"{x:.3e}" for x in loss),
+              flush=True)
+        Y = A.mlp_eval(X[v])
+        print("in-sample R2 per output", r2(Y, T[v], M[v]).round(3), flush=True)
+        assert loss[-1] < 0.5 * loss0
+        A.mlp_mode(warm=True)
+        ia, ea = run_steps(A, 15)
+        ib, eb = run_steps(B, 15)
+        st = A.mlp_stats()
+        print(f"iterations learned {ia.mean():.2f} {ia.tolist()} fallbacks {st['fallbacks']}")
+        print(f"iterations plain   {ib.mean():.2f} {ib.tolist()}")
+        assert np.all(ea <= tol) and np.all(eb <= tol)
+        assert st["learned"] == 15
+        assert ia.mean() < ib.mean()
