@@ -31,6 +31,15 @@ HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per fluid particle per launch (fields read or written
 # once, natural sizes: float3 = 12 B).  See DESIGN.md, "Measurement".
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE,
+# scaled by a 256 MiB copy (profiles/iisph_mf_step_1m_hbm_traffic.md)
+TRAFFIC_BYTES = {
+    "iisph_iter": 353.7e6,
+    "iisph_dp": 273.0e6,
+    "mf_alpha": 3637.2e6,
+    "mf_force": 2792.7e6,
+}
+
 ALGO_BYTES = {
     "iisph_dp": 48,        # x 12, p 4, m 4, rho 4, d_ii 12 -> dijpj 12
     "iisph_iter": 84,      # x, m, rho, rho0, d_ii, a_ii, dijpj, rho_adv, p -> p, res, a_p
@@ -234,7 +243,9 @@ def main():
           f"epochs in {train_s:.2f} s, loss " + " ".join(f"{x:.3e}" for x in loss),
           file=sys.stderr, flush=True)
     learn.mlp_mode(warm=True, drift=args.drift, record=False)
-    olw = lw if not args.drift else (lw[0], 0.0, 0.0, 0.0)
+    # online refresh trains the pressure output only (the drift outputs keep
+    # their offline fit), so the recorder needs no drift targets
+    olw = (lw[0], 0.0, 0.0, 0.0)
     if args.online_every > 0:
         learn.mlp_online(args.online_every, args.online_iters, args.online_batch, args.online_lr,
                          olw)
@@ -316,7 +327,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES.get(dom),
+            "traffic_source": "rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated "
+                              "(profiles/iisph_mf_step_1m_hbm_traffic.md)",
             "algorithmic_bytes_per_launch": algo,
             "avg_launch_ms": avg_ms,
         },

@@ -364,7 +364,9 @@ int sphx_mlp_get_weights(sphx_context* ctx, float* w, int n);
 int sphx_mlp_mode(sphx_context* ctx, int warm, int drift, int record);
 /* Online refresh: every `every` solves (0 = off) run `iters` Adam steps of
  * `batch` samples drawn on the device from the sample store, which then
- * records every solve as a ring; lw[4] weights the outputs' errors. */
+ * records every solve as a ring; lw[4] weights the outputs' errors (with
+ * lw[1..3] zero the recorder skips the extra drift pass of the drift
+ * targets). */
 int sphx_mlp_online(sphx_context* ctx, int every, int iters, int batch, float lr,
                     const float* lw);
 int sphx_mlp_set_drift_blend(sphx_context* ctx, float blend);

@@ -651,7 +651,9 @@ int mlp_solve(sphx_context* c, int* iters, double* err) {
     if (base + n <= m.max_samples) {
       const float4* u_old = nullptr;
       const float4* u_new = nullptr;
-      if (mf && c->nphase >= 2 && !m.drift) {
+      // drift targets cost one more drift pass: only when they are used
+      const bool drift_w = m.online_lw[1] != 0.f || m.online_lw[2] != 0.f || m.online_lw[3] != 0.f;
+      if (mf && c->nphase >= 2 && !m.drift && (m.record || drift_w)) {
         SPHX_TRY(m.unew.alloc((size_t)c->cap * 3));
         float4* u = m.unew.p;
         SPHX_TRY(mf_drift_into(c, nullptr, u, u + c->cap, u + 2 * (size_t)c->cap));
