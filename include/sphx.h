@@ -343,6 +343,52 @@ void MfComputeCorrectionCUDA(void);
 void ComputeForceCUDA_ProjectU(void);
 
 /* ------------------------------------------------------------------------ */
+/* MPM transfer with Drucker-Prager plasticity                               */
+/* (SPH_Solver/fluids/MpmSolver.cu; node grid of ParamCarrier mpmXmin,       */
+/* mpmcellsize, mpmRes, node id GetMpmIdx :35-38 = (y*resz + z)*resx + x).   */
+/* ------------------------------------------------------------------------ */
+
+/* Allocate the node grid of the current params (MpmAllocateBufferCUDA
+ * host.cu:228).  kernel: 0 the reference's cubic B-spline (:155-195), 1 a
+ * quadratic B-spline.  physical: 0 reference gradients in cell units and
+ * node force sigma grad N / rho (:700); 1 gradients in metres and node force
+ * m_p / (rho_p m_I) sigma grad N.  Fails if an SPH grid cell is narrower
+ * than the spline support (the reference's node sums would miss
+ * particles). */
+int sphx_mpm_setup(sphx_context* ctx, int kernel, int physical);
+/* MpmParticleToGrid_APIC (apic=1, :823) / MpmParticleToGrid (apic=0, :197) */
+int sphx_mpm_p2g(sphx_context* ctx, int apic);
+int sphx_mpm_stress(sphx_context* ctx);          /* MpmParticleStress :477 */
+int sphx_mpm_node_update(sphx_context* ctx);     /* MpmNodeUpdate :658 */
+/* MpmParticleUpdate_APIC (apic=1, :891) / MpmParticleUpdate (apic=0, :706) */
+int sphx_mpm_g2p(sphx_context* ctx, int apic);
+/* RunMPM (fluid_system.cpp:541-564): sort, P2G, stress, node update, G2P */
+int sphx_mpm_substep(sphx_context* ctx);
+/* Node velocities [N][3] and masses [N] (either may be NULL). */
+int sphx_mpm_download_nodes(sphx_context* ctx, float* v, float* mass);
+/* Set node velocities and masses (marks every node block active). */
+int sphx_mpm_upload_nodes(sphx_context* ctx, const float* v, const float* mass);
+/* Number of 4^3-node blocks the last transfer touched. */
+int sphx_mpm_active_blocks(sphx_context* ctx, int* count);
+
+/* Legacy MPM entry points (fluid_system_host.cuh; host.cu:228-296). */
+void MpmAllocateBufferCUDA(sphx_ParamCarrier* params);
+void IndexMPMSortCUDA(void);
+void MpmParticleToGrid_APIC_CUDA(void);
+void MpmParticleStress_CUDA(void);
+void MpmNodeUpdate_CUDA(void);
+void MpmParticleUpdate_APIC_CUDA(void);
+void MpmParticleToGrid_CUDA(void);
+void MpmParticleUpdate_CUDA(void);
+
+/* SetupMPMGrid (fluid_system.cpp:1004-1025): node grid over the soft
+ * bounds +- 10 with cell 1.5 * pSpacing. */
+void sphx_setup_mpm_grid(sphx_ParamCarrier* p, float pSpacing);
+/* SetDruckerPragerParam (fluid_system.cpp:882-892): a_phi, a_psi (degrees)
+ * and k_c (cohesion) to the Drucker-Prager constants, in place. */
+void sphx_set_drucker_prager(sphx_ParamCarrier* p);
+
+/* ------------------------------------------------------------------------ */
 /* Learned warm start and drift closure.  No counterpart in the reference:  */
 /* a 24-64-64-4 ReLU MLP (6,020 weights; layout W1[24][64] b1[64]           */
 /* W2[64][64] b2[64] W3[64][4] b3[4]) evaluated per fluid particle after    */
@@ -445,6 +491,17 @@ int sphx_scene_bubbles(int n_fluid, float Lx, float Ly, float Lz, float fill, fl
                        int n_bubbles, float rmin, float rmax, int layers, uint64_t seed,
                        sphx_ParamCarrier* params, sphx_displayPack* disp,
                        sphx_calculationPack* calc, float* alpha, sphx_SceneInfo* info);
+
+/* Granular block for the MPM transfer: n particles uniform in [lo,hi]^3
+ * (metres) over a res^3 node grid of spacing dx from the origin; mass
+ * rho (hi-lo)^3 / n; G, K from E and nu (fluid_system.cpp:290-291);
+ * Drucker-Prager friction angle phi_deg, no cohesion or dilation;
+ * deformation gradients I + N(0, f_noise^2); soft walls 3 nodes inside the
+ * grid; SPH sort grid of cells 4 dx.  Returns n (arrays may be NULL). */
+int sphx_scene_sand(int n, float lo, float hi, float dx, int res, float E, float nu,
+                    float phi_deg, float rho, float f_noise, uint64_t seed,
+                    sphx_ParamCarrier* params, sphx_displayPack* disp,
+                    sphx_calculationPack* calc, sphx_SceneInfo* info);
 
 #ifdef __cplusplus
 }

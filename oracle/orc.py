@@ -261,3 +261,67 @@ def mlp_loss_grad(w, X, T, lw):
     L.orc_mlp_loss_grad.restype = C.c_double
     loss = L.orc_mlp_loss_grad(_dp(w), _dp(X), _dp(T), C.c_int(len(X)), _dp(lw), _dp(g))
     return loss, g
+
+
+# -- MPM transfer -----------------------------------------------------------
+class OrcMPM(C.Structure):
+    _fields_ = [("kernel", C.c_int), ("physical", C.c_int), ("node_v", C.c_void_p),
+                ("node_m", C.c_void_p), ("B", C.c_void_p), ("stress", C.c_void_p),
+                ("G", C.c_void_p), ("K", C.c_void_p)]
+
+
+class MPM:
+    """MPM arrays attached to an oracle State (same dtype); node arrays in
+    the y-major node order of the library."""
+
+    def __init__(self, st: State, params, kernel=0, physical=0, calc=None):
+        dt = st.dtype
+        N = params.mpmRes.x * params.mpmRes.y * params.mpmRes.z
+        self.st = st
+        self.a = dict(node_v=np.zeros((N, 3), dt), node_m=np.zeros(N, dt),
+                      B=np.zeros((st.n, 9), dt), stress=np.zeros((st.n, 9), dt),
+                      G=np.zeros(st.n, dt), K=np.zeros(st.n, dt))
+        if calc is not None:
+            self.a["B"][:] = calc["B"].reshape(st.n, 9)
+            self.a["stress"][:] = calc["stress"].reshape(st.n, 9)
+            self.a["G"][:] = calc["solidG"]
+            self.a["K"][:] = calc["solidK"]
+        self.m = OrcMPM()
+        self.m.kernel, self.m.physical = kernel, physical
+        for k in ["node_v", "node_m", "B", "stress", "G", "K"]:
+            setattr(self.m, k, self.a[k].ctypes.data)
+
+    def __getitem__(self, k):
+        return self.a[k]
+
+    def _call(self, name, *extra):
+        getattr(lib(), self.st.pfx + name)(C.byref(self.st.params), C.byref(self.st.s),
+                                           C.byref(self.m), *extra)
+
+    def p2g(self, apic=1): self._call("mpm_p2g", C.c_int(apic))
+    def stress(self): self._call("mpm_stress")
+    def node_update(self): self._call("mpm_node_update")
+    def g2p(self, apic=1): self._call("mpm_g2p", C.c_int(apic))
+
+
+def mpm_weight(kernel, x, dtype=np.float64):
+    L = lib()
+    f = L.orc_f64_mpm_weight if dtype == np.float64 else L.orc_f32_mpm_weight
+    f.restype = C.c_double
+    return f(C.c_int(kernel), C.c_double(x))
+
+
+def mpm_dweight(kernel, x, dtype=np.float64):
+    L = lib()
+    f = L.orc_f64_mpm_dweight if dtype == np.float64 else L.orc_f32_mpm_dweight
+    f.restype = C.c_double
+    return f(C.c_int(kernel), C.c_double(x))
+
+
+def drucker_prager(params, G, K, eps, sig):
+    out = np.zeros(9)
+    e = np.ascontiguousarray(eps, np.float64).reshape(9)
+    s = np.ascontiguousarray(sig, np.float64).reshape(9)
+    lib().orc_f64_mpm_drucker_prager(C.byref(params), C.c_double(G), C.c_double(K), _dp(e), _dp(s),
+                                     _dp(out))
+    return out.reshape(3, 3)

@@ -188,6 +188,20 @@ struct sphx_context {
   sphx::DevBuf<float4> mf_alpha_new, mf_drift[3], mf_amf;
   bool mf_pending = false;        // alpha_new holds the corrected fractions
 
+  // MPM transfer (particle-centric scatter into a dense node grid whose
+  // touched 4^3-node blocks are tracked in a list)
+  struct MPM {
+    bool on = false;
+    int kernel = 0;        // 0 cubic (reference), 1 quadratic
+    int physical = 0;      // 1: gradients in metres, momentum-conserving node force
+    bool loaded = false;   // hot B / sigma hold the particles' values
+    sphx::DevBuf<float> B[2], S[2];          // [9][cap] component-major, permuted
+    sphx::DevBuf<float4> acc, force, nodev;  // per node: {mv, m}, {f}, {v, m}
+    sphx::DevBuf<int> flags, list, count;    // per node block; active list
+    int nblk[3] = {0, 0, 0};
+    size_t nodes = 0, blocks = 0;
+  } mpm;
+
   // learned warm start / drift closure (24-64-64-4 MLP)
   struct MLP {
     bool ready = false;

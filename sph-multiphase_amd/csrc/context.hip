@@ -128,6 +128,9 @@ __global__ void k_gather(const int* __restrict__ perm, int n, const T* __restric
   for (int k = 0; k < width; ++k) out[(size_t)s * width + k] = in[src * width + k];
 }
 
+int mpm_sync_cold(sphx_context* c);
+void mpm_invalidate(sphx_context* c);
+
 static int materialize_cold(sphx_context* c) {
   int n = c->num;
   if (n == 0) return 0;
@@ -363,7 +366,9 @@ int sphx_upload(sphx_context* c, const sphx_displayPack* disp, const sphx_calcul
              "range exceeds capacity (maxNum)");
   SPHX_TRY(ensure_device(c));
   if (count == 0) return 0;
+  SPHX_TRY(mpm_sync_cold(c));   // particles not in this upload keep their MPM state
   SPHX_TRY(materialize_cold(c));
+  mpm_invalidate(c);
   c->nbr_valid = false;
   std::vector<float4> pos(count), vel(count), ve(count), acc(count), phys(count), color(count), X(count);
   std::vector<int> born(count);
@@ -419,6 +424,7 @@ int sphx_download(sphx_context* c, sphx_displayPack* disp, sphx_calculationPack*
   SPHX_HIP(hipStreamSynchronize(c->stream));
   int n = c->num;
   int b = c->cur;
+  SPHX_TRY(mpm_sync_cold(c));
   // gather cold fields into slot order on the device
   DevBuf<float4> color, X;
   DevBuf<float> mat;

@@ -294,6 +294,7 @@ __global__ void k_fill_i32(int* p, int n, int v) {
 
 int mf_reorder(sphx_context* c, int a, int b);
 int mlp_reorder(sphx_context* c, int a, int b);
+int mpm_reorder(sphx_context* c, int a, int b);
 
 int grid_hash(sphx_context* c) {
   int n = c->num;
@@ -375,6 +376,7 @@ int grid_reorder(sphx_context* c) {
   SPHX_HIP(hipGetLastError());
   SPHX_TRY(mf_reorder(c, a, b));
   SPHX_TRY(mlp_reorder(c, a, b));
+  SPHX_TRY(mpm_reorder(c, a, b));
   c->cur = b;
   return 0;
 }

@@ -113,6 +113,40 @@ void MfComputeAlphaAdvanceCUDA(void) { run("MfComputeAlphaAdvanceCUDA", sphx_mf_
 void MfComputeCorrectionCUDA(void) { run("MfComputeCorrectionCUDA", sphx_mf_correction); }
 void ComputeForceCUDA_ProjectU(void) { run("ComputeForceCUDA_ProjectU", sphx_mf_force); }
 
+// MPM (fluid_system_host.cu:228-296)
+void MpmAllocateBufferCUDA(sphx_ParamCarrier* params) {
+  if (!g_ctx) {
+    std::fprintf(stderr, "SPHX ERROR: MpmAllocateBufferCUDA: FluidSetupCUDA has not been called\n");
+    return;
+  }
+  int rc = params ? sphx_set_params(g_ctx, params) : 0;
+  if (rc == 0) {
+    g_params = *params;
+    rc = sphx_mpm_setup(g_ctx, 0, 0);
+  }
+  report("MpmAllocateBufferCUDA", rc);
+}
+// initMpm (MpmSolver.cu:39-73) stores node positions and SPH cells; node
+// positions are computed where they are used here, so this only checks
+// that the node grid exists.
+void IndexMPMSortCUDA(void) {
+  run("IndexMPMSortCUDA", [](sphx_context* c) { int n; return sphx_mpm_active_blocks(c, &n); });
+}
+void MpmParticleToGrid_APIC_CUDA(void) {
+  run("MpmParticleToGrid_APIC_CUDA", [](sphx_context* c) { return sphx_mpm_p2g(c, 1); });
+}
+void MpmParticleToGrid_CUDA(void) {
+  run("MpmParticleToGrid_CUDA", [](sphx_context* c) { return sphx_mpm_p2g(c, 0); });
+}
+void MpmParticleStress_CUDA(void) { run("MpmParticleStress_CUDA", sphx_mpm_stress); }
+void MpmNodeUpdate_CUDA(void) { run("MpmNodeUpdate_CUDA", sphx_mpm_node_update); }
+void MpmParticleUpdate_APIC_CUDA(void) {
+  run("MpmParticleUpdate_APIC_CUDA", [](sphx_context* c) { return sphx_mpm_g2p(c, 1); });
+}
+void MpmParticleUpdate_CUDA(void) {
+  run("MpmParticleUpdate_CUDA", [](sphx_context* c) { return sphx_mpm_g2p(c, 0); });
+}
+
 void sphx_legacy_upload(const sphx_displayPack* disp, const sphx_calculationPack* calc, int start,
                         int end) {
   if (!g_ctx) {

@@ -317,3 +317,98 @@ int sphx_scene_bubbles(int n_fluid, float Lx, float Ly, float Lz, float fill, fl
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// SetupMPMGrid (fluid_system.cpp:1004-1025), in float as cfloat3 / cint3:
+// the node count per axis is truncated to int (cint3::Set(int,...)).
+void sphx_setup_mpm_grid(sphx_ParamCarrier* p, float pSpacing) {
+  sphx_float3 lo = {p->softminx.x - 10.f, p->softminx.y - 10.f, p->softminx.z - 10.f};
+  sphx_float3 hi = {p->softmaxx.x + 10.f, p->softmaxx.y + 10.f, p->softmaxx.z + 10.f};
+  float cs = (float)(pSpacing * 1.5);
+  float tx = (hi.x - lo.x) / cs + 2.f, ty = (hi.y - lo.y) / cs + 2.f, tz = (hi.z - lo.z) / cs + 2.f;
+  sphx_int3 res = {(int)tx, (int)ty, (int)tz};
+  p->mpmXmin = lo;
+  p->mpmXmax = {lo.x + (float)(res.x - 1) * cs, lo.y + (float)(res.y - 1) * cs,
+                lo.z + (float)(res.z - 1) * cs};
+  p->mpmcellsize = cs;
+  p->mpmRes = res;
+  p->mpmNodeNum = res.x * res.y * res.z;
+}
+
+// SetDruckerPragerParam (fluid_system.cpp:882-892): friction and dilation
+// angles in degrees, cohesion in k_c.
+void sphx_set_drucker_prager(sphx_ParamCarrier* p) {
+  float phi = p->a_phi * (3.141593f / 180.0f);
+  float psi = p->a_psi * (3.141593f / 180.0f);
+  float kc = p->k_c;
+  p->a_phi = (float)(2 * std::sin(phi) / std::sqrt(3) / (3 - std::sin(phi)));
+  p->a_psi = (float)(2 * std::sin(psi) / std::sqrt(3) / (3 - std::sin(psi)));
+  p->k_c = (float)(3 * kc * std::cos(psi) / std::sqrt(3) / (3 - std::sin(phi)));
+}
+
+// Granular block for the MPM transfer: n particles uniform in the cube
+// [lo, hi]^3 (metres, simscale 1) over a res^3 node grid of spacing dx from
+// the origin; particle mass rho (hi-lo)^3 / n; G = E/(2(1+nu)),
+// K = E/(3(1-2nu)) (fluid_system.cpp:290-291); Drucker-Prager friction
+// angle phi (degrees), no cohesion or dilation; deformation gradients
+// I + N(0, f_noise^2).  Soft walls 3 nodes inside the grid.  The SPH sort
+// grid has cells of 4 dx (h = 2 dx).  Arrays may be NULL to query sizes.
+int sphx_scene_sand(int n, float lo, float hi, float dx, int res, float E, float nu,
+                    float phi_deg, float rho, float f_noise, uint64_t seed,
+                    sphx_ParamCarrier* p, sphx_displayPack* disp, sphx_calculationPack* calc,
+                    sphx_SceneInfo* info) {
+  if (n <= 0 || !(hi > lo) || !(dx > 0.f) || res < 8) return -1;
+  if (info) {
+    info->n_fluid = n;
+    info->n_boundary = 0;
+    info->n_total = n;
+    info->n_phase = 1;
+    info->spacing = std::cbrt((hi - lo) * (hi - lo) * (hi - lo) / (float)n);
+  }
+  if (!p) return n;
+  std::memset(p, 0, sizeof(*p));
+  p->simscale = 1.f;
+  p->dt = 5e-5f;
+  p->gravity = {0.f, -9.81f, 0.f};
+  p->restdensity = rho;
+  p->mass = rho * (hi - lo) * (hi - lo) * (hi - lo) / (float)n;
+  p->smoothradius = 2.f * dx;
+  sphx_setup_kernels(p);
+  float cell = p->cellsize / p->simscale;
+  float ext = (float)res * dx;
+  p->volmin = {-cell, -cell, -cell};
+  p->volmax = {ext + cell, ext + cell, ext + cell};
+  sphx_setup_grid(p);
+  p->softminx = {3.f * dx, 3.f * dx, 3.f * dx};
+  p->softmaxx = {(float)(res - 4) * dx, (float)(res - 4) * dx, (float)(res - 4) * dx};
+  p->mpmXmin = {0.f, 0.f, 0.f};
+  p->mpmcellsize = dx;
+  p->mpmRes = {res, res, res};
+  p->mpmXmax = {(float)(res - 1) * dx, (float)(res - 1) * dx, (float)(res - 1) * dx};
+  p->mpmNodeNum = res * res * res;
+  p->solidG = E / 2 / (1 + nu);
+  p->solidK = E / 3 / (1 - 2 * nu);
+  p->a_phi = phi_deg;
+  p->a_psi = 0.f;
+  p->k_c = 0.f;
+  sphx_set_drucker_prager(p);
+  p->num = n;
+  p->maxNum = n;
+  if (!disp || !calc) return n;
+  Rng rng(seed);
+  for (int i = 0; i < n; ++i) {
+    float x = lo + (hi - lo) * rng.uniform();
+    float y = lo + (hi - lo) * rng.uniform();
+    float z = lo + (hi - lo) * rng.uniform();
+    init_particle(disp[i], calc[i], x, y, z, SPHX_TYPE_GRANULAR, p->mass, rho, 0.f, i);
+    calc[i].dens = rho;
+    calc[i].solidG = p->solidG;
+    calc[i].solidK = p->solidK;
+    for (int k = 0; k < 9; ++k)
+      calc[i].deformGrad.data[k] = (k % 4 == 0 ? 1.f : 0.f) + (float)(f_noise * rng.normal());
+  }
+  return n;
+}
+
+}  // extern "C"

@@ -168,6 +168,21 @@ def lib() -> C.CDLL:
         L.sphx_mlp_grad.argtypes = [VP, VP, VP, VP, I, VP, VP, P(D)]
         L.sphx_mlp_train.argtypes = [VP, I, I, C.c_float, VP, C.c_uint64, VP]
         L.sphx_mlp_stats.argtypes = [VP, P(I64), P(I64), P(I)]
+        L.sphx_mpm_setup.argtypes = [VP, I, I]
+        L.sphx_mpm_p2g.argtypes = [VP, I]
+        L.sphx_mpm_g2p.argtypes = [VP, I]
+        for name in ["sphx_mpm_stress", "sphx_mpm_node_update", "sphx_mpm_substep"]:
+            getattr(L, name).argtypes = [VP]
+        L.sphx_mpm_download_nodes.argtypes = [VP, VP, VP]
+        L.sphx_mpm_upload_nodes.argtypes = [VP, VP, VP]
+        L.sphx_mpm_active_blocks.argtypes = [VP, P(I)]
+        L.sphx_setup_mpm_grid.argtypes = [P(ParamCarrier), C.c_float]
+        L.sphx_setup_mpm_grid.restype = None
+        L.sphx_set_drucker_prager.argtypes = [P(ParamCarrier)]
+        L.sphx_set_drucker_prager.restype = None
+        L.sphx_scene_sand.argtypes = [I, C.c_float, C.c_float, C.c_float, I, C.c_float, C.c_float,
+                                      C.c_float, C.c_float, C.c_float, C.c_uint64,
+                                      P(ParamCarrier), VP, VP, P(SceneInfo)]
         _lib = L
     return _lib
 
@@ -225,6 +240,21 @@ def scene_bubbles(n_fluid=1 << 20, L=(1.0, 1.0, 1.0), fill=0.8, h=0.01, n_bubble
 
 
 # ----------------------------------------------------------------- context --
+def scene_sand(n=4096, lo=0.3, hi=0.6, dx=1.0 / 32, res=32, E=3.5e5, nu=0.3, phi=30.0,
+               rho=1600.0, f_noise=1e-3, seed=20261205):
+    """Granular block over a res^3 MPM node grid (sphx_scene_sand)."""
+    L = lib()
+    p = ParamCarrier()
+    info = SceneInfo()
+    disp = np.zeros(n, display_dtype)
+    calc = np.zeros(n, calc_dtype)
+    rc = L.sphx_scene_sand(n, lo, hi, dx, res, E, nu, phi, rho, f_noise, seed, C.byref(p),
+                           ptr(disp), ptr(calc), C.byref(info))
+    if rc < 0:
+        raise SphxError("sphx_scene_sand: bad arguments")
+    return p, disp, calc, info
+
+
 class Context:
     """Owns one sphx_context (device buffers for params.maxNum particles)."""
 
@@ -426,6 +456,35 @@ class Context:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int()
         check(self.L.sphx_mlp_stats(self.h, C.byref(a), C.byref(b), C.byref(c)), "sphx_mlp_stats")
         return dict(learned=a.value, fallbacks=b.value, last_fallback=c.value)
+
+    # -- MPM --
+    def mpm_setup(self, kernel=0, physical=0):
+        check(self.L.sphx_mpm_setup(self.h, kernel, physical), "sphx_mpm_setup")
+        pr = self.params
+        self.mpm_res = (pr.mpmRes.x, pr.mpmRes.y, pr.mpmRes.z)
+
+    def mpm_p2g(self, apic=1):
+        check(self.L.sphx_mpm_p2g(self.h, apic), "sphx_mpm_p2g")
+
+    def mpm_g2p(self, apic=1):
+        check(self.L.sphx_mpm_g2p(self.h, apic), "sphx_mpm_g2p")
+
+    def mpm_nodes(self):
+        N = int(np.prod(self.mpm_res))
+        v = np.zeros((N, 3), np.float32)
+        m = np.zeros(N, np.float32)
+        check(self.L.sphx_mpm_download_nodes(self.h, ptr(v), ptr(m)), "sphx_mpm_download_nodes")
+        return v, m
+
+    def mpm_set_nodes(self, v, m):
+        v = np.ascontiguousarray(v, np.float32)
+        m = np.ascontiguousarray(m, np.float32)
+        check(self.L.sphx_mpm_upload_nodes(self.h, ptr(v), ptr(m)), "sphx_mpm_upload_nodes")
+
+    def mpm_active_blocks(self):
+        n = C.c_int()
+        check(self.L.sphx_mpm_active_blocks(self.h, C.byref(n)), "sphx_mpm_active_blocks")
+        return n.value
 
     def timing(self, on=True):
         check(self.L.sphx_timing_enable(self.h, 1 if on else 0), "timing")

@@ -1,0 +1,144 @@
+"""Pins on the MPM oracle (no GPU): B-spline identities, P2G conservation,
+APIC reproduction of affine node velocity fields, and the Drucker-Prager
+return mapping's consistency condition."""
+import numpy as np
+import pytest
+
+import orc
+import sphx
+
+UNDEF = 0xFFFFFFFF
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_spline_partition_of_unity_and_moments(kernel):
+    """sum_k N(x - k) = 1, sum_k N(x - k)(k - x) = 0 and sum_k N(x - k)(k - x)^2
+    = 1/3 (cubic) or 1/4 (quadratic) for every x; the reference's cubic uses
+    truncated constants (1.33333, 0.666666), so it holds to 1e-5."""
+    tol = 1e-5 if kernel == 0 else 1e-12
+    m2 = 1.0 / 3.0 if kernel == 0 else 0.25
+    for x in np.linspace(0.0, 1.0, 37):
+        ks = np.arange(-3, 5)
+        w = np.array([orc.mpm_weight(kernel, x - k) for k in ks])
+        assert abs(w.sum() - 1.0) < tol
+        assert abs((w * (ks - x)).sum()) < tol
+        assert abs((w * (ks - x) ** 2).sum() - m2) < 4 * tol
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_spline_derivative_is_derivative(kernel):
+    eps = 1e-6
+    for x in np.linspace(-2.2, 2.2, 89):
+        if any(abs(abs(x) - b) < 1e-3 for b in (0.0, 0.5, 1.0, 1.5, 2.0)):
+            continue
+        fd = (orc.mpm_weight(kernel, x + eps) - orc.mpm_weight(kernel, x - eps)) / (2 * eps)
+        assert abs(fd - orc.mpm_dweight(kernel, x)) < 2e-5, x
+
+
+def sand_state(kernel, n=1500, seed=3):
+    p, disp, calc, info = sphx.scene_sand(n=n, seed=seed)
+    st = orc.State(disp, calc, p, np.float64)
+    st.grid_build()
+    rng = np.random.default_rng(seed)
+    st["vel"][:] = rng.normal(0, 0.3, (st.n, 3))
+    mp = orc.MPM(st, p, kernel, 0, calc)
+    mp["B"][:] = rng.normal(0, 0.01, (st.n, 9))
+    return p, st, mp
+
+
+def node_positions(p):
+    res = (p.mpmRes.x, p.mpmRes.y, p.mpmRes.z)
+    y, z, x = np.meshgrid(np.arange(res[1]), np.arange(res[2]), np.arange(res[0]), indexing="ij")
+    xs = np.stack([x.ravel(), y.ravel(), z.ravel()], 1).astype(np.float64)
+    return np.array([p.mpmXmin.x, p.mpmXmin.y, p.mpmXmin.z]) + xs * p.mpmcellsize
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_p2g_conserves_mass_and_momentum(kernel):
+    p, st, mp = sand_state(kernel)
+    fl = (st["type"] != 1) & (st["mgcell"] != UNDEF)
+    mass = st["mass"][fl]
+    tol = 1e-5 if kernel == 0 else 1e-12
+    mp.p2g(apic=0)
+    assert abs(mp["node_m"].sum() - mass.sum()) < tol * mass.sum()
+    mom_n = (mp["node_v"] * mp["node_m"][:, None]).sum(0)
+    mom_p = (st["vel"][fl] * mass[:, None]).sum(0)
+    np.testing.assert_allclose(mom_n, mom_p, atol=tol * np.abs(mom_p).max() * 10)
+    # APIC: the affine term has zero first moment, so momentum is unchanged
+    mp.p2g(apic=1)
+    mom_a = (mp["node_v"] * mp["node_m"][:, None]).sum(0)
+    np.testing.assert_allclose(mom_a, mom_p, atol=tol * np.abs(mom_p).max() * 10)
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_g2p_reproduces_affine_node_velocity(kernel):
+    """Node velocities v_I = A x_I + b: G2P gives v_p = A x_p + b and
+    B_p = A D with D = dx^2/3 (cubic) or dx^2/4 (quadratic)."""
+    p, st, mp = sand_state(kernel)
+    rng = np.random.default_rng(8)
+    A = rng.normal(0, 1, (3, 3))
+    b = rng.normal(0, 1, 3)
+    xn = node_positions(p)
+    mp["node_v"][:] = xn @ A.T + b
+    mp["node_m"][:] = 1.0
+    x0 = st["pos"].copy()
+    p.dt = 0.0
+    mp.g2p(apic=1)
+    tol = 2e-5 if kernel == 0 else 1e-10
+    np.testing.assert_allclose(st["vel"], x0 @ A.T + b, atol=tol * 10)
+    D = p.mpmcellsize ** 2 / (3.0 if kernel == 0 else 4.0)
+    # B = A D + (A x_p + b) (sum_I w (x_I - x_p))^T: the second term is the
+    # first-moment defect of the truncated cubic constants (~1e-5 dx)
+    vmax = np.abs(x0 @ A.T + b).max()
+    atol_B = 3e-5 * p.mpmcellsize * vmax if kernel == 0 else 1e-10 * D
+    np.testing.assert_allclose(mp["B"].reshape(-1, 3, 3), np.broadcast_to(A * D, (st.n, 3, 3)),
+                               atol=atol_B)
+    np.testing.assert_array_equal(st["pos"], x0)   # dt = 0
+
+
+def dp_params():
+    p, _, _, _ = sphx.scene_sand(n=8)
+    return p
+
+
+def test_drucker_prager_constants():
+    p = dp_params()
+    s = np.sin(np.radians(30.0))
+    assert p.a_phi == pytest.approx(2 * s / np.sqrt(3) / (3 - s), rel=1e-6)
+    assert p.a_psi == 0.0 and p.k_c == 0.0
+
+
+def test_drucker_prager_hydrostatic_compression_does_not_yield():
+    p = dp_params()
+    sig = -1000.0 * np.eye(3)
+    eps = np.diag([-1.0, -2.0, -0.5])
+    assert np.all(orc.drucker_prager(p, 1e5, 2e5, eps, sig) == 0)
+
+
+def test_drucker_prager_consistency_condition():
+    """On the yield surface (f = 0) the plastic correction keeps the trial
+    stress rate tangent to it: f(sigma + dt (C eps - plastic)) = O(dt^2),
+    while the elastic rate alone leaves it at O(dt)."""
+    p = dp_params()
+    G, K = 134615.0, 291667.0
+    a = p.a_phi
+    # pure shear plus pressure, just outside the surface f = sqrt(J2) + a I1
+    s_dev = np.array([[0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    j2 = np.sqrt(0.5 * (s_dev ** 2).sum())
+    pr = j2 / (3 * a) * (1 - 1e-9)
+    sig = s_dev - pr * np.eye(3)
+
+    def f(sg):
+        d = sg - np.trace(sg) / 3 * np.eye(3)
+        return np.sqrt(0.5 * (d ** 2).sum()) + a * np.trace(sg)
+
+    assert 0 < f(sig) < 1e-8
+    eps = np.array([[0.0, 0.3, 0.0], [0.3, 0.0, 0.0], [0.0, 0.0, 0.0]])   # loading shear
+    C_eps = 2 * G * eps + (K - 2 * G / 3) * np.trace(eps) * np.eye(3)
+    pl = orc.drucker_prager(p, G, K, eps, sig)
+    assert np.abs(pl).max() > 0
+    dt = 1e-7
+    f_el = f(sig + dt * C_eps)
+    f_pl = f(sig + dt * (C_eps - pl))
+    assert f_el > 1e-3 * dt * G
+    assert abs(f_pl) < 1e-6 * abs(f_el)
