@@ -141,6 +141,123 @@ def cpu_baseline(sample_fluid=131072, mean_iters=10.0, timed_iters=20):
                       f"iterations/step (the GPU run's mean)"}
 
 
+MPM_ALGO_BYTES = {
+    "mpm_p2g": 100,      # x, v, m, B (84) -> node {mv, m} (16 per node, ~1 node/particle)
+    "mpm_stress": 120,   # x, phys, sigma r/w, G/K, perm (node gathers served by L2)
+    "mpm_force": 88,     # x, phys, sigma, key -> node force (16)
+    "mpm_g2p": 100,      # x r/w, v r/w, B w
+    "mpm_node": 48,      # per active node: {v, m} r/w, force r
+    "mpm_normalize": 32, # per active node: acc r, {v, m} w
+    "mpm_clear": 32,     # per active node: acc, force w
+    "reorder": 104,
+}
+
+
+def bench_sand(args, sphx):
+    """MPM sand: 2^21 particles uniform in a 0.5 m cube inside [0,1]^3 over
+    a 256^3 node grid (dx = 1/256, one particle per cell), quadratic
+    B-spline, APIC, rate-form stress with Drucker-Prager (E = 3.5e5,
+    nu = 0.3, phi = 30 deg), dt = 5e-5; a step is one substep of
+    RunMPM (sort, P2G, stress, node update, G2P)."""
+    n = args.n if args.n != (1 << 20) else (1 << 21)
+    p, disp, calc, info = sphx.scene_sand(n=n, lo=0.25, hi=0.75, dx=1.0 / 256, res=256)
+    p.dt = 5e-5
+    print(f"# sand: {n} particles, grid {p.mpmRes.x}^3, sort grid {p.gridres.x}^3",
+          file=sys.stderr, flush=True)
+    ctx = sphx.Context(p)
+    ctx.upload(disp, calc)
+    ctx.mpm_setup(1, 1)
+    ctx.call("grid_build")
+    for _ in range(args.warmup):
+        ctx.call("mpm_substep")
+    ctx.sync()
+    names = ["hash", "radix_sort", "cell_start", "reorder", "mpm_clear", "mpm_p2g",
+             "mpm_normalize", "mpm_stress", "mpm_force", "mpm_node", "mpm_g2p"]
+    ctx.timing(True)
+    ctx.timing_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.call("mpm_substep")
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    kt = {k: ctx.timing_get(k) for k in names}
+    active = ctx.mpm_active_blocks()
+    d, c = ctx.download()
+    ok = bool(np.all(np.isfinite(d["pos"])) and np.all(np.isfinite(c["stress"])))
+    ctx.close()
+    total_k = sum(t for t, _ in kt.values())
+    for k in names:
+        t, cnt = kt[k]
+        if cnt:
+            print(f"# {k:16s} {t / cnt * 1e3:9.1f} us/launch x {cnt:5d}  ({100 * t / total_k:5.1f}% of kernel time)",
+                  file=sys.stderr)
+    nodes_active = active * 64
+    dom = max(kt, key=lambda k: kt[k][0])
+    t_dom, n_dom = kt[dom]
+    avg_ms = t_dom / max(n_dom, 1)
+    units = nodes_active if dom in ("mpm_node", "mpm_normalize", "mpm_clear") else n
+    algo = MPM_ALGO_BYTES.get(dom, 0) * units
+    achieved = algo / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline_sand(sphx)
+        except Exception as e:
+            print(f"# cpu baseline failed: {e}", file=sys.stderr)
+    line = {
+        "metric": "particle-substeps/s, MPM sand (2M particles, 256^3 grid)",
+        "value": n * args.steps / elapsed,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "MPM sand block: quadratic B-spline APIC, Drucker-Prager, "
+                               "LDS-tile scatter", "n": n, "grid": 256, "dt": 5e-5,
+                   "active_node_blocks": active, "finite": ok},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": algo, "avg_launch_ms": avg_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_sand(sphx, sample=131072, substeps=2):
+    """The oracle's FP32 MPM stages (single-threaded) on a 131072-particle
+    cube of the same spacing and material over a 128^3 grid."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import orc
+    side = 0.5 * (sample / (1 << 21)) ** (1.0 / 3.0)
+    p, disp, calc, info = sphx.scene_sand(n=sample, lo=0.25 - side / 2, hi=0.25 + side / 2,
+                                          dx=1.0 / 256, res=128)
+    p.dt = 5e-5
+    st = orc.State(disp, calc, p, np.float32)
+    mp = orc.MPM(st, p, 1, 1, calc)
+    st.grid_build()
+    t0 = time.perf_counter()
+    for _ in range(substeps):
+        st.grid_build()
+        ok = st["mgcell"] != 0xFFFFFFFF
+        src = np.where(ok, st["midsort"], np.arange(st.n))
+        for k in ("B", "stress", "G", "K"):
+            mp.a[k][:] = mp.a[k][src]
+        mp.p2g(1)
+        mp.stress()
+        mp.node_update()
+        mp.g2p(1)
+    t = (time.perf_counter() - t0) / substeps
+    return {"value": sample / t, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"{substeps} substeps of a {sample}-particle cube (same dx, material, "
+                      f"dt) over a 128^3 grid, FP32 oracle MPM stages single-threaded "
+                      f"({t:.2f} s per substep)"}
+
+
 def timed_segment(ctx, steps, names):
     """`steps` timed steps between two device synchronisations; returns the
     wall time, per-step iterations and errors, and per-kernel event times."""
@@ -178,6 +295,8 @@ def main():
     ap.add_argument("--online-iters", type=int, default=16)
     ap.add_argument("--online-batch", type=int, default=8192)
     ap.add_argument("--online-lr", type=float, default=5e-4)
+    ap.add_argument("--workload", default="bubbles", choices=["bubbles", "sand"],
+                    help="bubbles: two-phase tank (headline); sand: MPM 2M particles")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
@@ -187,6 +306,12 @@ def main():
     if sphx.device_count() < 1:
         print("no HIP device visible: bench.py needs an MI355X", file=sys.stderr)
         sys.exit(1)
+
+    if args.workload == "sand":
+        if args.steps == 200:
+            args.steps = 100
+        bench_sand(args, sphx)
+        return
 
     p, disp, calc, alpha, info = build_scene(sphx, args.n)
     o = options(sphx)
