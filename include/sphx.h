@@ -197,6 +197,12 @@ typedef struct sphx_SolverOptions {
 const char* sphx_last_error(void);
 /* Number of HIP devices visible (0 without a GPU; never fails). */
 int sphx_device_count(void);
+/* Device buffers for callers of the entry points that take device pointers
+ * (getMobU, getMobU_walkthrough, sphx_rpy_apply_device); kind: 1 host to
+ * device, 2 device to host, 3 device to device. */
+int sphx_dev_alloc(void** out, size_t bytes);
+int sphx_dev_free(void* p);
+int sphx_dev_copy(void* dst, const void* src, size_t bytes, int kind);
 /* Library version string. */
 const char* sphx_version(void);
 
@@ -387,6 +393,46 @@ void sphx_setup_mpm_grid(sphx_ParamCarrier* p, float pSpacing);
 /* SetDruckerPragerParam (fluid_system.cpp:882-892): a_phi, a_psi (degrees)
  * and k_c (cohesion) to the Drucker-Prager constants, in place. */
 void sphx_set_drucker_prager(sphx_ParamCarrier* p);
+
+/* ------------------------------------------------------------------------ */
+/* Stokesian mobility (SPH_Solver/stokesian/Mobility.cu)                     */
+/* ------------------------------------------------------------------------ */
+
+/* Polydisperse Rotne-Prager-Yamakawa mobility applied on MFMA:
+ * U = M F for F, U [3n][nrhs] (row 3p + a), M_ii = I/(6 pi mu a_i),
+ * M_ij = (1/(8 pi mu r)) [(1 + (a_i^2+a_j^2)/(3r^2)) I + (1 - (a_i^2+a_j^2)/r^2) rr]
+ * (non-overlapping spheres); its equal-radius limit times 6 pi mu a is
+ * PairwiseMobMatrix's A1N (:37-82).  No counterpart in the reference for
+ * several right-hand sides or unequal radii. */
+typedef struct sphx_rpy sphx_rpy;
+int sphx_rpy_create(sphx_rpy** out, int n, int nrhs);   /* nrhs 16, 32, 48, 64 */
+int sphx_rpy_destroy(sphx_rpy* h);
+int sphx_rpy_set_spheres(sphx_rpy* h, const float* pos, const float* radius, float mu);
+/* Reduction segments per 64-sphere row tile (autotuned; default 16). */
+int sphx_rpy_set_split(sphx_rpy* h, int split);
+/* mode 1: bf16; 3: bf16x3 (hi*hi + hi*lo + lo*hi); 6: bf16x3 plus the
+ * second-order split terms.  Host arrays, synchronous. */
+int sphx_rpy_apply(sphx_rpy* h, const float* F, float* U, int mode);
+/* Same on device arrays, asynchronous on the handle's stream. */
+int sphx_rpy_apply_device(sphx_rpy* h, const float* dF, float* dU, int mode);
+int sphx_rpy_sync(sphx_rpy* h);
+/* The reference walkthrough step on device buffers (equal spheres a = 1,
+ * units 1/(6 pi mu a)): near-field force added to cuForce when with_force
+ * (Kern_getForce :257-300), background velocity interpolated from the MPM
+ * nodes of `nodes` or zero when NULL (Kern_interpolateVelocity :322-366),
+ * U = U_bg + F + sum A1N F and Omega = sum B1N F, x += U dt
+ * (Kern_getmobu_walkthrough :207-255; positions double-buffered, A10). */
+int sphx_rpy_walkthrough(sphx_context* nodes, const sphx_stokesianBufList* buf, int n,
+                         int with_force);
+/* MVmultiply / getMobU (:85-109): u = mat f, mat (6n x 6n) row-major,
+ * device pointers. */
+int sphx_mob_gemv(const float* mat, const float* f, float* u, int nsize);
+
+/* Legacy mobility entry points (Mobility.cuh:19-29; C aliases of the C++
+ * declarations, which are exported under their C++ names as well). */
+void setSignature(void);
+void getMobU(float* mat, float* f, float* u, int nsize);
+void getMobU_walkthrough(sphx_stokesianBufList buflist, int pnum);
 
 /* ------------------------------------------------------------------------ */
 /* Learned warm start and drift closure.  No counterpart in the reference:  */

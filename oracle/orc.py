@@ -325,3 +325,35 @@ def drucker_prager(params, G, K, eps, sig):
     lib().orc_f64_mpm_drucker_prager(C.byref(params), C.c_double(G), C.c_double(K), _dp(e), _dp(s),
                                      _dp(out))
     return out.reshape(3, 3)
+
+
+# -- Stokesian mobility -------------------------------------------------------
+def rpy_pair_ref(d):
+    """PairwiseMobMatrix's A1N and B1N (equal spheres a = 1) for d = x_k - x_i."""
+    d = np.ascontiguousarray(d, np.float64)
+    A = np.zeros(9)
+    B = np.zeros(9)
+    lib().orc_rpy_pair_ref(_dp(d), _dp(A), _dp(B))
+    return A.reshape(3, 3), B.reshape(3, 3)
+
+
+def rpy_walkthrough(pos, force, ubg, dt):
+    pos = np.ascontiguousarray(pos, np.float64)
+    force = np.ascontiguousarray(force, np.float64)
+    u = np.ascontiguousarray(ubg, np.float64).copy()
+    w = np.zeros_like(u)
+    pn = np.zeros_like(pos)
+    lib().orc_rpy_walkthrough(C.c_int(len(pos)), _dp(pos), _dp(force), C.c_double(dt), _dp(u),
+                              _dp(w), _dp(pn))
+    return u, w, pn
+
+
+def rpy_apply(pos, rad, mu, F):
+    """Polydisperse RPY U = M F in FP64 (F [3n][K])."""
+    pos = np.ascontiguousarray(pos, np.float64)
+    rad = np.ascontiguousarray(rad, np.float64)
+    F = np.ascontiguousarray(F, np.float64)
+    U = np.zeros_like(F)
+    lib().orc_rpy_apply(C.c_int(len(pos)), _dp(pos), _dp(rad), C.c_double(mu), _dp(F),
+                        C.c_int(F.shape[1]), _dp(U))
+    return U

@@ -205,6 +205,29 @@ int sphx_device_count(void) {
 
 const char* sphx_version(void) { return "sphx 0.1 (gfx950)"; }
 
+// Device buffers for callers of the entry points that take device pointers
+// (the reference host allocated them with cudaMalloc; an FFI host has no
+// other way to).  kind: 1 host->device, 2 device->host, 3 device->device.
+int sphx_dev_alloc(void** out, size_t bytes) {
+  SPHX_CHECK(out, "sphx_dev_alloc", "null argument");
+  *out = nullptr;
+  if (bytes == 0) return 0;
+  SPHX_HIP(hipMalloc(out, bytes));
+  return 0;
+}
+int sphx_dev_free(void* p) {
+  if (p) SPHX_HIP(hipFree(p));
+  return 0;
+}
+int sphx_dev_copy(void* dst, const void* src, size_t bytes, int kind) {
+  SPHX_CHECK(dst && src, "sphx_dev_copy", "null argument");
+  SPHX_CHECK(kind >= 1 && kind <= 3, "sphx_dev_copy", "kind must be 1, 2 or 3");
+  hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice
+                              : (kind == 2 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+  SPHX_HIP(hipMemcpy(dst, src, bytes, k));
+  return 0;
+}
+
 void sphx_default_options(sphx_SolverOptions* o) {
   std::memset(o, 0, sizeof(*o));
   o->omega = 0.1f;

@@ -147,6 +147,22 @@ void MpmParticleUpdate_CUDA(void) {
   run("MpmParticleUpdate_CUDA", [](sphx_context* c) { return sphx_mpm_g2p(c, 0); });
 }
 
+// Mobility (Mobility.cuh:19-29).  The antisymmetric tensor of
+// setSignature (:20-35) is written out inside the kernels.
+void setSignature(void) {}
+void getMobU(float* mat, float* f, float* u, int nsize) {
+  report("getMobU", sphx_mob_gemv(mat, f, u, nsize));
+}
+void getMobU_walkthrough(sphx_stokesianBufList buflist, int pnum) {
+  report("getMobU_walkthrough", sphx_rpy_walkthrough(g_ctx, &buflist, pnum, 1));
+}
+
+void sphx_legacy_setsignature(void) { setSignature(); }
+void sphx_legacy_getmobu(float* mat, float* f, float* u, int nsize) { getMobU(mat, f, u, nsize); }
+void sphx_legacy_walkthrough(const void* buflist, int pnum) {
+  getMobU_walkthrough(*static_cast<const sphx_stokesianBufList*>(buflist), pnum);
+}
+
 void sphx_legacy_upload(const sphx_displayPack* disp, const sphx_calculationPack* calc, int start,
                         int end) {
   if (!g_ctx) {

@@ -1,0 +1,517 @@
+// Stokesian mobility of spheres (SPH_Solver/stokesian/Mobility.cu).
+//
+// The polydisperse Rotne-Prager-Yamakawa mobility is applied to K
+// right-hand sides, U = M F (M is 3N x 3N, row 3p + a), without storing M:
+// each workgroup generates 64 x 32-sphere tiles of the 3x3 pair blocks on
+// the VALU, splits them into bf16 pieces in LDS and multiplies them against
+// the bf16 pieces of F on MFMA (v_mfma_f32_16x16x32_bf16, FP32
+// accumulation).  The dominant self term M_ii = I/(6 pi mu a_i) is applied
+// in FP32 outside the MFMA.  Precision modes:
+//   1  hi*hi                                  (plain bf16)
+//   3  hi*hi + hi*lo + lo*hi                  (bf16x3)
+//   6  bf16x3 + lo*lo + hi*F3 + M3*hi         (second-order split terms)
+// where x = hi + lo + x3 with each piece bf16 (round to nearest even).
+// The reduction over spheres is split over `split` workgroups per row tile
+// (partials in FP32, summed by a second kernel) so N = 4096 fills the chip.
+//
+// Also here: the reference's equal-sphere walkthrough (Kern_getForce
+// :257-300, Kern_interpolateVelocity :322-366, Kern_getmobu_walkthrough
+// :207-255) and the dense product getMobU (:85-109).
+#include "common.hpp"
+
+namespace sphx {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+constexpr int RPY_TP = 64;     // spheres per row tile (4 waves x 16)
+constexpr int RPY_TQ = 32;     // spheres per reduction chunk (MFMA K)
+constexpr int RPY_THREADS = 256;
+constexpr float RPY_PI = 3.14159265358979f;
+
+// round to nearest even to bf16 (finite inputs)
+__host__ __device__ __forceinline__ uint16_t bf16_bits(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf16_float(uint16_t b) {
+  uint32_t u = (uint32_t)b << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+// x = hi + lo + r3 (three bf16 pieces)
+__device__ __forceinline__ void split3(float x, uint16_t& h, uint16_t& l, uint16_t& r) {
+  h = bf16_bits(x);
+  float d = x - bf16_float(h);
+  l = bf16_bits(d);
+  r = bf16_bits(d - bf16_float(l));
+}
+
+// F [3n][K] FP32 row-major -> pieces FT[b][c][q] (sphere q contiguous),
+// zero-padded to np spheres.
+__global__ void k_rpy_split_f(const float* __restrict__ F, int n, int np, int K,
+                              uint16_t* __restrict__ Fh, uint16_t* __restrict__ Fl,
+                              uint16_t* __restrict__ F3) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;   // over 3 * K * np
+  if (t >= 3 * K * np) return;
+  int q = t % np, c = (t / np) % K, b = t / (np * K);
+  float x = q < n ? F[(3 * (size_t)q + b) * K + c] : 0.f;
+  uint16_t h, l, r;
+  split3(x, h, l, r);
+  Fh[t] = h;
+  Fl[t] = l;
+  F3[t] = r;
+}
+
+// Symmetric 3x3 block index: 00 11 22 01 02 12
+__device__ __forceinline__ int sym_index(int a, int b) {
+  if (a == b) return a;
+  int lo = a < b ? a : b, hi = a < b ? b : a;
+  return lo == 0 ? (hi == 1 ? 3 : 4) : 5;
+}
+
+__device__ __forceinline__ bf16x8 lds8(const uint16_t* p) {
+  return *reinterpret_cast<const bf16x8*>(p);
+}
+__device__ __forceinline__ bf16x8 glb8(const uint16_t* p) {
+  return *reinterpret_cast<const bf16x8*>(p);
+}
+
+// Partial products of the off-diagonal mobility for row tile blockIdx.x and
+// reduction segment blockIdx.y: part[s][a][p][c].
+__global__ __launch_bounds__(RPY_THREADS) void k_rpy_tile(
+    const float4* __restrict__ sph, int n, int np, float mu, int K, int seg, int mode,
+    const uint16_t* __restrict__ Fh, const uint16_t* __restrict__ Fl,
+    const uint16_t* __restrict__ F3, float* __restrict__ part) {
+  __shared__ __align__(16) uint16_t Mh[6][RPY_TP][RPY_TQ];
+  __shared__ __align__(16) uint16_t Ml[6][RPY_TP][RPY_TQ];
+  __shared__ __align__(16) uint16_t M3[6][RPY_TP][RPY_TQ];
+  const int p0 = blockIdx.x * RPY_TP;
+  const int s = blockIdx.y;
+  const int q_begin = s * seg, q_end = min(np, q_begin + seg);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  // pair generation: thread -> sphere p0 + (tid & 63), 8 consecutive q
+  const int gp = p0 + (tid & 63);
+  const int qo = (tid >> 6) * 8;
+  const float4 sp = gp < n ? sph[gp] : make_float4(0.f, 0.f, 0.f, 1.f);
+  const float inv8pimu = 1.f / (8.f * RPY_PI * mu);
+  f32x4 acc[3][4];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) acc[a][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int ncb = K / 16;
+  for (int q0 = q_begin; q0 < q_end; q0 += RPY_TQ) {
+    __syncthreads();
+    {
+      uint16_t h[6][8], l[6][8], r[6][8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        int gq = q0 + qo + j;
+        float m[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (gq < n && gp < n && gq != gp) {
+          float4 sq = sph[gq];
+          float dx = sq.x - sp.x, dy = sq.y - sp.y, dz = sq.z - sp.z;
+          float r2 = dx * dx + dy * dy + dz * dz;
+          float ir2 = 1.f / r2;
+          float c0 = inv8pimu * sqrtf(ir2);
+          float s2 = sp.w * sp.w + sq.w * sq.w;
+          float ci = c0 * (1.f + s2 * ir2 * (1.f / 3.f));
+          float cr = c0 * (1.f - s2 * ir2) * ir2;
+          m[0] = ci + cr * dx * dx;
+          m[1] = ci + cr * dy * dy;
+          m[2] = ci + cr * dz * dz;
+          m[3] = cr * dx * dy;
+          m[4] = cr * dx * dz;
+          m[5] = cr * dy * dz;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) split3(m[k], h[k][j], l[k][j], r[k][j]);
+      }
+      const int pl = tid & 63;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        uint4 vh, vl, vr;
+        vh.x = h[k][0] | (h[k][1] << 16); vh.y = h[k][2] | (h[k][3] << 16);
+        vh.z = h[k][4] | (h[k][5] << 16); vh.w = h[k][6] | (h[k][7] << 16);
+        vl.x = l[k][0] | (l[k][1] << 16); vl.y = l[k][2] | (l[k][3] << 16);
+        vl.z = l[k][4] | (l[k][5] << 16); vl.w = l[k][6] | (l[k][7] << 16);
+        vr.x = r[k][0] | (r[k][1] << 16); vr.y = r[k][2] | (r[k][3] << 16);
+        vr.z = r[k][4] | (r[k][5] << 16); vr.w = r[k][6] | (r[k][7] << 16);
+        *reinterpret_cast<uint4*>(&Mh[k][pl][qo]) = vh;
+        *reinterpret_cast<uint4*>(&Ml[k][pl][qo]) = vl;
+        if (mode == 6) *reinterpret_cast<uint4*>(&M3[k][pl][qo]) = vr;
+      }
+    }
+    __syncthreads();
+    // MFMA: wave handles rows p0 + 16 wave .. +16; lane layout of
+    // v_mfma_f32_16x16x32_bf16: A row = lane % 16, k = 8 (lane / 16) .. +8;
+    // B column = lane % 16, same k; D rows 4 (lane / 16) + i, column lane % 16.
+    const int arow = 16 * wave + (lane & 15);
+    const int kk = 8 * (lane >> 4);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      bf16x8 fh[4], fl[4], f3[4];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        if (cb >= ncb) break;
+        size_t off = ((size_t)b * K + 16 * cb + (lane & 15)) * np + q0 + kk;
+        fh[cb] = glb8(Fh + off);
+        if (mode >= 3) fl[cb] = glb8(Fl + off);
+        if (mode == 6) f3[cb] = glb8(F3 + off);
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int k6 = sym_index(a, b);
+        bf16x8 ah = lds8(&Mh[k6][arow][kk]);
+        bf16x8 al = lds8(&Ml[k6][arow][kk]);
+        bf16x8 a3;
+        if (mode == 6) a3 = lds8(&M3[k6][arow][kk]);
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+          if (cb >= ncb) break;
+          f32x4 c = acc[a][cb];
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, fh[cb], c, 0, 0, 0);
+          if (mode >= 3) {
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, fl[cb], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, fh[cb], c, 0, 0, 0);
+          }
+          if (mode == 6) {
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, fl[cb], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, f3[cb], c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, fh[cb], c, 0, 0, 0);
+          }
+          acc[a][cb] = c;
+        }
+      }
+    }
+  }
+  // partials: part[s][a][p][c]
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+      if (cb >= ncb) break;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        int p = p0 + 16 * wave + 4 * (lane >> 4) + i;
+        int c = 16 * cb + (lane & 15);
+        part[(((size_t)s * 3 + a) * np + p) * K + c] = acc[a][cb][i];
+      }
+    }
+}
+
+// U[3p + a][c] = F/(6 pi mu a_p) + sum_s part[s][a][p][c]
+__global__ void k_rpy_reduce(const float4* __restrict__ sph, int n, int np, float mu, int K,
+                             int nseg, const float* __restrict__ F,
+                             const float* __restrict__ part, float* __restrict__ U) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;   // over 3 n K
+  if (t >= 3 * n * K) return;
+  int c = t % K, row = t / K, a = row % 3, p = row / 3;
+  float v = F[t] / (6.f * RPY_PI * mu * sph[p].w);
+  for (int s = 0; s < nseg; ++s) v += part[(((size_t)s * 3 + a) * np + p) * K + c];
+  U[t] = v;
+}
+
+// ------------------------------------------------------------- reference --
+// PairwiseMobMatrix (:37-82): A1N and B1N for equal spheres a = 1.
+__device__ void pair_ref(float dx, float dy, float dz, float A[9], float B[9]) {
+  float dr = sqrtf(dx * dx + dy * dy + dz * dz);
+  float eu[3] = {dx / dr, dy / dr, dz / dr};
+  float dr3 = dr * dr * dr;
+  float X1NA = 1.5f / dr - 1.f / dr3;
+  float Y1NA = 0.75f / dr + 0.5f / dr3;
+  float Y1NB = -3.f / (4.f * dr * dr);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      A[i * 3 + j] = X1NA * eu[i] * eu[j] + Y1NA * ((i == j ? 1.f : 0.f) - eu[i] * eu[j]);
+  // B1N[i][j] = -Y1NB sum_k eps(j, i, k) e_k
+  B[0] = 0.f;
+  B[1] = -Y1NB * (-eu[2]);   // eps(1,0,2) = -1
+  B[2] = -Y1NB * (eu[1]);    // eps(2,0,1) = +1
+  B[3] = -Y1NB * (eu[2]);    // eps(0,1,2) = +1
+  B[4] = 0.f;
+  B[5] = -Y1NB * (-eu[0]);   // eps(2,1,0) = -1
+  B[6] = -Y1NB * (-eu[1]);   // eps(0,2,1) = -1
+  B[7] = -Y1NB * (eu[0]);    // eps(1,2,0) = +1
+  B[8] = 0.f;
+}
+
+// Kern_getForce (:257-300): near-field repulsion added to force.
+__global__ void k_walk_force(const sphx_displayPack* __restrict__ disp, int n,
+                             sphx_float3* __restrict__ force) {
+  __shared__ float4 tile[256];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float3 pi = i < n ? f3(disp[i].pos.x, disp[i].pos.y, disp[i].pos.z) : f3(0.f, 0.f, 0.f);
+  float3 f = f3(0.f, 0.f, 0.f);
+  for (int k0 = 0; k0 < n; k0 += 256) {
+    __syncthreads();
+    int k = k0 + threadIdx.x;
+    tile[threadIdx.x] = k < n ? make_float4(disp[k].pos.x, disp[k].pos.y, disp[k].pos.z, 0.f)
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    int cnt = min(256, n - k0);
+    for (int t = 0; t < cnt; ++t) {
+      if (k0 + t == i) continue;
+      float4 q = tile[t];
+      float dx = q.x - pi.x, dy = q.y - pi.y, dz = q.z - pi.z;
+      float dr = sqrtf(dx * dx + dy * dy + dz * dz);
+      const float actdist = 1.5f;
+      if (dr < actdist) {
+        float fac = 5.f * (expf(actdist / dr) - 2.7128f * dr / actdist);
+        f.x += -dx * fac; f.y += -dy * fac; f.z += -dz * fac;
+      }
+    }
+  }
+  if (i < n) { force[i].x += f.x; force[i].y += f.y; force[i].z += f.z; }
+}
+
+// Kern_interpolateVelocity (:322-366): trilinear background velocity from
+// the MPM nodes ({v, m} per node).
+__global__ void k_walk_background(const sphx_displayPack* __restrict__ disp, int n,
+                                  const float4* __restrict__ nodev, float3 xmin, float cs,
+                                  int3 res, float ss, sphx_float3* __restrict__ unew) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float px = disp[i].pos.x - xmin.x, py = disp[i].pos.y - xmin.y, pz = disp[i].pos.z - xmin.z;
+  int ix = (int)(px / cs), iy = (int)(py / cs), iz = (int)(pz / cs);
+  float3 v = f3(0.f, 0.f, 0.f);
+  for (int c = 0; c < 8; ++c) {
+    int x = ix + (c & 1), y = iy + ((c >> 1) & 1), z = iz + (c >> 2);
+    if (x < 0 || y < 0 || z < 0 || x >= res.x || y >= res.y || z >= res.z) continue;
+    float w = (1.f - fabsf(px - x * cs) / cs) * (1.f - fabsf(py - y * cs) / cs) *
+              (1.f - fabsf(pz - z * cs) / cs);
+    float4 nv = nodev[((size_t)y * res.z + z) * res.x + x];
+    v.x += nv.x * w; v.y += nv.y * w; v.z += nv.z * w;
+  }
+  unew[i] = {v.x / ss, v.y / ss, v.z / ss};
+}
+
+// Kern_getmobu_walkthrough (:207-255), positions double-buffered (the
+// reference moves them in place while other threads read them, A10).
+__global__ void k_walk_mob(const sphx_displayPack* __restrict__ disp, int n,
+                           const sphx_float3* __restrict__ force, float dt,
+                           sphx_float3* __restrict__ unew, sphx_float3* __restrict__ omega,
+                           float4* __restrict__ newpos) {
+  __shared__ float4 tp[256], tf[256];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float3 pi = i < n ? f3(disp[i].pos.x, disp[i].pos.y, disp[i].pos.z) : f3(0.f, 0.f, 0.f);
+  float3 u = i < n ? f3(unew[i].x, unew[i].y, unew[i].z) : f3(0.f, 0.f, 0.f);
+  float3 w = f3(0.f, 0.f, 0.f);
+  for (int k0 = 0; k0 < n; k0 += 256) {
+    __syncthreads();
+    int k = k0 + threadIdx.x;
+    tp[threadIdx.x] = k < n ? make_float4(disp[k].pos.x, disp[k].pos.y, disp[k].pos.z, 0.f)
+                            : make_float4(0.f, 0.f, 0.f, 0.f);
+    tf[threadIdx.x] = k < n ? make_float4(force[k].x, force[k].y, force[k].z, 0.f)
+                            : make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    int cnt = min(256, n - k0);
+    for (int t = 0; t < cnt; ++t) {
+      float4 q = tp[t], f = tf[t];
+      if (k0 + t != i) {
+        float A[9], B[9];
+        pair_ref(q.x - pi.x, q.y - pi.y, q.z - pi.z, A, B);
+        u.x += A[0] * f.x + A[1] * f.y + A[2] * f.z;
+        u.y += A[3] * f.x + A[4] * f.y + A[5] * f.z;
+        u.z += A[6] * f.x + A[7] * f.y + A[8] * f.z;
+        w.x += B[0] * f.x + B[1] * f.y + B[2] * f.z;
+        w.y += B[3] * f.x + B[4] * f.y + B[5] * f.z;
+        w.z += B[6] * f.x + B[7] * f.y + B[8] * f.z;
+      } else {
+        u.x += f.x; u.y += f.y; u.z += f.z;
+      }
+    }
+  }
+  if (i >= n) return;
+  unew[i] = {u.x, u.y, u.z};
+  omega[i] = {w.x, w.y, w.z};
+  newpos[i] = make_float4(pi.x + u.x * dt, pi.y + u.y * dt, pi.z + u.z * dt, 0.f);
+}
+
+__global__ void k_walk_commit(sphx_displayPack* __restrict__ disp, int n,
+                              const float4* __restrict__ newpos) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 p = newpos[i];
+  disp[i].pos = {p.x, p.y, p.z};
+}
+
+// MVmultiply (:85-98): one wave per row of the dense (6n x 6n) matrix.
+__global__ void k_gemv(const float* __restrict__ mat, const float* __restrict__ f,
+                       float* __restrict__ u, int dim) {
+  int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  int lane = threadIdx.x & 63;
+  if (row >= dim) return;
+  const float* m = mat + (size_t)row * dim;
+  double s = 0.0;
+  for (int j = lane; j < dim; j += 64) s += (double)m[j] * (double)f[j];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) u[row] = (float)s;
+}
+
+}  // namespace sphx
+
+using namespace sphx;
+
+struct sphx_rpy {
+  int n = 0, np = 0, K = 0, split = 16, device = 0;
+  float mu = 1.f;
+  hipStream_t stream = nullptr;
+  sphx::DevBuf<float4> sph;
+  sphx::DevBuf<uint16_t> Fh, Fl, F3;
+  sphx::DevBuf<float> part, F, U;
+};
+
+extern "C" {
+
+int sphx_rpy_create(sphx_rpy** out, int n, int nrhs) {
+  SPHX_CHECK(out, "sphx_rpy_create", "null argument");
+  *out = nullptr;
+  SPHX_CHECK(n > 0, "sphx_rpy_create", "n must be positive");
+  SPHX_CHECK(nrhs > 0 && nrhs <= 64 && nrhs % 16 == 0, "sphx_rpy_create",
+             "nrhs must be 16, 32, 48 or 64");
+  int dev = 0;
+  SPHX_HIP(hipGetDevice(&dev));
+  sphx_rpy* h = new sphx_rpy;
+  h->n = n;
+  h->np = (n + RPY_TP - 1) / RPY_TP * RPY_TP;
+  h->K = nrhs;
+  h->device = dev;
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete h; return hip_fail("hipStreamCreate", e); }
+  size_t fsz = 3 * (size_t)nrhs * h->np;
+  int rc = 0;
+  if ((rc = h->sph.alloc(h->np)) || (rc = h->Fh.alloc(fsz)) || (rc = h->Fl.alloc(fsz)) ||
+      (rc = h->F3.alloc(fsz)) || (rc = h->F.alloc(3 * (size_t)n * nrhs)) ||
+      (rc = h->U.alloc(3 * (size_t)n * nrhs))) {
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return 0;
+}
+
+int sphx_rpy_destroy(sphx_rpy* h) {
+  if (!h) return 0;
+  (void)hipStreamSynchronize(h->stream);
+  h->sph.release(); h->Fh.release(); h->Fl.release(); h->F3.release();
+  h->part.release(); h->F.release(); h->U.release();
+  (void)hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+// Sphere centres [n][3] and radii [n] (host), viscosity mu.
+int sphx_rpy_set_spheres(sphx_rpy* h, const float* pos, const float* radius, float mu) {
+  SPHX_CHECK(h && pos && radius, "sphx_rpy_set_spheres", "null argument");
+  SPHX_CHECK(mu > 0.f, "sphx_rpy_set_spheres", "mu must be positive");
+  std::vector<float4> s(h->np, make_float4(0.f, 0.f, 0.f, 1.f));
+  for (int i = 0; i < h->n; ++i) {
+    SPHX_CHECK(radius[i] > 0.f, "sphx_rpy_set_spheres", "radii must be positive");
+    s[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], radius[i]);
+  }
+  SPHX_HIP(hipSetDevice(h->device));
+  SPHX_HIP(hipMemcpy(h->sph.p, s.data(), h->np * sizeof(float4), hipMemcpyHostToDevice));
+  h->mu = mu;
+  return 0;
+}
+
+// Reduction segments per row tile (work split over the spheres).
+int sphx_rpy_set_split(sphx_rpy* h, int split) {
+  SPHX_CHECK(h && split >= 1, "sphx_rpy_set_split", "bad argument");
+  h->split = split;
+  return 0;
+}
+
+// U = M F with device arrays dF, dU of [3n][nrhs] FP32 (row 3p + a),
+// asynchronous on the handle's stream.  mode 1, 3 or 6 (see above).
+int sphx_rpy_apply_device(sphx_rpy* h, const float* dF, float* dU, int mode) {
+  SPHX_CHECK(h && dF && dU, "sphx_rpy_apply_device", "null argument");
+  SPHX_CHECK(mode == 1 || mode == 3 || mode == 6, "sphx_rpy_apply_device", "mode must be 1, 3 or 6");
+  SPHX_HIP(hipSetDevice(h->device));
+  const int np = h->np, K = h->K;
+  int chunks = np / RPY_TQ;
+  int split = std::max(1, std::min(h->split, chunks));
+  int seg = (chunks + split - 1) / split * RPY_TQ;
+  int nseg = (np + seg - 1) / seg;
+  SPHX_TRY(h->part.alloc((size_t)nseg * 3 * np * K));
+  int tot = 3 * K * np;
+  k_rpy_split_f<<<blocks_for(tot, 256), 256, 0, h->stream>>>(dF, h->n, np, K, h->Fh.p, h->Fl.p,
+                                                             h->F3.p);
+  dim3 grid(np / RPY_TP, nseg);
+  k_rpy_tile<<<grid, RPY_THREADS, 0, h->stream>>>(h->sph.p, h->n, np, h->mu, K, seg, mode,
+                                                  h->Fh.p, h->Fl.p, h->F3.p, h->part.p);
+  int outs = 3 * h->n * K;
+  k_rpy_reduce<<<blocks_for(outs, 256), 256, 0, h->stream>>>(h->sph.p, h->n, np, h->mu, K, nseg,
+                                                             dF, h->part.p, dU);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+// Host arrays F, U of [3n][nrhs]; synchronous.
+int sphx_rpy_apply(sphx_rpy* h, const float* F, float* U, int mode) {
+  SPHX_CHECK(h && F && U, "sphx_rpy_apply", "null argument");
+  SPHX_HIP(hipSetDevice(h->device));
+  size_t sz = 3 * (size_t)h->n * h->K * sizeof(float);
+  SPHX_HIP(hipMemcpyAsync(h->F.p, F, sz, hipMemcpyHostToDevice, h->stream));
+  SPHX_TRY(sphx_rpy_apply_device(h, h->F.p, h->U.p, mode));
+  SPHX_HIP(hipMemcpyAsync(U, h->U.p, sz, hipMemcpyDeviceToHost, h->stream));
+  SPHX_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int sphx_rpy_sync(sphx_rpy* h) {
+  SPHX_CHECK(h, "sphx_rpy_sync", "null argument");
+  SPHX_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+hipStream_t sphx_rpy_stream(sphx_rpy* h) { return h ? h->stream : nullptr; }
+
+// The reference's walkthrough step on device buffers: near-field force
+// (added to cuForce), background velocity from the node grid of `nodes`
+// (may be NULL: zero background), mobility velocity and rotation,
+// positions advanced by U dt.
+int sphx_rpy_walkthrough(sphx_context* nodes, const sphx_stokesianBufList* b, int n,
+                         int with_force) {
+  SPHX_CHECK(b && n >= 0, "sphx_rpy_walkthrough", "bad argument");
+  if (n == 0) return 0;
+  DevBuf<float4> np;
+  SPHX_TRY(np.alloc(n));
+  int bl = blocks_for(n, 256);
+  if (with_force) k_walk_force<<<bl, 256>>>(b->dispBuffer, n, b->cuForce);
+  if (nodes && nodes->mpm.on) {
+    const sphx_ParamCarrier& P = nodes->P;
+    k_walk_background<<<bl, 256>>>(b->dispBuffer, n, nodes->mpm.nodev.p,
+                                   make_float3(P.mpmXmin.x, P.mpmXmin.y, P.mpmXmin.z),
+                                   P.mpmcellsize, make_int3(P.mpmRes.x, P.mpmRes.y, P.mpmRes.z),
+                                   b->simscale, b->cuUnew);
+  } else {
+    SPHX_HIP(hipMemset(b->cuUnew, 0, n * sizeof(sphx_float3)));
+  }
+  k_walk_mob<<<bl, 256>>>(b->dispBuffer, n, b->cuForce, b->dt, b->cuUnew, b->cuOmega, np.p);
+  k_walk_commit<<<bl, 256>>>(b->dispBuffer, n, np.p);
+  SPHX_HIP(hipGetLastError());
+  SPHX_HIP(hipDeviceSynchronize());
+  return 0;
+}
+
+// MVmultiply / getMobU: u = mat f with mat (6n x 6n) row-major, device.
+int sphx_mob_gemv(const float* mat, const float* f, float* u, int nsize) {
+  SPHX_CHECK(mat && f && u && nsize >= 0, "sphx_mob_gemv", "bad argument");
+  int dim = 6 * nsize;
+  if (dim == 0) return 0;
+  k_gemv<<<blocks_for(dim, 4), 256>>>(mat, f, u, dim);
+  SPHX_HIP(hipGetLastError());
+  SPHX_HIP(hipDeviceSynchronize());
+  return 0;
+}
+
+}  // extern "C"

@@ -183,6 +183,18 @@ def lib() -> C.CDLL:
         L.sphx_scene_sand.argtypes = [I, C.c_float, C.c_float, C.c_float, I, C.c_float, C.c_float,
                                       C.c_float, C.c_float, C.c_float, C.c_uint64,
                                       P(ParamCarrier), VP, VP, P(SceneInfo)]
+        L.sphx_rpy_create.argtypes = [P(VP), I, I]
+        L.sphx_rpy_destroy.argtypes = [VP]
+        L.sphx_rpy_set_spheres.argtypes = [VP, VP, VP, C.c_float]
+        L.sphx_rpy_set_split.argtypes = [VP, I]
+        L.sphx_rpy_apply.argtypes = [VP, VP, VP, I]
+        L.sphx_rpy_apply_device.argtypes = [VP, VP, VP, I]
+        L.sphx_rpy_sync.argtypes = [VP]
+        L.sphx_rpy_walkthrough.argtypes = [VP, VP, I, I]
+        L.sphx_mob_gemv.argtypes = [VP, VP, VP, I]
+        L.sphx_dev_alloc.argtypes = [P(VP), C.c_size_t]
+        L.sphx_dev_free.argtypes = [VP]
+        L.sphx_dev_copy.argtypes = [VP, VP, C.c_size_t, I]
         _lib = L
     return _lib
 
@@ -500,3 +512,81 @@ class Context:
 
 def device_count() -> int:
     return lib().sphx_device_count()
+
+
+class Mobility:
+    """Polydisperse RPY mobility applied on MFMA (sphx_rpy_*)."""
+
+    def __init__(self, n: int, nrhs: int = 64):
+        self.L = lib()
+        self.h = C.c_void_p()
+        check(self.L.sphx_rpy_create(C.byref(self.h), n, nrhs), "sphx_rpy_create")
+        self.n, self.nrhs = n, nrhs
+
+    def set_spheres(self, pos, radius, mu=1.0):
+        pos = np.ascontiguousarray(pos, np.float32)
+        radius = np.ascontiguousarray(radius, np.float32)
+        check(self.L.sphx_rpy_set_spheres(self.h, ptr(pos), ptr(radius), mu), "set_spheres")
+
+    def set_split(self, split):
+        check(self.L.sphx_rpy_set_split(self.h, split), "set_split")
+
+    def apply(self, F, mode=6):
+        F = np.ascontiguousarray(F, np.float32)
+        assert F.shape == (3 * self.n, self.nrhs)
+        U = np.zeros_like(F)
+        check(self.L.sphx_rpy_apply(self.h, ptr(F), ptr(U), mode), "sphx_rpy_apply")
+        return U
+
+    def apply_device(self, dF: int, dU: int, mode=6):
+        check(self.L.sphx_rpy_apply_device(self.h, C.c_void_p(dF), C.c_void_p(dU), mode),
+              "sphx_rpy_apply_device")
+
+    def sync(self):
+        check(self.L.sphx_rpy_sync(self.h), "sphx_rpy_sync")
+
+    def close(self):
+        if self.h:
+            self.L.sphx_rpy_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class StokesianBufList(C.Structure):
+    _fields_ = [("cuUnew", C.c_void_p), ("dispBuffer", C.c_void_p), ("cuOmega", C.c_void_p),
+                ("cuForce", C.c_void_p), ("dt", C.c_float), ("simscale", C.c_float)]
+
+
+class DeviceArray:
+    """A device copy of a numpy array (sphx_dev_alloc); .ptr is the device
+    address, .get() copies it back."""
+
+    def __init__(self, a):
+        self.L = lib()
+        self.a = np.ascontiguousarray(a)
+        self.ptr = C.c_void_p()
+        check(self.L.sphx_dev_alloc(C.byref(self.ptr), max(self.a.nbytes, 1)), "sphx_dev_alloc")
+        if self.a.nbytes:
+            check(self.L.sphx_dev_copy(self.ptr, ptr(self.a), self.a.nbytes, 1), "sphx_dev_copy")
+
+    def get(self):
+        out = np.empty_like(self.a)
+        if out.nbytes:
+            check(self.L.sphx_dev_copy(ptr(out), self.ptr, out.nbytes, 2), "sphx_dev_copy")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.L.sphx_dev_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
