@@ -395,6 +395,32 @@ void sphx_setup_mpm_grid(sphx_ParamCarrier* p, float pSpacing);
 void sphx_set_drucker_prager(sphx_ParamCarrier* p);
 
 /* ------------------------------------------------------------------------ */
+/* Batched ensemble.  No counterpart in the reference (single scene).        */
+/* Scenes sit in disjoint boxes of one grid and share every launch; the     */
+/* scene index is bits 8-23 of the displayPack type word (bits 0-7 keep the */
+/* reference's type).                                                        */
+/* ------------------------------------------------------------------------ */
+
+/* Per-scene convergence of the pressure solve for nscenes (<= 256) scenes
+ * (0: one scene); the solve ends when every scene has converged, and a
+ * converged scene's pressures stay fixed meanwhile. */
+int sphx_ensemble_setup(sphx_context* ctx, int nscenes);
+/* Iterations and final error of each scene's last solve. */
+int sphx_scene_stats(sphx_context* ctx, int* iters, double* err);
+
+/* The two-phase step (sphx_mf_step) as HIP graphs, per buffer parity:
+ * graph A = sort .. advection prediction, the learned start and `budget`
+ * pressure iterations (early exit on the device); graph B = integration.
+ * Between them the host reads the done flag and finishes an open solve
+ * eagerly (and falls back from a learned start that missed the tolerance).
+ * Build after one eager sphx_mf_step; rebuild after uploads or parameter
+ * changes.  Recording and the online refresh of the learned start are not
+ * part of the graphs. */
+int sphx_graph_build(sphx_context* ctx, int budget);
+int sphx_graph_step(sphx_context* ctx, int* iters, double* err);
+int sphx_graph_release(sphx_context* ctx);
+
+/* ------------------------------------------------------------------------ */
 /* Stokesian mobility (SPH_Solver/stokesian/Mobility.cu)                     */
 /* ------------------------------------------------------------------------ */
 
@@ -548,6 +574,17 @@ int sphx_scene_sand(int n, float lo, float hi, float dx, int res, float E, float
                     float phi_deg, float rho, float f_noise, uint64_t seed,
                     sphx_ParamCarrier* params, sphx_displayPack* disp,
                     sphx_calculationPack* calc, sphx_SceneInfo* info);
+
+/* Batched ensemble: nscenes two-phase tanks of n_fluid[s] particles at the
+ * spacing of the 2^20-particle tank, bubbles seeded per scene, in
+ * cell-aligned boxes with 2-cell gaps of one grid; the scene index goes into
+ * bits 8-23 of the type word; scene_first [nscenes + 1].  Returns the
+ * particle count (arrays may be NULL). */
+int sphx_scene_ensemble(int nscenes, const int* n_fluid, float fill, float h,
+                        float bubbles_per_million, float rmin, float rmax, int layers,
+                        uint64_t seed, sphx_ParamCarrier* params, sphx_displayPack* disp,
+                        sphx_calculationPack* calc, float* alpha, int* scene_first,
+                        sphx_SceneInfo* info);
 
 #ifdef __cplusplus
 }

@@ -56,7 +56,11 @@ __host__ __device__ inline float3 add(float3 a, float3 b) { return f3(a.x + b.x,
 __host__ __device__ inline float3 sub(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __host__ __device__ inline float3 mul(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
 __host__ __device__ inline float dot3(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ inline int type_of(const float4& p) { return __float_as_int(p.w); }
+// pos.w holds the displayPack type word: particle type in bits 0-7, the
+// scene of a batched ensemble in bits 8-23 (0 for a single scene).
+__device__ inline int type_of(const float4& p) { return __float_as_int(p.w) & 0xFF; }
+__device__ inline int scene_of(const float4& p) { return (__float_as_int(p.w) >> 8) & 0xFFFF; }
+__host__ __device__ inline int type_bits(int t) { return t & 0xFF; }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2.  Give XCD x the contiguous tile range [x*q + min(x,r), ...) (q, r =
@@ -107,13 +111,41 @@ struct KP {
   float eos_B, eos_gamma, art_alpha, c0;
   int clamp;
   int clamp_p;
+  int nscenes;       // > 0: batched ensemble, per-scene convergence
+};
+
+constexpr int SPHX_MAX_SCENES = 256;
+
+// Per-scene solver state of a batched ensemble (device arrays).
+struct SceneCtl {
+  int n;             // number of scenes (0: single scene)
+  double* acc;       // [n][3] per-iteration sums: signed residual, error, count
+  int* state;        // [n][2] done, iterations
+  double* err;       // [n] last error
 };
 
 // ----------------------------------------------------------------- buffer --
+// Device array owned by its holder: freed by release() or the destructor,
+// movable (std::swap works), not copyable.
 template <class T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
   int alloc(size_t count) {
     if (count <= n && p) return 0;
     release();
@@ -229,6 +261,20 @@ struct sphx_context {
     sphx::DevBuf<double> loss;
     int batch_cap = 0;
   } mlp;
+
+  // batched ensemble: scenes in disjoint boxes of one grid
+  int nscenes = 0;
+  sphx::DevBuf<double> sc_acc, sc_err;
+  sphx::DevBuf<int> sc_state;
+  std::vector<int> scene_first, scene_count;   // upload order
+  sphx::SceneCtl scenes() {
+    return sphx::SceneCtl{nscenes, sc_acc.p, sc_state.p, sc_err.p};
+  }
+
+  // captured two-phase step: graph A (sort .. a fixed budget of pressure
+  // iterations) and graph B (integration), one of each per buffer parity
+  hipGraphExec_t graph_a[2] = {nullptr, nullptr}, graph_b[2] = {nullptr, nullptr};
+  int graph_budget = 0, graph_num = -1;
 
   // timing
   bool timing = false;

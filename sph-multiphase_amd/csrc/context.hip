@@ -56,6 +56,7 @@ KP make_kp(const sphx_context* c) {
   k.c0 = O.sound_speed;
   k.clamp = O.clamp_to_grid;
   k.clamp_p = O.clamp_pressure;
+  k.nscenes = c->nscenes;
   return k;
 }
 
@@ -329,6 +330,7 @@ int sphx_destroy(sphx_context* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)timer_flush(c);
+  (void)sphx_graph_release(c);
   for (int b = 0; b < 2; ++b) {
     c->pos[b].release(); c->vel[b].release(); c->veleval[b].release();
     c->accel[b].release(); c->phys[b].release(); c->bornid[b].release();

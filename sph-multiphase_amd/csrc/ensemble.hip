@@ -1,0 +1,143 @@
+// The two-phase step as HIP graphs.
+//
+// A step is captured once per particle-buffer parity as two graphs:
+//   A: sort, density, drift-flux stages, advection prediction, the learned
+//      start (when enabled) and a fixed budget of pressure iterations;
+//   B: the per-particle pressure increment of the learned start and the
+//      integration.
+// Between them the host reads the solver's done flag (8 bytes): a solve
+// still open after the budget continues eagerly, and a learned start that
+// missed the tolerance falls back to the plain start, so the plain solver
+// keeps judging every step.  Iterations past convergence exit on the device
+// flag.  Every launch covers all scenes of a batched ensemble at once.
+// Nothing in the captured work allocates, synchronises or clears memory
+// with a memset node (the solver state is reset by a kernel).
+#include "common.hpp"
+
+namespace sphx {
+
+int enqueue_solve_fixed(sphx_context* c, int n);
+int continue_solve(sphx_context* c, int* iters, double* err);
+int mlp_begin(sphx_context* c);
+int mlp_fallback(sphx_context* c, int* iters, double* err);
+int mlp_after(sphx_context* c, int record_ok);
+
+static void graph_release(sphx_context* c) {
+  for (int p = 0; p < 2; ++p) {
+    if (c->graph_a[p]) (void)hipGraphExecDestroy(c->graph_a[p]);
+    if (c->graph_b[p]) (void)hipGraphExecDestroy(c->graph_b[p]);
+    c->graph_a[p] = c->graph_b[p] = nullptr;
+  }
+  c->graph_num = -1;
+}
+
+// Host bookkeeping of graph A's stages (what the eager calls leave behind).
+static void after_graph_a(sphx_context* c, int p) {
+  c->cur = 1 - p;
+  c->sorted_valid = true;
+  c->nbr_valid = true;
+  c->mf_pending = true;
+}
+
+static int stage_a(sphx_context* c, int budget) {
+  SPHX_TRY(sphx_grid_build(c));
+  SPHX_TRY(sphx_iisph_density(c));
+  SPHX_TRY(sphx_mf_drift_velocity(c));
+  SPHX_TRY(sphx_mf_alpha_advance(c));
+  SPHX_TRY(sphx_mf_correction(c));
+  SPHX_TRY(sphx_iisph_predict_dii(c));
+  SPHX_TRY(sphx_mf_force(c));
+  SPHX_TRY(sphx_iisph_predict_aii(c));
+  SPHX_TRY(mlp_begin(c));
+  return enqueue_solve_fixed(c, budget);
+}
+
+static int stage_b(sphx_context* c) {
+  SPHX_TRY(mlp_after(c, 0));
+  return sphx_iisph_integrate(c);
+}
+
+template <class F>
+static int capture(sphx_context* c, F&& body, hipGraphExec_t* out) {
+  SPHX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  int rc = body();
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc != 0) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail("hipStreamEndCapture", e);
+  e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return hip_fail("hipGraphInstantiate", e);
+  return 0;
+}
+
+}  // namespace sphx
+
+using namespace sphx;
+
+extern "C" {
+
+// Capture the two-phase step with `budget` pressure iterations in graph A.
+// The context must have run at least one eager sphx_mf_step (so every
+// buffer exists); recapture after an upload or a parameter change.
+int sphx_graph_build(sphx_context* c, int budget) {
+  SPHX_CHECK(c, "sphx_graph_build", "null context");
+  SPHX_CHECK(c->nphase > 0, "sphx_graph_build", "multiphase not set up");
+  SPHX_CHECK(budget >= 1, "sphx_graph_build", "budget must be >= 1");
+  SPHX_CHECK(c->mf_pending && c->nbr_cap > 0 && c->red.p, "sphx_graph_build",
+             "run one sphx_mf_step before capturing");
+  SPHX_TRY(ensure_device(c));
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  graph_release(c);
+  const bool timing = c->timing;
+  c->timing = false;
+  int rc = 0;
+  for (int r = 0; r < 2 && rc == 0; ++r) {
+    const int p = c->cur;
+    rc = capture(c, [&] { return stage_a(c, budget); }, &c->graph_a[p]);
+    if (rc) break;
+    after_graph_a(c, p);
+    rc = capture(c, [&] { return stage_b(c); }, &c->graph_b[1 - p]);
+    c->nbr_valid = false;
+  }
+  c->timing = timing;
+  if (rc) {
+    graph_release(c);
+    return rc;
+  }
+  c->graph_budget = budget;
+  c->graph_num = c->num;
+  return 0;
+}
+
+// One two-phase step through the captured graphs (same result as
+// sphx_mf_step).  Reports the iterations of the final solve and its error.
+int sphx_graph_step(sphx_context* c, int* iters, double* err) {
+  SPHX_CHECK(c, "sphx_graph_step", "null context");
+  SPHX_CHECK(c->graph_num == c->num && c->graph_a[c->cur], "sphx_graph_step",
+             "no step graph for this state (call sphx_graph_build)");
+  SPHX_TRY(ensure_device(c));
+  const int p = c->cur;
+  SPHX_HIP(hipGraphLaunch(c->graph_a[p], c->stream));
+  after_graph_a(c, p);
+  int it = 0;
+  double e = 0.0;
+  SPHX_TRY(continue_solve(c, &it, &e));
+  SPHX_TRY(mlp_fallback(c, &it, &e));
+  SPHX_HIP(hipGraphLaunch(c->graph_b[1 - p], c->stream));
+  c->nbr_valid = false;
+  if (iters) *iters = it;
+  if (err) *err = e;
+  return 0;
+}
+
+int sphx_graph_release(sphx_context* c) {
+  SPHX_CHECK(c, "sphx_graph_release", "null context");
+  graph_release(c);
+  return 0;
+}
+
+}  // extern "C"

@@ -166,12 +166,13 @@ __global__ void k_ii_dp(KP k, NbrList L, const int* __restrict__ done, const uin
                         const int* __restrict__ cs, const float4* __restrict__ pos,
                         const float4* __restrict__ ps, const float4* __restrict__ phys,
                         const float4* __restrict__ dii, const float* __restrict__ press_l,
-                        float4* __restrict__ dijpj, float4* __restrict__ qm) {
+                        float4* __restrict__ dijpj, float4* __restrict__ qm, SceneCtl sc) {
   if (*done) return;
   int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
   if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
+  if (sc.n && sc.state[2 * scene_of(pi)]) return;   // converged scene: p is final
   uint32_t gc = key[i];
   if ((int)gc == k.gridtotal) return;
   float sx = 0.f, sy = 0.f, sz = 0.f;
@@ -221,15 +222,23 @@ __global__ __launch_bounds__(NB_THREADS) void k_ii_iter(
     const float4* __restrict__ dii, const float4* __restrict__ dijpj,
     const float4* __restrict__ qm, const float* __restrict__ rho_adv,
     float* __restrict__ press_l, float* __restrict__ press_l1, float4* __restrict__ ps,
-    float* __restrict__ residue, float4* __restrict__ accel, double* __restrict__ partial) {
+    float* __restrict__ residue, float4* __restrict__ accel, double* __restrict__ partial,
+    SceneCtl sc) {
   if (*done) return;
+  __shared__ double sacc[SPHX_MAX_SCENES * 3];
+  if (sc.n)
+    for (int t = threadIdx.x; t < 3 * sc.n; t += blockDim.x) sacc[t] = 0.0;
   int i = xcd_index();
   double rs = 0.0, ra = 0.0, rc = 0.0;
+  int scene = 0;
   if (i < k.n) {
     float4 pi = pos[i];
     uint32_t gc = key[i];
+    scene = scene_of(pi);
     if (type_of(pi) == SPHX_TYPE_BOUNDARY || (int)gc == k.gridtotal) {
       residue[i] = 0.f;
+    } else if (sc.n && sc.state[2 * scene]) {
+      // converged scene: pressures final, contributes nothing
     } else {
       const float4 qi = phys[i];
       const float mi = qi.x, rhoi = qi.y, rest = qi.w;
@@ -280,6 +289,19 @@ __global__ __launch_bounds__(NB_THREADS) void k_ii_iter(
       rc = 1.0;
     }
   }
+  if (sc.n) {
+    // per-scene sums: LDS atomics, then one global atomic per touched entry
+    __syncthreads();
+    if (rc != 0.0) {
+      atomicAdd(&sacc[3 * scene], rs);
+      atomicAdd(&sacc[3 * scene + 1], ra);
+      atomicAdd(&sacc[3 * scene + 2], rc);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 3 * sc.n; t += blockDim.x)
+      if (sacc[t] != 0.0) atomicAdd(&sc.acc[t], sacc[t]);
+    return;
+  }
   block_reduce3(rs, ra, rc, partial);
 }
 
@@ -314,6 +336,73 @@ __global__ __launch_bounds__(1024) void k_ii_finalize(int* __restrict__ state,
     int it = state[1] + 1;
     state[1] = it;
     if ((it >= ctl.min_iters && e <= (double)ctl.tol) || it >= ctl.max_iters) state[0] = 1;
+  }
+}
+
+// Ensemble version: per-scene error, iteration count and convergence; the
+// solve is done when every scene is.  err[0] = largest scene error,
+// state[1] = most iterations of any scene.
+__global__ __launch_bounds__(256) void k_ii_finalize_scenes(int* __restrict__ state,
+                                                            double* __restrict__ err,
+                                                            SceneCtl sc, IterCtl ctl,
+                                                            int respect_done) {
+  if (respect_done && state[0]) return;
+  __shared__ int s_open[256], s_it[256];
+  __shared__ double s_err[256];
+  int open = 0, maxit = 0;
+  double maxerr = 0.0;
+  for (int s = threadIdx.x; s < sc.n; s += blockDim.x) {
+    int done = sc.state[2 * s];
+    int it = sc.state[2 * s + 1];
+    if (!done) {
+      double cnt = sc.acc[3 * s + 2];
+      double e = cnt > 0 ? sc.acc[3 * s + 1] / cnt : 0.0;
+      sc.err[s] = e;
+      ++it;
+      sc.state[2 * s + 1] = it;
+      // a NaN error stops the scene, which then reports as unconverged
+      if ((it >= ctl.min_iters && e <= (double)ctl.tol) || it >= ctl.max_iters || !(e == e))
+        done = 1;
+      sc.state[2 * s] = done;
+    }
+    sc.acc[3 * s] = 0.0;
+    sc.acc[3 * s + 1] = 0.0;
+    sc.acc[3 * s + 2] = 0.0;
+    open += !done;
+    maxit = max(maxit, it);
+    double e = sc.err[s];
+    if (!(e == e) || e > maxerr) maxerr = e;   // NaN wins
+  }
+  s_open[threadIdx.x] = open;
+  s_it[threadIdx.x] = maxit;
+  s_err[threadIdx.x] = maxerr;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int o = 0, m = 0;
+    double e = 0.0;
+    for (int t = 0; t < (int)blockDim.x; ++t) {
+      o += s_open[t];
+      m = max(m, s_it[t]);
+      if (!(s_err[t] == s_err[t]) || s_err[t] > e) e = s_err[t];
+    }
+    state[0] = o == 0;
+    state[1] = m;
+    err[0] = e;
+    err[1] = 0.0;
+  }
+}
+
+// Start of a solve: clear the done flag, iteration count and (ensemble)
+// every scene's state.  A kernel, not a memset, so it can sit in a graph.
+__global__ void k_solve_reset(int* __restrict__ state, SceneCtl sc) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) { state[0] = 0; state[1] = 0; }
+  if (t < sc.n) {
+    sc.state[2 * t] = 0;
+    sc.state[2 * t + 1] = 0;
+    sc.acc[3 * t] = 0.0;
+    sc.acc[3 * t + 1] = 0.0;
+    sc.acc[3 * t + 2] = 0.0;
   }
 }
 
@@ -439,25 +528,37 @@ static int ensure_red(sphx_context* c) {
   return c->red.alloc(3 * nb + 16);
 }
 
+int solve_reset(sphx_context* c) {
+  SceneCtl sc = c->scenes();
+  k_solve_reset<<<blocks_for(std::max(sc.n, 1), 256), 256, 0, c->stream>>>(c->solve_state.p, sc);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
 // one DP + Iter + finalize; respect_done=1 inside the pipelined solve
 static int enqueue_iteration(sphx_context* c, const KP& k, int respect_done) {
   int nb = blocks_for(k.n, NB_THREADS);
   NbrList L;
   SPHX_TRY(nbr_list(c, &L));
   IterCtl ctl{c->O.min_iters, c->O.max_iters, c->O.tol};
+  SceneCtl sc = c->scenes();
   timer_begin(c, "iisph_dp");
   k_ii_dp<<<nb, NB_THREADS, 0, c->stream>>>(k, L, c->solve_state.p, c->key.p, c->cell_start.p,
                                             c->POS(), c->ps.p, c->PHYS(), c->dii.p,
-                                            c->press_l.p, c->dijpj.p, c->qm.p);
+                                            c->press_l.p, c->dijpj.p, c->qm.p, sc);
   timer_end(c);
   timer_begin(c, "iisph_iter");
   k_ii_iter<<<nb, NB_THREADS, 0, c->stream>>>(k, L, c->solve_state.p, c->key.p, c->cell_start.p,
                                               c->POS(), c->PHYS(), c->dii.p, c->dijpj.p, c->qm.p,
                                               c->rho_adv.p, c->press_l.p, c->press_l1.p,
-                                              c->ps.p, c->residue.p, c->ACCEL(), c->red.p);
+                                              c->ps.p, c->residue.p, c->ACCEL(), c->red.p, sc);
   timer_end(c);
-  k_ii_finalize<<<1, 1024, 0, c->stream>>>(c->solve_state.p, c->solve_err.p, c->red.p, nb, k.n,
-                                           ctl, respect_done);
+  if (sc.n)
+    k_ii_finalize_scenes<<<1, 256, 0, c->stream>>>(c->solve_state.p, c->solve_err.p, sc, ctl,
+                                                   respect_done);
+  else
+    k_ii_finalize<<<1, 1024, 0, c->stream>>>(c->solve_state.p, c->solve_err.p, c->red.p, nb,
+                                             k.n, ctl, respect_done);
   SPHX_HIP(hipGetLastError());
   return 0;
 }
@@ -534,7 +635,7 @@ int sphx_iisph_pressure_iteration(sphx_context* c, double* mean_signed, double* 
   SPHX_TRY(ensure_red(c));
   KP k = make_kp(c);
   if (k.n == 0) return 0;
-  SPHX_HIP(hipMemsetAsync(c->solve_state.p, 0, 2 * sizeof(int), c->stream));
+  SPHX_TRY(solve_reset(c));
   SPHX_TRY(enqueue_iteration(c, k, 0));
   if (mean_signed || mean_abs_rel) {
     SPHX_TRY(read_state(c));
@@ -561,7 +662,7 @@ int iisph_solve_plain(sphx_context* c, int* iters, double* err) {
   if (iters) *iters = 0;
   if (err) *err = 0.0;
   if (k.n == 0) return 0;
-  SPHX_HIP(hipMemsetAsync(c->solve_state.p, 0, 2 * sizeof(int), c->stream));
+  SPHX_TRY(solve_reset(c));
   int enq = 0;
   int chunk = std::max(c->O.min_iters, 2);
   // enqueue about as many iterations as the previous solve needed before
@@ -580,6 +681,36 @@ int iisph_solve_plain(sphx_context* c, int* iters, double* err) {
   if (iters) *iters = it;
   if (err) *err = c->host_pinned[0];
   return timer_flush(c);
+}
+
+// Capturable part of a solve: reset and n iterations, no host reads
+// (iterations past convergence exit on the device flag).
+int enqueue_solve_fixed(sphx_context* c, int n) {
+  SPHX_TRY(ensure_red(c));
+  KP k = make_kp(c);
+  if (k.n == 0) return 0;
+  SPHX_TRY(solve_reset(c));
+  for (int t = 0; t < n; ++t) SPHX_TRY(enqueue_iteration(c, k, 1));
+  return 0;
+}
+
+// Finish a solve begun by enqueue_solve_fixed: check on the host and go on,
+// two iterations at a time, until it converged or reached max_iters.
+int continue_solve(sphx_context* c, int* iters, double* err) {
+  KP k = make_kp(c);
+  if (iters) *iters = 0;
+  if (err) *err = 0.0;
+  if (k.n == 0) return 0;
+  SPHX_TRY(read_state(c));
+  while (c->host_pinned[2] == 0.0 && (int)c->host_pinned[3] < c->O.max_iters) {
+    for (int t = 0; t < 2; ++t) SPHX_TRY(enqueue_iteration(c, k, 1));
+    SPHX_TRY(read_state(c));
+  }
+  int it = (int)c->host_pinned[3];
+  c->last_iters = it;
+  if (iters) *iters = it;
+  if (err) *err = c->host_pinned[0];
+  return 0;
 }
 
 }  // namespace sphx
@@ -610,6 +741,39 @@ int sphx_iisph_step(sphx_context* c, int* iters, double* err) {
   SPHX_TRY(sphx_iisph_predict_advection(c));
   SPHX_TRY(sphx_iisph_pressure_solve(c, iters, err));
   return sphx_iisph_integrate(c);
+}
+
+// Batched ensemble of nscenes scenes (scene index in bits 8-23 of the
+// type word): each scene's pressure solve converges on its own.
+int sphx_ensemble_setup(sphx_context* c, int nscenes) {
+  SPHX_CHECK(c, "sphx_ensemble_setup", "null context");
+  SPHX_CHECK(nscenes >= 0 && nscenes <= SPHX_MAX_SCENES, "sphx_ensemble_setup",
+             "nscenes must be in [0, 256]");
+  SPHX_TRY(ensure_device(c));
+  if (nscenes > 0) {
+    SPHX_TRY(c->sc_acc.alloc(3 * (size_t)nscenes));
+    SPHX_TRY(c->sc_err.alloc(nscenes));
+    SPHX_TRY(c->sc_state.alloc(2 * (size_t)nscenes));
+    SPHX_HIP(hipMemset(c->sc_acc.p, 0, 3 * (size_t)nscenes * sizeof(double)));
+    SPHX_HIP(hipMemset(c->sc_err.p, 0, (size_t)nscenes * sizeof(double)));
+    SPHX_HIP(hipMemset(c->sc_state.p, 0, 2 * (size_t)nscenes * sizeof(int)));
+  }
+  c->nscenes = nscenes;
+  return 0;
+}
+
+// Iterations and final error of each scene's last pressure solve.
+int sphx_scene_stats(sphx_context* c, int* iters, double* err) {
+  SPHX_CHECK(c, "sphx_scene_stats", "null context");
+  SPHX_CHECK(c->nscenes > 0, "sphx_scene_stats", "not an ensemble");
+  SPHX_TRY(ensure_device(c));
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  std::vector<int> st(2 * (size_t)c->nscenes);
+  SPHX_HIP(hipMemcpy(st.data(), c->sc_state.p, st.size() * sizeof(int), hipMemcpyDeviceToHost));
+  if (iters)
+    for (int s = 0; s < c->nscenes; ++s) iters[s] = st[2 * s + 1];
+  if (err) SPHX_HIP(hipMemcpy(err, c->sc_err.p, c->nscenes * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
 }
 
 int sphx_download_iisph(sphx_context* c, float* dii, float* dijpj, float* aii, float* rho_adv,
@@ -673,7 +837,7 @@ int sphx_upload_iisph(sphx_context* c, const float* dii, const float* dijpj, con
     for (int i = 0; i < n; ++i) {
       int t;
       std::memcpy(&t, &pos[i].w, 4);
-      s[i].w = t == SPHX_TYPE_BOUNDARY ? 0.f : ph[i].x * press_l[i] / ph[i].y / ph[i].y;
+      s[i].w = type_bits(t) == SPHX_TYPE_BOUNDARY ? 0.f : ph[i].x * press_l[i] / ph[i].y / ph[i].y;
     }
     SPHX_HIP(hipMemcpy(c->ps.p, s.data(), n * sizeof(float4), hipMemcpyHostToDevice));
   }

@@ -222,10 +222,13 @@ __global__ __launch_bounds__(256) void k_mlp_eval(const float* __restrict__ X, i
 __global__ void k_mlp_plain_start(KP k, const float4* __restrict__ pos,
                                   const uint32_t* __restrict__ key, float4* __restrict__ phys,
                                   const float* __restrict__ pprev, float* __restrict__ press_l,
-                                  float4* __restrict__ ps) {
+                                  float4* __restrict__ ps, SceneCtl sc, float tol) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= k.n) return;
-  if (type_of(pos[i]) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) return;
+  const float4 pp = pos[i];
+  if (type_of(pp) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) return;
+  // ensemble: only the scenes whose solve missed the tolerance restart
+  if (sc.n && sc.err[scene_of(pp)] <= (double)tol) return;
   float4 q = phys[i];
   q.z = pprev[i];
   phys[i] = q;
@@ -590,15 +593,14 @@ static int mlp_online_refresh(sphx_context* c) {
 
 // The pressure solve of a step with the learned start, the recorder and
 // the fallback.  Called after the a_ii pass.
-int mlp_solve(sphx_context* c, int* iters, double* err) {
+// Before a solve: features (or just p(t-1)) and the learned start.
+int mlp_begin(sphx_context* c) {
   auto& m = c->mlp;
-  if (!m.ready || c->num == 0) return iisph_solve_plain(c, iters, err);
+  if (!m.ready || c->num == 0) return 0;
   KP k = make_kp(c);
   MLPP mp = make_mlpp(c);
-  size_t n = (size_t)c->num;
   const bool mf = c->nphase > 0;
-  const bool online = m.online_every > 0;
-  const bool record = m.record || online;
+  const bool record = m.record || m.online_every > 0;
   SPHX_TRY(m.X.alloc((size_t)c->cap * MLP_NF));
   SPHX_TRY(m.pprev.alloc((size_t)c->cap));
   float4* hist = m.hist[c->cur].p;
@@ -623,25 +625,45 @@ int mlp_solve(sphx_context* c, int* iters, double* err) {
     timer_end(c);
     SPHX_HIP(hipGetLastError());
   }
-  int it = 0;
-  double e = 0.0;
-  SPHX_TRY(iisph_solve_plain(c, &it, &e));
+  return 0;
+}
+
+// After a solve from the learned start: the plain solver judges it; a solve
+// that missed the tolerance is redone from the plain start (per scene in an
+// ensemble), and its iterations are added.
+int mlp_fallback(sphx_context* c, int* iters, double* err) {
+  auto& m = c->mlp;
+  if (!m.ready || !m.warm || c->num == 0) return 0;
+  ++m.learned_solves;
   m.last_fallback = 0;
-  if (m.warm) {
-    ++m.learned_solves;
-    // the done flag is also set at max_iters: judge by the error itself
-    bool converged = e == e && e <= (double)c->O.tol;
-    if (!converged) {
-      ++m.fallbacks;
-      m.last_fallback = 1;
-      k_mlp_plain_start<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
-          k, c->POS(), c->key.p, c->PHYS(), m.pprev.p, c->press_l.p, c->ps.p);
-      SPHX_HIP(hipGetLastError());
-      int it2 = 0;
-      SPHX_TRY(iisph_solve_plain(c, &it2, &e));
-      it += it2;
-    }
-  }
+  // the done flag is also set at max_iters: judge by the error itself
+  double e = *err;
+  if (e == e && e <= (double)c->O.tol) return 0;
+  ++m.fallbacks;
+  m.last_fallback = 1;
+  KP k = make_kp(c);
+  k_mlp_plain_start<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+      k, c->POS(), c->key.p, c->PHYS(), m.pprev.p, c->press_l.p, c->ps.p, c->scenes(), c->O.tol);
+  SPHX_HIP(hipGetLastError());
+  int it2 = 0;
+  SPHX_TRY(iisph_solve_plain(c, &it2, err));
+  *iters += it2;
+  return 0;
+}
+
+// After the final solve: the per-particle pressure increment (feature 20),
+// then recording and the online refresh when enabled (record_ok = 0 inside
+// a captured graph).
+int mlp_after(sphx_context* c, int record_ok) {
+  auto& m = c->mlp;
+  if (!m.ready || c->num == 0) return 0;
+  KP k = make_kp(c);
+  MLPP mp = make_mlpp(c);
+  size_t n = (size_t)c->num;
+  const bool mf = c->nphase > 0;
+  const bool online = m.online_every > 0;
+  const bool record = record_ok && (m.record || online);
+  float4* hist = m.hist[c->cur].p;
   k_mlp_hist<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(k, c->POS(), c->key.p, c->PHYS(),
                                                           m.pprev.p, hist);
   SPHX_HIP(hipGetLastError());
@@ -670,12 +692,28 @@ int mlp_solve(sphx_context* c, int* iters, double* err) {
       m.nsamples = std::max(m.nsamples, m.wpos);
     }
   }
-  ++m.solves;
-  if (online && m.solves % m.online_every == 0 && m.nsamples > 0) {
-    timer_begin(c, "mlp_train");
-    SPHX_TRY(mlp_online_refresh(c));
-    timer_end(c);
+  if (record_ok) {
+    ++m.solves;
+    if (online && m.solves % m.online_every == 0 && m.nsamples > 0) {
+      timer_begin(c, "mlp_train");
+      SPHX_TRY(mlp_online_refresh(c));
+      timer_end(c);
+    }
   }
+  return 0;
+}
+
+// The pressure solve of a step with the learned start, the recorder and
+// the fallback.  Called after the a_ii pass.
+int mlp_solve(sphx_context* c, int* iters, double* err) {
+  auto& m = c->mlp;
+  if (!m.ready || c->num == 0) return iisph_solve_plain(c, iters, err);
+  SPHX_TRY(mlp_begin(c));
+  int it = 0;
+  double e = 0.0;
+  SPHX_TRY(iisph_solve_plain(c, &it, &e));
+  SPHX_TRY(mlp_fallback(c, &it, &e));
+  SPHX_TRY(mlp_after(c, 1));
   if (iters) *iters = it;
   if (err) *err = e;
   return 0;

@@ -291,25 +291,34 @@ MFP make_mfp(const sphx_context* c) {
   return m;
 }
 
+__global__ void k_mf_copy(const float4* __restrict__ a, float4* __restrict__ b, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
 // Make the corrected fractions the current ones (alpha_new is per slot).
+// A copy, not a pointer swap: the buffers bound to every launch depend only
+// on the particle-buffer parity, so a captured step graph stays valid.
 int mf_commit(sphx_context* c) {
   if (!c->mf_pending) return 0;
-  std::swap(c->mf_alpha[c->cur], c->mf_alpha_new);
+  if (c->num > 0) {
+    k_mf_copy<<<blocks_for(c->num, 256), 256, 0, c->stream>>>(c->mf_alpha_new.p,
+                                                              c->mf_alpha[c->cur].p, c->num);
+    SPHX_HIP(hipGetLastError());
+  }
   c->mf_pending = false;
   return 0;
 }
 
 // Called by the grid reorder after it gathered the core fields from buffer
-// `a` into buffer `b`.
+// `a` into buffer `b`; pending corrected fractions are gathered directly.
 int mf_reorder(sphx_context* c, int a, int b) {
   if (c->nphase == 0 || c->num == 0) return 0;
-  if (c->mf_pending) {
-    std::swap(c->mf_alpha[a], c->mf_alpha_new);
-    c->mf_pending = false;
-  }
+  const float4* src = c->mf_pending ? c->mf_alpha_new.p : c->mf_alpha[a].p;
+  c->mf_pending = false;
   int n = c->num;
   k_mf_reorder<<<blocks_for(n, 256), 256, 0, c->stream>>>(
-      c->key.p, c->idsort.p, n, (uint32_t)c->P.gridtotal, c->mf_alpha[a].p, c->mf_alpha[b].p,
+      c->key.p, c->idsort.p, n, (uint32_t)c->P.gridtotal, src, c->mf_alpha[b].p,
       c->mf_force[a].p, c->mf_force[b].p);
   SPHX_HIP(hipGetLastError());
   return 0;
@@ -406,7 +415,7 @@ int sphx_mf_upload(sphx_context* c, const float* alpha, int start, int count) {
     f[i] = make_float4(c->P.gravity.x, c->P.gravity.y, c->P.gravity.z, 0.f);
     int t;
     std::memcpy(&t, &ps[i].w, 4);
-    if (t != SPHX_TYPE_BOUNDARY) v[i].w = nm;
+    if (type_bits(t) != SPHX_TYPE_BOUNDARY) v[i].w = nm;
   }
   SPHX_HIP(hipMemcpy(c->mf_alpha[c->cur].p + start, a.data(), count * sizeof(float4), hipMemcpyHostToDevice));
   SPHX_HIP(hipMemcpy(c->mf_force[c->cur].p + start, f.data(), count * sizeof(float4), hipMemcpyHostToDevice));

@@ -412,3 +412,98 @@ int sphx_scene_sand(int n, float lo, float hi, float dx, int res, float E, float
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Batched ensemble: nscenes two-phase tanks (as sphx_scene_bubbles) of
+// n_fluid[s] particles, each a cube of side (n_fluid[s] / 2^20)^(1/3) m
+// filled to `fill` (the spacing of the 2^20-particle tank), with
+// bubbles_per_million * n_fluid[s] / 2^20 bubbles seeded per scene, laid
+// out in disjoint, cell-aligned boxes of one grid (2-cell gaps, wider than
+// the kernel support, so the scenes never interact).  The scene index goes
+// into bits 8-23 of each particle's type word.  scene_first[s] receives the
+// first particle of scene s (length nscenes + 1).  Returns the particle
+// count (arrays may be NULL to query it).
+int sphx_scene_ensemble(int nscenes, const int* n_fluid, float fill, float h,
+                        float bubbles_per_million, float rmin, float rmax, int layers,
+                        uint64_t seed, sphx_ParamCarrier* params, sphx_displayPack* disp,
+                        sphx_calculationPack* calc, float* alpha, int* scene_first,
+                        sphx_SceneInfo* info) {
+  if (nscenes <= 0 || nscenes > 65535 || !n_fluid || !(h > 0)) return -1;
+  std::vector<float> side(nscenes);
+  std::vector<int> total(nscenes), fluid(nscenes);
+  std::vector<sphx_ParamCarrier> sp(nscenes);
+  float ext_max = 0.f;
+  int ntot = 0, nfl = 0, nbd = 0;
+  float spacing = 0.f;
+  for (int s = 0; s < nscenes; ++s) {
+    if (n_fluid[s] <= 0) return -1;
+    side[s] = (float)std::cbrt((double)n_fluid[s] / (double)(1 << 20));
+    int nb = (int)std::lround(bubbles_per_million * n_fluid[s] / (double)(1 << 20));
+    sphx_SceneInfo si;
+    int rc = sphx_scene_bubbles(n_fluid[s], side[s], side[s], side[s], fill, h, nb, rmin, rmax,
+                                layers, seed + 7919ull * (uint64_t)s, &sp[s], nullptr, nullptr,
+                                nullptr, &si);
+    if (rc < 0) return -1;
+    total[s] = si.n_total;
+    fluid[s] = si.n_fluid;
+    ntot += si.n_total;
+    nfl += si.n_fluid;
+    nbd += si.n_boundary;
+    spacing = si.spacing;
+    ext_max = std::max(ext_max, side[s] + 2.f * layers * si.spacing);
+  }
+  if (info) {
+    info->n_fluid = nfl;
+    info->n_boundary = nbd;
+    info->n_total = ntot;
+    info->n_phase = 2;
+    info->spacing = spacing;
+  }
+  const float cell = 2.f * h;
+  const int per = (int)std::ceil(std::cbrt((double)nscenes) - 1e-9);
+  const int slot_cells = (int)std::ceil(ext_max / cell) + 2;   // 2-cell gap
+  const float slot = slot_cells * cell;
+  const int wall_cells = (int)std::ceil(layers * spacing / cell) + 1;
+  if (params) {
+    *params = sp[0];
+    sphx_ParamCarrier& P = *params;
+    float ext = per * slot + 2.f * cell;
+    P.volmin = {-cell, -cell, -cell};
+    P.volmax = {ext, ext, ext};
+    P.softminx = {0.f, 0.f, 0.f};
+    P.softmaxx = {ext - cell, ext - cell, ext - cell};
+    sphx_setup_grid(&P);
+    P.num = ntot;
+    P.maxNum = ntot;
+  }
+  if (scene_first) {
+    int o = 0;
+    for (int s = 0; s < nscenes; ++s) { scene_first[s] = o; o += total[s]; }
+    scene_first[nscenes] = o;
+  }
+  if (!disp || !calc) return ntot;
+  int o = 0;
+  for (int s = 0; s < nscenes; ++s) {
+    int ix = s % per, iz = (s / per) % per, iy = s / (per * per);
+    float ox = ix * slot + wall_cells * cell, oy = iy * slot + wall_cells * cell,
+          oz = iz * slot + wall_cells * cell;
+    int nb = (int)std::lround(bubbles_per_million * n_fluid[s] / (double)(1 << 20));
+    sphx_SceneInfo si;
+    sphx_ParamCarrier tmp;
+    sphx_scene_bubbles(n_fluid[s], side[s], side[s], side[s], fill, h, nb, rmin, rmax, layers,
+                       seed + 7919ull * (uint64_t)s, &tmp, disp + o, calc + o,
+                       alpha ? alpha + 2 * (size_t)o : nullptr, &si);
+    for (int i = 0; i < total[s]; ++i) {
+      disp[o + i].pos.x += ox;
+      disp[o + i].pos.y += oy;
+      disp[o + i].pos.z += oz;
+      disp[o + i].type |= s << 8;
+      calc[o + i].bornid = o + i;
+    }
+    o += total[s];
+  }
+  return ntot;
+}
+
+}  // extern "C"

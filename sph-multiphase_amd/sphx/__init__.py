@@ -195,6 +195,14 @@ def lib() -> C.CDLL:
         L.sphx_dev_alloc.argtypes = [P(VP), C.c_size_t]
         L.sphx_dev_free.argtypes = [VP]
         L.sphx_dev_copy.argtypes = [VP, VP, C.c_size_t, I]
+        L.sphx_scene_ensemble.argtypes = [I, VP, C.c_float, C.c_float, C.c_float, C.c_float,
+                                          C.c_float, I, C.c_uint64, P(ParamCarrier), VP, VP, VP,
+                                          VP, P(SceneInfo)]
+        L.sphx_ensemble_setup.argtypes = [VP, I]
+        L.sphx_scene_stats.argtypes = [VP, VP, VP]
+        L.sphx_graph_build.argtypes = [VP, I]
+        L.sphx_graph_step.argtypes = [VP, P(I), P(D)]
+        L.sphx_graph_release.argtypes = [VP]
         _lib = L
     return _lib
 
@@ -252,6 +260,28 @@ def scene_bubbles(n_fluid=1 << 20, L=(1.0, 1.0, 1.0), fill=0.8, h=0.01, n_bubble
 
 
 # ----------------------------------------------------------------- context --
+def scene_ensemble(n_fluid, fill=0.8, h=0.01, bubbles_per_million=512.0, rmin=0.01, rmax=0.04,
+                   layers=2, seed=20261207):
+    """Batched ensemble of two-phase tanks (sphx_scene_ensemble); returns
+    params, disp, calc, alpha [n][2], scene_first [nscenes + 1], info."""
+    L = lib()
+    nf = np.ascontiguousarray(n_fluid, np.int32)
+    S = len(nf)
+    p = ParamCarrier()
+    info = SceneInfo()
+    first = np.zeros(S + 1, np.int32)
+    n = L.sphx_scene_ensemble(S, ptr(nf), fill, h, bubbles_per_million, rmin, rmax, layers, seed,
+                              C.byref(p), None, None, None, ptr(first), C.byref(info))
+    if n < 0:
+        raise SphxError("sphx_scene_ensemble: bad arguments")
+    disp = np.zeros(n, display_dtype)
+    calc = np.zeros(n, calc_dtype)
+    alpha = np.zeros((n, 2), np.float32)
+    L.sphx_scene_ensemble(S, ptr(nf), fill, h, bubbles_per_million, rmin, rmax, layers, seed,
+                          C.byref(p), ptr(disp), ptr(calc), ptr(alpha), ptr(first), C.byref(info))
+    return p, disp, calc, alpha, first, info
+
+
 def scene_sand(n=4096, lo=0.3, hi=0.6, dx=1.0 / 32, res=32, E=3.5e5, nu=0.3, phi=30.0,
                rho=1600.0, f_noise=1e-3, seed=20261205):
     """Granular block over a res^3 MPM node grid (sphx_scene_sand)."""
@@ -468,6 +498,25 @@ class Context:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int()
         check(self.L.sphx_mlp_stats(self.h, C.byref(a), C.byref(b), C.byref(c)), "sphx_mlp_stats")
         return dict(learned=a.value, fallbacks=b.value, last_fallback=c.value)
+
+    # -- batched ensemble / step graphs --
+    def ensemble_setup(self, nscenes):
+        check(self.L.sphx_ensemble_setup(self.h, nscenes), "sphx_ensemble_setup")
+        self.nscenes = nscenes
+
+    def scene_stats(self):
+        it = np.zeros(self.nscenes, np.int32)
+        err = np.zeros(self.nscenes, np.float64)
+        check(self.L.sphx_scene_stats(self.h, ptr(it), ptr(err)), "sphx_scene_stats")
+        return it, err
+
+    def graph_build(self, budget):
+        check(self.L.sphx_graph_build(self.h, budget), "sphx_graph_build")
+
+    def graph_step(self):
+        it, err = C.c_int(), C.c_double()
+        check(self.L.sphx_graph_step(self.h, C.byref(it), C.byref(err)), "sphx_graph_step")
+        return it.value, err.value
 
     # -- MPM --
     def mpm_setup(self, kernel=0, physical=0):
