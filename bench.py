@@ -367,6 +367,122 @@ def bench_mobility(args, sphx):
     print(json.dumps(line), flush=True)
 
 
+def bench_ensemble(args, sphx):
+    """Batched ensemble: 64 two-phase tanks with fluid counts log-uniform in
+    [20k, 400k] (seeded), bubble layouts seeded per scene, all in one grid so
+    every stage is one launch; the step runs through HIP graphs (host check
+    of the done flag between the solve graph and the integration graph).
+    Two ensembles from the same state: the plain solver and the learned warm
+    start (trained on >= 10M recorded samples, refreshed online every 10
+    steps by an eager recording step).  Workgroup sizes from the autotuner."""
+    S = 64
+    rng = np.random.default_rng(20261207)
+    sizes = np.exp(rng.uniform(np.log(2e4), np.log(4e5), S)).astype(np.int64)
+    p, disp, calc, alpha, first, info = sphx.scene_ensemble(sizes.tolist(), seed=20261208)
+    p.dt = 1e-3
+    o = options(sphx)
+    nf = int(info.n_fluid)
+    print(f"# ensemble: {S} scenes, {nf} fluid + {info.n_boundary} boundary particles, grid "
+          f"{p.gridres.x}^3", file=sys.stderr, flush=True)
+    ctxs = []
+    for _ in range(2):
+        c = sphx.Context(p, o)
+        c.upload(disp, calc)
+        c.mf_setup(RHO, NU)
+        c.mf_upload(alpha)
+        c.ensemble_setup(S)
+        c.call("grid_build")
+        ctxs.append(c)
+    del disp, calc
+    plain, learn = ctxs
+    for s in range(args.warmup):
+        plain.mf_step()
+        learn.mf_step()
+    # learned start: record >= 10M samples, train on the GPU
+    g = float(np.linalg.norm(p.gravity.tolist()))
+    d = learn.mf_download()["drift"][:, 1, :]
+    dl, _ = learn.download()
+    fl = dl["type"] == 0
+    U = max(float(np.sqrt(np.mean(np.sum(d[fl].astype(np.float64) ** 2, axis=1)))), 1e-9)
+    del d, dl, fl
+    learn.mlp_init(20261202, 1000.0 * g * 0.8 * float(np.cbrt(sizes.mean() / (1 << 20))), U)
+    rec = int(np.ceil(1e7 / nf))
+    learn.mlp_reserve((rec + 1) * learn.num)
+    learn.mlp_mode(record=True)
+    for _ in range(rec):
+        learn.mf_step()
+        plain.mf_step()
+    total, valid = learn.mlp_samples()
+    _, T, M = learn.mlp_download_samples(0, min(total, 1 << 21))
+    var = np.var(T[M > 0], axis=0) + 1e-12
+    lw = (4.0 / var[0], 1.0 / var[1], 1.0 / var[2], 1.0 / var[3])
+    t0 = time.perf_counter()
+    loss = learn.mlp_train(epochs=args.epochs, batch=args.batch, lr=args.lr, lw=lw, seed=5)
+    learn.sync()
+    train_s = time.perf_counter() - t0
+    learn.mlp_mode(warm=True, record=False)
+    learn.mlp_online(1, args.online_iters, args.online_batch, args.online_lr, (lw[0], 0, 0, 0))
+    print(f"# recorded {valid} samples in {rec} steps, trained {args.epochs} epochs in "
+          f"{train_s:.1f} s, loss {loss[-1]:.3e}", file=sys.stderr, flush=True)
+    # autotune (real steps on both ensembles, kept in step)
+    tp = plain.autotune(1)
+    tl = learn.autotune(1)
+    print(f"# tuned plain {tp}\n# tuned learned {tl}", file=sys.stderr, flush=True)
+    ip, _ = plain.mf_step()
+    il, _ = learn.mf_step()
+    plain.graph_build(max(2, ip))
+    learn.graph_build(max(2, il))
+    res = {}
+    for name, c, every in (("plain", plain, 0), ("learned", learn, args.online_every)):
+        c.sync()
+        its, errs = [], []
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            if every and s % every == every - 1:
+                it, err = c.mf_step()        # eager: records and refreshes online
+            else:
+                it, err = c.graph_step()
+            its.append(it)
+            errs.append(err)
+        c.sync()
+        el = time.perf_counter() - t0
+        res[name] = (el, its, errs)
+        print(f"# {name}: {1e3 * el / args.steps:.2f} ms/step, max-over-scenes iterations mean "
+              f"{np.mean(its):.2f} (max {max(its)}), max error {max(errs):.2e}", file=sys.stderr,
+              flush=True)
+    stats = learn.mlp_stats()
+    plain.close()
+    learn.close()
+    el, its, errs = res["learned"]
+    line = {
+        "metric": "aggregate particle-steps/s, 64-scene two-phase ensemble (log-uniform 20k-400k)",
+        "value": nf * args.steps / el,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "64 bubble tanks in one grid, one launch per stage, HIP-graph step",
+                   "scenes": S, "n_fluid": nf, "n_boundary": int(info.n_boundary),
+                   "pressure_iters_plain_mean": float(np.mean(res["plain"][1])),
+                   "pressure_iters_learned_mean": float(np.mean(its)),
+                   "iteration_count": "most iterations of any scene, per step",
+                   "plain_ms_per_step": 1e3 * res["plain"][0] / args.steps,
+                   "max_density_error_learned": float(max(errs)),
+                   "max_density_error_plain": float(max(res["plain"][2])),
+                   "learned_fallbacks": int(stats["fallbacks"]), "train_samples": int(valid),
+                   "train_seconds": train_s, "tuning": tl},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def timed_segment(ctx, steps, names):
     """`steps` timed steps between two device synchronisations; returns the
     wall time, per-step iterations and errors, and per-kernel event times."""
@@ -404,7 +520,8 @@ def main():
     ap.add_argument("--online-iters", type=int, default=16)
     ap.add_argument("--online-batch", type=int, default=8192)
     ap.add_argument("--online-lr", type=float, default=5e-4)
-    ap.add_argument("--workload", default="bubbles", choices=["bubbles", "sand", "mobility"],
+    ap.add_argument("--workload", default="bubbles",
+                    choices=["bubbles", "sand", "mobility", "ensemble"],
                     help="bubbles: two-phase tank (headline); sand: MPM 2M particles; "
                          "mobility: RPY 4096 spheres x 64 RHS")
     ap.add_argument("--rpy-split", type=int, default=0)
@@ -418,6 +535,13 @@ def main():
         print("no HIP device visible: bench.py needs an MI355X", file=sys.stderr)
         sys.exit(1)
 
+    if args.workload == "ensemble":
+        if args.steps == 200:
+            args.steps = 60
+        if args.warmup == 20:
+            args.warmup = 5
+        bench_ensemble(args, sphx)
+        return
     if args.workload == "mobility":
         if args.steps == 200:
             args.steps = 50

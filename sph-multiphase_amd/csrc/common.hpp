@@ -116,6 +116,17 @@ struct KP {
 
 constexpr int SPHX_MAX_SCENES = 256;
 
+// Workgroup size of each neighbour-loop kernel class, set by the autotuner
+// (multiples of 64 up to 256; the kernels do not depend on it otherwise).
+enum TuneId {
+  T_DENSITY, T_DII, T_AII, T_DP, T_ITER, T_NBR, T_MF_DRIFT, T_MF_ALPHA, T_MF_FORCE,
+  T_MLP_FEAT, T_MLP_INFER, T_COUNT
+};
+// timer names of the kernels, in TuneId order
+constexpr const char* TUNE_NAMES[T_COUNT] = {
+    "iisph_density", "iisph_dii", "iisph_aii", "iisph_dp", "iisph_iter", "nbr_build",
+    "mf_drift", "mf_alpha", "mf_force", "mlp_features", "mlp_infer"};
+
 // Per-scene solver state of a batched ensemble (device arrays).
 struct SceneCtl {
   int n;             // number of scenes (0: single scene)
@@ -275,6 +286,9 @@ struct sphx_context {
   // iterations) and graph B (integration), one of each per buffer parity
   hipGraphExec_t graph_a[2] = {nullptr, nullptr}, graph_b[2] = {nullptr, nullptr};
   int graph_budget = 0, graph_num = -1;
+
+  // workgroup sizes per kernel class (TuneId)
+  int tune[sphx::T_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256, 256, 256, 256};
 
   // timing
   bool timing = false;

@@ -14,6 +14,8 @@
 // with a memset node (the solver state is reset by a kernel).
 #include "common.hpp"
 
+#include <cstring>
+
 namespace sphx {
 
 int enqueue_solve_fixed(sphx_context* c, int n);
@@ -137,6 +139,74 @@ int sphx_graph_step(sphx_context* c, int* iters, double* err) {
 int sphx_graph_release(sphx_context* c) {
   SPHX_CHECK(c, "sphx_graph_release", "null context");
   graph_release(c);
+  return 0;
+}
+
+}  // extern "C"
+
+namespace sphx {
+
+static int tune_index(const char* name) {
+  for (int t = 0; t < T_COUNT; ++t)
+    if (std::strcmp(name, TUNE_NAMES[t]) == 0) return t;
+  return -1;
+}
+
+}  // namespace sphx
+
+extern "C" {
+
+int sphx_tune_set(sphx_context* c, const char* kernel, int block) {
+  SPHX_CHECK(c && kernel, "sphx_tune_set", "null argument");
+  int t = tune_index(kernel);
+  SPHX_CHECK(t >= 0, "sphx_tune_set", "unknown kernel name");
+  SPHX_CHECK(block >= 64 && block <= 256 && block % 64 == 0, "sphx_tune_set",
+             "block must be 64, 128, 192 or 256");
+  c->tune[t] = block;
+  return 0;
+}
+
+int sphx_tune_get(sphx_context* c, const char* kernel, int* block) {
+  SPHX_CHECK(c && kernel && block, "sphx_tune_get", "null argument");
+  int t = tune_index(kernel);
+  SPHX_CHECK(t >= 0, "sphx_tune_get", "unknown kernel name");
+  *block = c->tune[t];
+  return 0;
+}
+
+// Project autotuner: times every neighbour-loop kernel class of the
+// two-phase step at each candidate workgroup size over `steps` real steps
+// (HIP events on the context stream) and keeps, per kernel, the size with
+// the shortest mean launch.  The steps advance the simulation.  Step graphs
+// are recaptured with the new sizes.  Block sizes do not change any
+// particle's arithmetic.
+int sphx_autotune(sphx_context* c, int steps) {
+  SPHX_CHECK(c, "sphx_autotune", "null context");
+  SPHX_CHECK(c->nphase > 0, "sphx_autotune", "tunes the two-phase step (call sphx_mf_setup)");
+  SPHX_CHECK(steps >= 1, "sphx_autotune", "steps must be >= 1");
+  const int cand[3] = {64, 128, 256};
+  double best[T_COUNT];
+  int pick[T_COUNT];
+  for (int t = 0; t < T_COUNT; ++t) { best[t] = 1e300; pick[t] = c->tune[t]; }
+  const bool timing = c->timing;
+  const bool had_graph = c->graph_num >= 0;
+  c->timing = true;
+  for (int k = 0; k < 3; ++k) {
+    for (int t = 0; t < T_COUNT; ++t) c->tune[t] = cand[k];
+    SPHX_TRY(sphx_timing_reset(c));
+    for (int s = 0; s < steps; ++s) SPHX_TRY(sphx_mf_step(c, nullptr, nullptr));
+    SPHX_TRY(timer_flush(c));
+    for (int t = 0; t < T_COUNT; ++t) {
+      double ms = 0.0;
+      int n = 0;
+      SPHX_TRY(sphx_timing_get(c, TUNE_NAMES[t], &ms, &n));
+      if (n > 0 && ms / n < best[t]) { best[t] = ms / n; pick[t] = cand[k]; }
+    }
+  }
+  for (int t = 0; t < T_COUNT; ++t) c->tune[t] = pick[t];
+  SPHX_TRY(sphx_timing_reset(c));
+  c->timing = timing;
+  if (had_graph) SPHX_TRY(sphx_graph_build(c, c->graph_budget));
   return 0;
 }
 

@@ -197,7 +197,7 @@ __global__ void k_ii_dp(KP k, NbrList L, const int* __restrict__ done, const uin
 
 // Block reduction of (signed residue, |residue|/rho0, fluid count) in FP64.
 __device__ void block_reduce3(double a, double b, double c, double* out) {
-  __shared__ double sh[3][NB_THREADS / 64];
+  __shared__ double sh[3][NB_THREADS / 64];   // blockDim <= NB_THREADS
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int off = 32; off > 0; off >>= 1) {
     a += __shfl_down(a, off, 64);
@@ -208,7 +208,7 @@ __device__ void block_reduce3(double a, double b, double c, double* out) {
   __syncthreads();
   if (threadIdx.x == 0) {
     double x = 0, y = 0, z = 0;
-    for (int q = 0; q < NB_THREADS / 64; ++q) { x += sh[0][q]; y += sh[1][q]; z += sh[2][q]; }
+    for (int q = 0; q < (int)(blockDim.x + 63) / 64; ++q) { x += sh[0][q]; y += sh[1][q]; z += sh[2][q]; }
     out[3 * blockIdx.x + 0] = x;
     out[3 * blockIdx.x + 1] = y;
     out[3 * blockIdx.x + 2] = z;
@@ -510,7 +510,8 @@ int nbr_list(sphx_context* c, NbrList* out) {
     if (n > 0) {
       KP k = make_kp(c);
       timer_begin(c, "nbr_build");
-      k_nbr_build<<<blocks_for(n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
+      const int tb = c->tune[T_NBR];
+      k_nbr_build<<<blocks_for(n, tb), tb, 0, c->stream>>>(
           k, c->key.p, c->cell_start.p, c->POS(), c->nbr_idx.p, c->nbr_cnt.p, c->ps.p, cap);
       timer_end(c);
       SPHX_HIP(hipGetLastError());
@@ -524,7 +525,7 @@ int nbr_list(sphx_context* c, NbrList* out) {
 }
 
 static int ensure_red(sphx_context* c) {
-  size_t nb = (size_t)blocks_for(c->cap, NB_THREADS);
+  size_t nb = (size_t)blocks_for(c->cap, 64);   // smallest tuned workgroup
   return c->red.alloc(3 * nb + 16);
 }
 
@@ -537,18 +538,19 @@ int solve_reset(sphx_context* c) {
 
 // one DP + Iter + finalize; respect_done=1 inside the pipelined solve
 static int enqueue_iteration(sphx_context* c, const KP& k, int respect_done) {
-  int nb = blocks_for(k.n, NB_THREADS);
+  const int tdp = c->tune[T_DP], tit = c->tune[T_ITER];
+  int nb = blocks_for(k.n, tit);
   NbrList L;
   SPHX_TRY(nbr_list(c, &L));
   IterCtl ctl{c->O.min_iters, c->O.max_iters, c->O.tol};
   SceneCtl sc = c->scenes();
   timer_begin(c, "iisph_dp");
-  k_ii_dp<<<nb, NB_THREADS, 0, c->stream>>>(k, L, c->solve_state.p, c->key.p, c->cell_start.p,
+  k_ii_dp<<<blocks_for(k.n, tdp), tdp, 0, c->stream>>>(k, L, c->solve_state.p, c->key.p, c->cell_start.p,
                                             c->POS(), c->ps.p, c->PHYS(), c->dii.p,
                                             c->press_l.p, c->dijpj.p, c->qm.p, sc);
   timer_end(c);
   timer_begin(c, "iisph_iter");
-  k_ii_iter<<<nb, NB_THREADS, 0, c->stream>>>(k, L, c->solve_state.p, c->key.p, c->cell_start.p,
+  k_ii_iter<<<nb, tit, 0, c->stream>>>(k, L, c->solve_state.p, c->key.p, c->cell_start.p,
                                               c->POS(), c->PHYS(), c->dii.p, c->dijpj.p, c->qm.p,
                                               c->rho_adv.p, c->press_l.p, c->press_l1.p,
                                               c->ps.p, c->residue.p, c->ACCEL(), c->red.p, sc);
@@ -589,7 +591,7 @@ int sphx_iisph_density(sphx_context* c) {
   NbrList L;
   SPHX_TRY(nbr_list(c, &L));
   timer_begin(c, "iisph_density");
-  k_ii_density<<<blocks_for(k.n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
+  k_ii_density<<<blocks_for(k.n, c->tune[T_DENSITY]), c->tune[T_DENSITY], 0, c->stream>>>(
       k, L, c->key.p, c->cell_start.p, c->POS(), c->PHYS());
   timer_end(c);
   SPHX_HIP(hipGetLastError());
@@ -603,7 +605,7 @@ int sphx_iisph_predict_dii(sphx_context* c) {
   NbrList L;
   SPHX_TRY(nbr_list(c, &L));
   timer_begin(c, "iisph_dii");
-  k_ii_dii<<<blocks_for(k.n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
+  k_ii_dii<<<blocks_for(k.n, c->tune[T_DII]), c->tune[T_DII], 0, c->stream>>>(
       k, L, c->key.p, c->cell_start.p, c->POS(), c->VEL(), c->PHYS(), c->VELEVAL(), c->dii.p);
   timer_end(c);
   SPHX_HIP(hipGetLastError());
@@ -617,7 +619,7 @@ int sphx_iisph_predict_aii(sphx_context* c) {
   NbrList L;
   SPHX_TRY(nbr_list(c, &L));
   timer_begin(c, "iisph_aii");
-  k_ii_aii<<<blocks_for(k.n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
+  k_ii_aii<<<blocks_for(k.n, c->tune[T_AII]), c->tune[T_AII], 0, c->stream>>>(
       k, L, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p, c->rho_adv.p,
       c->press_l.p, c->ps.p);
   timer_end(c);

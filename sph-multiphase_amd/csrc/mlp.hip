@@ -608,7 +608,7 @@ int mlp_begin(sphx_context* c) {
     NbrList L;
     SPHX_TRY(nbr_list(c, &L));
     timer_begin(c, "mlp_features");
-    k_mlp_features<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+    k_mlp_features<<<blocks_for(k.n, c->tune[T_MLP_FEAT]), c->tune[T_MLP_FEAT], 0, c->stream>>>(
         k, L, mp, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p,
         c->rho_adv.p, mf ? c->mf_alpha[c->cur].p : nullptr, hist, m.X.p, m.pprev.p);
     timer_end(c);
@@ -619,7 +619,7 @@ int mlp_begin(sphx_context* c) {
   float4* dcorr = m.drift && mf ? hist : nullptr;
   if (m.warm || dcorr) {
     timer_begin(c, "mlp_infer");
-    k_mlp_infer<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+    k_mlp_infer<<<blocks_for(k.n, c->tune[T_MLP_INFER]), c->tune[T_MLP_INFER], 0, c->stream>>>(
         k, mp, m.warm, m.X.p, reinterpret_cast<const __half*>(m.w16.p), c->POS(), c->key.p,
         c->PHYS(), c->press_l.p, c->ps.p, dcorr);
     timer_end(c);

@@ -336,7 +336,7 @@ int mf_drift_into(sphx_context* c, const float4* dcorr, float4* out0, float4* ou
   MFP m = make_mfp(c);
   NbrList L;
   SPHX_TRY(nbr_list(c, &L));
-  k_mf_drift<<<blocks_for(k.n, MF_THREADS), MF_THREADS, 0, c->stream>>>(
+  k_mf_drift<<<blocks_for(k.n, c->tune[T_MF_DRIFT]), c->tune[T_MF_DRIFT], 0, c->stream>>>(
       k, m, L, c->key.p, c->cell_start.p, c->POS(), c->VEL(), c->PHYS(), c->mf_alpha[c->cur].p,
       c->mf_force[c->cur].p, dcorr, out0, out1, out2);
   SPHX_HIP(hipGetLastError());
@@ -479,7 +479,7 @@ int sphx_mf_alpha_advance(sphx_context* c) {
   NbrList L;
   SPHX_TRY(nbr_list(c, &L));
   timer_begin(c, "mf_alpha");
-  k_mf_alpha<<<blocks_for(k.n, MF_THREADS), MF_THREADS, 0, c->stream>>>(
+  k_mf_alpha<<<blocks_for(k.n, c->tune[T_MF_ALPHA]), c->tune[T_MF_ALPHA], 0, c->stream>>>(
       k, m, L, c->key.p, c->cell_start.p, c->POS(), c->VEL(), c->PHYS(), c->mf_alpha[c->cur].p,
       c->mf_drift[0].p, c->mf_drift[1].p, c->mf_drift[2].p, c->mf_alpha_new.p);
   timer_end(c);
@@ -512,7 +512,7 @@ int sphx_mf_force(sphx_context* c) {
   SPHX_TRY(nbr_list(c, &L));
   const float4* an = c->mf_pending ? c->mf_alpha_new.p : c->mf_alpha[c->cur].p;
   timer_begin(c, "mf_force");
-  k_mf_force<<<blocks_for(k.n, MF_THREADS), MF_THREADS, 0, c->stream>>>(
+  k_mf_force<<<blocks_for(k.n, c->tune[T_MF_FORCE]), c->tune[T_MF_FORCE], 0, c->stream>>>(
       k, m, L, c->key.p, c->cell_start.p, c->POS(), c->PHYS(), c->mf_alpha[c->cur].p, an,
       c->mf_drift[0].p, c->mf_drift[1].p, c->mf_drift[2].p, c->VELEVAL(), c->mf_amf.p);
   timer_end(c);

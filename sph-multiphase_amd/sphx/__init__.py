@@ -203,6 +203,9 @@ def lib() -> C.CDLL:
         L.sphx_graph_build.argtypes = [VP, I]
         L.sphx_graph_step.argtypes = [VP, P(I), P(D)]
         L.sphx_graph_release.argtypes = [VP]
+        L.sphx_tune_set.argtypes = [VP, C.c_char_p, I]
+        L.sphx_tune_get.argtypes = [VP, C.c_char_p, P(I)]
+        L.sphx_autotune.argtypes = [VP, I]
         _lib = L
     return _lib
 
@@ -517,6 +520,24 @@ class Context:
         it, err = C.c_int(), C.c_double()
         check(self.L.sphx_graph_step(self.h, C.byref(it), C.byref(err)), "sphx_graph_step")
         return it.value, err.value
+
+    TUNE_KERNELS = ("iisph_density", "iisph_dii", "iisph_aii", "iisph_dp", "iisph_iter",
+                    "nbr_build", "mf_drift", "mf_alpha", "mf_force", "mlp_features", "mlp_infer")
+
+    def tune_set(self, kernel, block):
+        check(self.L.sphx_tune_set(self.h, kernel.encode(), block), "sphx_tune_set")
+
+    def tuning(self):
+        out = {}
+        for k in self.TUNE_KERNELS:
+            v = C.c_int()
+            check(self.L.sphx_tune_get(self.h, k.encode(), C.byref(v)), "sphx_tune_get")
+            out[k] = v.value
+        return out
+
+    def autotune(self, steps=2):
+        check(self.L.sphx_autotune(self.h, steps), "sphx_autotune")
+        return self.tuning()
 
     # -- MPM --
     def mpm_setup(self, kernel=0, physical=0):

@@ -116,3 +116,25 @@ def test_graph_steps_equal_eager_steps(gpu, learned):
             assert sa["fallbacks"] == sb["fallbacks"]
             if learned == "fallback":
                 assert sb["fallbacks"] >= 6
+
+
+def test_block_sizes_leave_results_unchanged_and_autotune(gpu):
+    sizes = [3000, 4000]
+    p, disp, calc, alpha, first, info = sphx.scene_ensemble(sizes, seed=9)
+    with make(p, disp, calc, alpha, 2) as A, make(p, disp, calc, alpha, 2) as B:
+        for k in B.TUNE_KERNELS:
+            B.tune_set(k, 64)
+        for _ in range(3):
+            ia, _ = A.mf_step()
+            ib, _ = B.mf_step()
+            assert ia == ib
+        dA, cA = by_born(A)
+        dB, cB = by_born(B)
+        np.testing.assert_array_equal(dA["pos"], dB["pos"])
+        np.testing.assert_array_equal(cA["pressure"], cB["pressure"])
+        B.graph_build(2)
+        t = B.autotune(1)
+        print("tuned", t)
+        assert set(t.values()) <= {64, 128, 256}
+        B.graph_step()
+        B.sync()
