@@ -396,8 +396,9 @@ def bench_ensemble(args, sphx):
     del disp, calc
     plain, learn = ctxs
     for s in range(args.warmup):
-        plain.mf_step()
+        it, err = plain.mf_step()
         learn.mf_step()
+        print(f"# warmup step {s}: {it} iterations, error {err:.2e}", file=sys.stderr, flush=True)
     # learned start: record >= 10M samples, train on the GPU
     g = float(np.linalg.norm(p.gravity.tolist()))
     d = learn.mf_download()["drift"][:, 1, :]
@@ -406,7 +407,7 @@ def bench_ensemble(args, sphx):
     U = max(float(np.sqrt(np.mean(np.sum(d[fl].astype(np.float64) ** 2, axis=1)))), 1e-9)
     del d, dl, fl
     learn.mlp_init(20261202, 1000.0 * g * 0.8 * float(np.cbrt(sizes.mean() / (1 << 20))), U)
-    rec = int(np.ceil(1e7 / nf))
+    rec = max(int(np.ceil(1e7 / nf)), args.record_steps)
     learn.mlp_reserve((rec + 1) * learn.num)
     learn.mlp_mode(record=True)
     for _ in range(rec):
@@ -447,6 +448,7 @@ def bench_ensemble(args, sphx):
         c.sync()
         el = time.perf_counter() - t0
         res[name] = (el, its, errs)
+        print(f"# {name} iterations " + " ".join(map(str, its)), file=sys.stderr)
         print(f"# {name}: {1e3 * el / args.steps:.2f} ms/step, max-over-scenes iterations mean "
               f"{np.mean(its):.2f} (max {max(its)}), max error {max(errs):.2e}", file=sys.stderr,
               flush=True)
@@ -538,8 +540,6 @@ def main():
     if args.workload == "ensemble":
         if args.steps == 200:
             args.steps = 60
-        if args.warmup == 20:
-            args.warmup = 5
         bench_ensemble(args, sphx)
         return
     if args.workload == "mobility":
