@@ -142,13 +142,13 @@ def cpu_baseline(sample_fluid=131072, mean_iters=10.0, timed_iters=20):
 
 
 MPM_ALGO_BYTES = {
-    "mpm_p2g": 100,      # x, v, m, B (84) -> node {mv, m} (16 per node, ~1 node/particle)
+    "mpm_p2g": 88,       # per particle: x, v, m (phys), B, key
+    "mpm_gather": 16,    # per active node: {v, m} w
     "mpm_stress": 120,   # x, phys, sigma r/w, G/K, perm (node gathers served by L2)
-    "mpm_force": 88,     # x, phys, sigma, key -> node force (16)
+    "mpm_force": 72,     # per particle: x, phys, sigma, key
     "mpm_g2p": 100,      # x r/w, v r/w, B w
-    "mpm_node": 48,      # per active node: {v, m} r/w, force r
-    "mpm_normalize": 32, # per active node: acc r, {v, m} w
-    "mpm_clear": 32,     # per active node: acc, force w
+    "mpm_node": 32,      # per active node: {v, m} r/w
+    "mpm_clear": 4,      # per active block: flag w
     "reorder": 104,
 }
 
@@ -172,7 +172,7 @@ def bench_sand(args, sphx):
         ctx.call("mpm_substep")
     ctx.sync()
     names = ["hash", "radix_sort", "cell_start", "reorder", "mpm_clear", "mpm_p2g",
-             "mpm_normalize", "mpm_stress", "mpm_force", "mpm_node", "mpm_g2p"]
+             "mpm_gather", "mpm_stress", "mpm_force", "mpm_node", "mpm_g2p"]
     ctx.timing(True)
     ctx.timing_reset()
     t0 = time.perf_counter()
@@ -195,7 +195,8 @@ def bench_sand(args, sphx):
     dom = max(kt, key=lambda k: kt[k][0])
     t_dom, n_dom = kt[dom]
     avg_ms = t_dom / max(n_dom, 1)
-    units = nodes_active if dom in ("mpm_node", "mpm_normalize", "mpm_clear") else n
+    units = (nodes_active if dom in ("mpm_node", "mpm_gather") else
+             active if dom == "mpm_clear" else n)
     algo = MPM_ALGO_BYTES.get(dom, 0) * units
     achieved = algo / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     cpu = None

@@ -231,8 +231,8 @@ struct sphx_context {
   sphx::DevBuf<float4> mf_alpha_new, mf_drift[3], mf_amf;
   bool mf_pending = false;        // alpha_new holds the corrected fractions
 
-  // MPM transfer (particle-centric scatter into a dense node grid whose
-  // touched 4^3-node blocks are tracked in a list)
+  // MPM transfer (particle-centric scatter into per-workgroup node tiles,
+  // summed per touched 4^3-node block; the touched blocks are listed)
   struct MPM {
     bool on = false;
     int kernel = 0;        // 0 cubic (reference), 1 quadratic
@@ -241,6 +241,10 @@ struct sphx_context {
     sphx::DevBuf<float> B[2], S[2];          // [9][cap] component-major, permuted
     sphx::DevBuf<float4> acc, force, nodev;  // per node: {mv, m}, {f}, {v, m}
     sphx::DevBuf<int> flags, list, count;    // per node block; active list
+    sphx::DevBuf<float4> tiles;              // [workgroup][tile] node sums
+    sphx::DevBuf<int4> hdr;                  // [workgroup][3] tile boxes
+    sphx::DevBuf<int> direct;                // [2] a pass added directly
+    size_t tiles_cap = 0;                    // workgroups the tiles hold
     int nblk[3] = {0, 0, 0};
     size_t nodes = 0, blocks = 0;
   } mpm;

@@ -5,12 +5,15 @@
 // particle-centric gathers over 4^3 nodes (stress :477-592, G2P :891-951),
 // one thread per node or particle.  Here every transfer is particle-centric
 // over the cell-sorted particles:
-//   * P2G and the node force are scatters.  A workgroup of 256 consecutive
-//     sorted particles accumulates into an LDS tile covering the bounding
-//     box of its stencils (ds_add_f32), then adds each touched tile node to
-//     global memory once: ~14 global atomics per particle at one particle
-//     per cell instead of 4 x 27 (quadratic) or 4 x 64 (cubic).  A
-//     workgroup whose box exceeds the tile adds directly.
+//   * P2G and the node force run in two passes without float atomics.  A
+//     workgroup of 256 consecutive sorted particles bins them by stencil
+//     base cell in LDS (one int atomic each), and each thread then sums,
+//     for the nodes it owns in the bounding box of the stencils, the
+//     particles of the W x W cell rows that reach the node; the box is
+//     stored whole as the workgroup's tile.  A per-node-block pass sums
+//     the tiles of the workgroups whose sorted particle range can reach the
+//     block (found through the sort grid's cell_start).  Particles of a
+//     box too large for a tile add to the nodes directly with atomics.
 //   * The node grid is dense (y-major ids as GetMpmIdx :35-38); only the
 //     4^3-node blocks a transfer touched are listed, cleared and updated,
 //     so a mostly empty grid costs nothing.
@@ -112,102 +115,381 @@ __device__ __forceinline__ int block_id(const MP& m, int bx, int by, int bz) {
 }
 
 __device__ __forceinline__ void mark_block(int* flags, int* list, int* count, int b) {
-  if (atomicCAS(&flags[b], 0, 1) == 0) list[atomicAdd(count, 1)] = b;
+  if (flags[b] == 0 && atomicCAS(&flags[b], 0, 1) == 0) list[atomicAdd(&count[0], 1)] = b;
 }
 
-// Adds contrib(s, a, b, c) (float4) of every valid lane to the nodes of its
-// stencil: through an LDS tile when the workgroup's box fits, else
-// directly.  Marks every node block the stencils overlap.  Must be called
-// by the whole workgroup (it synchronises).
-template <int W, class F>
-__device__ void scatter(const MP& m, bool valid, const Stencil<W>& s, float4* __restrict__ out,
-                        int* __restrict__ flags, int* __restrict__ list, int* __restrict__ count,
-                        F&& contrib) {
-  __shared__ float4 tile[MPM_TILE];
-  __shared__ int box[7];
-  if (threadIdx.x < 3) { box[threadIdx.x] = INT_MAX; box[3 + threadIdx.x] = INT_MIN; }
-  if (threadIdx.x == 0) box[6] = 0;
+// Where a scatter leaves its sums.  Workgroup w (sorted particles
+// [256 w, 256 w + 256)) writes its node sums, box by box, to
+// tiles[w * MPM_TILE ...] and the boxes to hdr[3 w ..]:
+//   hdr[3w]   = {lo0.x, lo0.y, lo0.z, ex0.x}
+//   hdr[3w+1] = {ex0.y, ex0.z, lo1.x, lo1.y}
+//   hdr[3w+2] = {lo1.z, ex1.x, ex1.y, ex1.z}
+// (extent 0: no tile for that box).  Tile entries are y-major within a box,
+// ((y - lo.y) ex.z + (z - lo.z)) ex.x + (x - lo.x); box 1 follows box 0.
+// Particles of a box that does not fit add to `direct` with atomics and
+// set *direct_flag.
+struct TileOut {
+  float4* tiles;
+  int4* hdr;
+  float4* direct;
+  int* direct_flag;
+  int* flags;
+  int* list;
+  int* count;
+};
+
+// Spline weight and derivative along one axis for node n (argument
+// (x_p - x_I) / dx, x_I = xmin + n dx: the expression of make_stencil).
+template <int W>
+__device__ __forceinline__ void axis_weight(float pc, float xm, int n, float dx, float& w, float& dw) {
+  float xi = xm + (float)n * dx;
+  float r = (pc - xi) / dx;
+  w = W == 4 ? sp_cubic(r) : sp_quad(r);
+  dw = W == 4 ? sp_dcubic(r) : sp_dquad(r);
+}
+
+template <int W>
+__device__ __forceinline__ void stencil_base(const MP& m, float4 p, int base[3]) {
+  const float pc[3] = {p.x, p.y, p.z};
+  const float xm[3] = {m.xmin.x, m.xmin.y, m.xmin.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float rel = (pc[a] - xm[a]) / m.dx;
+    base[a] = W == 4 ? (int)rel - 1 : (int)floorf(rel - 0.5f);
+  }
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+constexpr int MPM_CELLS = 1536;   // base cells binned per workgroup
+
+// LDS of a workgroup's binning.
+struct BinShared {
+  int sb[13];                   // [box][lo x y z, hi x y z] of bases, first row
+  int cnt[MPM_CELLS];
+  int start[MPM_CELLS + 1];
+  int part[MPM_THREADS / 64];
+};
+
+// The workgroup's two boxes (first sort-grid row, the rest): base cells
+// lo .. lo + nb - 1 per axis, tile nodes lo .. lo + ex - 1 (ex = nb + W - 1;
+// nb = ex = 0 for a box not tiled), and where each box's tile entries and
+// cell bins start.
+struct BinBox {
+  int lo[2][3], nb[2][3], ex[2][3];
+  int off[2], coff[2];
+  // element k of box g (g varies by lane: a select, not an indexed load)
+  __device__ __forceinline__ static int get(const int (&a)[2][3], int g, int k) {
+    return g ? a[1][k] : a[0][k];
+  }
+  __device__ __forceinline__ bool tiled(int g) const { return get(nb, g, 0) > 0; }
+};
+
+// Bins the valid particles of the workgroup by base cell.  The particles
+// split into the first sort-grid row (y, z) they occupy and the rest, so a
+// workgroup that wraps from the end of one row to the start of the next
+// keeps two compact boxes.  A box is tiled when its nodes fit MPM_TILE and
+// its cells MPM_CELLS; seg is the lane's box, slot its place in the binned
+// order (-1: not binned).  S.start[c] .. S.start[c + 1] are the slots of
+// cell c; cells of one x-row are consecutive.  Must be called by the whole
+// workgroup (it synchronises).
+template <int W>
+__device__ void block_bins(BinShared& S, bool valid, int row, const int base[3], BinBox& B,
+                           int& seg, int& slot) {
+  const int t0 = threadIdx.x, lane = t0 & 63, wv = t0 >> 6;
+  if (t0 < 13) S.sb[t0] = (t0 % 6 < 3 || t0 == 12) ? INT_MAX : INT_MIN;
   __syncthreads();
-  if (valid) {
-    for (int a = 0; a < 3; ++a) { atomicMin(&box[a], s.base[a]); atomicMax(&box[3 + a], s.base[a]); }
-    box[6] = 1;
+  const int r0 = wave_min(valid ? row : INT_MAX);
+  if (lane == 0 && r0 != INT_MAX) atomicMin(&S.sb[12], r0);
+  __syncthreads();
+  seg = valid && row != S.sb[12] ? 1 : 0;
+#pragma unroll
+  for (int g = 0; g < 2; ++g)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const bool in = valid && seg == g;
+      const int mn = wave_min(in ? base[a] : INT_MAX);
+      const int mx = wave_max(in ? base[a] : INT_MIN);
+      if (lane == 0 && mn != INT_MAX) {
+        atomicMin(&S.sb[6 * g + a], mn);
+        atomicMax(&S.sb[6 * g + 3 + a], mx);
+      }
+    }
+  __syncthreads();
+  long long V[2], C[2];
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    const bool has = S.sb[6 * g] != INT_MAX;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      B.lo[g][a] = S.sb[6 * g + a];
+      B.nb[g][a] = has ? S.sb[6 * g + 3 + a] - B.lo[g][a] + 1 : 0;
+      B.ex[g][a] = has ? B.nb[g][a] + W - 1 : 0;
+    }
+    V[g] = (long long)B.ex[g][0] * B.ex[g][1] * B.ex[g][2];
+    C[g] = (long long)B.nb[g][0] * B.nb[g][1] * B.nb[g][2];
+  }
+  const bool tiled0 = V[0] > 0 && V[0] <= MPM_TILE && C[0] <= MPM_CELLS;
+  B.off[0] = 0;
+  B.coff[0] = 0;
+  B.off[1] = tiled0 ? (int)V[0] : 0;
+  B.coff[1] = tiled0 ? (int)C[0] : 0;
+  const bool tiled1 = V[1] > 0 && B.off[1] + V[1] <= MPM_TILE && B.coff[1] + C[1] <= MPM_CELLS;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (!tiled0) B.nb[0][a] = B.ex[0][a] = 0;
+    if (!tiled1) B.nb[1][a] = B.ex[1][a] = 0;
+  }
+  const int ctot = B.coff[1] + (tiled1 ? (int)C[1] : 0);
+  for (int c = t0; c < ctot; c += blockDim.x) S.cnt[c] = 0;
+  __syncthreads();
+  int cb = -1, rank = 0;
+  if (valid && B.tiled(seg)) {
+    const int l0 = B.get(B.lo, seg, 0), l1 = B.get(B.lo, seg, 1), l2 = B.get(B.lo, seg, 2);
+    const int n0 = B.get(B.nb, seg, 0), n2 = B.get(B.nb, seg, 2);
+    cb = (seg ? B.coff[1] : 0) + ((base[1] - l1) * n2 + (base[2] - l2)) * n0 + (base[0] - l0);
+    rank = atomicAdd(&S.cnt[cb], 1);
   }
   __syncthreads();
-  if (box[6] == 0) return;
-  const int lo[3] = {box[0], box[1], box[2]};
-  const int hi[3] = {box[3] + W - 1, box[4] + W - 1, box[5] + W - 1};
-  const long long ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
-  const long long V = ex * ey * ez;
-  if (V <= MPM_TILE) {
-    for (int t = threadIdx.x; t < (int)V; t += blockDim.x) tile[t] = make_float4(0.f, 0.f, 0.f, 0.f);
-    __syncthreads();
-    if (valid) {
+  // exclusive scan of cnt[0 .. ctot) into start[0 .. ctot]
+  const int per = (ctot + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int c0 = min(t0 * per, ctot), c1 = min(c0 + per, ctot);
+  int local = 0;
+  for (int c = c0; c < c1; ++c) local += S.cnt[c];
+  int incl = local;
 #pragma unroll
-      for (int b = 0; b < W; ++b)
-#pragma unroll
-        for (int c = 0; c < W; ++c)
-#pragma unroll
-          for (int a = 0; a < W; ++a) {
-            int x = s.base[0] + a, y = s.base[1] + b, z = s.base[2] + c;
-            if (!in_res(m, x, y, z)) continue;
-            float4 v = contrib(a, b, c);
-            int li = (int)(((y - lo[1]) * ez + (z - lo[2])) * ex + (x - lo[0]));
-            atomicAdd(&tile[li].x, v.x);
-            atomicAdd(&tile[li].y, v.y);
-            atomicAdd(&tile[li].z, v.z);
-            atomicAdd(&tile[li].w, v.w);
-          }
+  for (int o = 1; o < 64; o <<= 1) {
+    int v = __shfl_up(incl, o);
+    if (lane >= o) incl += v;
+  }
+  if (lane == 63) S.part[wv] = incl;
+  __syncthreads();
+  int run = incl - local;
+  for (int k = 0; k < wv; ++k) run += S.part[k];
+  for (int c = c0; c < c1; ++c) {
+    S.start[c] = run;
+    run += S.cnt[c];
+  }
+  if (t0 == (int)blockDim.x - 1) S.start[ctot] = run;
+  __syncthreads();
+  slot = cb >= 0 ? S.start[cb] + rank : -1;
+}
+
+template <int W, int G, class F>
+__device__ __forceinline__ void tile_sums_box(const MP& m, const BinShared& S, const BinBox& B,
+                                              float4* __restrict__ dst, F& pair_sum) {
+  const int l0 = B.lo[G][0], l1 = B.lo[G][1], l2 = B.lo[G][2];
+  const int e0 = B.ex[G][0], e1 = B.ex[G][1], e2 = B.ex[G][2];
+  const int n0 = B.nb[G][0], n1 = B.nb[G][1], n2 = B.nb[G][2];
+  if (e0 == 0) return;
+  const int V = e0 * e1 * e2, coff = B.coff[G], off = B.off[G];
+  for (int t = threadIdx.x; t < V; t += blockDim.x) {
+    const int tx = t % e0, tz = (t / e0) % e2, ty = t / (e0 * e2);
+    const int x = l0 + tx, y = l1 + ty, z = l2 + tz;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (in_res(m, x, y, z)) {
+      const int cx0 = max(tx - (W - 1), 0), cx1 = min(tx, n0 - 1);
+      for (int ky = 0; ky < W; ++ky) {
+        const int cy = ty - ky;
+        if (cy < 0 || cy >= n1) continue;
+        for (int kz = 0; kz < W; ++kz) {
+          const int cz = tz - kz;
+          if (cz < 0 || cz >= n2) continue;
+          const int rc = coff + (cy * n2 + cz) * n0;
+          pair_sum(x, y, z, S.start[rc + cx0], S.start[rc + cx1 + 1], acc);
+        }
+      }
     }
-    __syncthreads();
-    for (int t = threadIdx.x; t < (int)V; t += blockDim.x) {
-      float4 v = tile[t];
-      if (v.x == 0.f && v.y == 0.f && v.z == 0.f && v.w == 0.f) continue;
-      int x = lo[0] + (int)(t % ex), z = lo[2] + (int)((t / ex) % ez), y = lo[1] + (int)(t / (ex * ez));
-      if (!in_res(m, x, y, z)) continue;
-      float4* o = out + node_id(m, x, y, z);
-      atomicAdd(&o->x, v.x);
-      atomicAdd(&o->y, v.y);
-      atomicAdd(&o->z, v.z);
-      atomicAdd(&o->w, v.w);
-    }
-    // node blocks of the box (clamped to the grid)
+    dst[off + t] = acc;
+  }
+}
+
+// Writes, for each node of each tiled box, pair_sum(x, y, z, q0, q1, acc)
+// over the binned slots of every cell row whose particles reach the node
+// (the W x W rows of cells node - k, k in [0, W)); nodes outside the grid
+// get zeros (the gather never reads them).
+template <int W, class F>
+__device__ void tile_sums(const MP& m, const BinShared& S, const BinBox& B, float4* __restrict__ dst,
+                          F&& pair_sum) {
+  tile_sums_box<W, 0>(m, S, B, dst, pair_sum);
+  tile_sums_box<W, 1>(m, S, B, dst, pair_sum);
+}
+
+// Header of workgroup w and the node blocks of its tiled boxes (clamped to
+// the grid) marked active.
+template <int W>
+__device__ void tile_finish(const MP& m, const TileOut& o, int w, const BinBox& B) {
+  const int t0 = threadIdx.x;
+  if (t0 == 0) {
+    o.hdr[3 * (size_t)w] = make_int4(B.lo[0][0], B.lo[0][1], B.lo[0][2], B.ex[0][0]);
+    o.hdr[3 * (size_t)w + 1] = make_int4(B.ex[0][1], B.ex[0][2], B.lo[1][0], B.lo[1][1]);
+    o.hdr[3 * (size_t)w + 2] = make_int4(B.lo[1][2], B.ex[1][0], B.ex[1][1], B.ex[1][2]);
+  }
+#pragma unroll
+  for (int g = 0; g < 2; ++g) {
+    if (B.ex[g][0] == 0) continue;
     int b0[3], nb[3];
     for (int a = 0; a < 3; ++a) {
       int r = a == 0 ? m.res.x : (a == 1 ? m.res.y : m.res.z);
-      int l = max(lo[a], 0), h = min(hi[a], r - 1);
-      b0[a] = l / MPM_BLK;
-      nb[a] = h >= l ? h / MPM_BLK - b0[a] + 1 : 0;
+      int lw = max(B.lo[g][a], 0), hg = min(B.lo[g][a] + B.ex[g][a] - 1, r - 1);
+      b0[a] = lw / MPM_BLK;
+      nb[a] = hg >= lw ? hg / MPM_BLK - b0[a] + 1 : 0;
     }
-    int tot = nb[0] * nb[1] * nb[2];
-    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+    int nt = nb[0] * nb[1] * nb[2];
+    for (int t = t0; t < nt; t += blockDim.x) {
       int bx = b0[0] + t % nb[0], bz = b0[2] + (t / nb[0]) % nb[2], by = b0[1] + t / (nb[0] * nb[2]);
-      mark_block(flags, list, count, block_id(m, bx, by, bz));
+      mark_block(o.flags, o.list, o.count, block_id(m, bx, by, bz));
     }
-  } else if (valid) {
-#pragma unroll
-    for (int b = 0; b < W; ++b)
-#pragma unroll
-      for (int c = 0; c < W; ++c)
-#pragma unroll
-        for (int a = 0; a < W; ++a) {
-          int x = s.base[0] + a, y = s.base[1] + b, z = s.base[2] + c;
-          if (!in_res(m, x, y, z)) continue;
-          float4 v = contrib(a, b, c);
-          float4* o = out + node_id(m, x, y, z);
-          atomicAdd(&o->x, v.x);
-          atomicAdd(&o->y, v.y);
-          atomicAdd(&o->z, v.z);
-          atomicAdd(&o->w, v.w);
-        }
-    for (int b = 0; b < 2; ++b)
-      for (int c = 0; c < 2; ++c)
-        for (int a = 0; a < 2; ++a) {
-          int x = min(max(s.base[0] + a * (W - 1), 0), m.res.x - 1);
-          int y = min(max(s.base[1] + b * (W - 1), 0), m.res.y - 1);
-          int z = min(max(s.base[2] + c * (W - 1), 0), m.res.z - 1);
-          mark_block(flags, list, count, block_id(m, x / MPM_BLK, y / MPM_BLK, z / MPM_BLK));
-        }
   }
+}
+
+// A particle of a box that is not tiled adds contrib(x, y, z) to the
+// nodes of its stencil (base .. base + W - 1) in global memory and marks
+// the blocks its stencil spans.
+template <int W, class F>
+__device__ void direct_add(const MP& m, const TileOut& o, const int base[3], F&& contrib) {
+  for (int b = 0; b < W; ++b)
+    for (int c = 0; c < W; ++c)
+#pragma unroll
+      for (int a = 0; a < W; ++a) {
+        int x = base[0] + a, y = base[1] + b, z = base[2] + c;
+        if (!in_res(m, x, y, z)) continue;
+        float4 v = contrib(x, y, z);
+        float4* d = o.direct + node_id(m, x, y, z);
+        atomicAdd(&d->x, v.x);
+        atomicAdd(&d->y, v.y);
+        atomicAdd(&d->z, v.z);
+        atomicAdd(&d->w, v.w);
+      }
+  o.direct_flag[0] = 1;
+  for (int b = 0; b < 2; ++b)
+    for (int c = 0; c < 2; ++c)
+      for (int a = 0; a < 2; ++a) {
+        int x = min(max(base[0] + a * (W - 1), 0), m.res.x - 1);
+        int y = min(max(base[1] + b * (W - 1), 0), m.res.y - 1);
+        int z = min(max(base[2] + c * (W - 1), 0), m.res.z - 1);
+        mark_block(o.flags, o.list, o.count, block_id(m, x / MPM_BLK, y / MPM_BLK, z / MPM_BLK));
+      }
+}
+
+// What one particle gives node (x, y, z).
+// P2G (:823-889 / :197-236): {w m_p (v_p + B_p iDp (x_I - x_p) ss), w m_p}.
+struct P2GPart {
+  // x, y, z, vx, vy, vz, m, B (row-major, already times iDp)
+  static constexpr int NF = 16;
+  float f[NF];
+};
+template <int W>
+__device__ __forceinline__ float4 p2g_contrib(const MP& m, int apic, const P2GPart& q, int x, int y,
+                                              int z) {
+  const float* f = q.f;
+  float w0, w1, w2, dw;
+  axis_weight<W>(f[0], m.xmin.x, x, m.dx, w0, dw);
+  axis_weight<W>(f[1], m.xmin.y, y, m.dx, w1, dw);
+  axis_weight<W>(f[2], m.xmin.z, z, m.dx, w2, dw);
+  const float wt = w0 * w1 * w2;
+  float vb0 = f[3], vb1 = f[4], vb2 = f[5];
+  if (apic) {
+    const float d0 = (m.xmin.x + (float)x * m.dx - f[0]) * m.ss;
+    const float d1 = (m.xmin.y + (float)y * m.dx - f[1]) * m.ss;
+    const float d2 = (m.xmin.z + (float)z * m.dx - f[2]) * m.ss;
+    vb0 += f[7] * d0 + f[8] * d1 + f[9] * d2;
+    vb1 += f[10] * d0 + f[11] * d1 + f[12] * d2;
+    vb2 += f[13] * d0 + f[14] * d1 + f[15] * d2;
+  }
+  const float mp = f[6];
+  return make_float4(vb0 * mp * wt, vb1 * mp * wt, vb2 * mp * wt, wt * mp);
+}
+
+// Node force (:676-697): sigma_p grad N scale_p, with the reference's
+// derivative taken at (x_I - x_p)/dx (the stencil's, sign flipped).
+struct ForcePart {
+  // x, y, z, sigma (row-major), scale
+  static constexpr int NF = 13;
+  float f[NF];
+};
+template <int W>
+__device__ __forceinline__ float4 force_contrib(const MP& m, const ForcePart& q, int x, int y, int z) {
+  const float* f = q.f;
+  float n0, n1, n2, d0, d1, d2;
+  axis_weight<W>(f[0], m.xmin.x, x, m.dx, n0, d0);
+  axis_weight<W>(f[1], m.xmin.y, y, m.dx, n1, d1);
+  axis_weight<W>(f[2], m.xmin.z, z, m.dx, n2, d2);
+  const float nw0 = -d0 * n1 * n2, nw1 = -d1 * n0 * n2, nw2 = -d2 * n0 * n1;
+  const float f0 = f[3] * nw0 + f[4] * nw1 + f[5] * nw2;
+  const float f1 = f[6] * nw0 + f[7] * nw1 + f[8] * nw2;
+  const float f2 = f[9] * nw0 + f[10] * nw1 + f[11] * nw2;
+  return make_float4(f0 * f[12], f1 * f[12], f2 * f[12], 0.f);
+}
+
+// The sort grid as the gather needs it: which sorted particles lie in
+// which cell (cell_start), and the map from a position to a cell (the
+// hash's (p - gmin) * fac, truncated; valid cells 1..res-2 per axis).
+struct SortGrid {
+  const int* cell_start;
+  int3 res;
+  float3 gmin, fac;
+};
+
+// Sum, for node (x, y, z) of node block (bx, by, bz), the tile entries of
+// every workgroup whose particles can reach the block.  Those particles lie
+// within R cells of the block (R = support + 1/2 cell of slack, far above
+// the hash's rounding), so in the sort-grid cells that range maps to; their
+// sorted ranges name the workgroups.  Rows of cells are visited in key
+// order, so workgroup ids only grow and each is summed once.
+template <int W>
+__device__ float4 gather_tiles(const MP& m, const SortGrid& g, const int4* __restrict__ hdr,
+                               const float4* __restrict__ tiles, int bx, int by, int bz, int x,
+                               int y, int z) {
+  const int R = W == 4 ? 3 : 2;
+  const int bb[3] = {bx, by, bz};
+  const float xm[3] = {m.xmin.x, m.xmin.y, m.xmin.z};
+  const float gm[3] = {g.gmin.x, g.gmin.y, g.gmin.z};
+  const float fc[3] = {g.fac.x, g.fac.y, g.fac.z};
+  const int rs[3] = {g.res.x, g.res.y, g.res.z};
+  int clo[3], chi[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float lo = xm[a] + (float)(bb[a] * MPM_BLK - R) * m.dx;
+    float hi = xm[a] + (float)(bb[a] * MPM_BLK + MPM_BLK - 1 + R) * m.dx;
+    clo[a] = max((int)floorf((lo - gm[a]) * fc[a]), 1);
+    chi[a] = min((int)floorf((hi - gm[a]) * fc[a]), rs[a] - 2);
+  }
+  float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (clo[0] > chi[0]) return sum;
+  int last = -1;
+  for (int cy = clo[1]; cy <= chi[1]; ++cy)
+    for (int cz = clo[2]; cz <= chi[2]; ++cz) {
+      const int rowbase = (cy * g.res.z + cz) * g.res.x;
+      const int s0 = g.cell_start[rowbase + clo[0]], s1 = g.cell_start[rowbase + chi[0] + 1];
+      if (s1 <= s0) continue;
+      const int w1 = (s1 - 1) / MPM_THREADS;
+      for (int w = max(s0 / MPM_THREADS, last + 1); w <= w1; ++w) {
+        const int4 h0 = hdr[3 * (size_t)w], h1 = hdr[3 * (size_t)w + 1], h2 = hdr[3 * (size_t)w + 2];
+        const float4* tw = tiles + (size_t)w * MPM_TILE;
+        int dx = x - h0.x, dy = y - h0.y, dz = z - h0.z;
+        if (h0.w > 0 && dx >= 0 && dx < h0.w && dy >= 0 && dy < h1.x && dz >= 0 && dz < h1.y) {
+          float4 v = tw[(dy * h1.y + dz) * h0.w + dx];
+          sum.x += v.x; sum.y += v.y; sum.z += v.z; sum.w += v.w;
+        }
+        dx = x - h1.z; dy = y - h1.w; dz = z - h2.x;
+        if (h2.y > 0 && dx >= 0 && dx < h2.y && dy >= 0 && dy < h2.z && dz >= 0 && dz < h2.w) {
+          float4 v = tw[h0.w * h1.x * h1.y + (dy * h2.w + dz) * h2.y + dx];
+          sum.x += v.x; sum.y += v.y; sum.z += v.z; sum.w += v.w;
+        }
+      }
+      last = max(last, w1);
+    }
+  return sum;
 }
 
 // ------------------------------------------------------------------ P2G --
@@ -217,39 +499,55 @@ template <int W>
 __global__ __launch_bounds__(MPM_THREADS) void k_mpm_p2g(
     MP m, int apic, const uint32_t* __restrict__ key, const float4* __restrict__ pos,
     const float4* __restrict__ vel, const float4* __restrict__ phys, const float* __restrict__ B,
-    int cap, float4* __restrict__ acc, int* __restrict__ flags, int* __restrict__ list,
-    int* __restrict__ count) {
+    int cap, int sort_resx, TileOut o) {
+  constexpr int NF = P2GPart::NF;
+  __shared__ BinShared S;
+  __shared__ float pay[NF][MPM_THREADS];   // the binned particles, field-major
   const int i = xcd_index();
   bool valid = false;
-  float4 p = make_float4(0.f, 0.f, 0.f, 0.f), v = p;
-  float mp = 0.f, Bi[9];
+  int row = 0;
+  float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+  P2GPart me = {};
   if (i < m.n) {
     p = pos[i];
-    valid = type_of(p) != SPHX_TYPE_BOUNDARY && key[i] != m.sentinel;
-    v = vel[i];
-    mp = phys[i].x;
+    const uint32_t k = key[i];
+    valid = type_of(p) != SPHX_TYPE_BOUNDARY && k != m.sentinel;
+    row = (int)(k / (uint32_t)sort_resx);
+    const float4 v = vel[i];
+    me.f[0] = p.x; me.f[1] = p.y; me.f[2] = p.z;
+    me.f[3] = v.x; me.f[4] = v.y; me.f[5] = v.z;
+    me.f[6] = phys[i].x;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) Bi[q] = apic ? B[(size_t)q * cap + i] * m.iDp : 0.f;
+    for (int q = 0; q < 9; ++q) me.f[7 + q] = apic ? B[(size_t)q * cap + i] * m.iDp : 0.f;
   }
-  Stencil<W> s;
-  make_stencil<W>(m, p, s);
-  scatter<W>(m, valid, s, acc, flags, list, count, [&](int a, int b, int c) {
-    float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
-    float vb[3] = {v.x, v.y, v.z};
-    if (apic) {
-      float xi = m.xmin.x + (float)(s.base[0] + a) * m.dx;
-      float yi = m.xmin.y + (float)(s.base[1] + b) * m.dx;
-      float zi = m.xmin.z + (float)(s.base[2] + c) * m.dx;
-      float d0 = (xi - p.x) * m.ss, d1 = (yi - p.y) * m.ss, d2 = (zi - p.z) * m.ss;
+  int base[3];
+  stencil_base<W>(m, p, base);
+  BinBox bx;
+  int seg, slot;
+  block_bins<W>(S, valid, row, base, bx, seg, slot);
+  if (slot >= 0)
 #pragma unroll
-      for (int q = 0; q < 3; ++q) vb[q] += Bi[3 * q] * d0 + Bi[3 * q + 1] * d1 + Bi[3 * q + 2] * d2;
+    for (int q = 0; q < NF; ++q) pay[q][slot] = me.f[q];
+  __syncthreads();
+  const int w = xcd_tile();
+  tile_sums<W>(m, S, bx, o.tiles + (size_t)w * MPM_TILE,
+               [&](int x, int y, int z, int q0, int q1, float4& acc) {
+    for (int q = q0; q < q1; ++q) {
+      P2GPart o2;
+#pragma unroll
+      for (int k = 0; k < NF; ++k) o2.f[k] = pay[k][q];
+      const float4 c = p2g_contrib<W>(m, apic, o2, x, y, z);
+      acc.x += c.x; acc.y += c.y; acc.z += c.z; acc.w += c.w;
     }
-    return make_float4(vb[0] * mp * wt, vb[1] * mp * wt, vb[2] * mp * wt, wt * mp);
   });
+  if (valid && !bx.tiled(seg))
+    direct_add<W>(m, o, base, [&](int x, int y, int z) { return p2g_contrib<W>(m, apic, me, x, y, z); });
+  tile_finish<W>(m, o, w, bx);
 }
 
-// Persistent loop over the listed node blocks: 64 threads, one node each.
-#define FOR_LISTED_NODES(m, list, count, body)                                      \
+// Persistent loop over the listed node blocks: 64 threads, one node each,
+// nodes outside the grid included (the body checks in_res).
+#define FOR_LISTED_NODES(m, list, count, body)                                  \
   for (int bi = blockIdx.x; bi < *(count); bi += gridDim.x) {                      \
     const int blk = (list)[bi];                                                     \
     const int bx = blk % (m).nblk.x, bz = (blk / (m).nblk.x) % (m).nblk.z,          \
@@ -257,49 +555,49 @@ __global__ __launch_bounds__(MPM_THREADS) void k_mpm_p2g(
     const int t = threadIdx.x;                                                      \
     const int x = bx * MPM_BLK + (t & 3), z = bz * MPM_BLK + ((t >> 2) & 3),        \
               y = by * MPM_BLK + (t >> 4);                                          \
-    if (in_res((m), x, y, z)) {                                                     \
-      const size_t id = node_id((m), x, y, z);                                      \
-      body                                                                          \
-    }                                                                               \
+    body                                                                            \
   }
 
-// node velocity = momentum / mass (or zero below 1e-9, :881-888)
-__global__ __launch_bounds__(64) void k_mpm_normalize(MP m, const int* __restrict__ list,
-                                                      const int* __restrict__ count,
-                                                      const float4* __restrict__ acc,
-                                                      float4* __restrict__ nodev) {
+// Per listed node block (64 threads, one node each): the node's sum over
+// the workgroup tiles, plus what was added directly (then cleared), then
+// velocity = momentum / mass (or zero below 1e-9, :881-888).
+template <int W>
+__global__ __launch_bounds__(64) void k_mpm_gather_p2g(MP m, SortGrid g, const int* __restrict__ list,
+                                                       const int* __restrict__ count,
+                                                       const int4* __restrict__ hdr,
+                                                       const float4* __restrict__ tiles,
+                                                       float4* __restrict__ acc,
+                                                       const int* __restrict__ direct_flag,
+                                                       float4* __restrict__ nodev) {
+  const int any_direct = direct_flag[0];
   FOR_LISTED_NODES(m, list, count, {
-    float4 a = acc[id];
-    nodev[id] = a.w > 0.000000001f ? make_float4(a.x / a.w, a.y / a.w, a.z / a.w, a.w)
-                                   : make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 a = gather_tiles<W>(m, g, hdr, tiles, bx, by, bz, x, y, z);
+    if (in_res(m, x, y, z)) {
+      const size_t id = node_id(m, x, y, z);
+      if (any_direct) {
+        float4 d = acc[id];
+        a.x += d.x; a.y += d.y; a.z += d.z; a.w += d.w;
+        acc[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+      nodev[id] = a.w > 0.000000001f ? make_float4(a.x / a.w, a.y / a.w, a.z / a.w, a.w)
+                                     : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
   })
 }
 
-// zero the accumulators of the listed blocks and unlist them
-__global__ __launch_bounds__(64) void k_mpm_clear(MP m, const int* __restrict__ list,
+// unlist the listed blocks (their direct-add accumulators are already zero:
+// each gather clears what the direct path added)
+__global__ __launch_bounds__(64) void k_mpm_clear(const int* __restrict__ list,
                                                   const int* __restrict__ count,
-                                                  float4* __restrict__ acc,
-                                                  float4* __restrict__ force, int* __restrict__ flags) {
-  for (int bi = blockIdx.x; bi < *count; bi += gridDim.x) {
-    const int blk = list[bi];
-    if (threadIdx.x == 0) flags[blk] = 0;
-    const int bx = blk % m.nblk.x, bz = (blk / m.nblk.x) % m.nblk.z, by = blk / (m.nblk.x * m.nblk.z);
-    const int t = threadIdx.x;
-    const int x = bx * MPM_BLK + (t & 3), z = bz * MPM_BLK + ((t >> 2) & 3), y = by * MPM_BLK + (t >> 4);
-    if (in_res(m, x, y, z)) {
-      const size_t id = node_id(m, x, y, z);
-      acc[id] = make_float4(0.f, 0.f, 0.f, 0.f);
-      force[id] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-  }
+                                                  int* __restrict__ flags) {
+  for (int bi = blockIdx.x * 64 + threadIdx.x; bi < *count; bi += gridDim.x * 64) flags[list[bi]] = 0;
 }
 
-__global__ void k_mpm_set_count(int* count, int v) { *count = v; }
-
-__global__ __launch_bounds__(64) void k_mpm_zero_force(MP m, const int* __restrict__ list,
-                                                       const int* __restrict__ count,
-                                                       float4* __restrict__ force) {
-  FOR_LISTED_NODES(m, list, count, { force[id] = make_float4(0.f, 0.f, 0.f, 0.f); })
+// count = v; the direct-add flags of P2G and the node force cleared
+__global__ void k_mpm_set_count(int* count, int v, int* direct) {
+  *count = v;
+  direct[0] = 0;
+  direct[1] = 0;
 }
 
 // ------------------------------------------------------ Drucker-Prager --
@@ -425,33 +723,50 @@ __global__ __launch_bounds__(MPM_THREADS) void k_mpm_stress(
 template <int W>
 __global__ __launch_bounds__(MPM_THREADS) void k_mpm_force(
     MP m, const uint32_t* __restrict__ key, const float4* __restrict__ pos,
-    const float4* __restrict__ phys, const float* __restrict__ S, int cap,
-    float4* __restrict__ force, int* __restrict__ flags, int* __restrict__ list,
-    int* __restrict__ count) {
+    const float4* __restrict__ phys, const float* __restrict__ S9, int cap, int sort_resx,
+    TileOut o) {
+  constexpr int NF = ForcePart::NF;
+  __shared__ BinShared S;
+  __shared__ float pay[NF][MPM_THREADS];   // the binned particles, field-major
   const int i = xcd_index();
   bool valid = false;
+  int row = 0;
   float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-  float sig[9], scale = 0.f;
+  ForcePart me = {};
   if (i < m.n) {
     p = pos[i];
-    valid = type_of(p) != SPHX_TYPE_BOUNDARY && key[i] != m.sentinel;
-    float4 q = phys[i];
-    scale = m.physical ? q.x / (q.y * m.rc) : 1.f / q.y;
+    const uint32_t k = key[i];
+    valid = type_of(p) != SPHX_TYPE_BOUNDARY && k != m.sentinel;
+    row = (int)(k / (uint32_t)sort_resx);
+    const float4 q = phys[i];
+    me.f[0] = p.x; me.f[1] = p.y; me.f[2] = p.z;
+    me.f[12] = m.physical ? q.x / (q.y * m.rc) : 1.f / q.y;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) sig[k] = S[(size_t)k * cap + i];
+    for (int k9 = 0; k9 < 9; ++k9) me.f[3 + k9] = S9[(size_t)k9 * cap + i];
   }
-  Stencil<W> s;
-  make_stencil<W>(m, p, s);
-  scatter<W>(m, valid, s, force, flags, list, count, [&](int a, int b, int c) {
-    float n0 = s.w[0][a], n1 = s.w[1][b], n2 = s.w[2][c];
-    // the reference differentiates at (x_I - x_p)/dx: the stencil's
-    // derivative (at (x_p - x_I)/dx) with the sign flipped
-    float nw0 = -s.dw[0][a] * n1 * n2, nw1 = -s.dw[1][b] * n0 * n2, nw2 = -s.dw[2][c] * n0 * n1;
-    float f0 = sig[0] * nw0 + sig[1] * nw1 + sig[2] * nw2;
-    float f1 = sig[3] * nw0 + sig[4] * nw1 + sig[5] * nw2;
-    float f2 = sig[6] * nw0 + sig[7] * nw1 + sig[8] * nw2;
-    return make_float4(f0 * scale, f1 * scale, f2 * scale, 0.f);
+  int base[3];
+  stencil_base<W>(m, p, base);
+  BinBox bx;
+  int seg, slot;
+  block_bins<W>(S, valid, row, base, bx, seg, slot);
+  if (slot >= 0)
+#pragma unroll
+    for (int q = 0; q < NF; ++q) pay[q][slot] = me.f[q];
+  __syncthreads();
+  const int w = xcd_tile();
+  tile_sums<W>(m, S, bx, o.tiles + (size_t)w * MPM_TILE,
+               [&](int x, int y, int z, int q0, int q1, float4& acc) {
+    for (int q = q0; q < q1; ++q) {
+      ForcePart o2;
+#pragma unroll
+      for (int k = 0; k < NF; ++k) o2.f[k] = pay[k][q];
+      const float4 c = force_contrib<W>(m, o2, x, y, z);
+      acc.x += c.x; acc.y += c.y; acc.z += c.z;
+    }
   });
+  if (valid && !bx.tiled(seg))
+    direct_add<W>(m, o, base, [&](int x, int y, int z) { return force_contrib<W>(m, me, x, y, z); });
+  tile_finish<W>(m, o, w, bx);
 }
 
 // GridCollisionDetection (:595-656), EPSILON 1e-6 at this use site
@@ -464,22 +779,37 @@ __device__ __forceinline__ void collide(const MP& m, float x, float y, float z, 
   if ((z - m.smax.z) * m.ss > eps && -v.z < 0.f) v.z = 0.f;
 }
 
-// MpmNodeUpdate (:658-704): v += (f + g) dt for nodes with mass, collision.
-__global__ __launch_bounds__(64) void k_mpm_node(MP m, const int* __restrict__ list,
+// MpmNodeUpdate (:658-704): the node force summed from the workgroup tiles
+// (plus what was added directly, then cleared), v += (f + g) dt for nodes
+// with mass, collision.
+template <int W>
+__global__ __launch_bounds__(64) void k_mpm_node(MP m, SortGrid g, const int* __restrict__ list,
                                                  const int* __restrict__ count,
-                                                 const float4* __restrict__ force,
+                                                 const int4* __restrict__ hdr,
+                                                 const float4* __restrict__ tiles,
+                                                 float4* __restrict__ force,
+                                                 const int* __restrict__ direct_flag,
                                                  float4* __restrict__ nodev) {
+  const int any_direct = direct_flag[1];
   FOR_LISTED_NODES(m, list, count, {
-    float4 v = nodev[id];
-    if (v.w >= 0.000000001f) {
-      float4 f = force[id];
-      float sc = m.physical ? 1.f / v.w : 1.f;
-      v.x += (f.x * sc + m.g.x) * m.dt;
-      v.y += (f.y * sc + m.g.y) * m.dt;
-      v.z += (f.z * sc + m.g.z) * m.dt;
-      collide(m, m.xmin.x + (float)x * m.dx, m.xmin.y + (float)y * m.dx,
-              m.xmin.z + (float)z * m.dx, v);
-      nodev[id] = v;
+    float4 f = gather_tiles<W>(m, g, hdr, tiles, bx, by, bz, x, y, z);
+    if (in_res(m, x, y, z)) {
+      const size_t id = node_id(m, x, y, z);
+      if (any_direct) {
+        float4 d = force[id];
+        f.x += d.x; f.y += d.y; f.z += d.z;
+        force[id] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+      float4 v = nodev[id];
+      if (v.w >= 0.000000001f) {
+        float sc = m.physical ? 1.f / v.w : 1.f;
+        v.x += (f.x * sc + m.g.x) * m.dt;
+        v.y += (f.y * sc + m.g.y) * m.dt;
+        v.z += (f.z * sc + m.g.z) * m.dt;
+        collide(m, m.xmin.x + (float)x * m.dx, m.xmin.y + (float)y * m.dx,
+                m.xmin.z + (float)z * m.dx, v);
+        nodev[id] = v;
+      }
     }
   })
 }
@@ -639,6 +969,24 @@ static int persistent_blocks(const sphx_context* c) {
   return (int)std::min<size_t>(c->mpm.blocks, 4096);
 }
 
+static TileOut tile_out(sphx_context* c, float4* direct, int pass) {
+  auto& M = c->mpm;
+  return TileOut{M.tiles.p, M.hdr.p, direct, M.direct.p + pass, M.flags.p, M.list.p, M.count.p};
+}
+
+static SortGrid sort_grid(const sphx_context* c) {
+  KP k = make_kp(c);
+  return SortGrid{c->cell_start.p, k.gridres, k.gridmin, k.gridfac};
+}
+
+// The scatters index tiles by workgroup: room for every workgroup of a
+// full context.
+static int mpm_tiles_fit(sphx_context* c, const char* who) {
+  SPHX_CHECK((size_t)blocks_for(c->num, MPM_THREADS) <= c->mpm.tiles_cap, who,
+             "more particles than the MPM tiles were sized for");
+  return 0;
+}
+
 int grid_hash(sphx_context* c);
 int grid_sort(sphx_context* c);
 int grid_reorder(sphx_context* c);
@@ -685,6 +1033,11 @@ int sphx_mpm_setup(sphx_context* c, int kernel, int physical) {
   SPHX_TRY(M.flags.alloc(M.blocks));
   SPHX_TRY(M.list.alloc(M.blocks));
   SPHX_TRY(M.count.alloc(1));
+  M.tiles_cap = (size_t)blocks_for((int)cap, MPM_THREADS);
+  SPHX_TRY(M.tiles.alloc(M.tiles_cap * MPM_TILE));
+  SPHX_TRY(M.hdr.alloc(M.tiles_cap * 3));
+  SPHX_TRY(M.direct.alloc(2));
+  SPHX_HIP(hipMemsetAsync(M.direct.p, 0, 2 * sizeof(int), c->stream));
   SPHX_HIP(hipMemsetAsync(M.acc.p, 0, nodes * sizeof(float4), c->stream));
   SPHX_HIP(hipMemsetAsync(M.force.p, 0, nodes * sizeof(float4), c->stream));
   SPHX_HIP(hipMemsetAsync(M.nodev.p, 0, nodes * sizeof(float4), c->stream));
@@ -701,27 +1054,33 @@ int sphx_mpm_setup(sphx_context* c, int kernel, int physical) {
 int sphx_mpm_p2g(sphx_context* c, int apic) {
   SPHX_TRY(mpm_need(c, "sphx_mpm_p2g"));
   SPHX_CHECK(c->sorted_valid, "sphx_mpm_p2g", "grid not built (call sphx_grid_build first)");
+  SPHX_TRY(mpm_tiles_fit(c, "sphx_mpm_p2g"));
   auto& M = c->mpm;
   MP m = make_mp(c);
+  SortGrid g = sort_grid(c);
+  TileOut o = tile_out(c, M.acc.p, 0);
   int pb = persistent_blocks(c);
   timer_begin(c, "mpm_clear");
-  k_mpm_clear<<<pb, 64, 0, c->stream>>>(m, M.list.p, M.count.p, M.acc.p, M.force.p, M.flags.p);
-  k_mpm_set_count<<<1, 1, 0, c->stream>>>(M.count.p, 0);
+  k_mpm_clear<<<pb, 64, 0, c->stream>>>(M.list.p, M.count.p, M.flags.p);
+  k_mpm_set_count<<<1, 1, 0, c->stream>>>(M.count.p, 0, M.direct.p);
   timer_end(c);
   if (m.n > 0) {
     timer_begin(c, "mpm_p2g");
     if (M.kernel == 0)
       k_mpm_p2g<4><<<blocks_for(m.n, MPM_THREADS), MPM_THREADS, 0, c->stream>>>(
-          m, apic, c->key.p, c->POS(), c->VEL(), c->PHYS(), M.B[c->cur].p, c->cap, M.acc.p,
-          M.flags.p, M.list.p, M.count.p);
+          m, apic, c->key.p, c->POS(), c->VEL(), c->PHYS(), M.B[c->cur].p, c->cap, g.res.x, o);
     else
       k_mpm_p2g<3><<<blocks_for(m.n, MPM_THREADS), MPM_THREADS, 0, c->stream>>>(
-          m, apic, c->key.p, c->POS(), c->VEL(), c->PHYS(), M.B[c->cur].p, c->cap, M.acc.p,
-          M.flags.p, M.list.p, M.count.p);
+          m, apic, c->key.p, c->POS(), c->VEL(), c->PHYS(), M.B[c->cur].p, c->cap, g.res.x, o);
     timer_end(c);
   }
-  timer_begin(c, "mpm_normalize");
-  k_mpm_normalize<<<pb, 64, 0, c->stream>>>(m, M.list.p, M.count.p, M.acc.p, M.nodev.p);
+  timer_begin(c, "mpm_gather");
+  if (M.kernel == 0)
+    k_mpm_gather_p2g<4><<<pb, 64, 0, c->stream>>>(m, g, M.list.p, M.count.p, M.hdr.p, M.tiles.p,
+                                                  M.acc.p, M.direct.p, M.nodev.p);
+  else
+    k_mpm_gather_p2g<3><<<pb, 64, 0, c->stream>>>(m, g, M.list.p, M.count.p, M.hdr.p, M.tiles.p,
+                                                  M.acc.p, M.direct.p, M.nodev.p);
   timer_end(c);
   SPHX_HIP(hipGetLastError());
   return 0;
@@ -749,24 +1108,30 @@ int sphx_mpm_stress(sphx_context* c) {
 int sphx_mpm_node_update(sphx_context* c) {
   SPHX_TRY(mpm_need(c, "sphx_mpm_node_update"));
   SPHX_CHECK(c->sorted_valid, "sphx_mpm_node_update", "grid not built");
+  SPHX_TRY(mpm_tiles_fit(c, "sphx_mpm_node_update"));
   auto& M = c->mpm;
   MP m = make_mp(c);
-  k_mpm_zero_force<<<persistent_blocks(c), 64, 0, c->stream>>>(m, M.list.p, M.count.p, M.force.p);
+  SortGrid g = sort_grid(c);
+  TileOut o = tile_out(c, M.force.p, 1);
   if (m.n > 0) {
     timer_begin(c, "mpm_force");
     if (M.kernel == 0)
       k_mpm_force<4><<<blocks_for(m.n, MPM_THREADS), MPM_THREADS, 0, c->stream>>>(
-          m, c->key.p, c->POS(), c->PHYS(), M.S[c->cur].p, c->cap, M.force.p, M.flags.p,
-          M.list.p, M.count.p);
+          m, c->key.p, c->POS(), c->PHYS(), M.S[c->cur].p, c->cap, g.res.x, o);
     else
       k_mpm_force<3><<<blocks_for(m.n, MPM_THREADS), MPM_THREADS, 0, c->stream>>>(
-          m, c->key.p, c->POS(), c->PHYS(), M.S[c->cur].p, c->cap, M.force.p, M.flags.p,
-          M.list.p, M.count.p);
+          m, c->key.p, c->POS(), c->PHYS(), M.S[c->cur].p, c->cap, g.res.x, o);
     timer_end(c);
   }
   timer_begin(c, "mpm_node");
-  k_mpm_node<<<persistent_blocks(c), 64, 0, c->stream>>>(m, M.list.p, M.count.p, M.force.p,
-                                                          M.nodev.p);
+  if (M.kernel == 0)
+    k_mpm_node<4><<<persistent_blocks(c), 64, 0, c->stream>>>(m, g, M.list.p, M.count.p, M.hdr.p,
+                                                               M.tiles.p, M.force.p, M.direct.p,
+                                                               M.nodev.p);
+  else
+    k_mpm_node<3><<<persistent_blocks(c), 64, 0, c->stream>>>(m, g, M.list.p, M.count.p, M.hdr.p,
+                                                               M.tiles.p, M.force.p, M.direct.p,
+                                                               M.nodev.p);
   timer_end(c);
   SPHX_HIP(hipGetLastError());
   return 0;
@@ -828,12 +1193,10 @@ int sphx_mpm_upload_nodes(sphx_context* c, const float* v, const float* mass) {
   std::vector<float4> t(M.nodes);
   for (size_t i = 0; i < M.nodes; ++i) t[i] = make_float4(v[3 * i], v[3 * i + 1], v[3 * i + 2], mass[i]);
   SPHX_HIP(hipMemcpy(M.nodev.p, t.data(), M.nodes * sizeof(float4), hipMemcpyHostToDevice));
-  MP m = make_mp(c);
-  k_mpm_clear<<<persistent_blocks(c), 64, 0, c->stream>>>(m, M.list.p, M.count.p, M.acc.p,
-                                                           M.force.p, M.flags.p);
+  k_mpm_clear<<<persistent_blocks(c), 64, 0, c->stream>>>(M.list.p, M.count.p, M.flags.p);
   k_mpm_list_all<<<blocks_for((int)M.blocks, 256), 256, 0, c->stream>>>(M.flags.p, M.list.p,
                                                                         (int)M.blocks);
-  k_mpm_set_count<<<1, 1, 0, c->stream>>>(M.count.p, (int)M.blocks);
+  k_mpm_set_count<<<1, 1, 0, c->stream>>>(M.count.p, (int)M.blocks, M.direct.p);
   SPHX_HIP(hipGetLastError());
   SPHX_HIP(hipStreamSynchronize(c->stream));
   return 0;
