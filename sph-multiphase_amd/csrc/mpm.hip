@@ -305,7 +305,7 @@ __device__ __forceinline__ void tile_sums_box(const MP& m, const BinShared& S, c
           const int cz = tz - kz;
           if (cz < 0 || cz >= n2) continue;
           const int rc = coff + (cy * n2 + cz) * n0;
-          pair_sum(x, y, z, S.start[rc + cx0], S.start[rc + cx1 + 1], acc);
+          pair_sum(x, y, z, ky, kz, S.start[rc + cx0], S.start[rc + cx1 + 1], acc);
         }
       }
     }
@@ -313,7 +313,7 @@ __device__ __forceinline__ void tile_sums_box(const MP& m, const BinShared& S, c
   }
 }
 
-// Writes, for each node of each tiled box, pair_sum(x, y, z, q0, q1, acc)
+// Writes, for each node of each tiled box, pair_sum(x, y, z, ky, kz, q0, q1, acc)
 // over the binned slots of every cell row whose particles reach the node
 // (the W x W rows of cells node - k, k in [0, W)); nodes outside the grid
 // get zeros (the gather never reads them).
@@ -352,18 +352,20 @@ __device__ void tile_finish(const MP& m, const TileOut& o, int w, const BinBox& 
   }
 }
 
-// A particle of a box that is not tiled adds contrib(x, y, z) to the
+// A particle of a box that is not tiled adds contrib(x, y, z, kx, ky, kz) to the
 // nodes of its stencil (base .. base + W - 1) in global memory and marks
 // the blocks its stencil spans.
 template <int W, class F>
 __device__ void direct_add(const MP& m, const TileOut& o, const int base[3], F&& contrib) {
+#pragma unroll
   for (int b = 0; b < W; ++b)
+#pragma unroll
     for (int c = 0; c < W; ++c)
 #pragma unroll
       for (int a = 0; a < W; ++a) {
         int x = base[0] + a, y = base[1] + b, z = base[2] + c;
         if (!in_res(m, x, y, z)) continue;
-        float4 v = contrib(x, y, z);
+        float4 v = contrib(x, y, z, a, b, c);
         float4* d = o.direct + node_id(m, x, y, z);
         atomicAdd(&d->x, v.x);
         atomicAdd(&d->y, v.y);
@@ -381,54 +383,93 @@ __device__ void direct_add(const MP& m, const TileOut& o, const int base[3], F&&
       }
 }
 
-// What one particle gives node (x, y, z).
+// What one particle gives node (x, y, z) = base + (kx, ky, kz).  The
+// particle's 1D weights (and derivatives) are evaluated once per axis
+// (make_stencil's expressions) and kept with it, so a (particle, node) pair
+// is products and sums only.
 // P2G (:823-889 / :197-236): {w m_p (v_p + B_p iDp (x_I - x_p) ss), w m_p}.
+template <int W>
 struct P2GPart {
-  // x, y, z, vx, vy, vz, m, B (row-major, already times iDp)
-  static constexpr int NF = 16;
+  // x, y, z, vx, vy, vz, m, B (row-major, times iDp), w[axis][k], base x
+  static constexpr int WO = 16, BX = 16 + 3 * W, NF = BX + 1;
   float f[NF];
 };
 template <int W>
-__device__ __forceinline__ float4 p2g_contrib(const MP& m, int apic, const P2GPart& q, int x, int y,
-                                              int z) {
-  const float* f = q.f;
-  float w0, w1, w2, dw;
-  axis_weight<W>(f[0], m.xmin.x, x, m.dx, w0, dw);
-  axis_weight<W>(f[1], m.xmin.y, y, m.dx, w1, dw);
-  axis_weight<W>(f[2], m.xmin.z, z, m.dx, w2, dw);
-  const float wt = w0 * w1 * w2;
-  float vb0 = f[3], vb1 = f[4], vb2 = f[5];
+__device__ __forceinline__ void p2g_part(const MP& m, int apic, float4 p, float4 v, float mp,
+                                         const float* __restrict__ B, int cap, int i,
+                                         P2GPart<W>& q, int base[3]) {
+  Stencil<W> s;
+  make_stencil<W>(m, p, s);
+  q.f[0] = p.x; q.f[1] = p.y; q.f[2] = p.z;
+  q.f[3] = v.x; q.f[4] = v.y; q.f[5] = v.z;
+  q.f[6] = mp;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) q.f[7 + k] = apic ? B[(size_t)k * cap + i] * m.iDp : 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int k = 0; k < W; ++k) q.f[P2GPart<W>::WO + a * W + k] = s.w[a][k];
+  q.f[P2GPart<W>::BX] = __int_as_float(s.base[0]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) base[a] = s.base[a];
+}
+// get(k) returns field k of the particle (registers or LDS)
+template <int W, class G>
+__device__ __forceinline__ float4 p2g_contrib(const MP& m, int apic, G&& get, int x, int y, int z,
+                                              int kx, int ky, int kz) {
+  constexpr int WO = P2GPart<W>::WO;
+  const float wt = get(WO + kx) * get(WO + W + ky) * get(WO + 2 * W + kz);
+  float vb0 = get(3), vb1 = get(4), vb2 = get(5);
   if (apic) {
-    const float d0 = (m.xmin.x + (float)x * m.dx - f[0]) * m.ss;
-    const float d1 = (m.xmin.y + (float)y * m.dx - f[1]) * m.ss;
-    const float d2 = (m.xmin.z + (float)z * m.dx - f[2]) * m.ss;
-    vb0 += f[7] * d0 + f[8] * d1 + f[9] * d2;
-    vb1 += f[10] * d0 + f[11] * d1 + f[12] * d2;
-    vb2 += f[13] * d0 + f[14] * d1 + f[15] * d2;
+    const float d0 = (m.xmin.x + (float)x * m.dx - get(0)) * m.ss;
+    const float d1 = (m.xmin.y + (float)y * m.dx - get(1)) * m.ss;
+    const float d2 = (m.xmin.z + (float)z * m.dx - get(2)) * m.ss;
+    vb0 += get(7) * d0 + get(8) * d1 + get(9) * d2;
+    vb1 += get(10) * d0 + get(11) * d1 + get(12) * d2;
+    vb2 += get(13) * d0 + get(14) * d1 + get(15) * d2;
   }
-  const float mp = f[6];
+  const float mp = get(6);
   return make_float4(vb0 * mp * wt, vb1 * mp * wt, vb2 * mp * wt, wt * mp);
 }
 
 // Node force (:676-697): sigma_p grad N scale_p, with the reference's
 // derivative taken at (x_I - x_p)/dx (the stencil's, sign flipped).
+template <int W>
 struct ForcePart {
-  // x, y, z, sigma (row-major), scale
-  static constexpr int NF = 13;
+  // sigma (row-major), scale, w[axis][k], dw[axis][k], base x
+  static constexpr int WO = 10, DO = 10 + 3 * W, BX = 10 + 6 * W, NF = BX + 1;
   float f[NF];
 };
 template <int W>
-__device__ __forceinline__ float4 force_contrib(const MP& m, const ForcePart& q, int x, int y, int z) {
-  const float* f = q.f;
-  float n0, n1, n2, d0, d1, d2;
-  axis_weight<W>(f[0], m.xmin.x, x, m.dx, n0, d0);
-  axis_weight<W>(f[1], m.xmin.y, y, m.dx, n1, d1);
-  axis_weight<W>(f[2], m.xmin.z, z, m.dx, n2, d2);
+__device__ __forceinline__ void force_part(const MP& m, float4 p, float scale, const float* sig,
+                                           ForcePart<W>& q, int base[3]) {
+  Stencil<W> s;
+  make_stencil<W>(m, p, s);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) q.f[k] = sig[k];
+  q.f[9] = scale;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      q.f[ForcePart<W>::WO + a * W + k] = s.w[a][k];
+      q.f[ForcePart<W>::DO + a * W + k] = s.dw[a][k];
+    }
+  q.f[ForcePart<W>::BX] = __int_as_float(s.base[0]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) base[a] = s.base[a];
+}
+template <int W, class G>
+__device__ __forceinline__ float4 force_contrib(G&& get, int kx, int ky, int kz) {
+  constexpr int WO = ForcePart<W>::WO, DO = ForcePart<W>::DO;
+  const float n0 = get(WO + kx), n1 = get(WO + W + ky), n2 = get(WO + 2 * W + kz);
+  const float d0 = get(DO + kx), d1 = get(DO + W + ky), d2 = get(DO + 2 * W + kz);
   const float nw0 = -d0 * n1 * n2, nw1 = -d1 * n0 * n2, nw2 = -d2 * n0 * n1;
-  const float f0 = f[3] * nw0 + f[4] * nw1 + f[5] * nw2;
-  const float f1 = f[6] * nw0 + f[7] * nw1 + f[8] * nw2;
-  const float f2 = f[9] * nw0 + f[10] * nw1 + f[11] * nw2;
-  return make_float4(f0 * f[12], f1 * f[12], f2 * f[12], 0.f);
+  const float f0 = get(0) * nw0 + get(1) * nw1 + get(2) * nw2;
+  const float f1 = get(3) * nw0 + get(4) * nw1 + get(5) * nw2;
+  const float f2 = get(6) * nw0 + get(7) * nw1 + get(8) * nw2;
+  const float sc = get(9);
+  return make_float4(f0 * sc, f1 * sc, f2 * sc, 0.f);
 }
 
 // The sort grid as the gather needs it: which sorted particles lie in
@@ -500,48 +541,40 @@ __global__ __launch_bounds__(MPM_THREADS) void k_mpm_p2g(
     MP m, int apic, const uint32_t* __restrict__ key, const float4* __restrict__ pos,
     const float4* __restrict__ vel, const float4* __restrict__ phys, const float* __restrict__ B,
     int cap, int sort_resx, TileOut o) {
-  constexpr int NF = P2GPart::NF;
+  using Part = P2GPart<W>;
   __shared__ BinShared S;
-  __shared__ float pay[NF][MPM_THREADS];   // the binned particles, field-major
+  __shared__ float pay[Part::NF][MPM_THREADS];   // the binned particles, field-major
   const int i = xcd_index();
   bool valid = false;
-  int row = 0;
-  float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-  P2GPart me = {};
+  int row = 0, base[3] = {0, 0, 0};
+  Part me;
   if (i < m.n) {
-    p = pos[i];
+    const float4 p = pos[i];
     const uint32_t k = key[i];
     valid = type_of(p) != SPHX_TYPE_BOUNDARY && k != m.sentinel;
     row = (int)(k / (uint32_t)sort_resx);
-    const float4 v = vel[i];
-    me.f[0] = p.x; me.f[1] = p.y; me.f[2] = p.z;
-    me.f[3] = v.x; me.f[4] = v.y; me.f[5] = v.z;
-    me.f[6] = phys[i].x;
-#pragma unroll
-    for (int q = 0; q < 9; ++q) me.f[7 + q] = apic ? B[(size_t)q * cap + i] * m.iDp : 0.f;
+    p2g_part<W>(m, apic, p, vel[i], phys[i].x, B, cap, i, me, base);
   }
-  int base[3];
-  stencil_base<W>(m, p, base);
   BinBox bx;
   int seg, slot;
   block_bins<W>(S, valid, row, base, bx, seg, slot);
   if (slot >= 0)
 #pragma unroll
-    for (int q = 0; q < NF; ++q) pay[q][slot] = me.f[q];
+    for (int q = 0; q < Part::NF; ++q) pay[q][slot] = me.f[q];
   __syncthreads();
   const int w = xcd_tile();
   tile_sums<W>(m, S, bx, o.tiles + (size_t)w * MPM_TILE,
-               [&](int x, int y, int z, int q0, int q1, float4& acc) {
+               [&](int x, int y, int z, int ky, int kz, int q0, int q1, float4& acc) {
     for (int q = q0; q < q1; ++q) {
-      P2GPart o2;
-#pragma unroll
-      for (int k = 0; k < NF; ++k) o2.f[k] = pay[k][q];
-      const float4 c = p2g_contrib<W>(m, apic, o2, x, y, z);
+      const int kx = x - __float_as_int(pay[Part::BX][q]);
+      const float4 c = p2g_contrib<W>(m, apic, [&](int f) { return pay[f][q]; }, x, y, z, kx, ky, kz);
       acc.x += c.x; acc.y += c.y; acc.z += c.z; acc.w += c.w;
     }
   });
   if (valid && !bx.tiled(seg))
-    direct_add<W>(m, o, base, [&](int x, int y, int z) { return p2g_contrib<W>(m, apic, me, x, y, z); });
+    direct_add<W>(m, o, base, [&](int x, int y, int z, int kx, int ky, int kz) {
+      return p2g_contrib<W>(m, apic, [&](int f) { return me.f[f]; }, x, y, z, kx, ky, kz);
+    });
   tile_finish<W>(m, o, w, bx);
 }
 
@@ -725,47 +758,44 @@ __global__ __launch_bounds__(MPM_THREADS) void k_mpm_force(
     MP m, const uint32_t* __restrict__ key, const float4* __restrict__ pos,
     const float4* __restrict__ phys, const float* __restrict__ S9, int cap, int sort_resx,
     TileOut o) {
-  constexpr int NF = ForcePart::NF;
+  using Part = ForcePart<W>;
   __shared__ BinShared S;
-  __shared__ float pay[NF][MPM_THREADS];   // the binned particles, field-major
+  __shared__ float pay[Part::NF][MPM_THREADS];   // the binned particles, field-major
   const int i = xcd_index();
   bool valid = false;
-  int row = 0;
-  float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-  ForcePart me = {};
+  int row = 0, base[3] = {0, 0, 0};
+  Part me;
   if (i < m.n) {
-    p = pos[i];
+    const float4 p = pos[i];
     const uint32_t k = key[i];
     valid = type_of(p) != SPHX_TYPE_BOUNDARY && k != m.sentinel;
     row = (int)(k / (uint32_t)sort_resx);
     const float4 q = phys[i];
-    me.f[0] = p.x; me.f[1] = p.y; me.f[2] = p.z;
-    me.f[12] = m.physical ? q.x / (q.y * m.rc) : 1.f / q.y;
+    float sig[9];
 #pragma unroll
-    for (int k9 = 0; k9 < 9; ++k9) me.f[3 + k9] = S9[(size_t)k9 * cap + i];
+    for (int k9 = 0; k9 < 9; ++k9) sig[k9] = S9[(size_t)k9 * cap + i];
+    force_part<W>(m, p, m.physical ? q.x / (q.y * m.rc) : 1.f / q.y, sig, me, base);
   }
-  int base[3];
-  stencil_base<W>(m, p, base);
   BinBox bx;
   int seg, slot;
   block_bins<W>(S, valid, row, base, bx, seg, slot);
   if (slot >= 0)
 #pragma unroll
-    for (int q = 0; q < NF; ++q) pay[q][slot] = me.f[q];
+    for (int q = 0; q < Part::NF; ++q) pay[q][slot] = me.f[q];
   __syncthreads();
   const int w = xcd_tile();
   tile_sums<W>(m, S, bx, o.tiles + (size_t)w * MPM_TILE,
-               [&](int x, int y, int z, int q0, int q1, float4& acc) {
+               [&](int x, int y, int z, int ky, int kz, int q0, int q1, float4& acc) {
     for (int q = q0; q < q1; ++q) {
-      ForcePart o2;
-#pragma unroll
-      for (int k = 0; k < NF; ++k) o2.f[k] = pay[k][q];
-      const float4 c = force_contrib<W>(m, o2, x, y, z);
+      const int kx = x - __float_as_int(pay[Part::BX][q]);
+      const float4 c = force_contrib<W>([&](int f) { return pay[f][q]; }, kx, ky, kz);
       acc.x += c.x; acc.y += c.y; acc.z += c.z;
     }
   });
   if (valid && !bx.tiled(seg))
-    direct_add<W>(m, o, base, [&](int x, int y, int z) { return force_contrib<W>(m, me, x, y, z); });
+    direct_add<W>(m, o, base, [&](int x, int y, int z, int kx, int ky, int kz) {
+      return force_contrib<W>([&](int f) { return me.f[f]; }, kx, ky, kz);
+    });
   tile_finish<W>(m, o, w, bx);
 }
 
