@@ -481,17 +481,54 @@ struct SortGrid {
   float3 gmin, fac;
 };
 
+constexpr int GATHER_CAND = 512;   // candidate workgroups listed at a time
+constexpr int GATHER_BOX = 256;    // intersecting boxes held at a time
+
+// LDS of a one-wave gather workgroup.
+struct GatherShared {
+  int cand[GATHER_CAND];
+  int4 box[GATHER_BOX][2];   // {lo x, y, z, ex x}, {ex y, ex z, tile entry 0, -}
+};
+
+// Adds to `sum`, in list order, the tile entry of this lane's node (x, y,
+// z) from each of the first nbox listed boxes; four loads in flight.
+__device__ __forceinline__ void sum_boxes(const GatherShared& G, int nbox, const float4* __restrict__ tiles,
+                                          int x, int y, int z, float4& sum) {
+  for (int k = 0; k < nbox; k += 4) {
+    float4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (k + u < nbox) {
+        const int4 a = G.box[k + u][0], b = G.box[k + u][1];
+        const int dx = x - a.x, dy = y - a.y, dz = z - a.z;
+        if (dx >= 0 && dx < a.w && dy >= 0 && dy < b.x && dz >= 0 && dz < b.y)
+          v[u] = tiles[(size_t)b.z + (dy * b.y + dz) * a.w + dx];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      sum.x += v[u].x; sum.y += v[u].y; sum.z += v[u].z; sum.w += v[u].w;
+    }
+  }
+}
+
 // Sum, for node (x, y, z) of node block (bx, by, bz), the tile entries of
 // every workgroup whose particles can reach the block.  Those particles lie
 // within R cells of the block (R = support + 1/2 cell of slack, far above
 // the hash's rounding), so in the sort-grid cells that range maps to; their
-// sorted ranges name the workgroups.  Rows of cells are visited in key
-// order, so workgroup ids only grow and each is summed once.
+// sorted ranges name the workgroups.  The wave works lane-parallel: one
+// sort-grid row per lane gives a workgroup range (rows in key order, so a
+// running maximum removes the workgroups already listed), the candidates'
+// headers are loaded one per lane, and the boxes that meet the block are
+// listed in workgroup order and summed.  Called by the whole (one-wave)
+// workgroup.
 template <int W>
 __device__ float4 gather_tiles(const MP& m, const SortGrid& g, const int4* __restrict__ hdr,
-                               const float4* __restrict__ tiles, int bx, int by, int bz, int x,
-                               int y, int z) {
+                               const float4* __restrict__ tiles, GatherShared& G, int bx, int by,
+                               int bz, int x, int y, int z) {
   const int R = W == 4 ? 3 : 2;
+  const int lane = threadIdx.x & 63;
   const int bb[3] = {bx, by, bz};
   const float xm[3] = {m.xmin.x, m.xmin.y, m.xmin.z};
   const float gm[3] = {g.gmin.x, g.gmin.y, g.gmin.z};
@@ -506,30 +543,89 @@ __device__ float4 gather_tiles(const MP& m, const SortGrid& g, const int4* __res
     chi[a] = min((int)floorf((hi - gm[a]) * fc[a]), rs[a] - 2);
   }
   float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (clo[0] > chi[0]) return sum;
-  int last = -1;
-  for (int cy = clo[1]; cy <= chi[1]; ++cy)
-    for (int cz = clo[2]; cz <= chi[2]; ++cz) {
+  if (clo[0] > chi[0] || clo[1] > chi[1] || clo[2] > chi[2]) return sum;
+  const int nz = chi[2] - clo[2] + 1;
+  const int nrows = (chi[1] - clo[1] + 1) * nz;
+  const int nlo[3] = {bx * MPM_BLK, by * MPM_BLK, bz * MPM_BLK};
+  int last = -1, nbox = 0;
+  for (int r0 = 0; r0 < nrows; r0 += 64) {
+    const int r = r0 + lane;
+    int w0 = 0, w1 = -1;
+    if (r < nrows) {
+      const int cy = clo[1] + r / nz, cz = clo[2] + r % nz;
       const int rowbase = (cy * g.res.z + cz) * g.res.x;
       const int s0 = g.cell_start[rowbase + clo[0]], s1 = g.cell_start[rowbase + chi[0] + 1];
-      if (s1 <= s0) continue;
-      const int w1 = (s1 - 1) / MPM_THREADS;
-      for (int w = max(s0 / MPM_THREADS, last + 1); w <= w1; ++w) {
-        const int4 h0 = hdr[3 * (size_t)w], h1 = hdr[3 * (size_t)w + 1], h2 = hdr[3 * (size_t)w + 2];
-        const float4* tw = tiles + (size_t)w * MPM_TILE;
-        int dx = x - h0.x, dy = y - h0.y, dz = z - h0.z;
-        if (h0.w > 0 && dx >= 0 && dx < h0.w && dy >= 0 && dy < h1.x && dz >= 0 && dz < h1.y) {
-          float4 v = tw[(dy * h1.y + dz) * h0.w + dx];
-          sum.x += v.x; sum.y += v.y; sum.z += v.z; sum.w += v.w;
-        }
-        dx = x - h1.z; dy = y - h1.w; dz = z - h2.x;
-        if (h2.y > 0 && dx >= 0 && dx < h2.y && dy >= 0 && dy < h2.z && dz >= 0 && dz < h2.w) {
-          float4 v = tw[h0.w * h1.x * h1.y + (dy * h2.w + dz) * h2.y + dx];
-          sum.x += v.x; sum.y += v.y; sum.z += v.z; sum.w += v.w;
-        }
+      if (s1 > s0) {
+        w0 = s0 / MPM_THREADS;
+        w1 = (s1 - 1) / MPM_THREADS;
       }
-      last = max(last, w1);
     }
+    int pm = w1;   // running maximum of the last workgroup listed
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(pm, o);
+      if (lane >= o) pm = max(pm, v);
+    }
+    int prev = __shfl_up(pm, 1);
+    if (lane == 0) prev = -1;
+    w0 = max(w0, max(prev, last) + 1);
+    const int cnt = w1 >= w0 ? w1 - w0 + 1 : 0;
+    int inc = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(inc, o);
+      if (lane >= o) inc += v;
+    }
+    const int off = inc - cnt, total = __shfl(inc, 63);
+    last = max(last, __shfl(pm, 63));
+    for (int cb = 0; cb < total; cb += GATHER_CAND) {
+      const int j0 = max(off, cb), j1 = min(off + cnt, cb + GATHER_CAND);
+      for (int j = j0; j < j1; ++j) G.cand[j - cb] = w0 + (j - off);
+      __syncthreads();
+      const int nc = min(total - cb, GATHER_CAND);
+      for (int c0 = 0; c0 < nc; c0 += 64) {
+        const int c = c0 + lane;
+        bool hit[2] = {false, false};
+        int4 e[2][2];
+        if (c < nc) {
+          const int w = G.cand[c];
+          const int4 h0 = hdr[3 * (size_t)w], h1 = hdr[3 * (size_t)w + 1], h2 = hdr[3 * (size_t)w + 2];
+          const int lo0[3] = {h0.x, h0.y, h0.z}, ex0[3] = {h0.w, h1.x, h1.y};
+          const int lo1[3] = {h1.z, h1.w, h2.x}, ex1[3] = {h2.y, h2.z, h2.w};
+          hit[0] = ex0[0] > 0;
+          hit[1] = ex1[0] > 0;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            hit[0] = hit[0] && lo0[a] <= nlo[a] + MPM_BLK - 1 && lo0[a] + ex0[a] > nlo[a];
+            hit[1] = hit[1] && lo1[a] <= nlo[a] + MPM_BLK - 1 && lo1[a] + ex1[a] > nlo[a];
+          }
+          const int base = w * MPM_TILE;
+          e[0][0] = make_int4(lo0[0], lo0[1], lo0[2], ex0[0]);
+          e[0][1] = make_int4(ex0[1], ex0[2], base, 0);
+          e[1][0] = make_int4(lo1[0], lo1[1], lo1[2], ex1[0]);
+          e[1][1] = make_int4(ex1[1], ex1[2], base + ex0[0] * ex0[1] * ex0[2], 0);
+        }
+        const uint64_t b0 = __ballot(hit[0]), b1 = __ballot(hit[1]);
+        const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+        const int rel = __popcll(b0 & below) + __popcll(b1 & below);
+        const int add = __popcll(b0) + __popcll(b1);   // at most 128
+        if (nbox + add > GATHER_BOX) {   // full: sum what is listed first
+          __syncthreads();
+          sum_boxes(G, nbox, tiles, x, y, z, sum);
+          __syncthreads();
+          nbox = 0;
+        }
+        const int p0 = nbox + rel;
+        if (hit[0]) { G.box[p0][0] = e[0][0]; G.box[p0][1] = e[0][1]; }
+        if (hit[1]) { G.box[p0 + hit[0]][0] = e[1][0]; G.box[p0 + hit[0]][1] = e[1][1]; }
+        nbox += add;
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  sum_boxes(G, nbox, tiles, x, y, z, sum);
+  __syncthreads();
   return sum;
 }
 
@@ -602,9 +698,10 @@ __global__ __launch_bounds__(64) void k_mpm_gather_p2g(MP m, SortGrid g, const i
                                                        float4* __restrict__ acc,
                                                        const int* __restrict__ direct_flag,
                                                        float4* __restrict__ nodev) {
+  __shared__ GatherShared G;
   const int any_direct = direct_flag[0];
   FOR_LISTED_NODES(m, list, count, {
-    float4 a = gather_tiles<W>(m, g, hdr, tiles, bx, by, bz, x, y, z);
+    float4 a = gather_tiles<W>(m, g, hdr, tiles, G, bx, by, bz, x, y, z);
     if (in_res(m, x, y, z)) {
       const size_t id = node_id(m, x, y, z);
       if (any_direct) {
@@ -820,9 +917,10 @@ __global__ __launch_bounds__(64) void k_mpm_node(MP m, SortGrid g, const int* __
                                                  float4* __restrict__ force,
                                                  const int* __restrict__ direct_flag,
                                                  float4* __restrict__ nodev) {
+  __shared__ GatherShared G;
   const int any_direct = direct_flag[1];
   FOR_LISTED_NODES(m, list, count, {
-    float4 f = gather_tiles<W>(m, g, hdr, tiles, bx, by, bz, x, y, z);
+    float4 f = gather_tiles<W>(m, g, hdr, tiles, G, bx, by, bz, x, y, z);
     if (in_res(m, x, y, z)) {
       const size_t id = node_id(m, x, y, z);
       if (any_direct) {
@@ -1064,6 +1162,8 @@ int sphx_mpm_setup(sphx_context* c, int kernel, int physical) {
   SPHX_TRY(M.list.alloc(M.blocks));
   SPHX_TRY(M.count.alloc(1));
   M.tiles_cap = (size_t)blocks_for((int)cap, MPM_THREADS);
+  SPHX_CHECK(M.tiles_cap * MPM_TILE < (size_t)INT32_MAX, "sphx_mpm_setup",
+             "too many particles for the MPM tile index");
   SPHX_TRY(M.tiles.alloc(M.tiles_cap * MPM_TILE));
   SPHX_TRY(M.hdr.alloc(M.tiles_cap * 3));
   SPHX_TRY(M.direct.alloc(2));
