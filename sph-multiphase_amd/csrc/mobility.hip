@@ -66,6 +66,47 @@ __global__ void k_rpy_split_f(const float* __restrict__ F, int n, int np, int K,
   F3[t] = r;
 }
 
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+
+// Two floats to packed bf16 (element 0 in the low half), round to nearest
+// even (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint32_t pk_bf16(float a, float b) {
+  const bf16x2 v = {(__bf16)a, (__bf16)b};
+  return __builtin_bit_cast(uint32_t, v);
+}
+// x0 = hi + lo + r3 and x1 likewise, the pieces of both packed per word.
+__device__ __forceinline__ void split3_pk(float x0, float x1, uint32_t& h, uint32_t& l,
+                                          uint32_t& r) {
+  h = pk_bf16(x0, x1);
+  const float d0 = x0 - __uint_as_float(h << 16), d1 = x1 - __uint_as_float(h & 0xFFFF0000u);
+  l = pk_bf16(d0, d1);
+  const float e0 = d0 - __uint_as_float(l << 16), e1 = d1 - __uint_as_float(l & 0xFFFF0000u);
+  r = pk_bf16(e0, e1);
+}
+
+// The six distinct entries (00 11 22 01 02 12) of the polydisperse RPY
+// pair block for spheres p = {x, y, z, a} and q, zero for q == p or
+// outside the set.
+__device__ __forceinline__ void rpy_pair(float4 sp, float4 sq, bool valid, float inv8pimu,
+                                         float m[6]) {
+#pragma unroll
+  for (int k = 0; k < 6; ++k) m[k] = 0.f;
+  if (!valid) return;
+  const float dx = sq.x - sp.x, dy = sq.y - sp.y, dz = sq.z - sp.z;
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  const float ir2 = 1.f / r2;
+  const float c0 = inv8pimu * sqrtf(ir2);
+  const float s2 = sp.w * sp.w + sq.w * sq.w;
+  const float ci = c0 * (1.f + s2 * ir2 * (1.f / 3.f));
+  const float cr = c0 * (1.f - s2 * ir2) * ir2;
+  m[0] = ci + cr * dx * dx;
+  m[1] = ci + cr * dy * dy;
+  m[2] = ci + cr * dz * dz;
+  m[3] = cr * dx * dy;
+  m[4] = cr * dx * dz;
+  m[5] = cr * dy * dz;
+}
+
 // Symmetric 3x3 block index: 00 11 22 01 02 12
 __device__ __forceinline__ int sym_index(int a, int b) {
   if (a == b) return a;
@@ -93,10 +134,10 @@ __global__ __launch_bounds__(RPY_THREADS) void k_rpy_tile(
   const int s = blockIdx.y;
   const int q_begin = s * seg, q_end = min(np, q_begin + seg);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  // pair generation: thread -> sphere p0 + (tid & 63), 8 consecutive q
-  const int gp = p0 + (tid & 63);
-  const int qo = (tid >> 6) * 8;
-  const float4 sp = gp < n ? sph[gp] : make_float4(0.f, 0.f, 0.f, 1.f);
+  // pair generation: thread -> spheres q0 + 2 (tid & 15) + {0, 1} and rows
+  // p0 + (tid >> 4) + 16 t, t = 0..3; a wave writes 4 consecutive rows of
+  // packed words (256 contiguous bytes per store)
+  const int qp = 2 * (tid & 15), pr = tid >> 4;
   const float inv8pimu = 1.f / (8.f * RPY_PI * mu);
   f32x4 acc[3][4];
 #pragma unroll
@@ -107,43 +148,24 @@ __global__ __launch_bounds__(RPY_THREADS) void k_rpy_tile(
   for (int q0 = q_begin; q0 < q_end; q0 += RPY_TQ) {
     __syncthreads();
     {
-      uint16_t h[6][8], l[6][8], r[6][8];
+      const int gq0 = q0 + qp, gq1 = gq0 + 1;
+      const float4 sq0 = gq0 < n ? sph[gq0] : make_float4(0.f, 0.f, 0.f, 1.f);
+      const float4 sq1 = gq1 < n ? sph[gq1] : make_float4(0.f, 0.f, 0.f, 1.f);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        int gq = q0 + qo + j;
-        float m[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        if (gq < n && gp < n && gq != gp) {
-          float4 sq = sph[gq];
-          float dx = sq.x - sp.x, dy = sq.y - sp.y, dz = sq.z - sp.z;
-          float r2 = dx * dx + dy * dy + dz * dz;
-          float ir2 = 1.f / r2;
-          float c0 = inv8pimu * sqrtf(ir2);
-          float s2 = sp.w * sp.w + sq.w * sq.w;
-          float ci = c0 * (1.f + s2 * ir2 * (1.f / 3.f));
-          float cr = c0 * (1.f - s2 * ir2) * ir2;
-          m[0] = ci + cr * dx * dx;
-          m[1] = ci + cr * dy * dy;
-          m[2] = ci + cr * dz * dz;
-          m[3] = cr * dx * dy;
-          m[4] = cr * dx * dz;
-          m[5] = cr * dy * dz;
+      for (int t = 0; t < 4; ++t) {
+        const int pl = pr + 16 * t, gp = p0 + pl;
+        const float4 sp = gp < n ? sph[gp] : make_float4(0.f, 0.f, 0.f, 1.f);
+        float m0[6], m1[6];
+        rpy_pair(sp, sq0, gp < n && gq0 < n && gq0 != gp, inv8pimu, m0);
+        rpy_pair(sp, sq1, gp < n && gq1 < n && gq1 != gp, inv8pimu, m1);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          uint32_t h, l, r;
+          split3_pk(m0[k], m1[k], h, l, r);
+          *reinterpret_cast<uint32_t*>(&Mh[k][pl][qp]) = h;
+          if (mode >= 3) *reinterpret_cast<uint32_t*>(&Ml[k][pl][qp]) = l;
+          if (mode == 6) *reinterpret_cast<uint32_t*>(&M3[k][pl][qp]) = r;
         }
-#pragma unroll
-        for (int k = 0; k < 6; ++k) split3(m[k], h[k][j], l[k][j], r[k][j]);
-      }
-      const int pl = tid & 63;
-#pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        uint4 vh, vl, vr;
-        vh.x = h[k][0] | (h[k][1] << 16); vh.y = h[k][2] | (h[k][3] << 16);
-        vh.z = h[k][4] | (h[k][5] << 16); vh.w = h[k][6] | (h[k][7] << 16);
-        vl.x = l[k][0] | (l[k][1] << 16); vl.y = l[k][2] | (l[k][3] << 16);
-        vl.z = l[k][4] | (l[k][5] << 16); vl.w = l[k][6] | (l[k][7] << 16);
-        vr.x = r[k][0] | (r[k][1] << 16); vr.y = r[k][2] | (r[k][3] << 16);
-        vr.z = r[k][4] | (r[k][5] << 16); vr.w = r[k][6] | (r[k][7] << 16);
-        *reinterpret_cast<uint4*>(&Mh[k][pl][qo]) = vh;
-        *reinterpret_cast<uint4*>(&Ml[k][pl][qo]) = vl;
-        if (mode == 6) *reinterpret_cast<uint4*>(&M3[k][pl][qo]) = vr;
       }
     }
     __syncthreads();
@@ -152,7 +174,8 @@ __global__ __launch_bounds__(RPY_THREADS) void k_rpy_tile(
     // B column = lane % 16, same k; D rows 4 (lane / 16) + i, column lane % 16.
     const int arow = 16 * wave + (lane & 15);
     const int kk = 8 * (lane >> 4);
-#pragma unroll
+    // one b at a time: its F pieces (48 VGPRs) are all that is live
+#pragma unroll 1
     for (int b = 0; b < 3; ++b) {
       bf16x8 fh[4], fl[4], f3[4];
 #pragma unroll

@@ -5,12 +5,12 @@
 // particle-centric gathers over 4^3 nodes (stress :477-592, G2P :891-951),
 // one thread per node or particle.  Here every transfer is particle-centric
 // over the cell-sorted particles:
-//   * P2G and the node force run in two passes without float atomics.  A
-//     workgroup of 256 consecutive sorted particles bins them by stencil
-//     base cell in LDS (one int atomic each), and each thread then sums,
-//     for the nodes it owns in the bounding box of the stencils, the
-//     particles of the W x W cell rows that reach the node; the box is
-//     stored whole as the workgroup's tile.  A per-node-block pass sums
+//   * P2G and the node force run in two passes.  A workgroup of 256
+//     consecutive sorted particles bins them by stencil base cell in LDS
+//     (one int atomic each) and accumulates into an LDS tile covering the
+//     bounding box of their stencils: the W^3 stencil offsets are phases,
+//     in which distinct cells write distinct nodes and the particles of a
+//     cell add in rank order (no float atomics).  The tile is stored whole.  A per-node-block pass sums
 //     the tiles of the workgroups whose sorted particle range can reach the
 //     block (found through the sort grid's cell_start).  Particles of a
 //     box too large for a tile add to the nodes directly with atomics.
@@ -138,16 +138,6 @@ struct TileOut {
   int* count;
 };
 
-// Spline weight and derivative along one axis for node n (argument
-// (x_p - x_I) / dx, x_I = xmin + n dx: the expression of make_stencil).
-template <int W>
-__device__ __forceinline__ void axis_weight(float pc, float xm, int n, float dx, float& w, float& dw) {
-  float xi = xm + (float)n * dx;
-  float r = (pc - xi) / dx;
-  w = W == 4 ? sp_cubic(r) : sp_quad(r);
-  dw = W == 4 ? sp_dcubic(r) : sp_dquad(r);
-}
-
 template <int W>
 __device__ __forceinline__ void stencil_base(const MP& m, float4 p, int base[3]) {
   const float pc[3] = {p.x, p.y, p.z};
@@ -187,6 +177,7 @@ struct BinShared {
 struct BinBox {
   int lo[2][3], nb[2][3], ex[2][3];
   int off[2], coff[2];
+  int tot, nbin;   // tile entries of the tiled boxes; binned particles
   // element k of box g (g varies by lane: a select, not an indexed load)
   __device__ __forceinline__ static int get(const int (&a)[2][3], int g, int k) {
     return g ? a[1][k] : a[0][k];
@@ -199,12 +190,12 @@ struct BinBox {
 // workgroup that wraps from the end of one row to the start of the next
 // keeps two compact boxes.  A box is tiled when its nodes fit MPM_TILE and
 // its cells MPM_CELLS; seg is the lane's box, slot its place in the binned
-// order (-1: not binned).  S.start[c] .. S.start[c + 1] are the slots of
+// order and cell its bin (-1: not binned).  S.start[c] .. S.start[c + 1] are the slots of
 // cell c; cells of one x-row are consecutive.  Must be called by the whole
 // workgroup (it synchronises).
 template <int W>
-__device__ void block_bins(BinShared& S, bool valid, int row, const int base[3], BinBox& B,
-                           int& seg, int& slot) {
+__device__ __forceinline__ void block_bins(BinShared& S, bool valid, int row, const int base[3], BinBox& B,
+                           int& seg, int& slot, int& cell) {
   const int t0 = threadIdx.x, lane = t0 & 63, wv = t0 >> 6;
   if (t0 < 13) S.sb[t0] = (t0 % 6 < 3 || t0 == 12) ? INT_MAX : INT_MIN;
   __syncthreads();
@@ -250,6 +241,7 @@ __device__ void block_bins(BinShared& S, bool valid, int row, const int base[3],
     if (!tiled1) B.nb[1][a] = B.ex[1][a] = 0;
   }
   const int ctot = B.coff[1] + (tiled1 ? (int)C[1] : 0);
+  B.tot = B.off[1] + (tiled1 ? (int)V[1] : 0);
   for (int c = t0; c < ctot; c += blockDim.x) S.cnt[c] = 0;
   __syncthreads();
   int cb = -1, rank = 0;
@@ -282,52 +274,90 @@ __device__ void block_bins(BinShared& S, bool valid, int row, const int base[3],
   if (t0 == (int)blockDim.x - 1) S.start[ctot] = run;
   __syncthreads();
   slot = cb >= 0 ? S.start[cb] + rank : -1;
+  cell = cb;
+  B.nbin = S.start[ctot];
 }
 
-template <int W, int G, class F>
-__device__ __forceinline__ void tile_sums_box(const MP& m, const BinShared& S, const BinBox& B,
-                                              float4* __restrict__ dst, F& pair_sum) {
-  const int l0 = B.lo[G][0], l1 = B.lo[G][1], l2 = B.lo[G][2];
-  const int e0 = B.ex[G][0], e1 = B.ex[G][1], e2 = B.ex[G][2];
-  const int n0 = B.nb[G][0], n1 = B.nb[G][1], n2 = B.nb[G][2];
-  if (e0 == 0) return;
-  const int V = e0 * e1 * e2, coff = B.coff[G], off = B.off[G];
-  for (int t = threadIdx.x; t < V; t += blockDim.x) {
-    const int tx = t % e0, tz = (t / e0) % e2, ty = t / (e0 * e2);
-    const int x = l0 + tx, y = l1 + ty, z = l2 + tz;
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (in_res(m, x, y, z)) {
-      const int cx0 = max(tx - (W - 1), 0), cx1 = min(tx, n0 - 1);
-      for (int ky = 0; ky < W; ++ky) {
-        const int cy = ty - ky;
-        if (cy < 0 || cy >= n1) continue;
-        for (int kz = 0; kz < W; ++kz) {
-          const int cz = tz - kz;
-          if (cz < 0 || cz >= n2) continue;
-          const int rc = coff + (cy * n2 + cz) * n0;
-          pair_sum(x, y, z, ky, kz, S.start[rc + cx0], S.start[rc + cx1 + 1], acc);
-        }
-      }
-    }
-    dst[off + t] = acc;
-  }
+// LDS of the phased tile accumulation: the tile of the workgroup's boxes,
+// and the particle and bin of each binned slot.
+struct PhaseShared {
+  float4 tile[MPM_TILE];
+  int pidx[MPM_THREADS];
+  int pcell[MPM_THREADS];
+};
+
+// The binned slot a thread accumulates: its particle, the cell's place in
+// its box (local coordinates), the tile entry of the cell's node (0, 0, 0)
+// and the box extents, the particle's rank within its cell, and whether the
+// cell's slots straddle two waves.
+struct SlotRef {
+  bool act, cross;
+  int j, rank, entry0, ex0, exz;
+};
+
+__device__ __forceinline__ SlotRef slot_ref(const BinShared& S, const PhaseShared& T, const BinBox& B) {
+  SlotRef r = {false, false, 0, 0, 0, 0, 0};
+  const int t = threadIdx.x;
+  if (t >= B.nbin) return r;
+  r.act = true;
+  r.j = T.pidx[t];
+  const int c = T.pcell[t];
+  const int s0 = S.start[c], s1 = S.start[c + 1];
+  r.rank = t - s0;
+  r.cross = (s0 >> 6) != ((s1 - 1) >> 6);
+  // box of the cell: box 0 holds cells [0, coff1) when it is tiled
+  const int g = (B.tiled(0) && c < B.coff[1]) ? 0 : 1;
+  const int cc = c - (g ? B.coff[1] : 0);
+  const int n0 = BinBox::get(B.nb, g, 0), n2 = BinBox::get(B.nb, g, 2);
+  const int e0 = BinBox::get(B.ex, g, 0), e2 = BinBox::get(B.ex, g, 2);
+  const int cx = cc % n0, cz = (cc / n0) % n2, cy = cc / (n0 * n2);
+  r.ex0 = e0;
+  r.exz = e0 * e2;
+  r.entry0 = (g ? B.off[1] : 0) + (cy * e2 + cz) * e0 + cx;
+  return r;
 }
 
-// Writes, for each node of each tiled box, pair_sum(x, y, z, ky, kz, q0, q1, acc)
-// over the binned slots of every cell row whose particles reach the node
-// (the W x W rows of cells node - k, k in [0, W)); nodes outside the grid
-// get zeros (the gather never reads them).
+// Accumulates contrib(a, b, c) of every binned slot into the LDS tile
+// without float atomics.  The stencil offsets (a, b, c) are phases
+// separated by barriers; within a phase distinct cells write distinct
+// nodes, and the particles of one cell (consecutive slots, one wave) add in
+// rank order, each rank a separate read-modify-write of the wave.  Cells
+// whose slots straddle two waves add with LDS atomics.  Called by the whole
+// workgroup.
 template <int W, class F>
-__device__ void tile_sums(const MP& m, const BinShared& S, const BinBox& B, float4* __restrict__ dst,
-                          F&& pair_sum) {
-  tile_sums_box<W, 0>(m, S, B, dst, pair_sum);
-  tile_sums_box<W, 1>(m, S, B, dst, pair_sum);
+__device__ __forceinline__ void phased_tile(PhaseShared& T, const SlotRef& r, F&& contrib) {
+  const int maxr = wave_max(r.act && !r.cross ? r.rank : -1);
+#pragma clang loop unroll(full)
+  for (int b = 0; b < W; ++b)
+#pragma clang loop unroll(full)
+    for (int c = 0; c < W; ++c)
+#pragma clang loop unroll(full)
+      for (int a = 0; a < W; ++a) {
+        const float4 v = r.act ? contrib(a, b, c) : make_float4(0.f, 0.f, 0.f, 0.f);
+        float4* d = &T.tile[r.entry0 + b * r.exz + c * r.ex0 + a];
+        if (r.act && r.cross) {
+          atomicAdd(&d->x, v.x);
+          atomicAdd(&d->y, v.y);
+          atomicAdd(&d->z, v.z);
+          atomicAdd(&d->w, v.w);
+        }
+        for (int k = 0; k <= maxr; ++k) {
+          if (r.act && !r.cross && r.rank == k) {
+            float4 o = *d;
+            o.x += v.x; o.y += v.y; o.z += v.z; o.w += v.w;
+            *d = o;
+          }
+          // each rank's update reaches LDS before the next rank reads it
+          asm volatile("" ::: "memory");
+        }
+        __syncthreads();
+      }
 }
 
 // Header of workgroup w and the node blocks of its tiled boxes (clamped to
 // the grid) marked active.
 template <int W>
-__device__ void tile_finish(const MP& m, const TileOut& o, int w, const BinBox& B) {
+__device__ __forceinline__ void tile_finish(const MP& m, const TileOut& o, int w, const BinBox& B) {
   const int t0 = threadIdx.x;
   if (t0 == 0) {
     o.hdr[3 * (size_t)w] = make_int4(B.lo[0][0], B.lo[0][1], B.lo[0][2], B.ex[0][0]);
@@ -352,11 +382,11 @@ __device__ void tile_finish(const MP& m, const TileOut& o, int w, const BinBox& 
   }
 }
 
-// A particle of a box that is not tiled adds contrib(x, y, z, kx, ky, kz) to the
-// nodes of its stencil (base .. base + W - 1) in global memory and marks
-// the blocks its stencil spans.
+// A particle of a box that is not tiled adds contrib(a, b, c) to node
+// base + (a, b, c) of its stencil in global memory and marks the blocks its
+// stencil spans.
 template <int W, class F>
-__device__ void direct_add(const MP& m, const TileOut& o, const int base[3], F&& contrib) {
+__device__ __forceinline__ void direct_add(const MP& m, const TileOut& o, const int base[3], F&& contrib) {
 #pragma unroll
   for (int b = 0; b < W; ++b)
 #pragma unroll
@@ -365,7 +395,7 @@ __device__ void direct_add(const MP& m, const TileOut& o, const int base[3], F&&
       for (int a = 0; a < W; ++a) {
         int x = base[0] + a, y = base[1] + b, z = base[2] + c;
         if (!in_res(m, x, y, z)) continue;
-        float4 v = contrib(x, y, z, a, b, c);
+        float4 v = contrib(a, b, c);
         float4* d = o.direct + node_id(m, x, y, z);
         atomicAdd(&d->x, v.x);
         atomicAdd(&d->y, v.y);
@@ -381,95 +411,6 @@ __device__ void direct_add(const MP& m, const TileOut& o, const int base[3], F&&
         int z = min(max(base[2] + c * (W - 1), 0), m.res.z - 1);
         mark_block(o.flags, o.list, o.count, block_id(m, x / MPM_BLK, y / MPM_BLK, z / MPM_BLK));
       }
-}
-
-// What one particle gives node (x, y, z) = base + (kx, ky, kz).  The
-// particle's 1D weights (and derivatives) are evaluated once per axis
-// (make_stencil's expressions) and kept with it, so a (particle, node) pair
-// is products and sums only.
-// P2G (:823-889 / :197-236): {w m_p (v_p + B_p iDp (x_I - x_p) ss), w m_p}.
-template <int W>
-struct P2GPart {
-  // x, y, z, vx, vy, vz, m, B (row-major, times iDp), w[axis][k], base x
-  static constexpr int WO = 16, BX = 16 + 3 * W, NF = BX + 1;
-  float f[NF];
-};
-template <int W>
-__device__ __forceinline__ void p2g_part(const MP& m, int apic, float4 p, float4 v, float mp,
-                                         const float* __restrict__ B, int cap, int i,
-                                         P2GPart<W>& q, int base[3]) {
-  Stencil<W> s;
-  make_stencil<W>(m, p, s);
-  q.f[0] = p.x; q.f[1] = p.y; q.f[2] = p.z;
-  q.f[3] = v.x; q.f[4] = v.y; q.f[5] = v.z;
-  q.f[6] = mp;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) q.f[7 + k] = apic ? B[(size_t)k * cap + i] * m.iDp : 0.f;
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int k = 0; k < W; ++k) q.f[P2GPart<W>::WO + a * W + k] = s.w[a][k];
-  q.f[P2GPart<W>::BX] = __int_as_float(s.base[0]);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) base[a] = s.base[a];
-}
-// get(k) returns field k of the particle (registers or LDS)
-template <int W, class G>
-__device__ __forceinline__ float4 p2g_contrib(const MP& m, int apic, G&& get, int x, int y, int z,
-                                              int kx, int ky, int kz) {
-  constexpr int WO = P2GPart<W>::WO;
-  const float wt = get(WO + kx) * get(WO + W + ky) * get(WO + 2 * W + kz);
-  float vb0 = get(3), vb1 = get(4), vb2 = get(5);
-  if (apic) {
-    const float d0 = (m.xmin.x + (float)x * m.dx - get(0)) * m.ss;
-    const float d1 = (m.xmin.y + (float)y * m.dx - get(1)) * m.ss;
-    const float d2 = (m.xmin.z + (float)z * m.dx - get(2)) * m.ss;
-    vb0 += get(7) * d0 + get(8) * d1 + get(9) * d2;
-    vb1 += get(10) * d0 + get(11) * d1 + get(12) * d2;
-    vb2 += get(13) * d0 + get(14) * d1 + get(15) * d2;
-  }
-  const float mp = get(6);
-  return make_float4(vb0 * mp * wt, vb1 * mp * wt, vb2 * mp * wt, wt * mp);
-}
-
-// Node force (:676-697): sigma_p grad N scale_p, with the reference's
-// derivative taken at (x_I - x_p)/dx (the stencil's, sign flipped).
-template <int W>
-struct ForcePart {
-  // sigma (row-major), scale, w[axis][k], dw[axis][k], base x
-  static constexpr int WO = 10, DO = 10 + 3 * W, BX = 10 + 6 * W, NF = BX + 1;
-  float f[NF];
-};
-template <int W>
-__device__ __forceinline__ void force_part(const MP& m, float4 p, float scale, const float* sig,
-                                           ForcePart<W>& q, int base[3]) {
-  Stencil<W> s;
-  make_stencil<W>(m, p, s);
-#pragma unroll
-  for (int k = 0; k < 9; ++k) q.f[k] = sig[k];
-  q.f[9] = scale;
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int k = 0; k < W; ++k) {
-      q.f[ForcePart<W>::WO + a * W + k] = s.w[a][k];
-      q.f[ForcePart<W>::DO + a * W + k] = s.dw[a][k];
-    }
-  q.f[ForcePart<W>::BX] = __int_as_float(s.base[0]);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) base[a] = s.base[a];
-}
-template <int W, class G>
-__device__ __forceinline__ float4 force_contrib(G&& get, int kx, int ky, int kz) {
-  constexpr int WO = ForcePart<W>::WO, DO = ForcePart<W>::DO;
-  const float n0 = get(WO + kx), n1 = get(WO + W + ky), n2 = get(WO + 2 * W + kz);
-  const float d0 = get(DO + kx), d1 = get(DO + W + ky), d2 = get(DO + 2 * W + kz);
-  const float nw0 = -d0 * n1 * n2, nw1 = -d1 * n0 * n2, nw2 = -d2 * n0 * n1;
-  const float f0 = get(0) * nw0 + get(1) * nw1 + get(2) * nw2;
-  const float f1 = get(3) * nw0 + get(4) * nw1 + get(5) * nw2;
-  const float f2 = get(6) * nw0 + get(7) * nw1 + get(8) * nw2;
-  const float sc = get(9);
-  return make_float4(f0 * sc, f1 * sc, f2 * sc, 0.f);
 }
 
 // The sort grid as the gather needs it: which sorted particles lie in
@@ -637,40 +578,73 @@ __global__ __launch_bounds__(MPM_THREADS) void k_mpm_p2g(
     MP m, int apic, const uint32_t* __restrict__ key, const float4* __restrict__ pos,
     const float4* __restrict__ vel, const float4* __restrict__ phys, const float* __restrict__ B,
     int cap, int sort_resx, TileOut o) {
-  using Part = P2GPart<W>;
   __shared__ BinShared S;
-  __shared__ float pay[Part::NF][MPM_THREADS];   // the binned particles, field-major
+  __shared__ PhaseShared T;
   const int i = xcd_index();
   bool valid = false;
   int row = 0, base[3] = {0, 0, 0};
-  Part me;
   if (i < m.n) {
     const float4 p = pos[i];
     const uint32_t k = key[i];
     valid = type_of(p) != SPHX_TYPE_BOUNDARY && k != m.sentinel;
     row = (int)(k / (uint32_t)sort_resx);
-    p2g_part<W>(m, apic, p, vel[i], phys[i].x, B, cap, i, me, base);
+    stencil_base<W>(m, p, base);
   }
   BinBox bx;
-  int seg, slot;
-  block_bins<W>(S, valid, row, base, bx, seg, slot);
-  if (slot >= 0)
-#pragma unroll
-    for (int q = 0; q < Part::NF; ++q) pay[q][slot] = me.f[q];
+  int seg, slot, cell;
+  block_bins<W>(S, valid, row, base, bx, seg, slot, cell);
+  if (slot >= 0) {
+    T.pidx[slot] = i;
+    T.pcell[slot] = cell;
+  }
+  for (int t = threadIdx.x; t < bx.tot; t += blockDim.x) T.tile[t] = make_float4(0.f, 0.f, 0.f, 0.f);
   __syncthreads();
-  const int w = xcd_tile();
-  tile_sums<W>(m, S, bx, o.tiles + (size_t)w * MPM_TILE,
-               [&](int x, int y, int z, int ky, int kz, int q0, int q1, float4& acc) {
-    for (int q = q0; q < q1; ++q) {
-      const int kx = x - __float_as_int(pay[Part::BX][q]);
-      const float4 c = p2g_contrib<W>(m, apic, [&](int f) { return pay[f][q]; }, x, y, z, kx, ky, kz);
-      acc.x += c.x; acc.y += c.y; acc.z += c.z; acc.w += c.w;
+  // the particle of this thread's slot (or, for a box not tiled, its own)
+  const SlotRef r = slot_ref(S, T, bx);
+  const bool direct = valid && !bx.tiled(seg);
+  const int j = r.act ? r.j : i;
+  float4 p = make_float4(0.f, 0.f, 0.f, 0.f), v = p;
+  float mp = 0.f, Bi[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (r.act || direct) {
+    p = pos[j];
+    v = vel[j];
+    mp = phys[j].x;
+    if (apic)
+#pragma unroll
+      for (int q = 0; q < 9; ++q) Bi[q] = B[(size_t)q * cap + j] * m.iDp;
+  }
+  Stencil<W> s;
+  make_stencil<W>(m, p, s);
+  auto contrib = [&](int a, int b, int c) {
+    float wt = s.w[0][a] * s.w[1][b] * s.w[2][c];
+    float vb[3] = {v.x, v.y, v.z};
+    if (apic) {
+      float xi = m.xmin.x + (float)(s.base[0] + a) * m.dx;
+      float yi = m.xmin.y + (float)(s.base[1] + b) * m.dx;
+      float zi = m.xmin.z + (float)(s.base[2] + c) * m.dx;
+      float d0 = (xi - p.x) * m.ss, d1 = (yi - p.y) * m.ss, d2 = (zi - p.z) * m.ss;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) vb[q] += Bi[3 * q] * d0 + Bi[3 * q + 1] * d1 + Bi[3 * q + 2] * d2;
     }
-  });
-  if (valid && !bx.tiled(seg))
-    direct_add<W>(m, o, base, [&](int x, int y, int z, int kx, int ky, int kz) {
-      return p2g_contrib<W>(m, apic, [&](int f) { return me.f[f]; }, x, y, z, kx, ky, kz);
-    });
+    return make_float4(vb[0] * mp * wt, vb[1] * mp * wt, vb[2] * mp * wt, wt * mp);
+  };
+  phased_tile<W>(T, r, contrib);
+  const int w = xcd_tile();
+  float4* dst = o.tiles + (size_t)w * MPM_TILE;
+  for (int t = threadIdx.x; t < bx.tot; t += blockDim.x) dst[t] = T.tile[t];
+  if (direct) {
+    // this thread's own particle (slot threads above reloaded theirs)
+    if (r.act) {
+      p = pos[i];
+      v = vel[i];
+      mp = phys[i].x;
+      if (apic)
+#pragma unroll
+        for (int q = 0; q < 9; ++q) Bi[q] = B[(size_t)q * cap + i] * m.iDp;
+      make_stencil<W>(m, p, s);
+    }
+    direct_add<W>(m, o, s.base, contrib);
+  }
   tile_finish<W>(m, o, w, bx);
 }
 
@@ -855,44 +829,63 @@ __global__ __launch_bounds__(MPM_THREADS) void k_mpm_force(
     MP m, const uint32_t* __restrict__ key, const float4* __restrict__ pos,
     const float4* __restrict__ phys, const float* __restrict__ S9, int cap, int sort_resx,
     TileOut o) {
-  using Part = ForcePart<W>;
   __shared__ BinShared S;
-  __shared__ float pay[Part::NF][MPM_THREADS];   // the binned particles, field-major
+  __shared__ PhaseShared T;
   const int i = xcd_index();
   bool valid = false;
   int row = 0, base[3] = {0, 0, 0};
-  Part me;
   if (i < m.n) {
     const float4 p = pos[i];
     const uint32_t k = key[i];
     valid = type_of(p) != SPHX_TYPE_BOUNDARY && k != m.sentinel;
     row = (int)(k / (uint32_t)sort_resx);
-    const float4 q = phys[i];
-    float sig[9];
-#pragma unroll
-    for (int k9 = 0; k9 < 9; ++k9) sig[k9] = S9[(size_t)k9 * cap + i];
-    force_part<W>(m, p, m.physical ? q.x / (q.y * m.rc) : 1.f / q.y, sig, me, base);
+    stencil_base<W>(m, p, base);
   }
   BinBox bx;
-  int seg, slot;
-  block_bins<W>(S, valid, row, base, bx, seg, slot);
-  if (slot >= 0)
-#pragma unroll
-    for (int q = 0; q < Part::NF; ++q) pay[q][slot] = me.f[q];
+  int seg, slot, cell;
+  block_bins<W>(S, valid, row, base, bx, seg, slot, cell);
+  if (slot >= 0) {
+    T.pidx[slot] = i;
+    T.pcell[slot] = cell;
+  }
+  for (int t = threadIdx.x; t < bx.tot; t += blockDim.x) T.tile[t] = make_float4(0.f, 0.f, 0.f, 0.f);
   __syncthreads();
+  const SlotRef r = slot_ref(S, T, bx);
+  const bool direct = valid && !bx.tiled(seg);
+  float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+  float sig[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, scale = 0.f;
+  auto load = [&](int j) {
+    p = pos[j];
+    const float4 q = phys[j];
+    scale = m.physical ? q.x / (q.y * m.rc) : 1.f / q.y;
+#pragma unroll
+    for (int k9 = 0; k9 < 9; ++k9) sig[k9] = S9[(size_t)k9 * cap + j];
+  };
+  if (r.act) load(r.j);
+  else if (direct) load(i);
+  Stencil<W> s;
+  make_stencil<W>(m, p, s);
+  // the reference differentiates at (x_I - x_p)/dx: the stencil's
+  // derivative (at (x_p - x_I)/dx) with the sign flipped
+  auto contrib = [&](int a, int b, int c) {
+    float n0 = s.w[0][a], n1 = s.w[1][b], n2 = s.w[2][c];
+    float nw0 = -s.dw[0][a] * n1 * n2, nw1 = -s.dw[1][b] * n0 * n2, nw2 = -s.dw[2][c] * n0 * n1;
+    float f0 = sig[0] * nw0 + sig[1] * nw1 + sig[2] * nw2;
+    float f1 = sig[3] * nw0 + sig[4] * nw1 + sig[5] * nw2;
+    float f2 = sig[6] * nw0 + sig[7] * nw1 + sig[8] * nw2;
+    return make_float4(f0 * scale, f1 * scale, f2 * scale, 0.f);
+  };
+  phased_tile<W>(T, r, contrib);
   const int w = xcd_tile();
-  tile_sums<W>(m, S, bx, o.tiles + (size_t)w * MPM_TILE,
-               [&](int x, int y, int z, int ky, int kz, int q0, int q1, float4& acc) {
-    for (int q = q0; q < q1; ++q) {
-      const int kx = x - __float_as_int(pay[Part::BX][q]);
-      const float4 c = force_contrib<W>([&](int f) { return pay[f][q]; }, kx, ky, kz);
-      acc.x += c.x; acc.y += c.y; acc.z += c.z;
+  float4* dst = o.tiles + (size_t)w * MPM_TILE;
+  for (int t = threadIdx.x; t < bx.tot; t += blockDim.x) dst[t] = T.tile[t];
+  if (direct) {
+    if (r.act) {
+      load(i);
+      make_stencil<W>(m, p, s);
     }
-  });
-  if (valid && !bx.tiled(seg))
-    direct_add<W>(m, o, base, [&](int x, int y, int z, int kx, int ky, int kz) {
-      return force_contrib<W>([&](int f) { return me.f[f]; }, kx, ky, kz);
-    });
+    direct_add<W>(m, o, s.base, contrib);
+  }
   tile_finish<W>(m, o, w, bx);
 }
 
