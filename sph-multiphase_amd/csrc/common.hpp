@@ -119,12 +119,12 @@ constexpr int SPHX_MAX_SCENES = 256;
 // Workgroup size of each neighbour-loop kernel class, set by the autotuner
 // (multiples of 64 up to 256; the kernels do not depend on it otherwise).
 enum TuneId {
-  T_DENSITY, T_DII, T_AII, T_DP, T_ITER, T_NBR, T_MF_DRIFT, T_MF_ALPHA, T_MF_FORCE,
+  T_DENSITY, T_DII, T_AII, T_DP, T_ITER, T_MF_DRIFT, T_MF_ALPHA, T_MF_FORCE,
   T_MLP_FEAT, T_MLP_INFER, T_COUNT
 };
 // timer names of the kernels, in TuneId order
 constexpr const char* TUNE_NAMES[T_COUNT] = {
-    "iisph_density", "iisph_dii", "iisph_aii", "iisph_dp", "iisph_iter", "nbr_build",
+    "iisph_density", "iisph_dii", "iisph_aii", "iisph_dp", "iisph_iter",
     "mf_drift", "mf_alpha", "mf_force", "mlp_features", "mlp_infer"};
 
 // Per-scene solver state of a batched ensemble (device arrays).
@@ -229,6 +229,10 @@ struct sphx_context {
   float mf_rho[3] = {0, 0, 0}, mf_nu[3] = {0, 0, 0};
   sphx::DevBuf<float4> mf_alpha[2], mf_force[2];  // permuted with particles
   sphx::DevBuf<float4> mf_alpha_new, mf_drift[3], mf_amf;
+  // packed neighbour records of the drift-flux loops ({x, y, z, m / rho},
+  // {alpha, p}); mf_rec_ok while positions and densities are unchanged
+  sphx::DevBuf<float4> mf_rec, mf_reca;
+  bool mf_rec_ok = false;
   bool mf_pending = false;        // alpha_new holds the corrected fractions
 
   // MPM transfer (particle-centric scatter into per-workgroup node tiles,
@@ -292,7 +296,7 @@ struct sphx_context {
   int graph_budget = 0, graph_num = -1;
 
   // workgroup sizes per kernel class (TuneId)
-  int tune[sphx::T_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256, 256, 256, 256};
+  int tune[sphx::T_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256, 256, 256};
 
   // timing
   bool timing = false;
@@ -317,5 +321,12 @@ void timer_end(sphx_context* c);
 int timer_flush(sphx_context* c);
 
 inline int blocks_for(int n, int threads) { return (n + threads - 1) / threads; }
+
+// Particle positions or order changed: the neighbour list and the packed
+// drift-flux records are stale.
+inline void positions_changed(sphx_context* c) {
+  c->nbr_valid = false;
+  c->mf_rec_ok = false;
+}
 
 }  // namespace sphx

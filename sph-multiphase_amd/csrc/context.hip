@@ -394,7 +394,7 @@ int sphx_upload(sphx_context* c, const sphx_displayPack* disp, const sphx_calcul
   SPHX_TRY(mpm_sync_cold(c));   // particles not in this upload keep their MPM state
   SPHX_TRY(materialize_cold(c));
   mpm_invalidate(c);
-  c->nbr_valid = false;
+  positions_changed(c);
   std::vector<float4> pos(count), vel(count), ve(count), acc(count), phys(count), color(count), X(count);
   std::vector<int> born(count);
   std::vector<float> mat((size_t)count * 36);

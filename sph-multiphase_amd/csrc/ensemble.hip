@@ -103,7 +103,7 @@ int sphx_graph_build(sphx_context* c, int budget) {
     if (rc) break;
     after_graph_a(c, p);
     rc = capture(c, [&] { return stage_b(c); }, &c->graph_b[1 - p]);
-    c->nbr_valid = false;
+    positions_changed(c);
   }
   c->timing = timing;
   if (rc) {
@@ -130,7 +130,7 @@ int sphx_graph_step(sphx_context* c, int* iters, double* err) {
   SPHX_TRY(continue_solve(c, &it, &e));
   SPHX_TRY(mlp_fallback(c, &it, &e));
   SPHX_HIP(hipGraphLaunch(c->graph_b[1 - p], c->stream));
-  c->nbr_valid = false;
+  positions_changed(c);
   if (iters) *iters = it;
   if (err) *err = e;
   return 0;

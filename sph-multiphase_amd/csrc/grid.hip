@@ -299,7 +299,7 @@ int mpm_reorder(sphx_context* c, int a, int b);
 int grid_hash(sphx_context* c) {
   int n = c->num;
   c->sorted_valid = false;
-  c->nbr_valid = false;
+  positions_changed(c);
   if (n == 0) return 0;
   KP k = make_kp(c);
   timer_begin(c, "hash");
@@ -365,7 +365,7 @@ int grid_reorder(sphx_context* c) {
   if (n == 0) return 0;
   SPHX_CHECK(c->sorted_valid, "sphx_grid_reorder", "grid not sorted (call hash and sort first)");
   int a = c->cur, b = 1 - c->cur;
-  c->nbr_valid = false;
+  positions_changed(c);
   k_fill_i32<<<blocks_for(c->cap, 256), 256, 0, c->stream>>>(c->mfid.p, c->cap, -1);
   timer_begin(c, "reorder");
   k_reorder<<<blocks_for(n, 256), 256, 0, c->stream>>>(

@@ -448,31 +448,140 @@ __global__ void k_ii_integrate(KP k, const uint32_t* __restrict__ key, float4* _
 // Also seeds ps[i] = {x, y, z, 0}: the pressure pass gathers position and
 // s_j = m_j p_j / rho_j^2 as one record (s stays 0 for boundary particles,
 // whose pressure is not an unknown).
-__global__ void k_nbr_build(KP k, const uint32_t* __restrict__ key, const int* __restrict__ cs,
-                            const float4* __restrict__ pos, int* __restrict__ idx,
-                            int* __restrict__ cnt, float4* __restrict__ ps, int cap) {
-  int i = xcd_index();
-  if (i >= k.n) return;
-  float4 pi = pos[i];
-  ps[i] = make_float4(pi.x, pi.y, pi.z, 0.f);
-  uint32_t gc = key[i];
-  if (type_of(pi) == SPHX_TYPE_BOUNDARY || (int)gc == k.gridtotal) {
-    cnt[i] = 0;
-    return;
+constexpr int NB_STAGE = 3072;   // candidate positions staged per workgroup (48 KB)
+constexpr int NB_THREADS_BUILD = 256;
+
+__device__ __forceinline__ int nb_wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int nb_wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Neighbour list of the step (every j != i with |x_ij| < 2h, in the
+// reference's visiting order).  A workgroup's 256 sorted particles lie in
+// one or two (y, z) cell rows; for each such row the candidates of all its
+// particles are its 9 neighbour rows over the row's x-range plus one cell,
+// each one contiguous sorted range.  Those ranges are staged in LDS with
+// coalesced loads and every thread tests its own 9 sub-ranges there.  A
+// workgroup over three or more rows, or with more candidates than the
+// stage holds, tests straight from global memory.
+__global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
+    KP k, const uint32_t* __restrict__ key, const int* __restrict__ cs,
+    const float4* __restrict__ pos, int* __restrict__ idx, int* __restrict__ cnt,
+    float4* __restrict__ ps, int cap) {
+  __shared__ float4 spos[NB_STAGE];
+  __shared__ int sh[7];           // row r0, row r1, key min/max of r0, of r1, bad
+  __shared__ int rs[19], ro[19];  // range (row, neighbour row) begin; LDS offsets
+  const int t = threadIdx.x, lane = t & 63;
+  const int i = xcd_index();
+  bool valid = false;
+  float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t gc = 0;
+  int row = 0;
+  if (i < k.n) {
+    pi = pos[i];
+    ps[i] = make_float4(pi.x, pi.y, pi.z, 0.f);
+    gc = key[i];
+    valid = type_of(pi) != SPHX_TYPE_BOUNDARY && (int)gc != k.gridtotal;
+    row = (int)(gc / (uint32_t)k.gridres.x);
   }
+  if (t < 7) sh[t] = (t == 0 || t == 2 || t == 4) ? INT_MAX : (t == 6 ? 0 : INT_MIN);
+  __syncthreads();
+  {
+    const int mn = nb_wave_min(valid ? row : INT_MAX), mx = nb_wave_max(valid ? row : INT_MIN);
+    if (lane == 0 && mn != INT_MAX) {
+      atomicMin(&sh[0], mn);
+      atomicMax(&sh[1], mx);
+    }
+  }
+  __syncthreads();
+  const int r0 = sh[0], r1 = sh[1];
+  const int R = row == r0 ? 0 : 1;
+  {
+    const int g = (int)gc;
+    const int mn0 = nb_wave_min(valid && R == 0 ? g : INT_MAX);
+    const int mx0 = nb_wave_max(valid && R == 0 ? g : INT_MIN);
+    const int mn1 = nb_wave_min(valid && R == 1 ? g : INT_MAX);
+    const int mx1 = nb_wave_max(valid && R == 1 ? g : INT_MIN);
+    const bool bad = __ballot(valid && row != r0 && row != r1) != 0ull;
+    if (lane == 0) {
+      if (mn0 != INT_MAX) { atomicMin(&sh[2], mn0); atomicMax(&sh[3], mx0); }
+      if (mn1 != INT_MAX) { atomicMin(&sh[4], mn1); atomicMax(&sh[5], mx1); }
+      if (bad) sh[6] = 1;
+    }
+  }
+  __syncthreads();
+  if (t < 18) {
+    const int g = t / 9, r = t % 9;
+    const int kmin = sh[2 + 2 * g], kmax = sh[3 + 2 * g];
+    int b = 0, e = 0;
+    if (kmin != INT_MAX) {
+      b = cs[kmin + k.rowoff[r]];
+      e = cs[kmax + k.rowoff[r] + 3];
+    }
+    rs[t] = b;
+    ro[t] = e - b;   // count for now
+  }
+  __syncthreads();
+  if (t == 0) {
+    int run = 0;
+    for (int q = 0; q < 18; ++q) {
+      const int c = ro[q];
+      ro[q] = run;
+      run += c;
+    }
+    ro[18] = run;
+  }
+  __syncthreads();
+  const int total = ro[18];
+  const bool staged = sh[6] == 0 && total <= NB_STAGE;
   const float rc2 = 4.f * k.h2;
   int* __restrict__ out = idx + nbr_slot(i, 0, cap);
   int c = 0;
-  for_each_neighbor(k, cs, gc, [&](int j) {
-    float4 pj = pos[j];
-    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
-    float dsq = dx * dx + dy * dy + dz * dz;
-    if (dsq < rc2 && j != i) {
-      if (c < cap) out[(size_t)c * NBR_SLICE] = j;
-      ++c;
+  if (staged) {
+    for (int u = t; u < total; u += blockDim.x) {
+      int q = 0;
+      while (u >= ro[q + 1]) ++q;
+      spos[u] = pos[rs[q] + (u - ro[q])];
     }
-  });
-  cnt[i] = c;
+    __syncthreads();
+    if (valid) {
+      const int* rsR = rs + 9 * R;
+      const int* roR = ro + 9 * R;
+#pragma unroll 1
+      for (int r = 0; r < 9; ++r) {
+        const int c0 = (int)gc + k.rowoff[r];
+        const int jb = cs[c0], je = cs[c0 + 3];
+        const int base = roR[r] - rsR[r];
+#pragma unroll 4
+        for (int j = jb; j < je; ++j) {
+          const float4 pj = spos[base + j];
+          float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+          float dsq = dx * dx + dy * dy + dz * dz;
+          if (dsq < rc2 && j != i) {
+            if (c < cap) out[(size_t)c * NBR_SLICE] = j;
+            ++c;
+          }
+        }
+      }
+    }
+  } else if (valid) {
+    for_each_neighbor(k, cs, gc, [&](int j) {
+      float4 pj = pos[j];
+      float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+      float dsq = dx * dx + dy * dy + dz * dz;
+      if (dsq < rc2 && j != i) {
+        if (c < cap) out[(size_t)c * NBR_SLICE] = j;
+        ++c;
+      }
+    });
+  }
+  if (i < k.n) cnt[i] = c;
 }
 
 int grid_hash(sphx_context* c);
@@ -510,8 +619,7 @@ int nbr_list(sphx_context* c, NbrList* out) {
     if (n > 0) {
       KP k = make_kp(c);
       timer_begin(c, "nbr_build");
-      const int tb = c->tune[T_NBR];
-      k_nbr_build<<<blocks_for(n, tb), tb, 0, c->stream>>>(
+      k_nbr_build<<<blocks_for(n, NB_THREADS_BUILD), NB_THREADS_BUILD, 0, c->stream>>>(
           k, c->key.p, c->cell_start.p, c->POS(), c->nbr_idx.p, c->nbr_cnt.p, c->ps.p, cap);
       timer_end(c);
       SPHX_HIP(hipGetLastError());
@@ -588,6 +696,7 @@ int sphx_iisph_density(sphx_context* c) {
   SPHX_TRY(need_sorted(c, "sphx_iisph_density"));
   KP k = make_kp(c);
   if (k.n == 0) return 0;
+  c->mf_rec_ok = false;   // densities change
   NbrList L;
   SPHX_TRY(nbr_list(c, &L));
   timer_begin(c, "iisph_density");
@@ -723,7 +832,7 @@ int sphx_iisph_integrate(sphx_context* c) {
   SPHX_TRY(need_sorted(c, "sphx_iisph_integrate"));
   KP k = make_kp(c);
   if (k.n == 0) return 0;
-  c->nbr_valid = false;
+  positions_changed(c);
   timer_begin(c, "iisph_integrate");
   k_ii_integrate<<<blocks_for(k.n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
       k, c->key.p, c->POS(), c->VEL(), c->VELEVAL(), c->ACCEL(),

@@ -1266,7 +1266,7 @@ int sphx_mpm_g2p(sphx_context* c, int apic) {
   auto& M = c->mpm;
   MP m = make_mp(c);
   if (m.n == 0) return 0;
-  c->nbr_valid = false;
+  positions_changed(c);
   timer_begin(c, "mpm_g2p");
   if (M.kernel == 0)
     k_mpm_g2p<4><<<blocks_for(m.n, MPM_THREADS), MPM_THREADS, 0, c->stream>>>(

@@ -36,12 +36,30 @@ __device__ __forceinline__ float comp(const float4& a, int k) {
   return k == 0 ? a.x : (k == 1 ? a.y : a.z);
 }
 
+// Packed neighbour records of the drift-flux loops: rec = {x, y, z, V =
+// m / rho} (V = -1 marks a boundary particle, which those loops skip) and
+// reca = {alpha_0, alpha_1, alpha_2, p}.  One 16-byte gather per record
+// instead of pos + phys (+ alpha).
+__global__ void k_mf_pack(int n, const float4* __restrict__ pos, const float4* __restrict__ phys,
+                          const float4* __restrict__ alpha, float4* __restrict__ rec,
+                          float4* __restrict__ reca) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = pos[i], q = phys[i];
+  rec[i] = make_float4(p.x, p.y, p.z, type_of(p) == SPHX_TYPE_BOUNDARY ? -1.f : q.x / q.y);
+  if (reca) {
+    const float4 a = alpha[i];
+    reca[i] = make_float4(a.x, a.y, a.z, q.z);
+  }
+}
+
 // mfComputeDriftVel (multiphase.cu:19-166)
 __global__ __launch_bounds__(MF_THREADS) void k_mf_drift(
     KP k, MFP m, NbrList L, const uint32_t* __restrict__ key, const int* __restrict__ cs,
     const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float4* __restrict__ phys, const float4* __restrict__ alpha,
     const float4* __restrict__ mforce, const float4* __restrict__ dcorr,
+    const float4* __restrict__ rec, const float4* __restrict__ reca,
     float4* __restrict__ drift0, float4* __restrict__ drift1, float4* __restrict__ drift2) {
   int i = xcd_index();
   if (i >= k.n) return;
@@ -62,20 +80,19 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_drift(
     ag[f] = f3(0.f, 0.f, 0.f);
   }
   for_each_listed(k, L, cs, gc, i, [&](int j) {
-    float4 pj = pos[j];
-    if (type_of(pj) == SPHX_TYPE_BOUNDARY) return;
-    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+    const float4 rj = rec[j];
+    if (rj.w < 0.f) return;                                    // boundary
+    float dx = (pi.x - rj.x) * k.ss, dy = (pi.y - rj.y) * k.ss, dz = (pi.z - rj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
     if (!(dsq < 4.f * k.h2 && dsq > 0.f)) return;
     float r = sqrtf(dsq);
-    float4 qj = phys[j];
-    float4 aj = alpha[j];
-    float cm = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline) * qj.x / qj.y;   // V_j f(r)
+    const float4 aj = reca[j];                                 // alpha_j, p_j
+    float cm = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline) * rj.w;   // V_j f(r)
     float pg[3];
     float3 pgs = f3(0.f, 0.f, 0.f);
     for (int f = 0; f < m.nphase; ++f) {
       float ajf = comp(aj, f);
-      pg[f] = cm * (-apre[f] * ipress + ajf * qj.z);           // :62
+      pg[f] = cm * (-apre[f] * ipress + ajf * aj.w);           // :62
       pgs.x += pg[f] * dx * conc[f];
       pgs.y += pg[f] * dy * conc[f];
       pgs.z += pg[f] * dz * conc[f];
@@ -116,7 +133,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_drift(
     u.z += acc.z * tdiff - m.cdiff * m.sigma * (st[f].z - ssum.z);
     float cf = f == 1 ? 1.f : (f == 0 ? dc0 : 0.f);
     u = add(u, mul(dc, cf));
-    out[f][i] = make_float4(u.x, u.y, u.z, 0.f);
+    out[f][i] = make_float4(u.x, u.y, u.z, apre[f]);   // .w: alpha_f for the next loops
   }
 }
 
@@ -127,7 +144,8 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_alpha(
     const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float4* __restrict__ phys, const float4* __restrict__ alpha,
     const float4* __restrict__ drift0, const float4* __restrict__ drift1,
-    const float4* __restrict__ drift2, float4* __restrict__ alpha_new) {
+    const float4* __restrict__ drift2, const float4* __restrict__ rec,
+    float4* __restrict__ alpha_new) {
   int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
@@ -147,21 +165,19 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_alpha(
     ui[f] = f < m.nphase ? xyz(dr[f][i]) : f3(0.f, 0.f, 0.f);
   }
   for_each_listed(k, L, cs, gc, i, [&](int j) {
-    float4 pj = pos[j];
-    if (type_of(pj) == SPHX_TYPE_BOUNDARY) return;
-    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+    const float4 rj = rec[j];
+    if (rj.w < 0.f) return;                                              // boundary
+    float dx = (pi.x - rj.x) * k.ss, dy = (pi.y - rj.y) * k.ss, dz = (pi.z - rj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
     if (!(dsq < 4.f * k.h2 && dsq > 0.f)) return;
     float r = sqrtf(dsq);
-    float4 qj = phys[j];
     float4 vj = vel[j];
-    float4 aj = alpha[j];
-    float cm = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline) * qj.x / qj.y;
+    float cm = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline) * rj.w;
     float vmrd = (vj.x - vi.x) * dx + (vj.y - vi.y) * dy + (vj.z - vi.z) * dz;
     for (int f = 0; f < m.nphase; ++f) {
-      float ajf = comp(aj, f);
+      float4 uj = dr[f][j];                                              // u_f, alpha_f
+      float ajf = uj.w;
       ach[f] -= 0.5f * cm * (ajf + apre[f]) * vmrd;                        // :207
-      float4 uj = dr[f][j];
       float vkx = ajf * uj.x + apre[f] * ui[f].x;
       float vky = ajf * uj.y + apre[f] * ui[f].y;
       float vkz = ajf * uj.z + apre[f] * ui[f].z;
@@ -221,7 +237,8 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_force(
     const float4* __restrict__ pos, const float4* __restrict__ phys,
     const float4* __restrict__ alpha, const float4* __restrict__ alpha_new,
     const float4* __restrict__ drift0, const float4* __restrict__ drift1,
-    const float4* __restrict__ drift2, float4* __restrict__ veleval, float4* __restrict__ amf) {
+    const float4* __restrict__ drift2, const float4* __restrict__ rec,
+    float4* __restrict__ veleval, float4* __restrict__ amf) {
   int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
@@ -239,18 +256,16 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_force(
   }
   float fx = 0.f, fy = 0.f, fz = 0.f;
   for_each_listed(k, L, cs, gc, i, [&](int j) {
-    float4 pj = pos[j];
-    if (type_of(pj) == SPHX_TYPE_BOUNDARY) return;
-    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+    const float4 rj = rec[j];
+    if (rj.w < 0.f) return;                                              // boundary
+    float dx = (pi.x - rj.x) * k.ss, dy = (pi.y - rj.y) * k.ss, dz = (pi.z - rj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
     if (!(dsq < 4.f * k.h2 && dsq > 0.f)) return;
     float r = sqrtf(dsq);
-    float4 qj = phys[j];
-    float4 aj = alpha[j];
-    float cm = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline) * qj.x / qj.y;
+    float cm = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline) * rj.w;
     for (int f = 0; f < m.nphase; ++f) {
-      float4 uj = dr[f][j];
-      float sj = cm * (uj.x * dx + uj.y * dy + uj.z * dz) * comp(aj, f);
+      float4 uj = dr[f][j];                                              // u_f, alpha_f
+      float sj = cm * (uj.x * dx + uj.y * dy + uj.z * dz) * uj.w;
       float si = cm * (ui[f].x * dx + ui[f].y * dy + ui[f].z * dz) * apre[f];
       float w = m.rho[f] * idens;
       fx += (sj * uj.x + si * ui[f].x) * w;
@@ -336,10 +351,27 @@ int mf_drift_into(sphx_context* c, const float4* dcorr, float4* out0, float4* ou
   MFP m = make_mfp(c);
   NbrList L;
   SPHX_TRY(nbr_list(c, &L));
+  // records with this call's pressures and fractions (also when the
+  // recorder re-evaluates the drift after the solve)
+  k_mf_pack<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(k.n, c->POS(), c->PHYS(),
+                                                          c->mf_alpha[c->cur].p, c->mf_rec.p,
+                                                          c->mf_reca.p);
+  c->mf_rec_ok = true;
   k_mf_drift<<<blocks_for(k.n, c->tune[T_MF_DRIFT]), c->tune[T_MF_DRIFT], 0, c->stream>>>(
       k, m, L, c->key.p, c->cell_start.p, c->POS(), c->VEL(), c->PHYS(), c->mf_alpha[c->cur].p,
-      c->mf_force[c->cur].p, dcorr, out0, out1, out2);
+      c->mf_force[c->cur].p, dcorr, c->mf_rec.p, c->mf_reca.p, out0, out1, out2);
   SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+// {x, y, z, m / rho} for the loops after the drift pass, when a caller ran
+// them without it since the positions or densities last changed.
+static int mf_need_rec(sphx_context* c) {
+  if (c->mf_rec_ok || c->num == 0) return 0;
+  k_mf_pack<<<blocks_for(c->num, 256), 256, 0, c->stream>>>(c->num, c->POS(), c->PHYS(), nullptr,
+                                                             c->mf_rec.p, nullptr);
+  SPHX_HIP(hipGetLastError());
+  c->mf_rec_ok = true;
   return 0;
 }
 
@@ -370,6 +402,9 @@ int sphx_mf_setup(sphx_context* c, int nphase, const float* rho, const float* nu
   SPHX_TRY(c->mf_alpha_new.alloc(cap));
   SPHX_TRY(c->mf_amf.alloc(cap));
   for (int f = 0; f < 3; ++f) SPHX_TRY(c->mf_drift[f].alloc(cap));
+  SPHX_TRY(c->mf_rec.alloc(cap));
+  SPHX_TRY(c->mf_reca.alloc(cap));
+  c->mf_rec_ok = false;
   for (int b = 0; b < 2; ++b) {
     SPHX_HIP(hipMemsetAsync(c->mf_alpha[b].p, 0, cap * sizeof(float4), c->stream));
     SPHX_HIP(hipMemsetAsync(c->mf_force[b].p, 0, cap * sizeof(float4), c->stream));
@@ -478,10 +513,11 @@ int sphx_mf_alpha_advance(sphx_context* c) {
   MFP m = make_mfp(c);
   NbrList L;
   SPHX_TRY(nbr_list(c, &L));
+  SPHX_TRY(mf_need_rec(c));
   timer_begin(c, "mf_alpha");
   k_mf_alpha<<<blocks_for(k.n, c->tune[T_MF_ALPHA]), c->tune[T_MF_ALPHA], 0, c->stream>>>(
       k, m, L, c->key.p, c->cell_start.p, c->POS(), c->VEL(), c->PHYS(), c->mf_alpha[c->cur].p,
-      c->mf_drift[0].p, c->mf_drift[1].p, c->mf_drift[2].p, c->mf_alpha_new.p);
+      c->mf_drift[0].p, c->mf_drift[1].p, c->mf_drift[2].p, c->mf_rec.p, c->mf_alpha_new.p);
   timer_end(c);
   SPHX_HIP(hipGetLastError());
   return 0;
@@ -511,10 +547,11 @@ int sphx_mf_force(sphx_context* c) {
   NbrList L;
   SPHX_TRY(nbr_list(c, &L));
   const float4* an = c->mf_pending ? c->mf_alpha_new.p : c->mf_alpha[c->cur].p;
+  SPHX_TRY(mf_need_rec(c));
   timer_begin(c, "mf_force");
   k_mf_force<<<blocks_for(k.n, c->tune[T_MF_FORCE]), c->tune[T_MF_FORCE], 0, c->stream>>>(
       k, m, L, c->key.p, c->cell_start.p, c->POS(), c->PHYS(), c->mf_alpha[c->cur].p, an,
-      c->mf_drift[0].p, c->mf_drift[1].p, c->mf_drift[2].p, c->VELEVAL(), c->mf_amf.p);
+      c->mf_drift[0].p, c->mf_drift[1].p, c->mf_drift[2].p, c->mf_rec.p, c->VELEVAL(), c->mf_amf.p);
   timer_end(c);
   SPHX_HIP(hipGetLastError());
   return 0;

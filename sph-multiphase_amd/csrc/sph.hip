@@ -239,7 +239,7 @@ int sphx_wcsph_advance(sphx_context* c) {
   SPHX_TRY(need_sorted(c, "sphx_wcsph_advance"));
   KP k = make_kp(c);
   if (k.n == 0) return 0;
-  c->nbr_valid = false;
+  positions_changed(c);
   timer_begin(c, "wcsph_advance");
   k_advance<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(k, c->key.p, c->POS(), c->VEL(),
                                                           c->VELEVAL(), c->ACCEL());

@@ -522,7 +522,7 @@ class Context:
         return it.value, err.value
 
     TUNE_KERNELS = ("iisph_density", "iisph_dii", "iisph_aii", "iisph_dp", "iisph_iter",
-                    "nbr_build", "mf_drift", "mf_alpha", "mf_force", "mlp_features", "mlp_infer")
+                    "mf_drift", "mf_alpha", "mf_force", "mlp_features", "mlp_infer")
 
     def tune_set(self, kernel, block):
         check(self.L.sphx_tune_set(self.h, kernel.encode(), block), "sphx_tune_set")
