@@ -48,14 +48,17 @@ __global__ void k_ii_density(KP k, NbrList L, const uint32_t* __restrict__ key, 
   const float rest = qi.w;
   // self term (the reference's scan visits i itself, W(0) = kspline)
   float sum = k.kspline * qi.x;
-  for_each_listed(k, L, cs, gc, i, [&](int j) {
-    float4 pj = pos[j];
-    float4 qj = phys[j];
+  struct Rec { float4 p, q; };
+  for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j]}; },
+                    [&](int, const Rec& o) {
+    const float4 pj = o.p, qj = o.q;
     float mj = (type_of(pj) == SPHX_TYPE_BOUNDARY) ? rest * qj.y : qj.x;
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
     if (dsq >= 4.f * k.h2) return;
-    sum += cubic_w(sqrtf(dsq), k.inv_h, k.kspline) * mj;
+    float r, ir;
+    dist_rinv(dsq, r, ir);
+    sum += cubic_w(r, k.inv_h, k.kspline) * mj;
   });
   phys[i].y = sum;
 }
@@ -77,15 +80,17 @@ __global__ void k_ii_dii(KP k, NbrList L, const uint32_t* __restrict__ key, cons
   const float rest = qi.w;
   float ax = 0.f, ay = 0.f, az = 0.f;
   float dx_ = 0.f, dy_ = 0.f, dz_ = 0.f;
-  for_each_listed(k, L, cs, gc, i, [&](int j) {
-    float4 pj = pos[j];
+  struct Rec { float4 p, q; };
+  for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j]}; },
+                    [&](int j, const Rec& o) {
+    const float4 pj = o.p, qj = o.q;
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
-    float r = sqrtf(dsq);
+    float r, ir;
+    dist_rinv(dsq, r, ir);
     if (!(r < 2.f * k.h && r > 0.f)) return;
-    float gf = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline);
+    float gf = cubic_gradf(r, ir, k.inv_h, k.kspline);
     int tj = type_of(pj);
-    float4 qj = phys[j];
     if (tj == SPHX_TYPE_BOUNDARY) {
       // artificial boundary viscosity (:712-723)
       float4 vj = vel[j];
@@ -131,17 +136,18 @@ __global__ void k_ii_aii(KP k, NbrList L, const uint32_t* __restrict__ key, cons
   const float4 di = dii[i];
   const float cji = mi / rhoi / rhoi * k.dt * k.dt;
   float dchange = 0.f, aii = 0.f;
-  for_each_listed(k, L, cs, gc, i, [&](int j) {
-    float4 pj = pos[j];
+  struct Rec { float4 p, q, v; };
+  for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j], veleval[j]}; },
+                    [&](int, const Rec& o) {
+    const float4 pj = o.p, qj = o.q, vj = o.v;
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
     if (!in_support(k, dsq)) return;
-    float r = sqrtf(dsq);
-    float gf = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline);
+    float r, ir;
+    dist_rinv(dsq, r, ir);
+    float gf = cubic_gradf(r, ir, k.inv_h, k.kspline);
     float gx = dx * gf, gy = dy * gf, gz = dz * gf;
     int tj = type_of(pj);
-    float4 qj = phys[j];
-    float4 vj = veleval[j];
     float vdg = (vi.x - vj.x) * gx + (vi.y - vj.y) * gy + (vi.z - vj.z) * gz;
     if (tj != SPHX_TYPE_BOUNDARY) {
       dchange += vdg * qj.x * k.dt;
@@ -178,13 +184,13 @@ __global__ void k_ii_dp(KP k, NbrList L, const int* __restrict__ done, const uin
   float sx = 0.f, sy = 0.f, sz = 0.f;
   // one 16-byte gather per neighbour: {x_j, s_j}; boundary s_j = 0 adds
   // nothing, as the reference's skip of boundary j (:925-926)
-  for_each_listed(k, L, cs, gc, i, [&](int j) {
-    float4 pj = ps[j];
+  for_each_listed_b(k, L, cs, gc, i, [&](int j) { return ps[j]; }, [&](int, const float4& pj) {
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
     if (!in_support(k, dsq)) return;
-    float r = sqrtf(dsq);
-    float gf = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline) * pj.w;
+    float r, ir;
+    dist_rinv(dsq, r, ir);
+    float gf = cubic_gradf(r, ir, k.inv_h, k.kspline) * pj.w;
     sx += dx * gf; sy += dy * gf; sz += dz * gf;
   });
   float f = -k.dt * k.dt;
@@ -247,16 +253,19 @@ __global__ __launch_bounds__(NB_THREADS) void k_ii_iter(
       const float pl = press_l[i];
       const float cji = k.dt * k.dt * mi / rhoi / rhoi * pl;
       float S = 0.f;
-      for_each_listed(k, L, cs, gc, i, [&](int j) {
-        float4 pj = pos[j];
+      struct Rec { float4 p, q; };
+      for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], qm[j]}; },
+                        [&](int j, const Rec& o) {
+        const float4 pj = o.p;
         float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
         float dsq = dx * dx + dy * dy + dz * dz;
         if (!in_support(k, dsq)) return;
-        float r = sqrtf(dsq);
-        float gf = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline);
+        float r, ir;
+        dist_rinv(dsq, r, ir);
+        float gf = cubic_gradf(r, ir, k.inv_h, k.kspline);
         float gx = dx * gf, gy = dy * gf, gz = dz * gf;
         if (type_of(pj) != SPHX_TYPE_BOUNDARY) {
-          float4 q = qm[j];
+          const float4 q = o.q;
           float tx = dpi.x - q.x + cji * gx, ty = dpi.y - q.y + cji * gy, tz = dpi.z - q.z + cji * gz;
           S += q.w * (tx * gx + ty * gy + tz * gz);
         } else if (k.boundary_pressure) {

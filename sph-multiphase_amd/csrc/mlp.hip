@@ -63,23 +63,26 @@ __global__ __launch_bounds__(256) void k_mlp_features(
   float wsum = 0.f, wp = 0.f, pmax = p_i, pmin = p_i, wsrc = 0.f, gpx = 0.f, gpy = 0.f,
         gpz = 0.f;
   int nf = 0, nb = 0;
-  for_each_listed(k, L, cs, gc, i, [&](int j) {
-    float4 pj = pos[j];
+  struct Rec { float4 p, q; float ra; };
+  for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j], rho_adv[j]}; },
+                    [&](int, const Rec& o) {
+    const float4 pj = o.p;
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
-    float r = sqrtf(dsq);
+    float r, ir;
+    dist_rinv(dsq, r, ir);
     if (!(r < 2.f * k.h && r > 0.f)) return;
     if (type_of(pj) == SPHX_TYPE_BOUNDARY) { ++nb; return; }
     ++nf;
-    float4 qj = phys[j];
+    const float4 qj = o.q;
     float w = cubic_w(r, k.inv_h, k.kspline);
     float pjp = fmaxf(0.f, qj.z);
     wsum += w;
     wp += w * pjp;
     pmax = fmaxf(pmax, pjp);
     pmin = fminf(pmin, pjp);
-    wsrc += w * (rho_adv[j] - qj.w) / qj.w;
-    float v = qj.x / qj.y * cubic_gradf(r, 1.f / r, k.inv_h, k.kspline) * (pjp - p_i);
+    wsrc += w * (o.ra - qj.w) / qj.w;
+    float v = qj.x / qj.y * cubic_gradf(r, ir, k.inv_h, k.kspline) * (pjp - p_i);
     gpx += v * dx; gpy += v * dy; gpz += v * dz;
   });
   const float P = mp.p_scale;

@@ -79,18 +79,23 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_drift(
     dv[f] = f3(0.f, 0.f, 0.f);
     ag[f] = f3(0.f, 0.f, 0.f);
   }
-  for_each_listed(k, L, cs, gc, i, [&](int j) {
-    const float4 rj = rec[j];
+  struct Rec { float4 r, a; };
+  for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{rec[j], reca[j]}; },
+                    [&](int, const Rec& o) {
+    const float4 rj = o.r;
     if (rj.w < 0.f) return;                                    // boundary
     float dx = (pi.x - rj.x) * k.ss, dy = (pi.y - rj.y) * k.ss, dz = (pi.z - rj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
     if (!(dsq < 4.f * k.h2 && dsq > 0.f)) return;
-    float r = sqrtf(dsq);
-    const float4 aj = reca[j];                                 // alpha_j, p_j
-    float cm = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline) * rj.w;   // V_j f(r)
+    float r, ir;
+    dist_rinv(dsq, r, ir);
+    const float4 aj = o.a;                                     // alpha_j, p_j
+    float cm = cubic_gradf(r, ir, k.inv_h, k.kspline) * rj.w;  // V_j f(r)
     float pg[3];
     float3 pgs = f3(0.f, 0.f, 0.f);
-    for (int f = 0; f < m.nphase; ++f) {
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      if (f >= m.nphase) break;
       float ajf = comp(aj, f);
       pg[f] = cm * (-apre[f] * ipress + ajf * aj.w);           // :62
       pgs.x += pg[f] * dx * conc[f];
@@ -99,7 +104,9 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_drift(
       float ga = (ajf - apre[f]) * cm;                         // :67
       ag[f].x += ga * dx; ag[f].y += ga * dy; ag[f].z += ga * dz;
     }
-    for (int f = 0; f < m.nphase; ++f) {                       // :70-73
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {                              // :70-73
+      if (f >= m.nphase) break;
       dv[f].x -= relax * (pg[f] * dx - pgs.x);
       dv[f].y -= relax * (pg[f] * dy - pgs.y);
       dv[f].z -= relax * (pg[f] * dz - pgs.z);
@@ -107,8 +114,10 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_drift(
   });
   // :138-163
   float dsum = 0.f;
-  float3 ssum = f3(0.f, 0.f, 0.f), st[3];
-  for (int f = 0; f < m.nphase; ++f) {
+  float3 ssum = f3(0.f, 0.f, 0.f), st[3] = {f3(0.f, 0.f, 0.f), f3(0.f, 0.f, 0.f), f3(0.f, 0.f, 0.f)};
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    if (f >= m.nphase) break;
     float t = apre[f];
     dsum += t * m.rho[f] * m.rho[f] * invdens;
     st[f] = t > 0.0001f ? mul(ag[f], 1.f / t) : f3(0.f, 0.f, 0.f);
@@ -125,7 +134,9 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_drift(
     dc0 = -conc[1] / conc[0];
   }
   float4* out[3] = {drift0, drift1, drift2};
-  for (int f = 0; f < m.nphase; ++f) {
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    if (f >= m.nphase) break;
     float tdiff = (m.rho[f] - dsum) * relax;
     float3 u = mul(dv[f], m.cpress);
     u.x += acc.x * tdiff - m.cdiff * m.sigma * (st[f].x - ssum.x);
@@ -164,18 +175,31 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_alpha(
     ach[f] = 0.f;
     ui[f] = f < m.nphase ? xyz(dr[f][i]) : f3(0.f, 0.f, 0.f);
   }
-  for_each_listed(k, L, cs, gc, i, [&](int j) {
-    const float4 rj = rec[j];
+  struct Rec { float4 r, v, u[3]; };
+  const bool three = m.nphase > 2;
+  for_each_listed_b(k, L, cs, gc, i, [&](int j) {
+    Rec o;
+    o.r = rec[j];
+    o.v = vel[j];
+    o.u[0] = drift0[j];
+    o.u[1] = drift1[j];
+    o.u[2] = three ? drift2[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    return o;
+  }, [&](int, const Rec& o) {
+    const float4 rj = o.r;
     if (rj.w < 0.f) return;                                              // boundary
     float dx = (pi.x - rj.x) * k.ss, dy = (pi.y - rj.y) * k.ss, dz = (pi.z - rj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
     if (!(dsq < 4.f * k.h2 && dsq > 0.f)) return;
-    float r = sqrtf(dsq);
-    float4 vj = vel[j];
-    float cm = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline) * rj.w;
+    float r, ir;
+    dist_rinv(dsq, r, ir);
+    const float4 vj = o.v;
+    float cm = cubic_gradf(r, ir, k.inv_h, k.kspline) * rj.w;
     float vmrd = (vj.x - vi.x) * dx + (vj.y - vi.y) * dy + (vj.z - vi.z) * dz;
-    for (int f = 0; f < m.nphase; ++f) {
-      float4 uj = dr[f][j];                                              // u_f, alpha_f
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      if (f >= m.nphase) break;
+      const float4 uj = o.u[f];                                          // u_f, alpha_f
       float ajf = uj.w;
       ach[f] -= 0.5f * cm * (ajf + apre[f]) * vmrd;                        // :207
       float vkx = ajf * uj.x + apre[f] * ui[f].x;
@@ -255,16 +279,28 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_force(
     apre[f] = f < m.nphase ? comp(ai, f) : 0.f;
   }
   float fx = 0.f, fy = 0.f, fz = 0.f;
-  for_each_listed(k, L, cs, gc, i, [&](int j) {
-    const float4 rj = rec[j];
+  struct Rec { float4 r, u[3]; };
+  const bool three = m.nphase > 2;
+  for_each_listed_b(k, L, cs, gc, i, [&](int j) {
+    Rec o;
+    o.r = rec[j];
+    o.u[0] = drift0[j];
+    o.u[1] = drift1[j];
+    o.u[2] = three ? drift2[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    return o;
+  }, [&](int, const Rec& o) {
+    const float4 rj = o.r;
     if (rj.w < 0.f) return;                                              // boundary
     float dx = (pi.x - rj.x) * k.ss, dy = (pi.y - rj.y) * k.ss, dz = (pi.z - rj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
     if (!(dsq < 4.f * k.h2 && dsq > 0.f)) return;
-    float r = sqrtf(dsq);
-    float cm = cubic_gradf(r, 1.f / r, k.inv_h, k.kspline) * rj.w;
-    for (int f = 0; f < m.nphase; ++f) {
-      float4 uj = dr[f][j];                                              // u_f, alpha_f
+    float r, ir;
+    dist_rinv(dsq, r, ir);
+    float cm = cubic_gradf(r, ir, k.inv_h, k.kspline) * rj.w;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      if (f >= m.nphase) break;
+      const float4 uj = o.u[f];                                          // u_f, alpha_f
       float sj = cm * (uj.x * dx + uj.y * dy + uj.z * dz) * uj.w;
       float si = cm * (ui[f].x * dx + ui[f].y * dy + ui[f].z * dz) * apre[f];
       float w = m.rho[f] * idens;

@@ -63,4 +63,44 @@ __device__ __forceinline__ void for_each_listed(const KP& k, const NbrList& L,
   }
 }
 
+// Batched traversal of the list: NBR_BATCH entries at a time, and the
+// records of all of them fetched (load(j)) before any is used (use(j, rec)),
+// so a batch's gathers are in flight together instead of one dependent
+// round trip per neighbour.  Entries past the count reload the last entry
+// (inside the particle's own list) and are not used; the visiting order,
+// and so every sum's order, is that of for_each_listed.
+constexpr int NBR_BATCH = 4;
+
+template <class Load, class Use>
+__device__ __forceinline__ void for_each_listed_b(const KP& k, const NbrList& L,
+                                                  const int* __restrict__ cell_start,
+                                                  uint32_t cell, int i, Load&& load, Use&& use) {
+  const int c = L.cnt[i];
+  if (c <= L.cap) {
+    const int* __restrict__ p = L.idx + nbr_slot(i, 0, L.cap);
+    using Rec = decltype(load(0));
+    for (int q0 = 0; q0 < c; q0 += NBR_BATCH) {
+      int j[NBR_BATCH];
+      Rec v[NBR_BATCH];
+#pragma unroll
+      for (int b = 0; b < NBR_BATCH; ++b) j[b] = p[(size_t)min(q0 + b, c - 1) * NBR_SLICE];
+#pragma unroll
+      for (int b = 0; b < NBR_BATCH; ++b) v[b] = load(j[b]);
+#pragma unroll
+      for (int b = 0; b < NBR_BATCH; ++b)
+        if (q0 + b < c) use(j[b], v[b]);
+    }
+  } else {
+    for_each_neighbor(k, cell_start, cell, [&](int j) {
+      if (j != i) use(j, load(j));
+    });
+  }
+}
+
+// r = |x| and 1/r from r^2 (v_rsq_f32, about 1 ulp; r = 0 for r^2 = 0).
+__device__ __forceinline__ void dist_rinv(float dsq, float& r, float& ir) {
+  ir = __builtin_amdgcn_rsqf(dsq);
+  r = dsq > 0.f ? dsq * ir : 0.f;
+}
+
 }  // namespace sphx
