@@ -49,8 +49,8 @@ __global__ void k_ii_density(KP k, NbrList L, const uint32_t* __restrict__ key, 
   // self term (the reference's scan visits i itself, W(0) = kspline)
   float sum = k.kspline * qi.x;
   struct Rec { float4 p, q; };
-  for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j]}; },
-                    [&](int, const Rec& o) {
+  for_each_listed_b<NBR_BATCH_SMALL>(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j]}; },
+                                     [&](int, const Rec& o) {
     const float4 pj = o.p, qj = o.q;
     float mj = (type_of(pj) == SPHX_TYPE_BOUNDARY) ? rest * qj.y : qj.x;
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
@@ -184,7 +184,8 @@ __global__ void k_ii_dp(KP k, NbrList L, const int* __restrict__ done, const uin
   float sx = 0.f, sy = 0.f, sz = 0.f;
   // one 16-byte gather per neighbour: {x_j, s_j}; boundary s_j = 0 adds
   // nothing, as the reference's skip of boundary j (:925-926)
-  for_each_listed_b(k, L, cs, gc, i, [&](int j) { return ps[j]; }, [&](int, const float4& pj) {
+  for_each_listed_b<NBR_BATCH_SMALL>(k, L, cs, gc, i, [&](int j) { return ps[j]; },
+                                     [&](int, const float4& pj) {
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
     if (!in_support(k, dsq)) return;

@@ -63,15 +63,16 @@ __device__ __forceinline__ void for_each_listed(const KP& k, const NbrList& L,
   }
 }
 
-// Batched traversal of the list: NBR_BATCH entries at a time, and the
+// Batched traversal of the list: B entries at a time, and the
 // records of all of them fetched (load(j)) before any is used (use(j, rec)),
 // so a batch's gathers are in flight together instead of one dependent
 // round trip per neighbour.  Entries past the count reload the last entry
 // (inside the particle's own list) and are not used; the visiting order,
 // and so every sum's order, is that of for_each_listed.
-constexpr int NBR_BATCH = 4;
+constexpr int NBR_BATCH = 8;    // default batch (wide records)
+constexpr int NBR_BATCH_SMALL = 16;   // loops with few live values (density, DP)
 
-template <class Load, class Use>
+template <int B = NBR_BATCH, class Load, class Use>
 __device__ __forceinline__ void for_each_listed_b(const KP& k, const NbrList& L,
                                                   const int* __restrict__ cell_start,
                                                   uint32_t cell, int i, Load&& load, Use&& use) {
@@ -79,15 +80,15 @@ __device__ __forceinline__ void for_each_listed_b(const KP& k, const NbrList& L,
   if (c <= L.cap) {
     const int* __restrict__ p = L.idx + nbr_slot(i, 0, L.cap);
     using Rec = decltype(load(0));
-    for (int q0 = 0; q0 < c; q0 += NBR_BATCH) {
-      int j[NBR_BATCH];
-      Rec v[NBR_BATCH];
+    for (int q0 = 0; q0 < c; q0 += B) {
+      int j[B];
+      Rec v[B];
 #pragma unroll
-      for (int b = 0; b < NBR_BATCH; ++b) j[b] = p[(size_t)min(q0 + b, c - 1) * NBR_SLICE];
+      for (int b = 0; b < B; ++b) j[b] = p[(size_t)min(q0 + b, c - 1) * NBR_SLICE];
 #pragma unroll
-      for (int b = 0; b < NBR_BATCH; ++b) v[b] = load(j[b]);
+      for (int b = 0; b < B; ++b) v[b] = load(j[b]);
 #pragma unroll
-      for (int b = 0; b < NBR_BATCH; ++b)
+      for (int b = 0; b < B; ++b)
         if (q0 + b < c) use(j[b], v[b]);
     }
   } else {
