@@ -34,10 +34,16 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE,
 # scaled by a 256 MiB copy (profiles/iisph_mf_step_1m_hbm_traffic.md)
 TRAFFIC_BYTES = {
-    "iisph_iter": 353.7e6,
-    "iisph_dp": 273.0e6,
-    "mf_alpha": 3637.2e6,
-    "mf_force": 2792.7e6,
+    "iisph_iter": 366.3e6,
+    "iisph_dp": 280.2e6,
+    "mf_alpha": 421.2e6,
+    "mf_force": 416.8e6,
+    "mf_drift": 360.0e6,
+    "iisph_aii": 333.0e6,
+    "iisph_dii": 269.7e6,
+    "nbr_build": 254.1e6,
+    "iisph_density": 230.3e6,
+    "mlp_features": 443.2e6,
 }
 
 ALGO_BYTES = {
