@@ -272,6 +272,7 @@ struct sphx_context {
     int last_fallback = 0;
     sphx::DevBuf<float> w, m, v, g;
     sphx::DevBuf<uint16_t> w16;         // FP16 bits of w
+    sphx::DevBuf<uint4> wmf;            // w16 in the MFMA operand layout
     sphx::DevBuf<float> X, pprev;       // per slot, current step
     sphx::DevBuf<float4> unew;          // drift with the converged pressure
     sphx::DevBuf<float> sX, sT, sM;     // recorded samples

@@ -113,82 +113,198 @@ __global__ __launch_bounds__(256) void k_mlp_features(
   x[23] = 1.f;
 }
 
-// ------------------------------------------------------------ inference --
-// FP16 weights staged in LDS, FP32 accumulation.  Loops run over output
-// units innermost so each LDS read fetches consecutive weights.
-// Layer 1 is unrolled over inputs and units; layers 2 and 3 are fused in
-// chunks of 8 hidden units, so no register array is indexed dynamically.
+// Hidden units per chunk of the training kernels' fused layers.
 constexpr int MLP_CH = 8;
 
-__device__ __forceinline__ void mlp_forward16(const __half* __restrict__ sw,
-                                              const float* __restrict__ xin, float* __restrict__ y) {
-  float h1[MLP_H];
-#pragma unroll
-  for (int a = 0; a < MLP_H; ++a) h1[a] = __half2float(sw[MLP_B1 + a]);
-#pragma unroll
-  for (int q = 0; q < MLP_NF; ++q) {
-    const float xq = xin[q];
-#pragma unroll
-    for (int a = 0; a < MLP_H; ++a) h1[a] = fmaf(xq, __half2float(sw[MLP_W1 + q * MLP_H + a]), h1[a]);
+// ------------------------------------------------- inference on MFMA --
+// One wave runs 64 particles as four 16-particle column tiles on
+// v_mfma_f32_16x16x32_f16 with every layer transposed (H^T = W^T X^T).  A
+// layer's accumulators (rows 4 (lane / 16) + i, column lane % 16) are then
+// the next layer's B operand (k = 8 (lane / 16) .. +8, same column) once the
+// K index within each 32-unit chunk is read as u(g, s) = s < 4 ? 4 g + s :
+// 16 + 4 g + s - 4; the weights are staged in that order.  Weights are the
+// network's FP16 values exactly; activations enter as two FP16 pieces
+// (x = hi + lo, two products each), so a layer's result is its FP32-input
+// value to about 2^-22, accumulated in FP32.
+typedef __attribute__((ext_vector_type(8))) _Float16 h8;
+typedef __attribute__((ext_vector_type(4))) float f4v;
+
+struct MlpMfmaShared {
+  uint4 a1[4][64];      // W1^T, M tile m: lane (r, g) holds W1[8 g + s][16 m + r]
+  uint4 a2[4][2][64];   // W2^T, M tile m, K chunk c: W2[32 c + u(g, s)][16 m + r]
+  uint4 a3[2][64];      // W3^T (rows 4..15 zero), K chunk c: W3[32 c + u(g, s)][r]
+  float b1[MLP_H], b2[MLP_H], b3[MLP_NO];
+};
+
+__device__ __forceinline__ int mfma_unit(int g, int s) { return s < 4 ? 4 * g + s : 16 + 4 * g + s - 4; }
+
+__device__ void stage_mfma(MlpMfmaShared& S, const __half* __restrict__ w16) {
+  const uint16_t* w = reinterpret_cast<const uint16_t*>(w16);
+  uint16_t* a1 = reinterpret_cast<uint16_t*>(&S.a1[0][0]);
+  uint16_t* a2 = reinterpret_cast<uint16_t*>(&S.a2[0][0][0]);
+  uint16_t* a3 = reinterpret_cast<uint16_t*>(&S.a3[0][0]);
+  for (int t = threadIdx.x; t < 4 * 64 * 8; t += blockDim.x) {
+    const int sl = t & 7, lane = (t >> 3) & 63, m = t >> 9;
+    const int r = lane & 15, g = lane >> 4, f = 8 * g + sl;
+    a1[t] = f < MLP_NF ? w[MLP_W1 + f * MLP_H + 16 * m + r] : (uint16_t)0;
   }
-#pragma unroll
-  for (int a = 0; a < MLP_H; ++a) h1[a] = fmaxf(h1[a], 0.f);
-#pragma unroll
-  for (int o = 0; o < MLP_NO; ++o) y[o] = __half2float(sw[MLP_B3 + o]);
-#pragma unroll 1
-  for (int bc = 0; bc < MLP_H; bc += MLP_CH) {
-    float acc[MLP_CH];
-#pragma unroll
-    for (int j = 0; j < MLP_CH; ++j) acc[j] = __half2float(sw[MLP_B2 + bc + j]);
-#pragma unroll
-    for (int a = 0; a < MLP_H; ++a) {
-      const __half* wr = sw + MLP_W2 + a * MLP_H + bc;
-#pragma unroll
-      for (int j = 0; j < MLP_CH; ++j) acc[j] = fmaf(h1[a], __half2float(wr[j]), acc[j]);
-    }
-#pragma unroll
-    for (int j = 0; j < MLP_CH; ++j) {
-      const float hb = fmaxf(acc[j], 0.f);
-#pragma unroll
-      for (int o = 0; o < MLP_NO; ++o)
-        y[o] = fmaf(hb, __half2float(sw[MLP_W3 + (bc + j) * MLP_NO + o]), y[o]);
-    }
+  for (int t = threadIdx.x; t < 4 * 2 * 64 * 8; t += blockDim.x) {
+    const int sl = t & 7, lane = (t >> 3) & 63, c = (t >> 9) & 1, m = t >> 10;
+    const int r = lane & 15, g = lane >> 4, u = 32 * c + mfma_unit(g, sl);
+    a2[t] = w[MLP_W2 + u * MLP_H + 16 * m + r];
+  }
+  for (int t = threadIdx.x; t < 2 * 64 * 8; t += blockDim.x) {
+    const int sl = t & 7, lane = (t >> 3) & 63, c = t >> 9;
+    const int r = lane & 15, g = lane >> 4, u = 32 * c + mfma_unit(g, sl);
+    a3[t] = r < MLP_NO ? w[MLP_W3 + u * MLP_NO + r] : (uint16_t)0;
+  }
+  for (int t = threadIdx.x; t < MLP_H; t += blockDim.x) {
+    S.b1[t] = __half2float(w16[MLP_B1 + t]);
+    S.b2[t] = __half2float(w16[MLP_B2 + t]);
+    if (t < MLP_NO) S.b3[t] = __half2float(w16[MLP_B3 + t]);
   }
 }
 
-__device__ __forceinline__ void stage_w16(__half* sw, const __half* __restrict__ w16) {
-  const uint32_t* src = reinterpret_cast<const uint32_t*>(w16);
-  uint32_t* dst = reinterpret_cast<uint32_t*>(sw);
-  for (int t = threadIdx.x; t < MLP_NP / 2; t += blockDim.x) dst[t] = src[t];
+__device__ __forceinline__ h8 as_h8(uint4 v) { return __builtin_bit_cast(h8, v); }
+
+constexpr int MLP_MFMA_U4 = (int)(sizeof(MlpMfmaShared) / sizeof(uint4));
+static_assert(sizeof(MlpMfmaShared) % sizeof(uint4) == 0, "MFMA weight block is whole uint4s");
+
+// The MFMA operand layout of the current FP16 weights, built once per call
+// of the network (one workgroup) and copied whole by every inference
+// workgroup.
+__global__ __launch_bounds__(256) void k_mlp_mfma_pack(const __half* __restrict__ w16,
+                                                       MlpMfmaShared* __restrict__ out) {
+  stage_mfma(*out, w16);
+}
+
+__device__ __forceinline__ void load_mfma(MlpMfmaShared& S, const uint4* __restrict__ wmf) {
+  uint4* dst = reinterpret_cast<uint4*>(&S);
+  for (int t = threadIdx.x; t < MLP_MFMA_U4; t += blockDim.x) dst[t] = wmf[t];
+}
+
+// x = hi + lo, both FP16
+__device__ __forceinline__ void split_h8(const float v[8], h8& hi, h8& lo) {
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const _Float16 h = (_Float16)v[s];
+    hi[s] = h;
+    lo[s] = (_Float16)(v[s] - (float)h);
+  }
+}
+
+// Column tiles (of 16 particles) per pass of the forward: two passes of two
+// keep the live accumulators and operands at half.
+constexpr int MLP_TPP = 2;
+
+// Next layer's B operands from this layer's accumulators acc[m][t] (4 M
+// tiles of 16 units), with bias and ReLU.
+__device__ __forceinline__ void relu_to_b(const f4v acc[4][MLP_TPP], const float* bias, int g,
+                                          h8 bh[2][MLP_TPP], h8 bl[2][MLP_TPP]) {
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int t = 0; t < MLP_TPP; ++t) {
+      float v[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const int m = 2 * c + (s < 4 ? 0 : 1), i = s & 3;
+        v[s] = fmaxf(acc[m][t][i] + bias[32 * c + mfma_unit(g, s)], 0.f);
+      }
+      split_h8(v, bh[c][t], bl[c][t]);
+    }
+}
+
+// Network outputs of particles base .. base + 63 (one wave) into y[64][4]
+// (LDS).  Particles past n repeat particle n - 1.
+__device__ void mlp_forward_mfma(const MlpMfmaShared& S, const float* __restrict__ X, int n,
+                                 int base, float* y) {
+  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+  const f4v zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+  for (int t0 = 0; t0 < 4; t0 += MLP_TPP) {
+    h8 xh[MLP_TPP], xl[MLP_TPP];
+#pragma unroll
+    for (int t = 0; t < MLP_TPP; ++t) {
+      const int p = min(base + 16 * (t0 + t) + r, n - 1);
+      float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      if (8 * g < MLP_NF) {
+        const float4* src = reinterpret_cast<const float4*>(X + (size_t)p * MLP_NF + 8 * g);
+        const float4 u0 = src[0], u1 = src[1];
+        v[0] = u0.x; v[1] = u0.y; v[2] = u0.z; v[3] = u0.w;
+        v[4] = u1.x; v[5] = u1.y; v[6] = u1.z; v[7] = u1.w;
+      }
+      split_h8(v, xh[t], xl[t]);
+    }
+    f4v acc[4][MLP_TPP];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const h8 a = as_h8(S.a1[m][lane]);
+#pragma unroll
+      for (int t = 0; t < MLP_TPP; ++t) {
+        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, xh[t], zero, 0, 0, 0);
+        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, xl[t], acc[m][t], 0, 0, 0);
+      }
+    }
+    h8 bh[2][MLP_TPP], bl[2][MLP_TPP];
+    relu_to_b(acc, S.b1, g, bh, bl);
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int t = 0; t < MLP_TPP; ++t) {
+        f4v e = zero;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const h8 a = as_h8(S.a2[m][c][lane]);
+          e = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bh[c][t], e, 0, 0, 0);
+          e = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bl[c][t], e, 0, 0, 0);
+        }
+        acc[m][t] = e;
+      }
+    relu_to_b(acc, S.b2, g, bh, bl);
+#pragma unroll
+    for (int t = 0; t < MLP_TPP; ++t) {
+      f4v o = zero;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const h8 a = as_h8(S.a3[c][lane]);
+        o = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bh[c][t], o, 0, 0, 0);
+        o = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bl[c][t], o, 0, 0, 0);
+      }
+      if (g == 0)   // rows 0..3 are the four outputs
+#pragma unroll
+        for (int i = 0; i < MLP_NO; ++i) y[(16 * (t0 + t) + r) * MLP_NO + i] = o[i] + S.b3[i];
+    }
+  }
 }
 
 // Writes the warm start p0 = max(0, p(t-1) + P y0 / 10) (and s = m p0 /
 // rho^2 for the pressure pass) when `warm`, and the drift correction for
-// the next step when dcorr is non-null.
-__global__ __launch_bounds__(256) void k_mlp_infer(
-    KP k, MLPP mp, int warm, const float* __restrict__ X, const __half* __restrict__ w16,
+// the next step when dcorr is non-null.  256 threads: four waves of 64
+// particles.
+__global__ __launch_bounds__(256, 2) void k_mlp_infer(
+    KP k, MLPP mp, int warm, const float* __restrict__ X, const uint4* __restrict__ wmf,
     const float4* __restrict__ pos, const uint32_t* __restrict__ key,
     const float4* __restrict__ phys, float* __restrict__ press_l, float4* __restrict__ ps,
     float4* __restrict__ dcorr) {
-  __shared__ __align__(16) __half sw[MLP_NP];
-  stage_w16(sw, w16);
+  __shared__ MlpMfmaShared S;
+  __shared__ float ys[4][64 * MLP_NO];
+  load_mfma(S, wmf);
   __syncthreads();
-  int i = xcd_index();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int base = xcd_tile() * (int)blockDim.x + 64 * wave;
+  if (base >= k.n) return;   // whole wave past the end
+  mlp_forward_mfma(S, X, k.n, base, ys[wave]);
+  // the wave's LDS writes above reach LDS before its reads below
+  asm volatile("" ::: "memory");
+  const int i = base + lane;
   if (i >= k.n) return;
   float4 pi = pos[i];
   if (type_of(pi) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) {
     if (dcorr) dcorr[i] = make_float4(0.f, 0.f, 0.f, dcorr[i].w);
     return;
   }
-  const float4* xv = reinterpret_cast<const float4*>(X + (size_t)i * MLP_NF);
-  float xin[MLP_NF];
-#pragma unroll
-  for (int q = 0; q < MLP_NF / 4; ++q) {
-    float4 t = xv[q];
-    xin[4 * q] = t.x; xin[4 * q + 1] = t.y; xin[4 * q + 2] = t.z; xin[4 * q + 3] = t.w;
-  }
-  float y[MLP_NO];
-  mlp_forward16(sw, xin, y);
+  const float* y = ys[wave] + lane * MLP_NO;
   if (warm) {
     const float4 qi = phys[i];
     float p0 = fmaxf(0.f, fmaxf(0.f, qi.z) + y[0] * (mp.p_scale / MLP_TSCALE));
@@ -197,27 +313,41 @@ __global__ __launch_bounds__(256) void k_mlp_infer(
     ps[i].w = qi.x * p0 / (qi.y * qi.y);
   }
   if (dcorr) {
-    float s = mp.drift_blend * mp.u_scale / MLP_TSCALE;
-    float4 d = make_float4(y[1] * s, y[2] * s, y[3] * s, dcorr[i].w);
+    float sc = mp.drift_blend * mp.u_scale / MLP_TSCALE;
+    float4 d = make_float4(y[1] * sc, y[2] * sc, y[3] * sc, dcorr[i].w);
     if (!(d.x == d.x && d.y == d.y && d.z == d.z)) d = make_float4(0.f, 0.f, 0.f, d.w);
     dcorr[i] = d;
   }
 }
 
-// The network alone on given features (n x 24) -> Y (n x 4).
-__global__ __launch_bounds__(256) void k_mlp_eval(const float* __restrict__ X, int n,
-                                                  const __half* __restrict__ w16,
-                                                  float* __restrict__ Y) {
-  __shared__ __align__(16) __half sw[MLP_NP];
-  stage_w16(sw, w16);
+// The network on stored feature rows (tests, evaluation): the same MFMA
+// path as the solver's warm start.
+__global__ __launch_bounds__(256, 2) void k_mlp_eval(const float* __restrict__ X, int n,
+                                                     const uint4* __restrict__ wmf,
+                                                     float* __restrict__ Y) {
+  __shared__ MlpMfmaShared S;
+  __shared__ float ys[4][64 * MLP_NO];
+  load_mfma(S, wmf);
   __syncthreads();
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int base = blockIdx.x * (int)blockDim.x + 64 * wave;
+  if (base >= n) return;
+  mlp_forward_mfma(S, X, n, base, ys[wave]);
+  asm volatile("" ::: "memory");
+  const int i = base + lane;
   if (i >= n) return;
-  float xin[MLP_NF];
-  for (int q = 0; q < MLP_NF; ++q) xin[q] = X[(size_t)i * MLP_NF + q];
-  float y[MLP_NO];
-  mlp_forward16(sw, xin, y);
-  for (int o = 0; o < MLP_NO; ++o) Y[(size_t)i * MLP_NO + o] = y[o];
+  for (int o = 0; o < MLP_NO; ++o) Y[(size_t)i * MLP_NO + o] = ys[wave][lane * MLP_NO + o];
+}
+
+
+// The weights in the MFMA operand layout (one small launch per use).
+static int mlp_pack_mfma(sphx_context* c) {
+  auto& m = c->mlp;
+  SPHX_TRY(m.wmf.alloc(MLP_MFMA_U4));
+  k_mlp_mfma_pack<<<1, 256, 0, c->stream>>>(reinterpret_cast<const __half*>(m.w16.p),
+                                            reinterpret_cast<MlpMfmaShared*>(m.wmf.p));
+  SPHX_HIP(hipGetLastError());
+  return 0;
 }
 
 // Plain warm start p0 = warm * p(t-1) (fluid_system_kern.cu:899), used when
@@ -622,9 +752,10 @@ int mlp_begin(sphx_context* c) {
   float4* dcorr = m.drift && mf ? hist : nullptr;
   if (m.warm || dcorr) {
     timer_begin(c, "mlp_infer");
+    SPHX_TRY(mlp_pack_mfma(c));
     k_mlp_infer<<<blocks_for(k.n, c->tune[T_MLP_INFER]), c->tune[T_MLP_INFER], 0, c->stream>>>(
-        k, mp, m.warm, m.X.p, reinterpret_cast<const __half*>(m.w16.p), c->POS(), c->key.p,
-        c->PHYS(), c->press_l.p, c->ps.p, dcorr);
+        k, mp, m.warm, m.X.p, m.wmf.p, c->POS(), c->key.p, c->PHYS(), c->press_l.p, c->ps.p,
+        dcorr);
     timer_end(c);
     SPHX_HIP(hipGetLastError());
   }
@@ -927,8 +1058,8 @@ int sphx_mlp_eval(sphx_context* c, const float* X, int n, float* Y) {
   SPHX_TRY(dx.alloc((size_t)n * MLP_NF));
   SPHX_TRY(dy.alloc((size_t)n * MLP_NO));
   SPHX_HIP(hipMemcpy(dx.p, X, (size_t)n * MLP_NF * sizeof(float), hipMemcpyHostToDevice));
-  k_mlp_eval<<<blocks_for(n, 256), 256, 0, c->stream>>>(
-      dx.p, n, reinterpret_cast<const __half*>(c->mlp.w16.p), dy.p);
+  SPHX_TRY(mlp_pack_mfma(c));
+  k_mlp_eval<<<blocks_for(n, 256), 256, 0, c->stream>>>(dx.p, n, c->mlp.wmf.p, dy.p);
   SPHX_HIP(hipGetLastError());
   SPHX_HIP(hipStreamSynchronize(c->stream));
   SPHX_HIP(hipMemcpy(Y, dy.p, (size_t)n * MLP_NO * sizeof(float), hipMemcpyDeviceToHost));
