@@ -139,11 +139,17 @@ __global__ __launch_bounds__(RPY_THREADS) void k_rpy_tile(
   // packed words (256 contiguous bytes per store)
   const int qp = 2 * (tid & 15), pr = tid >> 4;
   const float inv8pimu = 1.f / (8.f * RPY_PI * mu);
-  f32x4 acc[3][4];
+  // MFMA work of a wave: rows p0 + 32 wr .. +32 (two 16-row tiles) and
+  // right-hand sides 32 wc .. +32 (two 16-column tiles), so each F piece is
+  // loaded by two waves of the workgroup instead of four
+  const int wr = wave & 1, wc = wave >> 1;
+  f32x4 acc[3][2][2];
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) acc[a][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) acc[a][rt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int ncb = K / 16;
   for (int q0 = q_begin; q0 < q_end; q0 += RPY_TQ) {
     __syncthreads();
@@ -169,45 +175,49 @@ __global__ __launch_bounds__(RPY_THREADS) void k_rpy_tile(
       }
     }
     __syncthreads();
-    // MFMA: wave handles rows p0 + 16 wave .. +16; lane layout of
-    // v_mfma_f32_16x16x32_bf16: A row = lane % 16, k = 8 (lane / 16) .. +8;
-    // B column = lane % 16, same k; D rows 4 (lane / 16) + i, column lane % 16.
-    const int arow = 16 * wave + (lane & 15);
+    // MFMA lane layout of v_mfma_f32_16x16x32_bf16: A row = lane % 16,
+    // k = 8 (lane / 16) .. +8; B column = lane % 16, same k; D rows
+    // 4 (lane / 16) + i, column lane % 16.
     const int kk = 8 * (lane >> 4);
-    // one b at a time: its F pieces (48 VGPRs) are all that is live
+    // one b at a time: its F pieces are all that is live of F
 #pragma unroll 1
     for (int b = 0; b < 3; ++b) {
-      bf16x8 fh[4], fl[4], f3[4];
+      bf16x8 fh[2], fl[2], f3[2];
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) {
+      for (int ct = 0; ct < 2; ++ct) {
+        const int cb = 2 * wc + ct;
         if (cb >= ncb) break;
         size_t off = ((size_t)b * K + 16 * cb + (lane & 15)) * np + q0 + kk;
-        fh[cb] = glb8(Fh + off);
-        if (mode >= 3) fl[cb] = glb8(Fl + off);
-        if (mode == 6) f3[cb] = glb8(F3 + off);
+        fh[ct] = glb8(Fh + off);
+        if (mode >= 3) fl[ct] = glb8(Fl + off);
+        if (mode == 6) f3[ct] = glb8(F3 + off);
       }
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         const int k6 = sym_index(a, b);
-        bf16x8 ah = lds8(&Mh[k6][arow][kk]);
-        bf16x8 al = lds8(&Ml[k6][arow][kk]);
-        bf16x8 a3;
-        if (mode == 6) a3 = lds8(&M3[k6][arow][kk]);
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-          if (cb >= ncb) break;
-          f32x4 c = acc[a][cb];
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, fh[cb], c, 0, 0, 0);
-          if (mode >= 3) {
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, fl[cb], c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, fh[cb], c, 0, 0, 0);
+        for (int rt = 0; rt < 2; ++rt) {
+          const int arow = 32 * wr + 16 * rt + (lane & 15);
+          bf16x8 ah = lds8(&Mh[k6][arow][kk]);
+          bf16x8 al = lds8(&Ml[k6][arow][kk]);
+          bf16x8 a3;
+          if (mode == 6) a3 = lds8(&M3[k6][arow][kk]);
+#pragma unroll
+          for (int ct = 0; ct < 2; ++ct) {
+            if (2 * wc + ct >= ncb) break;
+            f32x4 c = acc[a][rt][ct];
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, fh[ct], c, 0, 0, 0);
+            if (mode >= 3) {
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, fl[ct], c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, fh[ct], c, 0, 0, 0);
+            }
+            if (mode == 6) {
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, fl[ct], c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, f3[ct], c, 0, 0, 0);
+              c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, fh[ct], c, 0, 0, 0);
+            }
+            acc[a][rt][ct] = c;
           }
-          if (mode == 6) {
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, fl[cb], c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, f3[cb], c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, fh[cb], c, 0, 0, 0);
-          }
-          acc[a][cb] = c;
         }
       }
     }
@@ -216,15 +226,18 @@ __global__ __launch_bounds__(RPY_THREADS) void k_rpy_tile(
 #pragma unroll
   for (int a = 0; a < 3; ++a)
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
-      if (cb >= ncb) break;
+    for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        int p = p0 + 16 * wave + 4 * (lane >> 4) + i;
-        int c = 16 * cb + (lane & 15);
-        part[(((size_t)s * 3 + a) * np + p) * K + c] = acc[a][cb][i];
+      for (int ct = 0; ct < 2; ++ct) {
+        const int cb = 2 * wc + ct;
+        if (cb >= ncb) break;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          int p = p0 + 32 * wr + 16 * rt + 4 * (lane >> 4) + i;
+          int c = 16 * cb + (lane & 15);
+          part[(((size_t)s * 3 + a) * np + p) * K + c] = acc[a][rt][ct][i];
+        }
       }
-    }
 }
 
 // U[3p + a][c] = F/(6 pi mu a_p) + sum_s part[s][a][p][c]
