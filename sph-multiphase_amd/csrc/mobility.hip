@@ -123,13 +123,24 @@ __device__ __forceinline__ bf16x8 glb8(const uint16_t* p) {
 
 // Partial products of the off-diagonal mobility for row tile blockIdx.x and
 // reduction segment blockIdx.y: part[s][a][p][c].
-__global__ __launch_bounds__(RPY_THREADS) void k_rpy_tile(
-    const float4* __restrict__ sph, int n, int np, float mu, int K, int seg, int mode,
+// The pieces planes a precision mode multiplies: hi (mode 1), hi and lo
+// (mode 3), all three (mode 6).
+template <int MODE>
+struct RpyPieces {
+  static constexpr int N = MODE == 1 ? 1 : (MODE == 3 ? 2 : 3);
+  uint16_t M[N][6][RPY_TP][RPY_TQ];
+};
+
+template <int MODE>
+__global__ __launch_bounds__(RPY_THREADS, MODE == 1 ? 3 : 2) void k_rpy_tile(
+    const float4* __restrict__ sph, int n, int np, float mu, int K, int seg,
     const uint16_t* __restrict__ Fh, const uint16_t* __restrict__ Fl,
     const uint16_t* __restrict__ F3, float* __restrict__ part) {
-  __shared__ __align__(16) uint16_t Mh[6][RPY_TP][RPY_TQ];
-  __shared__ __align__(16) uint16_t Ml[6][RPY_TP][RPY_TQ];
-  __shared__ __align__(16) uint16_t M3[6][RPY_TP][RPY_TQ];
+  constexpr int mode = MODE;
+  __shared__ __align__(16) RpyPieces<MODE> P;
+  auto& Mh = P.M[0];
+  auto& Ml = P.M[MODE >= 3 ? 1 : 0];
+  auto& M3 = P.M[MODE == 6 ? 2 : 0];
   const int p0 = blockIdx.x * RPY_TP;
   const int s = blockIdx.y;
   const int q_begin = s * seg, q_end = min(np, q_begin + seg);
@@ -151,16 +162,25 @@ __global__ __launch_bounds__(RPY_THREADS) void k_rpy_tile(
 #pragma unroll
       for (int ct = 0; ct < 2; ++ct) acc[a][rt][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int ncb = K / 16;
+  // the thread's four row spheres are the same for every chunk; the two
+  // column spheres of the next chunk are fetched during this chunk's MFMAs
+  float4 sp4[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int gp = p0 + pr + 16 * t;
+    sp4[t] = gp < n ? sph[gp] : make_float4(0.f, 0.f, 0.f, 1.f);
+  }
+  auto col = [&](int gq) { return gq < n ? sph[gq] : make_float4(0.f, 0.f, 0.f, 1.f); };
+  float4 nq0 = col(q_begin + qp), nq1 = col(q_begin + qp + 1);
   for (int q0 = q_begin; q0 < q_end; q0 += RPY_TQ) {
     __syncthreads();
     {
       const int gq0 = q0 + qp, gq1 = gq0 + 1;
-      const float4 sq0 = gq0 < n ? sph[gq0] : make_float4(0.f, 0.f, 0.f, 1.f);
-      const float4 sq1 = gq1 < n ? sph[gq1] : make_float4(0.f, 0.f, 0.f, 1.f);
+      const float4 sq0 = nq0, sq1 = nq1;
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int pl = pr + 16 * t, gp = p0 + pl;
-        const float4 sp = gp < n ? sph[gp] : make_float4(0.f, 0.f, 0.f, 1.f);
+        const float4 sp = sp4[t];
         float m0[6], m1[6];
         rpy_pair(sp, sq0, gp < n && gq0 < n && gq0 != gp, inv8pimu, m0);
         rpy_pair(sp, sq1, gp < n && gq1 < n && gq1 != gp, inv8pimu, m1);
@@ -175,12 +195,16 @@ __global__ __launch_bounds__(RPY_THREADS) void k_rpy_tile(
       }
     }
     __syncthreads();
+    if (q0 + RPY_TQ < q_end) {
+      nq0 = col(q0 + RPY_TQ + qp);
+      nq1 = col(q0 + RPY_TQ + qp + 1);
+    }
     // MFMA lane layout of v_mfma_f32_16x16x32_bf16: A row = lane % 16,
     // k = 8 (lane / 16) .. +8; B column = lane % 16, same k; D rows
     // 4 (lane / 16) + i, column lane % 16.
     const int kk = 8 * (lane >> 4);
-    // one b at a time: its F pieces are all that is live of F
-#pragma unroll 1
+    // all three b unrolled: their F loads issue together
+#pragma unroll
     for (int b = 0; b < 3; ++b) {
       bf16x8 fh[2], fl[2], f3[2];
 #pragma unroll
@@ -482,8 +506,15 @@ int sphx_rpy_apply_device(sphx_rpy* h, const float* dF, float* dU, int mode) {
   k_rpy_split_f<<<blocks_for(tot, 256), 256, 0, h->stream>>>(dF, h->n, np, K, h->Fh.p, h->Fl.p,
                                                              h->F3.p);
   dim3 grid(np / RPY_TP, nseg);
-  k_rpy_tile<<<grid, RPY_THREADS, 0, h->stream>>>(h->sph.p, h->n, np, h->mu, K, seg, mode,
-                                                  h->Fh.p, h->Fl.p, h->F3.p, h->part.p);
+  if (mode == 1)
+    k_rpy_tile<1><<<grid, RPY_THREADS, 0, h->stream>>>(h->sph.p, h->n, np, h->mu, K, seg, h->Fh.p,
+                                                       h->Fl.p, h->F3.p, h->part.p);
+  else if (mode == 3)
+    k_rpy_tile<3><<<grid, RPY_THREADS, 0, h->stream>>>(h->sph.p, h->n, np, h->mu, K, seg, h->Fh.p,
+                                                       h->Fl.p, h->F3.p, h->part.p);
+  else
+    k_rpy_tile<6><<<grid, RPY_THREADS, 0, h->stream>>>(h->sph.p, h->n, np, h->mu, K, seg, h->Fh.p,
+                                                       h->Fl.p, h->F3.p, h->part.p);
   int outs = 3 * h->n * K;
   k_rpy_reduce<<<blocks_for(outs, 256), 256, 0, h->stream>>>(h->sph.p, h->n, np, h->mu, K, nseg,
                                                              dF, h->part.p, dU);
