@@ -452,6 +452,11 @@ int sphx_rpy_apply(sphx_rpy* h, const float* F, float* U, int mode);
 /* Same on device arrays, asynchronous on the handle's stream. */
 int sphx_rpy_apply_device(sphx_rpy* h, const float* dF, float* dU, int mode);
 int sphx_rpy_sync(sphx_rpy* h);
+/* Time each tile-kernel launch with HIP events on the handle's stream
+ * (benchmarks); sphx_rpy_timing_get returns the total ms and count since
+ * timing was switched on and forgets them. */
+int sphx_rpy_timing(sphx_rpy* h, int on);
+int sphx_rpy_timing_get(sphx_rpy* h, double* ms, int* count);
 /* The reference walkthrough step on device buffers (equal spheres a = 1,
  * units 1/(6 pi mu a)): near-field force added to cuForce when with_force
  * (Kern_getForce :257-300), background velocity interpolated from the MPM

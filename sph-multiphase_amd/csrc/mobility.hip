@@ -19,6 +19,9 @@
 // :207-255) and the dense product getMobU (:85-109).
 #include "common.hpp"
 
+#include <utility>
+#include <vector>
+
 namespace sphx {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
@@ -426,6 +429,9 @@ struct sphx_rpy {
   sphx::DevBuf<float4> sph;
   sphx::DevBuf<uint16_t> Fh, Fl, F3;
   sphx::DevBuf<float> part, F, U;
+  // tile-kernel timing: an event pair around each k_rpy_tile launch
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
 };
 
 extern "C" {
@@ -458,9 +464,18 @@ int sphx_rpy_create(sphx_rpy** out, int n, int nrhs) {
   return 0;
 }
 
+static void rpy_drop_events(sphx_rpy* h) {
+  for (auto& e : h->ev) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  h->ev.clear();
+}
+
 int sphx_rpy_destroy(sphx_rpy* h) {
   if (!h) return 0;
   (void)hipStreamSynchronize(h->stream);
+  rpy_drop_events(h);
   h->sph.release(); h->Fh.release(); h->Fl.release(); h->F3.release();
   h->part.release(); h->F.release(); h->U.release();
   (void)hipStreamDestroy(h->stream);
@@ -506,6 +521,13 @@ int sphx_rpy_apply_device(sphx_rpy* h, const float* dF, float* dU, int mode) {
   k_rpy_split_f<<<blocks_for(tot, 256), 256, 0, h->stream>>>(dF, h->n, np, K, h->Fh.p, h->Fl.p,
                                                              h->F3.p);
   dim3 grid(np / RPY_TP, nseg);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->timing) {
+    SPHX_HIP(hipEventCreate(&e0));
+    SPHX_HIP(hipEventCreate(&e1));
+    h->ev.emplace_back(e0, e1);
+    SPHX_HIP(hipEventRecord(e0, h->stream));
+  }
   if (mode == 1)
     k_rpy_tile<1><<<grid, RPY_THREADS, 0, h->stream>>>(h->sph.p, h->n, np, h->mu, K, seg, h->Fh.p,
                                                        h->Fl.p, h->F3.p, h->part.p);
@@ -515,6 +537,7 @@ int sphx_rpy_apply_device(sphx_rpy* h, const float* dF, float* dU, int mode) {
   else
     k_rpy_tile<6><<<grid, RPY_THREADS, 0, h->stream>>>(h->sph.p, h->n, np, h->mu, K, seg, h->Fh.p,
                                                        h->Fl.p, h->F3.p, h->part.p);
+  if (h->timing) SPHX_HIP(hipEventRecord(e1, h->stream));
   int outs = 3 * h->n * K;
   k_rpy_reduce<<<blocks_for(outs, 256), 256, 0, h->stream>>>(h->sph.p, h->n, np, h->mu, K, nseg,
                                                              dF, h->part.p, dU);
@@ -531,6 +554,33 @@ int sphx_rpy_apply(sphx_rpy* h, const float* F, float* U, int mode) {
   SPHX_TRY(sphx_rpy_apply_device(h, h->F.p, h->U.p, mode));
   SPHX_HIP(hipMemcpyAsync(U, h->U.p, sz, hipMemcpyDeviceToHost, h->stream));
   SPHX_HIP(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+// Time each k_rpy_tile launch with HIP events on the handle's stream (on)
+// or stop and forget the recorded launches (off).
+int sphx_rpy_timing(sphx_rpy* h, int on) {
+  SPHX_CHECK(h, "sphx_rpy_timing", "null handle");
+  SPHX_HIP(hipStreamSynchronize(h->stream));
+  rpy_drop_events(h);
+  h->timing = on != 0;
+  return 0;
+}
+
+// Total milliseconds and count of the tile launches timed since timing was
+// switched on; forgets them.
+int sphx_rpy_timing_get(sphx_rpy* h, double* ms, int* count) {
+  SPHX_CHECK(h && ms && count, "sphx_rpy_timing_get", "null argument");
+  SPHX_HIP(hipStreamSynchronize(h->stream));
+  double t = 0.0;
+  for (auto& e : h->ev) {
+    float f = 0.f;
+    SPHX_HIP(hipEventElapsedTime(&f, e.first, e.second));
+    t += f;
+  }
+  *ms = t;
+  *count = (int)h->ev.size();
+  rpy_drop_events(h);
   return 0;
 }
 

@@ -190,6 +190,8 @@ def lib() -> C.CDLL:
         L.sphx_rpy_apply.argtypes = [VP, VP, VP, I]
         L.sphx_rpy_apply_device.argtypes = [VP, VP, VP, I]
         L.sphx_rpy_sync.argtypes = [VP]
+        L.sphx_rpy_timing.argtypes = [VP, I]
+        L.sphx_rpy_timing_get.argtypes = [VP, P(D), P(I)]
         L.sphx_rpy_walkthrough.argtypes = [VP, VP, I, I]
         L.sphx_mob_gemv.argtypes = [VP, VP, VP, I]
         L.sphx_dev_alloc.argtypes = [P(VP), C.c_size_t]
@@ -614,6 +616,15 @@ class Mobility:
 
     def sync(self):
         check(self.L.sphx_rpy_sync(self.h), "sphx_rpy_sync")
+
+    def timing(self, on=True):
+        check(self.L.sphx_rpy_timing(self.h, int(on)), "sphx_rpy_timing")
+
+    def timing_get(self):
+        """(total ms, launches) of the tile kernel since timing(True)."""
+        ms, n = C.c_double(), C.c_int()
+        check(self.L.sphx_rpy_timing_get(self.h, C.byref(ms), C.byref(n)), "sphx_rpy_timing_get")
+        return ms.value, n.value
 
     def close(self):
         if self.h:
