@@ -217,9 +217,10 @@ struct sphx_context {
   sphx::DevBuf<double> solve_err; // [0]=last mean |rho-rho0|/rho0 [1]=mean signed
   double* host_pinned = nullptr;  // 4 doubles: done, iters, err, signed
   int last_iters = 0;             // iterations of the previous solve
-  // per-step neighbour list (ELL: idx[q * stride + i]); rebuilt after any
+  // per-step neighbour list (sliced, 16-bit entries packed in 16-byte groups); rebuilt after any
   // change of positions or order
-  sphx::DevBuf<int> nbr_idx, nbr_cnt;
+  sphx::DevBuf<uint4> nbr_idx;   // packed neighbour list groups
+  sphx::DevBuf<int> nbr_cnt;
   sphx::DevBuf<float4> ps;        // {x, y, z, s = m p / rho^2} for the DP pass
   int nbr_cap = 0;
   bool nbr_valid = false;

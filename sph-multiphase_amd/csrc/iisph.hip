@@ -49,7 +49,7 @@ __global__ void k_ii_density(KP k, NbrList L, const uint32_t* __restrict__ key, 
   // self term (the reference's scan visits i itself, W(0) = kspline)
   float sum = k.kspline * qi.x;
   struct Rec { float4 p, q; };
-  for_each_listed_b<NBR_BATCH_SMALL>(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j]}; },
+  for_each_listed_b<NBR_BATCH_SMALL, false>(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j]}; },
                                      [&](int, const Rec& o) {
     const float4 pj = o.p, qj = o.q;
     float mj = (type_of(pj) == SPHX_TYPE_BOUNDARY) ? rest * qj.y : qj.x;
@@ -137,7 +137,7 @@ __global__ void k_ii_aii(KP k, NbrList L, const uint32_t* __restrict__ key, cons
   const float cji = mi / rhoi / rhoi * k.dt * k.dt;
   float dchange = 0.f, aii = 0.f;
   struct Rec { float4 p, q, v; };
-  for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j], veleval[j]}; },
+  for_each_listed_b<NBR_BATCH, false>(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j], veleval[j]}; },
                     [&](int, const Rec& o) {
     const float4 pj = o.p, qj = o.q, vj = o.v;
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
@@ -184,7 +184,7 @@ __global__ void k_ii_dp(KP k, NbrList L, const int* __restrict__ done, const uin
   float sx = 0.f, sy = 0.f, sz = 0.f;
   // one 16-byte gather per neighbour: {x_j, s_j}; boundary s_j = 0 adds
   // nothing, as the reference's skip of boundary j (:925-926)
-  for_each_listed_b<NBR_BATCH_SMALL>(k, L, cs, gc, i, [&](int j) { return ps[j]; },
+  for_each_listed_b<NBR_BATCH_SMALL, false>(k, L, cs, gc, i, [&](int j) { return ps[j]; },
                                      [&](int, const float4& pj) {
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
@@ -482,7 +482,7 @@ __device__ __forceinline__ int nb_wave_max(int v) {
 // stage holds, tests straight from global memory.
 __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
     KP k, const uint32_t* __restrict__ key, const int* __restrict__ cs,
-    const float4* __restrict__ pos, int* __restrict__ idx, int* __restrict__ cnt,
+    const float4* __restrict__ pos, uint4* __restrict__ idx, int* __restrict__ cnt,
     float4* __restrict__ ps, int cap) {
   __shared__ float4 spos[NB_STAGE];
   __shared__ int sh[7];           // row r0, row r1, key min/max of r0, of r1, bad
@@ -551,8 +551,52 @@ __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
   const int total = ro[18];
   const bool staged = sh[6] == 0 && total <= NB_STAGE;
   const float rc2 = 4.f * k.h2;
-  int* __restrict__ out = idx + nbr_slot(i, 0, cap);
+  uint4* __restrict__ out = idx + nbr_gslot(i, 0, cap / NBR_GROUP);
   int c = 0;
+  bool ovf = false;
+  NbrPacker pk;
+  // entry of candidate j in neighbour row r; sd = start of row 3 (r / 3)
+  auto add = [&](int j, uint32_t dtag, int sd) {
+    const uint32_t t = (uint32_t)(j - sd);
+    ovf |= t > NBR_TMAX;
+    if (c < cap) {
+      pk.push(dtag | (t & NBR_TMAX));
+      if (((c + 1) & (NBR_GROUP - 1)) == 0) out[(size_t)(c / NBR_GROUP) * NBR_SLICE] = pk.get();
+    }
+    ++c;
+  };
+  // Candidates of one row are tested 32 at a time into a hit mask, whose
+  // set bits are then appended in order: the append runs once per hit of
+  // the lane with the most hits, not once per candidate.
+  auto scan_rows = [&](auto&& row_base, auto&& fetch) {
+    int sd = 0;
+#pragma unroll 1
+    for (int r = 0; r < 9; ++r) {
+      const int c0 = (int)gc + k.rowoff[r];
+      const int jb = cs[c0], je = cs[c0 + 3];
+      if (r % 3 == 0) sd = jb;
+      const uint32_t dtag = (uint32_t)(r / 3) << NBR_TBITS;
+      const int base = row_base(r);
+#pragma unroll 1
+      for (int j0 = jb; j0 < je; j0 += 32) {
+        const int m = min(32, je - j0);
+        uint32_t hit = 0;
+#pragma unroll 4
+        for (int u = 0; u < m; ++u) {
+          const int j = j0 + u;
+          const float4 pj = fetch(base + j);
+          float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+          float dsq = dx * dx + dy * dy + dz * dz;
+          hit |= (uint32_t)(dsq < rc2 && j != i) << u;
+        }
+        while (hit) {
+          const int u = __builtin_ctz(hit);
+          hit &= hit - 1;
+          add(j0 + u, dtag, sd);
+        }
+      }
+    }
+  };
   if (staged) {
     for (int u = t; u < total; u += blockDim.x) {
       int q = 0;
@@ -563,35 +607,18 @@ __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
     if (valid) {
       const int* rsR = rs + 9 * R;
       const int* roR = ro + 9 * R;
-#pragma unroll 1
-      for (int r = 0; r < 9; ++r) {
-        const int c0 = (int)gc + k.rowoff[r];
-        const int jb = cs[c0], je = cs[c0 + 3];
-        const int base = roR[r] - rsR[r];
-#pragma unroll 4
-        for (int j = jb; j < je; ++j) {
-          const float4 pj = spos[base + j];
-          float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
-          float dsq = dx * dx + dy * dy + dz * dz;
-          if (dsq < rc2 && j != i) {
-            if (c < cap) out[(size_t)c * NBR_SLICE] = j;
-            ++c;
-          }
-        }
-      }
+      // row r's candidates sit at LDS offset roR[r] - rsR[r] + j
+      scan_rows([&](int r) { return roR[r] - rsR[r]; }, [&](int u) { return spos[u]; });
     }
   } else if (valid) {
-    for_each_neighbor(k, cs, gc, [&](int j) {
-      float4 pj = pos[j];
-      float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
-      float dsq = dx * dx + dy * dy + dz * dz;
-      if (dsq < rc2 && j != i) {
-        if (c < cap) out[(size_t)c * NBR_SLICE] = j;
-        ++c;
-      }
-    });
+    scan_rows([&](int) { return 0; }, [&](int j) { return pos[j]; });
   }
-  if (i < k.n) cnt[i] = c;
+  if (c <= cap && (c & (NBR_GROUP - 1))) {
+    // zero entries pad the last group; they decode to S_0 and are never used
+    for (int u = c & (NBR_GROUP - 1); u < NBR_GROUP; ++u) pk.push(0u);
+    out[(size_t)(c / NBR_GROUP) * NBR_SLICE] = pk.get();
+  }
+  if (i < k.n) cnt[i] = (ovf || c > cap) ? cap + 1 : c;
 }
 
 int grid_hash(sphx_context* c);
@@ -622,7 +649,7 @@ int nbr_list(sphx_context* c, NbrList* out) {
   if (!c->nbr_valid) {
     int cap = nbr_capacity(c->P);
     size_t slices = (size_t)(n + NBR_SLICE - 1) / NBR_SLICE;
-    SPHX_TRY(c->nbr_idx.alloc(std::max<size_t>(slices, 1) * cap * NBR_SLICE));
+    SPHX_TRY(c->nbr_idx.alloc(std::max<size_t>(slices, 1) * (cap / NBR_GROUP) * NBR_SLICE));
     SPHX_TRY(c->nbr_cnt.alloc(std::max(n, 1)));
     SPHX_TRY(c->ps.alloc(c->cap));
     c->nbr_cap = cap;

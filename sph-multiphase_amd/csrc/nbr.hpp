@@ -25,66 +25,102 @@ __device__ __forceinline__ void for_each_neighbor(const KP& k, const int* __rest
   }
 }
 
-// Per-step neighbour list in sliced-ELL layout: particles are grouped in
-// slices of 256; entry q of particle i is idx[(slice * cap + q) * 256 +
-// (i & 255)], so a wave reading entry q of its 64 particles touches one
-// 256-byte run and a slice's whole list is one contiguous cap KiB block.
-// Entries follow the reference's visiting order and hold every j != i with
-// |x_ij| < 2h (the widest support any kernel of the step uses).  Positions
-// do not move between the density pass and the last pressure iteration, so
-// one list serves every neighbour loop of an IISPH step.  A particle with
-// more than `cap` neighbours keeps its true count and its loops fall back
-// to the cell scan, which visits a superset in the same order.
+// Per-step neighbour list, sliced and packed.  Entries are 16 bits:
+// e = d << 14 | (j - S_d), where d = dy + 1 is the neighbour's cell layer
+// relative to particle i and S_d = cell_start of the first cell of that
+// layer's first row (dz = -1, dx = -1), i.e. rows 3d .. 3d+2 of the 27-cell
+// neighbourhood are one index window starting at S_d.  Eight entries form
+// one 16-byte group; particles are grouped in slices of 256, and group g of
+// particle i is at idx[(slice * capg + g) * 256 + (i & 255)], so a wave
+// reading group g of its 64 particles touches one 1 KiB run.  Entries follow
+// the reference's visiting order and hold every j != i with |x_ij| < 2h (the
+// widest support any kernel of the step uses).  Positions do not move
+// between the density pass and the last pressure iteration, so one list
+// serves every neighbour loop of an IISPH step.  A particle with more than
+// `cap` neighbours, or one whose offset j - S_d does not fit 14 bits, has
+// count cap + 1 and its loops fall back to the cell scan, which visits a
+// superset in the same order.
 constexpr int NBR_SLICE = 256;
+constexpr int NBR_GROUP = 8;                 // entries per 16-byte group
+constexpr int NBR_TBITS = 14;                // offset bits of an entry
+constexpr uint32_t NBR_TMAX = (1u << NBR_TBITS) - 1;
 
-__host__ __device__ __forceinline__ size_t nbr_slot(int i, int q, int cap) {
-  return ((size_t)(i / NBR_SLICE) * cap + q) * NBR_SLICE + (i % NBR_SLICE);
+__host__ __device__ __forceinline__ size_t nbr_gslot(int i, int g, int capg) {
+  return ((size_t)(i / NBR_SLICE) * capg + g) * NBR_SLICE + (i % NBR_SLICE);
 }
 
 struct NbrList {
-  const int* __restrict__ idx;
-  const int* __restrict__ cnt;
-  int cap;
+  const uint4* __restrict__ idx;   // packed groups
+  const int* __restrict__ cnt;     // entries; > cap: use the cell scan
+  int cap;                         // entries per particle, a multiple of NBR_GROUP
 };
 
-template <class F>
-__device__ __forceinline__ void for_each_listed(const KP& k, const NbrList& L,
-                                                const int* __restrict__ cell_start, uint32_t cell,
-                                                int i, F&& f) {
-  const int c = L.cnt[i];
-  if (c <= L.cap) {
-    const int* __restrict__ p = L.idx + nbr_slot(i, 0, L.cap);
-#pragma unroll 4
-    for (int q = 0; q < c; ++q) f(p[(size_t)q * NBR_SLICE]);
-  } else {
-    for_each_neighbor(k, cell_start, cell, [&](int j) {
-      if (j != i) f(j);
-    });
-  }
+// Window starts S_0, S_1, S_2 of the three cell layers around `cell`.
+__device__ __forceinline__ void nbr_bases(const KP& k, const int* __restrict__ cs, uint32_t cell,
+                                          int& s0, int& s1, int& s2) {
+  s0 = cs[(int)cell + k.rowoff[0]];
+  s1 = cs[(int)cell + k.rowoff[3]];
+  s2 = cs[(int)cell + k.rowoff[6]];
 }
 
-// Batched traversal of the list: B entries at a time, and the
-// records of all of them fetched (load(j)) before any is used (use(j, rec)),
-// so a batch's gathers are in flight together instead of one dependent
-// round trip per neighbour.  Entries past the count reload the last entry
-// (inside the particle's own list) and are not used; the visiting order,
-// and so every sum's order, is that of for_each_listed.
-constexpr int NBR_BATCH = 8;    // default batch (wide records)
+__device__ __forceinline__ int nbr_decode(uint32_t e, int s0, int s1, int s2) {
+  const uint32_t d = e >> NBR_TBITS;
+  const int b = d == 0 ? s0 : (d == 1 ? s1 : s2);
+  return b + (int)(e & NBR_TMAX);
+}
+
+// Batched traversal of the list: B entries at a time, and the records of
+// all of them fetched (load(j)) before any is used (use(j, rec)), so a
+// batch's gathers are in flight together instead of one dependent round
+// trip per neighbour; with PREFETCH the next batch's entry groups are
+// fetched before the current batch's records are used (loops whose live
+// state leaves room for the extra registers).  Entries past the count decode to S_0
+// (a sorted index <= i, inside the arrays) and are not used; groups past
+// the last written one are not read.  The visiting order, and so every
+// sum's order, is that of the list.
+constexpr int NBR_BATCH = 8;          // default batch (wide records)
 constexpr int NBR_BATCH_SMALL = 16;   // loops with few live values (density, DP)
 
-template <int B = NBR_BATCH, class Load, class Use>
+template <int B = NBR_BATCH, bool PREFETCH = true, class Load, class Use>
 __device__ __forceinline__ void for_each_listed_b(const KP& k, const NbrList& L,
                                                   const int* __restrict__ cell_start,
                                                   uint32_t cell, int i, Load&& load, Use&& use) {
+  static_assert(B % NBR_GROUP == 0, "batch must be whole groups");
+  constexpr int G = B / NBR_GROUP;
   const int c = L.cnt[i];
   if (c <= L.cap) {
-    const int* __restrict__ p = L.idx + nbr_slot(i, 0, L.cap);
+    if (c <= 0) return;
+    int s0, s1, s2;
+    nbr_bases(k, cell_start, cell, s0, s1, s2);
+    const uint4* __restrict__ p = L.idx + nbr_gslot(i, 0, L.cap / NBR_GROUP);
+    const int glast = (c - 1) / NBR_GROUP;
+    uint4 g[G];
+    if (PREFETCH) {
+#pragma unroll
+      for (int u = 0; u < G; ++u) g[u] = p[(size_t)min(u, glast) * NBR_SLICE];
+    }
     using Rec = decltype(load(0));
     for (int q0 = 0; q0 < c; q0 += B) {
-      int j[B];
-      Rec v[B];
+      if (!PREFETCH) {
 #pragma unroll
-      for (int b = 0; b < B; ++b) j[b] = p[(size_t)min(q0 + b, c - 1) * NBR_SLICE];
+        for (int u = 0; u < G; ++u) g[u] = p[(size_t)min(q0 / NBR_GROUP + u, glast) * NBR_SLICE];
+      }
+      int j[B];
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        const uint32_t w[4] = {g[u].x, g[u].y, g[u].z, g[u].w};
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+          j[u * NBR_GROUP + 2 * h] = nbr_decode(w[h] & 0xFFFFu, s0, s1, s2);
+          j[u * NBR_GROUP + 2 * h + 1] = nbr_decode(w[h] >> 16, s0, s1, s2);
+        }
+      }
+      if (PREFETCH) {
+        const int gn = q0 / NBR_GROUP + G;
+#pragma unroll
+        for (int u = 0; u < G; ++u) g[u] = p[(size_t)min(gn + u, glast) * NBR_SLICE];
+      }
+      Rec v[B];
 #pragma unroll
       for (int b = 0; b < B; ++b) v[b] = load(j[b]);
 #pragma unroll
@@ -97,6 +133,20 @@ __device__ __forceinline__ void for_each_listed_b(const KP& k, const NbrList& L,
     });
   }
 }
+
+// Writer side of the packed list: entries are shifted in at the top of a
+// 128-bit register, so after eight pushes the first is the lowest 16 bits,
+// and each full group is stored as one 16-byte write.
+struct NbrPacker {
+  uint32_t w0 = 0, w1 = 0, w2 = 0, w3 = 0;
+  __device__ __forceinline__ void push(uint32_t e) {
+    w0 = __builtin_amdgcn_alignbit(w1, w0, 16);
+    w1 = __builtin_amdgcn_alignbit(w2, w1, 16);
+    w2 = __builtin_amdgcn_alignbit(w3, w2, 16);
+    w3 = __builtin_amdgcn_alignbit(e, w3, 16);
+  }
+  __device__ __forceinline__ uint4 get() const { return make_uint4(w0, w1, w2, w3); }
+};
 
 // r = |x| and 1/r from r^2 (v_rsq_f32, about 1 ulp; r = 0 for r^2 = 0).
 __device__ __forceinline__ void dist_rinv(float dsq, float& r, float& ir) {
