@@ -210,7 +210,10 @@ struct sphx_context {
   // IISPH per-slot buffers (not permuted, as in the reference)
   sphx::DevBuf<float4> dii;       // d_ii.xyz, a_ii
   sphx::DevBuf<float4> dijpj;     // sum_j d_ij p_j .xyz, unused
-  sphx::DevBuf<float4> qm;        // (d_ii p + sum d_ij p_j).xyz, mass
+  sphx::DevBuf<float4> mqa;       // {x, y, z, m q.x}, q = d_ii p + sum d_ij p_j (0 for boundary)
+  sphx::DevBuf<float2> mqb;       // {m q.y, m q.z}
+  sphx::DevBuf<float4> gh;        // sum m_j gradW_ij (+ boundary), sum m_j |gradW_ij|^2
+  bool gh_valid = false;          // gh matches the current positions and d_ii inputs
   sphx::DevBuf<float> press_l, press_l1, rho_adv, residue, sfac;
   sphx::DevBuf<double> red;       // reduction partials
   sphx::DevBuf<int> solve_state;  // [0]=done [1]=iters
@@ -328,6 +331,7 @@ inline int blocks_for(int n, int threads) { return (n + threads - 1) / threads; 
 // drift-flux records are stale.
 inline void positions_changed(sphx_context* c) {
   c->nbr_valid = false;
+  c->gh_valid = false;
   c->mf_rec_ok = false;
 }
 

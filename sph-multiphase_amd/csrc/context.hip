@@ -290,7 +290,9 @@ int sphx_create(sphx_context** out, const sphx_ParamCarrier* p) {
   chk(c->mfid.alloc(cap));
   chk(c->dii.alloc(cap));
   chk(c->dijpj.alloc(cap));
-  chk(c->qm.alloc(cap));
+  chk(c->mqa.alloc(cap));
+  chk(c->mqb.alloc(cap));
+  chk(c->gh.alloc(cap));
   chk(c->press_l.alloc(cap));
   chk(c->press_l1.alloc(cap));
   chk(c->rho_adv.alloc(cap));
@@ -339,7 +341,7 @@ int sphx_destroy(sphx_context* c) {
   c->cold_color.release(); c->cold_mat.release(); c->cold_X.release(); c->cold_GK.release();
   c->key.release(); c->idsort.release(); c->key_tmp.release(); c->val_tmp.release();
   c->block_hist.release(); c->cell_start.release(); c->mfid.release();
-  c->dii.release(); c->dijpj.release(); c->qm.release();
+  c->dii.release(); c->dijpj.release(); c->mqa.release(); c->mqb.release(); c->gh.release();
   c->press_l.release(); c->press_l1.release(); c->rho_adv.release(); c->residue.release();
   c->sfac.release(); c->red.release(); c->solve_state.release(); c->solve_err.release();
   if (c->host_pinned) (void)hipHostFree(c->host_pinned);

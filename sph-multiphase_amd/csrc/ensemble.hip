@@ -38,6 +38,7 @@ static void after_graph_a(sphx_context* c, int p) {
   c->cur = 1 - p;
   c->sorted_valid = true;
   c->nbr_valid = true;
+  c->gh_valid = true;
   c->mf_pending = true;
 }
 

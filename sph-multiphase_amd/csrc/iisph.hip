@@ -123,7 +123,7 @@ __global__ void k_ii_aii(KP k, NbrList L, const uint32_t* __restrict__ key, cons
                          const float4* __restrict__ pos, const float4* __restrict__ veleval,
                          const float4* __restrict__ phys, float4* __restrict__ dii,
                          float* __restrict__ rho_adv, float* __restrict__ press_l,
-                         float4* __restrict__ ps) {
+                         float4* __restrict__ ps, float4* __restrict__ gh) {
   int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
@@ -136,6 +136,7 @@ __global__ void k_ii_aii(KP k, NbrList L, const uint32_t* __restrict__ key, cons
   const float4 di = dii[i];
   const float cji = mi / rhoi / rhoi * k.dt * k.dt;
   float dchange = 0.f, aii = 0.f;
+  float Gx = 0.f, Gy = 0.f, Gz = 0.f, H = 0.f;
   struct Rec { float4 p, q, v; };
   for_each_listed_b<NBR_BATCH, false>(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j], veleval[j]}; },
                     [&](int, const Rec& o) {
@@ -147,19 +148,22 @@ __global__ void k_ii_aii(KP k, NbrList L, const uint32_t* __restrict__ key, cons
     dist_rinv(dsq, r, ir);
     float gf = cubic_gradf(r, ir, k.inv_h, k.kspline);
     float gx = dx * gf, gy = dy * gf, gz = dz * gf;
-    int tj = type_of(pj);
     float vdg = (vi.x - vj.x) * gx + (vi.y - vj.y) * gy + (vi.z - vj.z) * gz;
-    if (tj != SPHX_TYPE_BOUNDARY) {
-      dchange += vdg * qj.x * k.dt;
-      // d_ji = gradW m_i / rho_i^2 dt^2; a_ii += (d_ii - d_ji) . gradW m_j
-      float tx = di.x - gx * cji, ty = di.y - gy * cji, tz = di.z - gz * cji;
-      aii += (tx * gx + ty * gy + tz * gz) * qj.x;
-    } else if (k.boundary_pressure) {
-      float mb = rest * qj.y;   // :833-834
-      dchange += vdg * mb * k.dt;
-      aii += (di.x * gx + di.y * gy + di.z * gz) * mb;
-    }
+    // fluid j: weight m_j and d_ji = gradW m_i / rho_i^2 dt^2, so
+    // a_ii += (d_ii - d_ji) . gradW m_j; boundary j (when boundary pressure
+    // is on, :833-834): weight rho0_i V_j and no d_ji term; else nothing.
+    // Written with selects, one expression for both kinds.
+    const bool fl = type_of(pj) != SPHX_TYPE_BOUNDARY;
+    const float w = fl ? qj.x : (k.boundary_pressure ? rest * qj.y : 0.f);
+    const float wh = fl ? qj.x : 0.f;
+    const float cj = fl ? cji : 0.f;
+    dchange += vdg * w * k.dt;
+    float tx = di.x - gx * cj, ty = di.y - gy * cj, tz = di.z - gz * cj;
+    aii += (tx * gx + ty * gy + tz * gz) * w;
+    Gx += gx * w; Gy += gy * w; Gz += gz * w;
+    H += (gx * gx + gy * gy + gz * gz) * wh;
   });
+  gh[i] = make_float4(Gx, Gy, Gz, H);
   rho_adv[i] = rhoi + dchange;
   float p0 = k.warm * qi.z;
   press_l[i] = p0;
@@ -167,12 +171,50 @@ __global__ void k_ii_aii(KP k, NbrList L, const uint32_t* __restrict__ key, cons
   dii[i].w = aii;
 }
 
-// Pressure_DP (kern.cu:910-958) plus q_i = d_ii p_i + dijpj_i.
+// The position-only sums of the a_ii pass, G_i = sum_j m_j gradW_ij (plus
+// rho0_i V_j gradW_ij over boundary j when boundary pressure is on) and
+// H_i = sum_fluid m_j |gradW_ij|^2, for iterations that start from uploaded
+// solver state instead of a fresh a_ii pass.
+__global__ void k_ii_gh(KP k, NbrList L, const uint32_t* __restrict__ key, const int* __restrict__ cs,
+                        const float4* __restrict__ pos, const float4* __restrict__ phys,
+                        float4* __restrict__ gh) {
+  int i = xcd_index();
+  if (i >= k.n) return;
+  float4 pi = pos[i];
+  if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
+  uint32_t gc = key[i];
+  if ((int)gc == k.gridtotal) return;
+  const float rest = phys[i].w;
+  float Gx = 0.f, Gy = 0.f, Gz = 0.f, H = 0.f;
+  struct Rec { float4 p, q; };
+  for_each_listed_b<NBR_BATCH, false>(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j]}; },
+                                      [&](int, const Rec& o) {
+    const float4 pj = o.p, qj = o.q;
+    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+    float dsq = dx * dx + dy * dy + dz * dz;
+    if (!in_support(k, dsq)) return;
+    float r, ir;
+    dist_rinv(dsq, r, ir);
+    float gf = cubic_gradf(r, ir, k.inv_h, k.kspline);
+    float gx = dx * gf, gy = dy * gf, gz = dz * gf;
+    const bool fl = type_of(pj) != SPHX_TYPE_BOUNDARY;
+    const float w = fl ? qj.x : (k.boundary_pressure ? rest * qj.y : 0.f);
+    const float wh = fl ? qj.x : 0.f;
+    Gx += gx * w; Gy += gy * w; Gz += gz * w;
+    H += (gx * gx + gy * gy + gz * gz) * wh;
+  });
+  gh[i] = make_float4(Gx, Gy, Gz, H);
+}
+
+// Pressure_DP (kern.cu:910-958) plus q_i = d_ii p_i + dijpj_i, stored as
+// the record the Jacobi pass gathers ({x, m q.x}, {m q.y, m q.z}), and the
+// pressure acceleration q_i / dt^2 (kern.cu:1044-1047).
 __global__ void k_ii_dp(KP k, NbrList L, const int* __restrict__ done, const uint32_t* __restrict__ key,
                         const int* __restrict__ cs, const float4* __restrict__ pos,
                         const float4* __restrict__ ps, const float4* __restrict__ phys,
                         const float4* __restrict__ dii, const float* __restrict__ press_l,
-                        float4* __restrict__ dijpj, float4* __restrict__ qm, SceneCtl sc) {
+                        float4* __restrict__ dijpj, float4* __restrict__ mqa,
+                        float2* __restrict__ mqb, float4* __restrict__ accel, SceneCtl sc) {
   if (*done) return;
   int i = xcd_index();
   if (i >= k.n) return;
@@ -199,7 +241,12 @@ __global__ void k_ii_dp(KP k, NbrList L, const int* __restrict__ done, const uin
   dijpj[i] = d;
   float4 di = dii[i];
   float p = press_l[i];
-  qm[i] = make_float4(di.x * p + d.x, di.y * p + d.y, di.z * p + d.z, phys[i].x);
+  const float qx = di.x * p + d.x, qy = di.y * p + d.y, qz = di.z * p + d.z;
+  const float mi = phys[i].x;
+  mqa[i] = make_float4(pi.x, pi.y, pi.z, mi * qx);
+  mqb[i] = make_float2(mi * qy, mi * qz);
+  const float idt2 = 1.f / (k.dt * k.dt);
+  accel[i] = make_float4(qx * idt2, qy * idt2, qz * idt2, 0.f);
 }
 
 // Block reduction of (signed residue, |residue|/rho0, fluid count) in FP64.
@@ -227,10 +274,10 @@ __global__ __launch_bounds__(NB_THREADS) void k_ii_iter(
     KP k, NbrList L, const int* __restrict__ done, const uint32_t* __restrict__ key,
     const int* __restrict__ cs, const float4* __restrict__ pos, float4* __restrict__ phys,
     const float4* __restrict__ dii, const float4* __restrict__ dijpj,
-    const float4* __restrict__ qm, const float* __restrict__ rho_adv,
+    const float4* __restrict__ mqa, const float2* __restrict__ mqb,
+    const float4* __restrict__ gh, const float* __restrict__ rho_adv,
     float* __restrict__ press_l, float* __restrict__ press_l1, float4* __restrict__ ps,
-    float* __restrict__ residue, float4* __restrict__ accel, double* __restrict__ partial,
-    SceneCtl sc) {
+    float* __restrict__ residue, double* __restrict__ partial, SceneCtl sc) {
   if (*done) return;
   __shared__ double sacc[SPHX_MAX_SCENES * 3];
   if (sc.n)
@@ -253,27 +300,27 @@ __global__ __launch_bounds__(NB_THREADS) void k_ii_iter(
       const float4 dpi = dijpj[i];
       const float pl = press_l[i];
       const float cji = k.dt * k.dt * mi / rhoi / rhoi * pl;
-      float S = 0.f;
-      struct Rec { float4 p, q; };
-      for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], qm[j]}; },
-                        [&](int j, const Rec& o) {
-        const float4 pj = o.p;
-        float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+      // S_i = sum_j m_j (dijpj_i - q_j + d_ji p_i) . gradW_ij over fluid j
+      // (kern.cu:992-999), plus m_b dijpj_i . gradW_ij over boundary j when
+      // boundary pressure is on (:1000-1002).  The position-only parts are
+      // sums of the a_ii pass: S_i = dijpj_i . G_i + (d_ji p_i / gradW) H_i -
+      // sum_j m_j q_j . gradW_ij, so the loop gathers one record pair per
+      // neighbour, {x_j, m_j q_j.x} and {m_j q_j.y, m_j q_j.z} (zero q for
+      // boundary j).
+      float acc = 0.f;
+      struct Rec { float4 a; float2 b; };
+      for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{mqa[j], mqb[j]}; },
+                        [&](int, const Rec& o) {
+        float dx = (pi.x - o.a.x) * k.ss, dy = (pi.y - o.a.y) * k.ss, dz = (pi.z - o.a.z) * k.ss;
         float dsq = dx * dx + dy * dy + dz * dz;
         if (!in_support(k, dsq)) return;
         float r, ir;
         dist_rinv(dsq, r, ir);
         float gf = cubic_gradf(r, ir, k.inv_h, k.kspline);
-        float gx = dx * gf, gy = dy * gf, gz = dz * gf;
-        if (type_of(pj) != SPHX_TYPE_BOUNDARY) {
-          const float4 q = o.q;
-          float tx = dpi.x - q.x + cji * gx, ty = dpi.y - q.y + cji * gy, tz = dpi.z - q.z + cji * gz;
-          S += q.w * (tx * gx + ty * gy + tz * gz);
-        } else if (k.boundary_pressure) {
-          float mb = rest * phys[j].y;   // :1000-1002
-          S += mb * (dpi.x * gx + dpi.y * gy + dpi.z * gz);
-        }
+        acc += gf * (dx * o.a.w + dy * o.b.x + dz * o.b.y);
       });
+      const float4 G = gh[i];
+      const float S = dpi.x * G.x + dpi.y * G.y + dpi.z * G.z + cji * G.w - acc;
       float aii = di.w;
       float pn;
       if (fabsf(aii) < 1e-6f) pn = pl;
@@ -281,9 +328,6 @@ __global__ __launch_bounds__(NB_THREADS) void k_ii_iter(
       if (k.clamp_p && pn < 0.f) pn = 0.f;   // kern.cu:1035-1036 (commented there)
       float res = -rest + (rho_adv[i] + S + aii * pl);
       residue[i] = res;
-      float4 q = qm[i];
-      float idt2 = 1.f / (k.dt * k.dt);
-      accel[i] = make_float4(q.x * idt2, q.y * idt2, q.z * idt2, 0.f);
       phys[i].z = pn;
       press_l[i] = pn;
       press_l1[i] = pn;
@@ -483,7 +527,7 @@ __device__ __forceinline__ int nb_wave_max(int v) {
 __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
     KP k, const uint32_t* __restrict__ key, const int* __restrict__ cs,
     const float4* __restrict__ pos, uint4* __restrict__ idx, int* __restrict__ cnt,
-    float4* __restrict__ ps, int cap) {
+    float4* __restrict__ ps, float4* __restrict__ mqa, float2* __restrict__ mqb, int cap) {
   __shared__ float4 spos[NB_STAGE];
   __shared__ int sh[7];           // row r0, row r1, key min/max of r0, of r1, bad
   __shared__ int rs[19], ro[19];  // range (row, neighbour row) begin; LDS offsets
@@ -496,6 +540,10 @@ __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
   if (i < k.n) {
     pi = pos[i];
     ps[i] = make_float4(pi.x, pi.y, pi.z, 0.f);
+    // the Jacobi pass's record: q = 0 until the pressure pass sets a fluid
+    // particle's, and for good for a boundary particle
+    mqa[i] = make_float4(pi.x, pi.y, pi.z, 0.f);
+    mqb[i] = make_float2(0.f, 0.f);
     gc = key[i];
     valid = type_of(pi) != SPHX_TYPE_BOUNDARY && (int)gc != k.gridtotal;
     row = (int)(gc / (uint32_t)k.gridres.x);
@@ -657,7 +705,8 @@ int nbr_list(sphx_context* c, NbrList* out) {
       KP k = make_kp(c);
       timer_begin(c, "nbr_build");
       k_nbr_build<<<blocks_for(n, NB_THREADS_BUILD), NB_THREADS_BUILD, 0, c->stream>>>(
-          k, c->key.p, c->cell_start.p, c->POS(), c->nbr_idx.p, c->nbr_cnt.p, c->ps.p, cap);
+          k, c->key.p, c->cell_start.p, c->POS(), c->nbr_idx.p, c->nbr_cnt.p, c->ps.p, c->mqa.p,
+          c->mqb.p, cap);
       timer_end(c);
       SPHX_HIP(hipGetLastError());
     }
@@ -687,18 +736,25 @@ static int enqueue_iteration(sphx_context* c, const KP& k, int respect_done) {
   int nb = blocks_for(k.n, tit);
   NbrList L;
   SPHX_TRY(nbr_list(c, &L));
+  if (!c->gh_valid) {
+    k_ii_gh<<<blocks_for(k.n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
+        k, L, c->key.p, c->cell_start.p, c->POS(), c->PHYS(), c->gh.p);
+    SPHX_HIP(hipGetLastError());
+    c->gh_valid = true;
+  }
   IterCtl ctl{c->O.min_iters, c->O.max_iters, c->O.tol};
   SceneCtl sc = c->scenes();
   timer_begin(c, "iisph_dp");
   k_ii_dp<<<blocks_for(k.n, tdp), tdp, 0, c->stream>>>(k, L, c->solve_state.p, c->key.p, c->cell_start.p,
                                             c->POS(), c->ps.p, c->PHYS(), c->dii.p,
-                                            c->press_l.p, c->dijpj.p, c->qm.p, sc);
+                                            c->press_l.p, c->dijpj.p, c->mqa.p, c->mqb.p,
+                                            c->ACCEL(), sc);
   timer_end(c);
   timer_begin(c, "iisph_iter");
   k_ii_iter<<<nb, tit, 0, c->stream>>>(k, L, c->solve_state.p, c->key.p, c->cell_start.p,
-                                              c->POS(), c->PHYS(), c->dii.p, c->dijpj.p, c->qm.p,
-                                              c->rho_adv.p, c->press_l.p, c->press_l1.p,
-                                              c->ps.p, c->residue.p, c->ACCEL(), c->red.p, sc);
+                                              c->POS(), c->PHYS(), c->dii.p, c->dijpj.p, c->mqa.p,
+                                              c->mqb.p, c->gh.p, c->rho_adv.p, c->press_l.p,
+                                              c->press_l1.p, c->ps.p, c->residue.p, c->red.p, sc);
   timer_end(c);
   if (sc.n)
     k_ii_finalize_scenes<<<1, 256, 0, c->stream>>>(c->solve_state.p, c->solve_err.p, sc, ctl,
@@ -767,8 +823,9 @@ int sphx_iisph_predict_aii(sphx_context* c) {
   timer_begin(c, "iisph_aii");
   k_ii_aii<<<blocks_for(k.n, c->tune[T_AII]), c->tune[T_AII], 0, c->stream>>>(
       k, L, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p, c->rho_adv.p,
-      c->press_l.p, c->ps.p);
+      c->press_l.p, c->ps.p, c->gh.p);
   timer_end(c);
+  c->gh_valid = true;
   SPHX_HIP(hipGetLastError());
   return 0;
 }

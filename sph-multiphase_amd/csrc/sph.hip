@@ -207,6 +207,7 @@ int sphx_boundary_volume(sphx_context* c) {
   k_boundary_volume<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(k, c->key.p, c->cell_start.p,
                                                                   c->POS(), c->PHYS());
   timer_end(c);
+  c->gh_valid = false;   // boundary volumes enter the pressure pass's G_i
   SPHX_HIP(hipGetLastError());
   return 0;
 }
