@@ -545,6 +545,8 @@ def main():
     ap.add_argument("--online-iters", type=int, default=16)
     ap.add_argument("--online-batch", type=int, default=8192)
     ap.add_argument("--online-lr", type=float, default=5e-4)
+    ap.add_argument("--online-async", type=int, default=1,
+                    help="refresh on a second stream, adopted two solves later")
     ap.add_argument("--workload", default="bubbles",
                     choices=["bubbles", "sand", "mobility", "ensemble"],
                     help="bubbles: two-phase tank (headline); sand: MPM 2M particles; "
@@ -639,6 +641,7 @@ def main():
     if args.online_every > 0:
         learn.mlp_online(args.online_every, args.online_iters, args.online_batch, args.online_lr,
                          olw)
+        learn.mlp_online_async(bool(args.online_async))
 
     el_plain, it_plain, err_plain, kt_plain = timed_segment(plain, args.steps, names)
     plain.close()
@@ -706,7 +709,9 @@ def main():
             "train_epochs": args.epochs,
             "train_seconds": train_s,
             "online_refresh": (f"every {args.online_every} steps: {args.online_iters} Adam steps "
-                               f"of {args.online_batch} samples (inside the timed steps)"
+                               f"of {args.online_batch} samples (inside the timed steps"
+                               + (", on a second stream, adopted two steps later)"
+                                  if args.online_async else ")")
                                if args.online_every > 0 else None),
             "parallelism": "single GPU",
         },

@@ -502,6 +502,12 @@ int sphx_mlp_mode(sphx_context* ctx, int warm, int drift, int record);
  * targets). */
 int sphx_mlp_online(sphx_context* ctx, int every, int iters, int batch, float lr,
                     const float* lw);
+/* on != 0: a refresh's Adam steps run on a second stream (default priority)
+ * while the next solve uses the current weights; the solve after that
+ * adopts the refreshed ones.  Calls that read or set weights or samples,
+ * graph builds and graph steps first wait for a running refresh.  No
+ * counterpart in the reference (the learned start is new). */
+int sphx_mlp_online_async(sphx_context* ctx, int on);
 int sphx_mlp_set_drift_blend(sphx_context* ctx, float blend);
 /* Sample store: one sample per particle slot and solve (masked when the
  * slot is boundary or out of the grid). */

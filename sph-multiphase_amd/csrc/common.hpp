@@ -284,6 +284,15 @@ struct sphx_context {
     sphx::DevBuf<int> order;
     sphx::DevBuf<double> loss;
     int batch_cap = 0;
+    // asynchronous online refresh: the Adam steps run on their own stream
+    // (default priority) while the next step is solved, into staged FP16
+    // weights that the step after that adopts
+    int online_async = 0;
+    int train_pending = 0;       // solves until the staged weights are adopted
+    bool train_inflight = false; // a refresh was launched and not yet waited for
+    hipStream_t tstream = nullptr;
+    hipEvent_t ev_main = nullptr, ev_train = nullptr;
+    sphx::DevBuf<uint16_t> w16_stage;
   } mlp;
 
   // batched ensemble: scenes in disjoint boxes of one grid

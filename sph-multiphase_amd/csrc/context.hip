@@ -330,6 +330,12 @@ int sphx_create(sphx_context** out, const sphx_ParamCarrier* p) {
 int sphx_destroy(sphx_context* c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
+  if (c->mlp.tstream) {
+    (void)hipStreamSynchronize(c->mlp.tstream);
+    (void)hipStreamDestroy(c->mlp.tstream);
+    (void)hipEventDestroy(c->mlp.ev_main);
+    (void)hipEventDestroy(c->mlp.ev_train);
+  }
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)timer_flush(c);
   (void)sphx_graph_release(c);
@@ -380,6 +386,7 @@ int sphx_set_options(sphx_context* c, const sphx_SolverOptions* o) {
 
 int sphx_sync(sphx_context* c) {
   SPHX_CHECK(c, "sphx_sync", "null context");
+  if (c->mlp.tstream) SPHX_HIP(hipStreamSynchronize(c->mlp.tstream));   // training stream
   SPHX_HIP(hipStreamSynchronize(c->stream));
   return timer_flush(c);
 }

@@ -23,6 +23,7 @@ int continue_solve(sphx_context* c, int* iters, double* err);
 int mlp_begin(sphx_context* c);
 int mlp_fallback(sphx_context* c, int* iters, double* err);
 int mlp_after(sphx_context* c, int record_ok);
+int mlp_async_settle(sphx_context* c);
 
 static void graph_release(sphx_context* c) {
   for (int p = 0; p < 2; ++p) {
@@ -93,6 +94,7 @@ int sphx_graph_build(sphx_context* c, int budget) {
   SPHX_CHECK(c->mf_pending && c->nbr_cap > 0 && c->red.p, "sphx_graph_build",
              "run one sphx_mf_step before capturing");
   SPHX_TRY(ensure_device(c));
+  SPHX_TRY(mlp_async_settle(c));   // no cross-stream wait inside the capture
   SPHX_HIP(hipStreamSynchronize(c->stream));
   graph_release(c);
   const bool timing = c->timing;
@@ -123,6 +125,7 @@ int sphx_graph_step(sphx_context* c, int* iters, double* err) {
   SPHX_CHECK(c->graph_num == c->num && c->graph_a[c->cur], "sphx_graph_step",
              "no step graph for this state (call sphx_graph_build)");
   SPHX_TRY(ensure_device(c));
+  SPHX_TRY(mlp_async_settle(c));   // graph steps adopt a finished refresh at once
   const int p = c->cur;
   SPHX_HIP(hipGraphLaunch(c->graph_a[p], c->stream));
   after_graph_a(c, p);

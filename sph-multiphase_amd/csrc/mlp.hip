@@ -648,10 +648,13 @@ int nbr_list(sphx_context* c, NbrList* out);
 int mf_drift_into(sphx_context* c, const float4* dcorr, float4* out0, float4* out1,
                   float4* out2);
 
+static int mlp_quiesce(sphx_context* c);
+
 static int mlp_need(sphx_context* c, const char* who) {
   SPHX_CHECK(c, who, "null context");
   SPHX_TRY(ensure_device(c));
   SPHX_CHECK(c->mlp.ready, who, "network not initialised (call sphx_mlp_init)");
+  if (c->mlp.train_inflight) SPHX_TRY(mlp_quiesce(c));
   return 0;
 }
 
@@ -699,8 +702,11 @@ static int mlp_refresh16(sphx_context* c) {
 int iisph_solve_plain(sphx_context* c, int* iters, double* err);
 
 // Online refresh: online_iters Adam steps, each on online_batch samples
-// drawn uniformly (on the device) from the recorded ring.
-static int mlp_online_refresh(sphx_context* c) {
+// drawn uniformly (on the device) from the recorded ring.  The Adam steps
+// write the FP16 weights the inference reads, or, asynchronous, run on the
+// training stream after this step's samples are recorded and write staged
+// FP16 weights (FP32 master and moments are touched by training only).
+static int mlp_refresh_launch(sphx_context* c, __half* w16) {
   auto& m = c->mlp;
   const int nb = m.online_batch, steps = m.online_iters;
   const size_t draws = (size_t)nb * steps;
@@ -717,10 +723,57 @@ static int mlp_online_refresh(sphx_context* c) {
     ++m.adam_t;
     float bc1 = 1.f - std::pow(b1, (float)m.adam_t), bc2 = 1.f - std::pow(b2, (float)m.adam_t);
     k_mlp_adam<<<blocks_for(MLP_NP, 256), 256, 0, c->stream>>>(
-        m.w.p, m.m.p, m.v.p, m.g.p, reinterpret_cast<__half*>(m.w16.p), MLP_NP, m.online_lr, b1,
-        b2, eps, bc1, bc2);
+        m.w.p, m.m.p, m.v.p, m.g.p, w16, MLP_NP, m.online_lr, b1, b2, eps, bc1, bc2);
     SPHX_HIP(hipGetLastError());
   }
+  return 0;
+}
+
+static int mlp_online_refresh(sphx_context* c) {
+  auto& m = c->mlp;
+  if (!m.online_async) return mlp_refresh_launch(c, reinterpret_cast<__half*>(m.w16.p));
+  if (!m.tstream) {
+    SPHX_HIP(hipStreamCreate(&m.tstream));
+    SPHX_HIP(hipEventCreateWithFlags(&m.ev_main, hipEventDisableTiming));
+    SPHX_HIP(hipEventCreateWithFlags(&m.ev_train, hipEventDisableTiming));
+  }
+  SPHX_TRY(m.w16_stage.alloc(MLP_NP + 1));
+  SPHX_HIP(hipEventRecord(m.ev_main, c->stream));
+  SPHX_HIP(hipStreamWaitEvent(m.tstream, m.ev_main, 0));
+  hipStream_t main = c->stream;
+  c->stream = m.tstream;
+  int rc = mlp_refresh_launch(c, reinterpret_cast<__half*>(m.w16_stage.p));
+  c->stream = main;
+  SPHX_TRY(rc);
+  SPHX_HIP(hipEventRecord(m.ev_train, m.tstream));
+  m.train_pending = 2;
+  m.train_inflight = true;
+  return 0;
+}
+
+__global__ void k_u16_copy(const uint16_t* __restrict__ a, uint16_t* __restrict__ b, int n) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) b[e] = a[e];
+}
+
+// Adopt the staged weights now: the main stream waits for the training
+// stream, then copies them in.
+int mlp_async_settle(sphx_context* c) {
+  auto& m = c->mlp;
+  if (!m.train_inflight) return 0;
+  SPHX_HIP(hipStreamWaitEvent(c->stream, m.ev_train, 0));
+  k_u16_copy<<<blocks_for(MLP_NP, 256), 256, 0, c->stream>>>(m.w16_stage.p, m.w16.p, MLP_NP);
+  SPHX_HIP(hipGetLastError());
+  m.train_pending = 0;
+  m.train_inflight = false;
+  return 0;
+}
+
+// Host-side calls that read or replace the weights, moments or samples
+// first finish an asynchronous refresh.
+static int mlp_quiesce(sphx_context* c) {
+  SPHX_TRY(mlp_async_settle(c));
+  SPHX_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
 
@@ -730,6 +783,7 @@ static int mlp_online_refresh(sphx_context* c) {
 int mlp_begin(sphx_context* c) {
   auto& m = c->mlp;
   if (!m.ready || c->num == 0) return 0;
+  if (m.train_pending > 0 && --m.train_pending == 0) SPHX_TRY(mlp_async_settle(c));
   KP k = make_kp(c);
   MLPP mp = make_mlpp(c);
   const bool mf = c->nphase > 0;
@@ -816,6 +870,7 @@ int mlp_after(sphx_context* c, int record_ok) {
         u_old = c->mf_drift[1].p;
         u_new = u + c->cap;
       }
+      if (m.train_inflight) SPHX_HIP(hipStreamWaitEvent(c->stream, m.ev_train, 0));
       timer_begin(c, "mlp_record");
       k_mlp_record<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
           k, mp, c->POS(), c->key.p, c->PHYS(), m.pprev.p, m.X.p, u_old, u_new,
@@ -874,6 +929,7 @@ int sphx_mlp_init(sphx_context* c, uint64_t seed, float p_scale, float u_scale) 
   SPHX_CHECK(c, "sphx_mlp_init", "null context");
   SPHX_CHECK(p_scale > 0.f && u_scale > 0.f, "sphx_mlp_init", "scales must be positive");
   SPHX_TRY(ensure_device(c));
+  if (c->mlp.train_inflight) SPHX_TRY(mlp_quiesce(c));
   auto& m = c->mlp;
   SPHX_TRY(m.w.alloc(MLP_NP));
   SPHX_TRY(m.m.alloc(MLP_NP));
@@ -956,6 +1012,16 @@ int sphx_mlp_online(sphx_context* c, int every, int iters, int batch, float lr, 
   m.online_lr = lr;
   if (lw)
     for (int o = 0; o < 4; ++o) m.online_lw[o] = lw[o];
+  return 0;
+}
+
+// Asynchronous online refresh (on != 0): a refresh's Adam steps run on a
+// second stream while the next solve proceeds with the current weights;
+// the solve after that adopts the new ones.  Off: the refresh runs in the
+// step's own stream and the next solve uses its result.
+int sphx_mlp_online_async(sphx_context* c, int on) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_online_async"));
+  c->mlp.online_async = on != 0;
   return 0;
 }
 

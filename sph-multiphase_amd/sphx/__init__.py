@@ -164,6 +164,7 @@ def lib() -> C.CDLL:
         L.sphx_mlp_download_samples.argtypes = [VP, I64, I64, VP, VP, VP]
         L.sphx_mlp_features.argtypes = [VP, VP, VP, VP]
         L.sphx_mlp_online.argtypes = [VP, I, I, I, C.c_float, VP]
+        L.sphx_mlp_online_async.argtypes = [VP, I]
         L.sphx_mlp_eval.argtypes = [VP, VP, I, VP]
         L.sphx_mlp_grad.argtypes = [VP, VP, VP, VP, I, VP, VP, P(D)]
         L.sphx_mlp_train.argtypes = [VP, I, I, C.c_float, VP, C.c_uint64, VP]
@@ -474,6 +475,9 @@ class Context:
     def mlp_online(self, every, iters=0, batch=0, lr=0.0, lw=(1.0, 1.0, 1.0, 1.0)):
         lw = np.ascontiguousarray(lw, np.float32)
         check(self.L.sphx_mlp_online(self.h, every, iters, batch, lr, ptr(lw)), "sphx_mlp_online")
+
+    def mlp_online_async(self, on=True):
+        check(self.L.sphx_mlp_online_async(self.h, int(bool(on))), "sphx_mlp_online_async")
 
     def mlp_eval(self, X):
         X = np.ascontiguousarray(X, np.float32)

@@ -206,3 +206,39 @@ def test_learned_warm_start_cuts_iterations(gpu):
         assert np.all(ea <= tol) and np.all(eb <= tol)
         assert st["learned"] == 15
         assert ia.mean() < ib.mean()
+
+
+def test_online_refresh_async_matches_sync(gpu):
+    """The asynchronous online refresh (Adam steps on a second stream) trains
+    the same weights as the in-stream refresh from the same samples; both
+    solves keep the tolerance while the refreshed weights are adopted.
+    Weights agree to 1e-5 (the weight gradients are summed with atomics, so
+    their order is not fixed)."""
+    p, disp, calc, alpha, o = setup()
+    P, U = scales(p, 0.16)
+    lw = np.array([1.0, 0.0, 0.0, 0.0], np.float32)
+    with sphx.Context(p, o) as A, sphx.Context(p, o) as B:
+        for c in (A, B):
+            c.upload(disp, calc)
+            c.mf_setup(RHO, NU)
+            c.mf_upload(alpha)
+            c.call("grid_build")
+            c.mlp_init(11, P, U)
+            c.mlp_reserve(4 * c.num)
+            c.mlp_mode(warm=True)
+            c.mlp_online(2, 3, 512, 1e-3, lw)
+        B.mlp_online_async(True)
+        w0 = A.mlp_weights()
+        ea = [A.mf_step()[1] for _ in range(2)]
+        eb = [B.mf_step()[1] for _ in range(2)]
+        wa, wb = A.mlp_weights(), B.mlp_weights()
+        d = np.max(np.abs(wa - wb)) / np.max(np.abs(wa))
+        print(f"refreshed weights: change {np.max(np.abs(wa - w0)):.3e}, async vs sync {d:.3e}",
+              flush=True)
+        assert np.max(np.abs(wa - w0)) > 0
+        assert d <= 1e-5
+        ea += [A.mf_step()[1] for _ in range(4)]
+        eb += [B.mf_step()[1] for _ in range(4)]
+        B.sync()
+        print("errors sync", np.round(ea, 6), "async", np.round(eb, 6), flush=True)
+        assert max(ea) <= o.tol and max(eb) <= o.tol
