@@ -545,6 +545,8 @@ def main():
     ap.add_argument("--online-iters", type=int, default=16)
     ap.add_argument("--online-batch", type=int, default=8192)
     ap.add_argument("--online-lr", type=float, default=5e-4)
+    ap.add_argument("--record-stride", type=int, default=8,
+                    help="online recording keeps every n-th particle of a step")
     ap.add_argument("--online-async", type=int, default=1,
                     help="refresh on a second stream, adopted two solves later")
     ap.add_argument("--workload", default="bubbles",
@@ -642,6 +644,7 @@ def main():
         learn.mlp_online(args.online_every, args.online_iters, args.online_batch, args.online_lr,
                          olw)
         learn.mlp_online_async(bool(args.online_async))
+        learn.mlp_record_stride(args.record_stride)
 
     el_plain, it_plain, err_plain, kt_plain = timed_segment(plain, args.steps, names)
     plain.close()
@@ -712,6 +715,7 @@ def main():
                                f"of {args.online_batch} samples (inside the timed steps"
                                + (", on a second stream, adopted two steps later)"
                                   if args.online_async else ")")
+                               + f"; recording every {args.record_stride}th particle of a step"
                                if args.online_every > 0 else None),
             "parallelism": "single GPU",
         },

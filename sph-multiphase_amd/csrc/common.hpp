@@ -267,6 +267,7 @@ struct sphx_context {
     // online refresh: every `online_every` solves, `online_iters` Adam steps
     // on batches drawn from the sample ring
     int online_every = 0, online_iters = 0, online_batch = 0, solves = 0;
+    int record_stride = 1;        // online recording keeps every stride-th slot
     float online_lr = 0.f, online_lw[4] = {1.f, 1.f, 1.f, 1.f};
     uint64_t online_seed = 0;
     // per particle, permuted with the particles: {drift correction xyz,

@@ -574,17 +574,20 @@ __global__ void k_f32_to_f16(const float* __restrict__ a, __half* __restrict__ b
 // features computed before the solve, the converged pressure increment and
 // (multiphase) the change of the phase-1 drift when it is evaluated with the
 // converged instead of the previous pressure.
-__global__ void k_mlp_record(KP k, MLPP mp, const float4* __restrict__ pos,
+// Sample t of the call is slot i = phase + t * stride (stride 1: every slot).
+__global__ void k_mlp_record(KP k, MLPP mp, int count, int stride, int phase,
+                             const float4* __restrict__ pos,
                              const uint32_t* __restrict__ key, const float4* __restrict__ phys,
                              const float* __restrict__ pprev, const float* __restrict__ X,
                              const float4* __restrict__ u_old, const float4* __restrict__ u_new,
                              float* __restrict__ Xs, float* __restrict__ Ts,
                              float* __restrict__ Ms) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= k.n) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int i = phase + t * stride;
   bool ok = type_of(pos[i]) != SPHX_TYPE_BOUNDARY && (int)key[i] != k.gridtotal;
   const float4* xs = reinterpret_cast<const float4*>(X + (size_t)i * MLP_NF);
-  float4* xd = reinterpret_cast<float4*>(Xs + (size_t)i * MLP_NF);
+  float4* xd = reinterpret_cast<float4*>(Xs + (size_t)t * MLP_NF);
 #pragma unroll
   for (int q = 0; q < MLP_NF / 4; ++q) xd[q] = xs[q];
   float t0 = 0.f, t1 = 0.f, t2 = 0.f, t3 = 0.f;
@@ -598,9 +601,9 @@ __global__ void k_mlp_record(KP k, MLPP mp, const float4* __restrict__ pos,
   }
   bool finite = t0 == t0 && t1 == t1 && t2 == t2 && t3 == t3 && fabsf(t0) < 1e30f;
   float m = ok && finite ? 1.f : 0.f;
-  reinterpret_cast<float4*>(Ts)[i] = m > 0.f ? make_float4(t0, t1, t2, t3)
+  reinterpret_cast<float4*>(Ts)[t] = m > 0.f ? make_float4(t0, t1, t2, t3)
                                              : make_float4(0.f, 0.f, 0.f, 0.f);
-  Ms[i] = m;
+  Ms[t] = m;
 }
 
 // p(t-1) when the features are not computed.
@@ -855,10 +858,18 @@ int mlp_after(sphx_context* c, int record_ok) {
   k_mlp_hist<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(k, c->POS(), c->key.p, c->PHYS(),
                                                           m.pprev.p, hist);
   SPHX_HIP(hipGetLastError());
+  // Online recording with a stride keeps every stride-th slot of a solve
+  // (the first slot rotating with the solve count) in a ring of as many
+  // solves as the store holds full solves.
+  const int stride = (online && !m.record) ? m.record_stride : 1;
+  const int phase = stride > 1 ? (int)(m.solves % stride) : 0;
+  const size_t nrec = stride > 1 ? (n - (size_t)phase + stride - 1) / stride : n;
+  const size_t cap = stride > 1 ? std::min(m.max_samples, (m.max_samples / n) * ((n + stride - 1) / stride))
+                                : m.max_samples;
   if (record && m.max_samples >= n) {
     size_t base = m.wpos;
-    if (base + n > m.max_samples) base = online ? 0 : m.max_samples;   // ring or full
-    if (base + n <= m.max_samples) {
+    if (base + nrec > cap) base = online ? 0 : cap;   // ring or full
+    if (base + nrec <= cap) {
       const float4* u_old = nullptr;
       const float4* u_new = nullptr;
       // drift targets cost one more drift pass: only when they are used
@@ -872,13 +883,13 @@ int mlp_after(sphx_context* c, int record_ok) {
       }
       if (m.train_inflight) SPHX_HIP(hipStreamWaitEvent(c->stream, m.ev_train, 0));
       timer_begin(c, "mlp_record");
-      k_mlp_record<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
-          k, mp, c->POS(), c->key.p, c->PHYS(), m.pprev.p, m.X.p, u_old, u_new,
-          m.sX.p + base * MLP_NF, m.sT.p + base * MLP_NO, m.sM.p + base);
+      k_mlp_record<<<blocks_for((int)nrec, 256), 256, 0, c->stream>>>(
+          k, mp, (int)nrec, stride, phase, c->POS(), c->key.p, c->PHYS(), m.pprev.p, m.X.p,
+          u_old, u_new, m.sX.p + base * MLP_NF, m.sT.p + base * MLP_NO, m.sM.p + base);
       timer_end(c);
       SPHX_HIP(hipGetLastError());
-      m.wpos = base + n;
-      m.nsamples = std::max(m.nsamples, m.wpos);
+      m.wpos = base + nrec;
+      m.nsamples = std::min(std::max(m.nsamples, m.wpos), cap);
     }
   }
   if (record_ok) {
@@ -1022,6 +1033,16 @@ int sphx_mlp_online(sphx_context* c, int every, int iters, int batch, float lr, 
 int sphx_mlp_online_async(sphx_context* c, int on) {
   SPHX_TRY(mlp_need(c, "sphx_mlp_online_async"));
   c->mlp.online_async = on != 0;
+  return 0;
+}
+
+// Online recording keeps every `stride`-th particle slot of a solve
+// (stride 1: all).  The ring then holds as many solves as the sample store
+// holds full solves, so its window in solves does not change.
+int sphx_mlp_record_stride(sphx_context* c, int stride) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_record_stride"));
+  SPHX_CHECK(stride >= 1, "sphx_mlp_record_stride", "stride must be >= 1");
+  c->mlp.record_stride = stride;
   return 0;
 }
 

@@ -165,6 +165,7 @@ def lib() -> C.CDLL:
         L.sphx_mlp_features.argtypes = [VP, VP, VP, VP]
         L.sphx_mlp_online.argtypes = [VP, I, I, I, C.c_float, VP]
         L.sphx_mlp_online_async.argtypes = [VP, I]
+        L.sphx_mlp_record_stride.argtypes = [VP, I]
         L.sphx_mlp_eval.argtypes = [VP, VP, I, VP]
         L.sphx_mlp_grad.argtypes = [VP, VP, VP, VP, I, VP, VP, P(D)]
         L.sphx_mlp_train.argtypes = [VP, I, I, C.c_float, VP, C.c_uint64, VP]
@@ -478,6 +479,9 @@ class Context:
 
     def mlp_online_async(self, on=True):
         check(self.L.sphx_mlp_online_async(self.h, int(bool(on))), "sphx_mlp_online_async")
+
+    def mlp_record_stride(self, stride):
+        check(self.L.sphx_mlp_record_stride(self.h, int(stride)), "sphx_mlp_record_stride")
 
     def mlp_eval(self, X):
         X = np.ascontiguousarray(X, np.float32)

@@ -242,3 +242,40 @@ def test_online_refresh_async_matches_sync(gpu):
         B.sync()
         print("errors sync", np.round(ea, 6), "async", np.round(eb, 6), flush=True)
         assert max(ea) <= o.tol and max(eb) <= o.tol
+
+
+def test_strided_online_recording(gpu):
+    """Online recording with stride 4 keeps slots phase + 4 t of each solve,
+    the phase rotating with the solve count, bit-identical to the same slots
+    of a full recording of the same solves."""
+    p, disp, calc, alpha, o = setup()
+    P, U = scales(p, 0.16)
+    lw = np.array([1.0, 0.0, 0.0, 0.0], np.float32)
+    R = 4
+    with sphx.Context(p, o) as A, sphx.Context(p, o) as B:
+        for c in (A, B):
+            c.upload(disp, calc)
+            c.mf_setup(RHO, NU)
+            c.mf_upload(alpha)
+            c.call("grid_build")
+            c.mlp_init(5, P, U)
+            c.mlp_reserve(3 * c.num)
+            c.mlp_mode(warm=True)
+            c.mlp_online(1000, 1, 64, 1e-3, lw)   # ring recording, no refresh yet
+        B.mlp_record_stride(R)
+        n = A.num
+        for _ in range(2):
+            A.mf_step()
+            B.mf_step()
+        ta, _ = A.mlp_samples()
+        tb, _ = B.mlp_samples()
+        XA, TA, MA = A.mlp_download_samples(0, ta)
+        XB, TB, MB = B.mlp_download_samples(0, tb)
+        n0 = (n + R - 1) // R
+        n1 = (n - 1 + R - 1) // R
+        print(f"n {n}: full {ta} samples, strided {tb} ({n0} + {n1})", flush=True)
+        assert ta == 2 * n and tb == n0 + n1
+        sel = np.concatenate([np.arange(0, n, R), n + np.arange(1, n, R)])
+        np.testing.assert_array_equal(XB, XA[sel])
+        np.testing.assert_array_equal(TB, TA[sel])
+        np.testing.assert_array_equal(MB, MA[sel])
