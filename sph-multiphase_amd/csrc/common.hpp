@@ -290,6 +290,7 @@ struct sphx_context {
     // weights that the step after that adopts
     int online_async = 0;
     int train_pending = 0;       // solves until the staged weights are adopted
+    bool train_launch = false;   // a refresh is due, its samples recorded (ev_main)
     bool train_inflight = false; // a refresh was launched and not yet waited for
     hipStream_t tstream = nullptr;
     hipEvent_t ev_main = nullptr, ev_train = nullptr;

@@ -657,7 +657,7 @@ static int mlp_need(sphx_context* c, const char* who) {
   SPHX_CHECK(c, who, "null context");
   SPHX_TRY(ensure_device(c));
   SPHX_CHECK(c->mlp.ready, who, "network not initialised (call sphx_mlp_init)");
-  if (c->mlp.train_inflight) SPHX_TRY(mlp_quiesce(c));
+  if (c->mlp.train_inflight || c->mlp.train_launch) SPHX_TRY(mlp_quiesce(c));
   return 0;
 }
 
@@ -732,6 +732,10 @@ static int mlp_refresh_launch(sphx_context* c, __half* w16) {
   return 0;
 }
 
+// Asynchronous refresh, at the end of the solve that recorded the samples:
+// mark the main stream's position; the Adam steps are launched on the
+// training stream by the next solve's start (launching them takes the host
+// longer than the device needs to reach that point of its queue).
 static int mlp_online_refresh(sphx_context* c) {
   auto& m = c->mlp;
   if (!m.online_async) return mlp_refresh_launch(c, reinterpret_cast<__half*>(m.w16.p));
@@ -742,6 +746,14 @@ static int mlp_online_refresh(sphx_context* c) {
   }
   SPHX_TRY(m.w16_stage.alloc(MLP_NP + 1));
   SPHX_HIP(hipEventRecord(m.ev_main, c->stream));
+  m.train_launch = true;
+  return 0;
+}
+
+static int mlp_async_launch(sphx_context* c) {
+  auto& m = c->mlp;
+  if (!m.train_launch) return 0;
+  m.train_launch = false;
   SPHX_HIP(hipStreamWaitEvent(m.tstream, m.ev_main, 0));
   hipStream_t main = c->stream;
   c->stream = m.tstream;
@@ -749,7 +761,7 @@ static int mlp_online_refresh(sphx_context* c) {
   c->stream = main;
   SPHX_TRY(rc);
   SPHX_HIP(hipEventRecord(m.ev_train, m.tstream));
-  m.train_pending = 2;
+  m.train_pending = 1;
   m.train_inflight = true;
   return 0;
 }
@@ -763,6 +775,7 @@ __global__ void k_u16_copy(const uint16_t* __restrict__ a, uint16_t* __restrict_
 // stream, then copies them in.
 int mlp_async_settle(sphx_context* c) {
   auto& m = c->mlp;
+  SPHX_TRY(mlp_async_launch(c));
   if (!m.train_inflight) return 0;
   SPHX_HIP(hipStreamWaitEvent(c->stream, m.ev_train, 0));
   k_u16_copy<<<blocks_for(MLP_NP, 256), 256, 0, c->stream>>>(m.w16_stage.p, m.w16.p, MLP_NP);
@@ -787,6 +800,7 @@ int mlp_begin(sphx_context* c) {
   auto& m = c->mlp;
   if (!m.ready || c->num == 0) return 0;
   if (m.train_pending > 0 && --m.train_pending == 0) SPHX_TRY(mlp_async_settle(c));
+  SPHX_TRY(mlp_async_launch(c));
   KP k = make_kp(c);
   MLPP mp = make_mlpp(c);
   const bool mf = c->nphase > 0;
@@ -940,7 +954,7 @@ int sphx_mlp_init(sphx_context* c, uint64_t seed, float p_scale, float u_scale) 
   SPHX_CHECK(c, "sphx_mlp_init", "null context");
   SPHX_CHECK(p_scale > 0.f && u_scale > 0.f, "sphx_mlp_init", "scales must be positive");
   SPHX_TRY(ensure_device(c));
-  if (c->mlp.train_inflight) SPHX_TRY(mlp_quiesce(c));
+  if (c->mlp.train_inflight || c->mlp.train_launch) SPHX_TRY(mlp_quiesce(c));
   auto& m = c->mlp;
   SPHX_TRY(m.w.alloc(MLP_NP));
   SPHX_TRY(m.m.alloc(MLP_NP));
