@@ -179,26 +179,17 @@ def bench_sand(args, sphx):
     ctx.sync()
     names = ["hash", "radix_sort", "cell_start", "reorder", "mpm_clear", "mpm_p2g",
              "mpm_gather", "mpm_stress", "mpm_force", "mpm_node", "mpm_g2p"]
-    ctx.timing(True)
-    ctx.timing_reset()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.call("mpm_substep")
-    ctx.sync()
-    elapsed = time.perf_counter() - t0
-    kt = {k: ctx.timing_get(k) for k in names}
+    substep = lambda: ctx.call("mpm_substep")   # noqa: E731
+    # profile segment (every kernel timed), then the timed steps with events
+    # around the dominant kernel only
+    _, _, _, kt_prof = timed_segment(ctx, args.profile_steps, names, step=substep)
+    dom = kernel_table(kt_prof, names, f"profile segment, {args.profile_steps} substeps")
+    elapsed, _, _, kt = timed_segment(ctx, args.steps, names, only=dom, step=substep)
     active = ctx.mpm_active_blocks()
     d, c = ctx.download()
     ok = bool(np.all(np.isfinite(d["pos"])) and np.all(np.isfinite(c["stress"])))
     ctx.close()
-    total_k = sum(t for t, _ in kt.values())
-    for k in names:
-        t, cnt = kt[k]
-        if cnt:
-            print(f"# {k:16s} {t / cnt * 1e3:9.1f} us/launch x {cnt:5d}  ({100 * t / total_k:5.1f}% of kernel time)",
-                  file=sys.stderr)
     nodes_active = active * 64
-    dom = max(kt, key=lambda k: kt[k][0])
     t_dom, n_dom = kt[dom]
     avg_ms = t_dom / max(n_dom, 1)
     units = (nodes_active if dom in ("mpm_node", "mpm_gather") else
@@ -508,23 +499,42 @@ def bench_ensemble(args, sphx):
     print(json.dumps(line), flush=True)
 
 
-def timed_segment(ctx, steps, names):
+def timed_segment(ctx, steps, names, only=None, step=None):
     """`steps` timed steps between two device synchronisations; returns the
-    wall time, per-step iterations and errors, and per-kernel event times."""
+    wall time, per-step iterations and errors, and per-kernel event times.
+    With `only`, HIP events bracket only that kernel's launches (the timed
+    steps then carry two events per launch of it, not of every kernel)."""
+    step = step or ctx.mf_step
     ctx.sync()
+    ctx.timing_only(only)
     ctx.timing(True)
     ctx.timing_reset()
     iters, errs = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
-        it, err = ctx.mf_step()
-        iters.append(it)
-        errs.append(err)
+        r = step()
+        if r is not None:
+            iters.append(r[0])
+            errs.append(r[1])
     ctx.sync()
     elapsed = time.perf_counter() - t0
     kt = {k: ctx.timing_get(k) for k in names}
     ctx.timing(False)
+    ctx.timing_only(None)
     return elapsed, iters, errs, kt
+
+
+def kernel_table(kt, names, label):
+    """Print the per-kernel event table of a profile segment; returns the
+    kernel with the largest total time."""
+    total_k = sum(t for t, n in kt.values()) or 1.0
+    print(f"# per-kernel HIP-event times ({label})", file=sys.stderr)
+    for k in names:
+        t, n = kt[k]
+        if n:
+            print(f"# {k:16s} {t / n * 1e3:9.1f} us/launch x {n:5d}  ({100 * t / total_k:5.1f}% of kernel time)",
+                  file=sys.stderr)
+    return max(kt, key=lambda k: kt[k][0])
 
 
 def main():
@@ -532,6 +542,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--profile-steps", type=int, default=40,
+                    help="steps with every kernel timed, before the timed steps")
     ap.add_argument("--n", type=int, default=1 << 20, help="fluid particles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=131072)
@@ -646,31 +658,34 @@ def main():
         learn.mlp_online_async(bool(args.online_async))
         learn.mlp_record_stride(args.record_stride)
 
-    el_plain, it_plain, err_plain, kt_plain = timed_segment(plain, args.steps, names)
+    el_plain, it_plain, err_plain, kt_plain = timed_segment(plain, args.steps, names,
+                                                            only="iisph_iter")
     plain.close()
     print(f"# plain: {1e3 * el_plain / args.steps:.3f} ms/step, mean pressure iterations "
           f"{np.mean(it_plain):.2f} (max {max(it_plain)}), max error {max(err_plain):.2e}",
           file=sys.stderr, flush=True)
-    elapsed, iters, errs, kt = timed_segment(learn, args.steps, names)
+    # the timed steps cover the same steps as the plain run's, with HIP
+    # events around the Jacobi pass only (the dominant kernel of this step);
+    # a profile segment with every kernel timed follows them
+    dom = "iisph_iter"
+    elapsed, iters, errs, kt = timed_segment(learn, args.steps, names, only=dom)
     stats = learn.mlp_stats()
+    _, it_prof, _, kt_prof = timed_segment(learn, args.profile_steps, names)
+    dom_prof = kernel_table(kt_prof, names, f"learned, profile segment of {args.profile_steps} "
+                                            f"steps after the timed steps, "
+                                            f"{np.mean(it_prof):.2f} iterations/step")
     learn.close()
     print(f"# learned: {1e3 * elapsed / args.steps:.3f} ms/step, mean pressure iterations "
           f"{np.mean(iters):.2f} (max {max(iters)}), max error {max(errs):.2e}, "
           f"fallbacks {stats['fallbacks']} of {stats['learned']}", file=sys.stderr, flush=True)
     print("# iterations plain   " + " ".join(map(str, it_plain)), file=sys.stderr)
     print("# iterations learned " + " ".join(map(str, iters)), file=sys.stderr, flush=True)
-    total_k = sum(t for t, n in kt.values())
-    dom = max(kt, key=lambda k: kt[k][0])
     t_dom, n_dom = kt[dom]
     avg_ms = t_dom / max(n_dom, 1)
     nf = info.n_fluid
     algo = ALGO_BYTES.get(dom, 0) * nf
     achieved = algo / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    for k in names:
-        t, n = kt[k]
-        if n:
-            print(f"# {k:16s} {t / n * 1e3:9.1f} us/launch x {n:5d}  ({100 * t / total_k:5.1f}% of kernel time)",
-                  file=sys.stderr)
+    print(f"# timed steps: {dom} {avg_ms * 1e3:.1f} us/launch x {n_dom}", file=sys.stderr)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -722,6 +737,7 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": dom,
+            "profile_dominant": dom_prof,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -542,6 +542,8 @@ int sphx_mlp_stats(sphx_context* ctx, int64_t* learned, int64_t* fallbacks,
 /* Kernel timing (HIP events on the context stream).                         */
 /* ------------------------------------------------------------------------ */
 int sphx_timing_enable(sphx_context* ctx, int on);
+/* Time only the named kernel (null or "": every kernel). */
+int sphx_timing_only(sphx_context* ctx, const char* kernel);
 int sphx_timing_reset(sphx_context* ctx);
 /* Accumulated device time (ms) and launch count of the named kernel. */
 int sphx_timing_get(sphx_context* ctx, const char* kernel, double* total_ms,

@@ -177,6 +177,9 @@ struct Timer {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
   std::vector<std::string> names;
   std::map<std::string, std::pair<double, int>> acc;
+  std::vector<hipEvent_t> pool;   // events reused across flushes
+  std::string only;               // time only this kernel (empty: all)
+  size_t ended = 0;               // pending entries whose end is recorded
 };
 
 }  // namespace sphx

@@ -65,11 +65,22 @@ int ensure_device(sphx_context* c) {
   return 0;
 }
 
+static bool timer_event(sphx_context* c, hipEvent_t* e) {
+  auto& pool = c->timer.pool;
+  if (!pool.empty()) {
+    *e = pool.back();
+    pool.pop_back();
+    return true;
+  }
+  return hipEventCreate(e) == hipSuccess;
+}
+
 void timer_begin(sphx_context* c, const char* name) {
   if (!c->timing) return;
+  if (!c->timer.only.empty() && c->timer.only != name) return;
   hipEvent_t a, b;
-  if (hipEventCreate(&a) != hipSuccess) return;
-  if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
+  if (!timer_event(c, &a)) return;
+  if (!timer_event(c, &b)) { c->timer.pool.push_back(a); return; }
   (void)hipEventRecord(a, c->stream);
   c->timer.pending.push_back({a, b});
   c->timer.names.push_back(name);
@@ -77,7 +88,9 @@ void timer_begin(sphx_context* c, const char* name) {
 
 void timer_end(sphx_context* c) {
   if (!c->timing || c->timer.pending.empty()) return;
+  if (c->timer.pending.size() == c->timer.ended) return;   // its begin was filtered out
   (void)hipEventRecord(c->timer.pending.back().second, c->stream);
+  c->timer.ended = c->timer.pending.size();
 }
 
 int timer_flush(sphx_context* c) {
@@ -91,11 +104,12 @@ int timer_flush(sphx_context* c) {
       e.first += ms;
       e.second += 1;
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    c->timer.pool.push_back(a);
+    c->timer.pool.push_back(b);
   }
   c->timer.pending.clear();
   c->timer.names.clear();
+  c->timer.ended = 0;
   return 0;
 }
 
@@ -338,6 +352,8 @@ int sphx_destroy(sphx_context* c) {
   }
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)timer_flush(c);
+  for (hipEvent_t e : c->timer.pool) (void)hipEventDestroy(e);
+  c->timer.pool.clear();
   (void)sphx_graph_release(c);
   for (int b = 0; b < 2; ++b) {
     c->pos[b].release(); c->vel[b].release(); c->veleval[b].release();
@@ -529,6 +545,14 @@ int sphx_timing_enable(sphx_context* c, int on) {
   SPHX_CHECK(c, "sphx_timing_enable", "null context");
   SPHX_TRY(timer_flush(c));
   c->timing = on != 0;
+  return 0;
+}
+
+// Time only the named kernel (null or empty: every kernel).
+int sphx_timing_only(sphx_context* c, const char* kernel) {
+  SPHX_CHECK(c, "sphx_timing_only", "null context");
+  SPHX_TRY(timer_flush(c));
+  c->timer.only = kernel ? kernel : "";
   return 0;
 }
 

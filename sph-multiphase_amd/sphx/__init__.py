@@ -133,6 +133,7 @@ def lib() -> C.CDLL:
         L.sphx_upload_iisph.argtypes = [VP] * 6
         L.sphx_timing_enable.argtypes = [VP, I]
         L.sphx_timing_reset.argtypes = [VP]
+        L.sphx_timing_only.argtypes = [VP, C.c_char_p]
         L.sphx_timing_get.argtypes = [VP, C.c_char_p, P(D), P(I)]
         L.sphx_setup_grid.argtypes = [P(ParamCarrier)]
         L.sphx_setup_grid.restype = None
@@ -583,6 +584,10 @@ class Context:
 
     def timing_reset(self):
         check(self.L.sphx_timing_reset(self.h), "timing_reset")
+
+    def timing_only(self, name=None):
+        """Time only the named kernel (None: every kernel)."""
+        check(self.L.sphx_timing_only(self.h, name.encode() if name else None), "timing_only")
 
     def timing_get(self, name: str):
         t, n = C.c_double(), C.c_int()
