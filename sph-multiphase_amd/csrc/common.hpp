@@ -236,6 +236,10 @@ struct sphx_context {
   float mf_rho[3] = {0, 0, 0}, mf_nu[3] = {0, 0, 0};
   sphx::DevBuf<float4> mf_alpha[2], mf_force[2];  // permuted with particles
   sphx::DevBuf<float4> mf_alpha_new, mf_drift[3], mf_amf;
+  // drift-stress sum of the last alpha pass; valid while its inputs
+  // (positions, densities, drifts, fractions before the step) are unchanged
+  sphx::DevBuf<float4> mf_stress;
+  bool mf_stress_ok = false;
   // packed neighbour records of the drift-flux loops ({x, y, z, m / rho},
   // {alpha, p}); mf_rec_ok while positions and densities are unchanged
   sphx::DevBuf<float4> mf_rec, mf_reca;
@@ -346,6 +350,7 @@ inline int blocks_for(int n, int threads) { return (n + threads - 1) / threads; 
 inline void positions_changed(sphx_context* c) {
   c->nbr_valid = false;
   c->gh_valid = false;
+  c->mf_stress_ok = false;
   c->mf_rec_ok = false;
 }
 

@@ -790,6 +790,7 @@ int sphx_iisph_density(sphx_context* c) {
   KP k = make_kp(c);
   if (k.n == 0) return 0;
   c->mf_rec_ok = false;   // densities change
+  c->mf_stress_ok = false;
   NbrList L;
   SPHX_TRY(nbr_list(c, &L));
   timer_begin(c, "iisph_density");

@@ -149,14 +149,18 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_drift(
 }
 
 // mfComputeAlphaAdvance (multiphase.cu:169-272); advects with the particle
-// velocity `vel` (the spec's mveleval).
+// velocity `vel` (the spec's mveleval).  The same neighbour pass also sums
+// the drift-stress acceleration of contributeForce_projectu (:555-562),
+// which reads a subset of its records (positions, V_j, drifts and the
+// fractions before the step) and none of its outputs; it is stored in
+// `stress` for the force stage, which then only adds it (and buoyancy).
 __global__ __launch_bounds__(MF_THREADS) void k_mf_alpha(
     KP k, MFP m, NbrList L, const uint32_t* __restrict__ key, const int* __restrict__ cs,
     const float4* __restrict__ pos, const float4* __restrict__ vel,
     const float4* __restrict__ phys, const float4* __restrict__ alpha,
     const float4* __restrict__ drift0, const float4* __restrict__ drift1,
     const float4* __restrict__ drift2, const float4* __restrict__ rec,
-    float4* __restrict__ alpha_new) {
+    float4* __restrict__ alpha_new, float4* __restrict__ stress) {
   int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
@@ -167,6 +171,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_alpha(
   }
   const float4 vi = vel[i];
   const float4 ai = alpha[i];
+  const float idens = 1.f / phys[i].y;
   const float4* dr[3] = {drift0, drift1, drift2};
   float apre[3], ach[3];
   float3 ui[3];
@@ -175,6 +180,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_alpha(
     ach[f] = 0.f;
     ui[f] = f < m.nphase ? xyz(dr[f][i]) : f3(0.f, 0.f, 0.f);
   }
+  float fx = 0.f, fy = 0.f, fz = 0.f;
   struct Rec { float4 r, v, u[3]; };
   const bool three = m.nphase > 2;
   for_each_listed_b(k, L, cs, gc, i, [&](int j) {
@@ -206,8 +212,16 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_alpha(
       float vky = ajf * uj.y + apre[f] * ui[f].y;
       float vkz = ajf * uj.z + apre[f] * ui[f].z;
       ach[f] -= cm * (vkx * dx + vky * dy + vkz * dz);                     // :209-212
+      // drift stress (:555-562)
+      float sj = cm * (uj.x * dx + uj.y * dy + uj.z * dz) * uj.w;
+      float si = cm * (ui[f].x * dx + ui[f].y * dy + ui[f].z * dz) * apre[f];
+      float w = m.rho[f] * idens;
+      fx += (sj * uj.x + si * ui[f].x) * w;
+      fy += (sj * uj.y + si * ui[f].y) * w;
+      fz += (sj * uj.z + si * ui[f].z) * w;
     }
   });
+  stress[i] = make_float4(fx, fy, fz, 0.f);
   float o[3];
   for (int f = 0; f < 3; ++f) {
     float d = ach[f] * k.dt;
@@ -318,6 +332,27 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_force(
   amf[i] = make_float4(fx, fy, fz, 0.f);
 }
 
+// The force stage after a fused alpha pass: drift stress from `stress`,
+// plus buoyancy; same operations and order as k_mf_force's tail.
+__global__ void k_mf_force_add(KP k, MFP m, const float4* __restrict__ pos,
+                               const uint32_t* __restrict__ key,
+                               const float4* __restrict__ alpha_new,
+                               const float4* __restrict__ stress,
+                               float4* __restrict__ veleval, float4* __restrict__ amf) {
+  int i = xcd_index();
+  if (i >= k.n) return;
+  if (type_of(pos[i]) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) return;
+  const float4 st = stress[i];
+  float fx = st.x, fy = st.y, fz = st.z;
+  if (m.boussinesq) {
+    float b = (alpha_new[i].w - m.rho_ref) / m.rho_ref;
+    fx += b * k.g.x; fy += b * k.g.y; fz += b * k.g.z;
+  }
+  float4 ve = veleval[i];
+  veleval[i] = make_float4(ve.x + fx * k.dt, ve.y + fy * k.dt, ve.z + fz * k.dt, ve.w);
+  amf[i] = make_float4(fx, fy, fz, 0.f);
+}
+
 __global__ void k_mf_reorder(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ idsort,
                              int n, uint32_t sentinel, const float4* __restrict__ a_in,
                              float4* __restrict__ a_out, const float4* __restrict__ f_in,
@@ -352,6 +387,7 @@ __global__ void k_mf_copy(const float4* __restrict__ a, float4* __restrict__ b, 
 // on the particle-buffer parity, so a captured step graph stays valid.
 int mf_commit(sphx_context* c) {
   if (!c->mf_pending) return 0;
+  c->mf_stress_ok = false;   // the fractions before the step change
   if (c->num > 0) {
     k_mf_copy<<<blocks_for(c->num, 256), 256, 0, c->stream>>>(c->mf_alpha_new.p,
                                                               c->mf_alpha[c->cur].p, c->num);
@@ -437,6 +473,8 @@ int sphx_mf_setup(sphx_context* c, int nphase, const float* rho, const float* nu
   }
   SPHX_TRY(c->mf_alpha_new.alloc(cap));
   SPHX_TRY(c->mf_amf.alloc(cap));
+  SPHX_TRY(c->mf_stress.alloc(cap));
+  c->mf_stress_ok = false;
   for (int f = 0; f < 3; ++f) SPHX_TRY(c->mf_drift[f].alloc(cap));
   SPHX_TRY(c->mf_rec.alloc(cap));
   SPHX_TRY(c->mf_reca.alloc(cap));
@@ -467,6 +505,7 @@ int sphx_mf_upload(sphx_context* c, const float* alpha, int start, int count) {
   SPHX_CHECK(start >= 0 && count >= 0 && start + count <= c->num, "sphx_mf_upload",
              "range exceeds num");
   SPHX_TRY(ensure_device(c));
+  c->mf_stress_ok = false;
   SPHX_HIP(hipStreamSynchronize(c->stream));
   SPHX_TRY(mf_commit(c));
   if (count == 0) return 0;
@@ -532,6 +571,7 @@ int sphx_mf_download(sphx_context* c, float* alpha, float* rho_m, float* drift, 
 int sphx_mf_drift_velocity(sphx_context* c) {
   SPHX_TRY(need_mf(c, "sphx_mf_drift_velocity"));
   SPHX_TRY(mf_commit(c));
+  c->mf_stress_ok = false;
   if (c->num == 0) return 0;
   const float4* dcorr =
       c->mlp.ready && c->mlp.drift ? c->mlp.hist[c->cur].p : nullptr;
@@ -553,9 +593,11 @@ int sphx_mf_alpha_advance(sphx_context* c) {
   timer_begin(c, "mf_alpha");
   k_mf_alpha<<<blocks_for(k.n, c->tune[T_MF_ALPHA]), c->tune[T_MF_ALPHA], 0, c->stream>>>(
       k, m, L, c->key.p, c->cell_start.p, c->POS(), c->VEL(), c->PHYS(), c->mf_alpha[c->cur].p,
-      c->mf_drift[0].p, c->mf_drift[1].p, c->mf_drift[2].p, c->mf_rec.p, c->mf_alpha_new.p);
+      c->mf_drift[0].p, c->mf_drift[1].p, c->mf_drift[2].p, c->mf_rec.p, c->mf_alpha_new.p,
+      c->mf_stress.p);
   timer_end(c);
   SPHX_HIP(hipGetLastError());
+  c->mf_stress_ok = true;
   return 0;
 }
 
@@ -583,11 +625,18 @@ int sphx_mf_force(sphx_context* c) {
   NbrList L;
   SPHX_TRY(nbr_list(c, &L));
   const float4* an = c->mf_pending ? c->mf_alpha_new.p : c->mf_alpha[c->cur].p;
-  SPHX_TRY(mf_need_rec(c));
   timer_begin(c, "mf_force");
-  k_mf_force<<<blocks_for(k.n, c->tune[T_MF_FORCE]), c->tune[T_MF_FORCE], 0, c->stream>>>(
-      k, m, L, c->key.p, c->cell_start.p, c->POS(), c->PHYS(), c->mf_alpha[c->cur].p, an,
-      c->mf_drift[0].p, c->mf_drift[1].p, c->mf_drift[2].p, c->mf_rec.p, c->VELEVAL(), c->mf_amf.p);
+  if (c->mf_stress_ok) {
+    // the alpha pass summed the drift stress from these same inputs
+    k_mf_force_add<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+        k, m, c->POS(), c->key.p, an, c->mf_stress.p, c->VELEVAL(), c->mf_amf.p);
+  } else {
+    SPHX_TRY(mf_need_rec(c));
+    k_mf_force<<<blocks_for(k.n, c->tune[T_MF_FORCE]), c->tune[T_MF_FORCE], 0, c->stream>>>(
+        k, m, L, c->key.p, c->cell_start.p, c->POS(), c->PHYS(), c->mf_alpha[c->cur].p, an,
+        c->mf_drift[0].p, c->mf_drift[1].p, c->mf_drift[2].p, c->mf_rec.p, c->VELEVAL(),
+        c->mf_amf.p);
+  }
   timer_end(c);
   SPHX_HIP(hipGetLastError());
   return 0;

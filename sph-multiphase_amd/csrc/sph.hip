@@ -214,6 +214,7 @@ int sphx_boundary_volume(sphx_context* c) {
 
 int sphx_wcsph_density_pressure(sphx_context* c) {
   SPHX_TRY(need_sorted(c, "sphx_wcsph_density_pressure"));
+  c->mf_stress_ok = false;   // phys.y changes
   KP k = make_kp(c);
   if (k.n == 0) return 0;
   timer_begin(c, "wcsph_density");

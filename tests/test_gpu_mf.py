@@ -127,3 +127,38 @@ def test_mf_steps_keep_fractions_and_converge(gpu):
     y_air = np.average(d["pos"][fl, 1], weights=np.maximum(a[fl, 1], 0) + 1e-30)
     print("iterations", its, "air centroid", y_air0, "->", y_air)
     assert y_air > y_air0
+
+
+def test_force_from_fused_alpha_pass_equals_own_pass(gpu):
+    """The force stage adds the drift stress the fraction-advection pass
+    summed; when anything it depends on changed since (here: densities
+    recomputed, to the same values), it runs its own neighbour pass.  Both
+    give the same mixture acceleration (1e-6 normwise: the compiler may
+    contract the same sums into FMAs differently in the two kernels)."""
+    p, disp, calc, alpha, o = setup()
+    out = []
+    for redo_density in (False, True):
+        with sphx.Context(p, o) as ctx:
+            ctx.upload(disp, calc)
+            ctx.mf_setup(RHO, NU)
+            ctx.mf_upload(alpha)
+            ctx.call("grid_build")
+            for _ in range(3):
+                ctx.mf_step()
+            ctx.call("grid_build")
+            ctx.call("iisph_density")
+            for s in ["mf_drift_velocity", "mf_alpha_advance", "mf_correction"]:
+                ctx.call(s)
+            if redo_density:
+                ctx.call("iisph_density")
+            ctx.call("iisph_predict_dii")
+            d0, c0 = ctx.download()
+            ctx.call("mf_force")
+            d1, c1 = ctx.download()
+            out.append((c1["veleval"].astype(np.float64) - c0["veleval"]) / p.dt)
+            fl = (d1["type"] == 0)
+    a, b = out
+    err = np.max(np.abs(a[fl] - b[fl])) / np.max(np.abs(b[fl]))
+    print(f"fused vs own pass: {err:.3e} (max |a| {np.max(np.abs(b[fl])):.3e})", flush=True)
+    assert np.max(np.abs(b[fl])) > 0
+    assert err <= 1e-6
