@@ -525,6 +525,10 @@ int sphx_mlp_download_samples(sphx_context* ctx, int64_t start, int64_t count,
  * previous pressures [num] and each particle's pressure increment of the
  * last solve [num] (feature 20); any may be NULL. */
 int sphx_mlp_features(sphx_context* ctx, float* X, float* pprev, float* dp_last);
+/* The feature buffer as the last pass left it: the a_ii pass fills it when
+ * the network is in use (warm start or recording); sphx_mlp_features
+ * recomputes it. */
+int sphx_mlp_download_features(sphx_context* ctx, float* X, float* pprev);
 /* The network with FP16 weights and FP32 accumulation: Y [n][4]. */
 int sphx_mlp_eval(sphx_context* ctx, const float* X, int n, float* Y);
 /* Loss (1/n_valid) sum M_s sum_o lw_o (y_so - T_so)^2 and its gradient

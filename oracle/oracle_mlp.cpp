@@ -81,7 +81,7 @@ void orc_mlp_features(const sphx_ParamCarrier* P, const OrcState* s, const float
         wp += w * pj;
         pmax = std::max(pmax, pj);
         pmin = std::min(pmin, pj);
-        wsrc += w * (F(s->rho_adv, j) - F(s->restdens, j)) / F(s->restdens, j);
+        wsrc += w * (F(s->dens, j) - F(s->restdens, j)) / F(s->restdens, j);
         double vj = F(s->mass, j) / F(s->dens, j) * nabla_fac(r, h, ks) * (pj - pi);
         for (int q = 0; q < 3; ++q) gp[q] += vj * d[q];
       }

@@ -295,6 +295,7 @@ struct sphx_context {
     // asynchronous online refresh: the Adam steps run on their own stream
     // (default priority) while the next step is solved, into staged FP16
     // weights that the step after that adopts
+    bool feat_ok = false;        // X and pprev hold this step's features (a_ii pass)
     int online_async = 0;
     int train_pending = 0;       // solves until the staged weights are adopted
     bool train_launch = false;   // a refresh is due, its samples recorded (ev_main)
@@ -352,6 +353,7 @@ inline void positions_changed(sphx_context* c) {
   c->gh_valid = false;
   c->mf_stress_ok = false;
   c->mf_rec_ok = false;
+  c->mlp.feat_ok = false;
 }
 
 }  // namespace sphx

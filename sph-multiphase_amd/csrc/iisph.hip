@@ -17,7 +17,7 @@
 //      residue = rho_adv + S_i + a_ii p_i(old) - rho0,   s_i <- m_i p_i(new)/rho_i^2
 // B touches p only at its own index, so the update is in place and the
 // reference's press_l1 -> press_l copy (host.cu:536) is not needed.
-#include "nbr.hpp"
+#include "mlp_feat.hpp"
 
 #include <cmath>
 
@@ -118,34 +118,55 @@ __global__ void k_ii_dii(KP k, NbrList L, const uint32_t* __restrict__ key, cons
   dii[i] = make_float4(dx_ * f, dy_ * f, dz_ * f, 0.f);
 }
 
-// ComputeAii (kern.cu:801-908): rho_adv, warm start p0 = w p(t-1), a_ii.
-__global__ void k_ii_aii(KP k, NbrList L, const uint32_t* __restrict__ key, const int* __restrict__ cs,
+// ComputeAii (kern.cu:801-908): rho_adv, warm start p0 = w p(t-1), a_ii;
+// also the position-only sums G_i, H_i of the pressure pass, and, with
+// FEAT, the learned start's 24 features (the same map as k_mlp_features,
+// from the same neighbour records).
+struct FeatOut {
+  MLPP mp;
+  const float4* alpha;   // fractions before the step, or null (one phase)
+  const float4* hist;    // per particle: last solve's pressure increment in w
+  float* X;              // [n][24]
+  float* pprev;          // p(t-1)
+};
+
+template <bool FEAT>
+__global__ __launch_bounds__(NB_THREADS) void k_ii_aii(KP k, NbrList L, const uint32_t* __restrict__ key, const int* __restrict__ cs,
                          const float4* __restrict__ pos, const float4* __restrict__ veleval,
                          const float4* __restrict__ phys, float4* __restrict__ dii,
                          float* __restrict__ rho_adv, float* __restrict__ press_l,
-                         float4* __restrict__ ps, float4* __restrict__ gh) {
+                         float4* __restrict__ ps, float4* __restrict__ gh, FeatOut fo) {
   int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
-  if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
   uint32_t gc = key[i];
-  if ((int)gc == k.gridtotal) return;
+  if (type_of(pi) == SPHX_TYPE_BOUNDARY || (int)gc == k.gridtotal) {
+    if (FEAT) {
+      fo.pprev[i] = phys[i].z;
+      float* x = fo.X + (size_t)i * MLP_NF;
+      for (int q = 0; q < MLP_NF; ++q) x[q] = 0.f;
+    }
+    return;
+  }
   const float4 vi = veleval[i];
   const float4 qi = phys[i];
   const float rest = qi.w, mi = qi.x, rhoi = qi.y;
   const float4 di = dii[i];
   const float cji = mi / rhoi / rhoi * k.dt * k.dt;
+  const float p_i = fmaxf(0.f, qi.z);
   float dchange = 0.f, aii = 0.f;
   float Gx = 0.f, Gy = 0.f, Gz = 0.f, H = 0.f;
+  FeatAcc fa(p_i);
   struct Rec { float4 p, q, v; };
   for_each_listed_b<NBR_BATCH, false>(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j], veleval[j]}; },
                     [&](int, const Rec& o) {
     const float4 pj = o.p, qj = o.q, vj = o.v;
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
-    if (!in_support(k, dsq)) return;
     float r, ir;
     dist_rinv(dsq, r, ir);
+    if (FEAT) fa.add(k, dx, dy, dz, r, ir, pj, qj, p_i);
+    if (!in_support(k, dsq)) return;
     float gf = cubic_gradf(r, ir, k.inv_h, k.kspline);
     float gx = dx * gf, gy = dy * gf, gz = dz * gf;
     float vdg = (vi.x - vj.x) * gx + (vi.y - vj.y) * gy + (vi.z - vj.z) * gz;
@@ -164,11 +185,17 @@ __global__ void k_ii_aii(KP k, NbrList L, const uint32_t* __restrict__ key, cons
     H += (gx * gx + gy * gy + gz * gz) * wh;
   });
   gh[i] = make_float4(Gx, Gy, Gz, H);
-  rho_adv[i] = rhoi + dchange;
+  const float radv = rhoi + dchange;
+  rho_adv[i] = radv;
   float p0 = k.warm * qi.z;
   press_l[i] = p0;
   ps[i].w = mi * p0 / rhoi / rhoi;
   dii[i].w = aii;
+  if (FEAT) {
+    fo.pprev[i] = qi.z;
+    feat_write(k, fo.mp, fa, qi, radv, di, aii, vi, fo.alpha, fo.hist, i,
+               fo.X + (size_t)i * MLP_NF);
+  }
 }
 
 // The position-only sums of the a_ii pass, G_i = sum_j m_j gradW_ij (plus
@@ -821,12 +848,30 @@ int sphx_iisph_predict_aii(sphx_context* c) {
   if (k.n == 0) return 0;
   NbrList L;
   SPHX_TRY(nbr_list(c, &L));
+  // the learned start's features come from this pass when the network is
+  // in use (the feature stage then finds them ready)
+  auto& m = c->mlp;
+  const bool feat = m.ready && (m.warm || m.record || m.online_every > 0);
+  FeatOut fo{};
+  if (feat) {
+    SPHX_TRY(m.X.alloc((size_t)c->cap * MLP_NF));
+    SPHX_TRY(m.pprev.alloc((size_t)c->cap));
+    fo = FeatOut{MLPP{m.p_scale, m.u_scale, m.drift_blend},
+                 c->nphase > 0 ? c->mf_alpha[c->cur].p : nullptr, m.hist[c->cur].p, m.X.p,
+                 m.pprev.p};
+  }
   timer_begin(c, "iisph_aii");
-  k_ii_aii<<<blocks_for(k.n, c->tune[T_AII]), c->tune[T_AII], 0, c->stream>>>(
-      k, L, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p, c->rho_adv.p,
-      c->press_l.p, c->ps.p, c->gh.p);
+  if (feat)
+    k_ii_aii<true><<<blocks_for(k.n, c->tune[T_AII]), c->tune[T_AII], 0, c->stream>>>(
+        k, L, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p,
+        c->rho_adv.p, c->press_l.p, c->ps.p, c->gh.p, fo);
+  else
+    k_ii_aii<false><<<blocks_for(k.n, c->tune[T_AII]), c->tune[T_AII], 0, c->stream>>>(
+        k, L, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p,
+        c->rho_adv.p, c->press_l.p, c->ps.p, c->gh.p, fo);
   timer_end(c);
   c->gh_valid = true;
+  m.feat_ok = feat;
   SPHX_HIP(hipGetLastError());
   return 0;
 }

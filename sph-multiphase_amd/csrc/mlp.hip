@@ -14,7 +14,7 @@
 // sample (FP32), k_mlp_wgrad reduces the weight gradients over the batch
 // in LDS tiles, k_mlp_adam applies Adam and refreshes the FP16 copy the
 // inference kernel reads (FP16 weights, FP32 accumulation).
-#include "nbr.hpp"
+#include "mlp_feat.hpp"
 
 #include <hip/hip_fp16.h>
 
@@ -23,20 +23,13 @@
 
 namespace sphx {
 
-constexpr int MLP_NF = 24, MLP_H = 64, MLP_NO = 4;
 constexpr int MLP_W1 = 0, MLP_B1 = MLP_NF * MLP_H, MLP_W2 = MLP_B1 + MLP_H,
               MLP_B2 = MLP_W2 + MLP_H * MLP_H, MLP_W3 = MLP_B2 + MLP_H,
               MLP_B3 = MLP_W3 + MLP_H * MLP_NO, MLP_NP = MLP_B3 + MLP_NO;   // 6020
-constexpr float MLP_TSCALE = 10.f;   // target = (p* - p) / P * 10
-
-struct MLPP {
-  float p_scale;      // P (Pa)
-  float u_scale;      // drift normalisation (m/s)
-  float drift_blend;  // weight of the drift correction applied next step
-};
 
 // ------------------------------------------------------------- features --
-// Same map as the oracle's orc_mlp_features (24 values per fluid particle).
+// Stand-alone feature pass (the a_ii pass computes the same map in its own
+// neighbour loop when the network is in use).
 __global__ __launch_bounds__(256) void k_mlp_features(
     KP k, NbrList L, MLPP mp, const uint32_t* __restrict__ key, const int* __restrict__ cs,
     const float4* __restrict__ pos, const float4* __restrict__ veleval,
@@ -54,63 +47,20 @@ __global__ __launch_bounds__(256) void k_mlp_features(
     return;
   }
   const float4 qi = phys[i];
-  const float rho0 = qi.w, rho = qi.y, radv = rho_adv[i];
   const float4 di = dii[i];
-  const float aii = di.w;
   const float p_i = fmaxf(0.f, qi.z);
-  const float gn = sqrtf(k.g.x * k.g.x + k.g.y * k.g.y + k.g.z * k.g.z);
-  const float3 gh = gn > 0.f ? f3(k.g.x / gn, k.g.y / gn, k.g.z / gn) : f3(0.f, -1.f, 0.f);
-  float wsum = 0.f, wp = 0.f, pmax = p_i, pmin = p_i, wsrc = 0.f, gpx = 0.f, gpy = 0.f,
-        gpz = 0.f;
-  int nf = 0, nb = 0;
-  struct Rec { float4 p, q; float ra; };
-  for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j], rho_adv[j]}; },
+  FeatAcc fa(p_i);
+  struct Rec { float4 p, q; };
+  for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j]}; },
                     [&](int, const Rec& o) {
     const float4 pj = o.p;
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
     float r, ir;
     dist_rinv(dsq, r, ir);
-    if (!(r < 2.f * k.h && r > 0.f)) return;
-    if (type_of(pj) == SPHX_TYPE_BOUNDARY) { ++nb; return; }
-    ++nf;
-    const float4 qj = o.q;
-    float w = cubic_w(r, k.inv_h, k.kspline);
-    float pjp = fmaxf(0.f, qj.z);
-    wsum += w;
-    wp += w * pjp;
-    pmax = fmaxf(pmax, pjp);
-    pmin = fminf(pmin, pjp);
-    wsrc += w * (o.ra - qj.w) / qj.w;
-    float v = qj.x / qj.y * cubic_gradf(r, ir, k.inv_h, k.kspline) * (pjp - p_i);
-    gpx += v * dx; gpy += v * dy; gpz += v * dz;
+    fa.add(k, dx, dy, dz, r, ir, pj, o.q, p_i);
   });
-  const float P = mp.p_scale;
-  float pav = wsum > 0.f ? wp / wsum : p_i;
-  float pj_est = fabsf(aii) > 1e-12f ? fmaxf(0.f, (rho0 - radv) / aii) : 0.f;
-  x[0] = p_i / P;
-  x[1] = (radv - rho0) / rho0 * 100.f;
-  x[2] = (rho - rho0) / rho0 * 100.f;
-  x[3] = -aii * P / rho0;
-  x[4] = pj_est / P;
-  x[5] = pav / P;
-  x[6] = pmax / P;
-  x[7] = pmin / P;
-  x[8] = (pav - p_i) / P;
-  x[9] = nf / 64.f;
-  x[10] = nb / 64.f;
-  float4 al = alpha ? alpha[i] : make_float4(1.f, 0.f, 0.f, rho0);
-  x[11] = alpha ? al.y : 0.f;
-  x[12] = alpha ? al.w / rho0 : 1.f;
-  x[13] = di.x * P / k.h; x[14] = di.y * P / k.h; x[15] = di.z * P / k.h;
-  float4 v = veleval[i];
-  x[16] = v.x * k.dt / k.h; x[17] = v.y * k.dt / k.h; x[18] = v.z * k.dt / k.h;
-  float den = rho0 * (gn > 0.f ? gn : 1.f);
-  x[19] = (gpx * gh.x + gpy * gh.y + gpz * gh.z) / den;
-  x[20] = hist[i].w / P * MLP_TSCALE;   // pressure increment of the last solve
-  x[21] = gpz / den;
-  x[22] = wsum > 0.f ? wsrc / wsum * 100.f : 0.f;
-  x[23] = 1.f;
+  feat_write(k, mp, fa, qi, rho_adv[i], di, di.w, veleval[i], alpha, hist, i, x);
 }
 
 // Hidden units per chunk of the training kernels' fused layers.
@@ -808,7 +758,7 @@ int mlp_begin(sphx_context* c) {
   SPHX_TRY(m.X.alloc((size_t)c->cap * MLP_NF));
   SPHX_TRY(m.pprev.alloc((size_t)c->cap));
   float4* hist = m.hist[c->cur].p;
-  if (m.warm || record) {
+  if ((m.warm || record) && !m.feat_ok) {
     NbrList L;
     SPHX_TRY(nbr_list(c, &L));
     timer_begin(c, "mlp_features");
@@ -816,9 +766,10 @@ int mlp_begin(sphx_context* c) {
         k, L, mp, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p,
         c->rho_adv.p, mf ? c->mf_alpha[c->cur].p : nullptr, hist, m.X.p, m.pprev.p);
     timer_end(c);
-  } else {
+  } else if (!(m.warm || record)) {
     k_mlp_pprev<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(k.n, c->PHYS(), m.pprev.p);
   }
+  m.feat_ok = false;   // used once
   SPHX_HIP(hipGetLastError());
   float4* dcorr = m.drift && mf ? hist : nullptr;
   if (m.warm || dcorr) {
@@ -1147,6 +1098,18 @@ int sphx_mlp_features(sphx_context* c, float* X, float* pprev, float* dp_last) {
   }
   if (X) SPHX_HIP(hipMemcpy(X, m.X.p, (size_t)k.n * MLP_NF * sizeof(float), hipMemcpyDeviceToHost));
   if (pprev) SPHX_HIP(hipMemcpy(pprev, m.pprev.p, (size_t)k.n * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// The feature buffer as the last pass left it (the a_ii pass computes the
+// features when the network is in use; sphx_mlp_features recomputes them).
+int sphx_mlp_download_features(sphx_context* c, float* X, float* pprev) {
+  SPHX_TRY(mlp_need(c, "sphx_mlp_download_features"));
+  SPHX_CHECK(c->mlp.X.p && c->mlp.pprev.p, "sphx_mlp_download_features", "no features yet");
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  const size_t n = (size_t)c->num;
+  if (X) SPHX_HIP(hipMemcpy(X, c->mlp.X.p, n * MLP_NF * sizeof(float), hipMemcpyDeviceToHost));
+  if (pprev) SPHX_HIP(hipMemcpy(pprev, c->mlp.pprev.p, n * sizeof(float), hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -166,6 +166,7 @@ def lib() -> C.CDLL:
         L.sphx_mlp_features.argtypes = [VP, VP, VP, VP]
         L.sphx_mlp_online.argtypes = [VP, I, I, I, C.c_float, VP]
         L.sphx_mlp_online_async.argtypes = [VP, I]
+        L.sphx_mlp_download_features.argtypes = [VP, VP, VP]
         L.sphx_mlp_record_stride.argtypes = [VP, I]
         L.sphx_mlp_eval.argtypes = [VP, VP, I, VP]
         L.sphx_mlp_grad.argtypes = [VP, VP, VP, VP, I, VP, VP, P(D)]
@@ -466,6 +467,14 @@ class Context:
         check(self.L.sphx_mlp_download_samples(self.h, start, count, ptr(X), ptr(T), ptr(M)),
               "sphx_mlp_download_samples")
         return X, T, M
+
+    def mlp_download_features(self):
+        """(X [n][24], p(t-1) [n]) as the last pass left them."""
+        X = np.zeros((self.num, self.MLP_NF), np.float32)
+        pp = np.zeros(self.num, np.float32)
+        check(self.L.sphx_mlp_download_features(self.h, ptr(X), ptr(pp)),
+              "sphx_mlp_download_features")
+        return X, pp
 
     def mlp_features(self):
         X = np.zeros((self.num, self.MLP_NF), np.float32)

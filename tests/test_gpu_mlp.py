@@ -279,3 +279,31 @@ def test_strided_online_recording(gpu):
         np.testing.assert_array_equal(XB, XA[sel])
         np.testing.assert_array_equal(TB, TA[sel])
         np.testing.assert_array_equal(MB, MA[sel])
+
+
+def test_features_from_aii_pass_equal_feature_pass(gpu):
+    """With the network in use, the a_ii pass computes the features in its
+    own neighbour loop; the stand-alone feature pass recomputes them from the
+    same state.  Same values (1e-6 normwise per column: the compiler may
+    contract the same sums differently in the two kernels)."""
+    p, disp, calc, alpha, o = setup()
+    P, U = scales(p, 0.16)
+    with sphx.Context(p, o) as ctx:
+        ctx.upload(disp, calc)
+        ctx.mf_setup(RHO, NU)
+        ctx.mf_upload(alpha)
+        ctx.call("grid_build")
+        for _ in range(3):
+            ctx.mf_step()
+        ctx.mlp_init(1, P, U)
+        ctx.mlp_mode(warm=True)
+        staged_to_aii(ctx)
+        Xf, pf = ctx.mlp_download_features()
+        Xs, ps, _ = ctx.mlp_features()
+    np.testing.assert_array_equal(pf, ps)
+    worst = 0.0
+    for q in range(Xs.shape[1]):
+        scale = max(np.max(np.abs(Xs[:, q])), 1e-30)
+        worst = max(worst, np.max(np.abs(Xf[:, q] - Xs[:, q])) / scale)
+    print(f"a_ii-pass vs feature-pass features: worst column {worst:.3e}", flush=True)
+    assert worst <= 1e-6
