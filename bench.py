@@ -30,31 +30,41 @@ NU = [1e-6, 1.5e-5]
 HBM_PEAK_GBS = 8000.0
 
 # Algorithmic bytes per fluid particle per launch (fields read or written
-# once, natural sizes: float3 = 12 B).  See DESIGN.md, "Measurement".
+# once, natural sizes: float3 = 12 B); neighbour records re-read from L2 and
+# the neighbour list are not counted.
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE,
 # scaled by a 256 MiB copy (profiles/iisph_mf_step_1m_hbm_traffic.md)
 TRAFFIC_BYTES = {
-    "iisph_iter": 366.3e6,
-    "iisph_dp": 280.2e6,
-    "mf_alpha": 421.2e6,
-    "mf_force": 416.8e6,
-    "mf_drift": 360.0e6,
-    "iisph_aii": 333.0e6,
-    "iisph_dii": 269.7e6,
-    "nbr_build": 254.1e6,
-    "iisph_density": 230.3e6,
-    "mlp_features": 443.2e6,
+    "iisph_iter": 280.8e6,
+    "iisph_dp": 237.3e6,
+    "mf_alpha": 390.5e6,
+    "mf_drift": 272.9e6,
+    "iisph_aii": 490.8e6,    # with the learned start's features
+    "iisph_dii": 196.7e6,
+    "nbr_build": 160.0e6,
+    "iisph_density": 149.9e6,
+    "mf_force": 107.8e6,
+    "mlp_infer": 173.9e6,
+}
+# MPM sand (2M particles), same method (profiles/mpm_sand_2m_kernel_stats.md)
+MPM_TRAFFIC_BYTES = {
+    "mpm_p2g": 337.2e6,
+    "mpm_force": 300.9e6,
+    "mpm_stress": 536.8e6,
 }
 
 ALGO_BYTES = {
-    "iisph_dp": 48,        # x 12, p 4, m 4, rho 4, d_ii 12 -> dijpj 12
-    "iisph_iter": 84,      # x, m, rho, rho0, d_ii, a_ii, dijpj, rho_adv, p -> p, res, a_p
+    "iisph_dp": 68,        # x 12, p 4, m 4, d_ii 12 -> dijpj 12, m q 12, a_p 12
+    "iisph_iter": 76,      # x, m, rho, rho0, a_ii, dijpj, G, H, rho_adv, p -> p, res, s
     "iisph_density": 36,   # x 12, m 4, rho0 4, type 4 (+ boundary V 4) -> rho 4
     "iisph_dii": 60,       # x, v, rho, m, rho0 -> v_adv 12, d_ii 12
-    "iisph_aii": 60,       # x, v_adv, d_ii, m, rho, p -> rho_adv, a_ii, p0
+    # a_ii pass with the learned start's features: x, v_adv, d_ii, m, rho,
+    # p, rho0, alpha_1, rho_m, last increment -> rho_adv, a_ii, p0, s, G, H,
+    # 24 features, p(t-1)
+    "iisph_aii": 196,
     "mf_drift": 64,        # x, p, m, rho, alpha(2), nu, g-a -> u_m (2x12)
-    "mf_alpha": 72,        # x, v, m, rho, alpha, u_m -> alpha_new
-    "mf_force": 76,        # x, m, rho, alpha, alpha_new, u_m, v_adv -> v_adv
+    "mf_alpha": 84,        # x, v, m, rho, alpha, u_m -> alpha_new, drift stress
+    "mf_force": 52,        # drift stress, rho_m, v_adv -> v_adv, a_mf
     "radix_sort": 32,      # 2 passes x (key+val read, key+val write)
     "reorder": 104,        # 5 x 16 B + 2 x 4 B gathered and written
     "nbr_build": 24,       # x read, count + list head written
@@ -219,7 +229,10 @@ def bench_sand(args, sphx):
                                "LDS-tile scatter", "n": n, "grid": 256, "dt": 5e-5,
                    "active_node_blocks": active, "finite": ok},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MPM_TRAFFIC_BYTES.get(dom),
+                     "traffic_source": "rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated "
+                                       "(profiles/mpm_sand_2m_kernel_stats.md)",
                      "algorithmic_bytes_per_launch": algo, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
     }
@@ -568,8 +581,9 @@ def main():
     ap.add_argument("--rpy-split", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU "
-              "decomposition is described in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build measures "
+              "one GPU's share at its real shape; the multi-GPU decomposition is not built",
+              file=sys.stderr)
         sys.exit(2)
     import sphx
     if sphx.device_count() < 1:
