@@ -228,6 +228,13 @@ void ComputeDensityIISPH_CUDA(void);                  /* host.cu:505 */
 void PredictAdvection(void);                          /* host.cu:515 */
 void PressureSolve(void);                             /* host.cu:525 */
 void Integration(void);                               /* host.cu:554 */
+/* SPH elastic solid (fluid_system_host.cuh:96-100).  The reference's
+ * hyperelastic _X kernels (sph_solid.cu:650-661) are empty, so the two X
+ * entry points do nothing, as there. */
+void ComputeSolidTensorCUDA(void);                    /* host.cu:469 */
+void ComputeSolidForceCUDA(void);                     /* host.cu:475 */
+void ComputeSolidTensorX_CUDA(void);                  /* host.cu:481 */
+void ComputeSolidForceX_CUDA(void);                   /* host.cu:487 */
 
 /* Host-side transfers the reference performs with raw cudaMemcpy on the
  * global buffers (fluid_system.cpp:979-999). */
@@ -286,6 +293,16 @@ int sphx_wcsph_force(sphx_context* ctx);            /* :397 */
 int sphx_wcsph_advance(sphx_context* ctx);          /* :426 */
 /* sort + density/pressure + force + advance (fluid_system.cpp:514-539) */
 int sphx_wcsph_step(sphx_context* ctx);
+
+/* SPH elastic solid (sph_solid.cu) and surface tension, after the WCSPH
+ * density pass (dens = 1/rho).  Deformation gradients and stresses are the
+ * calculationPack's deformGrad and stress. */
+int sphx_solid_tensor(sphx_context* ctx);           /* ComputeSolidTensor :521 */
+int sphx_solid_force(sphx_context* ctx);            /* ComputeSolidForce :612 */
+/* SurfaceTension (fluid_system_kern.cu:605; its launch is commented out in
+ * MfComputeForceCUDA, host.cu:433): accel += sum_j C cos(1.5 pi r/tR)/r x_ij
+ * over 0 < |x_ij|^2 <= tR, tR = surfaceTensionK * smoothradius. */
+int sphx_surface_tension(sphx_context* ctx);
 
 /* IISPH (fluid_system_kern.cu:635-1115, fluid_system_host.cu:505-556) */
 int sphx_iisph_density(sphx_context* ctx);          /* :1093 */

@@ -142,6 +142,25 @@ class State:
     def ii_iter(self): self._call("ii_iter")
     def ii_integrate(self): self._call("ii_integrate", False)
 
+    # -- SPH elastic solid and surface tension (oracle_solid.cpp) --
+    def solid_tensor(self, F, stress):
+        """F [n][9] (row-major deformation gradients, this state's dtype) is
+        updated in place; stress [n][9] is written for elastic particles."""
+        assert F.dtype == self.dtype and stress.dtype == self.dtype
+        assert F.flags.c_contiguous and stress.flags.c_contiguous
+        getattr(lib(), self.pfx + "solid_tensor")(C.byref(self.params), C.byref(self.s),
+                                                  C.c_void_p(F.ctypes.data),
+                                                  C.c_void_p(stress.ctypes.data))
+
+    def solid_force(self, stress):
+        """accel of elastic particles from the stresses [n][9]."""
+        assert stress.dtype == self.dtype and stress.flags.c_contiguous
+        getattr(lib(), self.pfx + "solid_force")(C.byref(self.params), C.byref(self.s),
+                                                 C.c_void_p(stress.ctypes.data))
+
+    def surface_tension(self):
+        getattr(lib(), self.pfx + "surface_tension")(C.byref(self.params), C.byref(self.s))
+
     def ii_iteration(self):
         ms, ma = C.c_double(), C.c_double()
         getattr(lib(), self.pfx + "ii_iteration")(C.byref(self.params), C.byref(self.opts),

@@ -112,6 +112,8 @@ struct KP {
   int clamp;
   int clamp_p;
   int nscenes;       // > 0: batched ensemble, per-scene convergence
+  float solidK;      // elastic solid stiffness (ParamCarrier::solidK)
+  float st_K, st_C;  // surface tension radius factor and strength
 };
 
 constexpr int SPHX_MAX_SCENES = 256;

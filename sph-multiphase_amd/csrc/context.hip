@@ -57,6 +57,9 @@ KP make_kp(const sphx_context* c) {
   k.clamp = O.clamp_to_grid;
   k.clamp_p = O.clamp_pressure;
   k.nscenes = c->nscenes;
+  k.solidK = P.solidK;
+  k.st_K = P.surfaceTensionK;
+  k.st_C = P.surfaceTensionFluidC;
   return k;
 }
 

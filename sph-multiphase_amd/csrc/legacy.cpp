@@ -91,6 +91,12 @@ void ComputeBoundaryDensity(void) { run("ComputeBoundaryDensity", sphx_boundary_
 void MfComputePressureCUDA(void) { run("MfComputePressureCUDA", sphx_wcsph_density_pressure); }
 void MfComputeForceCUDA(void) { run("MfComputeForceCUDA", sphx_wcsph_force); }
 void MfAdvanceCUDA(void) { run("MfAdvanceCUDA", sphx_wcsph_advance); }
+void ComputeSolidTensorCUDA(void) { run("ComputeSolidTensorCUDA", sphx_solid_tensor); }
+void ComputeSolidForceCUDA(void) { run("ComputeSolidForceCUDA", sphx_solid_force); }
+// ComputeSolidTensor_X / ComputeSolidForce_X are empty kernels in the
+// reference (sph_solid.cu:650-661): nothing to do
+void ComputeSolidTensorX_CUDA(void) {}
+void ComputeSolidForceX_CUDA(void) {}
 void ComputeDensityIISPH_CUDA(void) { run("ComputeDensityIISPH_CUDA", sphx_iisph_density); }
 void PredictAdvection(void) { run("PredictAdvection", sphx_iisph_predict_advection); }
 
