@@ -582,6 +582,52 @@ void sphx_setup_grid(sphx_ParamCarrier* p);
  * (cellsize = 2h, kpoly6, kspiky, kspikydiff, klaplacian, kspline). */
 void sphx_setup_kernels(sphx_ParamCarrier* p);
 
+/* ------------------------------------------------------------------------ */
+/* State I/O in the reference's formats (host only).                        */
+/* ------------------------------------------------------------------------ */
+/* The values FluidSystem::ParseXML reads (fluid_system.cpp:897-977) that
+ * are not ParamCarrier fields. */
+typedef struct sphx_SceneXml {
+  int maxPointNum;          /* Num */
+  float scaleP;             /* ScaleP */
+  int fluidnum;             /* FluidCount */
+  sphx_float3 fvmin[2];     /* VolMin0, VolMin1 */
+  sphx_float3 fvmax[2];     /* VolMax0, VolMax1 */
+  float massratio[3];       /* MassRatio "a,b,c" */
+  float densratio[3];       /* DensityRatio */
+  float viscratio[3];       /* ViscRatio */
+} sphx_SceneXml;
+/* scene.xml text (Fluid, MultiScene id=caseid, BoundInfo): fills the
+ * ParamCarrier fields ParseXML sets, with the Drucker-Prager constants
+ * converted from degrees (SetDruckerPragerParam, fluid_system.cpp:882-892);
+ * absent values read as 0 (catXMLHelper.cpp:17-25); scene may be NULL.
+ * Returns -1 (message in sphx_io_last_error) when Fluid, MultiScene or the
+ * case is missing. */
+int sphx_scene_xml_parse(const char* text, int caseid, sphx_ParamCarrier* params,
+                         sphx_SceneXml* scene);
+int sphx_scene_xml_load(const char* path, int caseid, sphx_ParamCarrier* params,
+                        sphx_SceneXml* scene);
+/* Text frame of FluidSystem::outputFile (FluidIOUtility.cpp:17-38): the
+ * count, then "x y z mass restdens type" per particle (%f, %d).  Reading
+ * fills those fields of the first min(count, cap) particles; *n = count. */
+int sphx_frame_write(const char* path, const sphx_displayPack* disp,
+                     const sphx_calculationPack* calc, int n);
+int sphx_frame_read(const char* path, sphx_displayPack* disp, sphx_calculationPack* calc,
+                    int cap, int* n);
+/* Checkpoint: the ParamCarrier and the displayPack / calculationPack arrays
+ * in the reference's layout (binary, with their sizes checked on reading).
+ * read copies the particles when cap >= *n (cap = 0: size query). */
+int sphx_checkpoint_write(const char* path, const sphx_ParamCarrier* params,
+                          const sphx_displayPack* disp, const sphx_calculationPack* calc,
+                          int n);
+int sphx_checkpoint_read(const char* path, sphx_ParamCarrier* params, sphx_displayPack* disp,
+                         sphx_calculationPack* calc, int cap, int* n);
+/* A context's parameters and particles (slot order) to a checkpoint, and
+ * back into a context created for at least that many particles. */
+int sphx_checkpoint_save(sphx_context* ctx, const char* path);
+int sphx_checkpoint_restore(sphx_context* ctx, const char* path);
+const char* sphx_io_last_error(void);
+
 /* Counts for a scene so callers can size arrays; returns total particles. */
 typedef struct sphx_SceneInfo {
   int n_fluid;

@@ -91,6 +91,12 @@ class SceneInfo(C.Structure):
                 ("n_phase", C.c_int), ("spacing", C.c_float)]
 
 
+class SceneXml(C.Structure):
+    _fields_ = [("maxPointNum", C.c_int), ("scaleP", C.c_float), ("fluidnum", C.c_int),
+                ("fvmin", Float3 * 2), ("fvmax", Float3 * 2), ("massratio", C.c_float * 3),
+                ("densratio", C.c_float * 3), ("viscratio", C.c_float * 3)]
+
+
 class SphxError(RuntimeError):
     pass
 
@@ -167,6 +173,14 @@ def lib() -> C.CDLL:
         L.sphx_mlp_online.argtypes = [VP, I, I, I, C.c_float, VP]
         L.sphx_mlp_online_async.argtypes = [VP, I]
         L.sphx_mlp_download_features.argtypes = [VP, VP, VP]
+        L.sphx_io_last_error.restype = C.c_char_p
+        L.sphx_scene_xml_parse.argtypes = [C.c_char_p, I, VP, VP]
+        L.sphx_frame_write.argtypes = [C.c_char_p, VP, VP, I]
+        L.sphx_frame_read.argtypes = [C.c_char_p, VP, VP, I, VP]
+        L.sphx_checkpoint_write.argtypes = [C.c_char_p, VP, VP, VP, I]
+        L.sphx_checkpoint_read.argtypes = [C.c_char_p, VP, VP, VP, I, VP]
+        L.sphx_checkpoint_save.argtypes = [VP, C.c_char_p]
+        L.sphx_checkpoint_restore.argtypes = [VP, C.c_char_p]
         L.sphx_mlp_record_stride.argtypes = [VP, I]
         L.sphx_mlp_eval.argtypes = [VP, VP, I, VP]
         L.sphx_mlp_grad.argtypes = [VP, VP, VP, VP, I, VP, VP, P(D)]
@@ -306,6 +320,58 @@ def scene_sand(n=4096, lo=0.3, hi=0.6, dx=1.0 / 32, res=32, E=3.5e5, nu=0.3, phi
     return p, disp, calc, info
 
 
+# ----------------------------------------------------------- state I/O --
+def _io_check(rc, what):
+    if rc != 0:
+        raise SphxError(f"{what}: {lib().sphx_io_last_error().decode()}")
+
+
+def scene_xml_parse(text: str, caseid: int):
+    """(ParamCarrier, SceneXml) from scene.xml text (sphx_scene_xml_parse)."""
+    L = lib()
+    p = ParamCarrier()
+    sx = SceneXml()
+    _io_check(L.sphx_scene_xml_parse(text.encode(), caseid, C.byref(p), C.byref(sx)),
+              "sphx_scene_xml_parse")
+    return p, sx
+
+
+def frame_write(path, disp, calc):
+    _io_check(lib().sphx_frame_write(str(path).encode(), ptr(disp), ptr(calc), len(disp)),
+              "sphx_frame_write")
+
+
+def frame_read(path):
+    """(disp, calc) with pos, type, mass and restdens of a text frame."""
+    L = lib()
+    n = C.c_int()
+    _io_check(L.sphx_frame_read(str(path).encode(), None, None, 0, C.byref(n)), "sphx_frame_read")
+    disp = np.zeros(n.value, display_dtype)
+    calc = np.zeros(n.value, calc_dtype)
+    _io_check(L.sphx_frame_read(str(path).encode(), ptr(disp), ptr(calc), n.value, C.byref(n)),
+              "sphx_frame_read")
+    return disp, calc
+
+
+def checkpoint_write(path, params, disp, calc):
+    _io_check(lib().sphx_checkpoint_write(str(path).encode(), C.byref(params), ptr(disp),
+                                          ptr(calc), len(disp)), "sphx_checkpoint_write")
+
+
+def checkpoint_read(path):
+    """(params, disp, calc) of a checkpoint file."""
+    L = lib()
+    p = ParamCarrier()
+    n = C.c_int()
+    _io_check(L.sphx_checkpoint_read(str(path).encode(), C.byref(p), None, None, 0, C.byref(n)),
+              "sphx_checkpoint_read")
+    disp = np.zeros(n.value, display_dtype)
+    calc = np.zeros(n.value, calc_dtype)
+    _io_check(L.sphx_checkpoint_read(str(path).encode(), C.byref(p), ptr(disp), ptr(calc),
+                                     n.value, C.byref(n)), "sphx_checkpoint_read")
+    return p, disp, calc
+
+
 class Context:
     """Owns one sphx_context (device buffers for params.maxNum particles)."""
 
@@ -321,6 +387,13 @@ class Context:
         if self.h:
             self.L.sphx_destroy(self.h)
             self.h = C.c_void_p()
+
+    def checkpoint_save(self, path):
+        _io_check(self.L.sphx_checkpoint_save(self.h, str(path).encode()), "sphx_checkpoint_save")
+
+    def checkpoint_restore(self, path):
+        _io_check(self.L.sphx_checkpoint_restore(self.h, str(path).encode()),
+                  "sphx_checkpoint_restore")
 
     def __enter__(self):
         return self
