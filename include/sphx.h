@@ -626,6 +626,17 @@ int sphx_checkpoint_read(const char* path, sphx_ParamCarrier* params, sphx_displ
  * back into a context created for at least that many particles. */
 int sphx_checkpoint_save(sphx_context* ctx, const char* path);
 int sphx_checkpoint_restore(sphx_context* ctx, const char* path);
+/* Boundary particles from a shape file (LoadBoundary, AddShape.cpp:265-359):
+ * "WALL min max" and "OPEN_BOX min max thickness" commands, each wall a
+ * lattice of spacing (bmass / bRestdensity)^(1/3) / simscale as
+ * SetupAddWall fills it.  Particles go to slots start .. cap-1 (either
+ * array may be NULL to count); *added = the number the file describes. */
+int sphx_boundary_parse(const char* text, const sphx_ParamCarrier* params,
+                        sphx_displayPack* disp, sphx_calculationPack* calc, int start,
+                        int cap, int* added);
+int sphx_boundary_load(const char* path, const sphx_ParamCarrier* params,
+                       sphx_displayPack* disp, sphx_calculationPack* calc, int start,
+                       int cap, int* added);
 const char* sphx_io_last_error(void);
 
 /* Counts for a scene so callers can size arrays; returns total particles. */

@@ -388,4 +388,96 @@ int sphx_checkpoint_restore(sphx_context* ctx, const char* path) {
   return 0;
 }
 
+// Boundary particles from a shape file (FluidSystem::LoadBoundary,
+// AddShape.cpp:265-296): "WALL min max" and "OPEN_BOX min max thickness"
+// commands ("x,y,z" vectors), each wall filled as SetupAddWall does
+// (:298-346): spacing (bmass / bRestdensity)^(1/3) / simscale
+// (CalcBoundarySpacing :257-263), ceil(extent / spacing) columns in x and
+// z from min, layers y = min.y, min.y + spacing, ... <= max.y (a float
+// accumulator, as there).  Particles are appended from slot `start` up to
+// `cap` (AddParticle's defaults, :702-717, with bornid = slot); *added
+// receives the number the file asks for, also beyond cap.
+int sphx_boundary_parse(const char* text, const sphx_ParamCarrier* P, sphx_displayPack* disp,
+                        sphx_calculationPack* calc, int start, int cap, int* added) {
+  if (!text || !P || start < 0 || cap < start) return io_fail("sphx_boundary_parse: bad argument");
+  const float spacing = std::pow(P->bmass / P->bRestdensity, 1 / 3.0f) / P->simscale;
+  if (!(spacing > 0.f) || !std::isfinite(spacing))
+    return io_fail("sphx_boundary_parse: bmass / bRestdensity / simscale give no spacing");
+  int n = start;
+  auto add_wall = [&](sphx_float3 mn, sphx_float3 mx) {
+    const int cntx = (int)std::ceil((mx.x - mn.x) / spacing);
+    const int cntz = (int)std::ceil((mx.z - mn.z) / spacing);
+    const int cnt = cntx * cntz;
+    for (float y = mn.y; y <= mx.y; y += spacing) {
+      for (int xz = 0; xz < cnt; ++xz) {
+        const float x = mn.x + (xz % cntx) * spacing;
+        const float z = mn.z + (xz / cntx) * spacing;
+        if (n < cap) {
+          if (disp) {
+            sphx_displayPack& d = disp[n];
+            std::memset(&d, 0, sizeof(d));
+            d.pos.x = x; d.pos.y = y; d.pos.z = z;
+            d.color.x = 1.f; d.color.y = 1.f; d.color.z = 1.f;
+            d.color.w = z > 0 ? 0.f : 0.2f;
+            d.type = SPHX_TYPE_BOUNDARY;
+          }
+          if (calc) {
+            sphx_calculationPack& c = calc[n];
+            std::memset(&c, 0, sizeof(c));
+            c.bornid = n;
+            c.deformGrad.data[0] = c.deformGrad.data[4] = c.deformGrad.data[8] = 1.f;
+            c.mass = P->bmass;
+            c.restdens = P->bRestdensity;
+            c.visc = P->bvisc;
+          }
+        }
+        ++n;
+      }
+    }
+  };
+  auto read3 = [](const char*& c, sphx_float3& v) {
+    int used = 0;
+    if (std::sscanf(c, " %f , %f , %f%n", &v.x, &v.y, &v.z, &used) != 3) return false;
+    c += used;
+    return true;
+  };
+  const char* c = text;
+  char word[1000];
+  int used = 0;
+  while (std::sscanf(c, " %999s%n", word, &used) == 1) {
+    c += used;
+    sphx_float3 mn, mx;
+    if (!std::strcmp(word, "WALL")) {
+      if (!read3(c, mn) || !read3(c, mx)) return io_fail("sphx_boundary_parse: bad WALL");
+      add_wall(mn, mx);
+    } else if (!std::strcmp(word, "OPEN_BOX")) {
+      float t = 0.f;
+      if (!read3(c, mn) || !read3(c, mx) || std::sscanf(c, " %f%n", &t, &used) != 1)
+        return io_fail("sphx_boundary_parse: bad OPEN_BOX");
+      c += used;
+      // SetupAddOpenBox (:348-359): ground, front, back, left, right
+      add_wall({mn.x, mn.y - t, mn.z}, {mx.x, mn.y, mx.z});
+      add_wall({mn.x - t, mn.y, mn.z - t}, {mx.x + t, mx.y, mn.z});
+      add_wall({mn.x - t, mn.y, mx.z}, {mx.x + t, mx.y, mx.z + t});
+      add_wall({mn.x - t, mn.y, mn.z}, {mn.x, mx.y, mx.z});
+      add_wall({mx.x, mn.y, mn.z}, {mx.x + t, mx.y, mx.z});
+    }
+  }
+  if (added) *added = n - start;
+  return 0;
+}
+
+int sphx_boundary_load(const char* path, const sphx_ParamCarrier* P, sphx_displayPack* disp,
+                       sphx_calculationPack* calc, int start, int cap, int* added) {
+  if (!path) return io_fail("sphx_boundary_load: null path");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return io_fail(std::string("sphx_boundary_load: cannot open ") + path);
+  std::string text;
+  char buf[65536];
+  size_t got;
+  while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, got);
+  std::fclose(f);
+  return sphx_boundary_parse(text.c_str(), P, disp, calc, start, cap, added);
+}
+
 }  // extern "C"

@@ -180,6 +180,7 @@ def lib() -> C.CDLL:
         L.sphx_checkpoint_write.argtypes = [C.c_char_p, VP, VP, VP, I]
         L.sphx_checkpoint_read.argtypes = [C.c_char_p, VP, VP, VP, I, VP]
         L.sphx_checkpoint_save.argtypes = [VP, C.c_char_p]
+        L.sphx_boundary_parse.argtypes = [C.c_char_p, VP, VP, VP, I, I, VP]
         L.sphx_checkpoint_restore.argtypes = [VP, C.c_char_p]
         L.sphx_mlp_record_stride.argtypes = [VP, I]
         L.sphx_mlp_eval.argtypes = [VP, VP, I, VP]
@@ -370,6 +371,20 @@ def checkpoint_read(path):
     _io_check(L.sphx_checkpoint_read(str(path).encode(), C.byref(p), ptr(disp), ptr(calc),
                                      n.value, C.byref(n)), "sphx_checkpoint_read")
     return p, disp, calc
+
+
+def boundary_parse(text: str, params, start=0):
+    """Boundary particles of a shape file's text (WALL / OPEN_BOX commands);
+    returns (disp, calc) of the particles, bornid counted from `start`."""
+    L = lib()
+    n = C.c_int()
+    _io_check(L.sphx_boundary_parse(text.encode(), C.byref(params), None, None, start, start,
+                                    C.byref(n)), "sphx_boundary_parse")
+    disp = np.zeros(start + n.value, display_dtype)
+    calc = np.zeros(start + n.value, calc_dtype)
+    _io_check(L.sphx_boundary_parse(text.encode(), C.byref(params), ptr(disp), ptr(calc), start,
+                                    start + n.value, C.byref(n)), "sphx_boundary_parse")
+    return disp[start:], calc[start:]
 
 
 class Context:

@@ -142,3 +142,77 @@ def test_checkpoint_round_trip_and_checks(tmp_path):
     short.write_bytes(path.read_bytes()[:-100])
     with pytest.raises(sphx.SphxError, match="truncated"):
         sphx.checkpoint_read(short)
+
+
+def boundary_params(spacing=0.5, simscale=1.0):
+    p = sphx.ParamCarrier()
+    p.bRestdensity = 1000.0
+    p.bmass = 1000.0 * (spacing * simscale) ** 3
+    p.simscale = simscale
+    p.bvisc = 7.0
+    return p
+
+
+def wall_count(mn, mx, s):
+    """Columns ceil(extent / s) in x and z; layers min.y, min.y + s, ... <=
+    max.y with the spacing added in float32 as the reference's loop does."""
+    cx = int(np.ceil(np.float32(mx[0] - mn[0]) / np.float32(s)))
+    cz = int(np.ceil(np.float32(mx[2] - mn[2]) / np.float32(s)))
+    ny, y = 0, np.float32(mn[1])
+    while y <= np.float32(mx[1]):
+        ny += 1
+        y = np.float32(y + np.float32(s))
+    return cx * cz * ny
+
+
+def test_boundary_wall_lattice():
+    p = boundary_params(0.5)
+    s = np.float32(np.cbrt(np.float32(p.bmass / p.bRestdensity)))
+    disp, calc = sphx.boundary_parse("WALL\n-2, 0, -1.5\n2, 3, 1.5\n", p, start=5)
+    assert len(disp) == wall_count((-2, 0, -1.5), (2, 3, 1.5), s) == 8 * 6 * 7
+    pos = disp["pos"].astype(np.float64)
+    assert np.all(disp["type"] == 1)
+    np.testing.assert_array_equal(calc["bornid"], np.arange(5, 5 + len(disp)))
+    assert np.all(calc["mass"] == np.float32(p.bmass)) and np.all(calc["visc"] == 7.0)
+    assert np.all(calc["restdens"] == 1000.0)
+    np.testing.assert_array_equal(calc["deformGrad"][:, [0, 4, 8]], 1.0)
+    # lattice columns from min, layers from min.y
+    kx = (pos[:, 0] + 2) / s
+    kz = (pos[:, 2] + 1.5) / s
+    np.testing.assert_allclose(kx, np.round(kx), atol=1e-5)
+    np.testing.assert_allclose(kz, np.round(kz), atol=1e-5)
+    assert set(np.round(kx).astype(int)) == set(range(8))
+    assert set(np.round(kz).astype(int)) == set(range(6))
+    # colour alpha 0 above z = 0, 0.2 at and below
+    np.testing.assert_array_equal(disp["color"][:, 3], np.where(pos[:, 2] > 0, 0.0, np.float32(0.2)))
+
+
+def test_boundary_file_commands_and_open_box():
+    p = boundary_params(0.5)
+    s = np.float32(np.cbrt(np.float32(p.bmass / p.bRestdensity)))
+    walls = [((-5, 0, -5), (-4, 5, 5)), ((4, 0, -5), (5, 5, 5)), ((-5, -1, -5), (5, 0, 5))]
+    text = "".join(f"WALL\n{a[0]}, {a[1]}, {a[2]}\n {b[0]},{b[1]},{b[2]}\n" for a, b in walls)
+    disp, calc = sphx.boundary_parse(text, p)
+    assert len(disp) == sum(wall_count(a, b, s) for a, b in walls)
+    # OPEN_BOX = ground, front, back, left, right walls around the box
+    mn, mx, t = (-3.0, 0.0, -2.0), (3.0, 4.0, 2.0), 1.0
+    box = [((mn[0], mn[1] - t, mn[2]), (mx[0], mn[1], mx[2])),
+           ((mn[0] - t, mn[1], mn[2] - t), (mx[0] + t, mx[1], mn[2])),
+           ((mn[0] - t, mn[1], mx[2]), (mx[0] + t, mx[1], mx[2] + t)),
+           ((mn[0] - t, mn[1], mn[2]), (mn[0], mx[1], mx[2])),
+           ((mx[0], mn[1], mn[2]), (mx[0] + t, mx[1], mx[2]))]
+    d2, c2 = sphx.boundary_parse("OPEN_BOX\n-3, 0, -2\n3, 4, 2\n1", p)
+    assert len(d2) == sum(wall_count(a, b, s) for a, b in box)
+    parts = [sphx.boundary_parse(f"WALL {a[0]},{a[1]},{a[2]} {b[0]},{b[1]},{b[2]}", p)[0]["pos"]
+             for a, b in box]
+    np.testing.assert_array_equal(d2["pos"], np.concatenate(parts))
+    # a cap below the count: the count is still reported, nothing written past it
+    L = sphx.lib()
+    n = sphx.C.c_int()
+    buf_d = np.zeros(10, sphx.display_dtype)
+    buf_c = np.zeros(10, sphx.calc_dtype)
+    assert L.sphx_boundary_parse(b"OPEN_BOX -3,0,-2 3,4,2 1", sphx.C.byref(p), sphx.ptr(buf_d),
+                                 sphx.ptr(buf_c), 2, 10, sphx.C.byref(n)) == 0
+    assert n.value == len(d2)
+    np.testing.assert_array_equal(buf_d["pos"][2:], d2["pos"][:8])
+    assert np.all(buf_d["pos"][:2] == 0)
