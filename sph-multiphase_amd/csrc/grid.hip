@@ -26,7 +26,7 @@
 namespace sphx {
 
 constexpr int RS_THREADS = 256;
-constexpr int RS_ITEMS = 16;
+constexpr int RS_ITEMS = 8;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // 4096 keys per workgroup
 constexpr int RS_MAX_BITS = 9;
 constexpr int RS_MAX_BINS = 1 << RS_MAX_BITS;
