@@ -27,7 +27,7 @@ namespace sphx {
 
 constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 8;
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // 4096 keys per workgroup
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // 2048 keys per workgroup
 constexpr int RS_MAX_BITS = 9;
 constexpr int RS_MAX_BINS = 1 << RS_MAX_BITS;
 constexpr int RS_WAVES = RS_THREADS / 64;
