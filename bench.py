@@ -58,6 +58,8 @@ ALGO_BYTES = {
     "iisph_iter": 76,      # x, m, rho, rho0, a_ii, dijpj, G, H, rho_adv, p -> p, res, s
     "iisph_density": 36,   # x 12, m 4, rho0 4, type 4 (+ boundary V 4) -> rho 4
     "iisph_dii": 60,       # x, v, rho, m, rho0 -> v_adv 12, d_ii 12
+    # density, v_adv and d_ii in one pass (the composite steps)
+    "iisph_density_dii": 64,   # x, v, m, rho0, type -> rho 4, v_adv 12, d_ii 12
     # a_ii pass with the learned start's features: x, v_adv, d_ii, m, rho,
     # p, rho0, alpha_1, rho_m, last increment -> rho_adv, a_ii, p0, s, G, H,
     # 24 features, p(t-1)
@@ -631,8 +633,8 @@ def main():
         if s % 10 == 0:
             print(f"# warmup step {s}: {it} iterations, error {err:.2e}", file=sys.stderr,
                   flush=True)
-    names = ["hash", "radix_sort", "cell_start", "reorder", "nbr_build", "iisph_density",
-             "mf_drift", "mf_alpha", "mf_correction", "iisph_dii", "mf_force", "iisph_aii",
+    names = ["hash", "radix_sort", "cell_start", "reorder", "nbr_build", "iisph_density_dii",
+             "mf_drift", "mf_alpha", "mf_correction", "mf_force", "iisph_aii",
              "mlp_features", "mlp_infer", "iisph_dp", "iisph_iter", "iisph_integrate",
              "mlp_record", "mlp_train"]
 

@@ -324,6 +324,11 @@ int sphx_iisph_step(sphx_context* ctx, int* iters, double* err);
  * caller can add accelerations to the advected velocity in between. */
 int sphx_iisph_predict_dii(sphx_context* ctx);      /* ComputeDii :768 */
 int sphx_iisph_predict_aii(sphx_context* ctx);      /* ComputeAii :883 */
+/* ComputeDensityIISPH (:1093) then ComputeDii (:768) in one neighbour
+ * pass: density, advected velocity and d_ii, as the two calls back to back
+ * (the composite steps use it; calls that change x, v.xyz, m or rho0 may
+ * not sit between the two). */
+int sphx_iisph_density_dii(sphx_context* ctx);
 /* IISPH per-slot buffers (any pointer may be NULL); vectors are 3*num. */
 int sphx_download_iisph(sphx_context* ctx, float* dii, float* dijpj,
                         float* aii, float* rho_adv, float* press_l,

@@ -122,12 +122,12 @@ constexpr int SPHX_MAX_SCENES = 256;
 // (multiples of 64 up to 256; the kernels do not depend on it otherwise).
 enum TuneId {
   T_DENSITY, T_DII, T_AII, T_DP, T_ITER, T_MF_DRIFT, T_MF_ALPHA, T_MF_FORCE,
-  T_MLP_FEAT, T_MLP_INFER, T_COUNT
+  T_MLP_FEAT, T_MLP_INFER, T_DENS_DII, T_COUNT
 };
 // timer names of the kernels, in TuneId order
 constexpr const char* TUNE_NAMES[T_COUNT] = {
     "iisph_density", "iisph_dii", "iisph_aii", "iisph_dp", "iisph_iter",
-    "mf_drift", "mf_alpha", "mf_force", "mlp_features", "mlp_infer"};
+    "mf_drift", "mf_alpha", "mf_force", "mlp_features", "mlp_infer", "iisph_density_dii"};
 
 // Per-scene solver state of a batched ensemble (device arrays).
 struct SceneCtl {
@@ -322,7 +322,7 @@ struct sphx_context {
   int graph_budget = 0, graph_num = -1;
 
   // workgroup sizes per kernel class (TuneId)
-  int tune[sphx::T_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256, 256, 256};
+  int tune[sphx::T_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256, 256, 256, 256};
 
   // timing
   bool timing = false;

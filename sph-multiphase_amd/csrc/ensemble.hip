@@ -19,6 +19,7 @@
 namespace sphx {
 
 int enqueue_solve_fixed(sphx_context* c, int n);
+int iisph_density_dii(sphx_context* c);
 int continue_solve(sphx_context* c, int* iters, double* err);
 int mlp_begin(sphx_context* c);
 int mlp_fallback(sphx_context* c, int* iters, double* err);
@@ -45,11 +46,10 @@ static void after_graph_a(sphx_context* c, int p) {
 
 static int stage_a(sphx_context* c, int budget) {
   SPHX_TRY(sphx_grid_build(c));
-  SPHX_TRY(sphx_iisph_density(c));
+  SPHX_TRY(iisph_density_dii(c));
   SPHX_TRY(sphx_mf_drift_velocity(c));
   SPHX_TRY(sphx_mf_alpha_advance(c));
   SPHX_TRY(sphx_mf_correction(c));
-  SPHX_TRY(sphx_iisph_predict_dii(c));
   SPHX_TRY(sphx_mf_force(c));
   SPHX_TRY(sphx_iisph_predict_aii(c));
   SPHX_TRY(mlp_begin(c));

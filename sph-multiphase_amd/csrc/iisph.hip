@@ -118,6 +118,67 @@ __global__ void k_ii_dii(KP k, NbrList L, const uint32_t* __restrict__ key, cons
   dii[i] = make_float4(dx_ * f, dy_ * f, dz_ * f, 0.f);
 }
 
+// k_ii_density and k_ii_dii in one neighbour loop, for steps that run them
+// back to back (the fraction passes between them in the two-phase step
+// leave x, v.xyz, m and rho0 alone).  Both read the same {x, phys} record;
+// d_ii's sum needs no density, so rho_i only scales it at the end.  The
+// density sum visits the entries in the same order as k_ii_density, so
+// rho_i, v_adv and d_ii are bitwise those of the two passes.
+__global__ void k_ii_density_dii(KP k, NbrList L, const uint32_t* __restrict__ key,
+                                 const int* __restrict__ cs, const float4* __restrict__ pos,
+                                 const float4* __restrict__ vel, float4* __restrict__ phys,
+                                 float4* __restrict__ veleval, float4* __restrict__ dii) {
+  int i = xcd_index();
+  if (i >= k.n) return;
+  float4 pi = pos[i];
+  if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
+  uint32_t gc = key[i];
+  if ((int)gc == k.gridtotal) return;
+  const float4 vi = vel[i];
+  const float4 qi = phys[i];
+  const float rest = qi.w;
+  float sum = k.kspline * qi.x;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  float dx_ = 0.f, dy_ = 0.f, dz_ = 0.f;
+  struct Rec { float4 p, q; };
+  for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j]}; },
+                    [&](int j, const Rec& o) {
+    const float4 pj = o.p, qj = o.q;
+    const int tj = type_of(pj);
+    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+    float dsq = dx * dx + dy * dy + dz * dz;
+    float r, ir;
+    dist_rinv(dsq, r, ir);
+    if (dsq < 4.f * k.h2)
+      sum += cubic_w(r, k.inv_h, k.kspline) * ((tj == SPHX_TYPE_BOUNDARY) ? rest * qj.y : qj.x);
+    if (!(r < 2.f * k.h && r > 0.f)) return;
+    float gf = cubic_gradf(r, ir, k.inv_h, k.kspline);
+    if (tj == SPHX_TYPE_BOUNDARY) {
+      float4 vj = vel[j];
+      float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+      float pij = dx * vx + dy * vy + dz * vz;
+      if (pij < 0.f) {
+        pij = pij / (r * r + k.h2 * 0.01f);
+        pij = pij * k.bvisc * k.h / 2.f / rest;
+        pij = rest * qj.y * pij;
+        ax += dx * gf * pij; ay += dy * gf * pij; az += dz * gf * pij;
+      }
+    }
+    bool sup = k.full_support ? true : (r < k.h);
+    if (!sup) return;
+    float mj;
+    if (tj != SPHX_TYPE_BOUNDARY) mj = qj.x;
+    else if (k.boundary_pressure) mj = rest * qj.y;
+    else return;
+    dx_ -= dx * gf * mj; dy_ -= dy * gf * mj; dz_ -= dz * gf * mj;
+  });
+  phys[i].y = sum;
+  ax += k.g.x; ay += k.g.y; az += k.g.z;
+  veleval[i] = make_float4(vi.x + ax * k.dt, vi.y + ay * k.dt, vi.z + az * k.dt, 0.f);
+  float f = 1.f / sum / sum * k.dt * k.dt;
+  dii[i] = make_float4(dx_ * f, dy_ * f, dz_ * f, 0.f);
+}
+
 // ComputeAii (kern.cu:801-908): rho_adv, warm start p0 = w p(t-1), a_ii;
 // also the position-only sums G_i, H_i of the pressure pass, and, with
 // FEAT, the learned start's 24 features (the same map as k_mlp_features,
@@ -842,6 +903,35 @@ int sphx_iisph_predict_dii(sphx_context* c) {
   return 0;
 }
 
+}  // extern "C"
+
+namespace sphx {
+
+// sphx_iisph_density followed by sphx_iisph_predict_dii, as one pass; for
+// the composite steps, whose calls between the two touch neither x, v.xyz,
+// m nor rho0.
+int iisph_density_dii(sphx_context* c) {
+  SPHX_TRY(need_sorted(c, "sphx_iisph_density"));
+  KP k = make_kp(c);
+  if (k.n == 0) return 0;
+  c->mf_rec_ok = false;   // densities change
+  c->mf_stress_ok = false;
+  NbrList L;
+  SPHX_TRY(nbr_list(c, &L));
+  timer_begin(c, "iisph_density_dii");
+  k_ii_density_dii<<<blocks_for(k.n, c->tune[T_DENS_DII]), c->tune[T_DENS_DII], 0, c->stream>>>(
+      k, L, c->key.p, c->cell_start.p, c->POS(), c->VEL(), c->PHYS(), c->VELEVAL(), c->dii.p);
+  timer_end(c);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace sphx
+
+extern "C" {
+
+int sphx_iisph_density_dii(sphx_context* c) { return iisph_density_dii(c); }
+
 int sphx_iisph_predict_aii(sphx_context* c) {
   SPHX_TRY(need_sorted(c, "sphx_iisph_predict_aii"));
   KP k = make_kp(c);
@@ -988,8 +1078,8 @@ int sphx_iisph_step(sphx_context* c, int* iters, double* err) {
   SPHX_TRY(grid_hash(c));
   SPHX_TRY(grid_sort(c));
   SPHX_TRY(grid_reorder(c));
-  SPHX_TRY(sphx_iisph_density(c));
-  SPHX_TRY(sphx_iisph_predict_advection(c));
+  SPHX_TRY(iisph_density_dii(c));
+  SPHX_TRY(sphx_iisph_predict_aii(c));
   SPHX_TRY(sphx_iisph_pressure_solve(c, iters, err));
   return sphx_iisph_integrate(c);
 }

@@ -385,6 +385,8 @@ __global__ void k_mf_copy(const float4* __restrict__ a, float4* __restrict__ b, 
 // Make the corrected fractions the current ones (alpha_new is per slot).
 // A copy, not a pointer swap: the buffers bound to every launch depend only
 // on the particle-buffer parity, so a captured step graph stays valid.
+int iisph_density_dii(sphx_context* c);
+
 int mf_commit(sphx_context* c) {
   if (!c->mf_pending) return 0;
   c->mf_stress_ok = false;   // the fractions before the step change
@@ -643,24 +645,25 @@ int sphx_mf_force(sphx_context* c) {
 }
 
 int sphx_grid_build(sphx_context* c);
-int sphx_iisph_density(sphx_context* c);
-int sphx_iisph_predict_dii(sphx_context* c);
 int sphx_iisph_predict_aii(sphx_context* c);
 int sphx_iisph_pressure_solve(sphx_context* c, int* iters, double* err);
 int sphx_iisph_integrate(sphx_context* c);
 
-// One two-phase step: grid build, density, drift velocity, fraction
-// advection and correction, advected velocity (+ drift stress and
-// buoyancy), a_ii, pressure solve to tolerance, integration.
+// One two-phase step: grid build, density with the advected velocity and
+// d_ii (one pass), drift velocity, fraction advection and correction,
+// drift stress and buoyancy added to the advected velocity, a_ii, pressure
+// solve to tolerance, integration.  The same arithmetic as the calls
+// density, drift, alpha, correction, predict_dii, force, ...: the fraction
+// passes change neither x, v.xyz, m nor rho0, which are all d_ii and v_adv
+// read.
 int sphx_mf_step(sphx_context* c, int* iters, double* err) {
   SPHX_CHECK(c, "sphx_mf_step", "null context");
   SPHX_CHECK(c->nphase > 0, "sphx_mf_step", "multiphase not set up");
   SPHX_TRY(sphx_grid_build(c));
-  SPHX_TRY(sphx_iisph_density(c));
+  SPHX_TRY(iisph_density_dii(c));
   SPHX_TRY(sphx_mf_drift_velocity(c));
   SPHX_TRY(sphx_mf_alpha_advance(c));
   SPHX_TRY(sphx_mf_correction(c));
-  SPHX_TRY(sphx_iisph_predict_dii(c));
   SPHX_TRY(sphx_mf_force(c));
   SPHX_TRY(sphx_iisph_predict_aii(c));
   SPHX_TRY(sphx_iisph_pressure_solve(c, iters, err));

@@ -148,7 +148,8 @@ def lib() -> C.CDLL:
         L.sphx_scene_dambreak.argtypes = [I, I, I, C.c_float, C.c_float, C.c_float, C.c_float,
                                           C.c_float, I, C.c_float, C.c_uint64, I,
                                           P(ParamCarrier), VP, VP, P(SceneInfo)]
-        for name in ["sphx_iisph_predict_dii", "sphx_iisph_predict_aii", "sphx_mf_drift_velocity",
+        for name in ["sphx_iisph_predict_dii", "sphx_iisph_predict_aii", "sphx_iisph_density_dii",
+                     "sphx_mf_drift_velocity",
                      "sphx_mf_alpha_advance", "sphx_mf_correction", "sphx_mf_force"]:
             getattr(L, name).argtypes = [VP]
         L.sphx_mf_setup.argtypes = [VP, I, VP, VP]
@@ -630,7 +631,8 @@ class Context:
         return it.value, err.value
 
     TUNE_KERNELS = ("iisph_density", "iisph_dii", "iisph_aii", "iisph_dp", "iisph_iter",
-                    "mf_drift", "mf_alpha", "mf_force", "mlp_features", "mlp_infer")
+                    "mf_drift", "mf_alpha", "mf_force", "mlp_features", "mlp_infer",
+                    "iisph_density_dii")
 
     def tune_set(self, kernel, block):
         check(self.L.sphx_tune_set(self.h, kernel.encode(), block), "sphx_tune_set")

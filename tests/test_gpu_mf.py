@@ -162,3 +162,39 @@ def test_force_from_fused_alpha_pass_equals_own_pass(gpu):
     print(f"fused vs own pass: {err:.3e} (max |a| {np.max(np.abs(b[fl])):.3e})", flush=True)
     assert np.max(np.abs(b[fl])) > 0
     assert err <= 1e-6
+
+
+def test_density_dii_pass_equals_two_passes(gpu):
+    """The composite steps compute density, advected velocity and d_ii in one
+    neighbour pass at the density's place; the fraction passes in between
+    touch none of their inputs.  The one pass gives the values of
+    iisph_density ... iisph_predict_dii (1e-6 relative: the compiler may
+    contract the same sums into FMAs differently in the two kernels)."""
+    p, disp, calc, alpha, o = setup()
+    with sphx.Context(p, o) as ctx:
+        ctx.upload(disp, calc)
+        ctx.mf_setup(RHO, NU)
+        ctx.mf_upload(alpha)
+        ctx.call("grid_build")
+        for _ in range(3):
+            ctx.mf_step()
+        ctx.call("grid_build")
+        ctx.call("iisph_density")
+        for s in ["mf_drift_velocity", "mf_alpha_advance", "mf_correction"]:
+            ctx.call(s)
+        ctx.call("iisph_predict_dii")
+        d0, c0 = ctx.download()
+        b0 = ctx.download_iisph()
+        ctx.call("iisph_density_dii")
+        d1, c1 = ctx.download()
+        b1 = ctx.download_iisph()
+    fl = d0["type"] == 0
+    for name, a, b in [("density", c1["dens"], c0["dens"]),
+                       ("veleval", c1["veleval"], c0["veleval"]),
+                       ("dii", b1["dii"], b0["dii"])]:
+        a = np.asarray(a, np.float64)[fl]
+        b = np.asarray(b, np.float64)[fl]
+        err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+        print(f"{name}: one pass vs two {err:.3e} (exact {np.array_equal(a, b)})", flush=True)
+        assert np.max(np.abs(b)) > 0
+        assert err <= 1e-6, name
