@@ -329,6 +329,12 @@ int sphx_iisph_predict_aii(sphx_context* ctx);      /* ComputeAii :883 */
  * (the composite steps use it; calls that change x, v.xyz, m or rho0 may
  * not sit between the two). */
 int sphx_iisph_density_dii(sphx_context* ctx);
+/* Diagnostics of the neighbour list the loops above share (no reference
+ * counterpart: the reference scans the 27 cells in every loop).  counts[i]
+ * = neighbours j != i within 2h of particle i (sorted order), -1 where the
+ * list overflowed and the loops scan the cells instead; *cap = capacity.
+ * Boundary and out-of-grid particles report 0. */
+int sphx_nbr_counts(sphx_context* ctx, int* counts, int* cap);
 /* IISPH per-slot buffers (any pointer may be NULL); vectors are 3*num. */
 int sphx_download_iisph(sphx_context* ctx, float* dii, float* dijpj,
                         float* aii, float* rho_adv, float* press_l,

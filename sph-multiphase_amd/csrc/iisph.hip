@@ -932,6 +932,23 @@ extern "C" {
 
 int sphx_iisph_density_dii(sphx_context* c) { return iisph_density_dii(c); }
 
+// Per-particle neighbour counts of the step's list (builds it if needed):
+// counts[i] = number of j != i within 2h, or -1 where the list overflowed
+// and the loops scan the 27 cells instead.  *cap = the list's capacity.
+int sphx_nbr_counts(sphx_context* c, int* counts, int* cap) {
+  SPHX_TRY(need_sorted(c, "sphx_nbr_counts"));
+  NbrList L;
+  SPHX_TRY(nbr_list(c, &L));
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  if (cap) *cap = L.cap;
+  if (counts && c->num > 0) {
+    SPHX_HIP(hipMemcpy(counts, L.cnt, c->num * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < c->num; ++i)
+      if (counts[i] > L.cap) counts[i] = -1;
+  }
+  return 0;
+}
+
 int sphx_iisph_predict_aii(sphx_context* c) {
   SPHX_TRY(need_sorted(c, "sphx_iisph_predict_aii"));
   KP k = make_kp(c);

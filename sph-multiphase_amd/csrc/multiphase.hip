@@ -154,6 +154,8 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_drift(
 // which reads a subset of its records (positions, V_j, drifts and the
 // fractions before the step) and none of its outputs; it is stored in
 // `stress` for the force stage, which then only adds it (and buoyancy).
+// NP = the number of phases (1-3): each neighbour record holds NP drifts.
+template <int NP>
 __global__ __launch_bounds__(MF_THREADS) void k_mf_alpha(
     KP k, MFP m, NbrList L, const uint32_t* __restrict__ key, const int* __restrict__ cs,
     const float4* __restrict__ pos, const float4* __restrict__ vel,
@@ -176,20 +178,18 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_alpha(
   float apre[3], ach[3];
   float3 ui[3];
   for (int f = 0; f < 3; ++f) {
-    apre[f] = f < m.nphase ? comp(ai, f) : 0.f;
+    apre[f] = f < NP ? comp(ai, f) : 0.f;
     ach[f] = 0.f;
-    ui[f] = f < m.nphase ? xyz(dr[f][i]) : f3(0.f, 0.f, 0.f);
+    ui[f] = f < NP ? xyz(dr[f][i]) : f3(0.f, 0.f, 0.f);
   }
   float fx = 0.f, fy = 0.f, fz = 0.f;
-  struct Rec { float4 r, v, u[3]; };
-  const bool three = m.nphase > 2;
+  struct Rec { float4 r, v, u[NP]; };
   for_each_listed_b(k, L, cs, gc, i, [&](int j) {
     Rec o;
     o.r = rec[j];
     o.v = vel[j];
-    o.u[0] = drift0[j];
-    o.u[1] = drift1[j];
-    o.u[2] = three ? drift2[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int f = 0; f < NP; ++f) o.u[f] = dr[f][j];
     return o;
   }, [&](int, const Rec& o) {
     const float4 rj = o.r;
@@ -203,8 +203,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_alpha(
     float cm = cubic_gradf(r, ir, k.inv_h, k.kspline) * rj.w;
     float vmrd = (vj.x - vi.x) * dx + (vj.y - vi.y) * dy + (vj.z - vi.z) * dz;
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      if (f >= m.nphase) break;
+    for (int f = 0; f < NP; ++f) {
       const float4 uj = o.u[f];                                          // u_f, alpha_f
       float ajf = uj.w;
       ach[f] -= 0.5f * cm * (ajf + apre[f]) * vmrd;                        // :207
@@ -226,7 +225,7 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_alpha(
   for (int f = 0; f < 3; ++f) {
     float d = ach[f] * k.dt;
     if (d < -0.99f) d = -0.99f;                                            // :258-261
-    o[f] = f < m.nphase ? d + apre[f] : 0.f;
+    o[f] = f < NP ? d + apre[f] : 0.f;
   }
   alpha_new[i] = make_float4(o[0], o[1], o[2], ai.w);
 }
@@ -593,7 +592,8 @@ int sphx_mf_alpha_advance(sphx_context* c) {
   SPHX_TRY(nbr_list(c, &L));
   SPHX_TRY(mf_need_rec(c));
   timer_begin(c, "mf_alpha");
-  k_mf_alpha<<<blocks_for(k.n, c->tune[T_MF_ALPHA]), c->tune[T_MF_ALPHA], 0, c->stream>>>(
+  auto* kern = m.nphase == 1 ? k_mf_alpha<1> : (m.nphase == 2 ? k_mf_alpha<2> : k_mf_alpha<3>);
+  kern<<<blocks_for(k.n, c->tune[T_MF_ALPHA]), c->tune[T_MF_ALPHA], 0, c->stream>>>(
       k, m, L, c->key.p, c->cell_start.p, c->POS(), c->VEL(), c->PHYS(), c->mf_alpha[c->cur].p,
       c->mf_drift[0].p, c->mf_drift[1].p, c->mf_drift[2].p, c->mf_rec.p, c->mf_alpha_new.p,
       c->mf_stress.p);

@@ -127,6 +127,7 @@ def lib() -> C.CDLL:
         L.sphx_download.argtypes = [VP, VP, VP, I, I]
         L.sphx_num_particles.argtypes = [VP]
         L.sphx_download_grid.argtypes = [VP, VP, VP, VP, VP, VP, VP]
+        L.sphx_nbr_counts.argtypes = [VP, VP, VP]
         for name in ["sphx_grid_hash", "sphx_grid_sort", "sphx_grid_reorder", "sphx_grid_build",
                      "sphx_boundary_volume", "sphx_wcsph_density_pressure", "sphx_wcsph_force",
                      "sphx_wcsph_advance", "sphx_wcsph_step", "sphx_iisph_density",
@@ -457,6 +458,14 @@ class Context:
                                                    "gridcnt", "MFidTable"]]),
               "sphx_download_grid")
         return out
+
+    def nbr_counts(self):
+        """Per-particle neighbour counts of the step's list (-1: overflowed to
+        the cell scan) and the list capacity."""
+        out = np.zeros(self.num, np.int32)
+        cap = C.c_int()
+        check(self.L.sphx_nbr_counts(self.h, ptr(out), C.byref(cap)), "sphx_nbr_counts")
+        return out, cap.value
 
     def download_iisph(self):
         n = self.num

@@ -82,3 +82,72 @@ def test_long_rows_fall_back_to_cell_scan(gpu):
           flush=True)
     o = options(wcsph_kernel=1, full_support=1, boundary_pressure=1, omega=0.5, clamp_pressure=1)
     run_iisph_passes(p, disp, calc, o)
+
+
+def brute_counts(pos, typ, valid, ss, h):
+    """Neighbours j != i with |x_ij ss|^2 < 4 h^2 (FP32, as the list build
+    tests), bracketed by the counts at 4 h^2 (1 -+ 1e-5) for pairs whose
+    FMA-contracted distance may round either way."""
+    pos = pos.astype(np.float32)
+    rc2 = np.float32(4.0) * np.float32(h) * np.float32(h)
+    lo = np.zeros(len(pos), np.int64)
+    hi = np.zeros(len(pos), np.int64)
+    for i in np.where(valid)[0]:
+        d = (pos[i] - pos) * np.float32(ss)
+        dsq = np.sum(d * d, axis=1, dtype=np.float32)
+        dsq[i] = np.inf
+        lo[i] = np.count_nonzero(dsq < rc2 * np.float32(1 - 1e-5))
+        hi[i] = np.count_nonzero(dsq < rc2 * np.float32(1 + 1e-5))
+    return lo, hi
+
+
+def test_list_counts_match_brute_force(gpu):
+    """Per-particle list counts (boundary neighbours included) against an
+    all-pairs count on a jittered block; boundary and out-of-grid particles
+    report 0."""
+    s = h = 0.01
+    p, disp, calc, info = sphx.scene_dambreak(12, 10, 9, s, (0.18, 0.16, 0.15), h, 2, 1e-4, 5, 1)
+    fl = np.where(disp["type"] == 0)[0]
+    rng = np.random.default_rng(6)
+    disp["pos"][fl] += rng.uniform(-0.3 * s, 0.3 * s, (len(fl), 3)).astype(np.float32)
+    o = options(wcsph_kernel=1, full_support=1, boundary_pressure=1)
+    with sphx.Context(p, o) as ctx:
+        ctx.upload(disp, calc)
+        ctx.call("grid_build")
+        cnt, cap = ctx.nbr_counts()
+        d, _ = ctx.download()
+        g = ctx.download_grid()
+    valid = (d["type"] == 0) & (g["mgcell"] != UNDEF)
+    lo, hi = brute_counts(d["pos"], d["type"], valid, p.simscale, p.smoothradius)
+    print(f"{valid.sum()} fluid particles, cap {cap}, counts {cnt[valid].min()}..{cnt[valid].max()} "
+          f"mean {cnt[valid].mean():.2f}; {np.sum(hi > lo)} with a pair at the radius", flush=True)
+    assert np.all(cnt[~valid] == 0)
+    assert np.all((cnt[valid] >= lo[valid]) & (cnt[valid] <= hi[valid]))
+    assert cnt[valid].max() > 20
+
+
+def test_crowded_counts_report_overflow(gpu):
+    """Particles of the half-spacing clump have more neighbours than the
+    capacity and report -1 (cell scan); the others report their count."""
+    s = h = 0.01
+    p, disp, calc, info = sphx.scene_dambreak(10, 10, 10, s, (0.16, 0.16, 0.16), h, 2, 1e-4, 3, 1)
+    fl = np.where(disp["type"] == 0)[0]
+    pos = disp["pos"][fl]
+    c = pos.mean(axis=0)
+    near = fl[np.max(np.abs(pos - c), axis=1) < 0.021]
+    disp["pos"][near] = c + (disp["pos"][near] - c) * 0.5
+    o = options(wcsph_kernel=1, full_support=1, boundary_pressure=1)
+    with sphx.Context(p, o) as ctx:
+        ctx.upload(disp, calc)
+        ctx.call("grid_build")
+        cnt, cap = ctx.nbr_counts()
+        d, _ = ctx.download()
+        g = ctx.download_grid()
+    valid = (d["type"] == 0) & (g["mgcell"] != UNDEF)
+    lo, hi = brute_counts(d["pos"], d["type"], valid, p.simscale, p.smoothradius)
+    over = valid & (lo > cap)
+    fits = valid & (hi <= cap)
+    print(f"cap {cap}: {over.sum()} over, {np.sum(cnt == -1)} report -1", flush=True)
+    assert over.sum() > 0
+    assert np.all(cnt[over] == -1)
+    assert np.all((cnt[fits] >= lo[fits]) & (cnt[fits] <= hi[fits]))
