@@ -590,7 +590,10 @@ __global__ void k_ii_integrate(KP k, const uint32_t* __restrict__ key, float4* _
 // Also seeds ps[i] = {x, y, z, 0}: the pressure pass gathers position and
 // s_j = m_j p_j / rho_j^2 as one record (s stays 0 for boundary particles,
 // whose pressure is not an unknown).
-constexpr int NB_STAGE = 3072;   // candidate positions staged per workgroup (48 KB)
+// Candidate positions staged per workgroup, as 12-byte {x, y, z} (37.5 KB:
+// four workgroups per CU).
+constexpr int NB_STAGE = 3200;
+struct StagedPos { float x, y, z; };
 constexpr int NB_THREADS_BUILD = 256;
 
 __device__ __forceinline__ int nb_wave_min(int v) {
@@ -616,7 +619,7 @@ __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
     KP k, const uint32_t* __restrict__ key, const int* __restrict__ cs,
     const float4* __restrict__ pos, uint4* __restrict__ idx, int* __restrict__ cnt,
     float4* __restrict__ ps, float4* __restrict__ mqa, float2* __restrict__ mqb, int cap) {
-  __shared__ float4 spos[NB_STAGE];
+  __shared__ StagedPos spos[NB_STAGE];
   __shared__ int sh[7];           // row r0, row r1, key min/max of r0, of r1, bad
   __shared__ int rs[19], ro[19];  // range (row, neighbour row) begin; LDS offsets
   const int t = threadIdx.x, lane = t & 63;
@@ -737,14 +740,18 @@ __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
     for (int u = t; u < total; u += blockDim.x) {
       int q = 0;
       while (u >= ro[q + 1]) ++q;
-      spos[u] = pos[rs[q] + (u - ro[q])];
+      const float4 pj = pos[rs[q] + (u - ro[q])];
+      spos[u] = StagedPos{pj.x, pj.y, pj.z};
     }
     __syncthreads();
     if (valid) {
       const int* rsR = rs + 9 * R;
       const int* roR = ro + 9 * R;
       // row r's candidates sit at LDS offset roR[r] - rsR[r] + j
-      scan_rows([&](int r) { return roR[r] - rsR[r]; }, [&](int u) { return spos[u]; });
+      scan_rows([&](int r) { return roR[r] - rsR[r]; }, [&](int u) {
+        const StagedPos q = spos[u];
+        return make_float4(q.x, q.y, q.z, 0.f);
+      });
     }
   } else if (valid) {
     scan_rows([&](int) { return 0; }, [&](int j) { return pos[j]; });
