@@ -287,6 +287,7 @@ struct sphx_context {
     sphx::DevBuf<float> w, m, v, g;
     sphx::DevBuf<uint16_t> w16;         // FP16 bits of w
     sphx::DevBuf<uint4> wmf;            // w16 in the MFMA operand layout
+    bool wmf_ok = false;                // wmf packed from the current w16
     sphx::DevBuf<float> X, pprev;       // per slot, current step
     sphx::DevBuf<float4> unew;          // drift with the converged pressure
     sphx::DevBuf<float> sX, sT, sM;     // recorded samples

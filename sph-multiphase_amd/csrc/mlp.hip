@@ -230,31 +230,21 @@ __device__ void mlp_forward_mfma(const MlpMfmaShared& S, const float* __restrict
 
 // Writes the warm start p0 = max(0, p(t-1) + P y0 / 10) (and s = m p0 /
 // rho^2 for the pressure pass) when `warm`, and the drift correction for
-// the next step when dcorr is non-null.  256 threads: four waves of 64
-// particles.
-__global__ __launch_bounds__(256, 2) void k_mlp_infer(
-    KP k, MLPP mp, int warm, const float* __restrict__ X, const uint4* __restrict__ wmf,
-    const float4* __restrict__ pos, const uint32_t* __restrict__ key,
-    const float4* __restrict__ phys, float* __restrict__ press_l, float4* __restrict__ ps,
-    float4* __restrict__ dcorr) {
-  __shared__ MlpMfmaShared S;
-  __shared__ float ys[4][64 * MLP_NO];
-  load_mfma(S, wmf);
-  __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int base = xcd_tile() * (int)blockDim.x + 64 * wave;
-  if (base >= k.n) return;   // whole wave past the end
-  mlp_forward_mfma(S, X, k.n, base, ys[wave]);
-  // the wave's LDS writes above reach LDS before its reads below
-  asm volatile("" ::: "memory");
-  const int i = base + lane;
-  if (i >= k.n) return;
+// the next step when dcorr is non-null.  Workgroups of up to four waves of
+// 64 particles walk the 256-particle tiles grid-stride, so each stages the
+// weights in LDS once for several tiles.
+__device__ __forceinline__ void mlp_infer_out(const KP& k, const MLPP& mp, int warm, int i,
+                                              const float* y, const float4* __restrict__ pos,
+                                              const uint32_t* __restrict__ key,
+                                              const float4* __restrict__ phys,
+                                              float* __restrict__ press_l,
+                                              float4* __restrict__ ps,
+                                              float4* __restrict__ dcorr) {
   float4 pi = pos[i];
   if (type_of(pi) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) {
     if (dcorr) dcorr[i] = make_float4(0.f, 0.f, 0.f, dcorr[i].w);
     return;
   }
-  const float* y = ys[wave] + lane * MLP_NO;
   if (warm) {
     const float4 qi = phys[i];
     float p0 = fmaxf(0.f, fmaxf(0.f, qi.z) + y[0] * (mp.p_scale / MLP_TSCALE));
@@ -268,6 +258,36 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(
     if (!(d.x == d.x && d.y == d.y && d.z == d.z)) d = make_float4(0.f, 0.f, 0.f, d.w);
     dcorr[i] = d;
   }
+}
+
+__global__ __launch_bounds__(256, 2) void k_mlp_infer(
+    KP k, MLPP mp, int warm, const float* __restrict__ X, const uint4* __restrict__ wmf,
+    const float4* __restrict__ pos, const uint32_t* __restrict__ key,
+    const float4* __restrict__ phys, float* __restrict__ press_l, float4* __restrict__ ps,
+    float4* __restrict__ dcorr) {
+  __shared__ MlpMfmaShared S;
+  __shared__ float ys[4][64 * MLP_NO];
+  load_mfma(S, wmf);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tiles = (k.n + (int)blockDim.x - 1) / (int)blockDim.x;
+  for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const int base = tile * (int)blockDim.x + 64 * wave;
+    if (base >= k.n) continue;   // whole wave past the end
+    mlp_forward_mfma(S, X, k.n, base, ys[wave]);
+    // the wave's LDS writes above reach LDS before its reads below, and
+    // its reads before the next tile's writes
+    asm volatile("" ::: "memory");
+    const int i = base + lane;
+    if (i < k.n)
+      mlp_infer_out(k, mp, warm, i, ys[wave] + lane * MLP_NO, pos, key, phys, press_l, ps, dcorr);
+    asm volatile("" ::: "memory");
+  }
+}
+
+// Workgroups of the inference: one per tile up to four per CU.
+static int mlp_infer_blocks(int n, int threads) {
+  return std::min(blocks_for(n, threads), 4 * 256);
 }
 
 // The network on stored feature rows (tests, evaluation): the same MFMA
@@ -290,13 +310,16 @@ __global__ __launch_bounds__(256, 2) void k_mlp_eval(const float* __restrict__ X
 }
 
 
-// The weights in the MFMA operand layout (one small launch per use).
+// The weights in the MFMA operand layout.  Launched on the context stream
+// right after every write of the FP16 weights (so the inference, eager or
+// in a step graph, reads a layout that matches them in stream order).
 static int mlp_pack_mfma(sphx_context* c) {
   auto& m = c->mlp;
   SPHX_TRY(m.wmf.alloc(MLP_MFMA_U4));
   k_mlp_mfma_pack<<<1, 256, 0, c->stream>>>(reinterpret_cast<const __half*>(m.w16.p),
                                             reinterpret_cast<MlpMfmaShared*>(m.wmf.p));
   SPHX_HIP(hipGetLastError());
+  m.wmf_ok = true;
   return 0;
 }
 
@@ -649,7 +672,7 @@ static int mlp_refresh16(sphx_context* c) {
   k_f32_to_f16<<<blocks_for(MLP_NP, 256), 256, 0, c->stream>>>(
       m.w.p, reinterpret_cast<__half*>(m.w16.p), MLP_NP);
   SPHX_HIP(hipGetLastError());
-  return 0;
+  return mlp_pack_mfma(c);
 }
 
 int iisph_solve_plain(sphx_context* c, int* iters, double* err);
@@ -679,6 +702,8 @@ static int mlp_refresh_launch(sphx_context* c, __half* w16) {
         m.w.p, m.m.p, m.v.p, m.g.p, w16, MLP_NP, m.online_lr, b1, b2, eps, bc1, bc2);
     SPHX_HIP(hipGetLastError());
   }
+  // the staged copy is packed when it is adopted
+  if (w16 == reinterpret_cast<__half*>(m.w16.p)) SPHX_TRY(mlp_pack_mfma(c));
   return 0;
 }
 
@@ -730,6 +755,7 @@ int mlp_async_settle(sphx_context* c) {
   SPHX_HIP(hipStreamWaitEvent(c->stream, m.ev_train, 0));
   k_u16_copy<<<blocks_for(MLP_NP, 256), 256, 0, c->stream>>>(m.w16_stage.p, m.w16.p, MLP_NP);
   SPHX_HIP(hipGetLastError());
+  SPHX_TRY(mlp_pack_mfma(c));
   m.train_pending = 0;
   m.train_inflight = false;
   return 0;
@@ -773,9 +799,9 @@ int mlp_begin(sphx_context* c) {
   SPHX_HIP(hipGetLastError());
   float4* dcorr = m.drift && mf ? hist : nullptr;
   if (m.warm || dcorr) {
+    if (!m.wmf_ok) SPHX_TRY(mlp_pack_mfma(c));
     timer_begin(c, "mlp_infer");
-    SPHX_TRY(mlp_pack_mfma(c));
-    k_mlp_infer<<<blocks_for(k.n, c->tune[T_MLP_INFER]), c->tune[T_MLP_INFER], 0, c->stream>>>(
+    k_mlp_infer<<<mlp_infer_blocks(k.n, c->tune[T_MLP_INFER]), c->tune[T_MLP_INFER], 0, c->stream>>>(
         k, mp, m.warm, m.X.p, m.wmf.p, c->POS(), c->key.p, c->PHYS(), c->press_l.p, c->ps.p,
         dcorr);
     timer_end(c);
@@ -1218,7 +1244,7 @@ int sphx_mlp_train(sphx_context* c, int epochs, int batch, float lr, const float
       loss_out[ep] = l / nv;
     }
   }
-  return 0;
+  return mlp_pack_mfma(c);
 }
 
 // Learned solves so far, how many fell back to the plain start, and
