@@ -143,9 +143,10 @@ __device__ __forceinline__ void split_h8(const float v[8], h8& hi, h8& lo) {
   }
 }
 
-// Column tiles (of 16 particles) per pass of the forward: two passes of two
-// keep the live accumulators and operands at half.
-constexpr int MLP_TPP = 2;
+// Column tiles (of 16 particles) per pass of the forward: four passes of
+// one keep the live accumulators and operands at a quarter (142 VGPRs, three
+// waves per SIMD; two tiles per pass need 216, two waves).
+constexpr int MLP_TPP = 1;
 
 // Next layer's B operands from this layer's accumulators acc[m][t] (4 M
 // tiles of 16 units), with bias and ReLU.

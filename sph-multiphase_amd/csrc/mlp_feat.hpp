@@ -35,8 +35,10 @@ struct FeatAcc {
     wp += w * pjp;
     pmax = fmaxf(pmax, pjp);
     pmin = fminf(pmin, pjp);
-    wsrc += w * (qj.y - qj.w) / qj.w;
-    float v = qj.x / qj.y * cubic_gradf(r, ir, k.inv_h, k.kspline) * (pjp - p_i);
+    // quotients through v_rcp_f32 (1 ulp) rather than IEEE division: these
+    // are network inputs, not solver arithmetic
+    wsrc += w * (qj.y - qj.w) * __builtin_amdgcn_rcpf(qj.w);
+    float v = qj.x * __builtin_amdgcn_rcpf(qj.y) * cubic_gradf(r, ir, k.inv_h, k.kspline) * (pjp - p_i);
     gpx += v * dx; gpy += v * dy; gpz += v * dz;
   }
 };
