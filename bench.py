@@ -67,6 +67,9 @@ ALGO_BYTES = {
     "mf_drift": 64,        # x, p, m, rho, alpha(2), nu, g-a -> u_m (2x12)
     "mf_alpha": 84,        # x, v, m, rho, alpha, u_m -> alpha_new, drift stress
     "mf_force": 52,        # drift stress, rho_m, v_adv -> v_adv, a_mf
+    # correction and force add in one pass (the composite steps): alpha_new,
+    # alpha_pre, type, key, stress, v_adv -> alpha_new, nu, v_adv, a_mf
+    "mf_correction_force": 124,
     "radix_sort": 32,      # 2 passes x (key+val read, key+val write)
     "reorder": 104,        # 5 x 16 B + 2 x 4 B gathered and written
     "nbr_build": 24,       # x read, count + list head written
@@ -634,7 +637,7 @@ def main():
             print(f"# warmup step {s}: {it} iterations, error {err:.2e}", file=sys.stderr,
                   flush=True)
     names = ["hash", "radix_sort", "cell_start", "reorder", "nbr_build", "iisph_density_dii",
-             "mf_drift", "mf_alpha", "mf_correction", "mf_force", "iisph_aii",
+             "mf_drift", "mf_alpha", "mf_correction_force", "iisph_aii",
              "mlp_features", "mlp_infer", "iisph_dp", "iisph_iter", "iisph_integrate",
              "mlp_record", "mlp_train"]
 

@@ -20,6 +20,7 @@ namespace sphx {
 
 int enqueue_solve_fixed(sphx_context* c, int n);
 int iisph_density_dii(sphx_context* c);
+int mf_correction_force(sphx_context* c);
 int continue_solve(sphx_context* c, int* iters, double* err);
 int mlp_begin(sphx_context* c);
 int mlp_fallback(sphx_context* c, int* iters, double* err);
@@ -49,8 +50,7 @@ static int stage_a(sphx_context* c, int budget) {
   SPHX_TRY(iisph_density_dii(c));
   SPHX_TRY(sphx_mf_drift_velocity(c));
   SPHX_TRY(sphx_mf_alpha_advance(c));
-  SPHX_TRY(sphx_mf_correction(c));
-  SPHX_TRY(sphx_mf_force(c));
+  SPHX_TRY(mf_correction_force(c));
   SPHX_TRY(sphx_iisph_predict_aii(c));
   SPHX_TRY(mlp_begin(c));
   return enqueue_solve_fixed(c, budget);

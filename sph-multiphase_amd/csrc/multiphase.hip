@@ -233,14 +233,12 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_alpha(
 // mfComputeCorrection (multiphase.cu:275-349): clamp at 0, normalise, or
 // one-hot the phase that dominated before the step; mixture density and
 // viscosity.  (The WCSPH pressure modification of :330-331 has no IISPH
-// counterpart: the pressure solve enforces the density.)
-__global__ void k_mf_correction(KP k, MFP m, const float4* __restrict__ pos,
-                                const uint32_t* __restrict__ key,
-                                const float4* __restrict__ alpha_pre,
-                                float4* __restrict__ alpha_new, float4* __restrict__ vel) {
-  int i = xcd_index();
-  if (i >= k.n) return;
-  if (type_of(pos[i]) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) return;
+// counterpart: the pressure solve enforces the density.)  One particle;
+// returns its mixture density rho_m.
+__device__ __forceinline__ float mf_correct_one(const MFP& m, int i,
+                                                const float4* __restrict__ alpha_pre,
+                                                float4* __restrict__ alpha_new,
+                                                float4* __restrict__ vel) {
   float4 a = alpha_new[i];
   float4 ap = alpha_pre[i];
   float v[3] = {a.x, a.y, a.z};
@@ -263,6 +261,17 @@ __global__ void k_mf_correction(KP k, MFP m, const float4* __restrict__ pos,
   for (int f = m.nphase; f < 3; ++f) v[f] = 0.f;
   alpha_new[i] = make_float4(v[0], v[1], v[2], rho_m);
   vel[i].w = nu_m;
+  return rho_m;
+}
+
+__global__ void k_mf_correction(KP k, MFP m, const float4* __restrict__ pos,
+                                const uint32_t* __restrict__ key,
+                                const float4* __restrict__ alpha_pre,
+                                float4* __restrict__ alpha_new, float4* __restrict__ vel) {
+  int i = xcd_index();
+  if (i >= k.n) return;
+  if (type_of(pos[i]) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) return;
+  mf_correct_one(m, i, alpha_pre, alpha_new, vel);
 }
 
 // Non-pressure mixture acceleration added to the advected velocity:
@@ -333,6 +342,23 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_force(
 
 // The force stage after a fused alpha pass: drift stress from `stress`,
 // plus buoyancy; same operations and order as k_mf_force's tail.
+// The drift stress summed by the alpha pass plus buoyancy, added to one
+// particle's advected velocity; rho_m = its corrected mixture density.
+__device__ __forceinline__ void mf_force_add_one(const KP& k, const MFP& m, int i, float rho_m,
+                                                 const float4* __restrict__ stress,
+                                                 float4* __restrict__ veleval,
+                                                 float4* __restrict__ amf) {
+  const float4 st = stress[i];
+  float fx = st.x, fy = st.y, fz = st.z;
+  if (m.boussinesq) {
+    float b = (rho_m - m.rho_ref) / m.rho_ref;
+    fx += b * k.g.x; fy += b * k.g.y; fz += b * k.g.z;
+  }
+  float4 ve = veleval[i];
+  veleval[i] = make_float4(ve.x + fx * k.dt, ve.y + fy * k.dt, ve.z + fz * k.dt, ve.w);
+  amf[i] = make_float4(fx, fy, fz, 0.f);
+}
+
 __global__ void k_mf_force_add(KP k, MFP m, const float4* __restrict__ pos,
                                const uint32_t* __restrict__ key,
                                const float4* __restrict__ alpha_new,
@@ -341,15 +367,21 @@ __global__ void k_mf_force_add(KP k, MFP m, const float4* __restrict__ pos,
   int i = xcd_index();
   if (i >= k.n) return;
   if (type_of(pos[i]) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) return;
-  const float4 st = stress[i];
-  float fx = st.x, fy = st.y, fz = st.z;
-  if (m.boussinesq) {
-    float b = (alpha_new[i].w - m.rho_ref) / m.rho_ref;
-    fx += b * k.g.x; fy += b * k.g.y; fz += b * k.g.z;
-  }
-  float4 ve = veleval[i];
-  veleval[i] = make_float4(ve.x + fx * k.dt, ve.y + fy * k.dt, ve.z + fz * k.dt, ve.w);
-  amf[i] = make_float4(fx, fy, fz, 0.f);
+  mf_force_add_one(k, m, i, m.boussinesq ? alpha_new[i].w : 0.f, stress, veleval, amf);
+}
+
+// k_mf_correction then k_mf_force_add (same particles skipped), in one pass.
+__global__ void k_mf_correct_force(KP k, MFP m, const float4* __restrict__ pos,
+                                   const uint32_t* __restrict__ key,
+                                   const float4* __restrict__ alpha_pre,
+                                   float4* __restrict__ alpha_new, float4* __restrict__ vel,
+                                   const float4* __restrict__ stress,
+                                   float4* __restrict__ veleval, float4* __restrict__ amf) {
+  int i = xcd_index();
+  if (i >= k.n) return;
+  if (type_of(pos[i]) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) return;
+  const float rho_m = mf_correct_one(m, i, alpha_pre, alpha_new, vel);
+  mf_force_add_one(k, m, i, rho_m, stress, veleval, amf);
 }
 
 __global__ void k_mf_reorder(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ idsort,
@@ -646,16 +678,45 @@ int sphx_mf_force(sphx_context* c) {
 
 int sphx_grid_build(sphx_context* c);
 int sphx_iisph_predict_aii(sphx_context* c);
+
+}  // extern "C"
+
+namespace sphx {
+
+// sphx_mf_correction then sphx_mf_force: one pointwise pass when the alpha
+// pass summed the drift stress (the composite steps), else the two calls.
+int mf_correction_force(sphx_context* c) {
+  SPHX_TRY(need_mf(c, "sphx_mf_correction"));
+  if (!c->mf_stress_ok) {
+    SPHX_TRY(sphx_mf_correction(c));
+    return sphx_mf_force(c);
+  }
+  KP k = make_kp(c);
+  if (k.n == 0) return 0;
+  MFP m = make_mfp(c);
+  timer_begin(c, "mf_correction_force");
+  k_mf_correct_force<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+      k, m, c->POS(), c->key.p, c->mf_alpha[c->cur].p, c->mf_alpha_new.p, c->VEL(),
+      c->mf_stress.p, c->VELEVAL(), c->mf_amf.p);
+  timer_end(c);
+  SPHX_HIP(hipGetLastError());
+  c->mf_pending = true;
+  return 0;
+}
+
+}  // namespace sphx
+
+extern "C" {
 int sphx_iisph_pressure_solve(sphx_context* c, int* iters, double* err);
 int sphx_iisph_integrate(sphx_context* c);
 
 // One two-phase step: grid build, density with the advected velocity and
-// d_ii (one pass), drift velocity, fraction advection and correction,
-// drift stress and buoyancy added to the advected velocity, a_ii, pressure
-// solve to tolerance, integration.  The same arithmetic as the calls
-// density, drift, alpha, correction, predict_dii, force, ...: the fraction
-// passes change neither x, v.xyz, m nor rho0, which are all d_ii and v_adv
-// read.
+// d_ii (one pass), drift velocity, fraction advection (+ drift stress),
+// correction with the drift stress and buoyancy added to the advected
+// velocity (one pointwise pass), a_ii, pressure solve to tolerance,
+// integration.  The same arithmetic as the calls density, drift, alpha,
+// correction, predict_dii, force, ...: the fraction passes change neither
+// x, v.xyz, m nor rho0, which are all d_ii and v_adv read.
 int sphx_mf_step(sphx_context* c, int* iters, double* err) {
   SPHX_CHECK(c, "sphx_mf_step", "null context");
   SPHX_CHECK(c->nphase > 0, "sphx_mf_step", "multiphase not set up");
@@ -663,8 +724,7 @@ int sphx_mf_step(sphx_context* c, int* iters, double* err) {
   SPHX_TRY(iisph_density_dii(c));
   SPHX_TRY(sphx_mf_drift_velocity(c));
   SPHX_TRY(sphx_mf_alpha_advance(c));
-  SPHX_TRY(sphx_mf_correction(c));
-  SPHX_TRY(sphx_mf_force(c));
+  SPHX_TRY(mf_correction_force(c));
   SPHX_TRY(sphx_iisph_predict_aii(c));
   SPHX_TRY(sphx_iisph_pressure_solve(c, iters, err));
   return sphx_iisph_integrate(c);

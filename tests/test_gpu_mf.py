@@ -198,3 +198,39 @@ def test_density_dii_pass_equals_two_passes(gpu):
         print(f"{name}: one pass vs two {err:.3e} (exact {np.array_equal(a, b)})", flush=True)
         assert np.max(np.abs(b)) > 0
         assert err <= 1e-6, name
+
+
+def test_mf_step_equals_reference_call_order(gpu):
+    """sphx_mf_step fuses density with d_ii and the correction with the
+    force add; two steps of it give bitwise the state of two steps of the
+    reference's call order (grid, density, drift, alpha, correction,
+    predict_dii, force, predict_aii, solve, integrate)."""
+    p, disp, calc, alpha, o = setup()
+    out = []
+    for fused in (True, False):
+        with sphx.Context(p, o) as ctx:
+            ctx.upload(disp, calc)
+            ctx.mf_setup(RHO, NU)
+            ctx.mf_upload(alpha)
+            ctx.call("grid_build")
+            its = []
+            for _ in range(2):
+                if fused:
+                    its.append(ctx.mf_step()[0])
+                else:
+                    for s in ["grid_build", "iisph_density", "mf_drift_velocity",
+                              "mf_alpha_advance", "mf_correction", "iisph_predict_dii",
+                              "mf_force", "iisph_predict_aii"]:
+                        ctx.call(s)
+                    its.append(ctx.pressure_solve()[0])
+                    ctx.call("iisph_integrate")
+            d, c = ctx.download()
+            out.append((its, d, c, ctx.mf_download()))
+    (ia, da, ca, ma), (ib, db, cb, mb) = out
+    print(f"iterations fused {ia} calls {ib}; max |dx| {np.max(np.abs(da['pos'] - db['pos'])):.3e}",
+          flush=True)
+    assert ia == ib
+    np.testing.assert_array_equal(da["pos"], db["pos"])
+    np.testing.assert_array_equal(ca["vel"], cb["vel"])
+    np.testing.assert_array_equal(ca["pressure"], cb["pressure"])
+    np.testing.assert_array_equal(ma["alpha"], mb["alpha"])
