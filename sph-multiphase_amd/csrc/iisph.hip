@@ -1030,9 +1030,11 @@ int iisph_solve_plain(sphx_context* c, int* iters, double* err) {
   SPHX_TRY(solve_reset(c));
   int enq = 0;
   int chunk = std::max(c->O.min_iters, 2);
-  // enqueue about as many iterations as the previous solve needed before
-  // the first host check, then two at a time
-  if (c->last_iters - 1 > chunk) chunk = c->last_iters - 1;
+  // enqueue one more iteration than the previous solve needed before the
+  // first host check, then two at a time: a host round trip idles the GPU
+  // for longer than an iteration queued past convergence costs (its
+  // kernels exit on the device flag)
+  if (c->last_iters + 1 > chunk) chunk = c->last_iters + 1;
   for (;;) {
     int todo = std::min(chunk, c->O.max_iters - enq);
     for (int t = 0; t < todo; ++t) SPHX_TRY(enqueue_iteration(c, k, 1));
