@@ -570,10 +570,10 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--drift", type=int, default=0, help="learned drift closure on")
     ap.add_argument("--ring-steps", type=int, default=12, help="steps of samples kept")
-    ap.add_argument("--online-every", type=int, default=10)
-    ap.add_argument("--online-iters", type=int, default=16)
+    ap.add_argument("--online-every", type=int, default=5)
+    ap.add_argument("--online-iters", type=int, default=8)
     ap.add_argument("--online-batch", type=int, default=8192)
-    ap.add_argument("--online-lr", type=float, default=5e-4)
+    ap.add_argument("--online-lr", type=float, default=1e-3)
     ap.add_argument("--record-stride", type=int, default=8,
                     help="online recording keeps every n-th particle of a step")
     ap.add_argument("--online-async", type=int, default=1,
