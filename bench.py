@@ -570,6 +570,8 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--drift", type=int, default=0, help="learned drift closure on")
     ap.add_argument("--ring-steps", type=int, default=12, help="steps of samples kept")
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="steps per candidate workgroup size (0: 256 everywhere)")
     ap.add_argument("--online-every", type=int, default=5)
     ap.add_argument("--online-iters", type=int, default=8)
     ap.add_argument("--online-batch", type=int, default=8192)
@@ -635,6 +637,12 @@ def main():
         if s % 10 == 0:
             print(f"# warmup step {s}: {it} iterations, error {err:.2e}", file=sys.stderr,
                   flush=True)
+    if args.autotune > 0:
+        # workgroup sizes per neighbour-loop kernel from real steps (both
+        # contexts advance by the same steps; sizes change no arithmetic)
+        plain.autotune(args.autotune)
+        tuned = learn.autotune(args.autotune)
+        print(f"# autotuned workgroup sizes: {tuned}", file=sys.stderr, flush=True)
     names = ["hash", "radix_sort", "cell_start", "reorder", "nbr_build", "iisph_density_dii",
              "mf_drift", "mf_alpha", "mf_correction_force", "iisph_aii",
              "mlp_features", "mlp_infer", "iisph_dp", "iisph_iter", "iisph_integrate",

@@ -323,7 +323,10 @@ struct sphx_context {
   int graph_budget = 0, graph_num = -1;
 
   // workgroup sizes per kernel class (TuneId)
-  int tune[sphx::T_COUNT] = {256, 256, 256, 256, 256, 256, 256, 256, 256, 256, 256};
+  // defaults: the autotuner's picks on the 1M-particle two-phase tank
+  // (TuneId order: density, dii, aii, dp, iter, drift, alpha, force,
+  // features, infer, density+dii)
+  int tune[sphx::T_COUNT] = {256, 256, 64, 128, 128, 64, 64, 256, 256, 256, 64};
 
   // timing
   bool timing = false;
