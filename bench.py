@@ -35,13 +35,13 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE,
 # scaled by a 256 MiB copy (profiles/iisph_mf_step_1m_hbm_traffic.md)
 TRAFFIC_BYTES = {
-    "iisph_iter": 280.9e6,
-    "iisph_dp": 237.4e6,
-    "mf_alpha": 395.7e6,
-    "mf_drift": 271.6e6,
-    "iisph_aii": 492.1e6,    # with the learned start's features
-    "iisph_density_dii": 212.6e6,
-    "nbr_build": 162.3e6,
+    "iisph_iter": 280.6e6,
+    "iisph_dp": 236.9e6,
+    "mf_alpha": 395.0e6,
+    "mf_drift": 270.6e6,
+    "iisph_aii": 496.4e6,    # with the learned start's features
+    "iisph_density_dii": 210.0e6,
+    "nbr_build": 161.8e6,
     "mf_correction_force": 158.4e6,
     "mlp_infer": 174.0e6,
 }
