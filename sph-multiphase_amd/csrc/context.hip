@@ -381,6 +381,15 @@ int sphx_set_params(sphx_context* c, const sphx_ParamCarrier* p) {
   SPHX_CHECK(p->num >= 0 && p->num <= c->cap, "sphx_set_params", "num outside [0, capacity]");
   SPHX_TRY(validate_grid(*p));
   bool regrid = p->gridtotal != c->P.gridtotal;
+  // the sorted keys and cell ranges describe the old particle count and
+  // grid; neighbour loops must not read them until the grid is rebuilt
+  const sphx_ParamCarrier& o = c->P;
+  if (regrid || p->num != c->num || p->gridres.x != o.gridres.x || p->gridres.y != o.gridres.y ||
+      p->gridres.z != o.gridres.z || p->gridmin.x != o.gridmin.x || p->gridmin.y != o.gridmin.y ||
+      p->gridmin.z != o.gridmin.z || p->gridIdfac.x != o.gridIdfac.x ||
+      p->gridIdfac.y != o.gridIdfac.y || p->gridIdfac.z != o.gridIdfac.z ||
+      p->smoothradius != o.smoothradius || p->simscale != o.simscale)
+    c->sorted_valid = false;
   c->P = *p;
   c->P.maxNum = c->cap;
   c->num = p->num;
@@ -423,6 +432,7 @@ int sphx_upload(sphx_context* c, const sphx_displayPack* disp, const sphx_calcul
   SPHX_TRY(materialize_cold(c));
   mpm_invalidate(c);
   positions_changed(c);
+  c->sorted_valid = false;   // keys and cell ranges no longer match the particles
   std::vector<float4> pos(count), vel(count), ve(count), acc(count), phys(count), color(count), X(count);
   std::vector<int> born(count);
   std::vector<float> mat((size_t)count * 36);

@@ -122,3 +122,118 @@ def test_buoyancy_lifts_air():
     dv = (st["veleval"] - ve0)[:, 1] / st.params.dt
     assert np.allclose(dv[air], -9.81 * (1.2 - 1000.0) / 1000.0, rtol=1e-6)
     assert np.allclose(dv[water], 0.0, atol=1e-9)
+
+
+# -- closed forms of the drift velocity on a regular lattice -----------------
+# The drift-flux model of multiphase.cu:19-166 (Manninen et al. 1996, in the
+# SPH form of Ren et al. 2014) gives, per phase k,
+#   u_mk = tau (rho_k - sum_l c_l rho_l)(g - a)
+#          - tau cpress (grad p_k - sum_l c_l grad p_l)
+#          - cdiff sigma (grad alpha_k / alpha_k - sum_l c_l grad alpha_l / alpha_l)
+# with c_l = alpha_l rho_l / rho_m, p_k = alpha_k p and tau = relax / nu_i.
+# The SPH gradient of a linear field f is exactly M grad f with
+# M_i = sum_j V_j gradW_ij (x) (x_j - x_i); M is summed here by brute force
+# from the cubic spline's closed-form derivative, independently of the
+# oracle's cell traversal.
+
+def f32(x):
+    """The drift constants cross the C ABI as float."""
+    return float(np.float32(x))
+
+
+def cubic_dwdr(r, h, s):
+    """dW/dr of the cubic spline of normalisation s = 1 / (pi h^3) (pinned
+    by the lattice normalisation test), support 2h."""
+    q = r / h
+    return np.where(q <= 1.0, s / h * (-3.0 * q + 2.25 * q * q),
+                    np.where(q <= 2.0, -0.75 * s / h * (2.0 - q) ** 2, 0.0))
+
+
+def lattice_mf(n=12, s=0.01, h=0.012, **opt):
+    p, disp, calc, info = sphx.scene_dambreak(n, n, n, s, (n * s,) * 3, h, 0, 0.0, 1, 1)
+    o = options(wcsph_kernel=1, full_support=1, **opt)
+    st = orc.State(disp, calc, p, np.float64)
+    st.opts = o
+    st.grid_build()
+    assert np.all(st["type"] == 0) and np.all(st["mgcell"] != UNDEF)
+    x = st["pos"]
+    lo, hi = x.min(0), x.max(0)
+    inner = np.all((x > lo + 2 * h + 1e-9) & (x < hi - 2 * h - 1e-9), axis=1)
+    assert inner.sum() >= 100
+    st.a["mass"][:] = 1e-3
+    st.a["dens"][:] = 1003.0
+    V = st["mass"] / st["dens"]
+    h, ks = p.smoothradius, p.kspline      # the float32 values the oracle uses
+    assert ks == pytest.approx(1.0 / (np.pi * h ** 3), rel=1e-6)
+    M = np.zeros((st.n, 3, 3))
+    for i in np.where(inner)[0]:
+        d = x[i] - x                         # x_i - x_j
+        r = np.sqrt((d * d).sum(1))
+        nb = (r > 0) & (r < 2 * h)
+        gw = (cubic_dwdr(r[nb], h, ks) / r[nb])[:, None] * d[nb]        # grad_i W_ij
+        M[i] = (V[nb, None, None] * gw[:, :, None] * (-d[nb])[:, None, :]).sum(0)
+    return p, st, o, inner, M
+
+
+def test_drift_pressure_and_buoyancy_terms_closed_form():
+    """Uniform fractions and a linear pressure: the diffusion term vanishes
+    and u_mk = tau (rho_k - sum c rho)(g - a) - tau cpress (alpha_k - sum c
+    alpha) M grad p."""
+    relax, nu, cpress = f32(3e-11), 2e-6, f32(0.8)
+    p, st, o, inner, M = lattice_mf(drift_relax=relax, drift_pressure_scale=cpress,
+                                    drift_diffusion_scale=1.0, drift_diffusion=1e-4)
+    gp = np.array([120.0, -9000.0, 300.0])
+    st.a["pressure"][:] = 5000.0 + st["pos"] @ gp
+    st.a["visc"][:] = nu
+    alpha = np.tile([0.7, 0.3], (st.n, 1))
+    rho = np.array(RHO)
+    mf = orc.MF(st, RHO, NU, o)
+    mf.set_alpha(alpha, alpha @ rho)
+    acc = np.array([0.4, -9.81, -0.2])
+    mf.a["mforce"][:] = acc
+    mf.drift()
+    tau = relax / nu
+    rho_m = alpha[0] @ rho
+    c = alpha[0] * rho / rho_m
+    got = mf["drift"][inner][:, :2, :]
+    Mg = M[inner] @ gp                       # per particle (uniform on a lattice)
+    for k in range(2):
+        wk = (tau * (rho[k] - c @ rho) * acc[None, :]
+              - tau * cpress * (alpha[0, k] - c @ alpha[0]) * Mg)
+        scale = np.abs(wk).max()
+        assert scale > 0
+        assert np.abs(got[:, k, :] - wk).max() <= 1e-9 * scale, k
+    # both terms matter for the air phase: neither alone gives its drift
+    air_g = tau * (rho[1] - c @ rho) * acc
+    assert np.abs(got[:, 1, :] - air_g).max() > 0.1 * np.abs(air_g).max()
+    # the interior M is close to the identity (SPH gradient of a linear field)
+    assert np.allclose(M[inner], np.eye(3), atol=0.05)
+
+
+def test_drift_diffusion_term_closed_form():
+    """Linear fractions, no pressure and no acceleration: only the
+    diffusion term remains, u_mk = -cdiff sigma (M grad alpha_k / alpha_k -
+    sum_l c_l M grad alpha_l / alpha_l)."""
+    cdiff, sigma, nu = f32(1.3), f32(2e-4), 1e-6
+    p, st, o, inner, M = lattice_mf(drift_relax=2e-11, drift_pressure_scale=1.0,
+                                    drift_diffusion_scale=cdiff, drift_diffusion=sigma)
+    st.a["pressure"][:] = 0.0
+    st.a["visc"][:] = nu
+    ga = np.array([2.0, 1.5, -1.0])
+    a1 = 0.3 + (st["pos"] - 0.06) @ ga
+    assert a1.min() > 1e-3 and a1.max() < 0.999
+    alpha = np.stack([1.0 - a1, a1], 1)
+    rho = np.array(RHO)
+    mf = orc.MF(st, RHO, NU, o)
+    mf.set_alpha(alpha, alpha @ rho)
+    mf.a["mforce"][:] = 0.0
+    mf.drift()
+    got = mf["drift"][inner][:, :2, :]
+    a = alpha[inner]
+    c = a * rho / (a @ rho)[:, None]
+    grads = np.stack([M[inner] @ (-ga), M[inner] @ ga], 1)        # M grad alpha_k
+    st_k = grads / a[:, :, None]
+    want = -cdiff * sigma * (st_k - (c[:, :, None] * st_k).sum(1, keepdims=True))
+    scale = np.abs(want).max()
+    assert scale > 0
+    assert np.abs(got - want).max() <= 1e-9 * scale

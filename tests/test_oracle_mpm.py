@@ -142,3 +142,116 @@ def test_drucker_prager_consistency_condition():
     f_pl = f(sig + dt * (C_eps - pl))
     assert f_el > 1e-3 * dt * G
     assert abs(f_pl) < 1e-6 * abs(f_el)
+
+
+# -- rate-form stress (MpmSolver.cu:477-560) ---------------------------------
+# For a node velocity field v_I = L x_I + b the B-spline gradient is exactly
+# grad v = L (quadratic spline, physical units), and the Jaumann rate form
+# gives sigma' = sigma + dt (2G D + (K - 2G/3) tr(D) I + W sigma - sigma W)
+# with D = (L + L^T)/2, W = (L - L^T)/2.  Under a pure rotation the stress
+# must rotate rigidly: sigma' = R sigma R^T + O(dt^2), R = exp(W dt).
+
+def stress_state(L, b=(0.1, -0.2, 0.05), sigma0=None, typ=2, m0=0.5):
+    p, disp, calc, info = sphx.scene_sand(n=600, seed=11)
+    p.dt = 5e-5
+    disp["type"] = typ
+    st = orc.State(disp, calc, p, np.float64)
+    st.grid_build()
+    mp = orc.MPM(st, p, 1, 1, calc)
+    xn = node_positions(p)
+    mp["node_v"][:] = xn @ np.asarray(L).T + np.asarray(b)
+    mp["node_m"][:] = m0
+    s0 = np.array([[-3000.0, 200.0, -50.0], [200.0, -2000.0, 120.0], [-50.0, 120.0, -1000.0]])
+    if sigma0 is not None:
+        s0 = sigma0
+    mp["stress"][:] = s0.reshape(9)
+    G, K = 134615.0, 291667.0
+    mp["G"][:] = G
+    mp["K"][:] = K
+    ok = st["mgcell"] != UNDEF
+    assert ok.all()
+    mp.stress()
+    return p, st, mp, s0, G, K
+
+
+def jaumann_step(s0, L, G, K, dt):
+    D = 0.5 * (L + L.T)
+    W = 0.5 * (L - L.T)
+    return s0 + dt * (2 * G * D + (K - 2 * G / 3) * np.trace(D) * np.eye(3) + W @ s0 - s0 @ W)
+
+
+def test_stress_rate_pure_rotation_is_objective():
+    w = np.array([0.0, 1.5, -2.0])
+    L = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])   # v = w x x
+    p, st, mp, s0, G, K = stress_state(L)
+    dt = p.dt
+    s1 = mp["stress"].reshape(-1, 3, 3)
+    np.testing.assert_allclose(s1, np.broadcast_to(jaumann_step(s0, L, G, K, dt), s1.shape),
+                               rtol=0, atol=1e-9 * np.abs(s0).max())
+    # rigid rotation of the stress, to second order in dt
+    th = np.linalg.norm(w) * dt
+    k = w / np.linalg.norm(w)
+    Kx = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    R = np.eye(3) + np.sin(th) * Kx + (1 - np.cos(th)) * Kx @ Kx
+    rot = R @ s0 @ R.T
+    change = np.abs(s1[0] - s0).max()
+    assert change > 0
+    assert np.abs(s1 - rot).max() < 1e-3 * change
+    # particle density from the node masses: sum_I m_I w_Ip / dx^3 = m0 / dx^3
+    rc = p.mpmcellsize * p.simscale
+    np.testing.assert_allclose(st["dens"], 0.5 / rc ** 3, rtol=1e-12)
+
+
+def test_stress_rate_uniaxial_strain_moduli():
+    """v = (e x, 0, 0): d sigma_xx = (K + 4G/3) e dt, d sigma_yy = d sigma_zz
+    = (K - 2G/3) e dt, no shear change (constrained modulus and lambda)."""
+    e = 0.8
+    L = np.diag([e, 0.0, 0.0])
+    s0 = -1000.0 * np.eye(3)
+    p, st, mp, _, G, K = stress_state(L, sigma0=s0)
+    ds = (mp["stress"].reshape(-1, 3, 3) - s0) / (e * p.dt)
+    want = np.diag([K + 4 * G / 3, K - 2 * G / 3, K - 2 * G / 3])
+    np.testing.assert_allclose(ds, np.broadcast_to(want, ds.shape), rtol=1e-9, atol=1e-6 * K)
+
+
+def test_stress_rate_general_gradient():
+    rng = np.random.default_rng(4)
+    L = rng.normal(0, 1.0, (3, 3))
+    p, st, mp, s0, G, K = stress_state(L)
+    s1 = mp["stress"].reshape(-1, 3, 3)
+    np.testing.assert_allclose(s1, np.broadcast_to(jaumann_step(s0, L, G, K, p.dt), s1.shape),
+                               rtol=0, atol=1e-9 * np.abs(s0).max())
+
+
+# -- node update and grid collision (MpmSolver.cu:595-704) -------------------
+def test_node_update_gravity_and_soft_boundaries():
+    """Zero stress: every massive node gains g dt; then the soft bounds act
+    as a sticky floor (a node below the floor moving down stops), free-slip
+    side walls (the inward normal component of a node outside a wall is
+    removed) and an open top."""
+    p, disp, calc, info = sphx.scene_sand(n=600, seed=12)
+    p.dt = 1e-3
+    st = orc.State(disp, calc, p, np.float64)
+    st.grid_build()
+    mp = orc.MPM(st, p, 1, 1, calc)
+    mp["stress"][:] = 0.0
+    rng = np.random.default_rng(5)
+    N = len(mp["node_m"])
+    v0 = rng.normal(0, 1.0, (N, 3))
+    mp["node_v"][:] = v0
+    mp["node_m"][:] = 1.0
+    mp["node_m"][::7] = 0.0                # massless nodes are left alone
+    mp.node_update()
+    g = np.array(p.gravity.tolist(), np.float64)
+    x = node_positions(p)
+    lo = np.array(p.softminx.tolist(), np.float64)
+    hi = np.array(p.softmaxx.tolist(), np.float64)
+    v = v0 + g * p.dt
+    floor = (lo[1] - x[:, 1] > 1e-6) & (v[:, 1] < 0)
+    v[floor] = 0.0
+    for a in (0, 2):
+        v[(lo[a] - x[:, a] > 1e-6) & (v[:, a] < 0), a] = 0.0
+        v[(x[:, a] - hi[a] > 1e-6) & (v[:, a] > 0), a] = 0.0
+    v[::7] = v0[::7]
+    assert floor.sum() > 100 and (x[:, 1] - hi[1] > 1e-6).sum() > 100
+    np.testing.assert_allclose(mp["node_v"], v, rtol=1e-12, atol=1e-12)

@@ -236,3 +236,137 @@ def test_reference_variant_iterates_and_reduces_residual():
     errs = [st.ii_iteration()[1] for _ in range(10)]
     assert all(np.isfinite(errs))
     assert errs[-1] < errs[0]
+
+
+# ------------------------------------------------- boundary terms (Akinci) --
+# Boundary particles carry a volume V_b = 1 / sum_{b'} W(x_bb') over the
+# boundary particles around them (ComputeBoundaryVolume, kern.cu:249-277;
+# Akinci et al. 2012, "Versatile rigid-fluid coupling", eq. 4), and act on a
+# fluid particle with the pressure force a_i = -rho0_i V_b p_i / rho_i^2
+# gradW_ib (Akinci eq. 10 with Psi_b = rho0 V_b) and, while i approaches b,
+# the friction a_i = rho0_i V_b nu_b (v_ib . x_ib) / (|x_ib|^2 + 0.01 h^2)
+# gradW_ib with nu_b = bvisc h / (2 rho0_i) (Akinci eq. 11-12, the
+# reference's mapping of sigma c_s to bvisc and rho_i to rho0_i,
+# kern.cu:377-391).  The kernels below are written from their closed forms.
+
+def poly6_sum_term(r2, h):
+    return np.where(r2 < h * h, (h * h - r2) ** 3, 0.0)
+
+
+def cubic_w(r, h):
+    q = r / h
+    s = 1.0 / (np.pi * h ** 3)
+    return s * np.where(q <= 1.0, 1 - 1.5 * q * q + 0.75 * q ** 3,
+                        np.where(q <= 2.0, 0.25 * (2 - q) ** 3, 0.0))
+
+
+def grad_w(d, h, family):
+    """grad_i W for d = x_i - x_j: spiky 15/(pi h^6)(h - r)^3 (support h) or
+    the cubic spline (support 2h)."""
+    r = np.sqrt((d * d).sum())
+    if family == 0:
+        dwdr = -45.0 / (np.pi * h ** 6) * (h - r) ** 2 if r < h else 0.0
+    else:
+        q = r / h
+        s = 1.0 / (np.pi * h ** 3)
+        dwdr = s / h * (-3 * q + 2.25 * q * q) if q <= 1 else (
+            -0.75 * s / h * (2 - q) ** 2 if q <= 2 else 0.0)
+    return dwdr / r * d
+
+
+@pytest.mark.parametrize("family", [0, 1])
+def test_boundary_volume_brute_force(family):
+    """Every in-grid boundary particle's volume against an O(N^2) FP64 sum
+    over all boundary particles (no grid), with the kernel constants from
+    their closed forms; an isolated boundary particle has V = 1 / W(0)."""
+    h = 0.01 if family == 1 else 0.02
+    p, disp, calc, info = sphx.scene_dambreak(4, 6, 4, 0.01, (0.08, 0.1, 0.08), h, 2, 0.0, 1,
+                                              family)
+    st = orc.State(disp, calc, p, np.float64)
+    st.opts = options(wcsph_kernel=family)
+    st.grid_build()
+    st.boundary_volume()
+    hb = float(p.smoothradius)
+    b = (st["type"] == 1) & (st["mgcell"] != UNDEF)
+    assert b.sum() > 100
+    xb = st["pos"][st["type"] == 1]
+    for i in np.where(b)[0]:
+        d = st["pos"][i] - xb
+        r2 = (d * d).sum(1)
+        if family == 0:
+            want = 1.0 / (315.0 / (64 * np.pi * hb ** 9) * poly6_sum_term(r2, hb).sum())
+        else:
+            want = 1.0 / cubic_w(np.sqrt(r2), hb).sum()
+        assert st["dens"][i] == pytest.approx(want, rel=2e-6), i
+    # isolated boundary particle
+    d1, c1 = disp[:1].copy(), calc[:1].copy()
+    d1["type"] = 1
+    d1["pos"][0] = [0.04, 0.05, 0.04]
+    q = sphx.ParamCarrier.from_buffer_copy(p)
+    q.num = q.maxNum = 1
+    s1 = orc.State(d1, c1, q, np.float64)
+    s1.opts = options(wcsph_kernel=family)
+    s1.grid_build()
+    s1.boundary_volume()
+    w0 = 315.0 / (64 * np.pi * hb ** 9) * hb ** 6 if family == 0 else 1.0 / (np.pi * hb ** 3)
+    assert s1["dens"][0] == pytest.approx(1.0 / w0, rel=2e-6)
+
+
+def boundary_pair(family, r, direction, vi, vj, p_i=800.0, V_b=1.1e-6, bvisc=0.4):
+    h = 0.02
+    p, disp, calc = lattice_params(h, 0.01, 8, family)
+    disp, calc = disp[:2].copy(), calc[:2].copy()
+    e = np.asarray(direction, np.float64) / np.linalg.norm(direction)
+    disp["pos"][0] = [0.04, 0.04, 0.04]
+    disp["type"] = [0, 1]
+    calc["visc"] = 0.0
+    p.num = p.maxNum = 2
+    p.bvisc = bvisc
+    st = orc.State(disp, calc, p, np.float64)
+    st.a["pos"][1] = st["pos"][0] + r * e        # exact in FP64
+    st.opts = options(wcsph_kernel=family)
+    st.grid_build()
+    i = int(np.where(st["type"] == 0)[0][0])
+    j = 1 - i
+    rho_i, rho0 = 1010.0, 1000.0
+    st.a["pressure"][i] = p_i
+    st.a["dens"][i] = 1.0 / rho_i            # 1/rho after the density pass
+    st.a["restdens"][i] = rho0
+    st.a["dens"][j] = V_b                    # boundary volume
+    st.a["vel"][i] = vi
+    st.a["vel"][j] = vj
+    st.force()
+    x_ij = st["pos"][i] - st["pos"][j]
+    gw = grad_w(x_ij, float(p.smoothradius), family)
+    a_p = -rho0 * V_b * p_i / rho_i ** 2 * gw
+    hh = float(p.smoothradius)
+    v_ij = np.asarray(vi, np.float64) - np.asarray(vj, np.float64)
+    vx = v_ij @ x_ij
+    nu_b = float(np.float32(bvisc)) * hh / (2 * rho0)
+    a_v = rho0 * V_b * nu_b * min(vx, 0.0) / (x_ij @ x_ij + 0.01 * hh * hh) * gw
+    return st["accel"][i], a_p, a_v, x_ij
+
+
+@pytest.mark.parametrize("family", [0, 1])
+def test_boundary_pressure_and_friction_closed_form(family):
+    direction = [1.0, 0.5, -0.25]
+    r = 0.55 * 0.02
+    # receding: pressure only, pointing away from the boundary
+    a, a_p, a_v, x = boundary_pair(family, r, direction, vi=[-0.1, -0.05, 0.02], vj=[0, 0, 0])
+    assert np.all(a_v == 0)
+    np.testing.assert_allclose(a, a_p, rtol=1e-6, atol=1e-12 * np.abs(a_p).max())
+    assert a @ (-x) < 0                       # x = x_i - x_j: pushed away from j
+    # approaching: pressure plus friction, and the friction opposes the approach
+    a2, a_p2, a_v2, x2 = boundary_pair(family, r, direction, vi=[0.3, 0.1, -0.05], vj=[0, 0, 0])
+    assert np.abs(a_v2).max() > 1e-3 * np.abs(a_p2).max()
+    np.testing.assert_allclose(a2, a_p2 + a_v2, rtol=1e-6, atol=1e-12 * np.abs(a_p2).max())
+    v = np.array([0.3, 0.1, -0.05])
+    assert (a2 - a_p2) @ v < 0
+    # linear in p_i and in V_b
+    a3, a_p3, _, _ = boundary_pair(family, r, direction, vi=[-0.1, -0.05, 0.02], vj=[0, 0, 0],
+                                   p_i=1600.0, V_b=2.2e-6)
+    np.testing.assert_allclose(a3, 4 * a, rtol=1e-9, atol=1e-14)
+    # outside the support: nothing
+    a4, _, _, _ = boundary_pair(family, 1.01 * (0.02 if family == 0 else 0.04), direction,
+                                vi=[0.3, 0.1, -0.05], vj=[0, 0, 0])
+    assert np.all(a4 == 0)
