@@ -104,14 +104,235 @@ def build_scene(sphx, n_fluid, L=(1.0, 1.0, 1.0), seed=20261203):
     return p, disp, calc, alpha, info
 
 
-def cpu_baseline(sample_fluid=131072, mean_iters=10.0, timed_iters=20):
+TRAIN_SEED = 20261213   # bubble layout of the training tank (the bench tank: 20261203)
+
+
+def make_ctx(sphx, p, disp, calc, alpha, o):
+    c = sphx.Context(p, o)
+    c.upload(disp, calc)
+    c.mf_setup(RHO, NU)
+    c.mf_upload(alpha)
+    c.call("grid_build")
+    return c
+
+
+def train_network(sphx, args):
+    """The learned start's training run: a separate tank of the same
+    configuration with another bubble layout (seed TRAIN_SEED), advanced
+    with the reference solver started from p(t-1) (warm_start = 1) so the
+    recorded converged pressures keep the hydrostatic load; every particle
+    of `record_steps` steps is recorded and the network trained on the GPU."""
+    p, disp, calc, alpha, info = build_scene(sphx, args.n, seed=TRAIN_SEED)
+    o = options(sphx)
+    o.warm_start = 1.0
+    t = make_ctx(sphx, p, disp, calc, alpha, o)
+    del disp, calc
+    t.mf_step()
+    g = float(np.linalg.norm(p.gravity.tolist()))
+    P = 1000.0 * g * 0.8
+    d = t.mf_download()["drift"][:, 1, :]
+    dl, _ = t.download()
+    fl = dl["type"] == 0
+    U = max(float(np.sqrt(np.mean(np.sum(d[fl].astype(np.float64) ** 2, axis=1)))), 1e-9)
+    del d, dl, fl
+    t.mlp_init(20261202, P, U)
+    t.mlp_reserve(args.record_steps * t.num)
+    t.mlp_mode(record=True)
+    its = []
+    for _ in range(args.record_steps):
+        its.append(t.mf_step()[0])
+    t.sync()
+    total, valid = t.mlp_samples()
+    _, T, M = t.mlp_download_samples(0, min(total, 1 << 20))
+    var = np.var(T[M > 0], axis=0) + 1e-12
+    lw = (4.0 / var[0], 1.0 / var[1], 1.0 / var[2], 1.0 / var[3])
+    t0 = time.perf_counter()
+    loss = t.mlp_train(epochs=args.epochs, batch=args.batch, lr=args.lr, lw=lw, seed=5)
+    t.sync()
+    train_s = time.perf_counter() - t0
+    w = t.mlp_weights()
+    t.close()
+    print(f"# training tank (seed {TRAIN_SEED}, p0 = p(t-1)): {args.record_steps} steps, "
+          f"iterations {its}; {valid} samples; {args.epochs} epochs in {train_s:.2f} s, loss "
+          + " ".join(f"{x:.3e}" for x in loss), file=sys.stderr, flush=True)
+    return dict(w=w, P=P, U=U, lw=lw, samples=int(valid), train_s=train_s, epochs=args.epochs)
+
+
+def learned_ctx(sphx, args, p, disp, calc, alpha, net):
+    o = options(sphx)   # warm_start 0.5: the plain start the guard and fallback use
+    c = make_ctx(sphx, p, disp, calc, alpha, o)
+    c.mlp_init(20261202, net["P"], net["U"])
+    c.mlp_set_weights(net["w"])
+    c.mlp_reserve(args.ring_steps * c.num)
+    c.mlp_mode(warm=True, drift=args.drift, record=False)
+    c.mlp_guard(bool(args.guard))
+    if args.online_every > 0:
+        # online refresh trains the pressure output only
+        c.mlp_online(args.online_every, args.online_iters, args.online_batch, args.online_lr,
+                     (net["lw"][0], 0.0, 0.0, 0.0))
+        c.mlp_online_async(bool(args.online_async))
+        c.mlp_record_stride(args.record_stride)
+    return c
+
+
+NAMES = ["hash", "radix_sort", "cell_start", "reorder", "nbr_build", "iisph_density_dii",
+         "mf_drift", "mf_alpha", "mf_correction_force", "iisph_aii", "mlp_features", "mlp_infer",
+         "iisph_dp", "iisph_iter", "iisph_integrate", "mlp_record", "mlp_train", "totals"]
+
+
+def run_method(sphx, args, make, label, only=None, track=False):
+    """A fresh context from t = 0: `warmup` untimed steps, then `steps`
+    timed steps (HIP events around `only`'s launches, or none)."""
+    c = make()
+    for _ in range(args.warmup):
+        c.mf_step()
+    if track:
+        c.conservation_track(True)
+    el, its, errs, kt = timed_segment(c, args.steps, NAMES, only=only)
+    out = dict(elapsed=el, iters=its, errs=errs, kt=kt, ctx=c)
+    print(f"# {label}: {1e3 * el / args.steps:.3f} ms/step, pressure iterations {its} "
+          f"(mean {np.mean(its):.2f}), max error {max(errs):.2e}", file=sys.stderr, flush=True)
+    return out
+
+
+def bench_bubbles(args, sphx):
+    """The headline: the 1M two-phase tank, three warm starts of the same
+    solver, each a fresh context from the same seeded state at t = 0 over
+    the same steps: the reference p0 = 0.5 p(t-1) (plain), p0 = p(t-1)
+    (persistence, reported), and the learned start (trained beforehand on a
+    separate tank).  `value` is the learned run's throughput."""
+    p, disp, calc, alpha, info = build_scene(sphx, args.n)
+    print(f"# scene: {info.n_fluid} fluid + {info.n_boundary} boundary particles, "
+          f"grid {p.gridres.x}x{p.gridres.y}x{p.gridres.z}", file=sys.stderr, flush=True)
+    net = train_network(sphx, args)
+    plain = run_method(sphx, args, lambda: make_ctx(sphx, p, disp, calc, alpha, options(sphx)),
+                       "plain (p0 = 0.5 p(t-1))")
+    plain["ctx"].close()
+
+    def persist_ctx():
+        o = options(sphx)
+        o.warm_start = 1.0
+        return make_ctx(sphx, p, disp, calc, alpha, o)
+    persist = run_method(sphx, args, persist_ctx, "persistence (p0 = p(t-1))")
+    persist["ctx"].close()
+    # the learned run, profiled (every kernel timed) to find the dominant
+    # kernel, then again from t = 0 with events around that kernel only
+    prof = run_method(sphx, args, lambda: learned_ctx(sphx, args, p, disp, calc, alpha, net),
+                      "learned, profiled")
+    dom = kernel_table(prof["kt"], NAMES, f"learned run, every kernel timed, {args.steps} steps "
+                                          f"after {args.warmup} warm-up steps")
+    prof["ctx"].close()
+    run = run_method(sphx, args, lambda: learned_ctx(sphx, args, p, disp, calc, alpha, net),
+                     "learned", only=dom, track=True)
+    c = run["ctx"]
+    stats = c.mlp_stats()
+    guard = c.mlp_guard_stats()
+    hist = c.conservation_history()[:, 0, :]
+    c.close()
+    nf = info.n_fluid
+    elapsed, iters, errs, kt = run["elapsed"], run["iters"], run["errs"], run["kt"]
+    t_dom, n_dom = kt[dom]
+    # Launches of the pressure passes queued past convergence exit on the
+    # device flag at once: only the iterations the solves ran move the
+    # algorithmic bytes, and the early exits' time is charged to them.
+    n_work = int(np.sum(iters)) + args.steps if dom == "iisph_dp" else int(np.sum(iters))
+    n_eff = n_work if dom in ("iisph_iter", "iisph_dp") else n_dom
+    avg_ms = t_dom / max(n_eff, 1)
+    algo = ALGO_BYTES.get(dom, 0) * nf
+    achieved = algo / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    print(f"# timed steps: {dom} {t_dom / max(n_dom, 1) * 1e3:.1f} us/launch x {n_dom} launches"
+          + (f"; {avg_ms * 1e3:.1f} us per working launch x {n_eff}" if n_eff != n_dom else ""),
+          file=sys.stderr)
+    m0 = hist[0, 0]
+    cons = {"mass_rel_change": float(np.abs(hist[:, 0] - m0).max() / m0),
+            "momentum_y_first_last": [float(hist[0, 2]), float(hist[-1, 2])],
+            "kinetic_energy_first_last": [float(hist[0, 4]), float(hist[-1, 4])],
+            "com_y_first_last": [float(hist[0, 6] / m0), float(hist[-1, 6] / hist[-1, 0])]}
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(args.cpu_sample, float(np.mean(plain["iters"])))
+        except Exception as e:   # the baseline is a report, not the measurement
+            print(f"# cpu baseline failed: {e}", file=sys.stderr)
+    ip, ipers, il = (float(np.mean(x["iters"])) for x in (plain, persist, run))
+    line = {
+        "metric": "particle-steps/s at 1M two-phase particles; pressure iters learned vs plain @0.1% err",
+        "value": nf * args.steps / elapsed,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "two-phase water/air bubble tank, drift-flux mixture + IISPH to 0.1% error",
+            "n_fluid": nf,
+            "n_boundary": info.n_boundary,
+            "bubbles": 512,
+            "h": 0.01,
+            "dt": 1e-3,
+            "steps_timed": f"steps {args.warmup}..{args.warmup + args.steps - 1} from t = 0, "
+                           "each method on a fresh context from the same seeded state",
+            "pressure_iters_plain_mean": ip,
+            "pressure_iters_persistence_mean": ipers,
+            "pressure_iters_learned_mean": il,
+            "pressure_iters_ratio": ip / max(il, 1e-9),
+            "pressure_iters_ratio_vs_persistence": ipers / max(il, 1e-9),
+            "max_density_error_plain": float(max(plain["errs"])),
+            "max_density_error_persistence": float(max(persist["errs"])),
+            "max_density_error_learned": float(max(errs)),
+            "plain_ms_per_step": 1e3 * plain["elapsed"] / args.steps,
+            "persistence_ms_per_step": 1e3 * persist["elapsed"] / args.steps,
+            "learned_fallbacks": int(stats["fallbacks"]),
+            "guard_probes": guard["probes"],
+            "guard_kept_plain": guard["plain_picks"],
+            "warm_start": "learned MLP 24-64-64-4 (FP16 weights) with a device-side guard against "
+                          "0.5 p(t-1); fallback 0.5 p(t-1)",
+            "training": f"separate tank (bubble seed {TRAIN_SEED}), {args.record_steps} steps "
+                        f"with p0 = p(t-1), {net['samples']} samples, {net['epochs']} epochs "
+                        f"on the GPU in {net['train_s']:.2f} s (untimed)",
+            "drift_closure": bool(args.drift),
+            "online_refresh": (f"every {args.online_every} steps: {args.online_iters} Adam steps "
+                               f"of {args.online_batch} samples (inside the timed steps"
+                               + (", on a second stream, adopted two steps later)"
+                                  if args.online_async else ")")
+                               + f"; recording every {args.record_stride}th particle of a step"
+                               if args.online_every > 0 else None),
+            "conservation_fp64": cons,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": TRAFFIC_BYTES.get(dom),
+            "traffic_source": "rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated "
+                              "(profiles/iisph_mf_step_1m_hbm_traffic.md)",
+            "algorithmic_bytes_per_launch": algo,
+            "avg_launch_ms": avg_ms,
+            "launches": n_dom,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(sample_fluid=131072, mean_iters=10.0, timed_iters=10, reps=5):
     """The oracle (FP32 restatement of the reference step, OpenMP) on a
     bounded sample of the workload: a tank of the same spacing, h and bubble
-    density holding `sample_fluid` particles.  The fixed stages of one step
-    and `timed_iters` pressure iterations are timed apart, and the rate is
-    reported at the GPU run's mean iteration count:
-    n / (t_fixed + mean_iters * t_iteration)."""
+    density holding `sample_fluid` particles.  Each of `reps` steps times
+    its fixed stages and `timed_iters` pressure iterations apart; a step's
+    time is taken at the GPU run's mean iteration count, t_fixed + mean_iters
+    * t_iteration, and the median over the steps is reported."""
     sys.path.insert(0, str(ROOT / "oracle"))
+    sys.path.insert(0, str(ROOT / "tests"))
     import sphx
     import orc
     side = (sample_fluid / (1 << 20)) ** (1.0 / 3.0)
@@ -121,44 +342,44 @@ def cpu_baseline(sample_fluid=131072, mean_iters=10.0, timed_iters=20):
     st.opts = o
     mf = orc.MF(st, RHO, NU, o)
     rho = np.array([np.float32(r) for r in RHO], np.float64)
-    st.grid_build()
-    src = np.where(st["mgcell"] != 0xFFFFFFFF, st["midsort"], np.arange(st.n))
-    alpha = alpha[src]
     mf.set_alpha(alpha, alpha @ rho)
     mf.a["mforce"][:] = [0.0, -9.81, 0.0]
     fl = st["type"] == 0
     st.a["visc"][fl] = (alpha[fl] @ np.array(NU, np.float32))
-    # boundary volumes come with the scene (lattice volume)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.perf_counter()
-    before = mf.a["alpha"].copy(), mf.a["mforce"].copy()
-    st.grid_build()    # carries the fractions with the permutation
-    src = np.where(st["mgcell"] != 0xFFFFFFFF, st["midsort"], np.arange(st.n))
-    mf.a["alpha"][:] = before[0][src]
-    mf.a["mforce"][:] = before[1][src]
-    st.ii_density()
-    mf.drift()
-    mf.alpha_advance()
-    mf.correction()
-    st.ii_dii()
-    mf.force()
-    st.ii_aii()
-    t1 = time.perf_counter()
-    for _ in range(timed_iters):
-        st.ii_iteration()
-    t2 = time.perf_counter()
-    st.ii_integrate()
-    t3 = time.perf_counter()
-    t_fixed = (t1 - t0) + (t3 - t2)
-    t_iter = (t2 - t1) / timed_iters
-    step = t_fixed + mean_iters * t_iter
+    steps = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        st.grid_build()
+        src = np.where(st["mgcell"] != 0xFFFFFFFF, st["midsort"], np.arange(st.n))
+        for k in ("alpha", "mforce"):
+            mf.a[k][:] = mf.a[k][src]
+        st.ii_density()
+        mf.drift()
+        mf.alpha_advance()
+        mf.correction()
+        st.ii_dii()
+        mf.force()
+        st.ii_aii()
+        t1 = time.perf_counter()
+        for _ in range(timed_iters):
+            st.ii_iteration()
+        t2 = time.perf_counter()
+        st.ii_integrate()
+        mf.a["alpha"][:] = mf.a["alpha_new"]
+        t3 = time.perf_counter()
+        t_fixed = (t1 - t0) + (t3 - t2)
+        t_iter = (t2 - t1) / timed_iters
+        steps.append((t_fixed + mean_iters * t_iter, t_fixed, t_iter))
+    steps.sort()
+    step, t_fixed, t_iter = steps[len(steps) // 2]
     return {"value": info.n_fluid / step, "unit": "particle-steps/s", "cores": threads,
             "kind": "port",
-            "sample": f"one two-phase step of a {info.n_fluid}-particle tank (same spacing, h and "
-                      f"bubble density; {info.n_boundary} boundary particles), FP32 oracle on "
-                      f"{threads} threads: fixed stages {t_fixed:.2f} s + {timed_iters} pressure "
-                      f"iterations at {t_iter:.3f} s each, rate taken at {mean_iters:.1f} "
-                      f"iterations/step (the GPU run's mean)"}
+            "sample": f"median of {reps} two-phase steps of a {info.n_fluid}-particle tank (same "
+                      f"spacing, h and bubble density; {info.n_boundary} boundary particles), "
+                      f"FP32 oracle on {threads} threads: fixed stages {t_fixed:.2f} s + "
+                      f"{timed_iters} pressure iterations at {t_iter:.3f} s each, taken at "
+                      f"{mean_iters:.1f} iterations/step (the plain GPU run's mean)"}
 
 
 MPM_ALGO_BYTES = {
@@ -557,21 +778,22 @@ def kernel_table(kt, names, label):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--profile-steps", type=int, default=40,
                     help="steps with every kernel timed, before the timed steps")
     ap.add_argument("--n", type=int, default=1 << 20, help="fluid particles")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=131072)
-    ap.add_argument("--record-steps", type=int, default=10)
+    ap.add_argument("--record-steps", type=int, default=20,
+                    help="steps of the training tank recorded for the offline fit")
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--drift", type=int, default=0, help="learned drift closure on")
+    ap.add_argument("--guard", type=int, default=1,
+                    help="device-side guard of the learned start against 0.5 p(t-1)")
     ap.add_argument("--ring-steps", type=int, default=12, help="steps of samples kept")
-    ap.add_argument("--autotune", type=int, default=0,
-                    help="steps per candidate workgroup size (0: 256 everywhere)")
     ap.add_argument("--online-every", type=int, default=5)
     ap.add_argument("--online-iters", type=int, default=8)
     ap.add_argument("--online-batch", type=int, default=8192)
@@ -597,195 +819,15 @@ def main():
         sys.exit(1)
 
     if args.workload == "ensemble":
-        if args.steps == 200:
-            args.steps = 60
         bench_ensemble(args, sphx)
         return
     if args.workload == "mobility":
-        if args.steps == 200:
-            args.steps = 50
-        if args.warmup == 20:
-            args.warmup = 5
         bench_mobility(args, sphx)
         return
     if args.workload == "sand":
-        if args.steps == 200:
-            args.steps = 100
         bench_sand(args, sphx)
         return
-
-    p, disp, calc, alpha, info = build_scene(sphx, args.n)
-    o = options(sphx)
-    print(f"# scene: {info.n_fluid} fluid + {info.n_boundary} boundary particles, "
-          f"grid {p.gridres.x}x{p.gridres.y}x{p.gridres.z}", file=sys.stderr, flush=True)
-    # two contexts from the same seeded state: the plain solver (reference
-    # warm start p0 = 0.5 p(t-1)) and the learned warm start; the step is
-    # deterministic, so both see identical states until the learned start
-    # is switched on, and both are timed over the same stretch of the flow
-    ctxs = []
-    for _ in range(2):
-        c = sphx.Context(p, o)
-        c.upload(disp, calc)
-        c.mf_setup(RHO, NU)
-        c.mf_upload(alpha)
-        c.call("grid_build")
-        ctxs.append(c)
-    plain, learn = ctxs
-    for s in range(args.warmup):
-        it, err = plain.mf_step()
-        learn.mf_step()
-        if s % 10 == 0:
-            print(f"# warmup step {s}: {it} iterations, error {err:.2e}", file=sys.stderr,
-                  flush=True)
-    if args.autotune > 0:
-        # workgroup sizes per neighbour-loop kernel from real steps (both
-        # contexts advance by the same steps; sizes change no arithmetic)
-        plain.autotune(args.autotune)
-        tuned = learn.autotune(args.autotune)
-        print(f"# autotuned workgroup sizes: {tuned}", file=sys.stderr, flush=True)
-    names = ["hash", "radix_sort", "cell_start", "reorder", "nbr_build", "iisph_density_dii",
-             "mf_drift", "mf_alpha", "mf_correction_force", "iisph_aii",
-             "mlp_features", "mlp_infer", "iisph_dp", "iisph_iter", "iisph_integrate",
-             "mlp_record", "mlp_train"]
-
-    # learned warm start: record the solver's converged states, train on the
-    # GPU, then keep refreshing the network online from the running solve
-    g = float(np.linalg.norm(p.gravity.tolist()))
-    P = 1000.0 * g * 0.8
-    d = learn.mf_download()["drift"][:, 1, :]
-    dl, _ = learn.download()
-    fl = dl["type"] == 0
-    U = max(float(np.sqrt(np.mean(np.sum(d[fl].astype(np.float64) ** 2, axis=1)))), 1e-9)
-    learn.mlp_init(20261202, P, U)
-    learn.mlp_reserve(args.ring_steps * learn.num)
-    learn.mlp_mode(record=True)
-    for s in range(args.record_steps):
-        learn.mf_step()
-        plain.mf_step()
-    learn.sync()
-    total, valid = learn.mlp_samples()
-    _, T, M = learn.mlp_download_samples(0, min(total, 1 << 20))
-    var = np.var(T[M > 0], axis=0) + 1e-12
-    lw = (4.0 / var[0], 1.0 / var[1], 1.0 / var[2], 1.0 / var[3])
-    t0 = time.perf_counter()
-    loss = learn.mlp_train(epochs=args.epochs, batch=args.batch, lr=args.lr, lw=lw, seed=5)
-    learn.sync()
-    train_s = time.perf_counter() - t0
-    print(f"# recorded {valid} samples in {args.record_steps} steps; trained {args.epochs} "
-          f"epochs in {train_s:.2f} s, loss " + " ".join(f"{x:.3e}" for x in loss),
-          file=sys.stderr, flush=True)
-    learn.mlp_mode(warm=True, drift=args.drift, record=False)
-    # online refresh trains the pressure output only (the drift outputs keep
-    # their offline fit), so the recorder needs no drift targets
-    olw = (lw[0], 0.0, 0.0, 0.0)
-    if args.online_every > 0:
-        learn.mlp_online(args.online_every, args.online_iters, args.online_batch, args.online_lr,
-                         olw)
-        learn.mlp_online_async(bool(args.online_async))
-        learn.mlp_record_stride(args.record_stride)
-
-    el_plain, it_plain, err_plain, kt_plain = timed_segment(plain, args.steps, names,
-                                                            only="iisph_iter")
-    plain.close()
-    print(f"# plain: {1e3 * el_plain / args.steps:.3f} ms/step, mean pressure iterations "
-          f"{np.mean(it_plain):.2f} (max {max(it_plain)}), max error {max(err_plain):.2e}",
-          file=sys.stderr, flush=True)
-    # the timed steps cover the same steps as the plain run's, with HIP
-    # events around the Jacobi pass only (the dominant kernel of this step);
-    # a profile segment with every kernel timed follows them
-    dom = "iisph_iter"
-    elapsed, iters, errs, kt = timed_segment(learn, args.steps, names, only=dom)
-    stats = learn.mlp_stats()
-    _, it_prof, _, kt_prof = timed_segment(learn, args.profile_steps, names)
-    dom_prof = kernel_table(kt_prof, names, f"learned, profile segment of {args.profile_steps} "
-                                            f"steps after the timed steps, "
-                                            f"{np.mean(it_prof):.2f} iterations/step")
-    learn.close()
-    print(f"# learned: {1e3 * elapsed / args.steps:.3f} ms/step, mean pressure iterations "
-          f"{np.mean(iters):.2f} (max {max(iters)}), max error {max(errs):.2e}, "
-          f"fallbacks {stats['fallbacks']} of {stats['learned']}", file=sys.stderr, flush=True)
-    print("# iterations plain   " + " ".join(map(str, it_plain)), file=sys.stderr)
-    print("# iterations learned " + " ".join(map(str, iters)), file=sys.stderr, flush=True)
-    t_dom, n_dom = kt[dom]
-    # The solve queues a few launches past convergence, which exit on the
-    # device flag; only the iterations the solves ran move the algorithmic
-    # bytes, and the early exits' time is charged to them.
-    n_work = int(np.sum(iters))
-    avg_ms = t_dom / max(n_work, 1)
-    nf = info.n_fluid
-    algo = ALGO_BYTES.get(dom, 0) * nf
-    achieved = algo / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    print(f"# timed steps: {dom} {t_dom / max(n_dom, 1) * 1e3:.1f} us/launch x {n_dom} launches; "
-          f"{avg_ms * 1e3:.1f} us per working launch x {n_work} iterations", file=sys.stderr)
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        try:
-            cpu = cpu_baseline(args.cpu_sample, float(np.mean(it_plain)))
-        except Exception as e:   # the baseline is a report, not the measurement
-            print(f"# cpu baseline failed: {e}", file=sys.stderr)
-    value = nf * args.steps / elapsed
-    line = {
-        "metric": "particle-steps/s at 1M two-phase particles; pressure iters learned vs plain @0.1% err",
-        "value": value,
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": "two-phase water/air bubble tank, drift-flux mixture + IISPH to 0.1% error",
-            "n_fluid": nf,
-            "n_boundary": info.n_boundary,
-            "bubbles": 512,
-            "h": 0.01,
-            "dt": 1e-3,
-            "pressure_iters_plain_mean": float(np.mean(it_plain)),
-            "pressure_iters_learned_mean": float(np.mean(iters)),
-            "pressure_iters_ratio": float(np.mean(it_plain) / max(np.mean(iters), 1e-9)),
-            "max_density_error_plain": float(max(err_plain)),
-            "max_density_error_learned": float(max(errs)),
-            "learned_fallbacks": int(stats["fallbacks"]),
-            "plain_ms_per_step": 1e3 * el_plain / args.steps,
-            "warm_start": "learned MLP 24-64-64-4 (FP16 weights), fallback 0.5 p(t-1)",
-            "drift_closure": bool(args.drift),
-            "train_samples": int(valid),
-            "train_epochs": args.epochs,
-            "train_seconds": train_s,
-            "online_refresh": (f"every {args.online_every} steps: {args.online_iters} Adam steps "
-                               f"of {args.online_batch} samples (inside the timed steps"
-                               + (", on a second stream, adopted two steps later)"
-                                  if args.online_async else ")")
-                               + f"; recording every {args.record_stride}th particle of a step"
-                               if args.online_every > 0 else None),
-            "parallelism": "single GPU",
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom,
-            "profile_dominant": dom_prof,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": TRAFFIC_BYTES.get(dom),
-            "traffic_source": "rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated "
-                              "(profiles/iisph_mf_step_1m_hbm_traffic.md)",
-            "algorithmic_bytes_per_launch": algo,
-            "avg_launch_ms": avg_ms,
-            "avg_launch_note": "time of all launches in the timed steps / iterations the "
-                               "solves ran (launches queued past convergence exit early)",
-            "launches": n_dom,
-            "working_launches": n_work,
-        },
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
+    bench_bubbles(args, sphx)
 
 
 if __name__ == "__main__":

@@ -235,6 +235,9 @@ void ComputeSolidTensorCUDA(void);                    /* host.cu:469 */
 void ComputeSolidForceCUDA(void);                     /* host.cu:475 */
 void ComputeSolidTensorX_CUDA(void);                  /* host.cu:481 */
 void ComputeSolidForceX_CUDA(void);                   /* host.cu:487 */
+/* host.cu:492-501: grid hash, sort and reorder, then ComputeInvA, whose
+ * body does nothing past its guards (sph_solid.cu:662-675). */
+void InitializeSolid_CUDA(void);
 
 /* Host-side transfers the reference performs with raw cudaMemcpy on the
  * global buffers (fluid_system.cpp:979-999). */
@@ -404,6 +407,32 @@ int sphx_mpm_download_nodes(sphx_context* ctx, float* v, float* mass);
 int sphx_mpm_upload_nodes(sphx_context* ctx, const float* v, const float* mass);
 /* Number of 4^3-node blocks the last transfer touched. */
 int sphx_mpm_active_blocks(sphx_context* ctx, int* count);
+/* MpmGetMomentum (MpmSolver.cu:107-153): every node's velocity becomes the
+ * trilinear-weighted mean velocity of the particles within one node
+ * spacing on each axis, searched in the 27 SPH cells around the node's
+ * cell (initMpm :39-73); 0 where the weights sum to <= 1e-6 or the node's
+ * cell is outside the valid cells.  Node masses are kept.  Needs a built
+ * grid. */
+int sphx_mpm_get_momentum(sphx_context* ctx);
+/* MpmColorTest (MpmSolver.cu:75-105): particles in the 27 cells around a
+ * node left of x = 0 get colour (1, 1, 1, 1).  Needs a built grid. */
+int sphx_mpm_color_test(sphx_context* ctx);
+
+/* ---------------------------------------------------------------------- */
+/* FP64 conservation totals (no counterpart in the reference; the build's
+ * global mass / momentum reductions).  Per scene (1, or every scene of a
+ * batched ensemble), over the in-grid fluid particles:
+ *   [0] sum m, [1..3] sum m v, [4] sum m |v|^2 / 2, [5..7] sum m x
+ * (centre of mass = [5..7] / [0]).  out holds max(1, nscenes) * 8 doubles.
+ * A single scene's totals are reproducible bit for bit. */
+#define SPHX_TOTALS_Q 8
+int sphx_conservation_totals(sphx_context* ctx, double* out);
+/* on: every integration (eager or captured) appends its totals to a device
+ * ring of the last 4096 steps; the ring restarts at 0. */
+int sphx_conservation_track(sphx_context* ctx, int on);
+/* The last min(recorded, max_steps) steps of the ring, oldest first, as
+ * [steps][scenes][8]; *steps = how many were written. */
+int sphx_conservation_history(sphx_context* ctx, double* out, int max_steps, int* steps);
 
 /* Legacy MPM entry points (fluid_system_host.cuh; host.cu:228-296). */
 void MpmAllocateBufferCUDA(sphx_ParamCarrier* params);
@@ -414,6 +443,11 @@ void MpmNodeUpdate_CUDA(void);
 void MpmParticleUpdate_APIC_CUDA(void);
 void MpmParticleToGrid_CUDA(void);
 void MpmParticleUpdate_CUDA(void);
+void MpmGetMomentumCUDA(void);                        /* host.cu:257 */
+void MpmColorTestCUDA(void);                          /* host.cu:252 */
+/* host.cu:442: every kernel of it is commented out in the reference; it
+ * only synchronises. */
+void ComputeMpmForce(void);
 
 /* SetupMPMGrid (fluid_system.cpp:1004-1025): node grid over the soft
  * bounds +- 10 with cell 1.5 * pSpacing. */
@@ -569,6 +603,14 @@ int sphx_mlp_train(sphx_context* ctx, int epochs, int batch, float lr, const flo
                    uint64_t seed, double* loss_out);
 int sphx_mlp_stats(sphx_context* ctx, int64_t* learned, int64_t* fallbacks,
                    int* last_fallback);
+/* Guard of the learned start (single scene; on by default): after a
+ * learned solve's first iteration misses the tolerance, one probe
+ * iteration from the plain start p0 = warm_start * p(t-1) runs and the
+ * solve continues from the better first iterate; the probe counts as an
+ * iteration.  Decided on the device (graph-safe). */
+int sphx_mlp_guard(sphx_context* ctx, int on);
+/* Probes run and probes that kept the plain start, since sphx_mlp_init. */
+int sphx_mlp_guard_stats(sphx_context* ctx, int64_t* probes, int64_t* plain_picks);
 
 /* ------------------------------------------------------------------------ */
 /* Kernel timing (HIP events on the context stream).                         */

@@ -344,6 +344,55 @@ void g2p(const sphx_ParamCarrier& P, OrcState& s, OrcMPM& M, int apic) {
   }
 }
 
+// ------------------------------------------------------ MpmGetMomentum --
+// :107-153.  The node's SPH cell is initMpm's (:39-73): FP32 node position
+// mpmXmin + id * mpmcellsize, cell (x - gridmin) * gridIdfac truncated,
+// valid for 1 <= g <= res - 1.  The 27 cells are visited in neighborid
+// order (y, then z, then x); cells past the grid's last layer are skipped
+// (the reference reads past the row there, A11).  Invalid node cell: 0.
+template <class T>
+void mpm_momentum(const sphx_ParamCarrier& P, OrcState& s, OrcMPM& M) {
+  const int rx = P.mpmRes.x, ry = P.mpmRes.y, rz = P.mpmRes.z;
+  const int gx = P.gridres.x, gy = P.gridres.y, gz = P.gridres.z;
+  const float fx0 = P.mpmXmin.x, fy0 = P.mpmXmin.y, fz0 = P.mpmXmin.z, fc = P.mpmcellsize;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < rx * ry * rz; ++i) {
+    const int y = i / rx / rz, x = i % rx, z = i / rx % rz;
+    const float px = fx0 + (float)x * fc, py = fy0 + (float)y * fc, pz = fz0 + (float)z * fc;
+    const int cx = (int)((px - P.gridmin.x) * P.gridIdfac.x);
+    const int cy = (int)((py - P.gridmin.y) * P.gridIdfac.y);
+    const int cz = (int)((pz - P.gridmin.z) * P.gridIdfac.z);
+    T* v = &at<T>(M.node_v, 3 * (size_t)i);
+    T sum[3] = {0, 0, 0}, wsum = 0;
+    const T cs = (T)fc;
+    if (cx >= 1 && cx <= gx - 1 && cy >= 1 && cy <= gy - 1 && cz >= 1 && cz <= gz - 1) {
+      for (int a = -1; a <= 1; ++a)
+        for (int b = -1; b <= 1; ++b)
+          for (int c = -1; c <= 1; ++c) {
+            const int yy = cy + a, zz = cz + b, xx = cx + c;
+            if (yy >= gy || zz >= gz || xx >= gx) continue;
+            const int cell = (yy * gz + zz) * gx + xx;
+            if (s.gridcnt[cell] == 0) continue;
+            const int jb = s.gridoff[cell], je = jb + s.gridcnt[cell];
+            for (int j = jb; j < je; ++j) {
+              const T* pj = &at<T>(s.pos, 3 * (size_t)j);
+              const T d[3] = {(T)px - pj[0], (T)py - pj[1], (T)pz - pj[2]};
+              if (std::fabs(d[0]) >= cs || std::fabs(d[1]) >= cs || std::fabs(d[2]) >= cs) continue;
+              const T w = (1 - std::fabs(d[0]) / cs) * (1 - std::fabs(d[1]) / cs) *
+                          (1 - std::fabs(d[2]) / cs);
+              const T* vj = &at<T>(s.vel, 3 * (size_t)j);
+              for (int q = 0; q < 3; ++q) sum[q] += vj[q] * w;
+              wsum += w;
+            }
+          }
+    }
+    if (wsum > T(0.000001f))
+      for (int q = 0; q < 3; ++q) v[q] = sum[q] / wsum;
+    else
+      for (int q = 0; q < 3; ++q) v[q] = 0;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -359,6 +408,9 @@ extern "C" {
   }                                                                                            \
   void pfx##mpm_g2p(const sphx_ParamCarrier* P, OrcState* s, OrcMPM* M, int apic) {            \
     g2p<T>(*P, *s, *M, apic);                                                                   \
+  }                                                                                            \
+  void pfx##mpm_momentum(const sphx_ParamCarrier* P, OrcState* s, OrcMPM* M) {                 \
+    mpm_momentum<T>(*P, *s, *M);                                                                \
   }                                                                                            \
   double pfx##mpm_weight(int kernel, double x) { return (double)N1<T>(kernel, (T)x); }          \
   double pfx##mpm_dweight(int kernel, double x) { return (double)dN1<T>(kernel, (T)x); }       \

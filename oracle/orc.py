@@ -9,13 +9,16 @@ back.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import sys
 from pathlib import Path
 
 import numpy as np
 
 _HERE = Path(__file__).resolve().parent
-LIB_PATH = _HERE / "liboracle.so"
+# SPHX_ORACLE_LIB selects another build of the same sources (the sanitizer
+# build of tests/san)
+LIB_PATH = Path(os.environ.get("SPHX_ORACLE_LIB", str(_HERE / "liboracle.so")))
 sys.path.insert(0, str(_HERE.parent / "sph-multiphase_amd"))
 import sphx  # noqa: E402  (types only)
 
@@ -321,6 +324,7 @@ class MPM:
     def stress(self): self._call("mpm_stress")
     def node_update(self): self._call("mpm_node_update")
     def g2p(self, apic=1): self._call("mpm_g2p", C.c_int(apic))
+    def momentum(self): self._call("mpm_momentum")
 
 
 def mpm_weight(kernel, x, dtype=np.float64):

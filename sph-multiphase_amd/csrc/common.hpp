@@ -284,6 +284,13 @@ struct sphx_context {
     sphx::DevBuf<float4> hist[2];
     long long learned_solves = 0, fallbacks = 0;
     int last_fallback = 0;
+    // guard of the learned start (single scene): after the solve's first
+    // iteration, a probe iteration from the plain start when that iteration
+    // missed the tolerance, and the better first iterate kept
+    int guard = 1;                      // on by default
+    bool guard_armed = false;           // the next solve's first iteration gets the guard
+    sphx::DevBuf<int> gstate;           // [0] skip probe, [1] restore learned, [2] probes, [3] plain picks
+    sphx::DevBuf<float> pkeep;          // the learned start's first iterate
     sphx::DevBuf<float> w, m, v, g;
     sphx::DevBuf<uint16_t> w16;         // FP16 bits of w
     sphx::DevBuf<uint4> wmf;            // w16 in the MFMA operand layout
@@ -316,6 +323,13 @@ struct sphx_context {
   sphx::SceneCtl scenes() {
     return sphx::SceneCtl{nscenes, sc_acc.p, sc_state.p, sc_err.p};
   }
+
+  // FP64 conservation totals (per scene) and, while tracking, the totals
+  // of every integration in a device ring
+  sphx::DevBuf<double> tot_part, tot_out, tot_hist;
+  sphx::DevBuf<int> tot_count;
+  int tot_scenes = 0;             // scenes the buffers were sized for
+  bool tot_track = false;
 
   // captured two-phase step: graph A (sort .. a fixed budget of pressure
   // iterations) and graph B (integration), one of each per buffer parity

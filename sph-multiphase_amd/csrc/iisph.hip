@@ -534,6 +534,93 @@ __global__ __launch_bounds__(256) void k_ii_finalize_scenes(int* __restrict__ st
   }
 }
 
+// Guard of the learned warm start, enqueued after the solve's first
+// iteration (which ran from the learned start).  When that iteration's
+// error exceeds the tolerance (or is NaN), the learned first iterate is
+// kept aside and one probe iteration runs from the plain start p0 = warm *
+// p(t-1) (fluid_system_kern.cu:899); the solve then continues from whichever
+// first iterate has the smaller error, and the probe counts as an iteration.
+// Every decision is made on the device, so a captured step can hold it.
+// gs: [0] skip the probe, [1] restore the learned iterate, [2] probes run,
+// [3] probes that kept the plain start.
+__global__ void k_guard_prep(KP k, const double* __restrict__ err, float tol, int* __restrict__ gs,
+                             const float4* __restrict__ pos, const uint32_t* __restrict__ key,
+                             const float4* __restrict__ phys, const float* __restrict__ pprev,
+                             float* __restrict__ press_l, float4* __restrict__ ps,
+                             float* __restrict__ keep) {
+  const double e = err[0];
+  const bool probe = !(e <= (double)tol);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) gs[0] = probe ? 0 : 1;
+  if (!probe || i >= k.n) return;
+  const float4 pp = pos[i];
+  if (type_of(pp) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) return;
+  const float4 q = phys[i];
+  keep[i] = press_l[i];
+  const float p0 = k.warm * pprev[i];
+  press_l[i] = p0;
+  ps[i].w = q.x * p0 / (q.y * q.y);
+}
+
+__global__ __launch_bounds__(1024) void k_guard_pick(int* __restrict__ gs, int* __restrict__ state,
+                                                     double* __restrict__ err,
+                                                     const double* __restrict__ partial,
+                                                     int nblocks, int num, IterCtl ctl) {
+  if (gs[0]) return;
+  __shared__ double sh[3][16];
+  double a = 0, b = 0, c = 0;
+  for (int q = threadIdx.x; q < nblocks; q += 1024) {
+    a += partial[3 * q];
+    b += partial[3 * q + 1];
+    c += partial[3 * q + 2];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+    c += __shfl_down(c, off, 64);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) { sh[0][w] = a; sh[1][w] = b; sh[2][w] = c; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0, y = 0, z = 0;
+    for (int q = 0; q < 16; ++q) { x += sh[0][q]; y += sh[1][q]; z += sh[2][q]; }
+    const double ep = z > 0 ? y / z : 0.0;
+    const double el = err[0];
+    const int it = state[1] + 1;
+    state[1] = it;
+    gs[2] += 1;
+    if (ep < el || !(el == el)) {   // the plain start's first iterate is better
+      err[0] = ep;
+      err[1] = num > 0 ? x / num : 0.0;
+      gs[1] = 0;
+      gs[3] += 1;
+      if ((it >= ctl.min_iters && ep <= (double)ctl.tol) || it >= ctl.max_iters) state[0] = 1;
+    } else {
+      gs[1] = 1;
+      if (it >= ctl.max_iters) state[0] = 1;
+    }
+  }
+}
+
+__global__ void k_guard_restore(KP k, const int* __restrict__ gs, const float4* __restrict__ pos,
+                                const uint32_t* __restrict__ key, float4* __restrict__ phys,
+                                const float* __restrict__ keep, float* __restrict__ press_l,
+                                float* __restrict__ press_l1, float4* __restrict__ ps) {
+  if (gs[0] || !gs[1]) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.n) return;
+  const float4 pp = pos[i];
+  if (type_of(pp) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) return;
+  const float p = keep[i];
+  float4 q = phys[i];
+  q.z = p;
+  phys[i] = q;
+  press_l[i] = p;
+  press_l1[i] = p;
+  ps[i].w = q.x * p / (q.y * q.y);
+}
+
 // Start of a solve: clear the done flag, iteration count and (ensemble)
 // every scene's state.  A kernel, not a memset, so it can sit in a graph.
 __global__ void k_solve_reset(int* __restrict__ state, SceneCtl sc) {
@@ -768,6 +855,7 @@ int grid_hash(sphx_context* c);
 int grid_sort(sphx_context* c);
 int grid_reorder(sphx_context* c);
 int mlp_solve(sphx_context* c, int* iters, double* err);
+int totals_enqueue(sphx_context* c, bool ring);
 
 static int need_sorted(sphx_context* c, const char* who) {
   SPHX_CHECK(c, who, "null context");
@@ -858,6 +946,52 @@ static int enqueue_iteration(sphx_context* c, const KP& k, int respect_done) {
     k_ii_finalize<<<1, 1024, 0, c->stream>>>(c->solve_state.p, c->solve_err.p, c->red.p, nb,
                                              k.n, ctl, respect_done);
   SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+// The guard's launches (see k_guard_prep): prep, a probe DP + Iter gated on
+// gs[0] instead of the done flag, pick, restore.
+static int enqueue_guard(sphx_context* c, const KP& k) {
+  auto& m = c->mlp;
+  SPHX_TRY(m.pkeep.alloc((size_t)c->cap));
+  const int tdp = c->tune[T_DP], tit = c->tune[T_ITER];
+  const int nb = blocks_for(k.n, tit);
+  NbrList L;
+  SPHX_TRY(nbr_list(c, &L));
+  IterCtl ctl{c->O.min_iters, c->O.max_iters, c->O.tol};
+  SceneCtl none{0, nullptr, nullptr, nullptr};
+  int* gs = m.gstate.p;
+  k_guard_prep<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+      k, c->solve_err.p, c->O.tol, gs, c->POS(), c->key.p, c->PHYS(), m.pprev.p, c->press_l.p,
+      c->ps.p, m.pkeep.p);
+  timer_begin(c, "iisph_dp");
+  k_ii_dp<<<blocks_for(k.n, tdp), tdp, 0, c->stream>>>(k, L, gs, c->key.p, c->cell_start.p,
+                                                       c->POS(), c->ps.p, c->PHYS(), c->dii.p,
+                                                       c->press_l.p, c->dijpj.p, c->mqa.p,
+                                                       c->mqb.p, c->ACCEL(), none);
+  timer_end(c);
+  timer_begin(c, "iisph_iter");
+  k_ii_iter<<<nb, tit, 0, c->stream>>>(k, L, gs, c->key.p, c->cell_start.p, c->POS(), c->PHYS(),
+                                       c->dii.p, c->dijpj.p, c->mqa.p, c->mqb.p, c->gh.p,
+                                       c->rho_adv.p, c->press_l.p, c->press_l1.p, c->ps.p,
+                                       c->residue.p, c->red.p, none);
+  timer_end(c);
+  k_guard_pick<<<1, 1024, 0, c->stream>>>(gs, c->solve_state.p, c->solve_err.p, c->red.p, nb,
+                                          k.n, ctl);
+  k_guard_restore<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+      k, gs, c->POS(), c->key.p, c->PHYS(), m.pkeep.p, c->press_l.p, c->press_l1.p, c->ps.p);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+// One iteration of a solve; the solve's first one is followed by the guard
+// when the learned start armed it.
+static int enqueue_first_or_next(sphx_context* c, const KP& k, bool first) {
+  SPHX_TRY(enqueue_iteration(c, k, 1));
+  if (first && c->mlp.guard_armed) {
+    c->mlp.guard_armed = false;
+    if (c->nscenes == 0) SPHX_TRY(enqueue_guard(c, k));
+  }
   return 0;
 }
 
@@ -1037,7 +1171,7 @@ int iisph_solve_plain(sphx_context* c, int* iters, double* err) {
   if (c->last_iters + 1 > chunk) chunk = c->last_iters + 1;
   for (;;) {
     int todo = std::min(chunk, c->O.max_iters - enq);
-    for (int t = 0; t < todo; ++t) SPHX_TRY(enqueue_iteration(c, k, 1));
+    for (int t = 0; t < todo; ++t) SPHX_TRY(enqueue_first_or_next(c, k, enq + t == 0));
     enq += todo;
     SPHX_TRY(read_state(c));
     if (c->host_pinned[2] != 0.0 || enq >= c->O.max_iters) break;
@@ -1057,7 +1191,7 @@ int enqueue_solve_fixed(sphx_context* c, int n) {
   KP k = make_kp(c);
   if (k.n == 0) return 0;
   SPHX_TRY(solve_reset(c));
-  for (int t = 0; t < n; ++t) SPHX_TRY(enqueue_iteration(c, k, 1));
+  for (int t = 0; t < n; ++t) SPHX_TRY(enqueue_first_or_next(c, k, t == 0));
   return 0;
 }
 
@@ -1095,6 +1229,8 @@ int sphx_iisph_integrate(sphx_context* c) {
       c->nphase > 0 ? c->mf_force[c->cur].p : nullptr);
   timer_end(c);
   SPHX_HIP(hipGetLastError());
+  // FP64 mass / momentum / energy totals of the new state, kept per step
+  if (c->tot_track) SPHX_TRY(totals_enqueue(c, true));
   return 0;
 }
 

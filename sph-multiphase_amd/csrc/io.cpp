@@ -19,10 +19,19 @@ namespace {
 
 thread_local std::string g_io_error;
 
+}  // namespace
+
+namespace sphx {
+// Latch an I/O error message (read with sphx_io_last_error); returns -1.
 int io_fail(const std::string& msg) {
   g_io_error = msg;
   return -1;
 }
+}  // namespace sphx
+
+namespace {
+
+using sphx::io_fail;
 
 // The flat element structure scene.xml uses: a top-level element with a
 // name and attributes, whose children are elements holding text.
@@ -137,6 +146,7 @@ void get_float_n3(const std::string& s, const Elem* e, const char* name, float* 
 }
 
 constexpr char kCheckpointMagic[8] = {'S', 'P', 'H', 'X', 'C', 'K', 'P', '1'};
+
 
 }  // namespace
 
@@ -349,45 +359,6 @@ int sphx_checkpoint_read(const char* path, sphx_ParamCarrier* P, sphx_displayPac
   return 0;
 }
 
-// A context's state to a checkpoint: its parameter block and every
-// particle in slot order (stable cell-sorted after a step).
-int sphx_checkpoint_save(sphx_context* ctx, const char* path) {
-  if (!ctx || !path) return io_fail("sphx_checkpoint_save: null argument");
-  sphx_ParamCarrier P;
-  if (sphx_get_params(ctx, &P) != 0) return io_fail(sphx_last_error());
-  const int n = sphx_num_particles(ctx);
-  if (n < 0) return io_fail(sphx_last_error());
-  std::vector<sphx_displayPack> disp((size_t)n);
-  std::vector<sphx_calculationPack> calc((size_t)n);
-  if (n > 0 && sphx_download(ctx, disp.data(), calc.data(), 0, n) != 0)
-    return io_fail(sphx_last_error());
-  P.num = n;
-  return sphx_checkpoint_write(path, &P, disp.data(), calc.data(), n);
-}
-
-// Resume from a checkpoint into a context created for at least its
-// particle count: parameters, then the particles (which become slots
-// 0..n-1, num = n).
-int sphx_checkpoint_restore(sphx_context* ctx, const char* path) {
-  if (!ctx || !path) return io_fail("sphx_checkpoint_restore: null argument");
-  sphx_ParamCarrier P;
-  int n = 0;
-  if (sphx_checkpoint_read(path, &P, nullptr, nullptr, 0, &n) != 0) return -1;
-  sphx_ParamCarrier cur;
-  if (sphx_get_params(ctx, &cur) != 0) return io_fail(sphx_last_error());
-  if (n > cur.maxNum)
-    return io_fail("sphx_checkpoint_restore: checkpoint holds more particles than the context");
-  std::vector<sphx_displayPack> disp((size_t)n);
-  std::vector<sphx_calculationPack> calc((size_t)n);
-  if (sphx_checkpoint_read(path, &P, disp.data(), calc.data(), n, &n) != 0) return -1;
-  P.maxNum = cur.maxNum;
-  P.num = 0;
-  if (sphx_set_params(ctx, &P) != 0) return io_fail(sphx_last_error());
-  if (n > 0 && sphx_upload(ctx, disp.data(), calc.data(), 0, n, 1) != 0)
-    return io_fail(sphx_last_error());
-  return 0;
-}
-
 // Boundary particles from a shape file (FluidSystem::LoadBoundary,
 // AddShape.cpp:265-296): "WALL min max" and "OPEN_BOX min max thickness"
 // commands ("x,y,z" vectors), each wall filled as SetupAddWall does
@@ -403,12 +374,30 @@ int sphx_boundary_parse(const char* text, const sphx_ParamCarrier* P, sphx_displ
   const float spacing = std::pow(P->bmass / P->bRestdensity, 1 / 3.0f) / P->simscale;
   if (!(spacing > 0.f) || !std::isfinite(spacing))
     return io_fail("sphx_boundary_parse: bmass / bRestdensity / simscale give no spacing");
+  // more particles than any buffer could hold is a malformed file
+  constexpr double kMaxParticles = double(1 << 30);
   int n = start;
+  const char* bad = nullptr;
   auto add_wall = [&](sphx_float3 mn, sphx_float3 mx) {
-    const int cntx = (int)std::ceil((mx.x - mn.x) / spacing);
-    const int cntz = (int)std::ceil((mx.z - mn.z) / spacing);
+    if (bad) return;
+    const float ex = std::ceil((mx.x - mn.x) / spacing), ez = std::ceil((mx.z - mn.z) / spacing);
+    const double ey = std::floor(((double)mx.y - (double)mn.y) / spacing) + 1.0;
+    if (!std::isfinite(ex) || !std::isfinite(ez) || !std::isfinite(ey) || !std::isfinite(mn.y)) {
+      bad = "sphx_boundary_parse: wall bounds are not finite";
+      return;
+    }
+    const int cntx = ex > 0.f ? (int)std::fmin(ex, (float)kMaxParticles) : 0;
+    const int cntz = ez > 0.f ? (int)std::fmin(ez, (float)kMaxParticles) : 0;
+    if ((double)n + (double)cntx * cntz * std::fmax(ey, 0.0) > kMaxParticles) {
+      bad = "sphx_boundary_parse: the walls describe more than 2^30 particles";
+      return;
+    }
     const int cnt = cntx * cntz;
     for (float y = mn.y; y <= mx.y; y += spacing) {
+      if (y + spacing == y) {
+        bad = "sphx_boundary_parse: wall too far from the origin for its spacing";
+        return;
+      }
       for (int xz = 0; xz < cnt; ++xz) {
         const float x = mn.x + (xz % cntx) * spacing;
         const float z = mn.z + (xz / cntx) * spacing;
@@ -462,6 +451,7 @@ int sphx_boundary_parse(const char* text, const sphx_ParamCarrier* P, sphx_displ
       add_wall({mn.x - t, mn.y, mn.z}, {mn.x, mx.y, mx.z});
       add_wall({mx.x, mn.y, mn.z}, {mx.x + t, mx.y, mx.z});
     }
+    if (bad) return io_fail(bad);
   }
   if (added) *added = n - start;
   return 0;

@@ -97,6 +97,7 @@ void ComputeSolidForceCUDA(void) { run("ComputeSolidForceCUDA", sphx_solid_force
 // reference (sph_solid.cu:650-661): nothing to do
 void ComputeSolidTensorX_CUDA(void) {}
 void ComputeSolidForceX_CUDA(void) {}
+void InitializeSolid_CUDA(void) { run("InitializeSolid_CUDA", sphx_grid_build); }
 void ComputeDensityIISPH_CUDA(void) { run("ComputeDensityIISPH_CUDA", sphx_iisph_density); }
 void PredictAdvection(void) { run("PredictAdvection", sphx_iisph_predict_advection); }
 
@@ -125,7 +126,11 @@ void MpmAllocateBufferCUDA(sphx_ParamCarrier* params) {
     std::fprintf(stderr, "SPHX ERROR: MpmAllocateBufferCUDA: FluidSetupCUDA has not been called\n");
     return;
   }
-  int rc = params ? sphx_set_params(g_ctx, params) : 0;
+  if (!params) {
+    std::fprintf(stderr, "SPHX ERROR: MpmAllocateBufferCUDA: null ParamCarrier\n");
+    return;
+  }
+  int rc = sphx_set_params(g_ctx, params);
   if (rc == 0) {
     g_params = *params;
     rc = sphx_mpm_setup(g_ctx, 0, 0);
@@ -152,6 +157,9 @@ void MpmParticleUpdate_APIC_CUDA(void) {
 void MpmParticleUpdate_CUDA(void) {
   run("MpmParticleUpdate_CUDA", [](sphx_context* c) { return sphx_mpm_g2p(c, 0); });
 }
+void MpmGetMomentumCUDA(void) { run("MpmGetMomentumCUDA", sphx_mpm_get_momentum); }
+void MpmColorTestCUDA(void) { run("MpmColorTestCUDA", sphx_mpm_color_test); }
+void ComputeMpmForce(void) { run("ComputeMpmForce", [](sphx_context*) { return 0; }); }
 
 // Mobility (Mobility.cuh:19-29).  The antisymmetric tensor of
 // setSignature (:20-35) is written out inside the kernels.

@@ -784,6 +784,7 @@ int mlp_begin(sphx_context* c) {
   const bool record = m.record || m.online_every > 0;
   SPHX_TRY(m.X.alloc((size_t)c->cap * MLP_NF));
   SPHX_TRY(m.pprev.alloc((size_t)c->cap));
+  SPHX_TRY(m.pkeep.alloc((size_t)c->cap));
   float4* hist = m.hist[c->cur].p;
   if ((m.warm || record) && !m.feat_ok) {
     NbrList L;
@@ -808,6 +809,8 @@ int mlp_begin(sphx_context* c) {
     timer_end(c);
     SPHX_HIP(hipGetLastError());
   }
+  // the solve's first iteration is followed by the guard (single scene)
+  c->mlp.guard_armed = m.warm && m.guard && c->nscenes == 0;
   return 0;
 }
 
@@ -928,6 +931,24 @@ extern "C" {
 
 int sphx_mlp_nparam(void) { return MLP_NP; }
 
+int sphx_mlp_guard(sphx_context* c, int on) {
+  SPHX_CHECK(c, "sphx_mlp_guard", "null context");
+  c->mlp.guard = on != 0;
+  return 0;
+}
+
+int sphx_mlp_guard_stats(sphx_context* c, int64_t* probes, int64_t* plain_picks) {
+  SPHX_CHECK(c && probes && plain_picks, "sphx_mlp_guard_stats", "null argument");
+  *probes = *plain_picks = 0;
+  if (!c->mlp.gstate.p) return 0;
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  int g[4];
+  SPHX_HIP(hipMemcpy(g, c->mlp.gstate.p, sizeof(g), hipMemcpyDeviceToHost));
+  *probes = g[2];
+  *plain_picks = g[3];
+  return 0;
+}
+
 int sphx_mlp_init(sphx_context* c, uint64_t seed, float p_scale, float u_scale) {
   SPHX_CHECK(c, "sphx_mlp_init", "null context");
   SPHX_CHECK(p_scale > 0.f && u_scale > 0.f, "sphx_mlp_init", "scales must be positive");
@@ -940,6 +961,8 @@ int sphx_mlp_init(sphx_context* c, uint64_t seed, float p_scale, float u_scale) 
   SPHX_TRY(m.g.alloc(MLP_NP));
   SPHX_TRY(m.w16.alloc(MLP_NP + 1));
   SPHX_TRY(m.loss.alloc(4));
+  SPHX_TRY(m.gstate.alloc(4));
+  SPHX_HIP(hipMemset(m.gstate.p, 0, 4 * sizeof(int)));
   for (int b = 0; b < 2; ++b) {
     SPHX_TRY(m.hist[b].alloc((size_t)std::max(c->cap, 1)));
     SPHX_HIP(hipMemset(m.hist[b].p, 0, (size_t)std::max(c->cap, 1) * sizeof(float4)));
@@ -1259,3 +1282,73 @@ int sphx_mlp_stats(sphx_context* c, int64_t* learned, int64_t* fallbacks, int* l
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ checkpoint --
+namespace sphx {
+
+// The network's state: settings, the FP32 weights with their Adam moments
+// and step, the solve counter, and the per-particle history (drift
+// correction, last pressure increment) in slot order.  The online sample
+// ring is not part of it.
+struct MlpHead {
+  int32_t warm, drift, record, adam_t;
+  int32_t online_every, online_iters, online_batch, record_stride;
+  int32_t online_async, guard, np, n;
+  int64_t solves, seed;
+  float p_scale, u_scale, drift_blend, online_lr, online_lw[4];
+};
+
+int mlp_state_get(sphx_context* c, std::vector<char>& out) {
+  out.clear();
+  if (!c || !c->mlp.ready) return 0;
+  auto& m = c->mlp;
+  SPHX_TRY(mlp_quiesce(c));
+  MlpHead h{};
+  h.warm = m.warm; h.drift = m.drift; h.record = m.record; h.adam_t = m.adam_t;
+  h.online_every = m.online_every; h.online_iters = m.online_iters;
+  h.online_batch = m.online_batch; h.record_stride = m.record_stride;
+  h.online_async = m.online_async; h.guard = m.guard; h.np = MLP_NP; h.n = c->num;
+  h.solves = m.solves; h.seed = (int64_t)m.online_seed;
+  h.p_scale = m.p_scale; h.u_scale = m.u_scale; h.drift_blend = m.drift_blend;
+  h.online_lr = m.online_lr;
+  for (int o = 0; o < 4; ++o) h.online_lw[o] = m.online_lw[o];
+  const size_t wb = (size_t)MLP_NP * sizeof(float), hb = (size_t)c->num * sizeof(float4);
+  out.resize(sizeof(h) + 3 * wb + hb);
+  char* p = out.data();
+  std::memcpy(p, &h, sizeof(h));
+  SPHX_HIP(hipMemcpy(p + sizeof(h), m.w.p, wb, hipMemcpyDeviceToHost));
+  SPHX_HIP(hipMemcpy(p + sizeof(h) + wb, m.m.p, wb, hipMemcpyDeviceToHost));
+  SPHX_HIP(hipMemcpy(p + sizeof(h) + 2 * wb, m.v.p, wb, hipMemcpyDeviceToHost));
+  if (c->num > 0)
+    SPHX_HIP(hipMemcpy(p + sizeof(h) + 3 * wb, m.hist[c->cur].p, hb, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int mlp_state_set(sphx_context* c, const char* p, size_t bytes) {
+  SPHX_CHECK(c && p && bytes >= sizeof(MlpHead), "sphx_checkpoint_restore",
+             "truncated network section");
+  MlpHead h;
+  std::memcpy(&h, p, sizeof(h));
+  const size_t wb = (size_t)MLP_NP * sizeof(float), hb = (size_t)c->num * sizeof(float4);
+  SPHX_CHECK(h.np == MLP_NP && h.n == c->num && bytes == sizeof(h) + 3 * wb + hb,
+             "sphx_checkpoint_restore", "network section does not match this build or the particles");
+  SPHX_TRY(sphx_mlp_init(c, 1, h.p_scale, h.u_scale));
+  auto& m = c->mlp;
+  SPHX_HIP(hipMemcpy(m.w.p, p + sizeof(h), wb, hipMemcpyHostToDevice));
+  SPHX_HIP(hipMemcpy(m.m.p, p + sizeof(h) + wb, wb, hipMemcpyHostToDevice));
+  SPHX_HIP(hipMemcpy(m.v.p, p + sizeof(h) + 2 * wb, wb, hipMemcpyHostToDevice));
+  if (c->num > 0)
+    SPHX_HIP(hipMemcpy(m.hist[c->cur].p, p + sizeof(h) + 3 * wb, hb, hipMemcpyHostToDevice));
+  m.warm = h.warm; m.drift = h.drift; m.record = h.record; m.adam_t = h.adam_t;
+  m.online_every = h.online_every; m.online_iters = h.online_iters;
+  m.online_batch = h.online_batch; m.record_stride = h.record_stride;
+  m.online_async = h.online_async; m.guard = h.guard;
+  m.solves = (int)h.solves; m.online_seed = (uint64_t)h.seed;
+  m.drift_blend = h.drift_blend; m.online_lr = h.online_lr;
+  for (int o = 0; o < 4; ++o) m.online_lw[o] = h.online_lw[o];
+  SPHX_TRY(mlp_refresh16(c));
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // namespace sphx

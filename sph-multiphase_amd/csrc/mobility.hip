@@ -405,6 +405,11 @@ __global__ void k_walk_commit(sphx_displayPack* __restrict__ disp, int n,
   disp[i].pos = {p.x, p.y, p.z};
 }
 
+__global__ void k_walk_zero(sphx_float3* __restrict__ u, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) u[i] = {0.f, 0.f, 0.f};
+}
+
 // MVmultiply (:85-98): one wave per row of the dense (6n x 6n) matrix.
 __global__ void k_gemv(const float* __restrict__ mat, const float* __restrict__ f,
                        float* __restrict__ u, int dim) {
@@ -592,6 +597,36 @@ int sphx_rpy_sync(sphx_rpy* h) {
 
 hipStream_t sphx_rpy_stream(sphx_rpy* h) { return h ? h->stream : nullptr; }
 
+}  // extern "C"
+
+namespace {
+
+// The walkthrough's launches on the current device's null stream, with
+// `newpos` (n float4) as the position double buffer; synchronous.
+int walk_run(sphx_context* nodes, const sphx_stokesianBufList* b, int n, int with_force,
+             float4* newpos) {
+  int bl = blocks_for(n, 256);
+  if (with_force) k_walk_force<<<bl, 256>>>(b->dispBuffer, n, b->cuForce);
+  if (nodes && nodes->mpm.on) {
+    const sphx_ParamCarrier& P = nodes->P;
+    k_walk_background<<<bl, 256>>>(b->dispBuffer, n, nodes->mpm.nodev.p,
+                                   make_float3(P.mpmXmin.x, P.mpmXmin.y, P.mpmXmin.z),
+                                   P.mpmcellsize, make_int3(P.mpmRes.x, P.mpmRes.y, P.mpmRes.z),
+                                   b->simscale, b->cuUnew);
+  } else {
+    k_walk_zero<<<bl, 256>>>(b->cuUnew, n);
+  }
+  k_walk_mob<<<bl, 256>>>(b->dispBuffer, n, b->cuForce, b->dt, b->cuUnew, b->cuOmega, newpos);
+  k_walk_commit<<<bl, 256>>>(b->dispBuffer, n, newpos);
+  SPHX_HIP(hipGetLastError());
+  SPHX_HIP(hipDeviceSynchronize());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
 // The reference's walkthrough step on device buffers: near-field force
 // (added to cuForce), background velocity from the node grid of `nodes`
 // (may be NULL: zero background), mobility velocity and rotation,
@@ -602,22 +637,7 @@ int sphx_rpy_walkthrough(sphx_context* nodes, const sphx_stokesianBufList* b, in
   if (n == 0) return 0;
   DevBuf<float4> np;
   SPHX_TRY(np.alloc(n));
-  int bl = blocks_for(n, 256);
-  if (with_force) k_walk_force<<<bl, 256>>>(b->dispBuffer, n, b->cuForce);
-  if (nodes && nodes->mpm.on) {
-    const sphx_ParamCarrier& P = nodes->P;
-    k_walk_background<<<bl, 256>>>(b->dispBuffer, n, nodes->mpm.nodev.p,
-                                   make_float3(P.mpmXmin.x, P.mpmXmin.y, P.mpmXmin.z),
-                                   P.mpmcellsize, make_int3(P.mpmRes.x, P.mpmRes.y, P.mpmRes.z),
-                                   b->simscale, b->cuUnew);
-  } else {
-    SPHX_HIP(hipMemset(b->cuUnew, 0, n * sizeof(sphx_float3)));
-  }
-  k_walk_mob<<<bl, 256>>>(b->dispBuffer, n, b->cuForce, b->dt, b->cuUnew, b->cuOmega, np.p);
-  k_walk_commit<<<bl, 256>>>(b->dispBuffer, n, np.p);
-  SPHX_HIP(hipGetLastError());
-  SPHX_HIP(hipDeviceSynchronize());
-  return 0;
+  return walk_run(nodes, b, n, with_force, np.p);
 }
 
 // MVmultiply / getMobU: u = mat f with mat (6n x 6n) row-major, device.

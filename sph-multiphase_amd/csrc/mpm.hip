@@ -1334,3 +1334,119 @@ int sphx_mpm_active_blocks(sphx_context* c, int* count) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------ node velocity sampling --
+namespace sphx {
+
+// SPH cell of a node as initMpm computes it (MpmSolver.cu:49-61): node
+// position x_I = mpmXmin + (x, y, z) mpmcellsize, cell g = (x_I - gridmin)
+// gridIdfac truncated, valid when 1 <= g <= res - 1 on every axis (FP32
+// operations in the reference's order, no contraction).
+__device__ __forceinline__ bool node_cell(const KP& k, const MP& m, int i, float3& xn, int3& g) {
+  const int x = i % m.res.x, z = (i / m.res.x) % m.res.z, y = i / (m.res.x * m.res.z);
+  xn = make_float3(__fadd_rn(m.xmin.x, __fmul_rn((float)x, m.dx)),
+                   __fadd_rn(m.xmin.y, __fmul_rn((float)y, m.dx)),
+                   __fadd_rn(m.xmin.z, __fmul_rn((float)z, m.dx)));
+  g = make_int3((int)__fmul_rn(__fsub_rn(xn.x, k.gridmin.x), k.gridfac.x),
+                (int)__fmul_rn(__fsub_rn(xn.y, k.gridmin.y), k.gridfac.y),
+                (int)__fmul_rn(__fsub_rn(xn.z, k.gridmin.z), k.gridfac.z));
+  return g.x >= 1 && g.x <= k.gridres.x - 1 && g.y >= 1 && g.y <= k.gridres.y - 1 &&
+         g.z >= 1 && g.z <= k.gridres.z - 1;
+}
+
+// Visit the members of the 27 cells around cell g in the reference's order
+// (rows dy, dz; cells dx ascending; members ascending).  Cells past the
+// grid's last layer (g = res - 1 is a valid node cell, A11) are skipped.
+template <class F>
+__device__ __forceinline__ void node_cells(const KP& k, const int* __restrict__ cs, int3 g, F&& f) {
+  const int x0 = g.x - 1, x1 = min(g.x + 1, k.gridres.x - 1);
+#pragma unroll 1
+  for (int dy = -1; dy <= 1; ++dy) {
+    const int y = g.y + dy;
+    if (y >= k.gridres.y) continue;
+#pragma unroll 1
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int z = g.z + dz;
+      if (z >= k.gridres.z) continue;
+      const int row = (y * k.gridres.z + z) * k.gridres.x;
+      const int jb = cs[row + x0], je = cs[row + x1 + 1];
+#pragma unroll 1
+      for (int j = jb; j < je; ++j) f(j);
+    }
+  }
+}
+
+// MpmGetMomentum (MpmSolver.cu:107-153): node velocity = trilinear-weighted
+// mean of the velocities of the particles closer than one node spacing on
+// every axis; 0 when the weights sum to at most EPSILON = 1e-6 (A6) or the
+// node lies outside the valid cells.  The node mass (.w) is kept.
+__global__ void k_mpm_momentum(KP k, MP m, const int* __restrict__ cs,
+                               const float4* __restrict__ pos, const float4* __restrict__ vel,
+                               float4* __restrict__ nodev, int nodes) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nodes) return;
+  float3 xn;
+  int3 g;
+  float sx = 0.f, sy = 0.f, sz = 0.f, sw = 0.f;
+  if (node_cell(k, m, i, xn, g)) {
+    const float c = m.dx;
+    node_cells(k, cs, g, [&](int j) {
+      const float4 pj = pos[j];
+      const float dx = fabsf(xn.x - pj.x), dy = fabsf(xn.y - pj.y), dz = fabsf(xn.z - pj.z);
+      if (dx >= c || dy >= c || dz >= c) return;
+      const float w = __fmul_rn(__fmul_rn(1.f - dx / c, 1.f - dy / c), 1.f - dz / c);
+      const float4 v = vel[j];
+      sx = __fadd_rn(sx, __fmul_rn(v.x, w));
+      sy = __fadd_rn(sy, __fmul_rn(v.y, w));
+      sz = __fadd_rn(sz, __fmul_rn(v.z, w));
+      sw = __fadd_rn(sw, w);
+    });
+  }
+  float4 o = nodev[i];
+  if (sw > 1e-6f) {
+    o.x = sx / sw; o.y = sy / sw; o.z = sz / sw;
+  } else {
+    o.x = o.y = o.z = 0.f;
+  }
+  nodev[i] = o;
+}
+
+// MpmColorTest (MpmSolver.cu:75-105): particles in the 27 cells around a
+// node left of x = 0 are coloured white (a debugging aid of the reference).
+__global__ void k_mpm_color_test(KP k, MP m, const int* __restrict__ cs,
+                                 const int* __restrict__ perm, float4* __restrict__ color,
+                                 int nodes) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nodes) return;
+  float3 xn;
+  int3 g;
+  const bool ok = node_cell(k, m, i, xn, g);
+  if (!(xn.x < 0.f) || !ok) return;
+  node_cells(k, cs, g, [&](int j) { color[perm[j]] = make_float4(1.f, 1.f, 1.f, 1.f); });
+}
+
+}  // namespace sphx
+
+extern "C" {
+
+int sphx_mpm_get_momentum(sphx_context* c) {
+  SPHX_TRY(mpm_need(c, "sphx_mpm_get_momentum"));
+  SPHX_CHECK(c->sorted_valid, "sphx_mpm_get_momentum", "grid not built (call sphx_grid_build first)");
+  const int nodes = (int)c->mpm.nodes;
+  k_mpm_momentum<<<blocks_for(nodes, 256), 256, 0, c->stream>>>(
+      make_kp(c), make_mp(c), c->cell_start.p, c->POS(), c->VEL(), c->mpm.nodev.p, nodes);
+  SPHX_HIP(hipGetLastError());
+  return timer_flush(c);
+}
+
+int sphx_mpm_color_test(sphx_context* c) {
+  SPHX_TRY(mpm_need(c, "sphx_mpm_color_test"));
+  SPHX_CHECK(c->sorted_valid, "sphx_mpm_color_test", "grid not built (call sphx_grid_build first)");
+  const int nodes = (int)c->mpm.nodes;
+  k_mpm_color_test<<<blocks_for(nodes, 256), 256, 0, c->stream>>>(
+      make_kp(c), make_mp(c), c->cell_start.p, c->PERM(), c->cold_color.p, nodes);
+  SPHX_HIP(hipGetLastError());
+  return timer_flush(c);
+}
+
+}  // extern "C"

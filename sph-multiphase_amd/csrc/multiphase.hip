@@ -731,3 +731,62 @@ int sphx_mf_step(sphx_context* c, int* iters, double* err) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ checkpoint --
+namespace sphx {
+
+// Multiphase state of a context in slot order (the order sphx_download
+// uses): nphase, rho[3], nu[3], then fractions {alpha_0..2, rho_m} and g - a
+// of the last step, one float4 each per particle.
+int mf_state_get(sphx_context* c, std::vector<char>& out) {
+  out.clear();
+  if (!c || c->nphase == 0) return 0;
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  const int n = c->num;
+  std::vector<float4> a((size_t)n), f((size_t)n);
+  if (n > 0) {
+    const float4* src = c->mf_pending ? c->mf_alpha_new.p : c->mf_alpha[c->cur].p;
+    SPHX_HIP(hipMemcpy(a.data(), src, n * sizeof(float4), hipMemcpyDeviceToHost));
+    SPHX_HIP(hipMemcpy(f.data(), c->mf_force[c->cur].p, n * sizeof(float4), hipMemcpyDeviceToHost));
+  }
+  const size_t head = sizeof(int32_t) * 2 + sizeof(float) * 6;
+  out.resize(head + 2 * (size_t)n * sizeof(float4));
+  char* p = out.data();
+  const int32_t hn[2] = {c->nphase, n};
+  std::memcpy(p, hn, sizeof(hn));
+  std::memcpy(p + 8, c->mf_rho, 12);
+  std::memcpy(p + 20, c->mf_nu, 12);
+  std::memcpy(p + head, a.data(), n * sizeof(float4));
+  std::memcpy(p + head + n * sizeof(float4), f.data(), n * sizeof(float4));
+  return 0;
+}
+
+// Restore what mf_state_get wrote, after the particles were uploaded (the
+// particles' mixture viscosity came with them and is left as it is).
+int mf_state_set(sphx_context* c, const char* p, size_t bytes) {
+  SPHX_CHECK(c && p, "sphx_checkpoint_restore", "null multiphase section");
+  const size_t head = sizeof(int32_t) * 2 + sizeof(float) * 6;
+  SPHX_CHECK(bytes >= head, "sphx_checkpoint_restore", "truncated multiphase section");
+  int32_t hn[2];
+  std::memcpy(hn, p, sizeof(hn));
+  const int np = hn[0], n = hn[1];
+  SPHX_CHECK(np >= 1 && np <= 3 && n == c->num &&
+                 bytes == head + 2 * (size_t)n * sizeof(float4),
+             "sphx_checkpoint_restore", "multiphase section does not match the particles");
+  float rho[3], nu[3];
+  std::memcpy(rho, p + 8, 12);
+  std::memcpy(nu, p + 20, 12);
+  SPHX_TRY(sphx_mf_setup(c, np, rho, nu));
+  if (n > 0) {
+    SPHX_HIP(hipStreamSynchronize(c->stream));
+    SPHX_HIP(hipMemcpy(c->mf_alpha[c->cur].p, p + head, n * sizeof(float4), hipMemcpyHostToDevice));
+    SPHX_HIP(hipMemcpy(c->mf_force[c->cur].p, p + head + n * sizeof(float4), n * sizeof(float4),
+                       hipMemcpyHostToDevice));
+  }
+  c->mf_pending = false;
+  c->mf_stress_ok = false;
+  c->mf_rec_ok = false;
+  return 0;
+}
+
+}  // namespace sphx

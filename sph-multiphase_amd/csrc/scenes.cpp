@@ -326,14 +326,21 @@ void sphx_setup_mpm_grid(sphx_ParamCarrier* p, float pSpacing) {
   sphx_float3 lo = {p->softminx.x - 10.f, p->softminx.y - 10.f, p->softminx.z - 10.f};
   sphx_float3 hi = {p->softmaxx.x + 10.f, p->softmaxx.y + 10.f, p->softmaxx.z + 10.f};
   float cs = (float)(pSpacing * 1.5);
-  float tx = (hi.x - lo.x) / cs + 2.f, ty = (hi.y - lo.y) / cs + 2.f, tz = (hi.z - lo.z) / cs + 2.f;
-  sphx_int3 res = {(int)tx, (int)ty, (int)tz};
+  const bool ok = cs > 0.f && std::isfinite(cs);   // else an empty grid
+  float tx = ok ? (hi.x - lo.x) / cs + 2.f : 0.f, ty = ok ? (hi.y - lo.y) / cs + 2.f : 0.f,
+        tz = ok ? (hi.z - lo.z) / cs + 2.f : 0.f;
+  // a count that is not a finite non-negative int is stored as 0
+  auto to_int = [](float t) { return t >= 0.f && t < 2147483648.f ? (int)t : 0; };
+  sphx_int3 res = {to_int(tx), to_int(ty), to_int(tz)};
   p->mpmXmin = lo;
   p->mpmXmax = {lo.x + (float)(res.x - 1) * cs, lo.y + (float)(res.y - 1) * cs,
                 lo.z + (float)(res.z - 1) * cs};
   p->mpmcellsize = cs;
   p->mpmRes = res;
-  p->mpmNodeNum = res.x * res.y * res.z;
+  // -1: more nodes than the ParamCarrier's int count holds (sphx_mpm_setup
+  // refuses such a grid)
+  const long long nodes = (long long)res.x * res.y * res.z;
+  p->mpmNodeNum = nodes > 2147483647LL ? -1 : (int)nodes;
 }
 
 // SetDruckerPragerParam (fluid_system.cpp:882-892): friction and dilation

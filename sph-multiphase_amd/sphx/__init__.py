@@ -615,6 +615,14 @@ class Context:
               "sphx_mlp_train")
         return out[:epochs]
 
+    def mlp_guard(self, on=True):
+        check(self.L.sphx_mlp_guard(self.h, 1 if on else 0), "sphx_mlp_guard")
+
+    def mlp_guard_stats(self):
+        a, b = C.c_int64(), C.c_int64()
+        check(self.L.sphx_mlp_guard_stats(self.h, C.byref(a), C.byref(b)), "sphx_mlp_guard_stats")
+        return {"probes": a.value, "plain_picks": b.value}
+
     def mlp_stats(self):
         a, b, c = C.c_int64(), C.c_int64(), C.c_int()
         check(self.L.sphx_mlp_stats(self.h, C.byref(a), C.byref(b), C.byref(c)), "sphx_mlp_stats")
@@ -681,6 +689,25 @@ class Context:
         v = np.ascontiguousarray(v, np.float32)
         m = np.ascontiguousarray(m, np.float32)
         check(self.L.sphx_mpm_upload_nodes(self.h, ptr(v), ptr(m)), "sphx_mpm_upload_nodes")
+
+    # -- FP64 conservation totals --
+    def conservation_totals(self):
+        """[scenes][8]: sum m, sum m v (3), sum m |v|^2 / 2, sum m x (3)."""
+        S = max(1, getattr(self, "nscenes", 0))
+        out = np.zeros((S, 8), np.float64)
+        check(self.L.sphx_conservation_totals(self.h, ptr(out)), "sphx_conservation_totals")
+        return out
+
+    def conservation_track(self, on=True):
+        check(self.L.sphx_conservation_track(self.h, 1 if on else 0), "sphx_conservation_track")
+
+    def conservation_history(self, max_steps=4096):
+        S = max(1, getattr(self, "nscenes", 0))
+        out = np.zeros((max_steps, S, 8), np.float64)
+        n = C.c_int()
+        check(self.L.sphx_conservation_history(self.h, ptr(out), max_steps, C.byref(n)),
+              "sphx_conservation_history")
+        return out[:n.value]
 
     def mpm_active_blocks(self):
         n = C.c_int()

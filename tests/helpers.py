@@ -72,3 +72,72 @@ def assert_close(name, gpu_val, f32_val, f64_val, tol, mask=None, factor=4.0):
     bound = max(tol, factor * e_f32)
     assert e_gpu <= bound, f"{name}: gpu error {e_gpu:.3e} > bound {bound:.3e}"
     return e_gpu
+
+
+# -- the two-phase step on the oracle (sphx_mf_step's stage order) ----------
+UNDEF = 0xFFFFFFFF
+
+
+def oracle_mf_init(disp, calc, alpha, params, opts, rho, nu, dtype=np.float32):
+    """Oracle state and fractions as sphx_mf_upload leaves them: fractions
+    with rho_m = sum alpha_k rho_k and nu = sum alpha_k nu_k in FP32, and
+    g - a = g for the first drift pass."""
+    st = orc.State(disp, calc, params, dtype)
+    st.opts = opts
+    mf = orc.MF(st, rho, nu, opts)
+    a = np.asarray(alpha, np.float32)
+    r32 = np.asarray(rho, np.float32)
+    n32 = np.asarray(nu, np.float32)
+    rm = np.zeros(len(a), np.float32)
+    nm = np.zeros(len(a), np.float32)
+    for k in range(a.shape[1]):
+        rm = rm + a[:, k] * r32[k]
+        nm = nm + a[:, k] * n32[k]
+    mf.set_alpha(a, rm)
+    mf.a["mforce"][:] = params.gravity.tolist()
+    fl = (st["type"] & 0xFF) == 0
+    st.a["visc"][fl] = nm[fl]
+    return st, mf
+
+
+def oracle_mf_step(st, mf, params, opts):
+    """One two-phase step: grid build (fractions and g - a carried with the
+    permutation), density, drift, fraction advection, correction, d_ii,
+    mixture force, a_ii, pressure solve, integration; then the clamp into
+    the valid cells (SolverOptions.clamp_to_grid) and g - a of the step, as
+    the library's integration does.  Returns (iterations, error)."""
+    st.grid_build()
+    src = np.where(st["mgcell"] != UNDEF, st["midsort"], np.arange(st.n)).astype(np.int64)
+    for k in ("alpha", "mforce"):
+        mf.a[k][:] = mf.a[k][src]
+    st.ii_density()
+    mf.drift()
+    mf.alpha_advance()
+    mf.correction()
+    st.ii_dii()
+    mf.force()
+    st.ii_aii()
+    it, err = st.ii_solve()
+    v_old = st["vel"].copy()
+    st.ii_integrate()
+    live = ((st["type"] & 0xFF) != 1) & (st["mgcell"] != UNDEF)
+    f = np.float32 if st.dtype == np.float32 else np.float64
+    if opts.clamp_to_grid:
+        gm = np.array(params.gridmin.tolist(), f)
+        gf = np.array(params.gridIdfac.tolist(), f)
+        gr = np.array([params.gridres.x, params.gridres.y, params.gridres.z], f)
+        lo = gm + f(1.0001) / gf
+        hi = gm + (gr - f(1.0001)) / gf
+        x, v = st.a["pos"], st.a["vel"]
+        for a in range(3):
+            low = live & (x[:, a] < lo[a])
+            x[low, a] = lo[a]
+            v[low, a] = np.maximum(v[low, a], 0)
+            high = live & (x[:, a] > hi[a])
+            x[high, a] = hi[a]
+            v[high, a] = np.minimum(v[high, a], 0)
+    g = np.array(params.gravity.tolist(), f)
+    inv = f(1) / f(params.dt)
+    mf.a["mforce"][live] = g - (st["vel"][live] - v_old[live]) * inv
+    mf.a["alpha"][:] = mf.a["alpha_new"]
+    return it, err

@@ -182,3 +182,80 @@ def test_mpm_substeps_falling_block(gpu):
         assert y1 < y0
         assert abs((y0 - y1) - 0.5 * 9.81 * t * t) < 0.2 * 0.5 * 9.81 * t * t
         assert d["pos"].min() > 0 and d["pos"].max() < 1.0
+
+
+def offset_sand(n=3000, seed=21):
+    """A sand block near x = 0 in a grid widened by two cells on the low
+    side, with the node lattice shifted off the SPH cells (and partly left
+    of x = 0), so every one of a node's 27 cells can hold particles within a
+    node spacing and nodes left of x = 0 have valid cells."""
+    p, disp, calc = sand(1, 1, n=n, lo=0.02, hi=0.3, seed=seed)
+    cell = p.cellsize / p.simscale
+    p.volmin.x -= 2 * cell
+    sphx.lib().sphx_setup_grid(sphx.C.byref(p))
+    cs = p.mpmcellsize
+    p.mpmXmin.x = -2.6 * cs
+    p.mpmXmin.y += 0.7 * cs
+    p.mpmXmin.z += 0.15 * cs
+    disp["color"] = 0.0
+    return p, disp, calc
+
+
+def test_node_momentum_matches_oracle(gpu):
+    """MpmGetMomentum (MpmSolver.cu:107-153) vs the FP64 oracle from the
+    same sorted state; node masses are left as they were."""
+    p, disp, calc = offset_sand()
+    with sphx.Context(p) as ctx:
+        ctx.upload(disp, calc)
+        ctx.mpm_setup(1, 1)
+        ctx.call("grid_build")
+        v0, m0 = ctx.mpm_nodes()
+        ctx.call("mpm_get_momentum")
+        v, m = ctx.mpm_nodes()
+        st64, mp64 = oracle_from(ctx, p, 1, 1, nodes=False)
+        st32, mp32 = oracle_from(ctx, p, 1, 1, nodes=False, dtype=np.float32)
+    mp64.momentum()
+    mp32.momentum()
+    np.testing.assert_array_equal(m, m0)
+    live = np.abs(mp64["node_v"]).sum(1) > 0
+    assert live.sum() > 1000
+    bound = max(TOL, 4 * orc.relerr(mp32["node_v"], mp64["node_v"]))
+    check("node velocity", v, mp64["node_v"], bound=bound)
+    assert np.all(v[~live] == 0)
+
+
+def test_color_test_marks_cells_around_nodes_left_of_zero(gpu):
+    """MpmColorTest (MpmSolver.cu:75-105): exactly the particles in the 27
+    cells around the valid cell of a node with x < 0 turn white."""
+    p, disp, calc = offset_sand(seed=22)
+    with sphx.Context(p) as ctx:
+        ctx.upload(disp, calc)
+        ctx.mpm_setup(1, 1)
+        ctx.call("grid_build")
+        ctx.call("mpm_color_test")
+        d, c = ctx.download()
+        g = ctx.download_grid()
+    res = np.array([p.gridres.x, p.gridres.y, p.gridres.z])
+    rx, ry, rz = p.mpmRes.x, p.mpmRes.y, p.mpmRes.z
+    ids = np.arange(rx * ry * rz)
+    xyz = np.stack([ids % rx, ids // (rx * rz), (ids // rx) % rz], 1).astype(np.float32)
+    x0 = np.array([p.mpmXmin.x, p.mpmXmin.y, p.mpmXmin.z], np.float32)
+    xn = x0 + xyz * np.float32(p.mpmcellsize)
+    lo = np.array([p.gridmin.x, p.gridmin.y, p.gridmin.z], np.float32)
+    fac = np.array([p.gridIdfac.x, p.gridIdfac.y, p.gridIdfac.z], np.float32)
+    gc = ((xn - lo) * fac).astype(np.int64)
+    ok = np.all((gc >= 1) & (gc <= res - 1), axis=1) & (xn[:, 0] < 0)
+    assert ok.sum() > 10
+    cells = set()
+    for q in gc[ok]:
+        for a in (-1, 0, 1):
+            for b in (-1, 0, 1):
+                for e in (-1, 0, 1):
+                    y, z, x = q[1] + a, q[2] + b, q[0] + e
+                    if x < res[0] and y < res[1] and z < res[2]:
+                        cells.add((y * res[2] + z) * res[0] + x)
+    key = g["mgcell"].astype(np.int64)
+    want = np.isin(key, np.array(sorted(cells), np.int64)) & (g["mgcell"] != UNDEF)
+    assert 0 < want.sum() < len(want)
+    white = np.all(d["color"] == 1.0, axis=1)
+    np.testing.assert_array_equal(white, want)

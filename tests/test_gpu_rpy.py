@@ -92,3 +92,47 @@ def test_walkthrough_and_dense_product(gpu):
     L.getMobU(d_mat.ptr, d_f.ptr, d_out.ptr, m)
     ref = mat.astype(np.float64) @ fv.astype(np.float64)
     assert orc.relerr(d_out.get(), ref) <= 1e-6
+
+
+def test_walkthrough_background_from_node_grid(gpu):
+    """Kern_interpolateVelocity (Mobility.cu:322-366): trilinear node
+    velocities at the spheres, divided by simscale.  A linear node field
+    v_I = A x_I + b is reproduced exactly inside the grid, so the
+    background is A x + b in closed form; the walkthrough then follows the
+    FP64 oracle from that background."""
+    import ctypes as C
+    p, disp, calc, info = sphx.scene_sand(n=512, seed=32)
+    rng = np.random.default_rng(9)
+    A = rng.normal(0, 1.0, (3, 3))
+    b = rng.normal(0, 1.0, 3)
+    ss = 0.5
+    with sphx.Context(p) as nodes:
+        nodes.upload(disp, calc)
+        nodes.mpm_setup(1, 1)
+        res = (p.mpmRes.x, p.mpmRes.y, p.mpmRes.z)
+        y, z, x = np.meshgrid(*[np.arange(r) for r in (res[1], res[2], res[0])], indexing="ij")
+        xs = np.stack([x.ravel(), y.ravel(), z.ravel()], 1) * np.float64(p.mpmcellsize)
+        nodes.mpm_set_nodes((xs @ A.T + b).astype(np.float32), np.ones(len(xs), np.float32))
+        n = 200
+        hi = (res[0] - 1) * p.mpmcellsize
+        pos = rng.uniform(0.05 * hi, 0.95 * hi, (n, 3)).astype(np.float32)
+        f0 = rng.normal(0, 1, (n, 3)).astype(np.float32)
+        sd = np.zeros(n, sphx.display_dtype)
+        sd["pos"] = pos
+        d_disp = sphx.DeviceArray(sd)
+        d_force = sphx.DeviceArray(f0)
+        d_u = sphx.DeviceArray(np.zeros((n, 3), np.float32))
+        d_w = sphx.DeviceArray(np.zeros((n, 3), np.float32))
+        bl = sphx.StokesianBufList(d_u.ptr.value, d_disp.ptr.value, d_w.ptr.value,
+                                   d_force.ptr.value, 1e-5, ss)
+        L = sphx.lib()
+        sphx.check(L.sphx_rpy_walkthrough(nodes.h, C.byref(bl), n, 0), "walkthrough")
+        u_gpu, w_gpu, p_gpu = d_u.get(), d_w.get(), d_disp.get()["pos"]
+    ubg = (pos.astype(np.float64) @ A.T + b) / ss
+    u, w, pn = orc.rpy_walkthrough(pos, f0, ubg, 1e-5)
+    eu, ew = orc.relerr(u_gpu, u), orc.relerr(w_gpu, w)
+    print(f"walkthrough with node background: U {eu:.2e} Omega {ew:.2e}")
+    assert eu <= 1e-5 and ew <= 1e-5
+    assert orc.relerr(p_gpu, pn) <= 1e-6
+    # the background is a visible part of U
+    assert np.abs(ubg).max() > 0.05 * np.abs(u).max()

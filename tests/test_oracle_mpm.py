@@ -255,3 +255,60 @@ def test_node_update_gravity_and_soft_boundaries():
     v[::7] = v0[::7]
     assert floor.sum() > 100 and (x[:, 1] - hi[1] > 1e-6).sum() > 100
     np.testing.assert_allclose(mp["node_v"], v, rtol=1e-12, atol=1e-12)
+
+
+# -- MpmGetMomentum (MpmSolver.cu:107-153) -----------------------------------
+def node_cells_f32(p):
+    """initMpm's node cells (FP32, truncation) and validity 1 <= g <= res-1."""
+    res = (p.mpmRes.x, p.mpmRes.y, p.mpmRes.z)
+    y, z, x = np.meshgrid(np.arange(res[1]), np.arange(res[2]), np.arange(res[0]), indexing="ij")
+    ids = np.stack([x.ravel(), y.ravel(), z.ravel()], 1).astype(np.float32)
+    x0 = np.array([p.mpmXmin.x, p.mpmXmin.y, p.mpmXmin.z], np.float32)
+    pos = x0 + ids * np.float32(p.mpmcellsize)
+    lo = np.array([p.gridmin.x, p.gridmin.y, p.gridmin.z], np.float32)
+    fac = np.array([p.gridIdfac.x, p.gridIdfac.y, p.gridIdfac.z], np.float32)
+    g = ((pos - lo) * fac).astype(np.int64)
+    gr = np.array([p.gridres.x, p.gridres.y, p.gridres.z])
+    ok = np.all((g >= 1) & (g <= gr - 1), axis=1)
+    return pos, ok
+
+
+def test_node_momentum_brute_force_and_uniform_field():
+    """Every node against an O(N x nodes) trilinear-weighted mean over all
+    particles (no cell search), and a uniform particle velocity reproduced
+    exactly at every node that has particles within one spacing."""
+    p, disp, calc, info = sphx.scene_sand(n=1500, seed=13)
+    # nodes off the SPH cell lattice, so that every one of the 27 cells can
+    # hold particles within a node spacing
+    p.mpmXmin.x += 0.4 * p.mpmcellsize
+    p.mpmXmin.y += 0.7 * p.mpmcellsize
+    p.mpmXmin.z += 0.15 * p.mpmcellsize
+    st = orc.State(disp, calc, p, np.float64)
+    st.grid_build()
+    rng = np.random.default_rng(6)
+    st["vel"][:] = rng.normal(0, 0.5, (st.n, 3))
+    mp = orc.MPM(st, p, 1, 1, calc)
+    mp.momentum()
+    xn, ok = node_cells_f32(p)
+    cs = float(np.float32(p.mpmcellsize))
+    x = st["pos"]
+    v = st["vel"]
+    want = np.zeros((len(xn), 3))
+    hits = 0
+    for i in np.where(ok)[0]:
+        d = np.abs(xn[i].astype(np.float64) - x)
+        m = np.all(d < cs, axis=1)
+        if not m.any():
+            continue
+        w = np.prod(1.0 - d[m] / cs, axis=1)
+        if w.sum() > 1e-6:
+            want[i] = (w[:, None] * v[m]).sum(0) / w.sum()
+            hits += 1
+    assert hits > 500
+    np.testing.assert_allclose(mp["node_v"], want, rtol=0, atol=1e-12)
+    st["vel"][:] = [0.25, -1.5, 0.75]
+    mp.momentum()
+    nz = np.abs(want).sum(1) > 0
+    np.testing.assert_allclose(mp["node_v"][nz], np.broadcast_to([0.25, -1.5, 0.75], (nz.sum(), 3)),
+                               rtol=1e-13)
+    assert np.all(mp["node_v"][~nz] == 0)
