@@ -165,7 +165,7 @@ def learned_ctx(sphx, args, p, disp, calc, alpha, net):
     c.mlp_set_weights(net["w"])
     c.mlp_reserve(args.ring_steps * c.num)
     c.mlp_mode(warm=True, drift=args.drift, record=False)
-    c.mlp_guard(bool(args.guard))
+    c.mlp_guard(args.guard)
     if args.online_every > 0:
         # online refresh trains the pressure output only
         c.mlp_online(args.online_every, args.online_iters, args.online_batch, args.online_lr,
@@ -382,6 +382,131 @@ def cpu_baseline(sample_fluid=131072, mean_iters=10.0, timed_iters=10, reps=5):
                       f"{mean_iters:.1f} iterations/step (the plain GPU run's mean)"}
 
 
+DAMBREAK_ALGO_BYTES = {
+    # per particle: pos 16, vel 16, phys 16 -> rho, p (8)
+    "wcsph_density": 56,
+    # pos, vel, phys -> accel 16
+    "wcsph_force": 64,
+    # pos, vel, accel -> pos, vel
+    "wcsph_advance": 80,
+}
+
+
+def dambreak_scene(sphx):
+    """32^3 fluid lattice, spacing 0.01 m, jitter +-1e-4 m, in the corner
+    of a 0.64 x 0.32 x 0.32 m box of boundary particles; cubic spline
+    h = 0.02 m, Tait EOS (gamma 7, c0 20 m/s), artificial viscosity 0.01,
+    dt = 1e-4 s."""
+    p, disp, calc, info = sphx.scene_dambreak(32, 32, 32, 0.01, (0.64, 0.32, 0.32), 0.02, 2, 1e-4,
+                                              20261202, 1)
+    p.dt = 1e-4
+    o = sphx.default_options()
+    o.wcsph_kernel = 1
+    o.eos_gamma = 7.0
+    o.eos_B = 1000.0 * 20.0 ** 2 / 7.0
+    o.art_visc_alpha = 0.01
+    o.sound_speed = 20.0
+    return p, disp, calc, info, o
+
+
+def aggregates(pos, vel, fluid):
+    x = pos[fluid].astype(np.float64)
+    v = vel[fluid].astype(np.float64)
+    return x.mean(0), 0.5 * (v * v).sum(1).mean()
+
+
+def bench_dambreak(args, sphx):
+    """The reference's CPU-runnable configuration: the single-phase WCSPH
+    dam break, 100 steps from t = 0, on the GPU (sphx_wcsph_step) and on the
+    reference step restated on the CPU (the oracle, FP32, OpenMP), each
+    timed over the whole 100 steps; the two trajectories' centre of mass
+    and kinetic energy are compared at the end."""
+    steps = args.steps if args.steps != 20 else 100
+    p, disp, calc, info, o = dambreak_scene(sphx)
+    nf = info.n_fluid
+    print(f"# dam break: {nf} fluid + {info.n_boundary} boundary particles, {steps} steps",
+          file=sys.stderr, flush=True)
+
+    def fresh():
+        c = sphx.Context(p, o)
+        c.upload(disp, calc)
+        c.call("grid_build")
+        c.call("boundary_volume")
+        return c
+    names = ["hash", "radix_sort", "cell_start", "reorder", "wcsph_density",
+             "wcsph_force", "wcsph_advance"]
+    step = None
+    # profile pass (every kernel timed), then the timed pass from t = 0 with
+    # events around the dominant kernel only
+    c = fresh()
+    _, _, _, kt_prof = timed_segment(c, steps, names, step=lambda: c.call("wcsph_step"))
+    c.close()
+    dom = kernel_table(kt_prof, names, f"dam break, every kernel timed, {steps} steps")
+    c = fresh()
+    elapsed, _, _, kt = timed_segment(c, steps, names, only=dom,
+                                      step=lambda: c.call("wcsph_step"))
+    d, cc = c.download()
+    c.close()
+    t_dom, n_dom = kt[dom]
+    avg_ms = t_dom / max(n_dom, 1)
+    algo = DAMBREAK_ALGO_BYTES.get(dom, 0) * nf
+    achieved = algo / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    fl = d["type"] == 0
+    com_g, ke_g = aggregates(d["pos"], cc["vel"], fl)
+    cpu, cmp = None, None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import orc
+        st = orc.State(disp, calc, p, np.float32)
+        st.opts = o
+        st.grid_build()
+        st.boundary_volume()
+        per = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            st.grid_build()
+            st.density_pressure()
+            st.force()
+            st.advance()
+            per.append(time.perf_counter() - t0)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        com_c, ke_c = aggregates(st["pos"], st["vel"], st["type"] == 0)
+        cmp = {"com_gpu": com_g.tolist(), "com_cpu": com_c.tolist(),
+               "com_max_diff_m": float(np.abs(com_g - com_c).max()),
+               "ke_gpu": float(ke_g), "ke_cpu": float(ke_c),
+               "ke_rel_diff": float(abs(ke_g - ke_c) / max(ke_c, 1e-30))}
+        print(f"# CPU reference step: {sum(per):.2f} s for {steps} steps (median step "
+              f"{np.median(per) * 1e3:.1f} ms); GPU vs CPU after {steps} steps: {cmp}",
+              file=sys.stderr, flush=True)
+        cpu = {"value": nf * steps / sum(per), "unit": "particle-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": f"the whole workload: {steps} steps of the {nf}-particle dam break, "
+                         f"FP32 oracle (the reference step restated) on {threads} threads, "
+                         f"{sum(per):.2f} s (median step {np.median(per) * 1e3:.1f} ms)"}
+    line = {
+        "metric": "particle-steps/s, single-phase WCSPH dam break (32^3 particles, 100 steps)",
+        "value": nf * steps / elapsed,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": 1e3 * elapsed / steps,
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "WCSPH dam break, cubic spline h = 0.02, Tait gamma 7 c0 20, "
+                               "artificial viscosity 0.01, dt 1e-4, steps from t = 0",
+                   "n_fluid": nf, "n_boundary": info.n_boundary, "gpu_vs_cpu": cmp},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": algo, "avg_launch_ms": avg_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 MPM_ALGO_BYTES = {
     "mpm_p2g": 88,       # per particle: x, v, m (phys), B, key
     "mpm_gather": 16,    # per active node: {v, m} w
@@ -464,9 +589,10 @@ def bench_sand(args, sphx):
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline_sand(sphx, sample=131072, substeps=2):
-    """The oracle's FP32 MPM stages (single-threaded) on a 131072-particle
-    cube of the same spacing and material over a 128^3 grid."""
+def cpu_baseline_sand(sphx, sample=131072, reps=5):
+    """The oracle's FP32 MPM stages (OpenMP over particles, FP64 atomic node
+    sums) on a 131072-particle cube of the same spacing and material over a
+    128^3 grid: the median of `reps` substeps."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import orc
     side = 0.5 * (sample / (1 << 21)) ** (1.0 / 3.0)
@@ -476,8 +602,9 @@ def cpu_baseline_sand(sphx, sample=131072, substeps=2):
     st = orc.State(disp, calc, p, np.float32)
     mp = orc.MPM(st, p, 1, 1, calc)
     st.grid_build()
-    t0 = time.perf_counter()
-    for _ in range(substeps):
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
         st.grid_build()
         ok = st["mgcell"] != 0xFFFFFFFF
         src = np.where(ok, st["midsort"], np.arange(st.n))
@@ -487,10 +614,12 @@ def cpu_baseline_sand(sphx, sample=131072, substeps=2):
         mp.stress()
         mp.node_update()
         mp.g2p(1)
-    t = (time.perf_counter() - t0) / substeps
-    return {"value": sample / t, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"{substeps} substeps of a {sample}-particle cube (same dx, material, "
-                      f"dt) over a 128^3 grid, FP32 oracle MPM stages single-threaded "
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": sample / t, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"median of {reps} substeps of a {sample}-particle cube (same dx, material, "
+                      f"dt) over a 128^3 grid, FP32 oracle MPM stages on {threads} threads "
                       f"({t:.2f} s per substep)"}
 
 
@@ -619,120 +748,154 @@ def bench_mobility(args, sphx):
     print(json.dumps(line), flush=True)
 
 
+def ensemble_scene(sphx, S, seed):
+    rng = np.random.default_rng(20261207)
+    sizes = np.exp(rng.uniform(np.log(2e4), np.log(4e5), S)).astype(np.int64)
+    p, disp, calc, alpha, first, info = sphx.scene_ensemble(sizes.tolist(), seed=seed)
+    p.dt = 1e-3
+    return sizes, p, disp, calc, alpha, info
+
+
+def ensemble_ctx(sphx, S, p, disp, calc, alpha, warm_start=0.5):
+    o = options(sphx)
+    o.warm_start = warm_start
+    c = sphx.Context(p, o)
+    c.upload(disp, calc)
+    c.mf_setup(RHO, NU)
+    c.mf_upload(alpha)
+    c.ensemble_setup(S)
+    c.call("grid_build")
+    c.persistent_solve(True)
+    return c
+
+
 def bench_ensemble(args, sphx):
     """Batched ensemble: 64 two-phase tanks with fluid counts log-uniform in
     [20k, 400k] (seeded), bubble layouts seeded per scene, all in one grid so
-    every stage is one launch; the step runs through HIP graphs (host check
-    of the done flag between the solve graph and the integration graph).
-    Two ensembles from the same state: the plain solver and the learned warm
-    start (trained on >= 10M recorded samples, refreshed online every 10
-    steps by an eager recording step).  Workgroup sizes from the autotuner."""
+    every stage is one launch.  A step is graph A (sort .. the learned start),
+    the persistent solve (one cooperative launch; with the learned start, the
+    device-side fallback and a second launch) and graph B (integration),
+    with nothing read back between steps.  Three warm starts from the same
+    state: plain 0.5 p(t-1), persistence p(t-1) (reported), and the learned
+    start, trained on a separate ensemble (other bubble seeds) advanced with
+    p0 = p(t-1)."""
     S = 64
-    rng = np.random.default_rng(20261207)
-    sizes = np.exp(rng.uniform(np.log(2e4), np.log(4e5), S)).astype(np.int64)
-    p, disp, calc, alpha, first, info = sphx.scene_ensemble(sizes.tolist(), seed=20261208)
-    p.dt = 1e-3
-    o = options(sphx)
-    nf = int(info.n_fluid)
-    print(f"# ensemble: {S} scenes, {nf} fluid + {info.n_boundary} boundary particles, grid "
-          f"{p.gridres.x}^3", file=sys.stderr, flush=True)
-    ctxs = []
-    for _ in range(2):
-        c = sphx.Context(p, o)
-        c.upload(disp, calc)
-        c.mf_setup(RHO, NU)
-        c.mf_upload(alpha)
-        c.ensemble_setup(S)
-        c.call("grid_build")
-        ctxs.append(c)
-    del disp, calc
-    plain, learn = ctxs
-    for s in range(args.warmup):
-        it, err = plain.mf_step()
-        learn.mf_step()
-        print(f"# warmup step {s}: {it} iterations, error {err:.2e}", file=sys.stderr, flush=True)
-    # learned start: record >= 10M samples, train on the GPU
+    # training ensemble: same sizes, other bubble layouts
+    sizes, p, disp, calc, alpha, info = ensemble_scene(sphx, S, 20261218)
+    t = ensemble_ctx(sphx, S, p, disp, calc, alpha, warm_start=1.0)
+    del disp, calc, alpha
+    t.persistent_solve(False)
+    t.mf_step()
     g = float(np.linalg.norm(p.gravity.tolist()))
-    d = learn.mf_download()["drift"][:, 1, :]
-    dl, _ = learn.download()
-    fl = dl["type"] == 0
-    U = max(float(np.sqrt(np.mean(np.sum(d[fl].astype(np.float64) ** 2, axis=1)))), 1e-9)
-    del d, dl, fl
-    learn.mlp_init(20261202, 1000.0 * g * 0.8 * float(np.cbrt(sizes.mean() / (1 << 20))), U)
-    rec = max(int(np.ceil(1e7 / nf)), args.record_steps)
-    learn.mlp_reserve((rec + 1) * learn.num)
-    learn.mlp_mode(record=True)
+    t.mlp_init(20261202, 1000.0 * g * 0.8, 0.05)
+    rec = max(2, int(np.ceil(1e7 / info.n_fluid)))
+    t.mlp_reserve(rec * t.num)
+    t.mlp_mode(record=True)
     for _ in range(rec):
-        learn.mf_step()
-        plain.mf_step()
-    total, valid = learn.mlp_samples()
-    _, T, M = learn.mlp_download_samples(0, min(total, 1 << 21))
+        t.mf_step()
+    total, valid = t.mlp_samples()
+    _, T, M = t.mlp_download_samples(0, min(total, 1 << 21))
     var = np.var(T[M > 0], axis=0) + 1e-12
     lw = (4.0 / var[0], 1.0 / var[1], 1.0 / var[2], 1.0 / var[3])
     t0 = time.perf_counter()
-    loss = learn.mlp_train(epochs=args.epochs, batch=args.batch, lr=args.lr, lw=lw, seed=5)
-    learn.sync()
+    loss = t.mlp_train(epochs=args.epochs, batch=args.batch, lr=args.lr, lw=lw, seed=5)
+    t.sync()
     train_s = time.perf_counter() - t0
-    learn.mlp_mode(warm=True, record=False)
-    learn.mlp_online(1, args.online_iters, args.online_batch, args.online_lr, (lw[0], 0, 0, 0))
-    print(f"# recorded {valid} samples in {rec} steps, trained {args.epochs} epochs in "
+    w = t.mlp_weights()
+    t.close()
+    print(f"# training ensemble: {valid} samples in {rec} steps, {args.epochs} epochs in "
           f"{train_s:.1f} s, loss {loss[-1]:.3e}", file=sys.stderr, flush=True)
-    # autotune (real steps on both ensembles, kept in step)
-    tp = plain.autotune(1)
-    tl = learn.autotune(1)
-    print(f"# tuned plain {tp}\n# tuned learned {tl}", file=sys.stderr, flush=True)
-    ip, _ = plain.mf_step()
-    il, _ = learn.mf_step()
-    plain.graph_build(max(2, ip))
-    learn.graph_build(max(2, il))
-    res = {}
-    for name, c, every in (("plain", plain, 0), ("learned", learn, args.online_every)):
+    # the benchmark ensemble
+    sizes, p, disp, calc, alpha, info = ensemble_scene(sphx, S, 20261208)
+    nf = int(info.n_fluid)
+    print(f"# ensemble: {S} scenes, {nf} fluid + {info.n_boundary} boundary particles, grid "
+          f"{p.gridres.x}x{p.gridres.y}x{p.gridres.z}", file=sys.stderr, flush=True)
+
+    def run(label, ws, learned, only=None):
+        c = ensemble_ctx(sphx, S, p, disp, calc, alpha, warm_start=ws)
+        if learned:
+            c.mlp_init(20261202, 1000.0 * g * 0.8, 0.05)
+            c.mlp_set_weights(w)
+            c.mlp_mode(warm=True)
+        for _ in range(args.warmup):
+            c.mf_step()
+        c.persistent_solve(True)     # restarts the solve log
+        c.graph_build(2)
         c.sync()
-        its, errs = [], []
+        c.timing_only(only)
+        c.timing(only is not None)
+        c.timing_reset()
         t0 = time.perf_counter()
-        for s in range(args.steps):
-            if every and s % every == every - 1:
-                it, err = c.mf_step()        # eager: records and refreshes online
-            else:
-                it, err = c.graph_step()
-            its.append(it)
-            errs.append(err)
+        for _ in range(args.steps):
+            c.graph_step_async()
         c.sync()
         el = time.perf_counter() - t0
-        res[name] = (el, its, errs)
-        print(f"# {name} iterations " + " ".join(map(str, its)), file=sys.stderr)
-        print(f"# {name}: {1e3 * el / args.steps:.2f} ms/step, max-over-scenes iterations mean "
-              f"{np.mean(its):.2f} (max {max(its)}), max error {max(errs):.2e}", file=sys.stderr,
-              flush=True)
-    stats = learn.mlp_stats()
-    plain.close()
-    learn.close()
-    el, its, errs = res["learned"]
+        kt = c.timing_get(only) if only else (0.0, 0)
+        c.timing(False)
+        its, errs = c.solve_history(args.steps)
+        stats = c.mlp_stats() if learned else None
+        c.close()
+        print(f"# {label}: {1e3 * el / args.steps:.2f} ms/step, most iterations of any scene "
+              f"{its.tolist()} (mean {its.mean():.2f}), worst error {errs.max():.2e}",
+              file=sys.stderr, flush=True)
+        return dict(el=el, its=its, errs=errs, kt=kt, stats=stats)
+
+    plain = run("plain (p0 = 0.5 p(t-1))", 0.5, False)
+    pers = run("persistence (p0 = p(t-1))", 1.0, False)
+    dom = "iisph_solve"
+    learn = run("learned", 0.5, True, only=dom)
+    t_dom, n_dom = learn["kt"]
+    # the persistent solve moves the DP and Jacobi passes' bytes per
+    # iteration: (68 + 76) B per fluid particle (a scene that converged
+    # early stops moving them; the count is the most iterations of any scene)
+    it_total = float(np.sum(learn["its"]))
+    algo_per_launch = (ALGO_BYTES["iisph_dp"] + ALGO_BYTES["iisph_iter"]) * nf * it_total / max(n_dom, 1)
+    avg_ms = t_dom / max(n_dom, 1)
+    achieved = algo_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(65536, float(np.mean(plain["its"])))
+        except Exception as e:
+            print(f"# cpu baseline failed: {e}", file=sys.stderr)
+    ip, ipers, il = (float(np.mean(x["its"])) for x in (plain, pers, learn))
     line = {
         "metric": "aggregate particle-steps/s, 64-scene two-phase ensemble (log-uniform 20k-400k)",
-        "value": nf * args.steps / el,
+        "value": nf * args.steps / learn["el"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * el / args.steps,
+        "ms_per_step": 1e3 * learn["el"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "64 bubble tanks in one grid, one launch per stage, HIP-graph step",
+        "config": {"workload": "64 bubble tanks in one grid, one launch per stage; step = graph A, "
+                               "persistent cooperative solve (+ device fallback), graph B; no "
+                               "host read-back between steps",
                    "scenes": S, "n_fluid": nf, "n_boundary": int(info.n_boundary),
-                   "pressure_iters_plain_mean": float(np.mean(res["plain"][1])),
-                   "pressure_iters_learned_mean": float(np.mean(its)),
+                   "pressure_iters_plain_mean": ip,
+                   "pressure_iters_persistence_mean": ipers,
+                   "pressure_iters_learned_mean": il,
+                   "pressure_iters_ratio": ip / max(il, 1e-9),
                    "iteration_count": "most iterations of any scene, per step",
-                   "plain_ms_per_step": 1e3 * res["plain"][0] / args.steps,
-                   "max_density_error_learned": float(max(errs)),
-                   "max_density_error_plain": float(max(res["plain"][2])),
-                   "learned_fallbacks": int(stats["fallbacks"]), "train_samples": int(valid),
-                   "train_seconds": train_s, "tuning": tl},
-        "roofline": None,
-        "cpu_baseline": None,
+                   "plain_ms_per_step": 1e3 * plain["el"] / args.steps,
+                   "persistence_ms_per_step": 1e3 * pers["el"] / args.steps,
+                   "max_density_error_learned": float(learn["errs"].max()),
+                   "max_density_error_plain": float(plain["errs"].max()),
+                   "train_samples": int(valid), "train_seconds": train_s,
+                   "online_refresh": None},
+        "roofline": {"bound": "hbm", "kernel": "k_ii_solve_coop (persistent pressure solve)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": algo_per_launch, "avg_launch_ms": avg_ms,
+                     "launches": n_dom,
+                     "note": "HIP events around the persistent solve launches of the timed "
+                             "steps (outside the graphs); bytes = (68 + 76) B x fluid particles "
+                             "x iterations"},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
@@ -791,8 +954,9 @@ def main():
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--drift", type=int, default=0, help="learned drift closure on")
-    ap.add_argument("--guard", type=int, default=1,
-                    help="device-side guard of the learned start against 0.5 p(t-1)")
+    ap.add_argument("--guard", type=float, default=10.0,
+                    help="guard factor of the learned start (probe 0.5 p(t-1) when the first "
+                         "iteration's error exceeds factor * tol; 0: off)")
     ap.add_argument("--ring-steps", type=int, default=12, help="steps of samples kept")
     ap.add_argument("--online-every", type=int, default=5)
     ap.add_argument("--online-iters", type=int, default=8)
@@ -803,7 +967,7 @@ def main():
     ap.add_argument("--online-async", type=int, default=1,
                     help="refresh on a second stream, adopted two solves later")
     ap.add_argument("--workload", default="bubbles",
-                    choices=["bubbles", "sand", "mobility", "ensemble"],
+                    choices=["bubbles", "sand", "mobility", "ensemble", "dambreak"],
                     help="bubbles: two-phase tank (headline); sand: MPM 2M particles; "
                          "mobility: RPY 4096 spheres x 64 RHS")
     ap.add_argument("--rpy-split", type=int, default=0)
@@ -826,6 +990,9 @@ def main():
         return
     if args.workload == "sand":
         bench_sand(args, sphx)
+        return
+    if args.workload == "dambreak":
+        bench_dambreak(args, sphx)
         return
     bench_bubbles(args, sphx)
 

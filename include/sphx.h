@@ -477,10 +477,25 @@ int sphx_scene_stats(sphx_context* ctx, int* iters, double* err);
  * eagerly (and falls back from a learned start that missed the tolerance).
  * Build after one eager sphx_mf_step; rebuild after uploads or parameter
  * changes.  Recording and the online refresh of the learned start are not
- * part of the graphs. */
+ * part of the graphs.  With the persistent solve on, graph A ends before
+ * the solve, the step launches the persistent solve (and the learned
+ * start's device-side fallback) between the graphs, and a step called with
+ * iters = err = NULL reads nothing back: no host round trip. */
 int sphx_graph_build(sphx_context* ctx, int budget);
 int sphx_graph_step(sphx_context* ctx, int* iters, double* err);
 int sphx_graph_release(sphx_context* ctx);
+/* Persistent solve (no counterpart in the reference, whose PressureSolve
+ * makes host round trips every iteration, fluid_system_host.cu:525-552):
+ * the pressure solve of every step as one cooperative launch of
+ * co-resident workgroups, each iteration's DP pass, Jacobi pass and FP64
+ * convergence test separated by grid barriers; with a learned start, the
+ * fallback to the plain start runs on the device too.  Same arithmetic
+ * and results as the per-iteration launches.  Releases captured graphs. */
+int sphx_persistent_solve(sphx_context* ctx, int on);
+/* {iterations, final error} of the last min(solves, max_solves) persistent
+ * solves, oldest first (a device ring of 4096, restarted when the
+ * persistent solve is switched on); *count = how many were written. */
+int sphx_solve_history(sphx_context* ctx, int* iters, double* err, int max_solves, int* count);
 
 /* Project autotuner.  Workgroup size per neighbour-loop kernel class (timer
  * names iisph_density, iisph_dii, iisph_aii, iisph_dp, iisph_iter,
@@ -603,12 +618,15 @@ int sphx_mlp_train(sphx_context* ctx, int epochs, int batch, float lr, const flo
                    uint64_t seed, double* loss_out);
 int sphx_mlp_stats(sphx_context* ctx, int64_t* learned, int64_t* fallbacks,
                    int* last_fallback);
-/* Guard of the learned start (single scene; on by default): after a
- * learned solve's first iteration misses the tolerance, one probe
- * iteration from the plain start p0 = warm_start * p(t-1) runs and the
- * solve continues from the better first iterate; the probe counts as an
- * iteration.  Decided on the device (graph-safe). */
-int sphx_mlp_guard(sphx_context* ctx, int on);
+/* Guard of the learned start against a broken prediction (single scene;
+ * factor 10 by default, 0 switches it off): when a learned solve's first
+ * iteration leaves an error above factor * tol, one probe iteration from
+ * the plain start p0 = warm_start * p(t-1) runs and the solve continues
+ * from the better first iterate; the probe counts as an iteration.
+ * Decided on the device (graph-safe).  A factor near 1 makes the choice
+ * greedy per step, which prefers the plain start's slowly growing
+ * compression and loses the learned start's pressure over later steps. */
+int sphx_mlp_guard(sphx_context* ctx, float factor);
 /* Probes run and probes that kept the plain start, since sphx_mlp_init. */
 int sphx_mlp_guard_stats(sphx_context* ctx, int64_t* probes, int64_t* plain_picks);
 

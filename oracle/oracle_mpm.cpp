@@ -7,7 +7,8 @@
 // The reference gathers node sums over the 27 SPH cells around each node;
 // for an SPH cell at least as wide as the spline support that is exactly
 // the particles with a non-zero weight, so the sums are restated here per
-// particle with FP64 node accumulators (order-independent to rounding).
+// particle with FP64 node accumulators (order-independent to rounding), the
+// particles spread over OpenMP threads.
 // kernel 0 is the reference's cubic spline (gradients in cell units, as
 // the reference); kernel 1 the quadratic spline of the sand configuration.
 // physical = 1 scales gradients to metres and uses the momentum-conserving
@@ -118,6 +119,9 @@ void p2g(const sphx_ParamCarrier& P, OrcState& s, OrcMPM& M, int apic) {
   const T cs = (T)g.dx, ss = (T)g.ss;
   T iDp = cs * cs * ss * ss / (M.kernel == 0 ? T(3.0f) : T(4.0f));   // :844-845
   iDp = T(1) / iDp;
+  // particles in parallel; the FP64 node sums are order-independent to
+  // rounding, so they take atomic adds
+#pragma omp parallel for schedule(static)
   for (int p = 0; p < s.n; ++p) {
     if (s.type[p] == SPHX_TYPE_BOUNDARY || s.mgcell[p] == UNDEF) continue;
     const T* xp = &at<T>(s.pos, 3 * (size_t)p);
@@ -142,11 +146,18 @@ void p2g(const sphx_ParamCarrier& P, OrcState& s, OrcMPM& M, int apic) {
               vb[q] += (Bp[3 * q] * iDp) * d[0] + (Bp[3 * q + 1] * iDp) * d[1] + (Bp[3 * q + 2] * iDp) * d[2];
           }
           size_t id = node_id(g, x, y, z);
-          m[id] += (double)(wt * mp);
-          for (int q = 0; q < 3; ++q) mv[3 * id + q] += (double)(vb[q] * mp * wt);
+          const double dm = (double)(wt * mp);
+#pragma omp atomic
+          m[id] += dm;
+          for (int q = 0; q < 3; ++q) {
+            const double dq = (double)(vb[q] * mp * wt);
+#pragma omp atomic
+            mv[3 * id + q] += dq;
+          }
         }
   }
-  for (size_t i = 0; i < N; ++i) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)N; ++i) {
     if (m[i] > 1e-9) {
       for (int q = 0; q < 3; ++q) at<T>(M.node_v, 3 * i + q) = (T)(mv[3 * i + q] / m[i]);
       at<T>(M.node_m, i) = (T)m[i];
@@ -195,6 +206,7 @@ void stress(const sphx_ParamCarrier& P, OrcState& s, OrcMPM& M) {
   Grid g = grid_of(P);
   const T cs = (T)g.dx, rc = cs * (T)g.ss;
   const T dt = (T)P.dt;
+#pragma omp parallel for schedule(static)
   for (int p = 0; p < s.n; ++p) {
     if (s.type[p] == SPHX_TYPE_BOUNDARY) continue;
     const T* xp = &at<T>(s.pos, 3 * (size_t)p);
@@ -269,6 +281,7 @@ void node_update(const sphx_ParamCarrier& P, OrcState& s, OrcMPM& M) {
   size_t N = (size_t)g.res[0] * g.res[1] * g.res[2];
   std::vector<double> f(3 * N, 0.0);
   const T cs = (T)g.dx, rc = cs * (T)g.ss;
+#pragma omp parallel for schedule(static)
   for (int p = 0; p < s.n; ++p) {
     if (s.type[p] == SPHX_TYPE_BOUNDARY || s.mgcell[p] == UNDEF) continue;
     const T* xp = &at<T>(s.pos, 3 * (size_t)p);
@@ -293,11 +306,16 @@ void node_update(const sphx_ParamCarrier& P, OrcState& s, OrcMPM& M) {
           T sn[3];
           for (int q = 0; q < 3; ++q) sn[q] = sig[3 * q] * nw[0] + sig[3 * q + 1] * nw[1] + sig[3 * q + 2] * nw[2];
           T scale = M.physical ? mp / (rp * mi * rc) : T(1) / rp;
-          for (int q = 0; q < 3; ++q) f[3 * id + q] += (double)(sn[q] * scale);
+          for (int q = 0; q < 3; ++q) {
+            const double fq = (double)(sn[q] * scale);
+#pragma omp atomic
+            f[3 * id + q] += fq;
+          }
         }
   }
   const T dt = (T)P.dt;
   const T gv[3] = {(T)P.gravity.x, (T)P.gravity.y, (T)P.gravity.z};
+#pragma omp parallel for schedule(static)
   for (int y = 0; y < g.res[1]; ++y)
     for (int z = 0; z < g.res[2]; ++z)
       for (int x = 0; x < g.res[0]; ++x) {
@@ -316,6 +334,7 @@ template <class T>
 void g2p(const sphx_ParamCarrier& P, OrcState& s, OrcMPM& M, int apic) {
   Grid g = grid_of(P);
   const T cs = (T)g.dx, ss = (T)g.ss, dt = (T)P.dt;
+#pragma omp parallel for schedule(static)
   for (int p = 0; p < s.n; ++p) {
     if (s.type[p] == SPHX_TYPE_BOUNDARY) continue;
     T* xp = &at<T>(s.pos, 3 * (size_t)p);

@@ -285,9 +285,10 @@ struct sphx_context {
     long long learned_solves = 0, fallbacks = 0;
     int last_fallback = 0;
     // guard of the learned start (single scene): after the solve's first
-    // iteration, a probe iteration from the plain start when that iteration
-    // missed the tolerance, and the better first iterate kept
+    // iteration, a probe iteration from the plain start when that iteration's
+    // error exceeds guard_factor * tolerance, and the better first iterate kept
     int guard = 1;                      // on by default
+    float guard_factor = 10.f;
     bool guard_armed = false;           // the next solve's first iteration gets the guard
     sphx::DevBuf<int> gstate;           // [0] skip probe, [1] restore learned, [2] probes, [3] plain picks
     sphx::DevBuf<float> pkeep;          // the learned start's first iterate
@@ -335,6 +336,14 @@ struct sphx_context {
   // iterations) and graph B (integration), one of each per buffer parity
   hipGraphExec_t graph_a[2] = {nullptr, nullptr}, graph_b[2] = {nullptr, nullptr};
   int graph_budget = 0, graph_num = -1;
+
+  // persistent solve: the whole pressure solve as one cooperative launch
+  bool coop = false;
+  int coop_max = 0;                // co-resident workgroups (0: not queried yet)
+  int coop_fallbacks = 0;          // device-side fallbacks of the last solve
+  // per persistent solve {iterations, error}, appended on the device
+  sphx::DevBuf<double> solve_log;
+  sphx::DevBuf<int> solve_log_count;
 
   // workgroup sizes per kernel class (TuneId)
   // defaults: the autotuner's picks on the 1M-particle two-phase tank

@@ -22,6 +22,10 @@ int enqueue_solve_fixed(sphx_context* c, int n);
 int iisph_density_dii(sphx_context* c);
 int mf_correction_force(sphx_context* c);
 int continue_solve(sphx_context* c, int* iters, double* err);
+int enqueue_solve_coop(sphx_context* c);
+int read_solve(sphx_context* c, int* iters, double* err);
+int solve_log_setup(sphx_context* c);
+int solve_log_read(sphx_context* c, int* iters, double* err, int max, int* count);
 int mlp_begin(sphx_context* c);
 int mlp_fallback(sphx_context* c, int* iters, double* err);
 int mlp_after(sphx_context* c, int record_ok);
@@ -53,7 +57,8 @@ static int stage_a(sphx_context* c, int budget) {
   SPHX_TRY(mf_correction_force(c));
   SPHX_TRY(sphx_iisph_predict_aii(c));
   SPHX_TRY(mlp_begin(c));
-  return enqueue_solve_fixed(c, budget);
+  // the persistent solve is launched between the graphs
+  return c->coop ? 0 : enqueue_solve_fixed(c, budget);
 }
 
 static int stage_b(sphx_context* c) {
@@ -124,6 +129,8 @@ int sphx_graph_step(sphx_context* c, int* iters, double* err) {
   SPHX_CHECK(c, "sphx_graph_step", "null context");
   SPHX_CHECK(c->graph_num == c->num && c->graph_a[c->cur], "sphx_graph_step",
              "no step graph for this state (call sphx_graph_build)");
+  SPHX_CHECK(iters || err || c->coop, "sphx_graph_step",
+             "a step without outcome needs the persistent solve (sphx_persistent_solve)");
   SPHX_TRY(ensure_device(c));
   SPHX_TRY(mlp_async_settle(c));   // graph steps adopt a finished refresh at once
   const int p = c->cur;
@@ -131,13 +138,42 @@ int sphx_graph_step(sphx_context* c, int* iters, double* err) {
   after_graph_a(c, p);
   int it = 0;
   double e = 0.0;
-  SPHX_TRY(continue_solve(c, &it, &e));
-  SPHX_TRY(mlp_fallback(c, &it, &e));
+  if (c->coop) {
+    // the whole solve and the learned start's fallback on the device; the
+    // host reads the outcome only when asked for it
+    SPHX_TRY(enqueue_solve_coop(c));
+    if (iters || err) {
+      SPHX_TRY(read_solve(c, &it, &e));
+      SPHX_TRY(mlp_fallback(c, &it, &e));
+    }
+  } else {
+    SPHX_TRY(continue_solve(c, &it, &e));
+    SPHX_TRY(mlp_fallback(c, &it, &e));
+  }
   SPHX_HIP(hipGraphLaunch(c->graph_b[1 - p], c->stream));
   positions_changed(c);
   if (iters) *iters = it;
   if (err) *err = e;
   return 0;
+}
+
+// Persistent solve on or off (graphs are recaptured by the next
+// sphx_graph_build).
+int sphx_persistent_solve(sphx_context* c, int on) {
+  SPHX_CHECK(c, "sphx_persistent_solve", "null context");
+  SPHX_TRY(ensure_device(c));
+  if (c->coop != (on != 0)) graph_release(c);
+  c->coop = on != 0;
+  if (c->coop) {
+    SPHX_TRY(solve_log_setup(c));
+    SPHX_HIP(hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+int sphx_solve_history(sphx_context* c, int* iters, double* err, int max_solves, int* count) {
+  SPHX_CHECK(c && count && max_solves >= 0, "sphx_solve_history", "bad argument");
+  return solve_log_read(c, iters, err, max_solves, count);
 }
 
 int sphx_graph_release(sphx_context* c) {

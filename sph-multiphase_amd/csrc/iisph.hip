@@ -19,7 +19,10 @@
 // reference's press_l1 -> press_l copy (host.cu:536) is not needed.
 #include "mlp_feat.hpp"
 
+#include <hip/hip_cooperative_groups.h>
+
 #include <cmath>
+#include <vector>
 
 namespace sphx {
 
@@ -297,24 +300,33 @@ __global__ void k_ii_gh(KP k, NbrList L, const uint32_t* __restrict__ key, const
 // Pressure_DP (kern.cu:910-958) plus q_i = d_ii p_i + dijpj_i, stored as
 // the record the Jacobi pass gathers ({x, m q.x}, {m q.y, m q.z}), and the
 // pressure acceleration q_i / dt^2 (kern.cu:1044-1047).
-__global__ void k_ii_dp(KP k, NbrList L, const int* __restrict__ done, const uint32_t* __restrict__ key,
-                        const int* __restrict__ cs, const float4* __restrict__ pos,
-                        const float4* __restrict__ ps, const float4* __restrict__ phys,
-                        const float4* __restrict__ dii, const float* __restrict__ press_l,
-                        float4* __restrict__ dijpj, float4* __restrict__ mqa,
-                        float2* __restrict__ mqb, float4* __restrict__ accel, SceneCtl sc) {
-  if (*done) return;
-  int i = xcd_index();
+struct DpArgs {
+  const uint32_t* __restrict__ key;
+  const int* __restrict__ cs;
+  const float4* __restrict__ pos;
+  const float4* __restrict__ ps;
+  const float4* __restrict__ phys;
+  const float4* __restrict__ dii;
+  const float* __restrict__ press_l;
+  float4* __restrict__ dijpj;
+  float4* __restrict__ mqa;
+  float2* __restrict__ mqb;
+  float4* __restrict__ accel;
+};
+
+__device__ __forceinline__ void dp_particle(const KP& k, const NbrList& L, const DpArgs& a,
+                                            const SceneCtl& sc, int i) {
   if (i >= k.n) return;
-  float4 pi = pos[i];
+  float4 pi = a.pos[i];
   if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
   if (sc.n && sc.state[2 * scene_of(pi)]) return;   // converged scene: p is final
-  uint32_t gc = key[i];
+  uint32_t gc = a.key[i];
   if ((int)gc == k.gridtotal) return;
   float sx = 0.f, sy = 0.f, sz = 0.f;
   // one 16-byte gather per neighbour: {x_j, s_j}; boundary s_j = 0 adds
   // nothing, as the reference's skip of boundary j (:925-926)
-  for_each_listed_b<NBR_BATCH_SMALL, false>(k, L, cs, gc, i, [&](int j) { return ps[j]; },
+  const float4* __restrict__ ps = a.ps;
+  for_each_listed_b<NBR_BATCH_SMALL, false>(k, L, a.cs, gc, i, [&](int j) { return ps[j]; },
                                      [&](int, const float4& pj) {
     float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
     float dsq = dx * dx + dy * dy + dz * dz;
@@ -326,15 +338,20 @@ __global__ void k_ii_dp(KP k, NbrList L, const int* __restrict__ done, const uin
   });
   float f = -k.dt * k.dt;
   float4 d = make_float4(sx * f, sy * f, sz * f, 0.f);
-  dijpj[i] = d;
-  float4 di = dii[i];
-  float p = press_l[i];
+  a.dijpj[i] = d;
+  float4 di = a.dii[i];
+  float p = a.press_l[i];
   const float qx = di.x * p + d.x, qy = di.y * p + d.y, qz = di.z * p + d.z;
-  const float mi = phys[i].x;
-  mqa[i] = make_float4(pi.x, pi.y, pi.z, mi * qx);
-  mqb[i] = make_float2(mi * qy, mi * qz);
+  const float mi = a.phys[i].x;
+  a.mqa[i] = make_float4(pi.x, pi.y, pi.z, mi * qx);
+  a.mqb[i] = make_float2(mi * qy, mi * qz);
   const float idt2 = 1.f / (k.dt * k.dt);
-  accel[i] = make_float4(qx * idt2, qy * idt2, qz * idt2, 0.f);
+  a.accel[i] = make_float4(qx * idt2, qy * idt2, qz * idt2, 0.f);
+}
+
+__global__ void k_ii_dp(KP k, NbrList L, const int* __restrict__ done, DpArgs a, SceneCtl sc) {
+  if (*done) return;
+  dp_particle(k, L, a, sc, xcd_index());
 }
 
 // Block reduction of (signed residue, |residue|/rho0, fluid count) in FP64.
@@ -357,106 +374,132 @@ __device__ void block_reduce3(double a, double b, double c, double* out) {
   }
 }
 
-// Pressure_Iter (kern.cu:960-1050)
-__global__ __launch_bounds__(NB_THREADS) void k_ii_iter(
-    KP k, NbrList L, const int* __restrict__ done, const uint32_t* __restrict__ key,
-    const int* __restrict__ cs, const float4* __restrict__ pos, float4* __restrict__ phys,
-    const float4* __restrict__ dii, const float4* __restrict__ dijpj,
-    const float4* __restrict__ mqa, const float2* __restrict__ mqb,
-    const float4* __restrict__ gh, const float* __restrict__ rho_adv,
-    float* __restrict__ press_l, float* __restrict__ press_l1, float4* __restrict__ ps,
-    float* __restrict__ residue, double* __restrict__ partial, SceneCtl sc) {
+// Pressure_Iter (kern.cu:960-1050) for particle i: the relaxed Jacobi
+// update and the residual; rs, ra, rc receive its (signed residue,
+// error / rho0, 1) when it counts toward the solve's error, scene its scene.
+struct IterArgs {
+  const uint32_t* __restrict__ key;
+  const int* __restrict__ cs;
+  const float4* __restrict__ pos;
+  float4* __restrict__ phys;
+  const float4* __restrict__ dii;
+  const float4* __restrict__ dijpj;
+  const float4* __restrict__ mqa;
+  const float2* __restrict__ mqb;
+  const float4* __restrict__ gh;
+  const float* __restrict__ rho_adv;
+  float* __restrict__ press_l;
+  float* __restrict__ press_l1;
+  float4* __restrict__ ps;
+  float* __restrict__ residue;
+};
+
+__device__ __forceinline__ void iter_particle(const KP& k, const NbrList& L, const IterArgs& a,
+                                              const SceneCtl& sc, int i, double& rs, double& ra,
+                                              double& rc, int& scene) {
+  rs = ra = rc = 0.0;
+  scene = 0;
+  if (i >= k.n) return;
+  float4 pi = a.pos[i];
+  uint32_t gc = a.key[i];
+  scene = scene_of(pi);
+  if (type_of(pi) == SPHX_TYPE_BOUNDARY || (int)gc == k.gridtotal) {
+    a.residue[i] = 0.f;
+    return;
+  }
+  if (sc.n && sc.state[2 * scene]) return;   // converged scene: pressures final
+  const float4 qi = a.phys[i];
+  const float mi = qi.x, rhoi = qi.y, rest = qi.w;
+  const float4 di = a.dii[i];
+  const float4 dpi = a.dijpj[i];
+  const float pl = a.press_l[i];
+  const float cji = k.dt * k.dt * mi / rhoi / rhoi * pl;
+  // S_i = sum_j m_j (dijpj_i - q_j + d_ji p_i) . gradW_ij over fluid j
+  // (kern.cu:992-999), plus m_b dijpj_i . gradW_ij over boundary j when
+  // boundary pressure is on (:1000-1002).  The position-only parts are
+  // sums of the a_ii pass: S_i = dijpj_i . G_i + (d_ji p_i / gradW) H_i -
+  // sum_j m_j q_j . gradW_ij, so the loop gathers one record pair per
+  // neighbour, {x_j, m_j q_j.x} and {m_j q_j.y, m_j q_j.z} (zero q for
+  // boundary j).
+  float acc = 0.f;
+  struct Rec { float4 a; float2 b; };
+  const float4* __restrict__ mqa = a.mqa;
+  const float2* __restrict__ mqb = a.mqb;
+  for_each_listed_b(k, L, a.cs, gc, i, [&](int j) { return Rec{mqa[j], mqb[j]}; },
+                    [&](int, const Rec& o) {
+    float dx = (pi.x - o.a.x) * k.ss, dy = (pi.y - o.a.y) * k.ss, dz = (pi.z - o.a.z) * k.ss;
+    float dsq = dx * dx + dy * dy + dz * dz;
+    if (!in_support(k, dsq)) return;
+    float r, ir;
+    dist_rinv(dsq, r, ir);
+    float gf = cubic_gradf(r, ir, k.inv_h, k.kspline);
+    acc += gf * (dx * o.a.w + dy * o.b.x + dz * o.b.y);
+  });
+  const float4 G = a.gh[i];
+  const float S = dpi.x * G.x + dpi.y * G.y + dpi.z * G.z + cji * G.w - acc;
+  float aii = di.w;
+  float pn;
+  if (fabsf(aii) < 1e-6f) pn = pl;
+  else pn = (1.f - k.omega) * pl + k.omega / aii * (rest - a.rho_adv[i] - S);
+  if (k.clamp_p && pn < 0.f) pn = 0.f;   // kern.cu:1035-1036 (commented there)
+  float res = -rest + (a.rho_adv[i] + S + aii * pl);
+  a.residue[i] = res;
+  a.phys[i].z = pn;
+  a.press_l[i] = pn;
+  a.press_l1[i] = pn;
+  a.ps[i].w = mi * pn / rhoi / rhoi;
+  rs = res;
+  // With the clamp the solve is a complementarity problem (p > 0 and
+  // rho = rho0, or p = 0 and rho <= rho0): the error is |res| where the
+  // residual's pressure p_l is positive, compression only where it is
+  // zero.  NaN must propagate into the sum (fmax would drop it and
+  // report convergence), so the one-sided part is written as a branch.
+  double rr = (double)res;
+  ra = (k.clamp_p && !(pl > 0.f) ? (rr < 0.0 ? 0.0 : rr) : fabs(rr)) / (double)rest;
+  rc = 1.0;
+}
+
+// Per-scene sums of a workgroup: LDS atomics, then one global atomic per
+// touched entry (sacc holds 3 * sc.n doubles, zeroed by the caller).
+__device__ __forceinline__ void scene_accumulate(const SceneCtl& sc, double* sacc, int scene,
+                                                 double rs, double ra, double rc) {
+  __syncthreads();
+  if (rc != 0.0) {
+    atomicAdd(&sacc[3 * scene], rs);
+    atomicAdd(&sacc[3 * scene + 1], ra);
+    atomicAdd(&sacc[3 * scene + 2], rc);
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < 3 * sc.n; t += blockDim.x)
+    if (sacc[t] != 0.0) atomicAdd(&sc.acc[t], sacc[t]);
+}
+
+__global__ __launch_bounds__(NB_THREADS) void k_ii_iter(KP k, NbrList L,
+                                                        const int* __restrict__ done, IterArgs a,
+                                                        double* __restrict__ partial,
+                                                        SceneCtl sc) {
   if (*done) return;
   __shared__ double sacc[SPHX_MAX_SCENES * 3];
   if (sc.n)
     for (int t = threadIdx.x; t < 3 * sc.n; t += blockDim.x) sacc[t] = 0.0;
-  int i = xcd_index();
-  double rs = 0.0, ra = 0.0, rc = 0.0;
-  int scene = 0;
-  if (i < k.n) {
-    float4 pi = pos[i];
-    uint32_t gc = key[i];
-    scene = scene_of(pi);
-    if (type_of(pi) == SPHX_TYPE_BOUNDARY || (int)gc == k.gridtotal) {
-      residue[i] = 0.f;
-    } else if (sc.n && sc.state[2 * scene]) {
-      // converged scene: pressures final, contributes nothing
-    } else {
-      const float4 qi = phys[i];
-      const float mi = qi.x, rhoi = qi.y, rest = qi.w;
-      const float4 di = dii[i];
-      const float4 dpi = dijpj[i];
-      const float pl = press_l[i];
-      const float cji = k.dt * k.dt * mi / rhoi / rhoi * pl;
-      // S_i = sum_j m_j (dijpj_i - q_j + d_ji p_i) . gradW_ij over fluid j
-      // (kern.cu:992-999), plus m_b dijpj_i . gradW_ij over boundary j when
-      // boundary pressure is on (:1000-1002).  The position-only parts are
-      // sums of the a_ii pass: S_i = dijpj_i . G_i + (d_ji p_i / gradW) H_i -
-      // sum_j m_j q_j . gradW_ij, so the loop gathers one record pair per
-      // neighbour, {x_j, m_j q_j.x} and {m_j q_j.y, m_j q_j.z} (zero q for
-      // boundary j).
-      float acc = 0.f;
-      struct Rec { float4 a; float2 b; };
-      for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{mqa[j], mqb[j]}; },
-                        [&](int, const Rec& o) {
-        float dx = (pi.x - o.a.x) * k.ss, dy = (pi.y - o.a.y) * k.ss, dz = (pi.z - o.a.z) * k.ss;
-        float dsq = dx * dx + dy * dy + dz * dz;
-        if (!in_support(k, dsq)) return;
-        float r, ir;
-        dist_rinv(dsq, r, ir);
-        float gf = cubic_gradf(r, ir, k.inv_h, k.kspline);
-        acc += gf * (dx * o.a.w + dy * o.b.x + dz * o.b.y);
-      });
-      const float4 G = gh[i];
-      const float S = dpi.x * G.x + dpi.y * G.y + dpi.z * G.z + cji * G.w - acc;
-      float aii = di.w;
-      float pn;
-      if (fabsf(aii) < 1e-6f) pn = pl;
-      else pn = (1.f - k.omega) * pl + k.omega / aii * (rest - rho_adv[i] - S);
-      if (k.clamp_p && pn < 0.f) pn = 0.f;   // kern.cu:1035-1036 (commented there)
-      float res = -rest + (rho_adv[i] + S + aii * pl);
-      residue[i] = res;
-      phys[i].z = pn;
-      press_l[i] = pn;
-      press_l1[i] = pn;
-      ps[i].w = mi * pn / rhoi / rhoi;
-      rs = res;
-      // With the clamp the solve is a complementarity problem (p > 0 and
-      // rho = rho0, or p = 0 and rho <= rho0): the error is |res| where the
-      // residual's pressure p_l is positive, compression only where it is
-      // zero.  NaN must propagate into the sum (fmax would drop it and
-      // report convergence), so the one-sided part is written as a branch.
-      double rr = (double)res;
-      ra = (k.clamp_p && !(pl > 0.f) ? (rr < 0.0 ? 0.0 : rr) : fabs(rr)) / (double)rest;
-      rc = 1.0;
-    }
-  }
+  double rs, ra, rc;
+  int scene;
+  iter_particle(k, L, a, sc, xcd_index(), rs, ra, rc, scene);
   if (sc.n) {
-    // per-scene sums: LDS atomics, then one global atomic per touched entry
-    __syncthreads();
-    if (rc != 0.0) {
-      atomicAdd(&sacc[3 * scene], rs);
-      atomicAdd(&sacc[3 * scene + 1], ra);
-      atomicAdd(&sacc[3 * scene + 2], rc);
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < 3 * sc.n; t += blockDim.x)
-      if (sacc[t] != 0.0) atomicAdd(&sc.acc[t], sacc[t]);
+    scene_accumulate(sc, sacc, scene, rs, ra, rc);
     return;
   }
   block_reduce3(rs, ra, rc, partial);
 }
 
-// Sum the partials, count the iteration and decide convergence.
-__global__ __launch_bounds__(1024) void k_ii_finalize(int* __restrict__ state,
-                                                      double* __restrict__ err,
-                                                      const double* __restrict__ partial,
-                                                      int nblocks, int num, IterCtl ctl,
-                                                      int respect_done) {
-  if (respect_done && state[0]) return;
+// Sum the partials, count the iteration and decide convergence (one
+// workgroup of up to 1024 threads).
+__device__ void finalize_single(int* __restrict__ state, double* __restrict__ err,
+                                const double* __restrict__ partial, int nblocks, int num,
+                                IterCtl ctl) {
   __shared__ double sh[3][16];
   double a = 0, b = 0, c = 0;
-  for (int q = threadIdx.x; q < nblocks; q += 1024) {
+  for (int q = threadIdx.x; q < nblocks; q += blockDim.x) {
     a += partial[3 * q];
     b += partial[3 * q + 1];
     c += partial[3 * q + 2];
@@ -471,7 +514,7 @@ __global__ __launch_bounds__(1024) void k_ii_finalize(int* __restrict__ state,
   __syncthreads();
   if (threadIdx.x == 0) {
     double x = 0, y = 0, z = 0;
-    for (int q = 0; q < 16; ++q) { x += sh[0][q]; y += sh[1][q]; z += sh[2][q]; }
+    for (int q = 0; q < (int)(blockDim.x + 63) / 64; ++q) { x += sh[0][q]; y += sh[1][q]; z += sh[2][q]; }
     double e = z > 0 ? y / z : 0.0;
     err[0] = e;
     err[1] = num > 0 ? x / num : 0.0;
@@ -481,14 +524,20 @@ __global__ __launch_bounds__(1024) void k_ii_finalize(int* __restrict__ state,
   }
 }
 
+__global__ __launch_bounds__(1024) void k_ii_finalize(int* __restrict__ state,
+                                                      double* __restrict__ err,
+                                                      const double* __restrict__ partial,
+                                                      int nblocks, int num, IterCtl ctl,
+                                                      int respect_done) {
+  if (respect_done && state[0]) return;
+  finalize_single(state, err, partial, nblocks, num, ctl);
+}
+
 // Ensemble version: per-scene error, iteration count and convergence; the
 // solve is done when every scene is.  err[0] = largest scene error,
-// state[1] = most iterations of any scene.
-__global__ __launch_bounds__(256) void k_ii_finalize_scenes(int* __restrict__ state,
-                                                            double* __restrict__ err,
-                                                            SceneCtl sc, IterCtl ctl,
-                                                            int respect_done) {
-  if (respect_done && state[0]) return;
+// state[1] = most iterations of any scene.  One workgroup of 256 threads.
+__device__ void finalize_scenes(int* __restrict__ state, double* __restrict__ err,
+                                const SceneCtl& sc, IterCtl ctl) {
   __shared__ int s_open[256], s_it[256];
   __shared__ double s_err[256];
   int open = 0, maxit = 0;
@@ -534,22 +583,186 @@ __global__ __launch_bounds__(256) void k_ii_finalize_scenes(int* __restrict__ st
   }
 }
 
+__global__ __launch_bounds__(256) void k_ii_finalize_scenes(int* __restrict__ state,
+                                                            double* __restrict__ err,
+                                                            SceneCtl sc, IterCtl ctl,
+                                                            int respect_done) {
+  if (respect_done && state[0]) return;
+  finalize_scenes(state, err, sc, ctl);
+}
+
+// Start of a solve: clear the done flag, iteration count and (ensemble)
+// every scene's state.  A kernel, not a memset, so it can sit in a graph.
+__global__ void k_solve_reset(int* __restrict__ state, SceneCtl sc) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) { state[0] = 0; state[1] = 0; state[2] = 0; state[3] = 0; }
+  if (t < sc.n) {
+    sc.state[2 * t] = 0;
+    sc.state[2 * t + 1] = 0;
+    sc.acc[3 * t] = 0.0;
+    sc.acc[3 * t + 1] = 0.0;
+    sc.acc[3 * t + 2] = 0.0;
+  }
+}
+
+// The whole pressure solve as one cooperative launch of G co-resident
+// workgroups (SolverOptions-independent: the same passes and arithmetic as
+// the DP / Jacobi / finalize launches).  Per iteration: the DP pass over
+// every particle, a grid barrier, the Jacobi pass with per-workgroup sums,
+// a grid barrier, the convergence test by workgroup 0, and a grid barrier
+// after which every workgroup reads the same done flag; at most max_iters
+// iterations.  Workgroups walk 256-particle tiles: XCD x (workgroup b goes
+// to XCD b % 8) owns the contiguous tile range [x q + min(x, r), ...) of the
+// cell-sorted particles (q, r = tiles / 8, tiles % 8), so neighbour gathers
+// stay in its L2.
+__device__ __forceinline__ void coop_tiles(int ntiles, int& first, int& count, int& lane0,
+                                           int& step) {
+  const int G = gridDim.x, b = blockIdx.x;
+  if (G % 8 == 0) {
+    const int x = b & 7, q = ntiles >> 3, r = ntiles & 7;
+    first = x * q + min(x, r);
+    count = q + (x < r ? 1 : 0);
+    lane0 = b >> 3;
+    step = G >> 3;
+  } else {
+    first = 0;
+    count = ntiles;
+    lane0 = b;
+    step = G;
+  }
+}
+
+__global__ __launch_bounds__(NB_THREADS) void k_ii_solve_coop(KP k, NbrList L, DpArgs da,
+                                                              IterArgs ia,
+                                                              int* __restrict__ state,
+                                                              double* __restrict__ err,
+                                                              double* __restrict__ partial,
+                                                              SceneCtl sc, IterCtl ctl,
+                                                              int ntiles) {
+  // a solve already done (a fallback launch with nothing to redo): every
+  // workgroup leaves before the first barrier
+  if (state[0]) return;
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  __shared__ double sacc[SPHX_MAX_SCENES * 3];
+  int first, count, lane0, step;
+  coop_tiles(ntiles, first, count, lane0, step);
+  for (int it = 0; it < ctl.max_iters; ++it) {
+    for (int t = lane0; t < count; t += step)
+      dp_particle(k, L, da, sc, (first + t) * NB_THREADS + (int)threadIdx.x);
+    grid.sync();
+    if (sc.n) {
+      for (int t = threadIdx.x; t < 3 * sc.n; t += blockDim.x) sacc[t] = 0.0;
+      __syncthreads();
+    }
+    double rs = 0.0, ra = 0.0, rc = 0.0;
+    for (int t = lane0; t < count; t += step) {
+      double a, b, c;
+      int scene;
+      iter_particle(k, L, ia, sc, (first + t) * NB_THREADS + (int)threadIdx.x, a, b, c, scene);
+      if (sc.n) {
+        if (c != 0.0) {
+          atomicAdd(&sacc[3 * scene], a);
+          atomicAdd(&sacc[3 * scene + 1], b);
+          atomicAdd(&sacc[3 * scene + 2], c);
+        }
+      } else {
+        rs += a; ra += b; rc += c;
+      }
+    }
+    if (sc.n) {
+      __syncthreads();
+      for (int t = threadIdx.x; t < 3 * sc.n; t += blockDim.x)
+        if (sacc[t] != 0.0) atomicAdd(&sc.acc[t], sacc[t]);
+    } else {
+      block_reduce3(rs, ra, rc, partial);
+    }
+    grid.sync();
+    if (blockIdx.x == 0) {
+      if (sc.n) finalize_scenes(state, err, sc, ctl);
+      else finalize_single(state, err, partial, gridDim.x, k.n, ctl);
+    }
+    grid.sync();
+    if (state[0]) break;   // the same value in every workgroup after the barrier
+  }
+}
+
+// Device-side fallback of a learned start, between two persistent solves:
+// where the solve missed the tolerance (the whole scene, or each scene of an
+// ensemble) the pressures restart from the plain start p0 = warm * p(t-1)
+// (fluid_system_kern.cu:899) and the solve is reopened; the iterations run
+// so far are kept in state[2] and the fallbacks counted in state[3].
+__global__ void k_fallback_prep(KP k, int* __restrict__ state, const double* __restrict__ err,
+                                SceneCtl sc, float tol, const float4* __restrict__ pos,
+                                const uint32_t* __restrict__ key, float4* __restrict__ phys,
+                                const float* __restrict__ pprev, float* __restrict__ press_l,
+                                float4* __restrict__ ps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sc.n) {
+    if (blockIdx.x == 0) {
+      __shared__ int any;
+      if (threadIdx.x == 0) any = 0;
+      __syncthreads();
+      for (int t = threadIdx.x; t < sc.n; t += blockDim.x)
+        if (!(sc.err[t] <= (double)tol)) {
+          sc.state[2 * t] = 0;
+          sc.state[2 * t + 1] = 0;
+          atomicOr(&any, 1);
+        }
+      __syncthreads();
+      if (threadIdx.x == 0 && any) {
+        state[2] += state[1];
+        state[3] += 1;
+        state[1] = 0;
+        state[0] = 0;
+      }
+    }
+  } else if (i == 0 && !(err[0] <= (double)tol)) {
+    state[2] += state[1];
+    state[3] += 1;
+    state[1] = 0;
+    state[0] = 0;
+  }
+  if (i >= k.n) return;
+  const float4 pp = pos[i];
+  if (type_of(pp) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) return;
+  const double e = sc.n ? sc.err[scene_of(pp)] : err[0];
+  if (e <= (double)tol) return;
+  float4 q = phys[i];
+  q.z = pprev[i];
+  phys[i] = q;
+  const float p0 = k.warm * q.z;
+  press_l[i] = p0;
+  ps[i].w = q.x * p0 / (q.y * q.y);
+}
+
+// Appends {iterations (with those a device fallback redid), error} of the
+// solve just finished to a ring of `cap` entries.
+constexpr int SOLVE_LOG = 4096;
+__global__ void k_solve_log(const int* __restrict__ state, const double* __restrict__ err,
+                            double* __restrict__ ring, int* __restrict__ count) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int slot = *count % SOLVE_LOG;
+  ring[2 * slot] = (double)(state[1] + state[2]);
+  ring[2 * slot + 1] = err[0];
+  *count += 1;
+}
+
 // Guard of the learned warm start, enqueued after the solve's first
 // iteration (which ran from the learned start).  When that iteration's
-// error exceeds the tolerance (or is NaN), the learned first iterate is
+// error exceeds `limit` = factor * tolerance (or is NaN), the learned first iterate is
 // kept aside and one probe iteration runs from the plain start p0 = warm *
 // p(t-1) (fluid_system_kern.cu:899); the solve then continues from whichever
 // first iterate has the smaller error, and the probe counts as an iteration.
 // Every decision is made on the device, so a captured step can hold it.
 // gs: [0] skip the probe, [1] restore the learned iterate, [2] probes run,
 // [3] probes that kept the plain start.
-__global__ void k_guard_prep(KP k, const double* __restrict__ err, float tol, int* __restrict__ gs,
+__global__ void k_guard_prep(KP k, const double* __restrict__ err, double limit, int* __restrict__ gs,
                              const float4* __restrict__ pos, const uint32_t* __restrict__ key,
                              const float4* __restrict__ phys, const float* __restrict__ pprev,
                              float* __restrict__ press_l, float4* __restrict__ ps,
                              float* __restrict__ keep) {
   const double e = err[0];
-  const bool probe = !(e <= (double)tol);
+  const bool probe = !(e <= limit);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) gs[0] = probe ? 0 : 1;
   if (!probe || i >= k.n) return;
@@ -619,20 +832,6 @@ __global__ void k_guard_restore(KP k, const int* __restrict__ gs, const float4* 
   press_l[i] = p;
   press_l1[i] = p;
   ps[i].w = q.x * p / (q.y * q.y);
-}
-
-// Start of a solve: clear the done flag, iteration count and (ensemble)
-// every scene's state.  A kernel, not a memset, so it can sit in a graph.
-__global__ void k_solve_reset(int* __restrict__ state, SceneCtl sc) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) { state[0] = 0; state[1] = 0; }
-  if (t < sc.n) {
-    sc.state[2 * t] = 0;
-    sc.state[2 * t + 1] = 0;
-    sc.acc[3 * t] = 0.0;
-    sc.acc[3 * t + 1] = 0.0;
-    sc.acc[3 * t + 2] = 0.0;
-  }
 }
 
 // IntegrateIISPH (kern.cu:1052-1064); optional clamp into the valid cells.
@@ -901,6 +1100,17 @@ int nbr_list(sphx_context* c, NbrList* out) {
   return 0;
 }
 
+static DpArgs dp_args(sphx_context* c) {
+  return DpArgs{c->key.p, c->cell_start.p, c->POS(), c->ps.p, c->PHYS(), c->dii.p, c->press_l.p,
+                c->dijpj.p, c->mqa.p, c->mqb.p, c->ACCEL()};
+}
+
+static IterArgs iter_args(sphx_context* c) {
+  return IterArgs{c->key.p, c->cell_start.p, c->POS(), c->PHYS(), c->dii.p, c->dijpj.p,
+                  c->mqa.p, c->mqb.p, c->gh.p, c->rho_adv.p, c->press_l.p, c->press_l1.p,
+                  c->ps.p, c->residue.p};
+}
+
 static int ensure_red(sphx_context* c) {
   size_t nb = (size_t)blocks_for(c->cap, 64);   // smallest tuned workgroup
   return c->red.alloc(3 * nb + 16);
@@ -928,16 +1138,10 @@ static int enqueue_iteration(sphx_context* c, const KP& k, int respect_done) {
   IterCtl ctl{c->O.min_iters, c->O.max_iters, c->O.tol};
   SceneCtl sc = c->scenes();
   timer_begin(c, "iisph_dp");
-  k_ii_dp<<<blocks_for(k.n, tdp), tdp, 0, c->stream>>>(k, L, c->solve_state.p, c->key.p, c->cell_start.p,
-                                            c->POS(), c->ps.p, c->PHYS(), c->dii.p,
-                                            c->press_l.p, c->dijpj.p, c->mqa.p, c->mqb.p,
-                                            c->ACCEL(), sc);
+  k_ii_dp<<<blocks_for(k.n, tdp), tdp, 0, c->stream>>>(k, L, c->solve_state.p, dp_args(c), sc);
   timer_end(c);
   timer_begin(c, "iisph_iter");
-  k_ii_iter<<<nb, tit, 0, c->stream>>>(k, L, c->solve_state.p, c->key.p, c->cell_start.p,
-                                              c->POS(), c->PHYS(), c->dii.p, c->dijpj.p, c->mqa.p,
-                                              c->mqb.p, c->gh.p, c->rho_adv.p, c->press_l.p,
-                                              c->press_l1.p, c->ps.p, c->residue.p, c->red.p, sc);
+  k_ii_iter<<<nb, tit, 0, c->stream>>>(k, L, c->solve_state.p, iter_args(c), c->red.p, sc);
   timer_end(c);
   if (sc.n)
     k_ii_finalize_scenes<<<1, 256, 0, c->stream>>>(c->solve_state.p, c->solve_err.p, sc, ctl,
@@ -962,19 +1166,13 @@ static int enqueue_guard(sphx_context* c, const KP& k) {
   SceneCtl none{0, nullptr, nullptr, nullptr};
   int* gs = m.gstate.p;
   k_guard_prep<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
-      k, c->solve_err.p, c->O.tol, gs, c->POS(), c->key.p, c->PHYS(), m.pprev.p, c->press_l.p,
-      c->ps.p, m.pkeep.p);
+      k, c->solve_err.p, (double)m.guard_factor * (double)c->O.tol, gs, c->POS(), c->key.p,
+      c->PHYS(), m.pprev.p, c->press_l.p, c->ps.p, m.pkeep.p);
   timer_begin(c, "iisph_dp");
-  k_ii_dp<<<blocks_for(k.n, tdp), tdp, 0, c->stream>>>(k, L, gs, c->key.p, c->cell_start.p,
-                                                       c->POS(), c->ps.p, c->PHYS(), c->dii.p,
-                                                       c->press_l.p, c->dijpj.p, c->mqa.p,
-                                                       c->mqb.p, c->ACCEL(), none);
+  k_ii_dp<<<blocks_for(k.n, tdp), tdp, 0, c->stream>>>(k, L, gs, dp_args(c), none);
   timer_end(c);
   timer_begin(c, "iisph_iter");
-  k_ii_iter<<<nb, tit, 0, c->stream>>>(k, L, gs, c->key.p, c->cell_start.p, c->POS(), c->PHYS(),
-                                       c->dii.p, c->dijpj.p, c->mqa.p, c->mqb.p, c->gh.p,
-                                       c->rho_adv.p, c->press_l.p, c->press_l1.p, c->ps.p,
-                                       c->residue.p, c->red.p, none);
+  k_ii_iter<<<nb, tit, 0, c->stream>>>(k, L, gs, iter_args(c), c->red.p, none);
   timer_end(c);
   k_guard_pick<<<1, 1024, 0, c->stream>>>(gs, c->solve_state.p, c->solve_err.p, c->red.p, nb,
                                           k.n, ctl);
@@ -998,13 +1196,15 @@ static int enqueue_first_or_next(sphx_context* c, const KP& k, bool first) {
 static int read_state(sphx_context* c) {
   SPHX_HIP(hipMemcpyAsync(c->host_pinned, c->solve_err.p, 2 * sizeof(double),
                           hipMemcpyDeviceToHost, c->stream));
-  int st[2];
-  SPHX_HIP(hipMemcpyAsync(c->host_pinned + 4, c->solve_state.p, 2 * sizeof(int),
+  int st[4];
+  SPHX_HIP(hipMemcpyAsync(c->host_pinned + 4, c->solve_state.p, 4 * sizeof(int),
                           hipMemcpyDeviceToHost, c->stream));
   SPHX_HIP(hipStreamSynchronize(c->stream));
-  std::memcpy(st, c->host_pinned + 4, 2 * sizeof(int));
+  std::memcpy(st, c->host_pinned + 4, 4 * sizeof(int));
   c->host_pinned[2] = st[0];
-  c->host_pinned[3] = st[1];
+  // iterations of this solve plus those of a solve a device fallback redid
+  c->host_pinned[3] = st[1] + st[2];
+  c->host_pinned[6] = st[3];
   return 0;
 }
 
@@ -1153,6 +1353,105 @@ int sphx_iisph_pressure_solve(sphx_context* c, int* iters, double* err) {
 
 namespace sphx {
 
+// Workgroups of the persistent solve: as many as are co-resident on the
+// device (occupancy of k_ii_solve_coop times the CUs), at most one per
+// tile, a multiple of 8 for the XCD-aware tile walk.
+static int coop_grid(sphx_context* c, int ntiles, int* G) {
+  if (c->coop_max == 0) {
+    int per_cu = 0, cus = 0;
+    SPHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void*>(k_ii_solve_coop), NB_THREADS, 0));
+    SPHX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    SPHX_CHECK(per_cu > 0 && cus > 0, "persistent solve", "no co-resident workgroup fits");
+    c->coop_max = per_cu * cus;
+  }
+  int g = std::min(c->coop_max, ntiles);
+  if (g >= 8) g -= g % 8;
+  *G = std::max(g, 1);
+  return 0;
+}
+
+// One cooperative launch of the persistent solve (continuing the solve
+// state as it stands; a done solve returns at once).
+static int launch_coop(sphx_context* c, const KP& k) {
+  NbrList L;
+  SPHX_TRY(nbr_list(c, &L));
+  if (!c->gh_valid) {
+    k_ii_gh<<<blocks_for(k.n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
+        k, L, c->key.p, c->cell_start.p, c->POS(), c->PHYS(), c->gh.p);
+    SPHX_HIP(hipGetLastError());
+    c->gh_valid = true;
+  }
+  IterCtl ctl{c->O.min_iters, c->O.max_iters, c->O.tol};
+  SceneCtl sc = c->scenes();
+  int ntiles = blocks_for(k.n, NB_THREADS);
+  int G = 1;
+  SPHX_TRY(coop_grid(c, ntiles, &G));
+  KP kk = k;
+  DpArgs da = dp_args(c);
+  IterArgs ia = iter_args(c);
+  int* st = c->solve_state.p;
+  double* er = c->solve_err.p;
+  double* part = c->red.p;
+  void* args[] = {&kk, &L, &da, &ia, &st, &er, &part, &sc, &ctl, &ntiles};
+  timer_begin(c, "iisph_solve");
+  SPHX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_ii_solve_coop), dim3(G),
+                                      dim3(NB_THREADS), args, 0, c->stream));
+  timer_end(c);
+  return 0;
+}
+
+// The whole solve on the device, no host read: reset, persistent solve,
+// and when a learned start is in use the device-side fallback (restart of
+// what missed the tolerance from the plain start) and a second persistent
+// launch, which returns at once when nothing was restarted.
+int enqueue_solve_coop(sphx_context* c) {
+  SPHX_TRY(ensure_red(c));
+  KP k = make_kp(c);
+  if (k.n == 0) return 0;
+  c->mlp.guard_armed = false;   // the persistent solve carries no guard
+  SPHX_TRY(solve_reset(c));
+  SPHX_TRY(launch_coop(c, k));
+  if (c->mlp.ready && c->mlp.warm) {
+    k_fallback_prep<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+        k, c->solve_state.p, c->solve_err.p, c->scenes(), c->O.tol, c->POS(), c->key.p,
+        c->PHYS(), c->mlp.pprev.p, c->press_l.p, c->ps.p);
+    SPHX_HIP(hipGetLastError());
+    SPHX_TRY(launch_coop(c, k));
+  }
+  if (c->solve_log.p) {
+    k_solve_log<<<1, 64, 0, c->stream>>>(c->solve_state.p, c->solve_err.p, c->solve_log.p,
+                                         c->solve_log_count.p);
+    SPHX_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+int solve_log_setup(sphx_context* c) {
+  SPHX_TRY(c->solve_log.alloc(2 * SOLVE_LOG));
+  SPHX_TRY(c->solve_log_count.alloc(1));
+  SPHX_HIP(hipMemsetAsync(c->solve_log_count.p, 0, sizeof(int), c->stream));
+  return 0;
+}
+
+int solve_log_read(sphx_context* c, int* iters, double* err, int max, int* count) {
+  *count = 0;
+  if (!c->solve_log.p) return 0;
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  int n = 0;
+  SPHX_HIP(hipMemcpy(&n, c->solve_log_count.p, sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<double> r(2 * SOLVE_LOG);
+  SPHX_HIP(hipMemcpy(r.data(), c->solve_log.p, r.size() * sizeof(double), hipMemcpyDeviceToHost));
+  const int kept = std::min(n, SOLVE_LOG), take = std::min(kept, max);
+  for (int s = 0; s < take; ++s) {
+    const int slot = (n - take + s) % SOLVE_LOG;
+    if (iters) iters[s] = (int)r[2 * slot];
+    if (err) err[s] = r[2 * slot + 1];
+  }
+  *count = take;
+  return 0;
+}
+
 // PressureSolve (fluid_system_host.cu:525-552) from the start the a_ii pass
 // (or the learned start) left in press_l.
 int iisph_solve_plain(sphx_context* c, int* iters, double* err) {
@@ -1161,6 +1460,16 @@ int iisph_solve_plain(sphx_context* c, int* iters, double* err) {
   if (iters) *iters = 0;
   if (err) *err = 0.0;
   if (k.n == 0) return 0;
+  if (c->coop) {
+    SPHX_TRY(enqueue_solve_coop(c));
+    SPHX_TRY(read_state(c));
+    int it = (int)c->host_pinned[3];
+    c->last_iters = it;
+    c->coop_fallbacks = (int)c->host_pinned[6];
+    if (iters) *iters = it;
+    if (err) *err = c->host_pinned[0];
+    return timer_flush(c);
+  }
   SPHX_TRY(solve_reset(c));
   int enq = 0;
   int chunk = std::max(c->O.min_iters, 2);
@@ -1182,6 +1491,18 @@ int iisph_solve_plain(sphx_context* c, int* iters, double* err) {
   if (iters) *iters = it;
   if (err) *err = c->host_pinned[0];
   return timer_flush(c);
+}
+
+// The outcome of the solve on the stream: iterations (with those a device
+// fallback redid) and the error; synchronises.
+int read_solve(sphx_context* c, int* iters, double* err) {
+  SPHX_TRY(read_state(c));
+  const int it = (int)c->host_pinned[3];
+  c->last_iters = it;
+  c->coop_fallbacks = (int)c->host_pinned[6];
+  if (iters) *iters = it;
+  if (err) *err = c->host_pinned[0];
+  return 0;
 }
 
 // Capturable part of a solve: reset and n iterations, no host reads

@@ -822,6 +822,11 @@ int mlp_fallback(sphx_context* c, int* iters, double* err) {
   if (!m.ready || !m.warm || c->num == 0) return 0;
   ++m.learned_solves;
   m.last_fallback = 0;
+  if (c->coop_fallbacks) {   // the persistent solve already fell back on the device
+    m.fallbacks += c->coop_fallbacks;
+    m.last_fallback = 1;
+    c->coop_fallbacks = 0;
+  }
   // the done flag is also set at max_iters: judge by the error itself
   double e = *err;
   if (e == e && e <= (double)c->O.tol) return 0;
@@ -931,9 +936,11 @@ extern "C" {
 
 int sphx_mlp_nparam(void) { return MLP_NP; }
 
-int sphx_mlp_guard(sphx_context* c, int on) {
+int sphx_mlp_guard(sphx_context* c, float factor) {
   SPHX_CHECK(c, "sphx_mlp_guard", "null context");
-  c->mlp.guard = on != 0;
+  SPHX_CHECK(factor >= 0.f, "sphx_mlp_guard", "factor must be >= 0 (0: off)");
+  c->mlp.guard = factor > 0.f;
+  c->mlp.guard_factor = factor > 0.f ? factor : 1.f;
   return 0;
 }
 

@@ -615,8 +615,11 @@ class Context:
               "sphx_mlp_train")
         return out[:epochs]
 
-    def mlp_guard(self, on=True):
-        check(self.L.sphx_mlp_guard(self.h, 1 if on else 0), "sphx_mlp_guard")
+    def mlp_guard(self, factor=10.0):
+        """Probe the plain start when the learned start's first iteration
+        leaves an error above factor * tol (0: guard off)."""
+        self.L.sphx_mlp_guard.argtypes = [C.c_void_p, C.c_float]
+        check(self.L.sphx_mlp_guard(self.h, float(factor)), "sphx_mlp_guard")
 
     def mlp_guard_stats(self):
         a, b = C.c_int64(), C.c_int64()
@@ -641,6 +644,21 @@ class Context:
 
     def graph_build(self, budget):
         check(self.L.sphx_graph_build(self.h, budget), "sphx_graph_build")
+
+    def persistent_solve(self, on=True):
+        check(self.L.sphx_persistent_solve(self.h, 1 if on else 0), "sphx_persistent_solve")
+
+    def solve_history(self, max_solves=4096):
+        it = np.zeros(max_solves, np.int32)
+        err = np.zeros(max_solves, np.float64)
+        n = C.c_int()
+        check(self.L.sphx_solve_history(self.h, ptr(it), ptr(err), max_solves, C.byref(n)),
+              "sphx_solve_history")
+        return it[:n.value], err[:n.value]
+
+    def graph_step_async(self):
+        """One captured step with the persistent solve, no read-back."""
+        check(self.L.sphx_graph_step(self.h, None, None), "sphx_graph_step")
 
     def graph_step(self):
         it, err = C.c_int(), C.c_double()

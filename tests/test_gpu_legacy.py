@@ -7,7 +7,8 @@ called on a context: results must be bitwise equal.
 * Multiphase: the declared-only Mf* entry points around the IISPH solve.
 * MPM: MpmAllocateBufferCUDA, IndexMPMSortCUDA, then RunMPM
   (fluid_system.cpp:541-564) twice, then MpmGetMomentumCUDA,
-  MpmColorTestCUDA and ComputeMpmForce.
+  MpmColorTestCUDA and ComputeMpmForce (to 1e-5 normwise: the scatter's
+  sums are order-nondeterministic in their last bits).
 * InitializeSolid_CUDA (host.cu:492-501) = the three grid calls.
 """
 import ctypes as C
@@ -61,6 +62,14 @@ class Legacy:
 def assert_same(a, b, fields):
     for k in fields:
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def assert_close_all(a, b, fields, tol=1e-5):
+    for k in fields:
+        x = np.asarray(a[k], np.float64)
+        y = np.asarray(b[k], np.float64)
+        np.testing.assert_allclose(x, y, rtol=0, atol=tol * max(np.abs(y).max(), 1e-30),
+                                   err_msg=k)
 
 
 GRID = ("GetParticleIndexCUDA", "GetGridListCUDA", "RearrageDataCUDA")
@@ -179,10 +188,14 @@ def test_legacy_mpm_chain_matches_context(gpu):
         gv = np.zeros((N, 3), np.float32)
         gm = np.zeros(N, np.float32)
         sphx.check(leg.L.sphx_mpm_download_nodes(leg.h, sphx.ptr(gv), sphx.ptr(gm)), "nodes")
-    assert_same(gd, wd, ("pos", "color"))
-    assert_same(gc, wc, ("vel", "stress", "B", "dens"))
-    np.testing.assert_array_equal(gv, wv)
-    np.testing.assert_array_equal(gm, wmass)
+    # the particle-to-grid scatter ranks a cell's particles with LDS atomics,
+    # so its FP32 sums (and everything after them) agree between two runs
+    # to rounding, not bit for bit
+    assert_close_all(gd, wd, ("pos",))
+    assert_same(gd, wd, ("color",))
+    assert_close_all(gc, wc, ("vel", "stress", "B", "dens"))
+    np.testing.assert_allclose(gv, wv, rtol=0, atol=1e-5 * np.abs(wv).max())
+    np.testing.assert_allclose(gm, wmass, rtol=0, atol=1e-5 * np.abs(wmass).max())
     assert np.abs(wv).max() > 0
 
 

@@ -116,7 +116,9 @@ def test_walkthrough_background_from_node_grid(gpu):
         n = 200
         hi = (res[0] - 1) * p.mpmcellsize
         pos = rng.uniform(0.05 * hi, 0.95 * hi, (n, 3)).astype(np.float32)
-        f0 = rng.normal(0, 1, (n, 3)).astype(np.float32)
+        # forces small enough that the background is a visible part of U
+        # (the spheres are ~0.1 apart in units of their radius)
+        f0 = rng.normal(0, 1e-5, (n, 3)).astype(np.float32)
         sd = np.zeros(n, sphx.display_dtype)
         sd["pos"] = pos
         d_disp = sphx.DeviceArray(sd)
