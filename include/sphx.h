@@ -542,6 +542,16 @@ int sphx_rpy_timing_get(sphx_rpy* h, double* ms, int* count);
  * (Kern_getmobu_walkthrough :207-255; positions double-buffered, A10). */
 int sphx_rpy_walkthrough(sphx_context* nodes, const sphx_stokesianBufList* buf, int n,
                          int with_force);
+/* No counterpart in the reference: sphx_rpy_walkthrough for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results; the position double buffer is a static device array of
+ * SPHX_WALK_NOALLOC_MAX spheres (16 B each, 1 MiB) per GPU, held until the
+ * process ends, so it calls no hipMalloc/hipFree.  n above the maximum is
+ * an error.  Calls that overlap on one GPU share that array and race. */
+#define SPHX_WALK_NOALLOC_MAX 65536
+int sphx_rpy_walkthrough_noalloc(sphx_context* nodes, const sphx_stokesianBufList* buf, int n,
+                                 int with_force);
 /* MVmultiply / getMobU (:85-109): u = mat f, mat (6n x 6n) row-major,
  * device pointers. */
 int sphx_mob_gemv(const float* mat, const float* f, float* u, int nsize);

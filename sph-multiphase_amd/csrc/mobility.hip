@@ -19,6 +19,7 @@
 // :207-255) and the dense product getMobU (:85-109).
 #include "common.hpp"
 
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -410,6 +411,11 @@ __global__ void k_walk_zero(sphx_float3* __restrict__ u, int n) {
   if (i < n) u[i] = {0.f, 0.f, 0.f};
 }
 
+// Position double buffer of sphx_rpy_walkthrough_noalloc: one copy per GPU
+// the library runs on, held until the process ends.  Host code reaches it
+// only through the address hipGetSymbolAddress returns for that GPU.
+static __device__ __align__(256) float4 scratch[SPHX_WALK_NOALLOC_MAX];
+
 // MVmultiply (:85-98): one wave per row of the dense (6n x 6n) matrix.
 __global__ void k_gemv(const float* __restrict__ mat, const float* __restrict__ f,
                        float* __restrict__ u, int dim) {
@@ -623,6 +629,28 @@ int walk_run(sphx_context* nodes, const sphx_stokesianBufList* b, int n, int wit
   return 0;
 }
 
+// Device address of `scratch` on each GPU, looked up once per device with
+// that device current (each GPU holds its own copy of the array).
+constexpr int WALK_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+float4* g_scratch[WALK_MAX_DEVICES] = {};
+
+int walk_scratch(float4** out) {
+  int dev = 0;
+  SPHX_HIP(hipGetDevice(&dev));
+  SPHX_CHECK(dev >= 0 && dev < WALK_MAX_DEVICES, "sphx_rpy_walkthrough_noalloc",
+             "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    SPHX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SPHX_CHECK(p != nullptr, "sphx_rpy_walkthrough_noalloc", "no device address for the scratch");
+    g_scratch[dev] = static_cast<float4*>(p);
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -638,6 +666,21 @@ int sphx_rpy_walkthrough(sphx_context* nodes, const sphx_stokesianBufList* b, in
   DevBuf<float4> np;
   SPHX_TRY(np.alloc(n));
   return walk_run(nodes, b, n, with_force, np.p);
+}
+
+// Same step with the position double buffer in the library's static device
+// scratch: no allocation.  Every element the step reads of the scratch is
+// written by k_walk_mob of the same call first, so nothing is cleared.
+// Calls that overlap on one GPU share the scratch and race.
+int sphx_rpy_walkthrough_noalloc(sphx_context* nodes, const sphx_stokesianBufList* b, int n,
+                                 int with_force) {
+  SPHX_CHECK(b && n >= 0, "sphx_rpy_walkthrough_noalloc", "bad argument");
+  SPHX_CHECK(n <= SPHX_WALK_NOALLOC_MAX, "sphx_rpy_walkthrough_noalloc",
+             "more spheres than the static scratch holds (SPHX_WALK_NOALLOC_MAX)");
+  if (n == 0) return 0;
+  float4* np = nullptr;
+  SPHX_TRY(walk_scratch(&np));
+  return walk_run(nodes, b, n, with_force, np);
 }
 
 // MVmultiply / getMobU: u = mat f with mat (6n x 6n) row-major, device.
