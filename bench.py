@@ -151,11 +151,23 @@ def train_network(sphx, args):
     t.sync()
     train_s = time.perf_counter() - t0
     w = t.mlp_weights()
+    # held out: the next step's samples, which the fit has not seen
+    t.mlp_clear_samples()
+    t.mf_step()
+    total, _ = t.mlp_samples()
+    X, T, M = t.mlp_download_samples(0, total)
+    v = M > 0
+    Y = t.mlp_eval(X[v])
+    res = ((Y - T[v]) ** 2).sum(axis=0)
+    tot = ((T[v] - T[v].mean(axis=0)) ** 2).sum(axis=0) + 1e-30
+    r2 = (1.0 - res / tot).tolist()
     t.close()
     print(f"# training tank (seed {TRAIN_SEED}, p0 = p(t-1)): {args.record_steps} steps, "
           f"iterations {its}; {valid} samples; {args.epochs} epochs in {train_s:.2f} s, loss "
-          + " ".join(f"{x:.3e}" for x in loss), file=sys.stderr, flush=True)
-    return dict(w=w, P=P, U=U, lw=lw, samples=int(valid), train_s=train_s, epochs=args.epochs)
+          + " ".join(f"{x:.3e}" for x in loss) + "; held-out R2 (pressure increment, drift "
+          "correction x y z) " + " ".join(f"{x:.3f}" for x in r2), file=sys.stderr, flush=True)
+    return dict(w=w, P=P, U=U, lw=lw, samples=int(valid), train_s=train_s, epochs=args.epochs,
+                r2=r2)
 
 
 def learned_ctx(sphx, args, p, disp, calc, alpha, net):
@@ -233,16 +245,20 @@ def bench_bubbles(args, sphx):
     elapsed, iters, errs, kt = run["elapsed"], run["iters"], run["errs"], run["kt"]
     t_dom, n_dom = kt[dom]
     # Launches of the pressure passes queued past convergence exit on the
-    # device flag at once: only the iterations the solves ran move the
-    # algorithmic bytes, and the early exits' time is charged to them.
-    n_work = int(np.sum(iters)) + args.steps if dom == "iisph_dp" else int(np.sum(iters))
-    n_eff = n_work if dom in ("iisph_iter", "iisph_dp") else n_dom
-    avg_ms = t_dom / max(n_eff, 1)
-    algo = ALGO_BYTES.get(dom, 0) * nf
+    # device flag at once and move no bytes: the algorithmic bytes of the
+    # iterations the solves ran are spread over every launch, so the
+    # per-launch average and the launch time are both averages over all
+    # launches (as a kernel trace reports them).
+    n_iter = int(np.sum(iters))
+    if dom in ("iisph_iter", "iisph_dp"):
+        work_bytes = ALGO_BYTES[dom] * nf * n_iter   # guard probes count as iterations
+    else:
+        work_bytes = ALGO_BYTES.get(dom, 0) * nf * n_dom
+    avg_ms = t_dom / max(n_dom, 1)
+    algo = work_bytes / max(n_dom, 1)
     achieved = algo / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    print(f"# timed steps: {dom} {t_dom / max(n_dom, 1) * 1e3:.1f} us/launch x {n_dom} launches"
-          + (f"; {avg_ms * 1e3:.1f} us per working launch x {n_eff}" if n_eff != n_dom else ""),
-          file=sys.stderr)
+    print(f"# timed steps: {dom} {avg_ms * 1e3:.1f} us/launch x {n_dom} launches, "
+          f"{algo / 1e6:.1f} algorithmic MB per launch", file=sys.stderr)
     m0 = hist[0, 0]
     cons = {"mass_rel_change": float(np.abs(hist[:, 0] - m0).max() / m0),
             "momentum_y_first_last": [float(hist[0, 2]), float(hist[-1, 2])],
@@ -296,6 +312,8 @@ def bench_bubbles(args, sphx):
                         f"with p0 = p(t-1), {net['samples']} samples, {net['epochs']} epochs "
                         f"on the GPU in {net['train_s']:.2f} s (untimed)",
             "drift_closure": bool(args.drift),
+            "heldout_r2_pressure_increment": net["r2"][0],
+            "heldout_r2_drift_correction": net["r2"][1:],
             "online_refresh": (f"every {args.online_every} steps: {args.online_iters} Adam steps "
                                f"of {args.online_batch} samples (inside the timed steps"
                                + (", on a second stream, adopted two steps later)"
@@ -318,6 +336,7 @@ def bench_bubbles(args, sphx):
             "algorithmic_bytes_per_launch": algo,
             "avg_launch_ms": avg_ms,
             "launches": n_dom,
+            "solve_iterations": n_iter,
         },
         "cpu_baseline": cpu,
     }
@@ -765,17 +784,18 @@ def ensemble_ctx(sphx, S, p, disp, calc, alpha, warm_start=0.5):
     c.mf_upload(alpha)
     c.ensemble_setup(S)
     c.call("grid_build")
-    c.persistent_solve(True)
     return c
 
 
 def bench_ensemble(args, sphx):
     """Batched ensemble: 64 two-phase tanks with fluid counts log-uniform in
     [20k, 400k] (seeded), bubble layouts seeded per scene, all in one grid so
-    every stage is one launch.  A step is graph A (sort .. the learned start),
-    the persistent solve (one cooperative launch; with the learned start, the
-    device-side fallback and a second launch) and graph B (integration),
-    with nothing read back between steps.  Three warm starts from the same
+    every stage is one launch.  A step is graph A (sort .. the learned start
+    and a budget of launched pressure iterations), the persistent solve (one
+    cooperative launch that continues any scene still open and returns at
+    once when none is; with the learned start, the device-side fallback and
+    a second launch) and graph B (integration), with nothing read back
+    between steps.  Three warm starts from the same
     state: plain 0.5 p(t-1), persistence p(t-1) (reported), and the learned
     start, trained on a separate ensemble (other bubble seeds) advanced with
     p0 = p(t-1)."""
@@ -784,12 +804,14 @@ def bench_ensemble(args, sphx):
     sizes, p, disp, calc, alpha, info = ensemble_scene(sphx, S, 20261218)
     t = ensemble_ctx(sphx, S, p, disp, calc, alpha, warm_start=1.0)
     del disp, calc, alpha
-    t.persistent_solve(False)
     t.mf_step()
     g = float(np.linalg.norm(p.gravity.tolist()))
     t.mlp_init(20261202, 1000.0 * g * 0.8, 0.05)
-    rec = max(2, int(np.ceil(1e7 / info.n_fluid)))
-    t.mlp_reserve(rec * t.num)
+    # as many recorded steps as the single tank's training run, every 8th
+    # particle of each (about as many samples)
+    rec, stride = args.record_steps, 8
+    t.mlp_record_stride(stride)
+    t.mlp_reserve(rec * ((t.num + stride - 1) // stride))
     t.mlp_mode(record=True)
     for _ in range(rec):
         t.mf_step()
@@ -811,7 +833,7 @@ def bench_ensemble(args, sphx):
     print(f"# ensemble: {S} scenes, {nf} fluid + {info.n_boundary} boundary particles, grid "
           f"{p.gridres.x}x{p.gridres.y}x{p.gridres.z}", file=sys.stderr, flush=True)
 
-    def run(label, ws, learned, only=None):
+    def run(label, ws, learned):
         c = ensemble_ctx(sphx, S, p, disp, calc, alpha, warm_start=ws)
         if learned:
             c.mlp_init(20261202, 1000.0 * g * 0.8, 0.05)
@@ -820,20 +842,22 @@ def bench_ensemble(args, sphx):
         for _ in range(args.warmup):
             c.mf_step()
         c.persistent_solve(True)     # restarts the solve log
-        c.graph_build(2)
+        c.graph_build(args.graph_budget)
         c.sync()
-        c.timing_only(only)
-        c.timing(only is not None)
-        c.timing_reset()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             c.graph_step_async()
         c.sync()
         el = time.perf_counter() - t0
-        kt = c.timing_get(only) if only else (0.0, 0)
-        c.timing(False)
         its, errs = c.solve_history(args.steps)
         stats = c.mlp_stats() if learned else None
+        kt = None
+        if learned:
+            # the Jacobi pass timed with HIP events on eager steps of the same
+            # run (launches inside a graph carry no events)
+            c.persistent_solve(False)
+            _, eit, _, kt = timed_segment(c, 5, ["iisph_iter"], only="iisph_iter")
+            kt = dict(t=kt["iisph_iter"][0], n=kt["iisph_iter"][1], iters=int(np.sum(eit)))
         c.close()
         print(f"# {label}: {1e3 * el / args.steps:.2f} ms/step, most iterations of any scene "
               f"{its.tolist()} (mean {its.mean():.2f}), worst error {errs.max():.2e}",
@@ -842,16 +866,16 @@ def bench_ensemble(args, sphx):
 
     plain = run("plain (p0 = 0.5 p(t-1))", 0.5, False)
     pers = run("persistence (p0 = p(t-1))", 1.0, False)
-    dom = "iisph_solve"
-    learn = run("learned", 0.5, True, only=dom)
-    t_dom, n_dom = learn["kt"]
-    # the persistent solve moves the DP and Jacobi passes' bytes per
-    # iteration: (68 + 76) B per fluid particle (a scene that converged
-    # early stops moving them; the count is the most iterations of any scene)
-    it_total = float(np.sum(learn["its"]))
-    algo_per_launch = (ALGO_BYTES["iisph_dp"] + ALGO_BYTES["iisph_iter"]) * nf * it_total / max(n_dom, 1)
-    avg_ms = t_dom / max(n_dom, 1)
+    learn = run("learned", 0.5, True)
+    kt = learn["kt"]
+    # the Jacobi pass moves 76 B per fluid particle per iteration; launches
+    # queued past convergence exit at once and move none, so bytes and time
+    # are both averaged over all launches
+    avg_ms = kt["t"] / max(kt["n"], 1)
+    algo_per_launch = ALGO_BYTES["iisph_iter"] * nf * kt["iters"] / max(kt["n"], 1)
     achieved = algo_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    print(f"# eager steps of the learned run: iisph_iter {avg_ms * 1e3:.1f} us/launch x {kt['n']}",
+          file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -872,9 +896,10 @@ def bench_ensemble(args, sphx):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "64 bubble tanks in one grid, one launch per stage; step = graph A, "
-                               "persistent cooperative solve (+ device fallback), graph B; no "
-                               "host read-back between steps",
+        "config": {"workload": "64 bubble tanks in one grid, one launch per stage; step = graph A "
+                               f"(with {args.graph_budget} launched pressure iterations), the "
+                               "persistent cooperative solve continuing any scene still open "
+                               "(+ device fallback), graph B; no host read-back between steps",
                    "scenes": S, "n_fluid": nf, "n_boundary": int(info.n_boundary),
                    "pressure_iters_plain_mean": ip,
                    "pressure_iters_persistence_mean": ipers,
@@ -887,14 +912,14 @@ def bench_ensemble(args, sphx):
                    "max_density_error_plain": float(plain["errs"].max()),
                    "train_samples": int(valid), "train_seconds": train_s,
                    "online_refresh": None},
-        "roofline": {"bound": "hbm", "kernel": "k_ii_solve_coop (persistent pressure solve)",
+        "roofline": {"bound": "hbm", "kernel": "iisph_iter",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": algo_per_launch, "avg_launch_ms": avg_ms,
-                     "launches": n_dom,
-                     "note": "HIP events around the persistent solve launches of the timed "
-                             "steps (outside the graphs); bytes = (68 + 76) B x fluid particles "
-                             "x iterations"},
+                     "launches": kt["n"],
+                     "note": "HIP events around the Jacobi launches of 5 eager steps after the "
+                             "timed graph steps of the learned run; 76 B x fluid particles x "
+                             "iterations, averaged over launches"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -971,6 +996,9 @@ def main():
                     help="bubbles: two-phase tank (headline); sand: MPM 2M particles; "
                          "mobility: RPY 4096 spheres x 64 RHS")
     ap.add_argument("--rpy-split", type=int, default=0)
+    ap.add_argument("--graph-budget", type=int, default=12,
+                    help="ensemble: pressure iterations launched inside the step graph before "
+                         "the persistent solve takes over")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build measures "

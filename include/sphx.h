@@ -595,8 +595,8 @@ int sphx_mlp_online(sphx_context* ctx, int every, int iters, int batch, float lr
  * graph builds and graph steps first wait for a running refresh.  No
  * counterpart in the reference (the learned start is new). */
 int sphx_mlp_online_async(sphx_context* ctx, int on);
-/* Online recording keeps every `stride`-th particle slot of a solve (the
- * first slot rotating with the solve count; stride 1 = all); the ring then
+/* Recording keeps every `stride`-th particle slot of a solve (the first
+ * slot rotating with the solve count; stride 1 = all); the online ring then
  * holds as many solves as the sample store holds full solves.  No
  * counterpart in the reference. */
 int sphx_mlp_record_stride(sphx_context* ctx, int stride);

@@ -8,8 +8,11 @@
 // Between them the host reads the solver's done flag (8 bytes): a solve
 // still open after the budget continues eagerly, and a learned start that
 // missed the tolerance falls back to the plain start, so the plain solver
-// keeps judging every step.  Iterations past convergence exit on the device
-// flag.  Every launch covers all scenes of a batched ensemble at once.
+// keeps judging every step.  With the persistent solve on, that host check
+// is replaced by a cooperative launch that continues a solve still open
+// after the budget on the device (and returns at once when none is), so a
+// step needs no host read-back.  Iterations past convergence exit on the
+// device flag.  Every launch covers all scenes of a batched ensemble at once.
 // Nothing in the captured work allocates, synchronises or clears memory
 // with a memset node (the solver state is reset by a kernel).
 #include "common.hpp"
@@ -22,7 +25,7 @@ int enqueue_solve_fixed(sphx_context* c, int n);
 int iisph_density_dii(sphx_context* c);
 int mf_correction_force(sphx_context* c);
 int continue_solve(sphx_context* c, int* iters, double* err);
-int enqueue_solve_coop(sphx_context* c);
+int enqueue_solve_coop(sphx_context* c, bool reset);
 int read_solve(sphx_context* c, int* iters, double* err);
 int solve_log_setup(sphx_context* c);
 int solve_log_read(sphx_context* c, int* iters, double* err, int max, int* count);
@@ -57,8 +60,7 @@ static int stage_a(sphx_context* c, int budget) {
   SPHX_TRY(mf_correction_force(c));
   SPHX_TRY(sphx_iisph_predict_aii(c));
   SPHX_TRY(mlp_begin(c));
-  // the persistent solve is launched between the graphs
-  return c->coop ? 0 : enqueue_solve_fixed(c, budget);
+  return enqueue_solve_fixed(c, budget);
 }
 
 static int stage_b(sphx_context* c) {
@@ -139,9 +141,9 @@ int sphx_graph_step(sphx_context* c, int* iters, double* err) {
   int it = 0;
   double e = 0.0;
   if (c->coop) {
-    // the whole solve and the learned start's fallback on the device; the
-    // host reads the outcome only when asked for it
-    SPHX_TRY(enqueue_solve_coop(c));
+    // the rest of the solve and the learned start's fallback on the device;
+    // the host reads the outcome only when asked for it
+    SPHX_TRY(enqueue_solve_coop(c, false));
     if (iters || err) {
       SPHX_TRY(read_solve(c, &it, &e));
       SPHX_TRY(mlp_fallback(c, &it, &e));
@@ -161,6 +163,8 @@ int sphx_graph_step(sphx_context* c, int* iters, double* err) {
 // sphx_graph_build).
 int sphx_persistent_solve(sphx_context* c, int on) {
   SPHX_CHECK(c, "sphx_persistent_solve", "null context");
+  SPHX_CHECK(!on || c->mlp.online_every == 0, "sphx_persistent_solve",
+             "not available together with the learned start's online refresh");
   SPHX_TRY(ensure_device(c));
   if (c->coop != (on != 0)) graph_release(c);
   c->coop = on != 0;

@@ -297,6 +297,37 @@ __global__ void k_ii_gh(KP k, NbrList L, const uint32_t* __restrict__ key, const
   gh[i] = make_float4(Gx, Gy, Gz, H);
 }
 
+// Kernel gradient of the pressure-solve passes, branch-free, in the units
+// of the stored positions: with x_u = x_i - x_j before the simscale s, the
+// reference's grad W_ij = x_u s g(q) kspline / (h r), r = s |x_u|, q = r / h
+// (kern.cu:637-671) is x_u g(q) kspline / (h |x_u|): s cancels except in q.
+// g is -3q + 2.25q^2 (q <= 1) or -0.75(2 - q)^2, with kspline / h folded
+// into the coefficients; the support test 0 < |x_u|^2 < lim / s^2 is one
+// unsigned compare of the bit patterns (|x_u|^2 >= 0; NaN fails it).  Agrees
+// with cubic_gradf to rounding.
+struct SolveGrad {
+  float q_of_r;       // s / h
+  float co, ca, cb;   // -0.75, -3, 2.25 times kspline / h
+  uint32_t lim_m1;    // bits of lim / s^2, minus 1
+};
+
+__device__ __forceinline__ SolveGrad solve_grad(const KP& k) {
+  const float ks = k.kspline * k.inv_h;
+  const float lim = (k.full_support ? 4.f * k.h2 : k.h2) / (k.ss * k.ss);
+  return SolveGrad{k.ss * k.inv_h, -0.75f * ks, -3.f * ks, 2.25f * ks, __float_as_uint(lim) - 1u};
+}
+
+// f with grad W = x_u f; 0 outside the support (dsq_u = +inf: an unused slot).
+__device__ __forceinline__ float solve_gradf(const SolveGrad& s, float dsq_u) {
+  const float ir = __builtin_amdgcn_rsqf(dsq_u);
+  const float q = dsq_u * ir * s.q_of_r;
+  const float t = 2.f - q;
+  const float outer = s.co * (t * t);
+  const float inner = q * fmaf(s.cb, q, s.ca);
+  const float g = (q > 1.f ? outer : inner) * ir;
+  return (__float_as_uint(dsq_u) - 1u) < s.lim_m1 ? g : 0.f;
+}
+
 // Pressure_DP (kern.cu:910-958) plus q_i = d_ii p_i + dijpj_i, stored as
 // the record the Jacobi pass gathers ({x, m q.x}, {m q.y, m q.z}), and the
 // pressure acceleration q_i / dt^2 (kern.cu:1044-1047).
@@ -326,14 +357,13 @@ __device__ __forceinline__ void dp_particle(const KP& k, const NbrList& L, const
   // one 16-byte gather per neighbour: {x_j, s_j}; boundary s_j = 0 adds
   // nothing, as the reference's skip of boundary j (:925-926)
   const float4* __restrict__ ps = a.ps;
-  for_each_listed_b<NBR_BATCH_SMALL, false>(k, L, a.cs, gc, i, [&](int j) { return ps[j]; },
-                                     [&](int, const float4& pj) {
-    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
-    float dsq = dx * dx + dy * dy + dz * dz;
-    if (!in_support(k, dsq)) return;
-    float r, ir;
-    dist_rinv(dsq, r, ir);
-    float gf = cubic_gradf(r, ir, k.inv_h, k.kspline) * pj.w;
+  const SolveGrad sg = solve_grad(k);
+  for_each_listed_m<NBR_BATCH_SMALL, false>(
+      k, L, a.cs, gc, i, [&](uint32_t j) { return load_at(ps, j * 16u); },
+      [&](uint32_t, const float4& pj, bool valid) {
+    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const float dsq = valid ? dx * dx + dy * dy + dz * dz : INFINITY;
+    const float gf = solve_gradf(sg, dsq) * pj.w;
     sx += dx * gf; sy += dy * gf; sz += dz * gf;
   });
   float f = -k.dt * k.dt;
@@ -422,18 +452,17 @@ __device__ __forceinline__ void iter_particle(const KP& k, const NbrList& L, con
   // neighbour, {x_j, m_j q_j.x} and {m_j q_j.y, m_j q_j.z} (zero q for
   // boundary j).
   float acc = 0.f;
+  const SolveGrad sg = solve_grad(k);
   struct Rec { float4 a; float2 b; };
   const float4* __restrict__ mqa = a.mqa;
   const float2* __restrict__ mqb = a.mqb;
-  for_each_listed_b(k, L, a.cs, gc, i, [&](int j) { return Rec{mqa[j], mqb[j]}; },
-                    [&](int, const Rec& o) {
-    float dx = (pi.x - o.a.x) * k.ss, dy = (pi.y - o.a.y) * k.ss, dz = (pi.z - o.a.z) * k.ss;
-    float dsq = dx * dx + dy * dy + dz * dz;
-    if (!in_support(k, dsq)) return;
-    float r, ir;
-    dist_rinv(dsq, r, ir);
-    float gf = cubic_gradf(r, ir, k.inv_h, k.kspline);
-    acc += gf * (dx * o.a.w + dy * o.b.x + dz * o.b.y);
+  for_each_listed_m<NBR_BATCH, true>(
+      k, L, a.cs, gc, i,
+      [&](uint32_t j) { return Rec{load_at(mqa, j * 16u), load_at(mqb, j * 8u)}; },
+      [&](uint32_t, const Rec& o, bool valid) {
+    const float dx = pi.x - o.a.x, dy = pi.y - o.a.y, dz = pi.z - o.a.z;
+    const float dsq = valid ? dx * dx + dy * dy + dz * dz : INFINITY;
+    acc += solve_gradf(sg, dsq) * (dx * o.a.w + dy * o.b.x + dz * o.b.y);
   });
   const float4 G = a.gh[i];
   const float S = dpi.x * G.x + dpi.y * G.y + dpi.z * G.z + cji * G.w - acc;
@@ -479,7 +508,7 @@ __global__ __launch_bounds__(NB_THREADS) void k_ii_iter(KP k, NbrList L,
                                                         double* __restrict__ partial,
                                                         SceneCtl sc) {
   if (*done) return;
-  __shared__ double sacc[SPHX_MAX_SCENES * 3];
+  extern __shared__ double sacc[];   // 3 * sc.n doubles, ensembles only
   if (sc.n)
     for (int t = threadIdx.x; t < 3 * sc.n; t += blockDim.x) sacc[t] = 0.0;
   double rs, ra, rc;
@@ -643,7 +672,7 @@ __global__ __launch_bounds__(NB_THREADS) void k_ii_solve_coop(KP k, NbrList L, D
   // workgroup leaves before the first barrier
   if (state[0]) return;
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
-  __shared__ double sacc[SPHX_MAX_SCENES * 3];
+  extern __shared__ double sacc[];   // 3 * sc.n doubles, ensembles only
   int first, count, lane0, step;
   coop_tiles(ntiles, first, count, lane0, step);
   for (int it = 0; it < ctl.max_iters; ++it) {
@@ -994,12 +1023,17 @@ __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
   // set bits are then appended in order: the append runs once per hit of
   // the lane with the most hits, not once per candidate.
   auto scan_rows = [&](auto&& row_base, auto&& fetch) {
-    int sd = 0;
+    int sd = 0, s0 = 0;
 #pragma unroll 1
     for (int r = 0; r < 9; ++r) {
       const int c0 = (int)gc + k.rowoff[r];
       const int jb = cs[c0], je = cs[c0 + 3];
-      if (r % 3 == 0) sd = jb;
+      if (r % 3 == 0) {
+        // the decoder adds S_d - S_0 with a 24-bit multiply
+        if (r == 0) s0 = jb;
+        else ovf |= (uint32_t)(jb - s0) >= (1u << 24);
+        sd = jb;
+      }
       const uint32_t dtag = (uint32_t)(r / 3) << NBR_TBITS;
       const int base = row_base(r);
 #pragma unroll 1
@@ -1100,6 +1134,9 @@ int nbr_list(sphx_context* c, NbrList* out) {
   return 0;
 }
 
+// Dynamic LDS of the Jacobi pass: the per-scene sums of an ensemble.
+static size_t scene_lds(const SceneCtl& sc) { return (size_t)3 * sc.n * sizeof(double); }
+
 static DpArgs dp_args(sphx_context* c) {
   return DpArgs{c->key.p, c->cell_start.p, c->POS(), c->ps.p, c->PHYS(), c->dii.p, c->press_l.p,
                 c->dijpj.p, c->mqa.p, c->mqb.p, c->ACCEL()};
@@ -1141,7 +1178,8 @@ static int enqueue_iteration(sphx_context* c, const KP& k, int respect_done) {
   k_ii_dp<<<blocks_for(k.n, tdp), tdp, 0, c->stream>>>(k, L, c->solve_state.p, dp_args(c), sc);
   timer_end(c);
   timer_begin(c, "iisph_iter");
-  k_ii_iter<<<nb, tit, 0, c->stream>>>(k, L, c->solve_state.p, iter_args(c), c->red.p, sc);
+  k_ii_iter<<<nb, tit, scene_lds(sc), c->stream>>>(k, L, c->solve_state.p, iter_args(c),
+                                                   c->red.p, sc);
   timer_end(c);
   if (sc.n)
     k_ii_finalize_scenes<<<1, 256, 0, c->stream>>>(c->solve_state.p, c->solve_err.p, sc, ctl,
@@ -1360,7 +1398,8 @@ static int coop_grid(sphx_context* c, int ntiles, int* G) {
   if (c->coop_max == 0) {
     int per_cu = 0, cus = 0;
     SPHX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu, reinterpret_cast<const void*>(k_ii_solve_coop), NB_THREADS, 0));
+        &per_cu, reinterpret_cast<const void*>(k_ii_solve_coop), NB_THREADS,
+        (size_t)3 * SPHX_MAX_SCENES * sizeof(double)));
     SPHX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
     SPHX_CHECK(per_cu > 0 && cus > 0, "persistent solve", "no co-resident workgroup fits");
     c->coop_max = per_cu * cus;
@@ -1396,21 +1435,25 @@ static int launch_coop(sphx_context* c, const KP& k) {
   void* args[] = {&kk, &L, &da, &ia, &st, &er, &part, &sc, &ctl, &ntiles};
   timer_begin(c, "iisph_solve");
   SPHX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_ii_solve_coop), dim3(G),
-                                      dim3(NB_THREADS), args, 0, c->stream));
+                                      dim3(NB_THREADS), args, scene_lds(sc), c->stream));
   timer_end(c);
   return 0;
 }
 
-// The whole solve on the device, no host read: reset, persistent solve,
-// and when a learned start is in use the device-side fallback (restart of
-// what missed the tolerance from the plain start) and a second persistent
+// The solve on the device, no host read: with `reset` the whole solve
+// (reset, persistent solve), else the continuation of a solve whose first
+// iterations were launched one by one (a done solve returns at once); then,
+// when a learned start is in use, the device-side fallback (restart of what
+// missed the tolerance from the plain start) and a second persistent
 // launch, which returns at once when nothing was restarted.
-int enqueue_solve_coop(sphx_context* c) {
+int enqueue_solve_coop(sphx_context* c, bool reset) {
   SPHX_TRY(ensure_red(c));
   KP k = make_kp(c);
   if (k.n == 0) return 0;
-  c->mlp.guard_armed = false;   // the persistent solve carries no guard
-  SPHX_TRY(solve_reset(c));
+  if (reset) {
+    c->mlp.guard_armed = false;   // a whole persistent solve carries no guard
+    SPHX_TRY(solve_reset(c));
+  }
   SPHX_TRY(launch_coop(c, k));
   if (c->mlp.ready && c->mlp.warm) {
     k_fallback_prep<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
@@ -1461,7 +1504,7 @@ int iisph_solve_plain(sphx_context* c, int* iters, double* err) {
   if (err) *err = 0.0;
   if (k.n == 0) return 0;
   if (c->coop) {
-    SPHX_TRY(enqueue_solve_coop(c));
+    SPHX_TRY(enqueue_solve_coop(c, true));
     SPHX_TRY(read_state(c));
     int it = (int)c->host_pinned[3];
     c->last_iters = it;

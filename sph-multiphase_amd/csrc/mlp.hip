@@ -858,15 +858,17 @@ int mlp_after(sphx_context* c, int record_ok) {
   k_mlp_hist<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(k, c->POS(), c->key.p, c->PHYS(),
                                                           m.pprev.p, hist);
   SPHX_HIP(hipGetLastError());
-  // Online recording with a stride keeps every stride-th slot of a solve
-  // (the first slot rotating with the solve count) in a ring of as many
+  // Recording with a stride keeps every stride-th slot of a solve (the
+  // first slot rotating with the solve count); online, in a ring of as many
   // solves as the store holds full solves.
-  const int stride = (online && !m.record) ? m.record_stride : 1;
+  const int stride = m.record_stride;
   const int phase = stride > 1 ? (int)(m.solves % stride) : 0;
   const size_t nrec = stride > 1 ? (n - (size_t)phase + stride - 1) / stride : n;
-  const size_t cap = stride > 1 ? std::min(m.max_samples, (m.max_samples / n) * ((n + stride - 1) / stride))
-                                : m.max_samples;
-  if (record && m.max_samples >= n) {
+  const bool ring = online && !m.record;
+  const size_t cap = (ring && stride > 1)
+                         ? std::min(m.max_samples, (m.max_samples / n) * ((n + stride - 1) / stride))
+                         : m.max_samples;
+  if (record && (ring ? m.max_samples >= n : m.max_samples >= nrec)) {
     size_t base = m.wpos;
     if (base + nrec > cap) base = online ? 0 : cap;   // ring or full
     if (base + nrec <= cap) {
@@ -1038,6 +1040,8 @@ int sphx_mlp_online(sphx_context* c, int every, int iters, int batch, float lr, 
              "batch * iters too large");
   SPHX_CHECK(every == 0 || c->mlp.max_samples >= (size_t)c->num, "sphx_mlp_online",
              "reserve room for at least one step of samples first");
+  SPHX_CHECK(every == 0 || !c->coop, "sphx_mlp_online",
+             "not available together with the persistent solve");
   auto& m = c->mlp;
   m.online_every = every;
   m.online_iters = iters;
@@ -1339,6 +1343,8 @@ int mlp_state_set(sphx_context* c, const char* p, size_t bytes) {
   const size_t wb = (size_t)MLP_NP * sizeof(float), hb = (size_t)c->num * sizeof(float4);
   SPHX_CHECK(h.np == MLP_NP && h.n == c->num && bytes == sizeof(h) + 3 * wb + hb,
              "sphx_checkpoint_restore", "network section does not match this build or the particles");
+  SPHX_CHECK(h.online_every == 0 || !c->coop, "sphx_checkpoint_restore",
+             "the network's online refresh is not available together with the persistent solve");
   SPHX_TRY(sphx_mlp_init(c, 1, h.p_scale, h.u_scale));
   auto& m = c->mlp;
   SPHX_HIP(hipMemcpy(m.w.p, p + sizeof(h), wb, hipMemcpyHostToDevice));

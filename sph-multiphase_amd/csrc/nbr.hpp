@@ -37,9 +37,10 @@ __device__ __forceinline__ void for_each_neighbor(const KP& k, const int* __rest
 // widest support any kernel of the step uses).  Positions do not move
 // between the density pass and the last pressure iteration, so one list
 // serves every neighbour loop of an IISPH step.  A particle with more than
-// `cap` neighbours, or one whose offset j - S_d does not fit 14 bits, has
-// count cap + 1 and its loops fall back to the cell scan, which visits a
-// superset in the same order.
+// `cap` neighbours, one whose offset j - S_d does not fit 14 bits, or one
+// whose layer starts are 2^24 or more particles apart has count cap + 1
+// and its loops fall back to the cell scan, which visits a superset in the
+// same order.
 constexpr int NBR_SLICE = 256;
 constexpr int NBR_GROUP = 8;                 // entries per 16-byte group
 constexpr int NBR_TBITS = 14;                // offset bits of an entry
@@ -55,18 +56,25 @@ struct NbrList {
   int cap;                         // entries per particle, a multiple of NBR_GROUP
 };
 
-// Window starts S_0, S_1, S_2 of the three cell layers around `cell`.
-__device__ __forceinline__ void nbr_bases(const KP& k, const int* __restrict__ cs, uint32_t cell,
-                                          int& s0, int& s1, int& s2) {
-  s0 = cs[(int)cell + k.rowoff[0]];
-  s1 = cs[(int)cell + k.rowoff[3]];
-  s2 = cs[(int)cell + k.rowoff[6]];
+// Window starts of the three cell layers around `cell`: S_0 and the
+// offsets S_1 - S_0, S_2 - S_0 (the list builder falls back to the cell
+// scan for a particle whose offsets do not fit 24 bits).
+struct NbrBase {
+  int s0;
+  uint32_t d1, d2;
+};
+__device__ __forceinline__ NbrBase nbr_bases(const KP& k, const int* __restrict__ cs,
+                                             uint32_t cell) {
+  const int s0 = cs[(int)cell + k.rowoff[0]];
+  return NbrBase{s0, (uint32_t)(cs[(int)cell + k.rowoff[3]] - s0),
+                 (uint32_t)(cs[(int)cell + k.rowoff[6]] - s0)};
 }
 
-__device__ __forceinline__ int nbr_decode(uint32_t e, int s0, int s1, int s2) {
-  const uint32_t d = e >> NBR_TBITS;
-  const int b = d == 0 ? s0 : (d == 1 ? s1 : s2);
-  return b + (int)(e & NBR_TMAX);
+// j = S_d + t for the 16-bit entry e = d << 14 | t, d in {0, 1, 2}: the
+// layer bits select the offsets by 24-bit multiply-adds instead of compares.
+__device__ __forceinline__ int nbr_decode(uint32_t e, const NbrBase& b) {
+  const uint32_t t = (uint32_t)b.s0 + (e & NBR_TMAX);
+  return (int)(t + __umul24((e >> NBR_TBITS) & 1u, b.d1) + __umul24(e >> (NBR_TBITS + 1), b.d2));
 }
 
 // Batched traversal of the list: B entries at a time, and the records of
@@ -90,8 +98,7 @@ __device__ __forceinline__ void for_each_listed_b(const KP& k, const NbrList& L,
   const int c = L.cnt[i];
   if (c <= L.cap) {
     if (c <= 0) return;
-    int s0, s1, s2;
-    nbr_bases(k, cell_start, cell, s0, s1, s2);
+    const NbrBase nb = nbr_bases(k, cell_start, cell);
     const uint4* __restrict__ p = L.idx + nbr_gslot(i, 0, L.cap / NBR_GROUP);
     const int glast = (c - 1) / NBR_GROUP;
     uint4 g[G];
@@ -111,8 +118,8 @@ __device__ __forceinline__ void for_each_listed_b(const KP& k, const NbrList& L,
         const uint32_t w[4] = {g[u].x, g[u].y, g[u].z, g[u].w};
 #pragma unroll
         for (int h = 0; h < 4; ++h) {
-          j[u * NBR_GROUP + 2 * h] = nbr_decode(w[h] & 0xFFFFu, s0, s1, s2);
-          j[u * NBR_GROUP + 2 * h + 1] = nbr_decode(w[h] >> 16, s0, s1, s2);
+          j[u * NBR_GROUP + 2 * h] = nbr_decode(w[h] & 0xFFFFu, nb);
+          j[u * NBR_GROUP + 2 * h + 1] = nbr_decode(w[h] >> 16, nb);
         }
       }
       if (PREFETCH) {
@@ -130,6 +137,73 @@ __device__ __forceinline__ void for_each_listed_b(const KP& k, const NbrList& L,
   } else {
     for_each_neighbor(k, cell_start, cell, [&](int j) {
       if (j != i) use(j, load(j));
+    });
+  }
+}
+
+// A record of `base` at a 32-bit byte offset: the address is the base (a
+// kernel argument, in SGPRs) plus the zero-extended offset, which the
+// global_load saddr form adds in the address unit, so a gather costs one
+// shift per lane instead of 64-bit address arithmetic.  Every array the
+// solve passes gather is below 4 GiB.
+template <class T>
+__device__ __forceinline__ T load_at(const T* __restrict__ base, uint32_t byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+// Branch-free traversal for the pressure-solve passes: as
+// for_each_listed_b, but use(j, rec, valid) is called for every slot of
+// every batch, with valid false for the slots past the particle's count,
+// and the loop body has no per-neighbour branch, so the compiler
+// interleaves a batch's arithmetic.  use must add nothing for an invalid
+// slot (its record is a real particle's, at S_0).  j is the sorted index.
+template <int B, bool PREFETCH, class Load, class Use>
+__device__ __forceinline__ void for_each_listed_m(const KP& k, const NbrList& L,
+                                                  const int* __restrict__ cell_start,
+                                                  uint32_t cell, int i, Load&& load, Use&& use) {
+  static_assert(B % NBR_GROUP == 0, "batch must be whole groups");
+  constexpr int G = B / NBR_GROUP;
+  const int c = L.cnt[i];
+  if (c <= L.cap) {
+    if (c <= 0) return;
+    const NbrBase nb = nbr_bases(k, cell_start, cell);
+    const uint4* __restrict__ p = L.idx + nbr_gslot(i, 0, L.cap / NBR_GROUP);
+    const int glast = (c - 1) / NBR_GROUP;
+    using Rec = decltype(load(0u));
+    uint4 g[G];
+    if (PREFETCH) {
+#pragma unroll
+      for (int u = 0; u < G; ++u) g[u] = p[(size_t)min(u, glast) * NBR_SLICE];
+    }
+    for (int q0 = 0; q0 < c; q0 += B) {
+      if (!PREFETCH) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) g[u] = p[(size_t)min(q0 / NBR_GROUP + u, glast) * NBR_SLICE];
+      }
+      uint32_t j[B];
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        const uint32_t w[4] = {g[u].x, g[u].y, g[u].z, g[u].w};
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+          j[u * NBR_GROUP + 2 * h] = (uint32_t)nbr_decode(w[h] & 0xFFFFu, nb);
+          j[u * NBR_GROUP + 2 * h + 1] = (uint32_t)nbr_decode(w[h] >> 16, nb);
+        }
+      }
+      if (PREFETCH) {
+        const int gn = q0 / NBR_GROUP + G;
+#pragma unroll
+        for (int u = 0; u < G; ++u) g[u] = p[(size_t)min(gn + u, glast) * NBR_SLICE];
+      }
+      Rec v[B];
+#pragma unroll
+      for (int b = 0; b < B; ++b) v[b] = load(j[b]);
+#pragma unroll
+      for (int b = 0; b < B; ++b) use(j[b], v[b], q0 + b < c);
+    }
+  } else {
+    for_each_neighbor(k, cell_start, cell, [&](int j) {
+      if (j != i) use((uint32_t)j, load((uint32_t)j), true);
     });
   }
 }

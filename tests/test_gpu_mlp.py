@@ -159,53 +159,62 @@ def r2(Y, T, M):
 
 
 def test_learned_warm_start_cuts_iterations(gpu):
-    """Record the plain solver's converged pressures, train on the GPU, and
-    run the same column on from identical states with the learned and with
-    the plain (reference 0.5 p(t-1)) warm start."""
-    p, disp, calc, alpha, o = column()
-    g = float(np.linalg.norm(p.gravity.tolist()))
-    P = 1000.0 * g * 0.8
+    """The benchmark's protocol on a deep column (the 1M tank's depth): the
+    network is trained on the GPU on a separate column (other bubble layout)
+    advanced with p0 = p(t-1); then, from the same state at t = 0 of the
+    test column, the plain start (the reference's p0 = 0.5 p(t-1)) and the
+    learned start (guard factor 10) run the same steps.  Every solve meets
+    the tolerance, and the learned start needs at least 2x fewer iterations
+    over the run."""
+    rec, steps = 20, 40
+    p_t, disp_t, calc_t, alpha_t, o_t = column(seed=6)
+    o_t.warm_start = 1.0
+    P, _ = scales(p_t, 0.8)
+    with sphx.Context(p_t, o_t) as T:
+        T.upload(disp_t, calc_t)
+        T.mf_setup(RHO, NU)
+        T.mf_upload(alpha_t)
+        T.call("grid_build")
+        T.mf_step()
+        U = drift_scale(T)
+        T.mlp_init(7, P, U)
+        T.mlp_reserve(rec * T.num)
+        T.mlp_mode(record=True)
+        it_t, _ = run_steps(T, rec)
+        total, valid = T.mlp_samples()
+        X, Tg, M = T.mlp_download_samples(0, total)
+        assert valid >= 0.99 * rec * int((disp_t["type"] == 0).sum())
+        assert np.all(np.isfinite(X)) and np.all(np.isfinite(Tg))
+        lw = loss_weights(Tg, M)
+        v = M > 0
+        loss0 = float(np.mean(np.sum(lw * Tg[v].astype(np.float64) ** 2, axis=1)))
+        loss = T.mlp_train(epochs=20, batch=2048, lr=2e-3, lw=lw, seed=3)
+        print(f"training column iterations {it_t.tolist()}; train loss (zero predictor "
+              f"{loss0:.3e}): " + " ".join(f"{x:.3e}" for x in loss), flush=True)
+        print("in-sample R2 per output", r2(T.mlp_eval(X[v]), Tg[v], M[v]).round(3), flush=True)
+        assert loss[-1] < 0.5 * loss0
+        w = T.mlp_weights()
+    p, disp, calc, alpha, o = column(seed=5)
     tol = o.tol
-    rec = 30
     with sphx.Context(p, o) as A, sphx.Context(p, o) as B:
         for c in (A, B):
             c.upload(disp, calc)
             c.mf_setup(RHO, NU)
             c.mf_upload(alpha)
             c.call("grid_build")
-        run_steps(A, 10)
-        run_steps(B, 10)
-        U = drift_scale(A)
         A.mlp_init(7, P, U)
-        A.mlp_reserve(rec * A.num)
-        A.mlp_mode(record=True)
-        ir, _ = run_steps(A, rec)
-        run_steps(B, rec)
-        total, valid = A.mlp_samples()
-        print(f"samples {total} valid {valid}; plain iterations while recording {ir.mean():.2f}",
-              flush=True)
-        assert valid >= 0.99 * rec * disp["type"].tolist().count(0)
-        X, T, M = A.mlp_download_samples(0, total)
-        assert np.all(np.isfinite(X)) and np.all(np.isfinite(T))
-        lw = loss_weights(T, M)
-        v = M > 0
-        loss0 = float(np.mean(np.sum(lw * T[v].astype(np.float64) ** 2, axis=1)))
-        print("target mean", T[v].mean(axis=0), "std", T[v].std(axis=0), flush=True)
-        loss = A.mlp_train(epochs=20, batch=2048, lr=2e-3, lw=lw, seed=3)
-        print(f"train loss (zero predictor {loss0:.3e}):", " ".join(f"{x:.3e}" for x in loss),
-              flush=True)
-        Y = A.mlp_eval(X[v])
-        print("in-sample R2 per output", r2(Y, T[v], M[v]).round(3), flush=True)
-        assert loss[-1] < 0.5 * loss0
+        A.mlp_set_weights(w)
         A.mlp_mode(warm=True)
-        ia, ea = run_steps(A, 15)
-        ib, eb = run_steps(B, 15)
+        A.mlp_guard(10.0)
+        ia, ea = run_steps(A, steps)
+        ib, eb = run_steps(B, steps)
         st = A.mlp_stats()
-        print(f"iterations learned {ia.mean():.2f} {ia.tolist()} fallbacks {st['fallbacks']}")
+        print(f"iterations learned {ia.mean():.2f} {ia.tolist()} fallbacks {st['fallbacks']} "
+              f"guard {A.mlp_guard_stats()}")
         print(f"iterations plain   {ib.mean():.2f} {ib.tolist()}")
         assert np.all(ea <= tol) and np.all(eb <= tol)
-        assert st["learned"] == 15
-        assert ia.mean() < ib.mean()
+        assert st["learned"] == steps
+        assert ib.mean() >= 2.0 * ia.mean()
 
 
 def test_online_refresh_async_matches_sync(gpu):

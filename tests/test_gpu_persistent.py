@@ -91,3 +91,22 @@ def test_persistent_ensemble_graph_steps(gpu, learned):
         same_state(A, B)
         if learned == "fallback":
             assert B.mlp_stats()["fallbacks"] >= 1
+
+
+def test_persistent_solve_and_online_refresh_exclude_each_other(gpu):
+    """The persistent solve is refused while the learned start's online
+    refresh is on, and the online refresh while the persistent solve is on
+    (in either order the context stays as it was)."""
+    p, disp, calc, alpha, o = setup()
+    with single(p, disp, calc, alpha, o) as c:
+        c.mlp_init(1, 7848.0, 1e-3)
+        c.mlp_reserve(2 * c.num)
+        c.mlp_online(5, 2, 1024, 1e-3, (1.0, 0.0, 0.0, 0.0))
+        with pytest.raises(sphx.SphxError, match="online refresh"):
+            c.persistent_solve(True)
+        c.mlp_online(0, 0, 1024, 1e-3, (1.0, 0.0, 0.0, 0.0))
+        c.persistent_solve(True)
+        with pytest.raises(sphx.SphxError, match="persistent solve"):
+            c.mlp_online(5, 2, 1024, 1e-3, (1.0, 0.0, 0.0, 0.0))
+        c.persistent_solve(False)
+        c.mlp_online(5, 2, 1024, 1e-3, (1.0, 0.0, 0.0, 0.0))

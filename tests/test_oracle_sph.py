@@ -370,3 +370,143 @@ def test_boundary_pressure_and_friction_closed_form(family):
     a4, _, _, _ = boundary_pair(family, 1.01 * (0.02 if family == 0 else 0.04), direction,
                                 vi=[0.3, 0.1, -0.05], vj=[0, 0, 0])
     assert np.all(a4 == 0)
+
+
+# ------------------------------------------- IISPH operator (brute force) --
+# IISPH (Ihmsen et al. 2014) in its textbook form, all pairs in FP64, no
+# grid: with W the cubic spline of support 2h,
+#   rho_adv_i = rho_i + dt sum_j m_j (v_adv_i - v_adv_j) . gradW_ij
+#   d_ii = -dt^2 sum_j m_j / rho_i^2 gradW_ij,  d_ji = dt^2 m_i / rho_i^2 gradW_ij
+#   a_ii = sum_j m_j (d_ii - d_ji) . gradW_ij
+#   dt^2 a_p,i = -dt^2 sum_j m_j (p_i / rho_i^2 + p_j / rho_j^2) gradW_ij
+#   (A p)_i = sum_j m_j (dt^2 a_p,i - dt^2 a_p,j) . gradW_ij
+#   residual = rho_adv + (A p) - rho0,
+#   p_new = (1 - w) p + w / a_ii (rho0 - rho_adv - ((A p) - a_ii p)).
+# The reference's per-particle form (ComputeDii, ComputeAii, Pressure_DP,
+# Pressure_Iter: fluid_system_kern.cu:673-1050) is this operator regrouped;
+# the oracle's restatement must equal it to rounding.
+
+def grad_w_all(x, h, s):
+    """gradW_ij for all pairs, [i, j, 3] (zero on the diagonal); s is the
+    normalisation 1 / (pi h^3) as the parameters carry it (in FP32)."""
+    d = x[:, None, :] - x[None, :, :]
+    r = np.sqrt((d * d).sum(-1))
+    q = r / h
+    dwdr = np.where(q <= 1.0, s / h * (-3 * q + 2.25 * q * q),
+                    np.where(q <= 2.0, -0.75 * s / h * (2 - q) ** 2, 0.0))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        g = np.where(r[..., None] > 0, dwdr[..., None] / r[..., None] * d, 0.0)
+    return g
+
+
+def test_iisph_operator_against_brute_force_ppe():
+    h, s, n = 0.01, 0.0092, 7
+    p, disp, calc = lattice_params(h, s, n, 1)
+    rng = np.random.default_rng(11)
+    disp["pos"] += rng.uniform(-0.15, 0.15, disp["pos"].shape) * s / p.simscale
+    calc["vel"] = rng.normal(0, 0.05, (len(calc), 3))
+    calc["mass"] *= rng.uniform(0.8, 1.2, len(calc))   # unequal masses: m_i and m_j differ
+    p.dt = 1e-3
+    o = options(wcsph_kernel=1, full_support=1, boundary_pressure=1, omega=0.5, clamp_pressure=0)
+    st = orc.State(disp, calc, p, np.float64)
+    st.opts = o
+    st.grid_build()
+    st.ii_density()
+    st.ii_dii()
+    st.ii_aii()
+    fl = st["mgcell"] != UNDEF
+    assert fl.all() and (st["type"] == 0).all()
+    pr = rng.uniform(0.0, 300.0, st.n)
+    st.a["press_l"][:] = pr
+    st.a["press_l1"][:] = pr
+    st.ii_iteration()
+
+    dt, w = float(p.dt), 0.5
+    x = st["pos"].astype(np.float64) * float(p.simscale)
+    m = st["mass"].astype(np.float64)
+    rho = st["dens"].astype(np.float64)
+    rho0 = st["restdens"].astype(np.float64)
+    va = st["veleval"].astype(np.float64)
+    h32 = float(p.smoothradius)
+    assert float(p.kspline) == pytest.approx(1.0 / (np.pi * h32 ** 3), rel=1e-6)
+    G = grad_w_all(x, h32, float(p.kspline))
+    rho_adv = rho + dt * np.einsum("j,ijk,ijk->i", m, va[:, None, :] - va[None, :, :], G)
+    dii = -dt * dt * np.einsum("j,ijk->ik", m, G) / (rho ** 2)[:, None]
+    dji = dt * dt * (m / rho ** 2)[:, None, None] * G           # [i, j]: d_ji
+    aii = np.einsum("j,ijk,ijk->i", m, dii[:, None, :] - dji, G)
+    c = pr / rho ** 2
+    ap = -dt * dt * np.einsum("j,ijk->ik", m, (c[:, None] + c[None, :])[..., None] * G)
+    Ap = np.einsum("j,ijk,ijk->i", m, ap[:, None, :] - ap[None, :, :], G)
+    res = rho_adv + Ap - rho0
+    pnew = (1 - w) * pr + w / aii * (rho0 - rho_adv - (Ap - aii * pr))
+
+    def close(name, got, want):
+        e = np.abs(got - want).max() / np.abs(want).max()
+        print(f"{name}: {e:.2e}")
+        assert e < 1e-9, name
+
+    close("rho_adv", st["rho_adv"], rho_adv)
+    close("d_ii", st["dii"], dii)
+    close("a_ii", st["aii"], aii)
+    close("residual", st["residue"], res)
+    close("p after one Jacobi iteration", st["press_l"], pnew)
+
+
+# ------------------------------------------------ WCSPH fluid pair force --
+# MfComputeForceCUDA's fluid-fluid terms (fluid_system_kern.cu:365-376) for
+# two particles, from the closed-form kernel gradient: the pressure term is
+# asymmetric, a_i = -0.5 m_j p_j / (rho_i rho_j) gradW_ij (only p_j enters),
+# and the viscosity is a_i = m_j (nu_i + nu_j) / (rho_i rho_j) (v_i - v_j)
+# (gradW_ij . x_ij) / |x_ij|^2, i.e. along the velocity difference.
+@pytest.mark.parametrize("family,art", [(0, 0.0), (1, 0.0), (1, 0.3)])
+def test_wcsph_fluid_pair_closed_form(family, art):
+    """With art > 0 (an option beyond the reference, off by default) the
+    approaching pair also gets Monaghan's artificial viscosity
+    a_i = -m_j Pi gradW_ij, Pi = -art c mu / rho_bar,
+    mu = h (v_ij . x_ij) / (|x_ij|^2 + 0.01 h^2), rho_bar = (rho_i + rho_j) / 2."""
+    h = 0.02 if family == 0 else 0.01
+    p, disp, calc = lattice_params(h, 0.01, 8, family)
+    disp = disp[:2].copy()
+    calc = calc[:2].copy()
+    d = np.array([1.0, 0.5, -0.25])
+    d /= np.linalg.norm(d)
+    r = 0.55 * h
+    disp["pos"][0] = [0.03, 0.03, 0.03]
+    disp["pos"][1] = disp["pos"][0] - r * d / p.simscale
+    calc["mass"] = [2.0e-3, 1.5e-3]
+    p.num = p.maxNum = 2
+    st = orc.State(disp, calc, p, np.float64)
+    st.opts = options(wcsph_kernel=family, art_visc_alpha=art, sound_speed=20.0)
+    st.grid_build()
+    i = int(np.where(st["bornid"] == 0)[0][0])
+    j = 1 - i
+    rho = {i: 1010.0, j: 990.0}
+    pres = {i: 500.0, j: 1300.0}
+    nu = {i: 1e-3, j: 3e-3}
+    vel = {i: np.array([0.2, -0.1, 0.05]), j: np.array([-0.05, 0.1, 0.0])}
+    if art > 0:
+        vel = {i: -vel[i], j: -vel[j]}    # approaching: (v_i - v_j) . x_ij < 0
+    for k in (i, j):
+        st.a["dens"][k] = 1.0 / rho[k]
+        st.a["pressure"][k] = pres[k]
+        st.a["visc"][k] = nu[k]
+        st.a["vel"][k] = vel[k]
+        st.a["restdens"][k] = 1000.0
+    st.force()
+    x_ij = (st["pos"][i] - st["pos"][j]) * float(p.simscale)
+    gw = grad_w(x_ij, float(p.smoothradius), family)
+    m_j = st["mass"][j]
+    a_p = -0.5 * m_j * pres[j] / (rho[i] * rho[j]) * gw
+    v_ij = vel[i] - vel[j]
+    a_v = m_j * (nu[i] + nu[j]) / (rho[i] * rho[j]) * v_ij * (gw @ x_ij) / (x_ij @ x_ij)
+    a_art = np.zeros(3)
+    if art > 0:
+        hh = float(p.smoothradius)
+        assert v_ij @ x_ij < 0
+        mu = hh * (v_ij @ x_ij) / (x_ij @ x_ij + 0.01 * hh * hh)
+        Pi = -art * 20.0 * mu / (0.5 * (rho[i] + rho[j]))
+        a_art = -m_j * Pi * gw
+        assert np.abs(a_art).max() > 1e-3 * np.abs(a_p).max()
+    want = a_p + a_v + a_art
+    np.testing.assert_allclose(st["accel"][i], want, rtol=1e-5, atol=1e-9 * np.abs(want).max())
+    assert (a_p @ x_ij) > 0          # repelled from j
