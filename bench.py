@@ -189,7 +189,7 @@ def learned_ctx(sphx, args, p, disp, calc, alpha, net):
 
 NAMES = ["hash", "radix_sort", "cell_start", "reorder", "nbr_build", "iisph_density_dii",
          "mf_drift", "mf_alpha", "mf_correction_force", "iisph_aii", "mlp_features", "mlp_infer",
-         "iisph_dp", "iisph_iter", "iisph_integrate", "mlp_record", "mlp_train", "totals"]
+         "iisph_dp", "iisph_iter", "iisph_publish", "iisph_integrate", "mlp_record", "mlp_train", "totals"]
 
 
 def run_method(sphx, args, make, label, only=None, track=False):

@@ -506,6 +506,15 @@ int sphx_solve_history(sphx_context* ctx, int* iters, double* err, int max_solve
 int sphx_tune_set(sphx_context* ctx, const char* kernel, int block);
 int sphx_tune_get(sphx_context* ctx, const char* kernel, int* block);
 int sphx_autotune(sphx_context* ctx, int steps);
+/* The autotuner's cache: a JSON file of entries {"arch", "bucket", "kernel",
+ * "block"}, keyed by (GPU architecture, size bucket = floor(log2 of the
+ * context's particle count), kernel).  _save merges the context's current
+ * sizes into the file under its own key (other entries are kept); _load
+ * applies the entries for the context's architecture and bucket, writing
+ * how many it applied to *applied (0 when the file is absent).  No
+ * counterpart in the reference (fixed block size, fluid_system_host.cu:108). */
+int sphx_tune_cache_save(sphx_context* ctx, const char* path);
+int sphx_tune_cache_load(sphx_context* ctx, const char* path, int* applied);
 
 /* ------------------------------------------------------------------------ */
 /* Stokesian mobility (SPH_Solver/stokesian/Mobility.cu)                     */

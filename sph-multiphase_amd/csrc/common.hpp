@@ -213,7 +213,8 @@ struct sphx_context {
   bool sorted_valid = false;
 
   // IISPH per-slot buffers (not permuted, as in the reference)
-  sphx::DevBuf<float4> dii;       // d_ii.xyz, a_ii
+  sphx::DevBuf<float4> dii;       // d_ii.xyz, m
+  sphx::DevBuf<float4> cst;       // a_ii, rho0 - rho_adv, m / rho^2, rho0 (the Jacobi pass's constants)
   sphx::DevBuf<float4> dijpj;     // sum_j d_ij p_j .xyz, unused
   sphx::DevBuf<float4> mqa;       // {x, y, z, m q.x}, q = d_ii p + sum d_ij p_j (0 for boundary)
   sphx::DevBuf<float2> mqb;       // {m q.y, m q.z}

@@ -306,6 +306,7 @@ int sphx_create(sphx_context** out, const sphx_ParamCarrier* p) {
   chk(c->val_tmp.alloc(cap));
   chk(c->mfid.alloc(cap));
   chk(c->dii.alloc(cap));
+  chk(c->cst.alloc(cap));
   chk(c->dijpj.alloc(cap));
   chk(c->mqa.alloc(cap));
   chk(c->mqb.alloc(cap));
@@ -326,6 +327,7 @@ int sphx_create(sphx_context** out, const sphx_ParamCarrier* p) {
       k_iota<<<blocks_for(c->cap, 256), 256, 0, c->stream>>>(c->perm[b].p, c->cap);
     }
     if (hipMemsetAsync(c->dii.p, 0, cap * sizeof(float4), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->cst.p, 0, cap * sizeof(float4), c->stream) != hipSuccess ||
         hipMemsetAsync(c->dijpj.p, 0, cap * sizeof(float4), c->stream) != hipSuccess ||
         hipMemsetAsync(c->press_l.p, 0, cap * sizeof(float), c->stream) != hipSuccess ||
         hipMemsetAsync(c->press_l1.p, 0, cap * sizeof(float), c->stream) != hipSuccess ||
@@ -366,7 +368,7 @@ int sphx_destroy(sphx_context* c) {
   c->cold_color.release(); c->cold_mat.release(); c->cold_X.release(); c->cold_GK.release();
   c->key.release(); c->idsort.release(); c->key_tmp.release(); c->val_tmp.release();
   c->block_hist.release(); c->cell_start.release(); c->mfid.release();
-  c->dii.release(); c->dijpj.release(); c->mqa.release(); c->mqb.release(); c->gh.release();
+  c->dii.release(); c->cst.release(); c->dijpj.release(); c->mqa.release(); c->mqb.release(); c->gh.release();
   c->press_l.release(); c->press_l1.release(); c->rho_adv.release(); c->residue.release();
   c->sfac.release(); c->red.release(); c->solve_state.release(); c->solve_err.release();
   if (c->host_pinned) (void)hipHostFree(c->host_pinned);

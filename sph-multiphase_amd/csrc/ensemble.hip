@@ -18,6 +18,8 @@
 #include "common.hpp"
 
 #include <cstring>
+#include <fstream>
+#include <sstream>
 
 namespace sphx {
 
@@ -196,6 +198,64 @@ static int tune_index(const char* name) {
   return -1;
 }
 
+// ---------------------------------------------------------- tuning cache --
+// The file is a JSON array with one object per line,
+//   {"arch": "gfx950", "bucket": 20, "kernel": "iisph_iter", "block": 128},
+// written here and read back line by line (the reader accepts exactly what
+// the writer produces).
+struct TuneEntry {
+  std::string arch, kernel;
+  int bucket = 0, block = 0;
+};
+
+static int size_bucket(int n) {
+  int b = 0;
+  while (n > 1) { n >>= 1; ++b; }
+  return b;
+}
+
+static int device_arch(sphx_context* c, std::string* arch) {
+  hipDeviceProp_t prop;
+  SPHX_HIP(hipGetDeviceProperties(&prop, c->device));
+  std::string a = prop.gcnArchName;
+  *arch = a.substr(0, a.find(':'));   // gfx950 (feature suffixes dropped)
+  return 0;
+}
+
+static bool parse_str(const std::string& line, const char* key, std::string* out) {
+  const std::string k = std::string("\"") + key + "\": \"";
+  const size_t a = line.find(k);
+  if (a == std::string::npos) return false;
+  const size_t b = line.find('"', a + k.size());
+  if (b == std::string::npos) return false;
+  *out = line.substr(a + k.size(), b - a - k.size());
+  return true;
+}
+
+static bool parse_int(const std::string& line, const char* key, int* out) {
+  const std::string k = std::string("\"") + key + "\": ";
+  const size_t a = line.find(k);
+  if (a == std::string::npos) return false;
+  char* end = nullptr;
+  const long v = std::strtol(line.c_str() + a + k.size(), &end, 10);
+  if (end == line.c_str() + a + k.size()) return false;
+  *out = (int)v;
+  return true;
+}
+
+static std::vector<TuneEntry> read_cache(const char* path) {
+  std::vector<TuneEntry> out;
+  std::ifstream f(path);
+  std::string line;
+  while (std::getline(f, line)) {
+    TuneEntry e;
+    if (parse_str(line, "arch", &e.arch) && parse_str(line, "kernel", &e.kernel) &&
+        parse_int(line, "bucket", &e.bucket) && parse_int(line, "block", &e.block))
+      out.push_back(e);
+  }
+  return out;
+}
+
 }  // namespace sphx
 
 extern "C" {
@@ -215,6 +275,47 @@ int sphx_tune_get(sphx_context* c, const char* kernel, int* block) {
   int t = tune_index(kernel);
   SPHX_CHECK(t >= 0, "sphx_tune_get", "unknown kernel name");
   *block = c->tune[t];
+  return 0;
+}
+
+int sphx_tune_cache_save(sphx_context* c, const char* path) {
+  SPHX_CHECK(c && path, "sphx_tune_cache_save", "null argument");
+  std::string arch;
+  SPHX_TRY(device_arch(c, &arch));
+  const int bucket = size_bucket(c->num);
+  std::vector<TuneEntry> all;
+  for (const TuneEntry& e : read_cache(path))
+    if (!(e.arch == arch && e.bucket == bucket && tune_index(e.kernel.c_str()) >= 0)) all.push_back(e);
+  for (int t = 0; t < T_COUNT; ++t) all.push_back(TuneEntry{arch, TUNE_NAMES[t], bucket, c->tune[t]});
+  std::ostringstream o;
+  o << "[\n";
+  for (size_t q = 0; q < all.size(); ++q)
+    o << "  {\"arch\": \"" << all[q].arch << "\", \"bucket\": " << all[q].bucket
+      << ", \"kernel\": \"" << all[q].kernel << "\", \"block\": " << all[q].block << "}"
+      << (q + 1 < all.size() ? ",\n" : "\n");
+  o << "]\n";
+  std::ofstream f(path, std::ios::trunc);
+  SPHX_CHECK(f.good(), "sphx_tune_cache_save", "cannot open the cache file for writing");
+  f << o.str();
+  f.close();
+  SPHX_CHECK(f.good(), "sphx_tune_cache_save", "write failed");
+  return 0;
+}
+
+int sphx_tune_cache_load(sphx_context* c, const char* path, int* applied) {
+  SPHX_CHECK(c && path && applied, "sphx_tune_cache_load", "null argument");
+  *applied = 0;
+  std::string arch;
+  SPHX_TRY(device_arch(c, &arch));
+  const int bucket = size_bucket(c->num);
+  for (const TuneEntry& e : read_cache(path)) {
+    if (e.arch != arch || e.bucket != bucket) continue;
+    const int t = tune_index(e.kernel.c_str());
+    if (t < 0 || e.block < 64 || e.block > 256 || e.block % 64) continue;
+    c->tune[t] = e.block;
+    ++*applied;
+  }
+  if (*applied && c->graph_num >= 0) SPHX_TRY(sphx_graph_build(c, c->graph_budget));
   return 0;
 }
 

@@ -38,6 +38,66 @@ __device__ __forceinline__ bool in_support(const KP& k, float dsq) {
   return dsq < lim && dsq > 0.f;
 }
 
+// Per-pair terms of the density and d_ii passes, branch-free in the stored
+// position units (see solve_gradf): x_u = x_i - x_j, dsq = |x_u|^2 (+inf for
+// an unused slot).  The fused pass and the two separate ones call the same
+// functions, so they produce the same bits.
+struct DensDii {
+  SolveGrad g;          // support 2h
+  uint32_t lim_sup_m1;  // d_ii's support (SolverOptions::full_support): bits - 1
+  float kw;             // kspline
+  float lim2h;          // (2h)^2 / s^2
+};
+__device__ __forceinline__ DensDii dens_dii_consts(const KP& k) {
+  DensDii c;
+  c.g = solve_grad(k, true);
+  c.lim_sup_m1 = k.full_support ? c.g.lim_m1 : __float_as_uint(k.h2 / (k.ss * k.ss)) - 1u;
+  c.kw = k.kspline;
+  c.lim2h = 4.f * k.h2 / (k.ss * k.ss);
+  return c;
+}
+// sum + W(r) m_j of contributeIIDensity (kern.cu:1085-1100) for r < 2h
+// (r = 0 included: a coincident neighbour adds W(0)).  The accumulation is
+// an explicit FMA (as the d_ii sum below), so every kernel that calls these
+// helpers rounds the same way whatever the compiler fuses around them.
+__device__ __forceinline__ float pair_density(const DensDii& c, float dsq, float mj, float sum) {
+#pragma clang fp contract(off)
+  const float ir = __builtin_amdgcn_rsqf(dsq);
+  const float q = dsq > 0.f ? dsq * ir * c.g.q_of_r : 0.f;
+  const float t = 2.f - q;
+  const float w = q > 1.f ? 0.25f * t * t * t * c.kw : (1.f - 1.5f * q * q + 0.75f * q * q * q) * c.kw;
+  const bool in = dsq < c.lim2h;
+  return __builtin_fmaf(in ? w : 0.f, in ? mj : 0.f, sum);
+}
+// ComputeDii's per-pair terms (kern.cu:673-798): grad W = x_u g for
+// 0 < r < 2h; the artificial boundary viscosity of a boundary neighbour
+// (:712-723, only for an approaching pair) is added to a; returns the
+// factor f with d_ii -= x_u f (zero outside d_ii's support, or for a
+// boundary neighbour without boundary pressure).
+__device__ __forceinline__ float pair_dii(const KP& k, const DensDii& c, float dx, float dy,
+                                          float dz, float dsq, bool bnd, float mj_fluid,
+                                          float vb, float rest, const float4& vi,
+                                          const float4* __restrict__ vel, uint32_t j, float& ax,
+                                          float& ay, float& az) {
+#pragma clang fp contract(off)
+  const float g = solve_gradf(c.g, dsq);   // 0 outside 0 < r < 2h
+  if (bnd && g != 0.f) {
+    const float4 vj = vel[j];
+    float pij = k.ss * (dx * (vi.x - vj.x) + dy * (vi.y - vj.y) + dz * (vi.z - vj.z));
+    if (pij < 0.f) {
+      pij = pij / (k.ss * k.ss * dsq + k.h2 * 0.01f);
+      pij = pij * k.bvisc * k.h / 2.f / rest;
+      pij = rest * vb * pij;
+      const float gp = g * pij;
+      ax = __builtin_fmaf(dx, gp, ax);
+      ay = __builtin_fmaf(dy, gp, ay);
+      az = __builtin_fmaf(dz, gp, az);
+    }
+  }
+  const float mj = bnd ? (k.boundary_pressure ? rest * vb : 0.f) : mj_fluid;   // :763-764
+  return (__float_as_uint(dsq) - 1u) < c.lim_sup_m1 ? g * mj : 0.f;
+}
+
 // ComputeDensityIISPH + contributeIIDensity (kern.cu:1067-1115): rho = sum m W
 __global__ void k_ii_density(KP k, NbrList L, const uint32_t* __restrict__ key, const int* __restrict__ cs,
                              const float4* __restrict__ pos, float4* __restrict__ phys) {
@@ -51,17 +111,16 @@ __global__ void k_ii_density(KP k, NbrList L, const uint32_t* __restrict__ key, 
   const float rest = qi.w;
   // self term (the reference's scan visits i itself, W(0) = kspline)
   float sum = k.kspline * qi.x;
+  const DensDii dc = dens_dii_consts(k);
   struct Rec { float4 p, q; };
-  for_each_listed_b<NBR_BATCH_SMALL, false>(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j]}; },
-                                     [&](int, const Rec& o) {
+  for_each_listed_m<NBR_BATCH_SMALL, false>(
+      k, L, cs, gc, i, [&](uint32_t j) { return Rec{load_at(pos, j * 16u), load_at(phys, j * 16u)}; },
+      [&](uint32_t, const Rec& o, bool valid) {
     const float4 pj = o.p, qj = o.q;
-    float mj = (type_of(pj) == SPHX_TYPE_BOUNDARY) ? rest * qj.y : qj.x;
-    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
-    float dsq = dx * dx + dy * dy + dz * dz;
-    if (dsq >= 4.f * k.h2) return;
-    float r, ir;
-    dist_rinv(dsq, r, ir);
-    sum += cubic_w(r, k.inv_h, k.kspline) * mj;
+    const float mj = (type_of(pj) == SPHX_TYPE_BOUNDARY) ? rest * qj.y : qj.x;
+    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const float dsq = valid ? pair_dsq(dx, dy, dz) : INFINITY;
+    sum = pair_density(dc, dsq, mj, sum);
   });
   phys[i].y = sum;
 }
@@ -83,42 +142,25 @@ __global__ void k_ii_dii(KP k, NbrList L, const uint32_t* __restrict__ key, cons
   const float rest = qi.w;
   float ax = 0.f, ay = 0.f, az = 0.f;
   float dx_ = 0.f, dy_ = 0.f, dz_ = 0.f;
+  const DensDii dc = dens_dii_consts(k);
   struct Rec { float4 p, q; };
-  for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j]}; },
-                    [&](int j, const Rec& o) {
+  for_each_listed_m<NBR_BATCH, true>(
+      k, L, cs, gc, i, [&](uint32_t j) { return Rec{load_at(pos, j * 16u), load_at(phys, j * 16u)}; },
+      [&](uint32_t j, const Rec& o, bool valid) {
     const float4 pj = o.p, qj = o.q;
-    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
-    float dsq = dx * dx + dy * dy + dz * dz;
-    float r, ir;
-    dist_rinv(dsq, r, ir);
-    if (!(r < 2.f * k.h && r > 0.f)) return;
-    float gf = cubic_gradf(r, ir, k.inv_h, k.kspline);
-    int tj = type_of(pj);
-    if (tj == SPHX_TYPE_BOUNDARY) {
-      // artificial boundary viscosity (:712-723)
-      float4 vj = vel[j];
-      float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
-      float pij = dx * vx + dy * vy + dz * vz;
-      if (pij < 0.f) {
-        pij = pij / (r * r + k.h2 * 0.01f);
-        pij = pij * k.bvisc * k.h / 2.f / rest;
-        pij = rest * qj.y * pij;
-        ax += dx * gf * pij; ay += dy * gf * pij; az += dz * gf * pij;
-      }
-    }
-    bool sup = k.full_support ? true : (r < k.h);
-    if (!sup) return;
-    float mj;
-    if (tj != SPHX_TYPE_BOUNDARY) mj = qj.x;
-    else if (k.boundary_pressure) mj = rest * qj.y;   // :763-764
-    else return;
-    dx_ -= dx * gf * mj; dy_ -= dy * gf * mj; dz_ -= dz * gf * mj;
+    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const float dsq = valid ? pair_dsq(dx, dy, dz) : INFINITY;
+    const bool bnd = type_of(pj) == SPHX_TYPE_BOUNDARY;
+    const float f = pair_dii(k, dc, dx, dy, dz, dsq, bnd, qj.x, qj.y, rest, vi, vel, j, ax, ay, az);
+    dx_ = __builtin_fmaf(-dx, f, dx_);
+    dy_ = __builtin_fmaf(-dy, f, dy_);
+    dz_ = __builtin_fmaf(-dz, f, dz_);
   });
   ax += k.g.x; ay += k.g.y; az += k.g.z;
   veleval[i] = make_float4(vi.x + ax * k.dt, vi.y + ay * k.dt, vi.z + az * k.dt, 0.f);
   float rho = qi.y;
   float f = 1.f / rho / rho * k.dt * k.dt;
-  dii[i] = make_float4(dx_ * f, dy_ * f, dz_ * f, 0.f);
+  dii[i] = make_float4(dx_ * f, dy_ * f, dz_ * f, qi.x);   // d_ii, m (the DP pass's record)
 }
 
 // k_ii_density and k_ii_dii in one neighbour loop, for steps that run them
@@ -143,43 +185,26 @@ __global__ void k_ii_density_dii(KP k, NbrList L, const uint32_t* __restrict__ k
   float sum = k.kspline * qi.x;
   float ax = 0.f, ay = 0.f, az = 0.f;
   float dx_ = 0.f, dy_ = 0.f, dz_ = 0.f;
+  const DensDii dc = dens_dii_consts(k);
   struct Rec { float4 p, q; };
-  for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j]}; },
-                    [&](int j, const Rec& o) {
+  for_each_listed_m<NBR_BATCH, true>(
+      k, L, cs, gc, i, [&](uint32_t j) { return Rec{load_at(pos, j * 16u), load_at(phys, j * 16u)}; },
+      [&](uint32_t j, const Rec& o, bool valid) {
     const float4 pj = o.p, qj = o.q;
-    const int tj = type_of(pj);
-    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
-    float dsq = dx * dx + dy * dy + dz * dz;
-    float r, ir;
-    dist_rinv(dsq, r, ir);
-    if (dsq < 4.f * k.h2)
-      sum += cubic_w(r, k.inv_h, k.kspline) * ((tj == SPHX_TYPE_BOUNDARY) ? rest * qj.y : qj.x);
-    if (!(r < 2.f * k.h && r > 0.f)) return;
-    float gf = cubic_gradf(r, ir, k.inv_h, k.kspline);
-    if (tj == SPHX_TYPE_BOUNDARY) {
-      float4 vj = vel[j];
-      float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
-      float pij = dx * vx + dy * vy + dz * vz;
-      if (pij < 0.f) {
-        pij = pij / (r * r + k.h2 * 0.01f);
-        pij = pij * k.bvisc * k.h / 2.f / rest;
-        pij = rest * qj.y * pij;
-        ax += dx * gf * pij; ay += dy * gf * pij; az += dz * gf * pij;
-      }
-    }
-    bool sup = k.full_support ? true : (r < k.h);
-    if (!sup) return;
-    float mj;
-    if (tj != SPHX_TYPE_BOUNDARY) mj = qj.x;
-    else if (k.boundary_pressure) mj = rest * qj.y;
-    else return;
-    dx_ -= dx * gf * mj; dy_ -= dy * gf * mj; dz_ -= dz * gf * mj;
+    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const float dsq = valid ? pair_dsq(dx, dy, dz) : INFINITY;
+    const bool bnd = type_of(pj) == SPHX_TYPE_BOUNDARY;
+    sum = pair_density(dc, dsq, bnd ? rest * qj.y : qj.x, sum);
+    const float f = pair_dii(k, dc, dx, dy, dz, dsq, bnd, qj.x, qj.y, rest, vi, vel, j, ax, ay, az);
+    dx_ = __builtin_fmaf(-dx, f, dx_);
+    dy_ = __builtin_fmaf(-dy, f, dy_);
+    dz_ = __builtin_fmaf(-dz, f, dz_);
   });
   phys[i].y = sum;
   ax += k.g.x; ay += k.g.y; az += k.g.z;
   veleval[i] = make_float4(vi.x + ax * k.dt, vi.y + ay * k.dt, vi.z + az * k.dt, 0.f);
   float f = 1.f / sum / sum * k.dt * k.dt;
-  dii[i] = make_float4(dx_ * f, dy_ * f, dz_ * f, 0.f);
+  dii[i] = make_float4(dx_ * f, dy_ * f, dz_ * f, qi.x);   // d_ii, m (the DP pass's record)
 }
 
 // ComputeAii (kern.cu:801-908): rho_adv, warm start p0 = w p(t-1), a_ii;
@@ -197,9 +222,10 @@ struct FeatOut {
 template <bool FEAT>
 __global__ __launch_bounds__(NB_THREADS) void k_ii_aii(KP k, NbrList L, const uint32_t* __restrict__ key, const int* __restrict__ cs,
                          const float4* __restrict__ pos, const float4* __restrict__ veleval,
-                         const float4* __restrict__ phys, float4* __restrict__ dii,
+                         const float4* __restrict__ phys, const float4* __restrict__ dii,
                          float* __restrict__ rho_adv, float* __restrict__ press_l,
-                         float4* __restrict__ ps, float4* __restrict__ gh, FeatOut fo) {
+                         float4* __restrict__ ps, float4* __restrict__ gh,
+                         float4* __restrict__ cst, FeatOut fo) {
   int i = xcd_index();
   if (i >= k.n) return;
   float4 pi = pos[i];
@@ -254,7 +280,7 @@ __global__ __launch_bounds__(NB_THREADS) void k_ii_aii(KP k, NbrList L, const ui
   float p0 = k.warm * qi.z;
   press_l[i] = p0;
   ps[i].w = mi * p0 / rhoi / rhoi;
-  dii[i].w = aii;
+  cst[i] = make_float4(aii, rest - radv, mi / rhoi / rhoi, rest);
   if (FEAT) {
     fo.pprev[i] = qi.z;
     feat_write(k, fo.mp, fa, qi, radv, di, aii, vi, fo.alpha, fo.hist, i,
@@ -297,86 +323,53 @@ __global__ void k_ii_gh(KP k, NbrList L, const uint32_t* __restrict__ key, const
   gh[i] = make_float4(Gx, Gy, Gz, H);
 }
 
-// Kernel gradient of the pressure-solve passes, branch-free, in the units
-// of the stored positions: with x_u = x_i - x_j before the simscale s, the
-// reference's grad W_ij = x_u s g(q) kspline / (h r), r = s |x_u|, q = r / h
-// (kern.cu:637-671) is x_u g(q) kspline / (h |x_u|): s cancels except in q.
-// g is -3q + 2.25q^2 (q <= 1) or -0.75(2 - q)^2, with kspline / h folded
-// into the coefficients; the support test 0 < |x_u|^2 < lim / s^2 is one
-// unsigned compare of the bit patterns (|x_u|^2 >= 0; NaN fails it).  Agrees
-// with cubic_gradf to rounding.
-struct SolveGrad {
-  float q_of_r;       // s / h
-  float co, ca, cb;   // -0.75, -3, 2.25 times kspline / h
-  uint32_t lim_m1;    // bits of lim / s^2, minus 1
-};
-
-__device__ __forceinline__ SolveGrad solve_grad(const KP& k) {
-  const float ks = k.kspline * k.inv_h;
-  const float lim = (k.full_support ? 4.f * k.h2 : k.h2) / (k.ss * k.ss);
-  return SolveGrad{k.ss * k.inv_h, -0.75f * ks, -3.f * ks, 2.25f * ks, __float_as_uint(lim) - 1u};
-}
-
-// f with grad W = x_u f; 0 outside the support (dsq_u = +inf: an unused slot).
-__device__ __forceinline__ float solve_gradf(const SolveGrad& s, float dsq_u) {
-  const float ir = __builtin_amdgcn_rsqf(dsq_u);
-  const float q = dsq_u * ir * s.q_of_r;
-  const float t = 2.f - q;
-  const float outer = s.co * (t * t);
-  const float inner = q * fmaf(s.cb, q, s.ca);
-  const float g = (q > 1.f ? outer : inner) * ir;
-  return (__float_as_uint(dsq_u) - 1u) < s.lim_m1 ? g : 0.f;
-}
-
 // Pressure_DP (kern.cu:910-958) plus q_i = d_ii p_i + dijpj_i, stored as
-// the record the Jacobi pass gathers ({x, m q.x}, {m q.y, m q.z}), and the
-// pressure acceleration q_i / dt^2 (kern.cu:1044-1047).
+// the record the Jacobi pass gathers ({x, m q.x}, {m q.y, m q.z}).  The
+// pressure acceleration q_i / dt^2 (kern.cu:1044-1047) of the solve's last
+// DP pass is written once, when the solve ends (k_ii_publish), from that
+// record.  Per particle the pass reads its list, key, own {x, s} record,
+// {d_ii, m} and p; boundary and out-of-grid slots are marked in the list.
 struct DpArgs {
   const uint32_t* __restrict__ key;
   const int* __restrict__ cs;
-  const float4* __restrict__ pos;
+  const float4* __restrict__ pos;   // for the scene of an ensemble's particle
   const float4* __restrict__ ps;
-  const float4* __restrict__ phys;
   const float4* __restrict__ dii;
   const float* __restrict__ press_l;
   float4* __restrict__ dijpj;
   float4* __restrict__ mqa;
   float2* __restrict__ mqb;
-  float4* __restrict__ accel;
 };
 
 __device__ __forceinline__ void dp_particle(const KP& k, const NbrList& L, const DpArgs& a,
                                             const SceneCtl& sc, int i) {
   if (i >= k.n) return;
-  float4 pi = a.pos[i];
-  if (type_of(pi) == SPHX_TYPE_BOUNDARY) return;
-  if (sc.n && sc.state[2 * scene_of(pi)]) return;   // converged scene: p is final
-  uint32_t gc = a.key[i];
-  if ((int)gc == k.gridtotal) return;
+  if (L.cnt[i] == NBR_INVALID) return;   // boundary, or outside the grid
+  if (sc.n && sc.state[2 * scene_of(a.pos[i])]) return;   // converged scene: p is final
+  const uint32_t gc = a.key[i];
+  const float4* __restrict__ ps = a.ps;
+  const float4 pi = ps[i];
   float sx = 0.f, sy = 0.f, sz = 0.f;
   // one 16-byte gather per neighbour: {x_j, s_j}; boundary s_j = 0 adds
   // nothing, as the reference's skip of boundary j (:925-926)
-  const float4* __restrict__ ps = a.ps;
   const SolveGrad sg = solve_grad(k);
   for_each_listed_m<NBR_BATCH_SMALL, false>(
       k, L, a.cs, gc, i, [&](uint32_t j) { return load_at(ps, j * 16u); },
       [&](uint32_t, const float4& pj, bool valid) {
     const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-    const float dsq = valid ? dx * dx + dy * dy + dz * dz : INFINITY;
+    const float dsq = valid ? pair_dsq(dx, dy, dz) : INFINITY;
     const float gf = solve_gradf(sg, dsq) * pj.w;
     sx += dx * gf; sy += dy * gf; sz += dz * gf;
   });
   float f = -k.dt * k.dt;
   float4 d = make_float4(sx * f, sy * f, sz * f, 0.f);
   a.dijpj[i] = d;
-  float4 di = a.dii[i];
-  float p = a.press_l[i];
+  const float4 di = a.dii[i];
+  const float p = a.press_l[i];
   const float qx = di.x * p + d.x, qy = di.y * p + d.y, qz = di.z * p + d.z;
-  const float mi = a.phys[i].x;
+  const float mi = di.w;
   a.mqa[i] = make_float4(pi.x, pi.y, pi.z, mi * qx);
   a.mqb[i] = make_float2(mi * qy, mi * qz);
-  const float idt2 = 1.f / (k.dt * k.dt);
-  a.accel[i] = make_float4(qx * idt2, qy * idt2, qz * idt2, 0.f);
 }
 
 __global__ void k_ii_dp(KP k, NbrList L, const int* __restrict__ done, DpArgs a, SceneCtl sc) {
@@ -407,19 +400,20 @@ __device__ void block_reduce3(double a, double b, double c, double* out) {
 // Pressure_Iter (kern.cu:960-1050) for particle i: the relaxed Jacobi
 // update and the residual; rs, ra, rc receive its (signed residue,
 // error / rho0, 1) when it counts toward the solve's error, scene its scene.
+// Per particle the pass reads its list, key, own record {x, m q.x} (for x),
+// the constants {a_ii, rho0 - rho_adv, m / rho^2, rho0} of the a_ii pass,
+// dijpj, {G, H} and p, and writes p, the residual and the DP pass's record
+// {x, s}; the pressure field and press_l1 are written when the solve ends.
 struct IterArgs {
   const uint32_t* __restrict__ key;
   const int* __restrict__ cs;
-  const float4* __restrict__ pos;
-  float4* __restrict__ phys;
-  const float4* __restrict__ dii;
+  const float4* __restrict__ pos;   // for the scene of an ensemble's particle
+  const float4* __restrict__ cst;
   const float4* __restrict__ dijpj;
   const float4* __restrict__ mqa;
   const float2* __restrict__ mqb;
   const float4* __restrict__ gh;
-  const float* __restrict__ rho_adv;
   float* __restrict__ press_l;
-  float* __restrict__ press_l1;
   float4* __restrict__ ps;
   float* __restrict__ residue;
 };
@@ -430,20 +424,23 @@ __device__ __forceinline__ void iter_particle(const KP& k, const NbrList& L, con
   rs = ra = rc = 0.0;
   scene = 0;
   if (i >= k.n) return;
-  float4 pi = a.pos[i];
-  uint32_t gc = a.key[i];
-  scene = scene_of(pi);
-  if (type_of(pi) == SPHX_TYPE_BOUNDARY || (int)gc == k.gridtotal) {
+  if (L.cnt[i] == NBR_INVALID) {   // boundary, or outside the grid
     a.residue[i] = 0.f;
     return;
   }
-  if (sc.n && sc.state[2 * scene]) return;   // converged scene: pressures final
-  const float4 qi = a.phys[i];
-  const float mi = qi.x, rhoi = qi.y, rest = qi.w;
-  const float4 di = a.dii[i];
+  if (sc.n) {
+    scene = scene_of(a.pos[i]);
+    if (sc.state[2 * scene]) return;   // converged scene: pressures final
+  }
+  const uint32_t gc = a.key[i];
+  const float4* __restrict__ mqa = a.mqa;
+  const float2* __restrict__ mqb = a.mqb;
+  const float4 pi = mqa[i];
+  const float4 ci = a.cst[i];
+  const float aii = ci.x, b = ci.y, mr2 = ci.z, rest = ci.w;
   const float4 dpi = a.dijpj[i];
   const float pl = a.press_l[i];
-  const float cji = k.dt * k.dt * mi / rhoi / rhoi * pl;
+  const float cji = k.dt * k.dt * mr2 * pl;
   // S_i = sum_j m_j (dijpj_i - q_j + d_ji p_i) . gradW_ij over fluid j
   // (kern.cu:992-999), plus m_b dijpj_i . gradW_ij over boundary j when
   // boundary pressure is on (:1000-1002).  The position-only parts are
@@ -454,29 +451,24 @@ __device__ __forceinline__ void iter_particle(const KP& k, const NbrList& L, con
   float acc = 0.f;
   const SolveGrad sg = solve_grad(k);
   struct Rec { float4 a; float2 b; };
-  const float4* __restrict__ mqa = a.mqa;
-  const float2* __restrict__ mqb = a.mqb;
   for_each_listed_m<NBR_BATCH, true>(
       k, L, a.cs, gc, i,
       [&](uint32_t j) { return Rec{load_at(mqa, j * 16u), load_at(mqb, j * 8u)}; },
       [&](uint32_t, const Rec& o, bool valid) {
     const float dx = pi.x - o.a.x, dy = pi.y - o.a.y, dz = pi.z - o.a.z;
-    const float dsq = valid ? dx * dx + dy * dy + dz * dz : INFINITY;
+    const float dsq = valid ? pair_dsq(dx, dy, dz) : INFINITY;
     acc += solve_gradf(sg, dsq) * (dx * o.a.w + dy * o.b.x + dz * o.b.y);
   });
   const float4 G = a.gh[i];
   const float S = dpi.x * G.x + dpi.y * G.y + dpi.z * G.z + cji * G.w - acc;
-  float aii = di.w;
   float pn;
   if (fabsf(aii) < 1e-6f) pn = pl;
-  else pn = (1.f - k.omega) * pl + k.omega / aii * (rest - a.rho_adv[i] - S);
+  else pn = (1.f - k.omega) * pl + k.omega / aii * (b - S);
   if (k.clamp_p && pn < 0.f) pn = 0.f;   // kern.cu:1035-1036 (commented there)
-  float res = -rest + (a.rho_adv[i] + S + aii * pl);
+  const float res = (S + aii * pl) - b;   // rho_adv + S + a_ii p - rho0
   a.residue[i] = res;
-  a.phys[i].z = pn;
   a.press_l[i] = pn;
-  a.press_l1[i] = pn;
-  a.ps[i].w = mi * pn / rhoi / rhoi;
+  a.ps[i] = make_float4(pi.x, pi.y, pi.z, mr2 * pn);
   rs = res;
   // With the clamp the solve is a complementarity problem (p > 0 and
   // rho = rho0, or p = 0 and rho <= rho0): the error is |res| where the
@@ -846,21 +838,37 @@ __global__ __launch_bounds__(1024) void k_guard_pick(int* __restrict__ gs, int* 
 }
 
 __global__ void k_guard_restore(KP k, const int* __restrict__ gs, const float4* __restrict__ pos,
-                                const uint32_t* __restrict__ key, float4* __restrict__ phys,
+                                const uint32_t* __restrict__ key, const float4* __restrict__ phys,
                                 const float* __restrict__ keep, float* __restrict__ press_l,
-                                float* __restrict__ press_l1, float4* __restrict__ ps) {
+                                float4* __restrict__ ps) {
   if (gs[0] || !gs[1]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= k.n) return;
   const float4 pp = pos[i];
   if (type_of(pp) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) return;
   const float p = keep[i];
-  float4 q = phys[i];
-  q.z = p;
-  phys[i] = q;
+  const float4 q = phys[i];
   press_l[i] = p;
-  press_l1[i] = p;
   ps[i].w = q.x * p / (q.y * q.y);
+}
+
+// End of a solve: the pressure field and press_l1 take the final iterate
+// (the reference's Pressure_Iter writes both every iteration, :1037-1040),
+// and the pressure acceleration q / dt^2 of the last DP pass (:1044-1047)
+// is recovered from the record it stored, {x, m q.x}, {m q.y, m q.z}.
+__global__ void k_ii_publish(KP k, const int* __restrict__ cnt, const float* __restrict__ press_l,
+                             const float4* __restrict__ dii, const float4* __restrict__ mqa,
+                             const float2* __restrict__ mqb, float4* __restrict__ phys,
+                             float* __restrict__ press_l1, float4* __restrict__ accel) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.n || cnt[i] == NBR_INVALID) return;
+  const float p = press_l[i];
+  phys[i].z = p;
+  press_l1[i] = p;
+  const float f = 1.f / (dii[i].w * k.dt * k.dt);   // 1 / (m dt^2)
+  const float4 a = mqa[i];
+  const float2 b = mqb[i];
+  accel[i] = make_float4(a.w * f, b.x * f, b.y * f, 0.f);
 }
 
 // IntegrateIISPH (kern.cu:1052-1064); optional clamp into the valid cells.
@@ -1004,7 +1012,8 @@ __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
   __syncthreads();
   const int total = ro[18];
   const bool staged = sh[6] == 0 && total <= NB_STAGE;
-  const float rc2 = 4.f * k.h2;
+  // candidates are tested on unscaled differences: |x_u|^2 < 4 h^2 / s^2
+  const float rc2u = 4.f * k.h2 / (k.ss * k.ss);
   uint4* __restrict__ out = idx + nbr_gslot(i, 0, cap / NBR_GROUP);
   int c = 0;
   bool ovf = false;
@@ -1042,12 +1051,13 @@ __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
         uint32_t hit = 0;
 #pragma unroll 4
         for (int u = 0; u < m; ++u) {
-          const int j = j0 + u;
-          const float4 pj = fetch(base + j);
-          float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
-          float dsq = dx * dx + dy * dy + dz * dz;
-          hit |= (uint32_t)(dsq < rc2 && j != i) << u;
+          const float4 pj = fetch(base + j0 + u);
+          const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+          hit |= (uint32_t)(dx * dx + dy * dy + dz * dz < rc2u) << u;
         }
+        // the particle itself is not its own neighbour
+        const uint32_t self = (uint32_t)(i - j0);
+        if (self < (uint32_t)m) hit &= ~(1u << self);
         while (hit) {
           const int u = __builtin_ctz(hit);
           hit &= hit - 1;
@@ -1081,7 +1091,7 @@ __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
     for (int u = c & (NBR_GROUP - 1); u < NBR_GROUP; ++u) pk.push(0u);
     out[(size_t)(c / NBR_GROUP) * NBR_SLICE] = pk.get();
   }
-  if (i < k.n) cnt[i] = (ovf || c > cap) ? cap + 1 : c;
+  if (i < k.n) cnt[i] = !valid ? NBR_INVALID : ((ovf || c > cap) ? cap + 1 : c);
 }
 
 int grid_hash(sphx_context* c);
@@ -1138,14 +1148,13 @@ int nbr_list(sphx_context* c, NbrList* out) {
 static size_t scene_lds(const SceneCtl& sc) { return (size_t)3 * sc.n * sizeof(double); }
 
 static DpArgs dp_args(sphx_context* c) {
-  return DpArgs{c->key.p, c->cell_start.p, c->POS(), c->ps.p, c->PHYS(), c->dii.p, c->press_l.p,
-                c->dijpj.p, c->mqa.p, c->mqb.p, c->ACCEL()};
+  return DpArgs{c->key.p, c->cell_start.p, c->POS(), c->ps.p, c->dii.p, c->press_l.p,
+                c->dijpj.p, c->mqa.p, c->mqb.p};
 }
 
 static IterArgs iter_args(sphx_context* c) {
-  return IterArgs{c->key.p, c->cell_start.p, c->POS(), c->PHYS(), c->dii.p, c->dijpj.p,
-                  c->mqa.p, c->mqb.p, c->gh.p, c->rho_adv.p, c->press_l.p, c->press_l1.p,
-                  c->ps.p, c->residue.p};
+  return IterArgs{c->key.p, c->cell_start.p, c->POS(), c->cst.p, c->dijpj.p, c->mqa.p,
+                  c->mqb.p, c->gh.p, c->press_l.p, c->ps.p, c->residue.p};
 }
 
 static int ensure_red(sphx_context* c) {
@@ -1156,6 +1165,19 @@ static int ensure_red(sphx_context* c) {
 int solve_reset(sphx_context* c) {
   SceneCtl sc = c->scenes();
   k_solve_reset<<<blocks_for(std::max(sc.n, 1), 256), 256, 0, c->stream>>>(c->solve_state.p, sc);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+// Publish a solve's outcome (pressure field, press_l1, pressure
+// acceleration); enqueued after every solve and single iteration.
+static int publish_solve(sphx_context* c, const KP& k) {
+  if (k.n == 0) return 0;
+  timer_begin(c, "iisph_publish");
+  k_ii_publish<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+      k, c->nbr_cnt.p, c->press_l.p, c->dii.p, c->mqa.p, c->mqb.p, c->PHYS(), c->press_l1.p,
+      c->ACCEL());
+  timer_end(c);
   SPHX_HIP(hipGetLastError());
   return 0;
 }
@@ -1215,7 +1237,7 @@ static int enqueue_guard(sphx_context* c, const KP& k) {
   k_guard_pick<<<1, 1024, 0, c->stream>>>(gs, c->solve_state.p, c->solve_err.p, c->red.p, nb,
                                           k.n, ctl);
   k_guard_restore<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
-      k, gs, c->POS(), c->key.p, c->PHYS(), m.pkeep.p, c->press_l.p, c->press_l1.p, c->ps.p);
+      k, gs, c->POS(), c->key.p, c->PHYS(), m.pkeep.p, c->press_l.p, c->ps.p);
   SPHX_HIP(hipGetLastError());
   return 0;
 }
@@ -1322,8 +1344,10 @@ int sphx_nbr_counts(sphx_context* c, int* counts, int* cap) {
   if (cap) *cap = L.cap;
   if (counts && c->num > 0) {
     SPHX_HIP(hipMemcpy(counts, L.cnt, c->num * sizeof(int), hipMemcpyDeviceToHost));
-    for (int i = 0; i < c->num; ++i)
+    for (int i = 0; i < c->num; ++i) {
       if (counts[i] > L.cap) counts[i] = -1;
+      else if (counts[i] == NBR_INVALID) counts[i] = 0;
+    }
   }
   return 0;
 }
@@ -1350,11 +1374,11 @@ int sphx_iisph_predict_aii(sphx_context* c) {
   if (feat)
     k_ii_aii<true><<<blocks_for(k.n, c->tune[T_AII]), c->tune[T_AII], 0, c->stream>>>(
         k, L, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p,
-        c->rho_adv.p, c->press_l.p, c->ps.p, c->gh.p, fo);
+        c->rho_adv.p, c->press_l.p, c->ps.p, c->gh.p, c->cst.p, fo);
   else
     k_ii_aii<false><<<blocks_for(k.n, c->tune[T_AII]), c->tune[T_AII], 0, c->stream>>>(
         k, L, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p,
-        c->rho_adv.p, c->press_l.p, c->ps.p, c->gh.p, fo);
+        c->rho_adv.p, c->press_l.p, c->ps.p, c->gh.p, c->cst.p, fo);
   timer_end(c);
   c->gh_valid = true;
   m.feat_ok = feat;
@@ -1374,6 +1398,7 @@ int sphx_iisph_pressure_iteration(sphx_context* c, double* mean_signed, double* 
   if (k.n == 0) return 0;
   SPHX_TRY(solve_reset(c));
   SPHX_TRY(enqueue_iteration(c, k, 0));
+  SPHX_TRY(publish_solve(c, k));
   if (mean_signed || mean_abs_rel) {
     SPHX_TRY(read_state(c));
     if (mean_abs_rel) *mean_abs_rel = c->host_pinned[0];
@@ -1467,7 +1492,7 @@ int enqueue_solve_coop(sphx_context* c, bool reset) {
                                          c->solve_log_count.p);
     SPHX_HIP(hipGetLastError());
   }
-  return 0;
+  return publish_solve(c, k);
 }
 
 int solve_log_setup(sphx_context* c) {
@@ -1529,6 +1554,7 @@ int iisph_solve_plain(sphx_context* c, int* iters, double* err) {
     if (c->host_pinned[2] != 0.0 || enq >= c->O.max_iters) break;
     chunk = 2;
   }
+  SPHX_TRY(publish_solve(c, k));
   int it = (int)c->host_pinned[3];
   c->last_iters = it;
   if (iters) *iters = it;
@@ -1571,6 +1597,7 @@ int continue_solve(sphx_context* c, int* iters, double* err) {
     for (int t = 0; t < 2; ++t) SPHX_TRY(enqueue_iteration(c, k, 1));
     SPHX_TRY(read_state(c));
   }
+  SPHX_TRY(publish_solve(c, k));
   int it = (int)c->host_pinned[3];
   c->last_iters = it;
   if (iters) *iters = it;
@@ -1651,12 +1678,13 @@ int sphx_download_iisph(sphx_context* c, float* dii, float* dijpj, float* aii, f
   int n = c->num;
   if (n == 0) return 0;
   std::vector<float4> t(n);
-  if (dii || aii) {
+  if (dii) {
     SPHX_HIP(hipMemcpy(t.data(), c->dii.p, n * sizeof(float4), hipMemcpyDeviceToHost));
-    for (int i = 0; i < n; ++i) {
-      if (dii) { dii[3 * i] = t[i].x; dii[3 * i + 1] = t[i].y; dii[3 * i + 2] = t[i].z; }
-      if (aii) aii[i] = t[i].w;
-    }
+    for (int i = 0; i < n; ++i) { dii[3 * i] = t[i].x; dii[3 * i + 1] = t[i].y; dii[3 * i + 2] = t[i].z; }
+  }
+  if (aii) {
+    SPHX_HIP(hipMemcpy(t.data(), c->cst.p, n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) aii[i] = t[i].x;
   }
   if (dijpj) {
     SPHX_HIP(hipMemcpy(t.data(), c->dijpj.p, n * sizeof(float4), hipMemcpyDeviceToHost));
@@ -1676,13 +1704,11 @@ int sphx_upload_iisph(sphx_context* c, const float* dii, const float* dijpj, con
   SPHX_HIP(hipStreamSynchronize(c->stream));
   int n = c->num;
   if (n == 0) return 0;
-  std::vector<float4> t(n);
-  if (dii || aii) {
-    SPHX_HIP(hipMemcpy(t.data(), c->dii.p, n * sizeof(float4), hipMemcpyDeviceToHost));
-    for (int i = 0; i < n; ++i) {
-      if (dii) { t[i].x = dii[3 * i]; t[i].y = dii[3 * i + 1]; t[i].z = dii[3 * i + 2]; }
-      if (aii) t[i].w = aii[i];
-    }
+  std::vector<float4> t(n), ph(n);
+  SPHX_HIP(hipMemcpy(ph.data(), c->PHYS(), n * sizeof(float4), hipMemcpyDeviceToHost));
+  if (dii) {
+    // the DP pass's record {d_ii, m}
+    for (int i = 0; i < n; ++i) t[i] = make_float4(dii[3 * i], dii[3 * i + 1], dii[3 * i + 2], ph[i].x);
     SPHX_HIP(hipMemcpy(c->dii.p, t.data(), n * sizeof(float4), hipMemcpyHostToDevice));
   }
   if (dijpj) {
@@ -1690,14 +1716,24 @@ int sphx_upload_iisph(sphx_context* c, const float* dii, const float* dijpj, con
     SPHX_HIP(hipMemcpy(c->dijpj.p, t.data(), n * sizeof(float4), hipMemcpyHostToDevice));
   }
   if (rho_adv) SPHX_HIP(hipMemcpy(c->rho_adv.p, rho_adv, n * sizeof(float), hipMemcpyHostToDevice));
+  if (aii || rho_adv) {
+    // the Jacobi pass's constants {a_ii, rho0 - rho_adv, m / rho^2, rho0}
+    std::vector<float> ra(n);
+    SPHX_HIP(hipMemcpy(t.data(), c->cst.p, n * sizeof(float4), hipMemcpyDeviceToHost));
+    SPHX_HIP(hipMemcpy(ra.data(), c->rho_adv.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+      const float4 q = ph[i];
+      t[i] = make_float4(aii ? aii[i] : t[i].x, q.w - ra[i], q.x / q.y / q.y, q.w);
+    }
+    SPHX_HIP(hipMemcpy(c->cst.p, t.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  }
   if (press_l) {
     SPHX_HIP(hipMemcpy(c->press_l.p, press_l, n * sizeof(float), hipMemcpyHostToDevice));
     // keep s_j = m_j p_j / rho_j^2 consistent with the uploaded pressures
     NbrList L;
     SPHX_TRY(nbr_list(c, &L));
     SPHX_HIP(hipStreamSynchronize(c->stream));
-    std::vector<float4> ph(n), s(n);
-    SPHX_HIP(hipMemcpy(ph.data(), c->PHYS(), n * sizeof(float4), hipMemcpyDeviceToHost));
+    std::vector<float4> s(n);
     SPHX_HIP(hipMemcpy(s.data(), c->ps.p, n * sizeof(float4), hipMemcpyDeviceToHost));
     std::vector<float4> pos(n);
     SPHX_HIP(hipMemcpy(pos.data(), c->POS(), n * sizeof(float4), hipMemcpyDeviceToHost));

@@ -34,8 +34,9 @@ __global__ __launch_bounds__(256) void k_mlp_features(
     KP k, NbrList L, MLPP mp, const uint32_t* __restrict__ key, const int* __restrict__ cs,
     const float4* __restrict__ pos, const float4* __restrict__ veleval,
     const float4* __restrict__ phys, const float4* __restrict__ dii,
-    const float* __restrict__ rho_adv, const float4* __restrict__ alpha,
-    const float4* __restrict__ hist, float* __restrict__ X, float* __restrict__ pprev) {
+    const float4* __restrict__ cst, const float* __restrict__ rho_adv,
+    const float4* __restrict__ alpha, const float4* __restrict__ hist, float* __restrict__ X,
+    float* __restrict__ pprev) {
   int i = xcd_index();
   if (i >= k.n) return;
   float* x = X + (size_t)i * MLP_NF;
@@ -60,7 +61,7 @@ __global__ __launch_bounds__(256) void k_mlp_features(
     dist_rinv(dsq, r, ir);
     fa.add(k, dx, dy, dz, r, ir, pj, o.q, p_i);
   });
-  feat_write(k, mp, fa, qi, rho_adv[i], di, di.w, veleval[i], alpha, hist, i, x);
+  feat_write(k, mp, fa, qi, rho_adv[i], di, cst[i].x, veleval[i], alpha, hist, i, x);
 }
 
 // Hidden units per chunk of the training kernels' fused layers.
@@ -792,7 +793,7 @@ int mlp_begin(sphx_context* c) {
     timer_begin(c, "mlp_features");
     k_mlp_features<<<blocks_for(k.n, c->tune[T_MLP_FEAT]), c->tune[T_MLP_FEAT], 0, c->stream>>>(
         k, L, mp, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p,
-        c->rho_adv.p, mf ? c->mf_alpha[c->cur].p : nullptr, hist, m.X.p, m.pprev.p);
+        c->cst.p, c->rho_adv.p, mf ? c->mf_alpha[c->cur].p : nullptr, hist, m.X.p, m.pprev.p);
     timer_end(c);
   } else if (!(m.warm || record)) {
     k_mlp_pprev<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(k.n, c->PHYS(), m.pprev.p);
@@ -1148,7 +1149,7 @@ int sphx_mlp_features(sphx_context* c, float* X, float* pprev, float* dp_last) {
   SPHX_TRY(m.pprev.alloc((size_t)c->cap));
   k_mlp_features<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
       k, L, make_mlpp(c), c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p,
-      c->rho_adv.p, c->nphase > 0 ? c->mf_alpha[c->cur].p : nullptr, m.hist[c->cur].p, m.X.p,
+      c->cst.p, c->rho_adv.p, c->nphase > 0 ? c->mf_alpha[c->cur].p : nullptr, m.hist[c->cur].p, m.X.p,
       m.pprev.p);
   SPHX_HIP(hipGetLastError());
   SPHX_HIP(hipStreamSynchronize(c->stream));

@@ -79,18 +79,21 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_drift(
     dv[f] = f3(0.f, 0.f, 0.f);
     ag[f] = f3(0.f, 0.f, 0.f);
   }
+  // Branch-free loop over every slot of each batch: a boundary neighbour
+  // (V_j < 0) or a slot past the count gets an infinite distance, which
+  // the support test turns into a zero gradient.  Every x_ij below is
+  // multiplied by the gradient factor cm, so both are taken in the stored
+  // position units (grad W = x_u g, solve_gradf).
+  const SolveGrad sg = solve_grad(k, true);
   struct Rec { float4 r, a; };
-  for_each_listed_b(k, L, cs, gc, i, [&](int j) { return Rec{rec[j], reca[j]}; },
-                    [&](int, const Rec& o) {
+  for_each_listed_m<NBR_BATCH, true>(
+      k, L, cs, gc, i, [&](uint32_t j) { return Rec{load_at(rec, j * 16u), load_at(reca, j * 16u)}; },
+      [&](uint32_t, const Rec& o, bool valid) {
     const float4 rj = o.r;
-    if (rj.w < 0.f) return;                                    // boundary
-    float dx = (pi.x - rj.x) * k.ss, dy = (pi.y - rj.y) * k.ss, dz = (pi.z - rj.z) * k.ss;
-    float dsq = dx * dx + dy * dy + dz * dz;
-    if (!(dsq < 4.f * k.h2 && dsq > 0.f)) return;
-    float r, ir;
-    dist_rinv(dsq, r, ir);
+    const float dx = pi.x - rj.x, dy = pi.y - rj.y, dz = pi.z - rj.z;
+    const float dsq = (valid && rj.w >= 0.f) ? pair_dsq(dx, dy, dz) : INFINITY;
     const float4 aj = o.a;                                     // alpha_j, p_j
-    float cm = cubic_gradf(r, ir, k.inv_h, k.kspline) * rj.w;  // V_j f(r)
+    const float cm = solve_gradf(sg, dsq) * rj.w;              // V_j f(r)
     float pg[3];
     float3 pgs = f3(0.f, 0.f, 0.f);
 #pragma unroll
@@ -183,24 +186,24 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_alpha(
     ui[f] = f < NP ? xyz(dr[f][i]) : f3(0.f, 0.f, 0.f);
   }
   float fx = 0.f, fy = 0.f, fz = 0.f;
+  // Branch-free as the drift loop: boundary neighbours and slots past the
+  // count get an infinite distance (zero gradient); every x_ij is
+  // multiplied by cm, so both are in the stored position units.
+  const SolveGrad sg = solve_grad(k, true);
   struct Rec { float4 r, v, u[NP]; };
-  for_each_listed_b(k, L, cs, gc, i, [&](int j) {
+  for_each_listed_m<NBR_BATCH, false>(k, L, cs, gc, i, [&](uint32_t j) {
     Rec o;
-    o.r = rec[j];
-    o.v = vel[j];
+    o.r = load_at(rec, j * 16u);
+    o.v = load_at(vel, j * 16u);
 #pragma unroll
-    for (int f = 0; f < NP; ++f) o.u[f] = dr[f][j];
+    for (int f = 0; f < NP; ++f) o.u[f] = load_at(dr[f], j * 16u);
     return o;
-  }, [&](int, const Rec& o) {
+  }, [&](uint32_t, const Rec& o, bool valid) {
     const float4 rj = o.r;
-    if (rj.w < 0.f) return;                                              // boundary
-    float dx = (pi.x - rj.x) * k.ss, dy = (pi.y - rj.y) * k.ss, dz = (pi.z - rj.z) * k.ss;
-    float dsq = dx * dx + dy * dy + dz * dz;
-    if (!(dsq < 4.f * k.h2 && dsq > 0.f)) return;
-    float r, ir;
-    dist_rinv(dsq, r, ir);
+    const float dx = pi.x - rj.x, dy = pi.y - rj.y, dz = pi.z - rj.z;
+    const float dsq = (valid && rj.w >= 0.f) ? pair_dsq(dx, dy, dz) : INFINITY;
     const float4 vj = o.v;
-    float cm = cubic_gradf(r, ir, k.inv_h, k.kspline) * rj.w;
+    const float cm = solve_gradf(sg, dsq) * rj.w;
     float vmrd = (vj.x - vi.x) * dx + (vj.y - vi.y) * dy + (vj.z - vi.z) * dz;
 #pragma unroll
     for (int f = 0; f < NP; ++f) {
@@ -303,22 +306,21 @@ __global__ __launch_bounds__(MF_THREADS) void k_mf_force(
   float fx = 0.f, fy = 0.f, fz = 0.f;
   struct Rec { float4 r, u[3]; };
   const bool three = m.nphase > 2;
-  for_each_listed_b(k, L, cs, gc, i, [&](int j) {
+  // the fraction-advection loop's arithmetic (branch-free, stored position
+  // units), so its fused drift stress and this pass agree to rounding
+  const SolveGrad sg = solve_grad(k, true);
+  for_each_listed_m<NBR_BATCH, true>(k, L, cs, gc, i, [&](uint32_t j) {
     Rec o;
-    o.r = rec[j];
-    o.u[0] = drift0[j];
-    o.u[1] = drift1[j];
-    o.u[2] = three ? drift2[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    o.r = load_at(rec, j * 16u);
+    o.u[0] = load_at(drift0, j * 16u);
+    o.u[1] = load_at(drift1, j * 16u);
+    o.u[2] = three ? load_at(drift2, j * 16u) : make_float4(0.f, 0.f, 0.f, 0.f);
     return o;
-  }, [&](int, const Rec& o) {
+  }, [&](uint32_t, const Rec& o, bool valid) {
     const float4 rj = o.r;
-    if (rj.w < 0.f) return;                                              // boundary
-    float dx = (pi.x - rj.x) * k.ss, dy = (pi.y - rj.y) * k.ss, dz = (pi.z - rj.z) * k.ss;
-    float dsq = dx * dx + dy * dy + dz * dz;
-    if (!(dsq < 4.f * k.h2 && dsq > 0.f)) return;
-    float r, ir;
-    dist_rinv(dsq, r, ir);
-    float cm = cubic_gradf(r, ir, k.inv_h, k.kspline) * rj.w;
+    const float dx = pi.x - rj.x, dy = pi.y - rj.y, dz = pi.z - rj.z;
+    const float dsq = (valid && rj.w >= 0.f) ? pair_dsq(dx, dy, dz) : INFINITY;
+    const float cm = solve_gradf(sg, dsq) * rj.w;
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
       if (f >= m.nphase) break;

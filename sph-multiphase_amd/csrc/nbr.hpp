@@ -46,6 +46,10 @@ constexpr int NBR_GROUP = 8;                 // entries per 16-byte group
 constexpr int NBR_TBITS = 14;                // offset bits of an entry
 constexpr uint32_t NBR_TMAX = (1u << NBR_TBITS) - 1;
 
+// Count of a boundary particle or one outside the grid: it has no list
+// (the loops that skip such particles read only the count).
+constexpr int NBR_INVALID = -2;
+
 __host__ __device__ __forceinline__ size_t nbr_gslot(int i, int g, int capg) {
   return ((size_t)(i / NBR_SLICE) * capg + g) * NBR_SLICE + (i % NBR_SLICE);
 }
@@ -206,6 +210,50 @@ __device__ __forceinline__ void for_each_listed_m(const KP& k, const NbrList& L,
       if (j != i) use((uint32_t)j, load((uint32_t)j), true);
     });
   }
+}
+
+// Cubic-spline kernel gradient for branch-free neighbour loops, in the
+// units of the stored positions: with x_u = x_i - x_j before the simscale
+// s, the reference's grad W_ij = x_u s g(q) kspline / (h r), r = s |x_u|,
+// q = r / h (fluid_system_kern.cu:637-671), is x_u g(q) kspline / (h |x_u|):
+// s cancels except in q.  g is -3q + 2.25q^2 (q <= 1) or -0.75(2 - q)^2,
+// with kspline / h folded into the coefficients; the support test
+// 0 < |x_u|^2 < lim / s^2 is one unsigned compare of the bit patterns
+// (|x_u|^2 >= 0; NaN and +inf fail it).  Agrees with cubic_gradf to rounding.
+struct SolveGrad {
+  float q_of_r;       // s / h
+  float co, ca, cb;   // -0.75, -3, 2.25 times kspline / h
+  uint32_t lim_m1;    // bits of lim / s^2, minus 1
+};
+
+// support 2h when `wide`, else h
+__device__ __forceinline__ SolveGrad solve_grad(const KP& k, bool wide) {
+  const float ks = k.kspline * k.inv_h;
+  const float lim = (wide ? 4.f * k.h2 : k.h2) / (k.ss * k.ss);
+  return SolveGrad{k.ss * k.inv_h, -0.75f * ks, -3.f * ks, 2.25f * ks, __float_as_uint(lim) - 1u};
+}
+// the support of the IISPH loops (SolverOptions::full_support)
+__device__ __forceinline__ SolveGrad solve_grad(const KP& k) {
+  return solve_grad(k, k.full_support != 0);
+}
+
+// |x_u|^2 with the FMAs written out (the same bits in every kernel).
+__device__ __forceinline__ float pair_dsq(float dx, float dy, float dz) {
+  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// f with grad W = x_u f; 0 outside the support (dsq_u = +inf: an unused slot).
+// No contraction inside (FMAs only where written), so every kernel that
+// inlines it computes the same bits.
+__device__ __forceinline__ float solve_gradf(const SolveGrad& s, float dsq_u) {
+#pragma clang fp contract(off)
+  const float ir = __builtin_amdgcn_rsqf(dsq_u);
+  const float q = dsq_u * ir * s.q_of_r;
+  const float t = 2.f - q;
+  const float outer = s.co * (t * t);
+  const float inner = q * fmaf(s.cb, q, s.ca);
+  const float g = (q > 1.f ? outer : inner) * ir;
+  return (__float_as_uint(dsq_u) - 1u) < s.lim_m1 ? g : 0.f;
 }
 
 // Writer side of the packed list: entries are shifted in at the top of a

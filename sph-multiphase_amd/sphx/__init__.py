@@ -645,6 +645,15 @@ class Context:
     def graph_build(self, budget):
         check(self.L.sphx_graph_build(self.h, budget), "sphx_graph_build")
 
+    def tune_cache_save(self, path):
+        check(self.L.sphx_tune_cache_save(self.h, str(path).encode()), "sphx_tune_cache_save")
+
+    def tune_cache_load(self, path):
+        n = C.c_int()
+        check(self.L.sphx_tune_cache_load(self.h, str(path).encode(), C.byref(n)),
+              "sphx_tune_cache_load")
+        return n.value
+
     def persistent_solve(self, on=True):
         check(self.L.sphx_persistent_solve(self.h, 1 if on else 0), "sphx_persistent_solve")
 

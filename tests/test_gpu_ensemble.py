@@ -138,3 +138,38 @@ def test_block_sizes_leave_results_unchanged_and_autotune(gpu):
         assert set(t.values()) <= {64, 128, 256}
         B.graph_step()
         B.sync()
+
+
+def test_tune_cache_round_trip(gpu, tmp_path):
+    """The autotuner's JSON cache, keyed by (architecture, size bucket =
+    floor(log2 particles), kernel): a context of the same bucket reads back
+    the saved sizes, one of another bucket reads none, and saving from it
+    keeps the first bucket's entries."""
+    import json
+    path = tmp_path / "tune.json"
+    p, disp, calc, alpha, first, info = sphx.scene_ensemble([3000, 4000], seed=9)
+    n = len(disp)
+    picks = {k: (64, 128, 256)[t % 3] for t, k in enumerate(sphx.Context.TUNE_KERNELS)}
+    with make(p, disp, calc, alpha, 2) as A:
+        assert A.tune_cache_load(path) == 0          # no file yet
+        for k, b in picks.items():
+            A.tune_set(k, b)
+        A.tune_cache_save(path)
+    with make(p, disp, calc, alpha, 2) as B:
+        assert B.tune_cache_load(path) == len(picks)
+        assert B.tuning() == picks
+    entries = json.loads(path.read_text())
+    bucket = int(np.floor(np.log2(n)))
+    assert {e["bucket"] for e in entries} == {bucket}
+    assert all(e["arch"].startswith("gfx") for e in entries)
+    # another size bucket: nothing applies; its save adds its own entries
+    p2, d2, c2, a2, _, _ = sphx.scene_ensemble([9000, 9000], seed=9)
+    assert int(np.floor(np.log2(len(d2)))) != bucket
+    with make(p2, d2, c2, a2, 2) as C:
+        before = C.tuning()
+        assert C.tune_cache_load(path) == 0
+        assert C.tuning() == before
+        C.tune_cache_save(path)
+    entries = json.loads(path.read_text())
+    assert len(entries) == 2 * len(picks)
+    assert {e["bucket"] for e in entries} == {bucket, int(np.floor(np.log2(len(d2))))}

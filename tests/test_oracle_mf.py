@@ -237,3 +237,133 @@ def test_drift_diffusion_term_closed_form():
     scale = np.abs(want).max()
     assert scale > 0
     assert np.abs(got - want).max() <= 1e-9 * scale
+
+
+# -- fraction advection: exact moment identity for linear fields -------------
+# mfComputeAlphaAdvance (multiphase.cu:169-272) changes alpha_k by
+#   dt * -sum_j V_j [ (alpha_k,j + alpha_k,i) / 2 (v_j - v_i)
+#                     + (alpha_k,j u_k,j + alpha_k,i u_k,i) ] . gradW_ij.
+# For fields linear in x (alpha_k = a + g.x, v = A x + v0, u_k = B x + u0)
+# and D = x_j - x_i this is, for ANY particle arrangement, a combination of
+# the geometric moments G = sum V gradW, M_ab = sum V gradW_a D_b and
+# T_abc = sum V gradW_a D_b D_c (all summed by brute force here):
+#   term1 = alpha_i A:M + 1/2 A_ab g_c T_abc
+#   term2 = 2 alpha_i u_i.G + alpha_i B:M + u_i,a g_c M_ac + B_ab g_c T_abc
+# (G != 0 on a jittered lattice, so the alpha_i u_i part is pinned too;
+# physically -(term1 + term2) -> -(alpha div v + div(alpha u)) as M -> I).
+
+def test_alpha_advection_moment_identity():
+    n, s, h = 9, 0.01, 0.012
+    p, disp, calc, info = sphx.scene_dambreak(n, n, n, s, (n * s,) * 3, h, 0, 0.0, 1, 1)
+    rng = np.random.default_rng(4)
+    disp["pos"] += rng.uniform(-0.2, 0.2, disp["pos"].shape) * s
+    p.dt = 1e-3
+    o = options(wcsph_kernel=1, full_support=1)
+    st = orc.State(disp, calc, p, np.float64)
+    st.opts = o
+    st.grid_build()
+    assert np.all(st["type"] == 0) and np.all(st["mgcell"] != UNDEF)
+    st.a["mass"][:] = 1e-3 * rng.uniform(0.9, 1.1, st.n)
+    st.a["dens"][:] = 1003.0 * rng.uniform(0.98, 1.02, st.n)
+    x = st["pos"].astype(np.float64)
+    V = st["mass"] / st["dens"]
+    hh, ks = float(p.smoothradius), float(p.kspline)
+    G = np.zeros((st.n, 3)); M = np.zeros((st.n, 3, 3)); T = np.zeros((st.n, 3, 3, 3))
+    for i in range(st.n):
+        e = x[i] - x                          # x_i - x_j
+        r = np.sqrt((e * e).sum(1))
+        nb = (r > 0) & (r < 2 * hh)
+        gw = (cubic_dwdr(r[nb], hh, ks) / r[nb])[:, None] * e[nb] * V[nb, None]   # V_j gradW_ij
+        D = -e[nb]
+        G[i] = gw.sum(0)
+        M[i] = np.einsum("ja,jb->ab", gw, D)
+        T[i] = np.einsum("ja,jb,jc->abc", gw, D, D)
+    assert np.abs(G).max() > 1e-3 * np.abs(M).max()     # the jitter breaks the symmetry
+    x0 = x.mean(0)
+    g1 = np.array([1.5, -2.0, 0.8])
+    a1 = 0.35 + (x - x0) @ g1
+    alpha = np.stack([1.0 - a1, a1], 1)
+    A = np.array([[0.3, 0.1, -0.2], [0.05, -0.4, 0.1], [0.2, 0.0, 0.25]])
+    st.a["vel"][:] = (x - x0) @ A.T + np.array([0.01, -0.02, 0.03])
+    B = [np.array([[0.02, 0.0, 0.01], [0.0, -0.03, 0.0], [0.01, 0.0, 0.02]]),
+         np.array([[0.1, -0.05, 0.0], [0.02, 0.3, 0.04], [0.0, 0.06, -0.15]])]
+    u0 = [np.array([0.001, -0.002, 0.0]), np.array([-0.01, 0.05, 0.02])]
+    rho = np.array(RHO)
+    mf = orc.MF(st, RHO, NU, o)
+    mf.set_alpha(alpha, alpha @ rho)
+    for k in range(2):
+        mf.a["drift"][:, k, :] = u0[k] + (x - x0) @ B[k].T
+    mf.alpha_advance()
+    gks = [-g1, g1]
+    dt = float(p.dt)
+    for k in range(2):
+        ak = alpha[:, k]
+        u = u0[k] + (x - x0) @ B[k].T
+        t1 = ak * np.einsum("ab,iab->i", A, M) + 0.5 * np.einsum("ab,c,iabc->i", A, gks[k], T)
+        t2 = (2 * ak * (u * G).sum(1) + ak * np.einsum("ab,iab->i", B[k], M)
+              + np.einsum("ia,c,iac->i", u, gks[k], M) + np.einsum("ab,c,iabc->i", B[k], gks[k], T))
+        want = ak - dt * (t1 + t2)
+        got = mf["alpha_new"][:, k]
+        d = np.abs(got - want).max() / np.abs(t1 + t2).max() / dt
+        print(f"phase {k}: relative error of the change {d:.2e}")
+        assert d < 1e-9, k
+
+
+# -- drift stress: exact moment identity for linear drifts --------------------
+# contributeForce_projectu's drift-stress term (multiphase.cu:555-562) adds
+#   sum_j sum_k rho_k / rho_i V_j [alpha_k,j (u_k,j . gradW_ij) u_k,j
+#                                  + alpha_k,i (u_k,i . gradW_ij) u_k,i].
+# For uniform fractions and linear drifts u_k = u_k,i + B_k D (D = x_j - x_i)
+# this is, for any arrangement, with the moments of the advection test:
+#   sum_k rho_k / rho_i alpha_k [2 (u.G) u + (B:M) u + B M^T u + Q],
+#   Q_a = B_bc B_ad T_bcd.
+def test_drift_stress_moment_identity():
+    n, s, h = 9, 0.01, 0.012
+    p, disp, calc, info = sphx.scene_dambreak(n, n, n, s, (n * s,) * 3, h, 0, 0.0, 1, 1)
+    rng = np.random.default_rng(8)
+    disp["pos"] += rng.uniform(-0.2, 0.2, disp["pos"].shape) * s
+    p.dt = 1e-3
+    o = options(wcsph_kernel=1, full_support=1, boussinesq=0)
+    st = orc.State(disp, calc, p, np.float64)
+    st.opts = o
+    st.grid_build()
+    st.a["mass"][:] = 1e-3 * rng.uniform(0.9, 1.1, st.n)
+    st.a["dens"][:] = 1003.0 * rng.uniform(0.98, 1.02, st.n)
+    st.a["veleval"][:] = 0.0
+    x = st["pos"].astype(np.float64)
+    V = st["mass"] / st["dens"]
+    hh, ks = float(p.smoothradius), float(p.kspline)
+    G = np.zeros((st.n, 3)); M = np.zeros((st.n, 3, 3)); T = np.zeros((st.n, 3, 3, 3))
+    for i in range(st.n):
+        e = x[i] - x
+        r = np.sqrt((e * e).sum(1))
+        nb = (r > 0) & (r < 2 * hh)
+        gw = (cubic_dwdr(r[nb], hh, ks) / r[nb])[:, None] * e[nb] * V[nb, None]
+        D = -e[nb]
+        G[i] = gw.sum(0)
+        M[i] = np.einsum("ja,jb->ab", gw, D)
+        T[i] = np.einsum("ja,jb,jc->abc", gw, D, D)
+    x0 = x.mean(0)
+    alpha = np.tile([0.7, 0.3], (st.n, 1))
+    rho = np.array(RHO)
+    B = [np.array([[0.02, 0.0, 0.01], [0.0, -0.03, 0.0], [0.01, 0.0, 0.02]]),
+         np.array([[0.1, -0.05, 0.0], [0.02, 0.3, 0.04], [0.0, 0.06, -0.15]])]
+    u0 = [np.array([0.001, -0.002, 0.0]), np.array([-0.01, 0.05, 0.02])]
+    mf = orc.MF(st, RHO, NU, o)
+    mf.set_alpha(alpha, alpha @ rho)
+    mf.a["alpha_new"][:, :2] = alpha
+    mf.a["alpha_new"][:, 3] = alpha @ rho
+    for k in range(2):
+        mf.a["drift"][:, k, :] = u0[k] + (x - x0) @ B[k].T
+    mf.force()
+    want = np.zeros((st.n, 3))
+    for k in range(2):
+        u = u0[k] + (x - x0) @ B[k].T
+        Bk = B[k]
+        term = (2 * (u * G).sum(1)[:, None] * u + np.einsum("ab,iab->i", Bk, M)[:, None] * u
+                + np.einsum("ad,ibd,ib->ia", Bk, M, u) + np.einsum("bc,ad,ibcd->ia", Bk, Bk, T))
+        want += (rho[k] / st["dens"])[:, None] * alpha[:, k:k + 1] * term
+    got = mf["amf"]
+    d = np.abs(got - want).max() / np.abs(want).max()
+    print(f"drift stress: relative error {d:.2e}")
+    assert d < 1e-9
