@@ -294,6 +294,7 @@ struct sphx_context {
     sphx::DevBuf<int> gstate;           // [0] skip probe, [1] restore learned, [2] probes, [3] plain picks
     sphx::DevBuf<float> pkeep;          // the learned start's first iterate
     sphx::DevBuf<float> w, m, v, g;
+    sphx::DevBuf<float> gpart;          // per-workgroup weight gradients
     sphx::DevBuf<uint16_t> w16;         // FP16 bits of w
     sphx::DevBuf<uint4> wmf;            // w16 in the MFMA operand layout
     bool wmf_ok = false;                // wmf packed from the current w16

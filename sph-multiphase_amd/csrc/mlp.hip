@@ -458,14 +458,16 @@ __global__ __launch_bounds__(256) void k_mlp_fwdbwd(
 // Weight gradients over the batch: grad[e] += sum_s A_s[a] * B_s[b] for
 // every entry e (W1: X x dH1, W2: H1 x dH2, W3: H2 x dY, biases: sum dH).
 // A block takes a chunk of samples, stages them in LDS 32 at a time, each
-// thread accumulates ~24 entries in registers, then adds them to grad.
+// thread accumulates ~24 entries in registers, then writes them to its
+// block's row of gpart; k_mlp_gsum adds the rows in block order, so a
+// training run gives the same weights every time.
 constexpr int WG_SUB = 32;
 constexpr int WG_PER = (MLP_NP + 255) / 256;
 
 __global__ __launch_bounds__(256) void k_mlp_wgrad(
     const float* __restrict__ Xall, const int* __restrict__ perm, int base, int nb, int chunk,
     const float* __restrict__ H1, const float* __restrict__ dH1, const float* __restrict__ H2,
-    const float* __restrict__ dH2, const float* __restrict__ dY, float* __restrict__ grad) {
+    const float* __restrict__ dH2, const float* __restrict__ dY, float* __restrict__ gpart) {
   __shared__ float sX[WG_SUB][MLP_NF], sH1[WG_SUB][MLP_H], sdH1[WG_SUB][MLP_H],
       sH2[WG_SUB][MLP_H], sdH2[WG_SUB][MLP_H], sdY[WG_SUB][MLP_NO];
   float acc[WG_PER];
@@ -520,8 +522,17 @@ __global__ __launch_bounds__(256) void k_mlp_wgrad(
   }
   for (int u = 0; u < WG_PER; ++u) {
     int e = threadIdx.x + 256 * u;
-    if (e < MLP_NP) atomicAdd(&grad[e], acc[u]);
+    if (e < MLP_NP) gpart[(size_t)blockIdx.x * MLP_NP + e] = acc[u];
   }
+}
+
+// grad[e] = sum over the wgrad blocks of gpart[b][e], in block order.
+__global__ void k_mlp_gsum(const float* __restrict__ gpart, int blocks, float* __restrict__ grad) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= MLP_NP) return;
+  float s = 0.f;
+  for (int b = 0; b < blocks; ++b) s += gpart[(size_t)b * MLP_NP + e];
+  grad[e] = s;
 }
 
 __global__ void k_mlp_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
@@ -649,8 +660,8 @@ static int mlp_batch_buffers(sphx_context* c, int batch) {
   return 0;
 }
 
-// Gradient of the loss over samples order[base .. base+nb) into m.g (not
-// zeroed here); loss sum added to m.loss[0].
+// Gradient of the loss over samples order[base .. base+nb) into m.g; loss
+// sum added to m.loss[0].
 static int mlp_grad_batch(sphx_context* c, const float* X, const float* T, const float* M,
                           const int* order, int base, int nb, int nvalid, float4 lw) {
   auto& m = c->mlp;
@@ -659,12 +670,16 @@ static int mlp_grad_batch(sphx_context* c, const float* X, const float* T, const
       X, T, M, order, base, nb, nvalid, m.w.p, lw, m.H1.p, m.dH1.p, m.H2.p, m.dH2.p, m.dY.p,
       m.loss.p);
   SPHX_HIP(hipGetLastError());
-  int blocks = std::min(512, std::max(1, nb / 64));
+  constexpr int WG_BLOCKS_MAX = 512;
+  int blocks = std::min(WG_BLOCKS_MAX, std::max(1, nb / 64));
   int chunk = (nb + blocks - 1) / blocks;
   chunk = (chunk + WG_SUB - 1) / WG_SUB * WG_SUB;
   blocks = (nb + chunk - 1) / chunk;
+  SPHX_TRY(m.gpart.alloc((size_t)WG_BLOCKS_MAX * MLP_NP));
   k_mlp_wgrad<<<blocks, 256, 0, c->stream>>>(X, order, base, nb, chunk, m.H1.p, m.dH1.p, m.H2.p,
-                                             m.dH2.p, m.dY.p, m.g.p);
+                                             m.dH2.p, m.dY.p, m.gpart.p);
+  SPHX_HIP(hipGetLastError());
+  k_mlp_gsum<<<blocks_for(MLP_NP, 256), 256, 0, c->stream>>>(m.gpart.p, blocks, m.g.p);
   SPHX_HIP(hipGetLastError());
   return 0;
 }
@@ -696,7 +711,6 @@ static int mlp_refresh_launch(sphx_context* c, __half* w16) {
   float4 l4 = make_float4(m.online_lw[0], m.online_lw[1], m.online_lw[2], m.online_lw[3]);
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   for (int t = 0; t < steps; ++t) {
-    SPHX_HIP(hipMemsetAsync(m.g.p, 0, MLP_NP * sizeof(float), c->stream));
     SPHX_TRY(mlp_grad_batch(c, m.sX.p, m.sT.p, m.sM.p, m.order.p, t * nb, nb, nb, l4));
     ++m.adam_t;
     float bc1 = 1.f - std::pow(b1, (float)m.adam_t), bc2 = 1.f - std::pow(b2, (float)m.adam_t);
@@ -1219,7 +1233,6 @@ int sphx_mlp_grad(sphx_context* c, const float* X, const float* T, const float* 
   SPHX_HIP(hipMemcpy(dt.p, T, (size_t)n * MLP_NO * sizeof(float), hipMemcpyHostToDevice));
   SPHX_HIP(hipMemcpy(dm.p, msk.data(), n * sizeof(float), hipMemcpyHostToDevice));
   SPHX_HIP(hipMemcpy(dord.p, ord.data(), n * sizeof(int), hipMemcpyHostToDevice));
-  SPHX_HIP(hipMemsetAsync(m.g.p, 0, MLP_NP * sizeof(float), c->stream));
   SPHX_HIP(hipMemsetAsync(m.loss.p, 0, sizeof(double), c->stream));
   float4 l4 = make_float4(lw[0], lw[1], lw[2], lw[3]);
   SPHX_TRY(mlp_grad_batch(c, dx.p, dt.p, dm.p, dord.p, 0, n, std::max(nvalid, 1), l4));
@@ -1263,8 +1276,7 @@ int sphx_mlp_train(sphx_context* c, int epochs, int batch, float lr, const float
     SPHX_HIP(hipMemsetAsync(m.loss.p, 0, sizeof(double), c->stream));
     for (int base = 0; base < nv; base += batch) {
       int nb = std::min(batch, nv - base);
-      SPHX_HIP(hipMemsetAsync(m.g.p, 0, MLP_NP * sizeof(float), c->stream));
-      SPHX_TRY(mlp_grad_batch(c, m.sX.p, m.sT.p, m.sM.p, m.order.p, base, nb, nb, l4));
+        SPHX_TRY(mlp_grad_batch(c, m.sX.p, m.sT.p, m.sM.p, m.order.p, base, nb, nb, l4));
       ++m.adam_t;
       float bc1 = 1.f - std::pow(b1, (float)m.adam_t), bc2 = 1.f - std::pow(b2, (float)m.adam_t);
       k_mlp_adam<<<blocks_for(MLP_NP, 256), 256, 0, c->stream>>>(
