@@ -353,7 +353,7 @@ __device__ __forceinline__ void dp_particle(const KP& k, const NbrList& L, const
   // one 16-byte gather per neighbour: {x_j, s_j}; boundary s_j = 0 adds
   // nothing, as the reference's skip of boundary j (:925-926)
   const SolveGrad sg = solve_grad(k);
-  for_each_listed_m<NBR_BATCH_SMALL, false>(
+  for_each_listed_m<NBR_BATCH, true>(
       k, L, a.cs, gc, i, [&](uint32_t j) { return load_at(ps, j * 16u); },
       [&](uint32_t, const float4& pj, bool valid) {
     const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
@@ -942,7 +942,7 @@ __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
     KP k, const uint32_t* __restrict__ key, const int* __restrict__ cs,
     const float4* __restrict__ pos, uint4* __restrict__ idx, int* __restrict__ cnt,
     float4* __restrict__ ps, float4* __restrict__ mqa, float2* __restrict__ mqb, int cap) {
-  __shared__ StagedPos spos[NB_STAGE];
+  __shared__ StagedPos spos[NB_STAGE + 3];   // + the tests past a range's end
   __shared__ int sh[7];           // row r0, row r1, key min/max of r0, of r1, bad
   __shared__ int rs[19], ro[19];  // range (row, neighbour row) begin; LDS offsets
   const int t = threadIdx.x, lane = t & 63;
@@ -1048,20 +1048,30 @@ __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
 #pragma unroll 1
       for (int j0 = jb; j0 < je; j0 += 32) {
         const int m = min(32, je - j0);
+        const int m4 = (m + 3) & ~3;
+        // Candidates are tested four at a time, up to 3 past the range.
+        // A test enters the mask at bit 0 and moves up, so candidate u
+        // ends at bit m4 - 1 - u; its bit is the sign of d2 - rc2u, which
+        // is d2 < rc2u exactly (the difference of two finite floats is
+        // negative iff the first is smaller), shifted in by v_alignbit.
         uint32_t hit = 0;
-#pragma unroll 4
-        for (int u = 0; u < m; ++u) {
-          const float4 pj = fetch(base + j0 + u);
-          const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-          hit |= (uint32_t)(dx * dx + dy * dy + dz * dz < rc2u) << u;
+        for (int u0 = 0; u0 < m4; u0 += 4) {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const float4 pj = fetch(base + j0 + u0 + v);
+            const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dx, dx, dy * dy));
+            hit = __builtin_amdgcn_alignbit(hit, __float_as_uint(d2 - rc2u), 31);
+          }
         }
+        hit >>= (uint32_t)(m4 - m);   // candidate u now at bit m - 1 - u
         // the particle itself is not its own neighbour
         const uint32_t self = (uint32_t)(i - j0);
-        if (self < (uint32_t)m) hit &= ~(1u << self);
+        if (self < (uint32_t)m) hit &= ~(1u << (m - 1 - (int)self));
         while (hit) {
-          const int u = __builtin_ctz(hit);
-          hit &= hit - 1;
-          add(j0 + u, dtag, sd);
+          const int b = 31 - __builtin_clz(hit);
+          hit ^= 1u << b;
+          add(j0 + (m - 1 - b), dtag, sd);
         }
       }
     }
@@ -1084,7 +1094,7 @@ __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
       });
     }
   } else if (valid) {
-    scan_rows([&](int) { return 0; }, [&](int j) { return pos[j]; });
+    scan_rows([&](int) { return 0; }, [&](int j) { return pos[min(j, k.n - 1)]; });
   }
   if (c <= cap && (c & (NBR_GROUP - 1))) {
     // zero entries pad the last group; they decode to S_0 and are never used

@@ -91,7 +91,7 @@ __device__ __forceinline__ int nbr_decode(uint32_t e, const NbrBase& b) {
 // the last written one are not read.  The visiting order, and so every
 // sum's order, is that of the list.
 constexpr int NBR_BATCH = 8;          // default batch (wide records)
-constexpr int NBR_BATCH_SMALL = 16;   // loops with few live values (density, DP)
+constexpr int NBR_BATCH_SMALL = 16;   // loops with few live values (density)
 
 template <int B = NBR_BATCH, bool PREFETCH = true, class Load, class Use>
 __device__ __forceinline__ void for_each_listed_b(const KP& k, const NbrList& L,
