@@ -622,8 +622,10 @@ int sphx_mlp_download_samples(sphx_context* ctx, int64_t start, int64_t count,
  * last solve [num] (feature 20); any may be NULL. */
 int sphx_mlp_features(sphx_context* ctx, float* X, float* pprev, float* dp_last);
 /* The feature buffer as the last pass left it: the a_ii pass fills it when
- * the network is in use (warm start or recording); sphx_mlp_features
- * recomputes it. */
+ * the network is in use (warm start or recording); with the warm start on
+ * and a recording stride above 1 it fills only the rows the recorder keeps
+ * (the others hold an earlier step's).  sphx_mlp_features recomputes every
+ * row. */
 int sphx_mlp_download_features(sphx_context* ctx, float* X, float* pprev);
 /* The network with FP16 weights and FP32 accumulation: Y [n][4]. */
 int sphx_mlp_eval(sphx_context* ctx, const float* X, int n, float* Y);

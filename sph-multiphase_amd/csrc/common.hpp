@@ -309,6 +309,9 @@ struct sphx_context {
     // (default priority) while the next step is solved, into staged FP16
     // weights that the step after that adopts
     bool feat_ok = false;        // X and pprev hold this step's features (a_ii pass)
+    bool start_ok = false;       // the a_ii pass also wrote the learned start
+    bool settled = false;        // this step's due refresh is adopted
+    bool fuse_start = true;      // evaluate the network inside the a_ii pass
     int online_async = 0;
     int train_pending = 0;       // solves until the staged weights are adopted
     bool train_launch = false;   // a refresh is due, its samples recorded (ev_main)

@@ -17,11 +17,12 @@
 //      residue = rho_adv + S_i + a_ii p_i(old) - rho0,   s_i <- m_i p_i(new)/rho_i^2
 // B touches p only at its own index, so the update is in place and the
 // reference's press_l1 -> press_l copy (host.cu:536) is not needed.
-#include "mlp_feat.hpp"
+#include "mlp_mfma.hpp"
 
 #include <hip/hip_cooperative_groups.h>
 
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace sphx {
@@ -210,81 +211,138 @@ __global__ void k_ii_density_dii(KP k, NbrList L, const uint32_t* __restrict__ k
 // ComputeAii (kern.cu:801-908): rho_adv, warm start p0 = w p(t-1), a_ii;
 // also the position-only sums G_i, H_i of the pressure pass, and, with
 // FEAT, the learned start's 24 features (the same map as k_mlp_features,
-// from the same neighbour records).
+// from the same neighbour records).  With START the pass then evaluates
+// the network on its own features: each wave runs its 64 particles' rows
+// from LDS on MFMA and writes the learned start (as k_mlp_infer would
+// from the stored rows, with the same bits).
 struct FeatOut {
   MLPP mp;
   const float4* alpha;   // fractions before the step, or null (one phase)
   const float4* hist;    // per particle: last solve's pressure increment in w
   float* X;              // [n][24]
   float* pprev;          // p(t-1)
+  int xstride, xphase;   // rows of X written: slots i with i % xstride == xphase
+};
+struct StartOut {
+  const uint4* wmf;      // the weights in the MFMA operand layout
+  int warm;              // write the learned start
+  float4* dcorr;         // drift correction for the next step, or null
+};
+struct AiiStartLds {
+  MlpMfmaShared S;
+  float xs[NB_THREADS * MLP_NF];            // the workgroup's feature rows
+  float ys[NB_THREADS / 64][64 * MLP_NO];   // network outputs per wave
 };
 
-template <bool FEAT>
+template <bool FEAT, bool START>
 __global__ __launch_bounds__(NB_THREADS) void k_ii_aii(KP k, NbrList L, const uint32_t* __restrict__ key, const int* __restrict__ cs,
                          const float4* __restrict__ pos, const float4* __restrict__ veleval,
                          const float4* __restrict__ phys, const float4* __restrict__ dii,
                          float* __restrict__ rho_adv, float* __restrict__ press_l,
                          float4* __restrict__ ps, float4* __restrict__ gh,
-                         float4* __restrict__ cst, FeatOut fo) {
-  int i = xcd_index();
-  if (i >= k.n) return;
-  float4 pi = pos[i];
-  uint32_t gc = key[i];
-  if (type_of(pi) == SPHX_TYPE_BOUNDARY || (int)gc == k.gridtotal) {
-    if (FEAT) {
-      fo.pprev[i] = phys[i].z;
-      float* x = fo.X + (size_t)i * MLP_NF;
-      for (int q = 0; q < MLP_NF; ++q) x[q] = 0.f;
-    }
-    return;
+                         float4* __restrict__ cst, FeatOut fo, StartOut so) {
+  static_assert(FEAT || !START, "the learned start needs the features");
+  __shared__ typename std::conditional<START, AiiStartLds, int>::type lds;
+  if constexpr (START) {
+    load_mfma(lds.S, so.wmf);
+    __syncthreads();
   }
-  const float4 vi = veleval[i];
-  const float4 qi = phys[i];
-  const float rest = qi.w, mi = qi.x, rhoi = qi.y;
-  const float4 di = dii[i];
-  const float cji = mi / rhoi / rhoi * k.dt * k.dt;
-  const float p_i = fmaxf(0.f, qi.z);
-  float dchange = 0.f, aii = 0.f;
-  float Gx = 0.f, Gy = 0.f, Gz = 0.f, H = 0.f;
-  FeatAcc fa(p_i);
-  struct Rec { float4 p, q, v; };
-  for_each_listed_b<NBR_BATCH, false>(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j], veleval[j]}; },
-                    [&](int, const Rec& o) {
-    const float4 pj = o.p, qj = o.q, vj = o.v;
-    float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
-    float dsq = dx * dx + dy * dy + dz * dz;
-    float r, ir;
-    dist_rinv(dsq, r, ir);
-    if (FEAT) fa.add(k, dx, dy, dz, r, ir, pj, qj, p_i);
-    if (!in_support(k, dsq)) return;
-    float gf = cubic_gradf(r, ir, k.inv_h, k.kspline);
-    float gx = dx * gf, gy = dy * gf, gz = dz * gf;
-    float vdg = (vi.x - vj.x) * gx + (vi.y - vj.y) * gy + (vi.z - vj.z) * gz;
-    // fluid j: weight m_j and d_ji = gradW m_i / rho_i^2 dt^2, so
-    // a_ii += (d_ii - d_ji) . gradW m_j; boundary j (when boundary pressure
-    // is on, :833-834): weight rho0_i V_j and no d_ji term; else nothing.
-    // Written with selects, one expression for both kinds.
-    const bool fl = type_of(pj) != SPHX_TYPE_BOUNDARY;
-    const float w = fl ? qj.x : (k.boundary_pressure ? rest * qj.y : 0.f);
-    const float wh = fl ? qj.x : 0.f;
-    const float cj = fl ? cji : 0.f;
-    dchange += vdg * w * k.dt;
-    float tx = di.x - gx * cj, ty = di.y - gy * cj, tz = di.z - gz * cj;
-    aii += (tx * gx + ty * gy + tz * gz) * w;
-    Gx += gx * w; Gy += gy * w; Gz += gz * w;
-    H += (gx * gx + gy * gy + gz * gz) * wh;
-  });
-  gh[i] = make_float4(Gx, Gy, Gz, H);
-  const float radv = rhoi + dchange;
-  rho_adv[i] = radv;
-  float p0 = k.warm * qi.z;
-  press_l[i] = p0;
-  ps[i].w = mi * p0 / rhoi / rhoi;
-  cst[i] = make_float4(aii, rest - radv, mi / rhoi / rhoi, rest);
-  if (FEAT) {
-    fo.pprev[i] = qi.z;
-    feat_write(k, fo.mp, fa, qi, radv, di, aii, vi, fo.alpha, fo.hist, i,
-               fo.X + (size_t)i * MLP_NF);
+  const int i = xcd_index();
+  bool fluid = false;
+  float4 qi = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (i < k.n) {
+    const float4 pi = pos[i];
+    const uint32_t gc = key[i];
+    qi = phys[i];
+    fluid = !(type_of(pi) == SPHX_TYPE_BOUNDARY || (int)gc == k.gridtotal);
+    if (!fluid) {
+      if (FEAT) {
+        fo.pprev[i] = qi.z;
+        if (i % fo.xstride == fo.xphase) {
+          float* x = fo.X + (size_t)i * MLP_NF;
+          for (int q = 0; q < MLP_NF; ++q) x[q] = 0.f;
+        }
+      }
+      if (START && so.dcorr) so.dcorr[i] = make_float4(0.f, 0.f, 0.f, so.dcorr[i].w);
+    } else {
+      const float4 vi = veleval[i];
+      const float rest = qi.w, mi = qi.x, rhoi = qi.y;
+      const float4 di = dii[i];
+      const float cji = mi / rhoi / rhoi * k.dt * k.dt;
+      const float p_i = fmaxf(0.f, qi.z);
+      float dchange = 0.f, aii = 0.f;
+      float Gx = 0.f, Gy = 0.f, Gz = 0.f, H = 0.f;
+      FeatAcc fa(p_i);
+      struct Rec { float4 p, q, v; };
+      for_each_listed_b<NBR_BATCH, false>(k, L, cs, gc, i, [&](int j) { return Rec{pos[j], phys[j], veleval[j]}; },
+                        [&](int, const Rec& o) {
+        const float4 pj = o.p, qj = o.q, vj = o.v;
+        float dx = (pi.x - pj.x) * k.ss, dy = (pi.y - pj.y) * k.ss, dz = (pi.z - pj.z) * k.ss;
+        float dsq = dx * dx + dy * dy + dz * dz;
+        float r, ir;
+        dist_rinv(dsq, r, ir);
+        if (FEAT) fa.add(k, dx, dy, dz, r, ir, pj, qj, p_i);
+        if (!in_support(k, dsq)) return;
+        float gf = cubic_gradf(r, ir, k.inv_h, k.kspline);
+        float gx = dx * gf, gy = dy * gf, gz = dz * gf;
+        float vdg = (vi.x - vj.x) * gx + (vi.y - vj.y) * gy + (vi.z - vj.z) * gz;
+        // fluid j: weight m_j and d_ji = gradW m_i / rho_i^2 dt^2, so
+        // a_ii += (d_ii - d_ji) . gradW m_j; boundary j (when boundary
+        // pressure is on, :833-834): weight rho0_i V_j and no d_ji term;
+        // else nothing.  Written with selects, one expression for both kinds.
+        const bool fl = type_of(pj) != SPHX_TYPE_BOUNDARY;
+        const float w = fl ? qj.x : (k.boundary_pressure ? rest * qj.y : 0.f);
+        const float wh = fl ? qj.x : 0.f;
+        const float cj = fl ? cji : 0.f;
+        dchange += vdg * w * k.dt;
+        float tx = di.x - gx * cj, ty = di.y - gy * cj, tz = di.z - gz * cj;
+        aii += (tx * gx + ty * gy + tz * gz) * w;
+        Gx += gx * w; Gy += gy * w; Gz += gz * w;
+        H += (gx * gx + gy * gy + gz * gz) * wh;
+      });
+      gh[i] = make_float4(Gx, Gy, Gz, H);
+      const float radv = rhoi + dchange;
+      rho_adv[i] = radv;
+      if (!(START && so.warm)) {   // else the learned start below
+        float p0 = k.warm * qi.z;
+        press_l[i] = p0;
+        ps[i].w = mi * p0 / rhoi / rhoi;
+      }
+      cst[i] = make_float4(aii, rest - radv, mi / rhoi / rhoi, rest);
+      if (FEAT) {
+        fo.pprev[i] = qi.z;
+        float x[MLP_NF];
+        feat_write(k, fo.mp, fa, qi, radv, di, aii, vi, fo.alpha, fo.hist, i, x);
+        if (i % fo.xstride == fo.xphase) {
+          float4* xg = reinterpret_cast<float4*>(fo.X + (size_t)i * MLP_NF);
+#pragma unroll
+          for (int q = 0; q < MLP_NF / 4; ++q)
+            xg[q] = make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
+        }
+        if constexpr (START) {
+          float4* xl = reinterpret_cast<float4*>(lds.xs + threadIdx.x * MLP_NF);
+#pragma unroll
+          for (int q = 0; q < MLP_NF / 4; ++q)
+            xl[q] = make_float4(x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]);
+        }
+      }
+    }
+  }
+  if constexpr (START) {
+    // Each wave evaluates its own 64 rows as soon as it has written them
+    // (one wave's LDS accesses complete in order; the weights were staged
+    // before the barrier above).
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    asm volatile("" ::: "memory");
+    if (i - lane < k.n) {   // the wave holds particles (the same for all lanes)
+      // rows of lanes past n or off the fluid are not written; each
+      // particle's outputs depend on its own row only
+      mlp_forward_mfma(lds.S, lds.xs + wave * 64 * MLP_NF, 64, lds.ys[wave]);
+      asm volatile("" ::: "memory");
+      if (fluid)
+        mlp_start_out(k, fo.mp, so.warm, i, lds.ys[wave] + lane * MLP_NO, qi, press_l, ps,
+                      so.dcorr);
+    }
   }
 }
 
@@ -1068,10 +1126,29 @@ __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
         // the particle itself is not its own neighbour
         const uint32_t self = (uint32_t)(i - j0);
         if (self < (uint32_t)m) hit &= ~(1u << (m - 1 - (int)self));
-        while (hit) {
-          const int b = 31 - __builtin_clz(hit);
-          hit ^= 1u << b;
-          add(j0 + (m - 1 - b), dtag, sd);
+        if (hit == 0u) continue;
+        // hits are appended in candidate order, so the last has the largest
+        // offset from the layer start: one overflow test per chunk
+        const int tlast = j0 + (m - 1 - __builtin_ctz(hit)) - sd;
+        ovf |= (uint32_t)tlast > NBR_TMAX;
+        if (c + __popc(hit) <= cap) {
+          // the chunk fits: no capacity test per hit.  Entry of the hit at
+          // bit b: dtag + (its offset), the offset below 2^14 unless the
+          // list overflowed (then it is not used)
+          const uint32_t top = dtag + (uint32_t)(j0 + m - 1 - sd);
+          while (hit) {
+            const int b = 31 - __builtin_clz(hit);
+            hit ^= 1u << b;
+            pk.push(top - (uint32_t)b);
+            if (((c + 1) & (NBR_GROUP - 1)) == 0) out[(size_t)(c / NBR_GROUP) * NBR_SLICE] = pk.get();
+            ++c;
+          }
+        } else {
+          while (hit) {
+            const int b = 31 - __builtin_clz(hit);
+            hit ^= 1u << b;
+            add(j0 + (m - 1 - b), dtag, sd);
+          }
         }
       }
     }
@@ -1105,6 +1182,7 @@ __global__ __launch_bounds__(NB_THREADS_BUILD) void k_nbr_build(
 }
 
 int grid_hash(sphx_context* c);
+int mlp_prepare_start(sphx_context* c);
 int grid_sort(sphx_context* c);
 int grid_reorder(sphx_context* c);
 int mlp_solve(sphx_context* c, int* iters, double* err);
@@ -1378,20 +1456,32 @@ int sphx_iisph_predict_aii(sphx_context* c) {
     SPHX_TRY(m.pprev.alloc((size_t)c->cap));
     fo = FeatOut{MLPP{m.p_scale, m.u_scale, m.drift_blend},
                  c->nphase > 0 ? c->mf_alpha[c->cur].p : nullptr, m.hist[c->cur].p, m.X.p,
-                 m.pprev.p};
+                 m.pprev.p, 1, 0};
+  }
+  // the learned start too, when the network is evaluated this step; the
+  // feature rows in memory are then only the recorder's input (the slots
+  // it keeps, with the phase it will use after this step's solve)
+  float4* dcorr = (feat && m.drift && c->nphase > 0) ? m.hist[c->cur].p : nullptr;
+  const bool start = feat && m.fuse_start && (m.warm || dcorr);
+  StartOut so{};
+  if (start) {
+    SPHX_TRY(mlp_prepare_start(c));
+    so = StartOut{m.wmf.p, m.warm, dcorr};
+    const int stride = std::max(1, m.record_stride);
+    fo.xstride = stride;
+    fo.xphase = stride > 1 ? (int)(m.solves % stride) : 0;
   }
   timer_begin(c, "iisph_aii");
-  if (feat)
-    k_ii_aii<true><<<blocks_for(k.n, c->tune[T_AII]), c->tune[T_AII], 0, c->stream>>>(
-        k, L, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p,
-        c->rho_adv.p, c->press_l.p, c->ps.p, c->gh.p, c->cst.p, fo);
-  else
-    k_ii_aii<false><<<blocks_for(k.n, c->tune[T_AII]), c->tune[T_AII], 0, c->stream>>>(
-        k, L, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p,
-        c->rho_adv.p, c->press_l.p, c->ps.p, c->gh.p, c->cst.p, fo);
+  auto* kern = start ? k_ii_aii<true, true> : (feat ? k_ii_aii<true, false> : k_ii_aii<false, false>);
+  // with the network: four waves share each staged copy of the weights
+  const int tb = start ? NB_THREADS : c->tune[T_AII];
+  kern<<<blocks_for(k.n, tb), tb, 0, c->stream>>>(
+      k, L, c->key.p, c->cell_start.p, c->POS(), c->VELEVAL(), c->PHYS(), c->dii.p,
+      c->rho_adv.p, c->press_l.p, c->ps.p, c->gh.p, c->cst.p, fo, so);
   timer_end(c);
   c->gh_valid = true;
   m.feat_ok = feat;
+  m.start_ok = start;
   SPHX_HIP(hipGetLastError());
   return 0;
 }

@@ -14,7 +14,7 @@
 // sample (FP32), k_mlp_wgrad reduces the weight gradients over the batch
 // in LDS tiles, k_mlp_adam applies Adam and refreshes the FP16 copy the
 // inference kernel reads (FP16 weights, FP32 accumulation).
-#include "mlp_feat.hpp"
+#include "mlp_mfma.hpp"
 
 #include <hip/hip_fp16.h>
 
@@ -22,10 +22,6 @@
 #include <random>
 
 namespace sphx {
-
-constexpr int MLP_W1 = 0, MLP_B1 = MLP_NF * MLP_H, MLP_W2 = MLP_B1 + MLP_H,
-              MLP_B2 = MLP_W2 + MLP_H * MLP_H, MLP_W3 = MLP_B2 + MLP_H,
-              MLP_B3 = MLP_W3 + MLP_H * MLP_NO, MLP_NP = MLP_B3 + MLP_NO;   // 6020
 
 // ------------------------------------------------------------- features --
 // Stand-alone feature pass (the a_ii pass computes the same map in its own
@@ -68,59 +64,6 @@ __global__ __launch_bounds__(256) void k_mlp_features(
 constexpr int MLP_CH = 8;
 
 // ------------------------------------------------- inference on MFMA --
-// One wave runs 64 particles as four 16-particle column tiles on
-// v_mfma_f32_16x16x32_f16 with every layer transposed (H^T = W^T X^T).  A
-// layer's accumulators (rows 4 (lane / 16) + i, column lane % 16) are then
-// the next layer's B operand (k = 8 (lane / 16) .. +8, same column) once the
-// K index within each 32-unit chunk is read as u(g, s) = s < 4 ? 4 g + s :
-// 16 + 4 g + s - 4; the weights are staged in that order.  Weights are the
-// network's FP16 values exactly; activations enter as two FP16 pieces
-// (x = hi + lo, two products each), so a layer's result is its FP32-input
-// value to about 2^-22, accumulated in FP32.
-typedef __attribute__((ext_vector_type(8))) _Float16 h8;
-typedef __attribute__((ext_vector_type(4))) float f4v;
-
-struct MlpMfmaShared {
-  uint4 a1[4][64];      // W1^T, M tile m: lane (r, g) holds W1[8 g + s][16 m + r]
-  uint4 a2[4][2][64];   // W2^T, M tile m, K chunk c: W2[32 c + u(g, s)][16 m + r]
-  uint4 a3[2][64];      // W3^T (rows 4..15 zero), K chunk c: W3[32 c + u(g, s)][r]
-  float b1[MLP_H], b2[MLP_H], b3[MLP_NO];
-};
-
-__device__ __forceinline__ int mfma_unit(int g, int s) { return s < 4 ? 4 * g + s : 16 + 4 * g + s - 4; }
-
-__device__ void stage_mfma(MlpMfmaShared& S, const __half* __restrict__ w16) {
-  const uint16_t* w = reinterpret_cast<const uint16_t*>(w16);
-  uint16_t* a1 = reinterpret_cast<uint16_t*>(&S.a1[0][0]);
-  uint16_t* a2 = reinterpret_cast<uint16_t*>(&S.a2[0][0][0]);
-  uint16_t* a3 = reinterpret_cast<uint16_t*>(&S.a3[0][0]);
-  for (int t = threadIdx.x; t < 4 * 64 * 8; t += blockDim.x) {
-    const int sl = t & 7, lane = (t >> 3) & 63, m = t >> 9;
-    const int r = lane & 15, g = lane >> 4, f = 8 * g + sl;
-    a1[t] = f < MLP_NF ? w[MLP_W1 + f * MLP_H + 16 * m + r] : (uint16_t)0;
-  }
-  for (int t = threadIdx.x; t < 4 * 2 * 64 * 8; t += blockDim.x) {
-    const int sl = t & 7, lane = (t >> 3) & 63, c = (t >> 9) & 1, m = t >> 10;
-    const int r = lane & 15, g = lane >> 4, u = 32 * c + mfma_unit(g, sl);
-    a2[t] = w[MLP_W2 + u * MLP_H + 16 * m + r];
-  }
-  for (int t = threadIdx.x; t < 2 * 64 * 8; t += blockDim.x) {
-    const int sl = t & 7, lane = (t >> 3) & 63, c = t >> 9;
-    const int r = lane & 15, g = lane >> 4, u = 32 * c + mfma_unit(g, sl);
-    a3[t] = r < MLP_NO ? w[MLP_W3 + u * MLP_NO + r] : (uint16_t)0;
-  }
-  for (int t = threadIdx.x; t < MLP_H; t += blockDim.x) {
-    S.b1[t] = __half2float(w16[MLP_B1 + t]);
-    S.b2[t] = __half2float(w16[MLP_B2 + t]);
-    if (t < MLP_NO) S.b3[t] = __half2float(w16[MLP_B3 + t]);
-  }
-}
-
-__device__ __forceinline__ h8 as_h8(uint4 v) { return __builtin_bit_cast(h8, v); }
-
-constexpr int MLP_MFMA_U4 = (int)(sizeof(MlpMfmaShared) / sizeof(uint4));
-static_assert(sizeof(MlpMfmaShared) % sizeof(uint4) == 0, "MFMA weight block is whole uint4s");
-
 // The MFMA operand layout of the current FP16 weights, built once per call
 // of the network (one workgroup) and copied whole by every inference
 // workgroup.
@@ -129,112 +72,8 @@ __global__ __launch_bounds__(256) void k_mlp_mfma_pack(const __half* __restrict_
   stage_mfma(*out, w16);
 }
 
-__device__ __forceinline__ void load_mfma(MlpMfmaShared& S, const uint4* __restrict__ wmf) {
-  uint4* dst = reinterpret_cast<uint4*>(&S);
-  for (int t = threadIdx.x; t < MLP_MFMA_U4; t += blockDim.x) dst[t] = wmf[t];
-}
-
-// x = hi + lo, both FP16
-__device__ __forceinline__ void split_h8(const float v[8], h8& hi, h8& lo) {
-#pragma unroll
-  for (int s = 0; s < 8; ++s) {
-    const _Float16 h = (_Float16)v[s];
-    hi[s] = h;
-    lo[s] = (_Float16)(v[s] - (float)h);
-  }
-}
-
-// Column tiles (of 16 particles) per pass of the forward: four passes of
-// one keep the live accumulators and operands at a quarter (142 VGPRs, three
-// waves per SIMD; two tiles per pass need 216, two waves).
-constexpr int MLP_TPP = 1;
-
-// Next layer's B operands from this layer's accumulators acc[m][t] (4 M
-// tiles of 16 units), with bias and ReLU.
-__device__ __forceinline__ void relu_to_b(const f4v acc[4][MLP_TPP], const float* bias, int g,
-                                          h8 bh[2][MLP_TPP], h8 bl[2][MLP_TPP]) {
-#pragma unroll
-  for (int c = 0; c < 2; ++c)
-#pragma unroll
-    for (int t = 0; t < MLP_TPP; ++t) {
-      float v[8];
-#pragma unroll
-      for (int s = 0; s < 8; ++s) {
-        const int m = 2 * c + (s < 4 ? 0 : 1), i = s & 3;
-        v[s] = fmaxf(acc[m][t][i] + bias[32 * c + mfma_unit(g, s)], 0.f);
-      }
-      split_h8(v, bh[c][t], bl[c][t]);
-    }
-}
-
-// Network outputs of particles base .. base + 63 (one wave) into y[64][4]
-// (LDS).  Particles past n repeat particle n - 1.
-__device__ void mlp_forward_mfma(const MlpMfmaShared& S, const float* __restrict__ X, int n,
-                                 int base, float* y) {
-  const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
-  const f4v zero = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
-  for (int t0 = 0; t0 < 4; t0 += MLP_TPP) {
-    h8 xh[MLP_TPP], xl[MLP_TPP];
-#pragma unroll
-    for (int t = 0; t < MLP_TPP; ++t) {
-      const int p = min(base + 16 * (t0 + t) + r, n - 1);
-      float v[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-      if (8 * g < MLP_NF) {
-        const float4* src = reinterpret_cast<const float4*>(X + (size_t)p * MLP_NF + 8 * g);
-        const float4 u0 = src[0], u1 = src[1];
-        v[0] = u0.x; v[1] = u0.y; v[2] = u0.z; v[3] = u0.w;
-        v[4] = u1.x; v[5] = u1.y; v[6] = u1.z; v[7] = u1.w;
-      }
-      split_h8(v, xh[t], xl[t]);
-    }
-    f4v acc[4][MLP_TPP];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const h8 a = as_h8(S.a1[m][lane]);
-#pragma unroll
-      for (int t = 0; t < MLP_TPP; ++t) {
-        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, xh[t], zero, 0, 0, 0);
-        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, xl[t], acc[m][t], 0, 0, 0);
-      }
-    }
-    h8 bh[2][MLP_TPP], bl[2][MLP_TPP];
-    relu_to_b(acc, S.b1, g, bh, bl);
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-      for (int t = 0; t < MLP_TPP; ++t) {
-        f4v e = zero;
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          const h8 a = as_h8(S.a2[m][c][lane]);
-          e = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bh[c][t], e, 0, 0, 0);
-          e = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bl[c][t], e, 0, 0, 0);
-        }
-        acc[m][t] = e;
-      }
-    relu_to_b(acc, S.b2, g, bh, bl);
-#pragma unroll
-    for (int t = 0; t < MLP_TPP; ++t) {
-      f4v o = zero;
-#pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const h8 a = as_h8(S.a3[c][lane]);
-        o = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bh[c][t], o, 0, 0, 0);
-        o = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bl[c][t], o, 0, 0, 0);
-      }
-      if (g == 0)   // rows 0..3 are the four outputs
-#pragma unroll
-        for (int i = 0; i < MLP_NO; ++i) y[(16 * (t0 + t) + r) * MLP_NO + i] = o[i] + S.b3[i];
-    }
-  }
-}
-
-// Writes the warm start p0 = max(0, p(t-1) + P y0 / 10) (and s = m p0 /
-// rho^2 for the pressure pass) when `warm`, and the drift correction for
-// the next step when dcorr is non-null.  Workgroups of up to four waves of
-// 64 particles walk the 256-particle tiles grid-stride, so each stages the
-// weights in LDS once for several tiles.
+// The learned start of slot i (mlp_start_out); a boundary or out-of-grid
+// slot only gets a zero drift correction.
 __device__ __forceinline__ void mlp_infer_out(const KP& k, const MLPP& mp, int warm, int i,
                                               const float* y, const float4* __restrict__ pos,
                                               const uint32_t* __restrict__ key,
@@ -247,21 +86,12 @@ __device__ __forceinline__ void mlp_infer_out(const KP& k, const MLPP& mp, int w
     if (dcorr) dcorr[i] = make_float4(0.f, 0.f, 0.f, dcorr[i].w);
     return;
   }
-  if (warm) {
-    const float4 qi = phys[i];
-    float p0 = fmaxf(0.f, fmaxf(0.f, qi.z) + y[0] * (mp.p_scale / MLP_TSCALE));
-    if (!(p0 == p0)) p0 = k.warm * qi.z;   // NaN from the network: plain start
-    press_l[i] = p0;
-    ps[i].w = qi.x * p0 / (qi.y * qi.y);
-  }
-  if (dcorr) {
-    float sc = mp.drift_blend * mp.u_scale / MLP_TSCALE;
-    float4 d = make_float4(y[1] * sc, y[2] * sc, y[3] * sc, dcorr[i].w);
-    if (!(d.x == d.x && d.y == d.y && d.z == d.z)) d = make_float4(0.f, 0.f, 0.f, d.w);
-    dcorr[i] = d;
-  }
+  mlp_start_out(k, mp, warm, i, y, phys[i], press_l, ps, dcorr);
 }
 
+// The network on the feature rows X of every slot.  Workgroups of up to
+// four waves of 64 particles walk the 256-particle tiles grid-stride, so
+// each stages the weights in LDS once for several tiles.
 __global__ __launch_bounds__(256, 2) void k_mlp_infer(
     KP k, MLPP mp, int warm, const float* __restrict__ X, const uint4* __restrict__ wmf,
     const float4* __restrict__ pos, const uint32_t* __restrict__ key,
@@ -276,7 +106,7 @@ __global__ __launch_bounds__(256, 2) void k_mlp_infer(
   for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
     const int base = tile * (int)blockDim.x + 64 * wave;
     if (base >= k.n) continue;   // whole wave past the end
-    mlp_forward_mfma(S, X, k.n, base, ys[wave]);
+    mlp_forward_mfma(S, X + (size_t)base * MLP_NF, k.n - base, ys[wave]);
     // the wave's LDS writes above reach LDS before its reads below, and
     // its reads before the next tile's writes
     asm volatile("" ::: "memory");
@@ -304,7 +134,7 @@ __global__ __launch_bounds__(256, 2) void k_mlp_eval(const float* __restrict__ X
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int base = blockIdx.x * (int)blockDim.x + 64 * wave;
   if (base >= n) return;
-  mlp_forward_mfma(S, X, n, base, ys[wave]);
+  mlp_forward_mfma(S, X + (size_t)base * MLP_NF, n - base, ys[wave]);
   asm volatile("" ::: "memory");
   const int i = base + lane;
   if (i >= n) return;
@@ -785,13 +615,26 @@ static int mlp_quiesce(sphx_context* c) {
   return 0;
 }
 
+// Before the a_ii pass evaluates the network: adopt the step's due
+// refresh (which mlp_begin then skips) and pack the weights.
+int mlp_prepare_start(sphx_context* c) {
+  auto& m = c->mlp;
+  if (!m.settled) {
+    if (m.train_pending > 0 && --m.train_pending == 0) SPHX_TRY(mlp_async_settle(c));
+    m.settled = true;
+  }
+  if (!m.wmf_ok) SPHX_TRY(mlp_pack_mfma(c));
+  return 0;
+}
+
 // The pressure solve of a step with the learned start, the recorder and
 // the fallback.  Called after the a_ii pass.
 // Before a solve: features (or just p(t-1)) and the learned start.
 int mlp_begin(sphx_context* c) {
   auto& m = c->mlp;
   if (!m.ready || c->num == 0) return 0;
-  if (m.train_pending > 0 && --m.train_pending == 0) SPHX_TRY(mlp_async_settle(c));
+  if (!m.settled && m.train_pending > 0 && --m.train_pending == 0) SPHX_TRY(mlp_async_settle(c));
+  m.settled = false;
   SPHX_TRY(mlp_async_launch(c));
   KP k = make_kp(c);
   MLPP mp = make_mlpp(c);
@@ -815,7 +658,7 @@ int mlp_begin(sphx_context* c) {
   m.feat_ok = false;   // used once
   SPHX_HIP(hipGetLastError());
   float4* dcorr = m.drift && mf ? hist : nullptr;
-  if (m.warm || dcorr) {
+  if ((m.warm || dcorr) && !m.start_ok) {
     if (!m.wmf_ok) SPHX_TRY(mlp_pack_mfma(c));
     timer_begin(c, "mlp_infer");
     k_mlp_infer<<<mlp_infer_blocks(k.n, c->tune[T_MLP_INFER]), c->tune[T_MLP_INFER], 0, c->stream>>>(
@@ -824,6 +667,7 @@ int mlp_begin(sphx_context* c) {
     timer_end(c);
     SPHX_HIP(hipGetLastError());
   }
+  m.start_ok = false;   // used once
   // the solve's first iteration is followed by the guard (single scene)
   c->mlp.guard_armed = m.warm && m.guard && c->nscenes == 0;
   return 0;
