@@ -59,10 +59,11 @@ ALGO_BYTES = {
     "iisph_dii": 60,       # x, v, rho, m, rho0 -> v_adv 12, d_ii 12
     # density, v_adv and d_ii in one pass (the composite steps)
     "iisph_density_dii": 64,   # x, v, m, rho0, type -> rho 4, v_adv 12, d_ii 12
-    # a_ii pass with the learned start's features: x, v_adv, d_ii, m, rho,
-    # p, rho0, alpha_1, rho_m, last increment -> rho_adv, a_ii, p0, s, G, H,
-    # 24 features, p(t-1)
-    "iisph_aii": 196,
+    # a_ii pass with the learned start: x, v_adv, d_ii, m, rho, p, rho0,
+    # alpha, last increment (6 x 16 B) -> rho_adv, p0, s, G+H 16, a_ii
+    # record 16, p(t-1), and the recorder's feature rows (every 8th
+    # particle's 96 B)
+    "iisph_aii": 156,
     "mf_drift": 64,        # x, p, m, rho, alpha(2), nu, g-a -> u_m (2x12)
     "mf_alpha": 84,        # x, v, m, rho, alpha, u_m -> alpha_new, drift stress
     "mf_force": 52,        # drift stress, rho_m, v_adv -> v_adv, a_mf
@@ -71,7 +72,9 @@ ALGO_BYTES = {
     "mf_correction_force": 124,
     "radix_sort": 32,      # 2 passes x (key+val read, key+val write)
     "reorder": 104,        # 5 x 16 B + 2 x 4 B gathered and written
-    "nbr_build": 24,       # x read, count + list head written
+    # x and key read 20, count 4, ps and the Jacobi record seeded 40, and the list
+    # itself: 16 B per 8 entries (about 43 neighbours within 2h: 6 groups)
+    "nbr_build": 160,
     "mlp_features": 184,   # x, phys, d_ii, v_adv, alpha, rho_adv -> 24 features, p(t-1)
     "mlp_infer": 140,      # 24 features, x, phys, key -> p0, s
 }
