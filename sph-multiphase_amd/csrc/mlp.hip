@@ -356,13 +356,24 @@ __global__ __launch_bounds__(256) void k_mlp_wgrad(
   }
 }
 
-// grad[e] = sum over the wgrad blocks of gpart[b][e], in block order.
-__global__ void k_mlp_gsum(const float* __restrict__ gpart, int blocks, float* __restrict__ grad) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= MLP_NP) return;
+// grad[e] = sum over the wgrad blocks of gpart[b][e] in a fixed order: a
+// workgroup takes 64 entries, its four waves sum every fourth row each
+// (b = wave, wave + 4, ...), and the four partial sums are added in wave
+// order.
+constexpr int GS_ENTRIES = 64;
+__global__ __launch_bounds__(256) void k_mlp_gsum(const float* __restrict__ gpart, int blocks,
+                                                  float* __restrict__ grad) {
+  __shared__ float part[4][GS_ENTRIES];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int e = blockIdx.x * GS_ENTRIES + lane;
   float s = 0.f;
-  for (int b = 0; b < blocks; ++b) s += gpart[(size_t)b * MLP_NP + e];
-  grad[e] = s;
+  if (e < MLP_NP) {
+#pragma unroll 8
+    for (int b = wave; b < blocks; b += 4) s += gpart[(size_t)b * MLP_NP + e];
+  }
+  part[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && e < MLP_NP) grad[e] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
 }
 
 __global__ void k_mlp_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
@@ -509,7 +520,7 @@ static int mlp_grad_batch(sphx_context* c, const float* X, const float* T, const
   k_mlp_wgrad<<<blocks, 256, 0, c->stream>>>(X, order, base, nb, chunk, m.H1.p, m.dH1.p, m.H2.p,
                                              m.dH2.p, m.dY.p, m.gpart.p);
   SPHX_HIP(hipGetLastError());
-  k_mlp_gsum<<<blocks_for(MLP_NP, 256), 256, 0, c->stream>>>(m.gpart.p, blocks, m.g.p);
+  k_mlp_gsum<<<blocks_for(MLP_NP, GS_ENTRIES), 256, 0, c->stream>>>(m.gpart.p, blocks, m.g.p);
   SPHX_HIP(hipGetLastError());
   return 0;
 }
