@@ -29,29 +29,31 @@ RHO = [1000.0, 1.2]
 NU = [1e-6, 1.5e-5]
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per fluid particle per launch (fields read or written
-# once, natural sizes: float3 = 12 B); neighbour records re-read from L2 and
-# the neighbour list are not counted.
-# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE,
-# scaled by a 256 MiB copy (profiles/iisph_mf_step_1m_hbm_traffic.md)
+# HBM bytes per working launch measured with rocprofv3: FETCH_SIZE x 2.00 +
+# WRITE_SIZE x 1.00, the factors a 256 MiB device copy reads on the same
+# counters (profiles/iisph_mf_step_1m_hbm_traffic.md; TCC_EA0_RDREQ_128B x
+# 128 B gives the same read bytes)
 TRAFFIC_BYTES = {
-    "iisph_iter": 280.6e6,
-    "iisph_dp": 236.9e6,
-    "mf_alpha": 395.0e6,
-    "mf_drift": 270.6e6,
-    "iisph_aii": 496.4e6,    # with the learned start's features
-    "iisph_density_dii": 210.0e6,
-    "nbr_build": 161.8e6,
+    "iisph_iter": 212.2e6,
+    "iisph_dp": 177.6e6,
+    "mf_alpha": 398.4e6,
+    "mf_drift": 280.0e6,
+    "iisph_aii": 381.9e6,    # with the learned start
+    "iisph_density_dii": 218.7e6,
+    "nbr_build": 165.4e6,
     "mf_correction_force": 158.4e6,
-    "mlp_infer": 174.0e6,
 }
 # MPM sand (2M particles), same method (profiles/mpm_sand_2m_kernel_stats.md)
 MPM_TRAFFIC_BYTES = {
-    "mpm_p2g": 337.2e6,
-    "mpm_force": 300.9e6,
-    "mpm_stress": 536.8e6,
+    "mpm_p2g": 337.4e6,
+    "mpm_force": 301.1e6,
+    "mpm_stress": 536.9e6,
 }
 
+# Algorithmic bytes per fluid particle per launch (fields read or written
+# once, natural sizes: float3 = 12 B); neighbour records re-read from L2 and
+# the neighbour list a loop reads are not counted (the list build counts
+# the list it writes).
 ALGO_BYTES = {
     "iisph_dp": 68,        # x 12, p 4, m 4, d_ii 12 -> dijpj 12, m q 12, a_p 12
     "iisph_iter": 76,      # x, m, rho, rho0, a_ii, dijpj, G, H, rho_adv, p -> p, res, s
@@ -334,8 +336,8 @@ def bench_bubbles(args, sphx):
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": TRAFFIC_BYTES.get(dom),
-            "traffic_source": "rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated "
-                              "(profiles/iisph_mf_step_1m_hbm_traffic.md)",
+            "traffic_source": "rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 per working "
+                              "launch, copy-calibrated (profiles/iisph_mf_step_1m_hbm_traffic.md)",
             "algorithmic_bytes_per_launch": algo,
             "avg_launch_ms": avg_ms,
             "launches": n_dom,
@@ -603,8 +605,8 @@ def bench_sand(args, sphx):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": MPM_TRAFFIC_BYTES.get(dom),
-                     "traffic_source": "rocprofv3 FETCH_SIZE+WRITE_SIZE, copy-calibrated "
-                                       "(profiles/mpm_sand_2m_kernel_stats.md)",
+                     "traffic_source": "rocprofv3 FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, "
+                                       "copy-calibrated (profiles/mpm_sand_2m_kernel_stats.md)",
                      "algorithmic_bytes_per_launch": algo, "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
     }
