@@ -152,7 +152,7 @@ def train_network(sphx, args):
     var = np.var(T[M > 0], axis=0) + 1e-12
     lw = (4.0 / var[0], 1.0 / var[1], 1.0 / var[2], 1.0 / var[3])
     t0 = time.perf_counter()
-    loss = t.mlp_train(epochs=args.epochs, batch=args.batch, lr=args.lr, lw=lw, seed=5)
+    loss = t.mlp_train(epochs=args.epochs, batch=args.batch, lr=args.lr, lw=lw, seed=args.train_seed)
     t.sync()
     train_s = time.perf_counter() - t0
     w = t.mlp_weights()
@@ -825,7 +825,7 @@ def bench_ensemble(args, sphx):
     var = np.var(T[M > 0], axis=0) + 1e-12
     lw = (4.0 / var[0], 1.0 / var[1], 1.0 / var[2], 1.0 / var[3])
     t0 = time.perf_counter()
-    loss = t.mlp_train(epochs=args.epochs, batch=args.batch, lr=args.lr, lw=lw, seed=5)
+    loss = t.mlp_train(epochs=args.epochs, batch=args.batch, lr=args.lr, lw=lw, seed=args.train_seed)
     t.sync()
     train_s = time.perf_counter() - t0
     w = t.mlp_weights()
@@ -982,6 +982,7 @@ def main():
                     help="steps of the training tank recorded for the offline fit")
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--batch", type=int, default=16384)
+    ap.add_argument("--train-seed", type=int, default=1, help="seed of the training sample order")
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--drift", type=int, default=0, help="learned drift closure on")
     ap.add_argument("--guard", type=float, default=10.0,

@@ -301,10 +301,8 @@ struct sphx_context {
     sphx::DevBuf<float> X, pprev;       // per slot, current step
     sphx::DevBuf<float4> unew;          // drift with the converged pressure
     sphx::DevBuf<float> sX, sT, sM;     // recorded samples
-    sphx::DevBuf<float> H1, dH1, H2, dH2, dY;
     sphx::DevBuf<int> order;
     sphx::DevBuf<double> loss;
-    int batch_cap = 0;
     // asynchronous online refresh: the Adam steps run on their own stream
     // (default priority) while the next step is solved, into staged FP16
     // weights that the step after that adopts

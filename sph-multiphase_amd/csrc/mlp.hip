@@ -10,10 +10,11 @@
 // and the step is solved again from the reference warm start.
 //
 // Training is on the GPU with these kernels: the solver records (features,
-// converged increments) pairs, k_mlp_fwdbwd runs forward + backward per
-// sample (FP32), k_mlp_wgrad reduces the weight gradients over the batch
-// in LDS tiles, k_mlp_adam applies Adam and refreshes the FP16 copy the
-// inference kernel reads (FP16 weights, FP32 accumulation).
+// converged increments) pairs, k_mlp_train_mfma runs forward, backward and
+// the weight gradients of a batch on FP32 MFMA (each wave one row of
+// partial gradients), k_mlp_gsum adds the rows in a fixed order, and
+// k_mlp_adam applies Adam and refreshes the FP16 copy the inference reads
+// (FP16 weights, FP32 accumulation).
 #include "mlp_mfma.hpp"
 
 #include <hip/hip_fp16.h>
@@ -59,9 +60,6 @@ __global__ __launch_bounds__(256) void k_mlp_features(
   });
   feat_write(k, mp, fa, qi, rho_adv[i], di, cst[i].x, veleval[i], alpha, hist, i, x);
 }
-
-// Hidden units per chunk of the training kernels' fused layers.
-constexpr int MLP_CH = 8;
 
 // ------------------------------------------------- inference on MFMA --
 // The MFMA operand layout of the current FP16 weights, built once per call
@@ -175,189 +173,8 @@ __global__ void k_mlp_plain_start(KP k, const float4* __restrict__ pos,
   ps[i].w = q.x * p0 / (q.y * q.y);
 }
 
-// ------------------------------------------------------------- training --
-// One thread per sample: forward, loss, and the backward pass down to the
-// layer inputs.  Stores H1, dH1, H2, dH2, dY (batch-local rows) for the
-// weight gradients.  Same chunking as the inference to stay in registers.
-__global__ __launch_bounds__(256) void k_mlp_fwdbwd(
-    const float* __restrict__ Xall, const float* __restrict__ Tall, const float* __restrict__ Mall,
-    const int* __restrict__ perm, int base, int nb, int nvalid_total, const float* __restrict__ w,
-    float4 lw, float* __restrict__ H1, float* __restrict__ dH1, float* __restrict__ H2,
-    float* __restrict__ dH2, float* __restrict__ dY, double* __restrict__ loss) {
-  __shared__ float sw[MLP_NP];
-  __shared__ double sl[4];
-  for (int t = threadIdx.x; t < MLP_NP; t += blockDim.x) sw[t] = w[t];
-  if (threadIdx.x < 4) sl[threadIdx.x] = 0.0;
-  __syncthreads();
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  double my_loss = 0.0;
-  if (s < nb) {
-    const int src = perm[base + s];
-    const float4* xv = reinterpret_cast<const float4*>(Xall + (size_t)src * MLP_NF);
-    float xin[MLP_NF];
-#pragma unroll
-    for (int q = 0; q < MLP_NF / 4; ++q) {
-      float4 t = xv[q];
-      xin[4 * q] = t.x; xin[4 * q + 1] = t.y; xin[4 * q + 2] = t.z; xin[4 * q + 3] = t.w;
-    }
-    const float4 tv = reinterpret_cast<const float4*>(Tall)[src];
-    const float msk = Mall[src];
-    float h1[MLP_H];
-#pragma unroll
-    for (int a = 0; a < MLP_H; ++a) h1[a] = sw[MLP_B1 + a];
-#pragma unroll
-    for (int q = 0; q < MLP_NF; ++q) {
-#pragma unroll
-      for (int a = 0; a < MLP_H; ++a) h1[a] = fmaf(xin[q], sw[MLP_W1 + q * MLP_H + a], h1[a]);
-    }
-    float4* h1o = reinterpret_cast<float4*>(H1 + (size_t)s * MLP_H);
-#pragma unroll
-    for (int a = 0; a < MLP_H; ++a) h1[a] = fmaxf(h1[a], 0.f);
-#pragma unroll
-    for (int a = 0; a < MLP_H / 4; ++a)
-      h1o[a] = make_float4(h1[4 * a], h1[4 * a + 1], h1[4 * a + 2], h1[4 * a + 3]);
-    float y[MLP_NO];
-#pragma unroll
-    for (int o = 0; o < MLP_NO; ++o) y[o] = sw[MLP_B3 + o];
-    float* h2o = H2 + (size_t)s * MLP_H;
-#pragma unroll 1
-    for (int bc = 0; bc < MLP_H; bc += MLP_CH) {
-      float acc[MLP_CH];
-#pragma unroll
-      for (int j = 0; j < MLP_CH; ++j) acc[j] = sw[MLP_B2 + bc + j];
-#pragma unroll
-      for (int a = 0; a < MLP_H; ++a) {
-#pragma unroll
-        for (int j = 0; j < MLP_CH; ++j)
-          acc[j] = fmaf(h1[a], sw[MLP_W2 + a * MLP_H + bc + j], acc[j]);
-      }
-#pragma unroll
-      for (int j = 0; j < MLP_CH; ++j) {
-        const float hb = fmaxf(acc[j], 0.f);
-        h2o[bc + j] = hb;
-#pragma unroll
-        for (int o = 0; o < MLP_NO; ++o) y[o] = fmaf(hb, sw[MLP_W3 + (bc + j) * MLP_NO + o], y[o]);
-      }
-    }
-    const float tt[4] = {tv.x, tv.y, tv.z, tv.w};
-    const float lwv[4] = {lw.x, lw.y, lw.z, lw.w};
-    float dy[MLP_NO];
-#pragma unroll
-    for (int o = 0; o < MLP_NO; ++o) {
-      float e = y[o] - tt[o];
-      my_loss += (double)msk * lwv[o] * e * e;
-      dy[o] = 2.f * msk * lwv[o] * e / (float)nvalid_total;
-    }
-    reinterpret_cast<float4*>(dY)[s] = make_float4(dy[0], dy[1], dy[2], dy[3]);
-    // backward: dh2 chunk by chunk, accumulating dh1 = W2 dh2
-    float g1[MLP_H];
-#pragma unroll
-    for (int a = 0; a < MLP_H; ++a) g1[a] = 0.f;
-    float* dh2o = dH2 + (size_t)s * MLP_H;
-#pragma unroll 1
-    for (int bc = 0; bc < MLP_H; bc += MLP_CH) {
-      float d2[MLP_CH];
-#pragma unroll
-      for (int j = 0; j < MLP_CH; ++j) {
-        float v = 0.f;
-#pragma unroll
-        for (int o = 0; o < MLP_NO; ++o) v = fmaf(dy[o], sw[MLP_W3 + (bc + j) * MLP_NO + o], v);
-        d2[j] = h2o[bc + j] > 0.f ? v : 0.f;
-        dh2o[bc + j] = d2[j];
-      }
-#pragma unroll
-      for (int a = 0; a < MLP_H; ++a) {
-#pragma unroll
-        for (int j = 0; j < MLP_CH; ++j) g1[a] = fmaf(d2[j], sw[MLP_W2 + a * MLP_H + bc + j], g1[a]);
-      }
-    }
-    float4* dh1o = reinterpret_cast<float4*>(dH1 + (size_t)s * MLP_H);
-#pragma unroll
-    for (int a = 0; a < MLP_H / 4; ++a)
-      dh1o[a] = make_float4(h1[4 * a] > 0.f ? g1[4 * a] : 0.f, h1[4 * a + 1] > 0.f ? g1[4 * a + 1] : 0.f,
-                            h1[4 * a + 2] > 0.f ? g1[4 * a + 2] : 0.f,
-                            h1[4 * a + 3] > 0.f ? g1[4 * a + 3] : 0.f);
-  }
-  // loss partial (FP64)
-  for (int off = 32; off > 0; off >>= 1) my_loss += __shfl_down(my_loss, off, 64);
-  if ((threadIdx.x & 63) == 0) sl[threadIdx.x >> 6] = my_loss;
-  __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(loss, sl[0] + sl[1] + sl[2] + sl[3]);
-}
-
-// Weight gradients over the batch: grad[e] += sum_s A_s[a] * B_s[b] for
-// every entry e (W1: X x dH1, W2: H1 x dH2, W3: H2 x dY, biases: sum dH).
-// A block takes a chunk of samples, stages them in LDS 32 at a time, each
-// thread accumulates ~24 entries in registers, then writes them to its
-// block's row of gpart; k_mlp_gsum adds the rows in block order, so a
-// training run gives the same weights every time.
-constexpr int WG_SUB = 32;
-constexpr int WG_PER = (MLP_NP + 255) / 256;
-
-__global__ __launch_bounds__(256) void k_mlp_wgrad(
-    const float* __restrict__ Xall, const int* __restrict__ perm, int base, int nb, int chunk,
-    const float* __restrict__ H1, const float* __restrict__ dH1, const float* __restrict__ H2,
-    const float* __restrict__ dH2, const float* __restrict__ dY, float* __restrict__ gpart) {
-  __shared__ float sX[WG_SUB][MLP_NF], sH1[WG_SUB][MLP_H], sdH1[WG_SUB][MLP_H],
-      sH2[WG_SUB][MLP_H], sdH2[WG_SUB][MLP_H], sdY[WG_SUB][MLP_NO];
-  float acc[WG_PER];
-  for (int u = 0; u < WG_PER; ++u) acc[u] = 0.f;
-  const int s0 = blockIdx.x * chunk, s1 = min(nb, s0 + chunk);
-  for (int sb = s0; sb < s1; sb += WG_SUB) {
-    int cnt = min(WG_SUB, s1 - sb);
-    __syncthreads();
-    for (int t = threadIdx.x; t < WG_SUB * MLP_H; t += blockDim.x) {
-      int r = t / MLP_H, c = t % MLP_H;
-      bool ok = r < cnt;
-      size_t g = (size_t)(sb + r) * MLP_H + c;
-      sH1[r][c] = ok ? H1[g] : 0.f;
-      sdH1[r][c] = ok ? dH1[g] : 0.f;
-      sH2[r][c] = ok ? H2[g] : 0.f;
-      sdH2[r][c] = ok ? dH2[g] : 0.f;
-    }
-    for (int t = threadIdx.x; t < WG_SUB * MLP_NF; t += blockDim.x) {
-      int r = t / MLP_NF, c = t % MLP_NF;
-      sX[r][c] = r < cnt ? Xall[(size_t)perm[base + sb + r] * MLP_NF + c] : 0.f;
-    }
-    for (int t = threadIdx.x; t < WG_SUB * MLP_NO; t += blockDim.x) {
-      int r = t / MLP_NO, c = t % MLP_NO;
-      sdY[r][c] = r < cnt ? dY[(size_t)(sb + r) * MLP_NO + c] : 0.f;
-    }
-    __syncthreads();
-    for (int u = 0; u < WG_PER; ++u) {
-      int e = threadIdx.x + 256 * u;
-      if (e >= MLP_NP) break;
-      float v = 0.f;
-      if (e < MLP_B1) {
-        int q = e / MLP_H, a = e % MLP_H;
-        for (int r = 0; r < WG_SUB; ++r) v += sX[r][q] * sdH1[r][a];
-      } else if (e < MLP_W2) {
-        int a = e - MLP_B1;
-        for (int r = 0; r < WG_SUB; ++r) v += sdH1[r][a];
-      } else if (e < MLP_B2) {
-        int a = (e - MLP_W2) / MLP_H, b = (e - MLP_W2) % MLP_H;
-        for (int r = 0; r < WG_SUB; ++r) v += sH1[r][a] * sdH2[r][b];
-      } else if (e < MLP_W3) {
-        int b = e - MLP_B2;
-        for (int r = 0; r < WG_SUB; ++r) v += sdH2[r][b];
-      } else if (e < MLP_B3) {
-        int b = (e - MLP_W3) / MLP_NO, o = (e - MLP_W3) % MLP_NO;
-        for (int r = 0; r < WG_SUB; ++r) v += sH2[r][b] * sdY[r][o];
-      } else {
-        int o = e - MLP_B3;
-        for (int r = 0; r < WG_SUB; ++r) v += sdY[r][o];
-      }
-      acc[u] += v;
-    }
-  }
-  for (int u = 0; u < WG_PER; ++u) {
-    int e = threadIdx.x + 256 * u;
-    if (e < MLP_NP) gpart[(size_t)blockIdx.x * MLP_NP + e] = acc[u];
-  }
-}
-
-// grad[e] = sum over the wgrad blocks of gpart[b][e] in a fixed order: a
-// workgroup takes 64 entries, its four waves sum every fourth row each
+// grad[e] = sum over the training waves' rows gpart[b][e] in a fixed order:
+// a workgroup takes 64 entries, its four waves sum every fourth row each
 // (b = wave, wave + 4, ...), and the four partial sums are added in wave
 // order.
 constexpr int GS_ENTRIES = 64;
@@ -374,6 +191,262 @@ __global__ __launch_bounds__(256) void k_mlp_gsum(const float* __restrict__ gpar
   part[wave][lane] = s;
   __syncthreads();
   if (wave == 0 && e < MLP_NP) grad[e] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+}
+
+// ----------------------------------------------------- training on MFMA --
+// The loss gradient of one Adam step on v_mfma_f32_16x16x4f32: FP32
+// operands and FP32 accumulation, so every product is the FP32 product of
+// the VALU form and only the order of the sums differs.  A wave takes tiles
+// of 16 samples.  Forward and backward keep the inference's transposed
+// layout: units on the rows of a 16x16 tile, samples on its columns, lane
+// (r = lane % 16, g = lane / 16) holding rows 4 g + i (i = 0..3) of column r.
+// A layer's accumulators are then the next product's B operand for the K
+// step (t, i) that takes units 16 t + 4 g + i, and the weights are read
+// from LDS in that K order.  The weight gradients sum over samples: their
+// operands need a unit on each lane and samples on the lane groups, so the
+// wave stores H1, dH1, H2 and dH2 of its tile in LDS as [unit][sample] and
+// reads them back transposed.  Each wave writes its partial gradient as one
+// row of gpart (k_mlp_gsum adds the rows in a fixed order) and adds its
+// loss to *loss (FP64, for reporting).
+constexpr int TR_TS = 16;               // samples per tile
+constexpr int TR_WAVES = 4;             // waves per workgroup
+constexpr int TR_LD = TR_TS + 1;        // [unit][sample] row stride (LDS banks)
+constexpr int TR_ROWS_MAX = 512;        // gpart rows (waves) per step
+struct MlpTrainLds {
+  float w[MLP_NP];                               // FP32 master weights
+  float act[TR_WAVES][4][MLP_H * TR_LD];         // per wave: H1, dH1, H2, dH2
+  float xs[TR_WAVES][TR_TS * 32];                // per wave: features [sample][feature]
+  float dys[TR_WAVES][TR_TS * MLP_NO];           // per wave: dY [sample][output]
+};
+
+// Sample k of an epoch: valid[perm(k)], perm a bijection of [0, n) keyed
+// per epoch: a four-round Feistel network on 2h bits (2^2h >= n, the round
+// function a SplitMix64 finaliser of key, round and half), walked until it
+// lands below n (it does: the walk stays on the cycle through k).  key 0:
+// the identity (order[k] as stored).
+struct SamplePerm {
+  uint64_t key;
+  uint32_t n;
+  int h;
+};
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint32_t perm_index(const SamplePerm& sp, uint32_t k) {
+  if (sp.key == 0) return k;
+  const uint32_t mask = (1u << sp.h) - 1u;
+  uint32_t x = k;
+  do {
+    uint32_t L = x >> sp.h, R = x & mask;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const uint32_t F = (uint32_t)mix64(sp.key + ((uint64_t)r << 40) + R) & mask;
+      const uint32_t t = R;
+      R = L ^ F;
+      L = t;
+    }
+    x = (L << sp.h) | R;
+  } while (x >= sp.n);
+  return x;
+}
+
+__device__ __forceinline__ f4v mfma4(float a, float b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+// sum over the 16 lanes r of each lane group (fixed order)
+__device__ __forceinline__ float group_sum16(float v) {
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(TR_WAVES * 64) void k_mlp_train_mfma(
+    const float* __restrict__ Xall, const float* __restrict__ Tall, const float* __restrict__ Mall,
+    const int* __restrict__ perm, SamplePerm sp, int base, int nb, int nvalid_total,
+    const float* __restrict__ w, float4 lw, int tpw, float* __restrict__ gpart,
+    double* __restrict__ loss) {
+  __shared__ MlpTrainLds S;
+  for (int t = threadIdx.x; t < MLP_NP; t += blockDim.x) S.w[t] = w[t];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r = lane & 15, g = lane >> 4;
+  const int row = blockIdx.x * TR_WAVES + wave;
+  const int ntiles = (nb + TR_TS - 1) / TR_TS;
+  const f4v z = {0.f, 0.f, 0.f, 0.f};
+  const float lwv[4] = {lw.x, lw.y, lw.z, lw.w};
+  f4v gw1[4][2], gw2[4][4], gw3[4];   // dW1^T (a x q), dW2 (a x b), dW3 (b x o) tiles
+  float sb1[4][4], sb2[4][4], sb3[4];  // bias sums of the lane's sample column
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    gw1[t][0] = z; gw1[t][1] = z; gw3[t] = z; sb3[t] = 0.f;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { gw2[t][u] = z; sb1[t][u] = 0.f; sb2[t][u] = 0.f; }
+  }
+  double my_loss = 0.0;
+  float* H1s = S.act[wave][0];
+  float* D1s = S.act[wave][1];
+  float* H2s = S.act[wave][2];
+  float* D2s = S.act[wave][3];
+  float* xs = S.xs[wave];
+  float* dys = S.dys[wave];
+  for (int k = 0; k < tpw; ++k) {
+    const int tile = row * tpw + k;
+    if (tile >= ntiles) break;   // the same for the whole wave
+    const int s0 = tile * TR_TS;
+    for (int e = lane; e < TR_TS * MLP_NF; e += 64) {
+      const int sl = e / MLP_NF, q = e - sl * MLP_NF;
+      xs[sl * 32 + q] =
+          s0 + sl < nb ? Xall[(size_t)perm[perm_index(sp, base + s0 + sl)] * MLP_NF + q] : 0.f;
+    }
+    // one wave's LDS accesses complete in order; keep the compiler's too
+    asm volatile("" ::: "memory");
+    // layer 1: H1^T = W1^T X^T (K = 24 features in steps of 4)
+    f4v h1[4] = {z, z, z, z};
+#pragma unroll
+    for (int ks = 0; ks < MLP_NF / 4; ++ks) {
+      const float b = xs[r * 32 + 4 * ks + g];
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt)
+        h1[mt] = mfma4(S.w[MLP_W1 + (4 * ks + g) * MLP_H + 16 * mt + r], b, h1[mt]);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h1[mt][i] = fmaxf(h1[mt][i] + S.w[MLP_B1 + 16 * mt + 4 * g + i], 0.f);
+    // layer 2: H2^T = W2^T H1^T (K step (t, i): units 16 t + 4 g + i)
+    f4v h2[4] = {z, z, z, z};
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+          h2[mt] = mfma4(S.w[MLP_W2 + (16 * t + 4 * g + i) * MLP_H + 16 * mt + r], h1[t][i], h2[mt]);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h2[mt][i] = fmaxf(h2[mt][i] + S.w[MLP_B2 + 16 * mt + 4 * g + i], 0.f);
+    // layer 3: Y^T = W3^T H2^T (rows 0..3: lanes of group 0 hold y_o = row i)
+    f4v y = z;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        y = mfma4(r < MLP_NO ? S.w[MLP_W3 + (16 * t + 4 * g + i) * MLP_NO + r] : 0.f, h2[t][i], y);
+    if (g == 0) {
+      float dy[MLP_NO] = {0.f, 0.f, 0.f, 0.f};
+      const int s = s0 + r;
+      if (s < nb) {
+        const int src = perm[perm_index(sp, base + s)];
+        const float4 tv = reinterpret_cast<const float4*>(Tall)[src];
+        const float msk = Mall[src];
+        const float tt[4] = {tv.x, tv.y, tv.z, tv.w};
+#pragma unroll
+        for (int o = 0; o < MLP_NO; ++o) {
+          const float e = y[o] + S.w[MLP_B3 + o] - tt[o];
+          my_loss += (double)msk * lwv[o] * e * e;
+          dy[o] = 2.f * msk * lwv[o] * e / (float)nvalid_total;
+        }
+      }
+#pragma unroll
+      for (int o = 0; o < MLP_NO; ++o) {
+        dys[r * MLP_NO + o] = dy[o];
+        sb3[o] += dy[o];
+      }
+    }
+    asm volatile("" ::: "memory");
+    // backward, layer 3: dH2^T = W3 dY^T (K = the 4 outputs), masked by H2 > 0
+    f4v d2[4];
+    const float bdy = dys[r * MLP_NO + g];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      d2[mt] = mfma4(S.w[MLP_W3 + (16 * mt + r) * MLP_NO + g], bdy, z);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) d2[mt][i] = h2[mt][i] > 0.f ? d2[mt][i] : 0.f;
+    }
+    // backward, layer 2: dH1^T = W2 dH2^T, masked by H1 > 0
+    f4v g1[4] = {z, z, z, z};
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+          g1[mt] = mfma4(S.w[MLP_W2 + (16 * mt + r) * MLP_H + 16 * t + 4 * g + i], d2[t][i], g1[mt]);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        g1[mt][i] = h1[mt][i] > 0.f ? g1[mt][i] : 0.f;
+        sb1[mt][i] += g1[mt][i];
+        sb2[mt][i] += d2[mt][i];
+        const int u = (16 * mt + 4 * g + i) * TR_LD + r;
+        H1s[u] = h1[mt][i];
+        D1s[u] = g1[mt][i];
+        H2s[u] = h2[mt][i];
+        D2s[u] = d2[mt][i];
+      }
+    asm volatile("" ::: "memory");
+    // weight gradients, K = the tile's 16 samples in steps of 4 (sample 4 ks + g)
+#pragma unroll
+    for (int ks = 0; ks < TR_TS / 4; ++ks) {
+      const int sk = 4 * ks + g;
+      float ah1[4], ad1[4], ah2[4], bd2[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int u = (16 * t + r) * TR_LD + sk;
+        ah1[t] = H1s[u];
+        ad1[t] = D1s[u];
+        ah2[t] = H2s[u];
+        bd2[t] = D2s[u];
+      }
+#pragma unroll
+      for (int ma = 0; ma < 4; ++ma)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) gw2[ma][nt] = mfma4(ah1[ma], bd2[nt], gw2[ma][nt]);
+#pragma unroll
+      for (int nq = 0; nq < 2; ++nq) {
+        const int q = 16 * nq + r;
+        const float bx = q < MLP_NF ? xs[sk * 32 + q] : 0.f;
+#pragma unroll
+        for (int ma = 0; ma < 4; ++ma) gw1[ma][nq] = mfma4(ad1[ma], bx, gw1[ma][nq]);
+      }
+      const float bo = r < MLP_NO ? dys[sk * MLP_NO + r] : 0.f;
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) gw3[mb] = mfma4(ah2[mb], bo, gw3[mb]);
+    }
+    asm volatile("" ::: "memory");   // this tile's reads before the next tile's writes
+  }
+  // this wave's row of partial gradients, in the weight order of k_mlp_adam
+  float* out = gpart + (size_t)row * MLP_NP;
+#pragma unroll
+  for (int ma = 0; ma < 4; ++ma)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int a = 16 * ma + 4 * g + i;
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) out[MLP_W2 + a * MLP_H + 16 * nt + r] = gw2[ma][nt][i];
+#pragma unroll
+      for (int nq = 0; nq < 2; ++nq) {
+        const int q = 16 * nq + r;
+        if (q < MLP_NF) out[MLP_W1 + q * MLP_H + a] = gw1[ma][nq][i];
+      }
+      if (r < MLP_NO) out[MLP_W3 + a * MLP_NO + r] = gw3[ma][i];
+      const float b1 = group_sum16(sb1[ma][i]), b2 = group_sum16(sb2[ma][i]);
+      if (r == 0) {
+        out[MLP_B1 + a] = b1;
+        out[MLP_B2 + a] = b2;
+      }
+    }
+#pragma unroll
+  for (int o = 0; o < MLP_NO; ++o) {
+    const float b3 = group_sum16(sb3[o]);   // lanes of group 0 hold the sums
+    if (lane == 0) out[MLP_B3 + o] = b3;
+  }
+  for (int off = 32; off > 0; off >>= 1) my_loss += __shfl_down(my_loss, off, 64);
+  if (lane == 0 && my_loss != 0.0) atomicAdd(loss, my_loss);
 }
 
 __global__ void k_mlp_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
@@ -488,39 +561,23 @@ static int mlp_need(sphx_context* c, const char* who) {
   return 0;
 }
 
-static int mlp_batch_buffers(sphx_context* c, int batch) {
-  auto& m = c->mlp;
-  if (batch <= m.batch_cap) return 0;
-  size_t b = (size_t)batch;
-  SPHX_TRY(m.H1.alloc(b * MLP_H));
-  SPHX_TRY(m.dH1.alloc(b * MLP_H));
-  SPHX_TRY(m.H2.alloc(b * MLP_H));
-  SPHX_TRY(m.dH2.alloc(b * MLP_H));
-  SPHX_TRY(m.dY.alloc(b * MLP_NO));
-  m.batch_cap = batch;
-  return 0;
-}
-
 // Gradient of the loss over samples order[base .. base+nb) into m.g; loss
 // sum added to m.loss[0].
 static int mlp_grad_batch(sphx_context* c, const float* X, const float* T, const float* M,
-                          const int* order, int base, int nb, int nvalid, float4 lw) {
+                          const int* order, int base, int nb, int nvalid, float4 lw,
+                          SamplePerm sp = SamplePerm{0, 0, 0}) {
   auto& m = c->mlp;
-  SPHX_TRY(mlp_batch_buffers(c, nb));
-  k_mlp_fwdbwd<<<blocks_for(nb, 256), 256, 0, c->stream>>>(
-      X, T, M, order, base, nb, nvalid, m.w.p, lw, m.H1.p, m.dH1.p, m.H2.p, m.dH2.p, m.dY.p,
-      m.loss.p);
+  // tiles of 16 samples, at most TR_ROWS_MAX waves (gpart rows)
+  const int tiles = (nb + TR_TS - 1) / TR_TS;
+  const int tpw = (tiles + TR_ROWS_MAX - 1) / TR_ROWS_MAX;
+  const int waves = (tiles + tpw - 1) / tpw;
+  const int blocks = (waves + TR_WAVES - 1) / TR_WAVES;
+  SPHX_TRY(m.gpart.alloc((size_t)TR_ROWS_MAX * MLP_NP));
+  k_mlp_train_mfma<<<blocks, TR_WAVES * 64, 0, c->stream>>>(
+      X, T, M, order, sp, base, nb, nvalid, m.w.p, lw, tpw, m.gpart.p, m.loss.p);
   SPHX_HIP(hipGetLastError());
-  constexpr int WG_BLOCKS_MAX = 512;
-  int blocks = std::min(WG_BLOCKS_MAX, std::max(1, nb / 64));
-  int chunk = (nb + blocks - 1) / blocks;
-  chunk = (chunk + WG_SUB - 1) / WG_SUB * WG_SUB;
-  blocks = (nb + chunk - 1) / chunk;
-  SPHX_TRY(m.gpart.alloc((size_t)WG_BLOCKS_MAX * MLP_NP));
-  k_mlp_wgrad<<<blocks, 256, 0, c->stream>>>(X, order, base, nb, chunk, m.H1.p, m.dH1.p, m.H2.p,
-                                             m.dH2.p, m.dY.p, m.gpart.p);
-  SPHX_HIP(hipGetLastError());
-  k_mlp_gsum<<<blocks_for(MLP_NP, GS_ENTRIES), 256, 0, c->stream>>>(m.gpart.p, blocks, m.g.p);
+  k_mlp_gsum<<<blocks_for(MLP_NP, GS_ENTRIES), 256, 0, c->stream>>>(m.gpart.p, blocks * TR_WAVES,
+                                                                      m.g.p);
   SPHX_HIP(hipGetLastError());
   return 0;
 }
@@ -1121,17 +1178,21 @@ int sphx_mlp_train(sphx_context* c, int epochs, int batch, float lr, const float
   SPHX_CHECK(!valid.empty(), "sphx_mlp_train", "no recorded samples");
   const int nv = (int)valid.size();
   SPHX_TRY(m.order.alloc(nv));
+  SPHX_HIP(hipMemcpyAsync(m.order.p, valid.data(), (size_t)nv * sizeof(int),
+                          hipMemcpyHostToDevice, c->stream));
+  // each epoch visits the valid samples in the order of a keyed bijection
+  // of [0, nv), evaluated on the device
+  int h = 1;
+  while ((uint64_t)1 << (2 * h) < (uint64_t)nv) ++h;
   std::mt19937_64 rng(seed);
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   float4 l4 = make_float4(lw[0], lw[1], lw[2], lw[3]);
   for (int ep = 0; ep < epochs; ++ep) {
-    std::shuffle(valid.begin(), valid.end(), rng);
-    SPHX_HIP(hipMemcpyAsync(m.order.p, valid.data(), (size_t)nv * sizeof(int),
-                            hipMemcpyHostToDevice, c->stream));
+    const SamplePerm sp{rng() | 1ull, (uint32_t)nv, h};
     SPHX_HIP(hipMemsetAsync(m.loss.p, 0, sizeof(double), c->stream));
     for (int base = 0; base < nv; base += batch) {
       int nb = std::min(batch, nv - base);
-        SPHX_TRY(mlp_grad_batch(c, m.sX.p, m.sT.p, m.sM.p, m.order.p, base, nb, nb, l4));
+      SPHX_TRY(mlp_grad_batch(c, m.sX.p, m.sT.p, m.sM.p, m.order.p, base, nb, nb, l4, sp));
       ++m.adam_t;
       float bc1 = 1.f - std::pow(b1, (float)m.adam_t), bc2 = 1.f - std::pow(b2, (float)m.adam_t);
       k_mlp_adam<<<blocks_for(MLP_NP, 256), 256, 0, c->stream>>>(
@@ -1139,8 +1200,6 @@ int sphx_mlp_train(sphx_context* c, int epochs, int batch, float lr, const float
           eps, bc1, bc2);
       SPHX_HIP(hipGetLastError());
     }
-    // host copy of `valid` is reshuffled next epoch: wait for the upload
-    SPHX_HIP(hipStreamSynchronize(c->stream));
     if (loss_out) {
       double l;
       SPHX_HIP(hipMemcpy(&l, m.loss.p, sizeof(double), hipMemcpyDeviceToHost));
