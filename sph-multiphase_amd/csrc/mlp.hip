@@ -295,10 +295,13 @@ __global__ __launch_bounds__(TR_WAVES * 64) void k_mlp_train_mfma(
     const int tile = row * tpw + k;
     if (tile >= ntiles) break;   // the same for the whole wave
     const int s0 = tile * TR_TS;
+    // lane r < 16 finds sample r's slot once; every lane loads 6 features
+    int src_r = -1;
+    if (lane < TR_TS && s0 + lane < nb) src_r = perm[perm_index(sp, base + s0 + lane)];
     for (int e = lane; e < TR_TS * MLP_NF; e += 64) {
       const int sl = e / MLP_NF, q = e - sl * MLP_NF;
-      xs[sl * 32 + q] =
-          s0 + sl < nb ? Xall[(size_t)perm[perm_index(sp, base + s0 + sl)] * MLP_NF + q] : 0.f;
+      const int src = __shfl(src_r, sl, 64);
+      xs[sl * 32 + q] = src >= 0 ? Xall[(size_t)src * MLP_NF + q] : 0.f;
     }
     // one wave's LDS accesses complete in order; keep the compiler's too
     asm volatile("" ::: "memory");
@@ -337,9 +340,8 @@ __global__ __launch_bounds__(TR_WAVES * 64) void k_mlp_train_mfma(
         y = mfma4(r < MLP_NO ? S.w[MLP_W3 + (16 * t + 4 * g + i) * MLP_NO + r] : 0.f, h2[t][i], y);
     if (g == 0) {
       float dy[MLP_NO] = {0.f, 0.f, 0.f, 0.f};
-      const int s = s0 + r;
-      if (s < nb) {
-        const int src = perm[perm_index(sp, base + s)];
+      if (src_r >= 0) {   // lane r of group 0: sample r
+        const int src = src_r;
         const float4 tv = reinterpret_cast<const float4*>(Tall)[src];
         const float msk = Mall[src];
         const float tt[4] = {tv.x, tv.y, tv.z, tv.w};
