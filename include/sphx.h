@@ -500,7 +500,9 @@ int sphx_solve_history(sphx_context* ctx, int* iters, double* err, int max_solve
 /* Project autotuner.  Workgroup size per neighbour-loop kernel class (timer
  * names iisph_density, iisph_dii, iisph_aii, iisph_dp, iisph_iter,
  * mf_drift, mf_alpha, mf_force, mlp_features, mlp_infer): 64, 128, 192 or
- * 256.  (The list build stages 256-particle tiles in LDS: fixed at 256.)  sphx_autotune times each class at 64, 128 and 256 over
+ * 256.  (The list build stages 256-particle tiles in LDS, and the a_ii pass
+ * that evaluates the learned start shares one LDS copy of the weights
+ * among four waves: both fixed at 256.)  sphx_autotune times each class at 64, 128 and 256 over
  * `steps` real two-phase steps per size and keeps the fastest; step graphs
  * are recaptured.  Sizes never change a particle's arithmetic. */
 int sphx_tune_set(sphx_context* ctx, const char* kernel, int block);
