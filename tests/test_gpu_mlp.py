@@ -111,6 +111,40 @@ def test_training_gradient_matches_oracle(gpu):
     assert loss_a == pytest.approx(loss_o2, rel=1e-6)
 
 
+def test_training_epoch_visits_every_sample_once(gpu):
+    """An epoch visits the recorded samples in a keyed order computed on the
+    device.  With one batch per epoch, the epoch's loss (summed before the
+    Adam update) is the loss of the starting weights over the samples the
+    order visited; it equals the loss over all valid samples (the gradient
+    call, which takes them in stored order) only if each is visited exactly
+    once.  The sample count is far from a power of 4, so the order walks
+    past the end of its domain; two seeds."""
+    p, disp, calc, alpha, o = column(n=3000, seed=5)
+    P, _ = scales(p, 0.8)
+    with sphx.Context(p, o) as ctx:
+        ctx.upload(disp, calc)
+        ctx.mf_setup(RHO, NU)
+        ctx.mf_upload(alpha)
+        ctx.call("grid_build")
+        ctx.mf_step()
+        ctx.mlp_init(7, P, drift_scale(ctx))
+        ctx.mlp_reserve(4 * ctx.num)
+        ctx.mlp_mode(record=True)
+        run_steps(ctx, 3)
+        total, valid = ctx.mlp_samples()
+        X, T, M = ctx.mlp_download_samples(0, total)
+        lw = np.array([1.0, 0.5, 0.25, 0.125], np.float32)
+        w0 = ctx.mlp_weights()
+        loss_ref, _ = ctx.mlp_grad(X, T, lw, M)
+        n4 = 4 ** int(np.ceil(np.log(valid) / np.log(4)))
+        print(f"{valid} valid of {total} samples, permutation domain {n4}", flush=True)
+        assert valid < 0.75 * n4
+        for seed in (1, 2):
+            ctx.mlp_set_weights(w0)
+            loss = ctx.mlp_train(epochs=1, batch=total + 1, lr=1e-3, lw=lw, seed=seed)
+            assert loss[0] == pytest.approx(loss_ref, rel=1e-9), seed
+
+
 def drift_scale(ctx):
     """RMS of the phase-1 drift over the fluid: the drift normalisation."""
     d = ctx.mf_download()["drift"][:, 1, :]
