@@ -73,6 +73,35 @@ __device__ __forceinline__ int xcd_tile() {
 }
 __device__ __forceinline__ int xcd_index() { return xcd_tile() * (int)blockDim.x + (int)threadIdx.x; }
 
+// FP64 conservation totals of one fluid particle: m, m v, m |v|^2 / 2, m x.
+// No contraction (the same bits in every kernel that calls it).
+__device__ __forceinline__ void particle_totals(double m, float vx, float vy, float vz, float px,
+                                                float py, float pz, double* t) {
+#pragma clang fp contract(off)
+  const double x = vx, y = vy, z = vz;
+  t[0] = m;
+  t[1] = m * x; t[2] = m * y; t[3] = m * z;
+  t[4] = 0.5 * m * (x * x + y * y + z * z);
+  t[5] = m * (double)px; t[6] = m * (double)py; t[7] = m * (double)pz;
+}
+// The workgroup's sums of t (one particle per thread, zeros elsewhere) into
+// row blockIdx.x of `rows`, in a fixed order; every thread calls it.
+__device__ __forceinline__ void block_totals(double* t, double* __restrict__ rows) {
+  __shared__ double sh[16][SPHX_TOTALS_Q];   // up to 16 waves
+#pragma unroll
+  for (int q = 0; q < SPHX_TOTALS_Q; ++q)
+    for (int off = 32; off > 0; off >>= 1) t[q] += __shfl_down(t[q], off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int q = 0; q < SPHX_TOTALS_Q; ++q) sh[w][q] = t[q];
+  __syncthreads();
+  if (threadIdx.x < SPHX_TOTALS_Q) {
+    double s = 0.0;
+    for (int u = 0; u < (int)(blockDim.x + 63) / 64; ++u) s += sh[u][threadIdx.x];
+    rows[(size_t)blockIdx.x * SPHX_TOTALS_Q + threadIdx.x] = s;
+  }
+}
+
 // Cubic spline kernel W and gradient, support 2h (fluid_system_kern.cu:637-671).
 // `r` is x_i - x_j in metres.
 __device__ inline float cubic_w(float rmod, float inv_h, float kspline) {

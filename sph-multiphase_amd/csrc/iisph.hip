@@ -930,40 +930,48 @@ __global__ void k_ii_publish(KP k, const int* __restrict__ cnt, const float* __r
 }
 
 // IntegrateIISPH (kern.cu:1052-1064); optional clamp into the valid cells.
-__global__ void k_ii_integrate(KP k, const uint32_t* __restrict__ key, float4* __restrict__ pos,
-                               float4* __restrict__ vel, const float4* __restrict__ veleval,
-                               const float4* __restrict__ accel, float4* __restrict__ mforce) {
-  int i = xcd_index();
-  if (i >= k.n) return;
-  float4 p = pos[i];
-  if (type_of(p) == SPHX_TYPE_BOUNDARY) return;
-  if ((int)key[i] == k.gridtotal) return;
-  float4 ve = veleval[i], a = accel[i], v = vel[i];
-  float vx = ve.x + a.x * k.dt, vy = ve.y + a.y * k.dt, vz = ve.z + a.z * k.dt;
-  p.x += vx * k.dt / k.ss;
-  p.y += vy * k.dt / k.ss;
-  p.z += vz * k.dt / k.ss;
-  if (k.clamp) {
-    // keep 1 <= cell < res-1 on each axis (reflect the normal velocity)
-    float lo[3], hi[3];
-    lo[0] = k.gridmin.x + 1.0001f / k.gridfac.x; hi[0] = k.gridmin.x + (k.gridres.x - 1.0001f) / k.gridfac.x;
-    lo[1] = k.gridmin.y + 1.0001f / k.gridfac.y; hi[1] = k.gridmin.y + (k.gridres.y - 1.0001f) / k.gridfac.y;
-    lo[2] = k.gridmin.z + 1.0001f / k.gridfac.z; hi[2] = k.gridmin.z + (k.gridres.z - 1.0001f) / k.gridfac.z;
-    if (p.x < lo[0]) { p.x = lo[0]; vx = fmaxf(vx, 0.f); }
-    if (p.x > hi[0]) { p.x = hi[0]; vx = fminf(vx, 0.f); }
-    if (p.y < lo[1]) { p.y = lo[1]; vy = fmaxf(vy, 0.f); }
-    if (p.y > hi[1]) { p.y = hi[1]; vy = fminf(vy, 0.f); }
-    if (p.z < lo[2]) { p.z = lo[2]; vz = fmaxf(vz, 0.f); }
-    if (p.z > hi[2]) { p.z = hi[2]; vz = fminf(vz, 0.f); }
+// With `tot`, also the FP64 conservation totals of the new state over the
+// in-grid fluid particles (sum m, m v, m |v|^2 / 2, m x), one row of
+// SPHX_TOTALS_Q partials per workgroup, summed by k_totals_final.
+__global__ __launch_bounds__(NB_THREADS) void k_ii_integrate(
+    KP k, const uint32_t* __restrict__ key, float4* __restrict__ pos, float4* __restrict__ vel,
+    const float4* __restrict__ veleval, const float4* __restrict__ accel,
+    float4* __restrict__ mforce, const float4* __restrict__ phys, double* __restrict__ tot) {
+  const int i = xcd_index();
+  double t[SPHX_TOTALS_Q] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (i < k.n) {
+    float4 p = pos[i];
+    if (type_of(p) != SPHX_TYPE_BOUNDARY && (int)key[i] != k.gridtotal) {
+      float4 ve = veleval[i], a = accel[i], v = vel[i];
+      float vx = ve.x + a.x * k.dt, vy = ve.y + a.y * k.dt, vz = ve.z + a.z * k.dt;
+      p.x += vx * k.dt / k.ss;
+      p.y += vy * k.dt / k.ss;
+      p.z += vz * k.dt / k.ss;
+      if (k.clamp) {
+        // keep 1 <= cell < res-1 on each axis (reflect the normal velocity)
+        float lo[3], hi[3];
+        lo[0] = k.gridmin.x + 1.0001f / k.gridfac.x; hi[0] = k.gridmin.x + (k.gridres.x - 1.0001f) / k.gridfac.x;
+        lo[1] = k.gridmin.y + 1.0001f / k.gridfac.y; hi[1] = k.gridmin.y + (k.gridres.y - 1.0001f) / k.gridfac.y;
+        lo[2] = k.gridmin.z + 1.0001f / k.gridfac.z; hi[2] = k.gridmin.z + (k.gridres.z - 1.0001f) / k.gridfac.z;
+        if (p.x < lo[0]) { p.x = lo[0]; vx = fmaxf(vx, 0.f); }
+        if (p.x > hi[0]) { p.x = hi[0]; vx = fminf(vx, 0.f); }
+        if (p.y < lo[1]) { p.y = lo[1]; vy = fmaxf(vy, 0.f); }
+        if (p.y > hi[1]) { p.y = hi[1]; vy = fminf(vy, 0.f); }
+        if (p.z < lo[2]) { p.z = lo[2]; vz = fmaxf(vz, 0.f); }
+        if (p.z > hi[2]) { p.z = hi[2]; vz = fminf(vz, 0.f); }
+      }
+      pos[i] = p;
+      vel[i] = make_float4(vx, vy, vz, v.w);
+      if (mforce) {
+        // g - a of this step, read by the next drift-velocity pass
+        float inv = 1.f / k.dt;
+        mforce[i] = make_float4(k.g.x - (vx - v.x) * inv, k.g.y - (vy - v.y) * inv,
+                                k.g.z - (vz - v.z) * inv, 0.f);
+      }
+      if (tot && type_of(p) == SPHX_TYPE_FLUID) particle_totals(phys[i].x, vx, vy, vz, p.x, p.y, p.z, t);
+    }
   }
-  pos[i] = p;
-  vel[i] = make_float4(vx, vy, vz, v.w);
-  if (mforce) {
-    // g - a of this step, read by the next drift-velocity pass
-    float inv = 1.f / k.dt;
-    mforce[i] = make_float4(k.g.x - (vx - v.x) * inv, k.g.y - (vy - v.y) * inv,
-                            k.g.z - (vz - v.z) * inv, 0.f);
-  }
+  if (tot) block_totals(t, tot);   // the same for the whole grid
 }
 
 // Neighbour list of every fluid particle: all j != i with |x_ij| < 2h, in
@@ -1187,6 +1195,7 @@ int grid_sort(sphx_context* c);
 int grid_reorder(sphx_context* c);
 int mlp_solve(sphx_context* c, int* iters, double* err);
 int totals_enqueue(sphx_context* c, bool ring);
+int totals_enqueue_final(sphx_context* c, int nblocks);
 
 static int need_sorted(sphx_context* c, const char* who) {
   SPHX_CHECK(c, who, "null context");
@@ -1714,14 +1723,19 @@ int sphx_iisph_integrate(sphx_context* c) {
   KP k = make_kp(c);
   if (k.n == 0) return 0;
   positions_changed(c);
+  // FP64 mass / momentum / energy totals of the new state, kept per step:
+  // a single scene's are summed by the integration pass itself
+  const int nb = blocks_for(k.n, NB_THREADS);
+  const bool fused = c->tot_track && c->nscenes <= 1 && c->tot_part.p && c->tot_scenes == 1 &&
+                     (size_t)nb * SPHX_TOTALS_Q <= c->tot_part.n;
   timer_begin(c, "iisph_integrate");
-  k_ii_integrate<<<blocks_for(k.n, NB_THREADS), NB_THREADS, 0, c->stream>>>(
+  k_ii_integrate<<<nb, NB_THREADS, 0, c->stream>>>(
       k, c->key.p, c->POS(), c->VEL(), c->VELEVAL(), c->ACCEL(),
-      c->nphase > 0 ? c->mf_force[c->cur].p : nullptr);
+      c->nphase > 0 ? c->mf_force[c->cur].p : nullptr, c->PHYS(), fused ? c->tot_part.p : nullptr);
   timer_end(c);
   SPHX_HIP(hipGetLastError());
-  // FP64 mass / momentum / energy totals of the new state, kept per step
-  if (c->tot_track) SPHX_TRY(totals_enqueue(c, true));
+  if (fused) SPHX_TRY(totals_enqueue_final(c, nb));
+  else if (c->tot_track) SPHX_TRY(totals_enqueue(c, true));
   return 0;
 }
 

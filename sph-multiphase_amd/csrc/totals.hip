@@ -3,10 +3,12 @@
 // Per scene (one for a single scene, every scene of a batched ensemble):
 //   [0] sum m, [1..3] sum m v, [4] sum m |v|^2 / 2, [5..7] sum m x
 // over the in-grid fluid particles (type 0), accumulated in FP64.  The
-// centre of mass is [5..7] / [0].  A fixed grid of TOT_BLOCKS workgroups
-// writes per-block partials that one workgroup sums in block order, so a
-// single scene's totals are reproducible bit for bit; per-scene sums of an
-// ensemble go through LDS atomics and are exact only to FP64 rounding.
+// centre of mass is [5..7] / [0].  For one scene, one workgroup per 256
+// particles writes its sums (the integration pass writes the same rows
+// for the state it produces) and one workgroup adds the rows in a fixed
+// order, so a single scene's totals are reproducible bit for bit; an
+// ensemble's per-scene sums (TOT_BLOCKS workgroups) go through LDS atomics
+// and are exact only to FP64 rounding.
 // While tracking is on, every integration appends its totals to a device
 // ring (the ring index is a device counter, so a captured step advances it
 // on every replay).
@@ -30,29 +32,18 @@ __global__ __launch_bounds__(TOT_THREADS) void k_totals_partial(
   const int S = nscenes > 1 ? nscenes : 1;
   const int stride = TOT_BLOCKS * TOT_THREADS;
   if (S == 1) {
+    // one particle per thread over blocks_for(n) workgroups, the layout and
+    // arithmetic of the integration pass's totals (the same bits)
     double a[TOT_Q] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int i = blockIdx.x * TOT_THREADS + threadIdx.x; i < n; i += stride) {
+    const int i = xcd_index();
+    if (i < n) {
       const float4 p = pos[i];
-      if (type_of(p) != SPHX_TYPE_FLUID || (int)key[i] == gridtotal) continue;
-      const float4 v = vel[i];
-      const double m = phys[i].x, vx = v.x, vy = v.y, vz = v.z;
-      a[0] += m;
-      a[1] += m * vx; a[2] += m * vy; a[3] += m * vz;
-      a[4] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
-      a[5] += m * (double)p.x; a[6] += m * (double)p.y; a[7] += m * (double)p.z;
+      if (type_of(p) == SPHX_TYPE_FLUID && (int)key[i] != gridtotal) {
+        const float4 v = vel[i];
+        particle_totals(phys[i].x, v.x, v.y, v.z, p.x, p.y, p.z, a);
+      }
     }
-#pragma unroll
-    for (int q = 0; q < TOT_Q; ++q)
-      for (int off = 32; off > 0; off >>= 1) a[q] += __shfl_down(a[q], off, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0)
-      for (int q = 0; q < TOT_Q; ++q) sh[w * TOT_Q + q] = a[q];
-    __syncthreads();
-    if (threadIdx.x < TOT_Q) {
-      double t = 0.0;
-      for (int q = 0; q < TOT_THREADS / 64; ++q) t += sh[q * TOT_Q + threadIdx.x];
-      partial[(size_t)blockIdx.x * TOT_Q + threadIdx.x] = t;
-    }
+    block_totals(a, partial);
     return;
   }
   for (int t = threadIdx.x; t < S * TOT_Q; t += TOT_THREADS) sh[t] = 0.0;
@@ -76,27 +67,45 @@ __global__ __launch_bounds__(TOT_THREADS) void k_totals_partial(
     partial[(size_t)blockIdx.x * S * TOT_Q + t] = sh[t];
 }
 
-// One workgroup: entry t of [S][Q] = sum over blocks in block order; with a
-// ring, also stored at slot (*count % cap), and the count advanced.
-__global__ __launch_bounds__(TOT_THREADS) void k_totals_final(
-    const double* __restrict__ partial, int nscenes, double* __restrict__ out,
+// One workgroup: entry t of [S][Q] = the sum over the blocks' partials in a
+// fixed order (TOT_PARTS interleaved partial sums over blocks b = part,
+// part + TOT_PARTS, ..., then the parts in order); with a ring, also stored
+// at slot (*count % cap), and the count advanced.
+constexpr int TOT_PARTS = 128;
+constexpr int TOT_FINAL_THREADS = TOT_Q * TOT_PARTS;
+__global__ __launch_bounds__(TOT_FINAL_THREADS) void k_totals_final(
+    const double* __restrict__ partial, int nblocks, int nscenes, double* __restrict__ out,
     int* __restrict__ count, double* __restrict__ hist, int cap) {
+  __shared__ double ps[TOT_PARTS][TOT_Q];
   const int S = nscenes > 1 ? nscenes : 1;
   const int slot = hist ? (*count % cap) : 0;
-  for (int t = threadIdx.x; t < S * TOT_Q; t += TOT_THREADS) {
+  const int q = threadIdx.x % TOT_Q, part = threadIdx.x / TOT_Q;   // one (quantity, part) each
+  for (int t0 = 0; t0 < S * TOT_Q; t0 += TOT_Q) {
+    const int t = t0 + q;
     double a = 0.0;
-    for (int b = 0; b < TOT_BLOCKS; ++b) a += partial[(size_t)b * S * TOT_Q + t];
-    out[t] = a;
-    if (hist) hist[(size_t)slot * S * TOT_Q + t] = a;
+#pragma unroll 4
+    for (int b = part; b < nblocks; b += TOT_PARTS) a += partial[(size_t)b * S * TOT_Q + t];
+    ps[part][q] = a;
+    __syncthreads();
+    if (part == 0) {
+      double v = 0.0;
+      for (int u = 0; u < TOT_PARTS; ++u) v += ps[u][q];
+      out[t] = v;
+      if (hist) hist[(size_t)slot * S * TOT_Q + t] = v;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   if (hist && threadIdx.x == 0) *count += 1;
 }
+
 
 static int totals_alloc(sphx_context* c) {
   const int S = std::max(1, c->nscenes);
   if (c->tot_scenes != S || !c->tot_part.p) {
-    SPHX_TRY(c->tot_part.alloc((size_t)TOT_BLOCKS * S * TOT_Q));
+    // partials of TOT_BLOCKS workgroups, or of every integration workgroup
+    // (a single scene's totals are summed by the integration pass)
+    SPHX_TRY(c->tot_part.alloc((size_t)std::max(TOT_BLOCKS, blocks_for(c->cap, TOT_THREADS)) * S *
+                               TOT_Q));
     SPHX_TRY(c->tot_out.alloc((size_t)S * TOT_Q));
     SPHX_TRY(c->tot_hist.alloc((size_t)TOT_HIST * S * TOT_Q));
     SPHX_TRY(c->tot_count.alloc(1));
@@ -113,13 +122,28 @@ int totals_enqueue(sphx_context* c, bool ring) {
   SPHX_CHECK(c->tot_part.p && c->tot_scenes == std::max(1, c->nscenes), "conservation totals",
              "buffers not set up for this scene count");
   KP k = make_kp(c);
+  // one scene: a workgroup per 256 particles; an ensemble: TOT_BLOCKS
+  const int nb = c->nscenes > 1 ? TOT_BLOCKS : blocks_for(k.n, TOT_THREADS);
+  SPHX_CHECK((size_t)nb * c->tot_scenes * TOT_Q <= c->tot_part.n, "conservation totals",
+             "more particles than the totals were set up for");
   timer_begin(c, "totals");
-  k_totals_partial<<<TOT_BLOCKS, TOT_THREADS, 0, c->stream>>>(
+  k_totals_partial<<<nb, TOT_THREADS, 0, c->stream>>>(
       k.n, k.gridtotal, c->key.p, c->POS(), c->VEL(), c->PHYS(), c->nscenes, c->tot_part.p);
-  k_totals_final<<<1, TOT_THREADS, 0, c->stream>>>(c->tot_part.p, c->nscenes, c->tot_out.p,
+  k_totals_final<<<1, TOT_FINAL_THREADS, 0, c->stream>>>(c->tot_part.p, nb, c->nscenes, c->tot_out.p,
                                                    c->tot_count.p, ring ? c->tot_hist.p : nullptr,
                                                    TOT_HIST);
   timer_end(c);
+  SPHX_HIP(hipGetLastError());
+  return 0;
+}
+
+// The ring entry of a single scene whose integration pass wrote `nblocks`
+// per-workgroup partials into tot_part.
+int totals_enqueue_final(sphx_context* c, int nblocks) {
+  SPHX_CHECK(c->tot_part.p && c->tot_scenes == 1, "conservation totals",
+             "buffers not set up for one scene");
+  k_totals_final<<<1, TOT_FINAL_THREADS, 0, c->stream>>>(c->tot_part.p, nblocks, 1, c->tot_out.p,
+                                                   c->tot_count.p, c->tot_hist.p, TOT_HIST);
   SPHX_HIP(hipGetLastError());
   return 0;
 }
