@@ -404,6 +404,7 @@ int ensure_device(sphx_context* c);
 void timer_begin(sphx_context* c, const char* name);
 void timer_end(sphx_context* c);
 int timer_flush(sphx_context* c);
+int timer_collect(sphx_context* c);
 
 inline int blocks_for(int n, int threads) { return (n + threads - 1) / threads; }
 

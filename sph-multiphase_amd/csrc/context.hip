@@ -96,23 +96,53 @@ void timer_end(sphx_context* c) {
   c->timer.ended = c->timer.pending.size();
 }
 
-int timer_flush(sphx_context* c) {
-  if (c->timer.pending.empty()) return 0;
-  SPHX_HIP(hipStreamSynchronize(c->stream));
-  for (size_t i = 0; i < c->timer.pending.size(); ++i) {
+// Adds the first `count` timed launches to the totals and returns their
+// events to the pool.
+static void timer_take(sphx_context* c, size_t count) {
+  auto& T = c->timer;
+  for (size_t i = 0; i < count; ++i) {
     float ms = 0.f;
-    hipEvent_t a = c->timer.pending[i].first, b = c->timer.pending[i].second;
+    hipEvent_t a = T.pending[i].first, b = T.pending[i].second;
     if (hipEventElapsedTime(&ms, a, b) == hipSuccess) {
-      auto& e = c->timer.acc[c->timer.names[i]];
+      auto& e = T.acc[T.names[i]];
       e.first += ms;
       e.second += 1;
     }
-    c->timer.pool.push_back(a);
-    c->timer.pool.push_back(b);
+    T.pool.push_back(a);
+    T.pool.push_back(b);
   }
-  c->timer.pending.clear();
-  c->timer.names.clear();
-  c->timer.ended = 0;
+  T.pending.erase(T.pending.begin(), T.pending.begin() + count);
+  T.names.erase(T.names.begin(), T.names.begin() + count);
+  T.ended = T.ended > count ? T.ended - count : 0;
+}
+
+// Every timed launch, after the stream has drained (the readers of the
+// totals call this).
+int timer_flush(sphx_context* c) {
+  if (c->timer.pending.empty()) return 0;
+  SPHX_HIP(hipStreamSynchronize(c->stream));
+  timer_take(c, c->timer.pending.size());
+  return 0;
+}
+
+// The timed launches that have already completed, without waiting: the end
+// of a step calls this, so that timing a run adds no synchronisation to it.
+// The events of one stream complete in order, so the first end event still
+// pending stops the scan; pairs from `ended` on may lack their end event.
+int timer_collect(sphx_context* c) {
+  auto& T = c->timer;
+  size_t done = 0;
+  while (done < T.ended) {
+    const hipError_t q = hipEventQuery(T.pending[done].second);
+    if (q != hipSuccess) {
+      // hipErrorNotReady is left as the thread's last error: clear it so
+      // that the next launch check does not report it
+      (void)hipGetLastError();
+      break;
+    }
+    ++done;
+  }
+  if (done) timer_take(c, done);
   return 0;
 }
 

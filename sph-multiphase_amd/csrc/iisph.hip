@@ -1668,7 +1668,7 @@ int iisph_solve_plain(sphx_context* c, int* iters, double* err) {
   c->last_iters = it;
   if (iters) *iters = it;
   if (err) *err = c->host_pinned[0];
-  return timer_flush(c);
+  return timer_collect(c);
 }
 
 // The outcome of the solve on the stream: iterations (with those a device

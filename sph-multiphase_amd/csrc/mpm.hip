@@ -1290,7 +1290,7 @@ int sphx_mpm_substep(sphx_context* c) {
   SPHX_TRY(sphx_mpm_stress(c));
   SPHX_TRY(sphx_mpm_node_update(c));
   SPHX_TRY(sphx_mpm_g2p(c, 1));
-  return timer_flush(c);
+  return timer_collect(c);
 }
 
 // Node velocities [N][3] and masses [N] in GetMpmIdx order (either NULL).
@@ -1436,7 +1436,7 @@ int sphx_mpm_get_momentum(sphx_context* c) {
   k_mpm_momentum<<<blocks_for(nodes, 256), 256, 0, c->stream>>>(
       make_kp(c), make_mp(c), c->cell_start.p, c->POS(), c->VEL(), c->mpm.nodev.p, nodes);
   SPHX_HIP(hipGetLastError());
-  return timer_flush(c);
+  return timer_collect(c);
 }
 
 int sphx_mpm_color_test(sphx_context* c) {
@@ -1446,7 +1446,7 @@ int sphx_mpm_color_test(sphx_context* c) {
   k_mpm_color_test<<<blocks_for(nodes, 256), 256, 0, c->stream>>>(
       make_kp(c), make_mp(c), c->cell_start.p, c->PERM(), c->cold_color.p, nodes);
   SPHX_HIP(hipGetLastError());
-  return timer_flush(c);
+  return timer_collect(c);
 }
 
 }  // extern "C"
