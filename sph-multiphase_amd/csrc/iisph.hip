@@ -1350,12 +1350,18 @@ static int enqueue_first_or_next(sphx_context* c, const KP& k, bool first) {
   return 0;
 }
 
+// The solve's errors and state written straight into pinned host memory:
+// one small launch in place of two copies on the stream.
+__global__ void k_state_out(const double* __restrict__ err, const int* __restrict__ state,
+                            double* __restrict__ host) {
+  if (threadIdx.x < 2) host[threadIdx.x] = err[threadIdx.x];
+  else if (threadIdx.x < 6) reinterpret_cast<int*>(host + 4)[threadIdx.x - 2] = state[threadIdx.x - 2];
+}
+
 static int read_state(sphx_context* c) {
-  SPHX_HIP(hipMemcpyAsync(c->host_pinned, c->solve_err.p, 2 * sizeof(double),
-                          hipMemcpyDeviceToHost, c->stream));
+  k_state_out<<<1, 64, 0, c->stream>>>(c->solve_err.p, c->solve_state.p, c->host_pinned);
+  SPHX_HIP(hipGetLastError());
   int st[4];
-  SPHX_HIP(hipMemcpyAsync(c->host_pinned + 4, c->solve_state.p, 4 * sizeof(int),
-                          hipMemcpyDeviceToHost, c->stream));
   SPHX_HIP(hipStreamSynchronize(c->stream));
   std::memcpy(st, c->host_pinned + 4, 4 * sizeof(int));
   c->host_pinned[2] = st[0];
