@@ -477,24 +477,22 @@ __global__ void k_f32_to_f16(const float* __restrict__ a, __half* __restrict__ b
 // (multiphase) the change of the phase-1 drift when it is evaluated with the
 // converged instead of the previous pressure.
 // Sample t of the call is slot i = phase + t * stride (stride 1: every slot).
-__global__ void k_mlp_record(KP k, MLPP mp, int count, int stride, int phase,
-                             const float4* __restrict__ pos,
-                             const uint32_t* __restrict__ key, const float4* __restrict__ phys,
-                             const float* __restrict__ pprev, const float* __restrict__ X,
-                             const float4* __restrict__ u_old, const float4* __restrict__ u_new,
-                             float* __restrict__ Xs, float* __restrict__ Ts,
-                             float* __restrict__ Ms) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= count) return;
-  const int i = phase + t * stride;
-  bool ok = type_of(pos[i]) != SPHX_TYPE_BOUNDARY && (int)key[i] != k.gridtotal;
+// One recorded sample: the feature row of slot i (as the a_ii pass left
+// it), the targets and the mask into row t of the sample store.  `d` is
+// slot i's pressure increment p - max(0, p(t-1)) (0 off the fluid).
+__device__ __forceinline__ void record_sample(const MLPP& mp, int i, int t, bool ok, float d,
+                                              const float* __restrict__ X,
+                                              const float4* __restrict__ u_old,
+                                              const float4* __restrict__ u_new,
+                                              float* __restrict__ Xs, float* __restrict__ Ts,
+                                              float* __restrict__ Ms) {
   const float4* xs = reinterpret_cast<const float4*>(X + (size_t)i * MLP_NF);
   float4* xd = reinterpret_cast<float4*>(Xs + (size_t)t * MLP_NF);
 #pragma unroll
   for (int q = 0; q < MLP_NF / 4; ++q) xd[q] = xs[q];
   float t0 = 0.f, t1 = 0.f, t2 = 0.f, t3 = 0.f;
   if (ok) {
-    t0 = (phys[i].z - fmaxf(0.f, pprev[i])) / mp.p_scale * MLP_TSCALE;
+    t0 = d / mp.p_scale * MLP_TSCALE;
     if (u_old && u_new) {
       float s = MLP_TSCALE / mp.u_scale;
       float4 a = u_old[i], b = u_new[i];
@@ -508,22 +506,33 @@ __global__ void k_mlp_record(KP k, MLPP mp, int count, int stride, int phase,
   Ms[t] = m;
 }
 
+// hist.w = p - max(0, p(t-1)) after a solve (0 off the fluid) for every
+// slot and, with `count` > 0, the sample of every slot i = phase + t stride
+// (t < count) in the same coalesced sweep.
+__global__ void k_mlp_hist(KP k, MLPP mp, const float4* __restrict__ pos,
+                           const uint32_t* __restrict__ key, const float4* __restrict__ phys,
+                           const float* __restrict__ pprev, float4* __restrict__ hist,
+                           int count, int stride, int phase, const float* __restrict__ X,
+                           const float4* __restrict__ u_old, const float4* __restrict__ u_new,
+                           float* __restrict__ Xs, float* __restrict__ Ts,
+                           float* __restrict__ Ms) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k.n) return;
+  bool ok = type_of(pos[i]) != SPHX_TYPE_BOUNDARY && (int)key[i] != k.gridtotal;
+  const float inc = phys[i].z - fmaxf(0.f, pprev[i]);
+  float d = ok ? inc : 0.f;
+  if (!(d == d)) d = 0.f;
+  hist[i].w = d;
+  const int r = i - phase;
+  if (count == 0 || r < 0 || r % stride != 0) return;
+  const int t = r / stride;
+  if (t < count) record_sample(mp, i, t, ok, inc, X, u_old, u_new, Xs, Ts, Ms);
+}
+
 // p(t-1) when the features are not computed.
 __global__ void k_mlp_pprev(int n, const float4* __restrict__ phys, float* __restrict__ pprev) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) pprev[i] = phys[i].z;
-}
-
-// hist.w = p - max(0, p(t-1)) after a solve (0 off the fluid).
-__global__ void k_mlp_hist(KP k, const float4* __restrict__ pos, const uint32_t* __restrict__ key,
-                           const float4* __restrict__ phys, const float* __restrict__ pprev,
-                           float4* __restrict__ hist) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= k.n) return;
-  bool ok = type_of(pos[i]) != SPHX_TYPE_BOUNDARY && (int)key[i] != k.gridtotal;
-  float d = ok ? phys[i].z - fmaxf(0.f, pprev[i]) : 0.f;
-  if (!(d == d)) d = 0.f;
-  hist[i].w = d;
 }
 
 __global__ void k_mlp_reorder(const uint32_t* __restrict__ skey,
@@ -784,9 +793,6 @@ int mlp_after(sphx_context* c, int record_ok) {
   const bool online = m.online_every > 0;
   const bool record = record_ok && (m.record || online);
   float4* hist = m.hist[c->cur].p;
-  k_mlp_hist<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(k, c->POS(), c->key.p, c->PHYS(),
-                                                          m.pprev.p, hist);
-  SPHX_HIP(hipGetLastError());
   // Recording with a stride keeps every stride-th slot of a solve (the
   // first slot rotating with the solve count); online, in a ring of as many
   // solves as the store holds full solves.
@@ -794,6 +800,7 @@ int mlp_after(sphx_context* c, int record_ok) {
   const int phase = stride > 1 ? (int)(m.solves % stride) : 0;
   const size_t nrec = stride > 1 ? (n - (size_t)phase + stride - 1) / stride : n;
   const bool ring = online && !m.record;
+  bool recorded = false;
   const size_t cap = (ring && stride > 1)
                          ? std::min(m.max_samples, (m.max_samples / n) * ((n + stride - 1) / stride))
                          : m.max_samples;
@@ -813,15 +820,23 @@ int mlp_after(sphx_context* c, int record_ok) {
         u_new = u + c->cap;
       }
       if (m.train_inflight) SPHX_HIP(hipStreamWaitEvent(c->stream, m.ev_train, 0));
+      // the increments and the samples in one sweep over the slots
       timer_begin(c, "mlp_record");
-      k_mlp_record<<<blocks_for((int)nrec, 256), 256, 0, c->stream>>>(
-          k, mp, (int)nrec, stride, phase, c->POS(), c->key.p, c->PHYS(), m.pprev.p, m.X.p,
+      k_mlp_hist<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+          k, mp, c->POS(), c->key.p, c->PHYS(), m.pprev.p, hist, (int)nrec, stride, phase, m.X.p,
           u_old, u_new, m.sX.p + base * MLP_NF, m.sT.p + base * MLP_NO, m.sM.p + base);
       timer_end(c);
       SPHX_HIP(hipGetLastError());
       m.wpos = base + nrec;
       m.nsamples = std::min(std::max(m.nsamples, m.wpos), cap);
+      recorded = true;
     }
+  }
+  if (!recorded) {
+    k_mlp_hist<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
+        k, mp, c->POS(), c->key.p, c->PHYS(), m.pprev.p, hist, 0, 1, 0, nullptr, nullptr, nullptr,
+        nullptr, nullptr, nullptr);
+    SPHX_HIP(hipGetLastError());
   }
   if (record_ok) {
     ++m.solves;
