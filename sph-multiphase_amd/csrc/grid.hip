@@ -10,11 +10,14 @@
 //  * keys are cell ids in [0, gridtotal); out-of-grid particles get the
 //    sentinel `gridtotal`, so a key needs only bit_width(gridtotal) bits
 //    (18 for a 52^3 grid) and the sort runs ceil(bits/9) passes of <= 9 bits.
-//  * each pass is upsweep (per-tile digit histogram) -> one-workgroup scan
-//    of the digit-major histogram -> downsweep (stable in-tile ranking with
-//    wave64 ballots: lanes holding equal digits find each other with one
-//    ballot per digit bit; a per-wave running count in LDS orders the
-//    rounds; a block prefix over the four waves orders the waves).
+//  * each pass is upsweep (per-tile digit histogram, one LDS atomic per run
+//    of equal digits in lane order) -> one-workgroup scan of the digit-major
+//    histogram -> downsweep (stable in-tile ranking: when a round's digits
+//    do not decrease in lane order, as for nearly sorted keys, a lane's
+//    rank is its offset in its run of equal digits; otherwise lanes holding
+//    equal digits find each other with one wave64 ballot per digit bit; a
+//    per-wave running count in LDS orders the rounds; a block prefix over
+//    the four waves orders the waves).
 //  * cell ranges are stored dense: cell_start[c] = number of valid particles
 //    in cells < c, for every c in [0, gridtotal].  The reference's
 //    mgridoff/mgridcnt are derived from it on download.  Because cell ids
@@ -83,10 +86,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_upsweep(const uint32_t* __res
     int idx = base + it * 64 + lane;
     bool valid = idx < n;
     uint32_t d = valid ? ((keys[idx] >> shift) & mask) : 0u;
-    uint64_t active = __ballot(valid);
-    if (!valid) continue;
-    uint64_t peers = match_digit(d, bits, active);
-    if ((peers & lanemask_lt()) == 0) atomicAdd(&h[d], (uint32_t)__popcll(peers));
+    // Each run of equal digits in lane order adds its length once (the
+    // keys arrive nearly sorted, so runs are long); the valid lanes are a
+    // prefix of the wave.
+    const int nvalid = __popcll(__ballot(valid));
+    const uint32_t dprev = __shfl_up(d, 1, 64);
+    const bool head = valid && (lane == 0 || d != dprev);
+    const uint64_t above = __ballot(head) & ~((2ull << lane) - 1ull);
+    const int next = above ? __builtin_ctzll(above) : 64;
+    if (head) atomicAdd(&h[d], (uint32_t)(min(next, nvalid) - lane));
   }
   __syncthreads();
   for (int t = threadIdx.x; t < bins; t += RS_THREADS) hist[(size_t)t * nblocks + blockIdx.x] = h[t];
@@ -209,17 +217,40 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_downsweep(
     uint32_t v = valid ? vin[idx] : 0u;
     uint32_t d = (k >> shift) & mask;
     uint64_t active = __ballot(valid);
-    uint64_t peers = match_digit(d, bits, active);
+    // When the round's digits do not decrease in lane order (the common
+    // case for nearly sorted keys), a lane's peers are its run of equal
+    // digits: rank and count come from the run heads.  Otherwise the
+    // ballot match finds them.
+    const uint32_t dprev = __shfl_up(d, 1, 64);
+    const bool mono = __ballot(valid && lane > 0 && d < dprev) == 0ull;
+    uint32_t rank, count;
+    bool leader;
+    if (mono) {
+      const bool head = valid && (lane == 0 || d != dprev);
+      const uint64_t heads = __ballot(head);
+      const uint64_t upto = (2ull << lane) - 1ull;   // lanes 0..lane
+      const uint64_t le = heads & upto, above = heads & ~upto;
+      const int start = le ? 63 - __builtin_clzll(le) : 0;
+      const int next = above ? __builtin_ctzll(above) : 64;
+      rank = (uint32_t)(lane - start);
+      count = (uint32_t)(min(next, __popcll(active)) - lane);
+      leader = head;
+    } else {
+      const uint64_t peers = match_digit(d, bits, active);
+      rank = (uint32_t)__popcll(peers & lt);
+      count = (uint32_t)__popcll(peers);
+      leader = (peers & lt) == 0;
+    }
     uint32_t prev = 0;
     if (valid) prev = wcnt[wave][d];
     // every peer has read `prev` before the group leader publishes the
     // new running count (same wave, program order on the LDS queue)
     __builtin_amdgcn_wave_barrier();
-    if (valid && (peers & lt) == 0) wcnt[wave][d] = prev + (uint32_t)__popcll(peers);
+    if (valid && leader) wcnt[wave][d] = prev + count;
     __builtin_amdgcn_wave_barrier();
     kk[it] = k;
     vv[it] = v;
-    rk[it] = valid ? prev + (uint32_t)__popcll(peers & lt) : 0xFFFFFFFFu;
+    rk[it] = valid ? prev + rank : 0xFFFFFFFFu;
   }
   __syncthreads();
   for (int t = threadIdx.x; t < bins; t += RS_THREADS) {
