@@ -272,12 +272,38 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_downsweep(
 }
 
 // cell_start[c] = lower_bound(sorted keys, c) for c in [0, gridtotal]: one
-// thread per cell, binary search over the sorted keys (no serial gaps).
+// thread per cell (no serial gaps).  The buffer's previous contents (the
+// last grid build's ranges, or anything) are only the starting guess: a
+// galloping search from it brackets the bound, and a binary search inside
+// the bracket finds it.  Particles move little between steps, so the bound
+// is usually found in two or three loads instead of log2(n).
 __global__ void k_cell_start(const uint32_t* __restrict__ skey, int n, int gridtotal,
                              int* __restrict__ cell_start) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c > gridtotal) return;
-  int lo = 0, hi = n;
+  const int g = min(max(cell_start[c], 0), n);
+  // the bound lies in [lo, hi]: keys before lo are < c, the key at hi (if
+  // hi < n) is >= c
+  int lo, hi;
+  if (g < n && (int)skey[g] < c) {
+    lo = g + 1;
+    hi = n;
+    for (int step = 1;; step <<= 1) {
+      const int p = g + step;
+      if (p >= n) break;
+      if ((int)skey[p] >= c) { hi = p; break; }
+      lo = p + 1;
+    }
+  } else {
+    lo = 0;
+    hi = g;
+    for (int step = 1;; step <<= 1) {
+      const int p = g - step;
+      if (p < 0) break;
+      if ((int)skey[p] < c) { lo = p + 1; break; }
+      hi = p;
+    }
+  }
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
     if ((int)skey[mid] < c) lo = mid + 1;
