@@ -72,7 +72,42 @@ def test_grid_build_twice_nearly_sorted(gpu):
     st = oracle_build(p, d1, c1)
     assert np.array_equal(g["mgcell"], st["mgcell"])
     assert np.array_equal(g["midsort"], st["midsort"])
+    assert np.array_equal(g["gridoff"], st["gridoff"])
+    assert np.array_equal(g["gridcnt"], st["gridcnt"])
     assert np.array_equal(d["pos"], st["pos"])
+
+
+@pytest.mark.parametrize("frac", [0.03, 0.5])
+def test_grid_rebuild_after_moves(gpu, frac):
+    """Rebuild after a fraction of the particles moved to other cells: the
+    sort's digit runs see nearly sorted keys with strays (or, at frac 0.5,
+    mostly unsorted ones), and the cell ranges start their search from the
+    last build's ranges, which are then off."""
+    p, disp, calc, info = shuffled_scene(n_out=3, seed=12)
+    rng = np.random.default_rng(5)
+    with sphx.Context(p) as ctx:
+        ctx.upload(disp, calc)
+        ctx.call("grid_build")
+        d1, c1 = ctx.download()
+        cell = 1.0 / p.gridIdfac.x
+        moved = rng.random(len(d1)) < frac
+        step = rng.integers(-2, 3, size=(len(d1), 3)).astype(np.float32) * np.float32(cell)
+        inside = d1["pos"][:, 0] > -999.0
+        sel = moved & inside
+        d1["pos"][sel] = d1["pos"][sel] + step[sel]
+        print(f"rebuild: n={len(d1)} moved={int(sel.sum())}", flush=True)
+        ctx.upload(d1, c1)
+        ctx.call("grid_build")
+        g = ctx.download_grid()
+        d, c = ctx.download()
+    st = oracle_build(p, d1, c1)
+    assert np.array_equal(g["mgcell"], st["mgcell"])
+    assert np.array_equal(g["midsort"], st["midsort"])
+    assert np.array_equal(g["gridoff"], st["gridoff"])
+    assert np.array_equal(g["gridcnt"], st["gridcnt"])
+    valid = st["mgcell"] != UNDEF
+    assert np.array_equal(g["mgndx"][valid], st["mgndx"][valid])
+    assert np.array_equal(d["pos"], st["pos"].astype(np.float32))
 
 
 def test_grid_empty_and_single(gpu):
