@@ -219,3 +219,40 @@ def test_iisph_divergence_is_not_reported_as_converged(gpu):
     print(f"diverging solve: gpu {it} iters err {err}; oracle {it_ref} err {err_ref}")
     assert it_ref == o.max_iters and not (err_ref <= o.tol)
     assert it == o.max_iters and not (err <= o.tol)
+
+
+def test_kernel_timing_counts_every_launch(gpu):
+    """Kernel timing: with one kernel named, only its launches carry events,
+    and every one of them is counted once although the steps end without a
+    synchronisation (completed events are taken as they complete, the rest
+    when the totals are read).  Two contexts run the same four steps, one
+    timing the Jacobi pass only, one timing every kernel."""
+    s = h = 0.01
+    p, disp, calc, info = sphx.scene_dambreak(8, 16, 8, s, (0.08, 0.21, 0.08), h, 3, 1e-4, 9, 1)
+    p.dt = 1e-3
+    o = options(wcsph_kernel=1, full_support=1, boundary_pressure=1, omega=0.5,
+                clamp_pressure=1, tol=1e-3, max_iters=300)
+    out = []
+    for only in ("iisph_iter", None):
+        with sphx.Context(p, o) as ctx:
+            ctx.upload(disp, calc)
+            ctx.call("grid_build")
+            ctx.call("boundary_volume")
+            ctx.iisph_step()
+            ctx.timing_only(only)
+            ctx.timing(True)
+            ctx.timing_reset()
+            iters = [ctx.iisph_step()[0] for _ in range(4)]
+            kt = {k: ctx.timing_get(k) for k in ("iisph_iter", "iisph_dp", "hash", "iisph_integrate")}
+            ctx.timing(False)
+        print(f"timing only={only}: iterations {iters}, launches "
+              + ", ".join(f"{k} {n}" for k, (t, n) in kt.items()), flush=True)
+        out.append((iters, kt))
+    (it_a, a), (it_b, b) = out
+    assert it_a == it_b
+    # every queued Jacobi pass is timed once in both runs (the solves queue
+    # their iterations plus some past convergence, the same in both)
+    assert a["iisph_iter"][1] == b["iisph_iter"][1] == b["iisph_dp"][1] >= sum(it_a)
+    assert a["iisph_iter"][0] > 0.0 and b["iisph_iter"][0] > 0.0
+    assert a["hash"] == (0.0, 0) and a["iisph_dp"] == (0.0, 0)
+    assert b["hash"][1] == 4 and b["iisph_integrate"][1] == 4
