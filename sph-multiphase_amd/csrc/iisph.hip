@@ -1655,19 +1655,33 @@ int iisph_solve_plain(sphx_context* c, int* iters, double* err) {
   }
   SPHX_TRY(solve_reset(c));
   int enq = 0;
-  int chunk = std::max(c->O.min_iters, 2);
-  // enqueue one more iteration than the previous solve needed before the
-  // first host check, then two at a time: a host round trip idles the GPU
-  // for longer than an iteration queued past convergence costs (its
-  // kernels exit on the device flag)
-  if (c->last_iters + 1 > chunk) chunk = c->last_iters + 1;
+  // Iterations queued before each host check.  A host round trip idles the
+  // GPU for longer than an iteration queued past convergence costs (its
+  // kernels exit on the device flag), so the first check comes after one
+  // more iteration than the previous solve needed; after a long solve, at a
+  // third of that (at least 16), so that a solve that needs far fewer does
+  // not queue dozens of idle launches.  Later checks add a quarter of the
+  // iterations so far (at least two) until two checks give the error's decay
+  // rate, then the iterations that rate predicts (at most doubling).
+  const int first = std::max(std::max(c->O.min_iters, 2),
+                             std::min(c->last_iters + 1, std::max(16, (c->last_iters + 3) / 3)));
+  int target = first, k_prev = 0;
+  double e_prev = 0.0;
   for (;;) {
-    int todo = std::min(chunk, c->O.max_iters - enq);
-    for (int t = 0; t < todo; ++t) SPHX_TRY(enqueue_first_or_next(c, k, enq + t == 0));
-    enq += todo;
+    target = std::min(target, c->O.max_iters);
+    while (enq < target) SPHX_TRY(enqueue_first_or_next(c, k, enq++ == 0));
     SPHX_TRY(read_state(c));
     if (c->host_pinned[2] != 0.0 || enq >= c->O.max_iters) break;
-    chunk = 2;
+    const double e = c->host_pinned[0], tol = (double)c->O.tol;
+    int more = std::max(2, enq / 4);
+    if (k_prev > 0 && e > tol && e < e_prev && e_prev > 0.0) {
+      const double rate = std::pow(e / e_prev, 1.0 / (double)(enq - k_prev));
+      const double need = std::log(tol / e) / std::log(rate);
+      if (std::isfinite(need)) more = std::min(std::max(2, (int)std::ceil(need) + 1), std::max(2, enq));
+    }
+    k_prev = enq;
+    e_prev = e;
+    target = enq + more;
   }
   SPHX_TRY(publish_solve(c, k));
   int it = (int)c->host_pinned[3];
