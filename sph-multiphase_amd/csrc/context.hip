@@ -75,7 +75,9 @@ static bool timer_event(sphx_context* c, hipEvent_t* e) {
     pool.pop_back();
     return true;
   }
-  return hipEventCreate(e) == hipSuccess;
+  // timing only: no system-scope fence (with it each record writes back and
+  // invalidates the caches, which slows the work that follows)
+  return hipEventCreateWithFlags(e, hipEventDisableSystemFence) == hipSuccess;
 }
 
 void timer_begin(sphx_context* c, const char* name) {
