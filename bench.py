@@ -34,20 +34,20 @@ HBM_PEAK_GBS = 8000.0
 # counters (profiles/iisph_mf_step_1m_hbm_traffic.md; TCC_EA0_RDREQ_128B x
 # 128 B gives the same read bytes)
 TRAFFIC_BYTES = {
-    "iisph_iter": 212.0e6,
+    "iisph_iter": 212.3e6,
     "iisph_dp": 177.5e6,
-    "mf_alpha": 397.6e6,
-    "mf_drift": 279.7e6,
-    "iisph_aii": 381.3e6,    # with the learned start
-    "iisph_density_dii": 219.0e6,
-    "nbr_build": 165.8e6,
+    "mf_alpha": 397.2e6,
+    "mf_drift": 279.1e6,
+    "iisph_aii": 381.6e6,    # with the learned start
+    "iisph_density_dii": 218.7e6,
+    "nbr_build": 165.6e6,
     "mf_correction_force": 158.4e6,
 }
 # MPM sand (2M particles), same method (profiles/mpm_sand_2m_kernel_stats.md)
 MPM_TRAFFIC_BYTES = {
     "mpm_p2g": 337.4e6,
     "mpm_force": 301.1e6,
-    "mpm_stress": 536.8e6,
+    "mpm_stress": 536.9e6,
 }
 
 # Algorithmic bytes per fluid particle per launch (fields read or written
