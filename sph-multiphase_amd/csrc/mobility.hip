@@ -534,8 +534,9 @@ int sphx_rpy_apply_device(sphx_rpy* h, const float* dF, float* dU, int mode) {
   dim3 grid(np / RPY_TP, nseg);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (h->timing) {
-    SPHX_HIP(hipEventCreate(&e0));
-    SPHX_HIP(hipEventCreate(&e1));
+    // timing only: no system-scope fence at the records
+    SPHX_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+    SPHX_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
     h->ev.emplace_back(e0, e1);
     SPHX_HIP(hipEventRecord(e0, h->stream));
   }
