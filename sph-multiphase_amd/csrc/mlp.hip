@@ -508,9 +508,11 @@ __device__ __forceinline__ void record_sample(const MLPP& mp, int i, int t, bool
 
 // hist.w = p - max(0, p(t-1)) after a solve (0 off the fluid) for every
 // slot and, with `count` > 0, the sample of every slot i = phase + t stride
-// (t < count) in the same coalesced sweep.
-__global__ void k_mlp_hist(KP k, MLPP mp, const float4* __restrict__ pos,
-                           const uint32_t* __restrict__ key, const float4* __restrict__ phys,
+// (t < count) in the same coalesced sweep.  Fluid slots are those the
+// solve's neighbour list marks valid (cnt: boundary and out-of-grid slots
+// hold NBR_INVALID).
+__global__ void k_mlp_hist(KP k, MLPP mp, const int* __restrict__ cnt,
+                           const float4* __restrict__ phys,
                            const float* __restrict__ pprev, float4* __restrict__ hist,
                            int count, int stride, int phase, const float* __restrict__ X,
                            const float4* __restrict__ u_old, const float4* __restrict__ u_new,
@@ -518,7 +520,7 @@ __global__ void k_mlp_hist(KP k, MLPP mp, const float4* __restrict__ pos,
                            float* __restrict__ Ms) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= k.n) return;
-  bool ok = type_of(pos[i]) != SPHX_TYPE_BOUNDARY && (int)key[i] != k.gridtotal;
+  const bool ok = cnt[i] != NBR_INVALID;
   const float inc = phys[i].z - fmaxf(0.f, pprev[i]);
   float d = ok ? inc : 0.f;
   if (!(d == d)) d = 0.f;
@@ -823,7 +825,7 @@ int mlp_after(sphx_context* c, int record_ok) {
       // the increments and the samples in one sweep over the slots
       timer_begin(c, "mlp_record");
       k_mlp_hist<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
-          k, mp, c->POS(), c->key.p, c->PHYS(), m.pprev.p, hist, (int)nrec, stride, phase, m.X.p,
+          k, mp, c->nbr_cnt.p, c->PHYS(), m.pprev.p, hist, (int)nrec, stride, phase, m.X.p,
           u_old, u_new, m.sX.p + base * MLP_NF, m.sT.p + base * MLP_NO, m.sM.p + base);
       timer_end(c);
       SPHX_HIP(hipGetLastError());
@@ -834,7 +836,7 @@ int mlp_after(sphx_context* c, int record_ok) {
   }
   if (!recorded) {
     k_mlp_hist<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
-        k, mp, c->POS(), c->key.p, c->PHYS(), m.pprev.p, hist, 0, 1, 0, nullptr, nullptr, nullptr,
+        k, mp, c->nbr_cnt.p, c->PHYS(), m.pprev.p, hist, 0, 1, 0, nullptr, nullptr, nullptr,
         nullptr, nullptr, nullptr);
     SPHX_HIP(hipGetLastError());
   }
