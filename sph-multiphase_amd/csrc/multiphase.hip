@@ -373,7 +373,10 @@ __global__ void k_mf_force_add(KP k, MFP m, const float4* __restrict__ pos,
 }
 
 // k_mf_correction then k_mf_force_add (same particles skipped), in one pass.
-__global__ void k_mf_correct_force(KP k, MFP m, const float4* __restrict__ pos,
+// `cnt`, when given, is the step's neighbour list counts (NBR_INVALID marks
+// boundary and out-of-grid slots): one 4-byte read instead of pos and key.
+__global__ void k_mf_correct_force(KP k, MFP m, const int* __restrict__ cnt,
+                                   const float4* __restrict__ pos,
                                    const uint32_t* __restrict__ key,
                                    const float4* __restrict__ alpha_pre,
                                    float4* __restrict__ alpha_new, float4* __restrict__ vel,
@@ -381,7 +384,9 @@ __global__ void k_mf_correct_force(KP k, MFP m, const float4* __restrict__ pos,
                                    float4* __restrict__ veleval, float4* __restrict__ amf) {
   int i = xcd_index();
   if (i >= k.n) return;
-  if (type_of(pos[i]) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal) return;
+  if (cnt ? cnt[i] == NBR_INVALID
+          : (type_of(pos[i]) == SPHX_TYPE_BOUNDARY || (int)key[i] == k.gridtotal))
+    return;
   const float rho_m = mf_correct_one(m, i, alpha_pre, alpha_new, vel);
   mf_force_add_one(k, m, i, rho_m, stress, veleval, amf);
 }
@@ -698,7 +703,8 @@ int mf_correction_force(sphx_context* c) {
   MFP m = make_mfp(c);
   timer_begin(c, "mf_correction_force");
   k_mf_correct_force<<<blocks_for(k.n, 256), 256, 0, c->stream>>>(
-      k, m, c->POS(), c->key.p, c->mf_alpha[c->cur].p, c->mf_alpha_new.p, c->VEL(),
+      k, m, c->nbr_valid ? c->nbr_cnt.p : nullptr, c->POS(), c->key.p, c->mf_alpha[c->cur].p,
+      c->mf_alpha_new.p, c->VEL(),
       c->mf_stress.p, c->VELEVAL(), c->mf_amf.p);
   timer_end(c);
   SPHX_HIP(hipGetLastError());
