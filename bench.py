@@ -35,10 +35,10 @@ HBM_PEAK_GBS = 8000.0
 # 128 B gives the same read bytes)
 TRAFFIC_BYTES = {
     "iisph_iter": 212.3e6,
-    "iisph_dp": 177.5e6,
-    "mf_alpha": 397.2e6,
+    "iisph_dp": 177.6e6,
+    "mf_alpha": 398.5e6,
     "mf_drift": 279.1e6,
-    "iisph_aii": 381.6e6,    # with the learned start
+    "iisph_aii": 381.7e6,    # with the learned start
     "iisph_density_dii": 218.7e6,
     "nbr_build": 165.6e6,
     "mf_correction_force": 158.4e6,
