@@ -73,7 +73,9 @@ ALGO_BYTES = {
     # alpha_pre, type, key, stress, v_adv -> alpha_new, nu, v_adv, a_mf
     "mf_correction_force": 124,
     "radix_sort": 32,      # 2 passes x (key+val read, key+val write)
-    "reorder": 104,        # 5 x 16 B + 2 x 4 B gathered and written
+    # per slot: key, index and 5 x 16 B + 2 x 4 B gathered (96 B), the same
+    # written plus the born-id map entry (92 B)
+    "reorder": 188,
     # x and key read 20, count 4, ps and the Jacobi record seeded 40, and the list
     # itself: 16 B per 8 entries (about 43 neighbours within 2h: 6 groups)
     "nbr_build": 160,
@@ -539,7 +541,7 @@ MPM_ALGO_BYTES = {
     "mpm_g2p": 100,      # x r/w, v r/w, B w
     "mpm_node": 32,      # per active node: {v, m} r/w
     "mpm_clear": 4,      # per active block: flag w
-    "reorder": 104,
+    "reorder": 188,      # per slot, as ALGO_BYTES
 }
 
 
